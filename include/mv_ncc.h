@@ -1,0 +1,130 @@
+/* Public C ABI of the MI355X NCC find path: the subset of mvkit's C ABI a
+ * caller of the NCC operators binds, with the reference's names, types,
+ * ownership and error conventions.  Each declaration cites the reference
+ * header it replaces (paths relative to the mvkit source tree).
+ *
+ * Conventions (mvkit include/mvkit/c/mv_tuple.h:18-32, mv_api.h:57-87):
+ *  - every function returns an MvStatus; nothing throws across the ABI;
+ *  - inputs are borrowed for the call, outputs are new owned tuples that the
+ *    caller releases with mv_destroy_tuple; all outputs of one call are
+ *    committed together or not at all;
+ *  - on failure the thread-local last error holds status, code, operation,
+ *    parameter, parameter index and message. */
+#ifndef MVKIT_AMD_MV_NCC_H
+#define MVKIT_AMD_MV_NCC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define MV_API __attribute__((visibility("default")))
+#else
+#define MV_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/mvkit/c/mv_status.h:10-24 */
+typedef int32_t MvStatus;
+#define MV_STATUS_OK ((int32_t)0)
+#define MV_STATUS_INVALID_ARGUMENT ((int32_t)-1)
+#define MV_STATUS_OUT_OF_RANGE ((int32_t)-2)
+#define MV_STATUS_TYPE_MISMATCH ((int32_t)-3)
+#define MV_STATUS_NOT_FOUND ((int32_t)-4)
+#define MV_STATUS_OUT_OF_MEMORY ((int32_t)-5)
+#define MV_STATUS_UNSUPPORTED ((int32_t)-6)
+#define MV_STATUS_INVALID_STATE ((int32_t)-7)
+#define MV_STATUS_IO_ERROR ((int32_t)-100)
+#define MV_STATUS_DECODE_ERROR ((int32_t)-101)
+#define MV_STATUS_ENCODE_ERROR ((int32_t)-102)
+#define MV_STATUS_LIMIT_EXCEEDED ((int32_t)-103)
+#define MV_STATUS_INTERNAL_ERROR ((int32_t)-1000)
+
+/* include/mvkit/c/mv_string.h:10-13 */
+typedef struct MvStringView { const char* data; size_t size; } MvStringView;
+
+/* include/mvkit/c/mv_value.h:12-20 */
+typedef struct MvHandleImpl* MvHandle;
+typedef int32_t MvValueKind;
+#define MV_VALUE_EMPTY ((int32_t)0)
+#define MV_VALUE_INT64 ((int32_t)1)
+#define MV_VALUE_REAL64 ((int32_t)2)
+#define MV_VALUE_STRING ((int32_t)4)
+#define MV_VALUE_HANDLE ((int32_t)16)
+
+/* include/mvkit/c/mv_tuple.h:18; mv_iconic.h:20 */
+typedef struct MvTupleImpl* MvTuple;
+typedef struct MvObjectImpl* MvObject;
+
+/* include/mvkit/c/mv_api.h:57-75 */
+typedef int32_t MvErrorCode;
+#define MV_ERROR_UNSPECIFIED ((int32_t)0)
+#define MV_ERROR_NULL_ARGUMENT ((int32_t)1)
+#define MV_ERROR_INVALID_CARDINALITY ((int32_t)2)
+#define MV_ERROR_INVALID_TYPE ((int32_t)3)
+#define MV_ERROR_INVALID_VALUE ((int32_t)4)
+#define MV_ERROR_INDEX_OUT_OF_RANGE ((int32_t)5)
+#define MV_ERROR_MISSING_ITEM ((int32_t)6)
+#define MV_ERROR_UNSUPPORTED_VALUE ((int32_t)7)
+#define MV_ERROR_INVALID_OBJECT_STATE ((int32_t)8)
+#define MV_ERROR_IO_FAILURE ((int32_t)9)
+#define MV_ERROR_INTERNAL_FAILURE ((int32_t)10)
+#define MV_ERROR_PARAMETER_INDEX_NONE ((int64_t)-1)
+typedef struct MvErrorInfo {
+  MvStatus status;
+  MvErrorCode code;
+  MvStringView operation;
+  MvStringView parameter;
+  int64_t parameter_index;
+  MvStringView message;
+} MvErrorInfo;
+
+/* include/mvkit/c/mv_api.h:75, 87 */
+MV_API MvStatus mv_get_last_error(MvStringView* error);
+MV_API MvStatus mv_get_last_error_info(MvErrorInfo* error);
+
+/* Control tuples: include/mvkit/c/mv_tuple.h:42-228 */
+MV_API MvStatus mv_create_tuple(MvTuple* tuple, size_t length);
+MV_API MvStatus mv_copy_tuple(MvTuple tuple, MvTuple* clone);
+MV_API MvStatus mv_destroy_tuple(MvTuple tuple);
+MV_API MvStatus mv_length_tuple(MvTuple tuple, size_t* size);
+MV_API MvStatus mv_get_type(MvTuple tuple, size_t index, MvValueKind* kind);
+MV_API MvStatus mv_get_i(MvTuple tuple, size_t index, int64_t* value);
+MV_API MvStatus mv_get_d(MvTuple tuple, size_t index, double* value);
+MV_API MvStatus mv_get_s(MvTuple tuple, size_t index, MvStringView* value);
+MV_API MvStatus mv_get_h(MvTuple tuple, size_t index, MvHandle* handle);
+MV_API MvStatus mv_resize_tuple(MvTuple tuple, size_t length);
+MV_API MvStatus mv_set_i(MvTuple tuple, int64_t value, size_t index);
+MV_API MvStatus mv_set_d(MvTuple tuple, double value, size_t index);
+MV_API MvStatus mv_set_s(MvTuple tuple, MvStringView value, size_t index);
+MV_API MvStatus mv_set_h(MvTuple tuple, MvHandle value, size_t index);
+
+/* Iconic objects: include/mvkit/c/mv_iconic.h:34-40; mv_image.h:45, 78.
+ * Type is "byte" or "uint2"; PixelPointer is the integer address of a
+ * row-major plane with stride == width.  gen_image1 copies the pixels;
+ * gen_image1_extern borrows them and calls ClearProc (integer address of a
+ * void(*)(void*), or 0) with the pointer when the object is cleared. */
+MV_API MvStatus mv_gen_empty_obj(MvObject* objects);
+MV_API MvStatus mv_clear_obj(MvObject objects);
+MV_API MvStatus mv_gen_image1(MvObject* Image, MvTuple Type, MvTuple Width, MvTuple Height, MvTuple PixelPointer);
+MV_API MvStatus mv_gen_image1_extern(MvObject* Image, MvTuple Type, MvTuple Width, MvTuple Height,
+                                     MvTuple PixelPointer, MvTuple ClearProc);
+
+/* Handles: include/mvkit/c/mv_handle.h:40.  Invalidates every alias. */
+MV_API MvStatus mv_clear_handle(MvTuple Handle);
+
+/* NCC operators: include/mvkit/c/mv_ncc_matching.h:30-123 */
+MV_API MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
+                                    MvTuple AngleStep, MvTuple Metric, MvTuple* ModelID);
+MV_API MvStatus mv_get_ncc_model_params(MvTuple ModelID, MvTuple* NumLevels, MvTuple* AngleStart,
+                                        MvTuple* AngleExtent, MvTuple* AngleStep, MvTuple* Metric);
+MV_API MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple AngleStart, MvTuple AngleExtent,
+                                  MvTuple MinScore, MvTuple NumMatches, MvTuple MaxOverlap, MvTuple SubPixel,
+                                  MvTuple NumLevels, MvTuple* Row, MvTuple* Column, MvTuple* Angle, MvTuple* Score);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

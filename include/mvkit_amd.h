@@ -1,0 +1,66 @@
+/* Extensions of the MI355X NCC build that have no counterpart in mvkit's C
+ * ABI: a device-resident find for callers whose images already live in HBM
+ * (benchmarks, GPU pipelines), a find that also returns every intermediate
+ * stage (for stage-by-stage parity checks), and read-only model
+ * introspection.  Results use the reference's internal pose convention
+ * (template centroid, no HALCON half-pixel shift). */
+#ifndef MVKIT_AMD_EXT_H
+#define MVKIT_AMD_EXT_H
+
+#include "mv_ncc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Coarse correlation forms (same values as NCCHIP_FORM_*). */
+#define MVAMD_FORM_FP32 0
+#define MVAMD_FORM_BF16X3 1
+#define MVAMD_FORM_FP64 2
+#define MVAMD_FORM_FP8_SCREEN 3
+
+typedef struct MvAmdFindOptions {
+  double angle_start, angle_extent, min_score, max_overlap;
+  int64_t num_matches;
+  int32_t subpixel;
+  int32_t highest_level, lowest_level;  /* 1-based; highest 0 = all */
+  int32_t form;                          /* MVAMD_FORM_* */
+  double screen;                         /* FP8 screen threshold (form FP8_SCREEN) */
+} MvAmdFindOptions;
+
+typedef struct MvAmdFindStats {
+  int32_t n_coarse, n_prepared, top, finest, angle_stride;
+  int32_t storage[4];
+  double t_coarse_us, t_track_us, t_refine_us, t_total_us;
+} MvAmdFindStats;
+
+/* Find on a u8 image given as a HIP device pointer on the calling thread's
+ * current device.  results: cap x 4 doubles (row, col, angle, score).
+ * Replaces, for device-resident inputs, mv_find_ncc_model
+ * (mvkit include/mvkit/c/mv_ncc_matching.h:71-74) minus the output shift. */
+MV_API MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* d_image, int64_t width, int64_t height,
+                                            int64_t pitch, const MvAmdFindOptions* options, double* results,
+                                            size_t cap, size_t* count, MvAmdFindStats* stats);
+
+/* Find on a host u8 image returning the stages: coarse peaks (cap_coarse x 4),
+ * full-scored candidates in score order (cap_prepared x 4) and, per such
+ * candidate, the refined pose and verdict (cap_prepared x 5). */
+MV_API MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* image, int64_t width, int64_t height,
+                                           int64_t pitch, const MvAmdFindOptions* options, double* results,
+                                           size_t cap, size_t* count, double* coarse, size_t cap_coarse,
+                                           size_t* n_coarse, double* prepared, double* refined, size_t cap_prepared,
+                                           size_t* n_prepared, MvAmdFindStats* stats);
+
+/* Model introspection.  info: levels, angles per level, angle step,
+ * origin row, origin column, sample count. */
+MV_API MvStatus mvamd_model_info(MvTuple ModelID, double* info);
+/* Raster of (level, angle index), level -1 = the unrotated original patch.
+ * header: n, angle, orow, ocol, rmin, rmax, cmin, cmax, weight sum, #runs,
+ * #weight runs.  rows/cols/weights receive up to cap samples. */
+MV_API MvStatus mvamd_model_raster(MvTuple ModelID, int32_t level, int32_t angle, double* header, int32_t* rows,
+                                   int32_t* cols, double* weights, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,293 @@
+"""ctypes binding of libmvkit_amd.so: the reference-compatible C ABI
+(include/mv_ncc.h) and the device/trace extensions (include/mvkit_amd.h).
+
+The NCC operators run on the GPU; importing this module only loads the
+library and fails loudly when it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmvkit_amd.so")
+
+MV_VALUE_EMPTY, MV_VALUE_INT64, MV_VALUE_REAL64, MV_VALUE_STRING, MV_VALUE_HANDLE = 0, 1, 2, 4, 16
+FORM_FP32, FORM_BF16X3, FORM_FP64, FORM_FP8_SCREEN = 0, 1, 2, 3
+
+STATUS_NAMES = {0: "OK", -1: "INVALID_ARGUMENT", -2: "OUT_OF_RANGE", -3: "TYPE_MISMATCH", -4: "NOT_FOUND",
+                -5: "OUT_OF_MEMORY", -6: "UNSUPPORTED", -7: "INVALID_STATE", -100: "IO_ERROR",
+                -101: "DECODE_ERROR", -102: "ENCODE_ERROR", -103: "LIMIT_EXCEEDED", -1000: "INTERNAL_ERROR"}
+
+# Symbols declared by include/mv_ncc.h and include/mvkit_amd.h.
+PUBLIC_SYMBOLS = [
+    "mv_get_last_error", "mv_get_last_error_info", "mv_create_tuple", "mv_copy_tuple", "mv_destroy_tuple",
+    "mv_length_tuple", "mv_get_type", "mv_get_i", "mv_get_d", "mv_get_s", "mv_get_h", "mv_resize_tuple", "mv_set_i",
+    "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
+    "mv_clear_handle", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model",
+    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
+]
+# Symbols declared by include/ncchip.h.
+NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_coarse", "ncchip_coarse_mfma",
+                  "ncchip_peaks", "ncchip_track", "ncchip_refine"]
+
+
+class StringView(C.Structure):
+    _fields_ = [("data", C.c_char_p), ("size", C.c_size_t)]
+
+    def text(self) -> str:
+        return C.string_at(self.data, self.size).decode() if self.data else ""
+
+
+class ErrorInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("code", C.c_int32), ("operation", StringView), ("parameter", StringView),
+                ("parameter_index", C.c_int64), ("message", StringView)]
+
+
+class FindOptions(C.Structure):
+    _fields_ = [("angle_start", C.c_double), ("angle_extent", C.c_double), ("min_score", C.c_double),
+                ("max_overlap", C.c_double), ("num_matches", C.c_int64), ("subpixel", C.c_int32),
+                ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
+                ("screen", C.c_double)]
+
+
+class FindStats(C.Structure):
+    _fields_ = [("n_coarse", C.c_int32), ("n_prepared", C.c_int32), ("top", C.c_int32), ("finest", C.c_int32),
+                ("angle_stride", C.c_int32), ("storage", C.c_int32 * 4), ("t_coarse_us", C.c_double),
+                ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double)]
+
+
+class MvError(RuntimeError):
+    def __init__(self, status: int, info: ErrorInfo | None):
+        self.status = status
+        self.code = info.code if info else 0
+        self.operation = info.operation.text() if info else ""
+        self.parameter = info.parameter.text() if info else ""
+        self.parameter_index = info.parameter_index if info else -1
+        self.message = info.message.text() if info else ""
+        super().__init__(f"{STATUS_NAMES.get(status, status)} in {self.operation}({self.parameter}): {self.message}")
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run mvkit_amd/build.py (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P, S, D, I64, I32 = C.c_void_p, C.c_size_t, C.c_double, C.c_int64, C.c_int32
+        for name in PUBLIC_SYMBOLS + NCCHIP_SYMBOLS:
+            getattr(L, name).restype = C.c_int32
+        L.mv_get_last_error_info.argtypes = [C.POINTER(ErrorInfo)]
+        L.mv_create_tuple.argtypes = [C.POINTER(P), S]
+        L.mv_copy_tuple.argtypes = [P, C.POINTER(P)]
+        L.mv_destroy_tuple.argtypes = [P]
+        L.mv_length_tuple.argtypes = [P, C.POINTER(S)]
+        L.mv_get_type.argtypes = [P, S, C.POINTER(I32)]
+        L.mv_get_i.argtypes = [P, S, C.POINTER(I64)]
+        L.mv_get_d.argtypes = [P, S, C.POINTER(D)]
+        L.mv_get_s.argtypes = [P, S, C.POINTER(StringView)]
+        L.mv_get_h.argtypes = [P, S, C.POINTER(P)]
+        L.mv_set_i.argtypes = [P, I64, S]
+        L.mv_set_d.argtypes = [P, D, S]
+        L.mv_set_s.argtypes = [P, StringView, S]
+        L.mv_set_h.argtypes = [P, P, S]
+        L.mv_gen_empty_obj.argtypes = [C.POINTER(P)]
+        L.mv_clear_obj.argtypes = [P]
+        L.mv_gen_image1.argtypes = [C.POINTER(P), P, P, P, P]
+        L.mv_gen_image1_extern.argtypes = [C.POINTER(P), P, P, P, P, P]
+        L.mv_clear_handle.argtypes = [P]
+        L.mv_create_ncc_model.argtypes = [P, P, P, P, P, P, C.POINTER(P)]
+        L.mv_get_ncc_model_params.argtypes = [P] + [C.POINTER(P)] * 5
+        L.mv_find_ncc_model.argtypes = [P] * 9 + [C.POINTER(P)] * 4
+        L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
+                                                  C.POINTER(FindStats)]
+        L.mvamd_find_ncc_model_trace.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S), P, S,
+                                                 C.POINTER(S), P, P, S, C.POINTER(S), C.POINTER(FindStats)]
+        L.mvamd_model_info.argtypes = [P, P]
+        L.mvamd_model_raster.argtypes = [P, I32, I32, P, P, P, P, S]
+        _lib = L
+    return _lib
+
+
+def _check(status: int):
+    if status != 0:
+        info = ErrorInfo()
+        lib().mv_get_last_error_info(C.byref(info))
+        raise MvError(status, info)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Tuple:
+    """Owned control tuple."""
+
+    def __init__(self, values=None, handle=None):
+        if handle is not None:
+            self.h = handle
+            return
+        vals = [] if values is None else (list(values) if isinstance(values, (list, tuple)) else [values])
+        h = C.c_void_p()
+        _check(lib().mv_create_tuple(C.byref(h), len(vals)))
+        self.h = h
+        self._keep = []
+        for i, v in enumerate(vals):
+            if isinstance(v, bool):
+                raise TypeError("booleans are not control values")
+            if isinstance(v, (int, np.integer)):
+                _check(lib().mv_set_i(self.h, int(v), i))
+            elif isinstance(v, (float, np.floating)):
+                _check(lib().mv_set_d(self.h, float(v), i))
+            elif isinstance(v, str):
+                b = v.encode()
+                self._keep.append(b)
+                _check(lib().mv_set_s(self.h, StringView(b, len(b)), i))
+            else:
+                raise TypeError(f"unsupported control value {v!r}")
+
+    def __len__(self):
+        n = C.c_size_t()
+        _check(lib().mv_length_tuple(self.h, C.byref(n)))
+        return n.value
+
+    def values(self) -> list:
+        out = []
+        for i in range(len(self)):
+            k = C.c_int32()
+            _check(lib().mv_get_type(self.h, i, C.byref(k)))
+            if k.value == MV_VALUE_INT64:
+                v = C.c_int64(); _check(lib().mv_get_i(self.h, i, C.byref(v))); out.append(v.value)
+            elif k.value == MV_VALUE_REAL64:
+                v = C.c_double(); _check(lib().mv_get_d(self.h, i, C.byref(v))); out.append(v.value)
+            elif k.value == MV_VALUE_STRING:
+                v = StringView(); _check(lib().mv_get_s(self.h, i, C.byref(v))); out.append(v.text())
+            elif k.value == MV_VALUE_HANDLE:
+                out.append("<handle>")
+            else:
+                out.append(None)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mv_destroy_tuple(self.h)
+            self.h = None
+
+
+class Image:
+    """Image object holding a copy of a u8 (or u16) plane."""
+
+    def __init__(self, pixels: np.ndarray | None):
+        h = C.c_void_p()
+        if pixels is None:
+            _check(lib().mv_gen_empty_obj(C.byref(h)))
+        else:
+            a = np.ascontiguousarray(pixels)
+            kind = {np.dtype(np.uint8): "byte", np.dtype(np.uint16): "uint2"}[a.dtype]
+            args = [Tuple(kind), Tuple(int(a.shape[1])), Tuple(int(a.shape[0])), Tuple(int(a.ctypes.data))]
+            _check(lib().mv_gen_image1(C.byref(h), *[t.h for t in args]))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mv_clear_obj(self.h)
+            self.h = None
+
+
+def _to_tuple(v):
+    return v if isinstance(v, Tuple) else Tuple(v)
+
+
+class NccModel:
+    """NCC model handle (mirrors mvkit's MvNCCModel, mv_ncc_matching.hpp)."""
+
+    def __init__(self, template: np.ndarray, num_levels="auto", angle_start=0.0, angle_extent=0.0,
+                 angle_step="auto", metric="use_polarity"):
+        img = Image(template)
+        out = C.c_void_p()
+        args = [_to_tuple(num_levels), _to_tuple(float(angle_start)), _to_tuple(float(angle_extent)),
+                _to_tuple(angle_step), _to_tuple(metric)]
+        _check(lib().mv_create_ncc_model(img.h, *[t.h for t in args], C.byref(out)))
+        self.tuple = Tuple(handle=out)
+        info = np.zeros(6)
+        _check(lib().mvamd_model_info(self.tuple.h, _ptr(info)))
+        self.n_levels, self.n_angles = int(info[0]), int(info[1])
+        self.step, self.origin_row, self.origin_col, self.sample_count = info[2], info[3], info[4], int(info[5])
+
+    def params(self):
+        outs = [C.c_void_p() for _ in range(5)]
+        _check(lib().mv_get_ncc_model_params(self.tuple.h, *[C.byref(o) for o in outs]))
+        return [Tuple(handle=o).values()[0] for o in outs]
+
+    def raster(self, level: int, angle: int) -> dict:
+        hdr = np.zeros(11)
+        _check(lib().mvamd_model_raster(self.tuple.h, level, angle, _ptr(hdr), None, None, None, 0))
+        n = int(hdr[0])
+        rows, cols, w = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        _check(lib().mvamd_model_raster(self.tuple.h, level, angle, _ptr(hdr), _ptr(rows), _ptr(cols), _ptr(w), n))
+        return {"n": n, "angle": hdr[1], "orow": hdr[2], "ocol": hdr[3], "rmin": int(hdr[4]), "rmax": int(hdr[5]),
+                "cmin": int(hdr[6]), "cmax": int(hdr[7]), "wsum": hdr[8], "n_runs": int(hdr[9]),
+                "n_wruns": int(hdr[10]), "rows": rows, "cols": cols, "w": w}
+
+    def clear(self):
+        _check(lib().mv_clear_handle(self.tuple.h))
+
+    def find(self, image, angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5,
+             subpixel="true", num_levels=0):
+        """mv_find_ncc_model; returns (n, 4) array of Row, Column, Angle, Score
+        in the C ABI's HALCON convention."""
+        img = image if isinstance(image, Image) else Image(image)
+        outs = [C.c_void_p() for _ in range(4)]
+        args = [_to_tuple(float(angle_start)), _to_tuple(float(angle_extent)), _to_tuple(float(min_score)),
+                _to_tuple(int(num_matches)), _to_tuple(float(max_overlap)), _to_tuple(subpixel),
+                _to_tuple(num_levels)]
+        _check(lib().mv_find_ncc_model(img.h, self.tuple.h, *[t.h for t in args], *[C.byref(o) for o in outs]))
+        cols = [Tuple(handle=o).values() for o in outs]
+        return np.array(cols, dtype=np.float64).T.reshape(-1, 4)
+
+    @staticmethod
+    def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
+                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf) -> FindOptions:
+        return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
+                           lowest, form, screen)
+
+    def find_device(self, d_image: int, width: int, height: int, pitch: int, opts: FindOptions, cap: int = 4096):
+        """Find on a device-resident u8 image (integer device address)."""
+        res = np.zeros((cap, 4))
+        n = C.c_size_t()
+        st = FindStats()
+        _check(lib().mvamd_find_ncc_model_device(self.tuple.h, C.c_void_p(d_image), width, height, pitch,
+                                                 C.byref(opts), _ptr(res), cap, C.byref(n), C.byref(st)))
+        return res[:n.value].copy(), st
+
+    def find_trace(self, image: np.ndarray, opts: FindOptions, cap: int = 20000):
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        res = np.zeros((4096, 4))
+        coarse = np.zeros((cap, 4))
+        prep = np.zeros((cap, 4))
+        refd = np.zeros((cap, 5))
+        n, nc, nprep = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        st = FindStats()
+        _check(lib().mvamd_find_ncc_model_trace(self.tuple.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1],
+                                                C.byref(opts), _ptr(res), 4096, C.byref(n), _ptr(coarse), cap,
+                                                C.byref(nc), _ptr(prep), _ptr(refd), cap, C.byref(nprep),
+                                                C.byref(st)))
+        return {"results": res[:n.value].copy(), "coarse": coarse[:min(nc.value, cap)].copy(),
+                "prepared": prep[:min(nprep.value, cap)].copy(), "refined": refd[:min(nprep.value, cap)].copy(),
+                "stats": st}
+
+
+def halcon_shift(poses: np.ndarray) -> np.ndarray:
+    """Internal (centroid) poses -> the C ABI's Row/Column convention
+    (ncc_matching_api.cpp:171-176)."""
+    out = poses.copy()
+    c, s = np.cos(poses[:, 2]), np.sin(poses[:, 2])
+    out[:, 0] = poses[:, 0] + 0.5 * (1 - c + s)
+    out[:, 1] = poses[:, 1] + 0.5 * (1 - c - s)
+    return out

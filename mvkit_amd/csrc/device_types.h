@@ -1,0 +1,65 @@
+// Plain-C descriptors shared by the host driver and the HIP kernels.
+// Every field is a plain integer, floating-point value or raw device pointer.
+#ifndef MVKIT_AMD_DEVICE_TYPES_H
+#define MVKIT_AMD_DEVICE_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Pixel storage of one search-pyramid level.  Level 0 is the caller's u8
+// image; level l >= 1 stores the exact integer block sum S_l of the 4^l
+// source pixels (u16 while it fits, i.e. l <= 4, u32 beyond).  The value the
+// reference computes in FP64 (matcher.cpp:436-454) is S_l * 4^-l - dc.
+enum { NCC_U8 = 0, NCC_U16 = 1, NCC_U32 = 2 };
+
+typedef struct NccLevel {
+  const void* data;   // device pointer
+  int32_t kind;       // NCC_U8 / NCC_U16 / NCC_U32
+  int32_t pitch;      // elements per row
+  int32_t width, height;
+  int32_t shift;      // 2*l: value = S * 2^-shift - dc
+  int32_t pad;
+} NccLevel;
+
+// One rotated template raster on the device (model.hpp:30-43).  Sample
+// coordinates are packed as (row << 16) | (col & 0xffff) in `rc`.
+typedef struct NccPatch {
+  double angle, orow, ocol, wsum;
+  double wsum_f;             // sum of the fp32 weights (exact in FP64)
+  int32_t n, rmin, rmax, cmin, cmax;
+  int32_t sample_off;        // into rc / w / wf
+  int32_t run_off, n_runs;   // into runs (int4: row, begin, end, 0)
+  int32_t wrun_off, n_wruns; // into wruns (int4) / wrun_w (double)
+  int32_t use_wruns;         // 2*#weight_runs < #samples (kernels_avx2.cpp:150, 175, 211)
+  int32_t dense_off;         // dense bbox weights for the correlation kernels (-1: none)
+  int32_t dense_h, dense_w;  // bbox rows / padded columns of the dense block
+  int32_t pad;
+} NccPatch;
+
+// Device-resident pool of patches (all pointers are device pointers).
+typedef struct NccPatchPool {
+  const NccPatch* patches;
+  const int32_t* rc;
+  const double* w;
+  const float* wf;
+  const int32_t* runs;      // 4 ints per run
+  const int32_t* wruns;     // 4 ints per weight run
+  const double* wrun_w;
+  const float* dense_f32;   // dense bbox weights, fp32
+  const uint16_t* dense_hi; // dense bbox weights, bf16 high part
+  const uint16_t* dense_lo; // dense bbox weights, bf16 low part (w - hi)
+} NccPatchPool;
+
+// A pose in the reference's internal (centroid) convention (model.hpp:26).
+typedef struct NccMatch { double row, col, angle, score; } NccMatch;
+
+// A coarse peak: angle index into the coarse patch list, anchor and score.
+typedef struct NccPeak { int32_t angle, row, col, pad; double score; } NccPeak;
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,657 @@
+// C ABI of the NCC find path: control tuples, image objects, model handles,
+// thread-local error state and the NCC operators, with the reference's
+// argument rules (src/c_api/ncc_matching_api.cpp, tuple_input.hpp,
+// error_state.hpp).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <variant>
+#include <vector>
+
+#include "../../include/mv_ncc.h"
+#include "../../include/mvkit_amd.h"
+#include "ncc_device.hpp"
+#include "ncc_model.hpp"
+
+using mvamd::Error;
+
+// ------------------------------------------------------------ object model
+struct ModelObject {
+  std::mutex mu;
+  std::shared_ptr<mvamd::NccModel> model;  // null once cleared
+};
+
+struct MvHandleImpl {
+  std::shared_ptr<ModelObject> object;
+};
+
+struct Value {
+  MvValueKind kind = MV_VALUE_EMPTY;
+  int64_t i = 0;
+  double d = 0;
+  std::string s;
+  std::unique_ptr<MvHandleImpl> h;
+  Value() = default;
+  Value(const Value& o) : kind(o.kind), i(o.i), d(o.d), s(o.s) {
+    if (o.h) h = std::make_unique<MvHandleImpl>(*o.h);
+  }
+  Value& operator=(const Value& o) {
+    if (this != &o) {
+      kind = o.kind; i = o.i; d = o.d; s = o.s;
+      h = o.h ? std::make_unique<MvHandleImpl>(*o.h) : nullptr;
+    }
+    return *this;
+  }
+};
+
+struct MvTupleImpl {
+  std::vector<Value> values;
+};
+
+struct MvObjectImpl {
+  bool empty = true;
+  size_t width = 0, height = 0;
+  int pixel_bytes = 1;  // 1: byte, 2: uint2
+  std::vector<uint8_t> owned;
+  const uint8_t* pixels = nullptr;
+  void (*clear)(void*) = nullptr;
+  ~MvObjectImpl() {
+    if (clear && pixels) clear(const_cast<uint8_t*>(pixels));
+  }
+};
+
+// ------------------------------------------------------------- error state
+namespace {
+struct LastError {
+  MvStatus status = MV_STATUS_OK;
+  MvErrorCode code = MV_ERROR_UNSPECIFIED;
+  std::string operation, parameter, message, text;
+  int64_t index = MV_ERROR_PARAMETER_INDEX_NONE;
+};
+thread_local LastError g_error;
+
+void set_error(MvStatus st, MvErrorCode code, std::string op, std::string param, std::string msg, int64_t index) {
+  g_error.status = st; g_error.code = code; g_error.operation = std::move(op); g_error.parameter = std::move(param);
+  g_error.message = std::move(msg); g_error.index = index;
+  g_error.text = g_error.operation + (g_error.parameter.empty() ? "" : "(" + g_error.parameter + ")") + ": " +
+                 g_error.message;
+}
+
+// Exceptions never cross the ABI (error_state.hpp:28-57).
+template <class F>
+MvStatus translate(F&& f) noexcept {
+  try {
+    f();
+    g_error = LastError{};
+    return MV_STATUS_OK;
+  } catch (const Error& e) {
+    set_error(e.status, e.code, e.operation, e.parameter, e.message, e.index);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_error(MV_STATUS_OUT_OF_MEMORY, MV_ERROR_INTERNAL_FAILURE, "", "", "out of memory", -1);
+    return MV_STATUS_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    set_error(MV_STATUS_INTERNAL_ERROR, MV_ERROR_INTERNAL_FAILURE, "", "", e.what(), -1);
+    return MV_STATUS_INTERNAL_ERROR;
+  } catch (...) {
+    set_error(MV_STATUS_INTERNAL_ERROR, MV_ERROR_INTERNAL_FAILURE, "", "", "unknown internal error", -1);
+    return MV_STATUS_INTERNAL_ERROR;
+  }
+}
+
+[[noreturn]] void fail(mvamd::Status st, mvamd::ErrorCode code, const char* op, const char* param, const char* msg,
+                       int64_t index = -1) {
+  throw Error(st, op, param, msg, index, code);
+}
+void required(const void* p, const char* op, const char* param) {
+  if (!p) fail(mvamd::kInvalidArgument, mvamd::kErrNullArgument, op, param, "must not be null");
+}
+
+// Read-only view of an input tuple with the reference's coercions
+// (tuple_input.hpp:79-133).  A null tuple is an empty input.
+class Input {
+ public:
+  Input(MvTuple t, const char* op, const char* param) : t_(t), op_(op), param_(param) {}
+  size_t size() const { return t_ ? t_->values.size() : 0; }
+  const Value& at(size_t i) const {
+    if (i >= size()) fail(mvamd::kInvalidArgument, mvamd::kErrIndexOutOfRange, op_, param_, "control tuple index out of range", int64_t(i));
+    return t_->values[i];
+  }
+  [[noreturn]] void bad(mvamd::ErrorCode code, int64_t index, const char* msg) const {
+    fail(code == mvamd::kErrInvalidType ? mvamd::kTypeMismatch : mvamd::kInvalidArgument, code, op_, param_, msg, index);
+  }
+  double number(size_t i) const {
+    const Value& v = at(i);
+    if (v.kind == MV_VALUE_INT64) return double(v.i);
+    if (v.kind == MV_VALUE_REAL64) return v.d;
+    bad(mvamd::kErrInvalidType, int64_t(i), "numeric control value required");
+  }
+  double scalar_number() const {
+    if (size() != 1) bad(mvamd::kErrInvalidCardinality, -1, "exactly one control value required");
+    const double v = number(0);
+    if (!std::isfinite(v)) bad(mvamd::kErrInvalidValue, 0, "finite control value required");
+    return v;
+  }
+  int64_t integer(size_t i) const {
+    const Value& v = at(i);
+    if (v.kind == MV_VALUE_INT64) return v.i;
+    bad(mvamd::kErrInvalidType, int64_t(i), "integer control value required");
+  }
+  int64_t scalar_integer() const {
+    if (size() != 1) bad(mvamd::kErrInvalidCardinality, -1, "exactly one control value required");
+    return integer(0);
+  }
+  const std::string& string(size_t i) const {
+    const Value& v = at(i);
+    if (v.kind != MV_VALUE_STRING) bad(mvamd::kErrInvalidType, int64_t(i), "string control value required");
+    return v.s;
+  }
+  MvValueKind kind(size_t i) const { return at(i).kind; }
+
+ private:
+  MvTuple t_;
+  const char* op_;
+  const char* param_;
+};
+
+// All outputs of one call are committed together (control_operator_outputs.hpp).
+template <size_t N>
+class Outputs {
+ public:
+  Outputs(const char* op, std::initializer_list<MvTuple*> targets) {
+    size_t k = 0;
+    for (MvTuple* t : targets) {
+      required(t, op, "output");
+      targets_[k++] = t;
+    }
+    for (auto& v : values_) v = std::make_unique<MvTupleImpl>();
+  }
+  MvTupleImpl& operator[](size_t i) { return *values_[i]; }
+  void commit() {
+    for (size_t i = 0; i < N; ++i) *targets_[i] = values_[i].release();
+  }
+
+ private:
+  MvTuple* targets_[N];
+  std::unique_ptr<MvTupleImpl> values_[N];
+};
+
+void push(MvTupleImpl& t, double v) { Value x; x.kind = MV_VALUE_REAL64; x.d = v; t.values.push_back(std::move(x)); }
+void push(MvTupleImpl& t, int64_t v) { Value x; x.kind = MV_VALUE_INT64; x.i = v; t.values.push_back(std::move(x)); }
+void push(MvTupleImpl& t, const char* v) { Value x; x.kind = MV_VALUE_STRING; x.s = v; t.values.push_back(std::move(x)); }
+
+const MvObjectImpl& single_image(MvObject o, const char* op, const char* param) {
+  required(o, op, param);
+  if (o->empty) fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, param, "one image required");
+  return *o;
+}
+
+std::shared_ptr<mvamd::NccModel> model_of(MvTuple t, const char* op) {
+  const Input view(t, op, "ModelID");
+  if (view.size() != 1) view.bad(mvamd::kErrInvalidCardinality, -1, "exactly one NCC model required");
+  if (view.kind(0) != MV_VALUE_HANDLE) view.bad(mvamd::kErrInvalidType, 0, "NCC model handle required");
+  const Value& v = view.at(0);
+  if (!v.h || !v.h->object) view.bad(mvamd::kErrInvalidType, 0, "control handle is not an NCC model");
+  std::lock_guard<std::mutex> lock(v.h->object->mu);
+  if (!v.h->object->model)
+    fail(mvamd::kInvalidState, mvamd::kErrInvalidObjectState, op, "ModelID", "NCC model handle was cleared", 0);
+  return v.h->object->model;
+}
+
+// Per-thread find pipelines, one per device.
+mvamd::FindContext& context() {
+  thread_local std::vector<std::unique_ptr<mvamd::FindContext>> contexts;
+  int dev = 0;
+  mvamd::check(hipGetDevice(&dev), "hipGetDevice");
+  if (contexts.size() <= size_t(dev)) contexts.resize(size_t(dev) + 1);
+  if (!contexts[size_t(dev)]) contexts[size_t(dev)] = std::make_unique<mvamd::FindContext>();
+  return *contexts[size_t(dev)];
+}
+
+// NccControls for one model (ncc_matching_api.cpp:126-161).
+mvamd::FindOptions controls(const char* op, MvTuple a, MvTuple e, MvTuple s, MvTuple n, MvTuple o, MvTuple p, MvTuple l) {
+  const Input start(a, op, "AngleStart"), extent(e, op, "AngleExtent"), score(s, op, "MinScore"),
+      matches(n, op, "NumMatches"), overlap(o, op, "MaxOverlap"), subpixel(p, op, "SubPixel"), levels(l, op, "NumLevels");
+  for (const Input* c : {&start, &extent, &score, &matches, &overlap, &subpixel})
+    if (c->size() != 1) c->bad(mvamd::kErrInvalidCardinality, -1, "one value or one value per model required");
+  if (levels.size() != 1 && levels.size() != 2)
+    levels.bad(mvamd::kErrInvalidCardinality, -1, "highest level or shared/per-model level pairs required");
+  auto number = [](const Input& v, double lo, double hi) {
+    const double x = v.number(0);
+    if (!std::isfinite(x) || x < lo || x > hi) v.bad(mvamd::kErrInvalidValue, 0, "numeric control outside allowed range");
+    return x;
+  };
+  mvamd::FindOptions q;
+  q.start = number(start, -std::numeric_limits<double>::max(), std::numeric_limits<double>::max());
+  q.extent = number(extent, 0, 6.28318530717958647692);
+  q.min_score = number(score, 0, 1);
+  q.max_overlap = number(overlap, 0, 1);
+  const int64_t quota = matches.integer(0);
+  if (quota < 0) matches.bad(mvamd::kErrInvalidValue, 0, "match count must be nonnegative");
+  q.num_matches = size_t(quota);
+  const std::string& mode = subpixel.string(0);
+  if (mode != "true" && mode != "false") subpixel.bad(mvamd::kErrUnsupportedValue, 0, "expected true or false");
+  q.subpixel = mode == "true";
+  const int64_t highest = levels.integer(0), lowest = levels.size() == 2 ? levels.integer(1) : 1;
+  if (highest < 0 || highest > std::numeric_limits<int>::max() || lowest < 0 || lowest > std::numeric_limits<int>::max())
+    levels.bad(mvamd::kErrInvalidValue, 0, "invalid highest/lowest level pair");
+  q.highest = int(highest);
+  q.lowest = int(std::max<int64_t>(1, lowest));
+  return q;
+}
+
+mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
+  required(o, "find_ncc_model", "options");
+  mvamd::FindOptions q;
+  q.start = o->angle_start; q.extent = o->angle_extent; q.min_score = o->min_score; q.max_overlap = o->max_overlap;
+  if (o->num_matches < 0) fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, "find_ncc_model", "NumMatches", "match count must be nonnegative");
+  q.num_matches = size_t(o->num_matches);
+  q.subpixel = o->subpixel != 0;
+  q.highest = o->highest_level; q.lowest = std::max(1, o->lowest_level);
+  q.form = o->form; q.screen = o->screen;
+  if (q.form < MVAMD_FORM_FP32 || q.form > MVAMD_FORM_FP8_SCREEN)
+    fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
+  return q;
+}
+
+// Rethrows algorithm errors under the C-ABI operation name (ncc_matching_api.cpp:21-26).
+template <class F>
+auto algorithm(const char* op, F&& f) -> decltype(f()) {
+  try {
+    return f();
+  } catch (Error& e) {
+    e.operation = op;
+    throw;
+  }
+}
+
+void put_handle(MvTupleImpl& t, std::shared_ptr<mvamd::NccModel> m) {
+  Value v;
+  v.kind = MV_VALUE_HANDLE;
+  v.h = std::make_unique<MvHandleImpl>();
+  v.h->object = std::make_shared<ModelObject>();
+  v.h->object->model = std::move(m);
+  t.values.push_back(std::move(v));
+}
+}  // namespace
+
+// ================================================================= errors
+extern "C" MvStatus mv_get_last_error(MvStringView* error) {
+  if (!error) return MV_STATUS_INVALID_ARGUMENT;
+  *error = {g_error.text.c_str(), g_error.text.size()};
+  return MV_STATUS_OK;
+}
+extern "C" MvStatus mv_get_last_error_info(MvErrorInfo* e) {
+  if (!e) return MV_STATUS_INVALID_ARGUMENT;
+  e->status = g_error.status;
+  e->code = g_error.code;
+  e->operation = {g_error.operation.c_str(), g_error.operation.size()};
+  e->parameter = {g_error.parameter.c_str(), g_error.parameter.size()};
+  e->parameter_index = g_error.index;
+  e->message = {g_error.message.c_str(), g_error.message.size()};
+  return MV_STATUS_OK;
+}
+
+// ================================================================= tuples
+extern "C" MvStatus mv_create_tuple(MvTuple* tuple, size_t length) {
+  return translate([&] {
+    required(tuple, "create_tuple", "tuple");
+    auto t = std::make_unique<MvTupleImpl>();
+    t->values.resize(length);
+    *tuple = t.release();
+  });
+}
+extern "C" MvStatus mv_copy_tuple(MvTuple tuple, MvTuple* clone) {
+  return translate([&] {
+    required(tuple, "copy_tuple", "tuple");
+    required(clone, "copy_tuple", "clone");
+    *clone = new MvTupleImpl(*tuple);
+  });
+}
+extern "C" MvStatus mv_destroy_tuple(MvTuple tuple) {
+  delete tuple;
+  return MV_STATUS_OK;
+}
+extern "C" MvStatus mv_length_tuple(MvTuple tuple, size_t* size) {
+  return translate([&] {
+    required(tuple, "length_tuple", "tuple");
+    required(size, "length_tuple", "size");
+    *size = tuple->values.size();
+  });
+}
+extern "C" MvStatus mv_resize_tuple(MvTuple tuple, size_t length) {
+  return translate([&] {
+    required(tuple, "resize_tuple", "tuple");
+    tuple->values.resize(length);
+  });
+}
+namespace {
+Value& slot(MvTuple t, size_t index, const char* op) {
+  required(t, op, "tuple");
+  if (index >= t->values.size())
+    fail(mvamd::kOutOfRange, mvamd::kErrIndexOutOfRange, op, "index", "tuple index out of range", int64_t(index));
+  return t->values[index];
+}
+}  // namespace
+extern "C" MvStatus mv_get_type(MvTuple tuple, size_t index, MvValueKind* kind) {
+  return translate([&] { required(kind, "get_type", "kind"); *kind = slot(tuple, index, "get_type").kind; });
+}
+extern "C" MvStatus mv_get_i(MvTuple tuple, size_t index, int64_t* value) {
+  return translate([&] {
+    required(value, "get_i", "value");
+    const Value& v = slot(tuple, index, "get_i");
+    if (v.kind != MV_VALUE_INT64) fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, "get_i", "tuple", "integer element required", int64_t(index));
+    *value = v.i;
+  });
+}
+extern "C" MvStatus mv_get_d(MvTuple tuple, size_t index, double* value) {
+  return translate([&] {
+    required(value, "get_d", "value");
+    const Value& v = slot(tuple, index, "get_d");
+    if (v.kind == MV_VALUE_REAL64) *value = v.d;
+    else if (v.kind == MV_VALUE_INT64) *value = double(v.i);
+    else fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, "get_d", "tuple", "numeric element required", int64_t(index));
+  });
+}
+extern "C" MvStatus mv_get_s(MvTuple tuple, size_t index, MvStringView* value) {
+  return translate([&] {
+    required(value, "get_s", "value");
+    const Value& v = slot(tuple, index, "get_s");
+    if (v.kind != MV_VALUE_STRING) fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, "get_s", "tuple", "string element required", int64_t(index));
+    *value = {v.s.c_str(), v.s.size()};
+  });
+}
+extern "C" MvStatus mv_get_h(MvTuple tuple, size_t index, MvHandle* handle) {
+  return translate([&] {
+    required(handle, "get_h", "handle");
+    const Value& v = slot(tuple, index, "get_h");
+    if (v.kind != MV_VALUE_HANDLE) fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, "get_h", "tuple", "handle element required", int64_t(index));
+    *handle = v.h.get();
+  });
+}
+extern "C" MvStatus mv_set_i(MvTuple tuple, int64_t value, size_t index) {
+  return translate([&] { Value& v = slot(tuple, index, "set_i"); v = Value{}; v.kind = MV_VALUE_INT64; v.i = value; });
+}
+extern "C" MvStatus mv_set_d(MvTuple tuple, double value, size_t index) {
+  return translate([&] { Value& v = slot(tuple, index, "set_d"); v = Value{}; v.kind = MV_VALUE_REAL64; v.d = value; });
+}
+extern "C" MvStatus mv_set_s(MvTuple tuple, MvStringView value, size_t index) {
+  return translate([&] {
+    if (!value.data && value.size) fail(mvamd::kInvalidArgument, mvamd::kErrNullArgument, "set_s", "value", "string data must not be null");
+    Value& v = slot(tuple, index, "set_s");
+    v = Value{};
+    v.kind = MV_VALUE_STRING;
+    v.s.assign(value.data ? value.data : "", value.size);
+  });
+}
+extern "C" MvStatus mv_set_h(MvTuple tuple, MvHandle value, size_t index) {
+  return translate([&] {
+    required(value, "set_h", "value");
+    Value& v = slot(tuple, index, "set_h");
+    Value x;
+    x.kind = MV_VALUE_HANDLE;
+    x.h = std::make_unique<MvHandleImpl>(*value);
+    v = std::move(x);
+  });
+}
+
+// ================================================================ objects
+extern "C" MvStatus mv_gen_empty_obj(MvObject* objects) {
+  return translate([&] { required(objects, "gen_empty_obj", "objects"); *objects = new MvObjectImpl(); });
+}
+extern "C" MvStatus mv_clear_obj(MvObject objects) {
+  delete objects;
+  return MV_STATUS_OK;
+}
+namespace {
+void image_controls(const char* op, MvTuple type, MvTuple width, MvTuple height, MvTuple pointer, int& bytes,
+                    size_t& w, size_t& h, uintptr_t& address) {
+  const Input t(type, op, "Type"), wi(width, op, "Width"), hi(height, op, "Height"), pi(pointer, op, "PixelPointer");
+  if (t.size() != 1) t.bad(mvamd::kErrInvalidCardinality, -1, "exactly one pixel type required");
+  const std::string& name = t.string(0);
+  if (name == "byte") bytes = 1;
+  else if (name == "uint2") bytes = 2;
+  else fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Type", "unsupported pixel type", 0);
+  const int64_t ww = wi.scalar_integer(), hh = hi.scalar_integer();
+  if (ww <= 0 || hh <= 0) fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Width", "positive image size required");
+  w = size_t(ww); h = size_t(hh);
+  const int64_t addr = pi.scalar_integer();
+  if (!addr) fail(mvamd::kInvalidArgument, mvamd::kErrNullArgument, op, "PixelPointer", "must not be null");
+  address = uintptr_t(addr);
+}
+}  // namespace
+extern "C" MvStatus mv_gen_image1(MvObject* Image, MvTuple Type, MvTuple Width, MvTuple Height, MvTuple PixelPointer) {
+  return translate([&] {
+    constexpr auto op = "gen_image1";
+    required(Image, op, "Image");
+    int bytes; size_t w, h; uintptr_t addr;
+    image_controls(op, Type, Width, Height, PixelPointer, bytes, w, h, addr);
+    auto img = std::make_unique<MvObjectImpl>();
+    img->empty = false; img->width = w; img->height = h; img->pixel_bytes = bytes;
+    img->owned.assign(reinterpret_cast<const uint8_t*>(addr), reinterpret_cast<const uint8_t*>(addr) + w * h * bytes);
+    img->pixels = img->owned.data();
+    *Image = img.release();
+  });
+}
+extern "C" MvStatus mv_gen_image1_extern(MvObject* Image, MvTuple Type, MvTuple Width, MvTuple Height,
+                                         MvTuple PixelPointer, MvTuple ClearProc) {
+  return translate([&] {
+    constexpr auto op = "gen_image1_extern";
+    required(Image, op, "Image");
+    int bytes; size_t w, h; uintptr_t addr;
+    image_controls(op, Type, Width, Height, PixelPointer, bytes, w, h, addr);
+    const Input cp(ClearProc, op, "ClearProc");
+    const int64_t proc = cp.size() ? cp.scalar_integer() : 0;
+    auto img = std::make_unique<MvObjectImpl>();
+    img->empty = false; img->width = w; img->height = h; img->pixel_bytes = bytes;
+    img->pixels = reinterpret_cast<const uint8_t*>(addr);
+    img->clear = reinterpret_cast<void (*)(void*)>(uintptr_t(proc));
+    *Image = img.release();
+  });
+}
+
+// ================================================================ handles
+extern "C" MvStatus mv_clear_handle(MvTuple Handle) {
+  return translate([&] {
+    const Input view(Handle, "clear_handle", "Handle");
+    for (size_t i = 0; i < view.size(); ++i) {
+      const Value& v = view.at(i);
+      if (v.kind != MV_VALUE_HANDLE || !v.h || !v.h->object) view.bad(mvamd::kErrInvalidType, int64_t(i), "handle required");
+      std::lock_guard<std::mutex> lock(v.h->object->mu);
+      v.h->object->model.reset();
+    }
+  });
+}
+
+// ============================================================ NCC operators
+extern "C" MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
+                                        MvTuple AngleStep, MvTuple Metric, MvTuple* ModelID) {
+  return translate([&] {
+    constexpr auto op = "create_ncc_model";
+    Outputs<1> out(op, {ModelID});
+    const MvObjectImpl& src = single_image(Template, op, "Template");
+    const Input levels(NumLevels, op, "NumLevels"), start(AngleStart, op, "AngleStart"),
+        extent(AngleExtent, op, "AngleExtent"), step(AngleStep, op, "AngleStep"), metric(Metric, op, "Metric");
+    auto automatic = [](const Input& in) {
+      if (in.size() != 1) in.bad(mvamd::kErrInvalidCardinality, -1, "exactly one value required");
+      if (in.kind(0) != MV_VALUE_STRING) return false;
+      if (in.string(0) != "auto") in.bad(mvamd::kErrUnsupportedValue, 0, "expected auto");
+      return true;
+    };
+    mvamd::CreateParams p;
+    if (!automatic(levels)) {
+      const int64_t n = levels.scalar_integer();
+      if (n < 0 || n > 16) levels.bad(mvamd::kErrInvalidValue, 0, "levels must be 0..16 or auto");
+      p.levels = int(n);
+    }
+    p.start = start.scalar_number();
+    p.extent = extent.scalar_number();
+    if (!automatic(step)) {
+      p.step = step.scalar_number();
+      if (p.step < 0 || p.step > 3.14159265358979323846 / 16) step.bad(mvamd::kErrInvalidValue, 0, "angle step must be in [0,pi/16] or auto");
+    }
+    if (metric.size() != 1) metric.bad(mvamd::kErrInvalidCardinality, -1, "exactly one metric required");
+    const std::string& name = metric.string(0);
+    if (name == "ignore_global_polarity") p.absolute = true;
+    else if (name != "use_polarity") metric.bad(mvamd::kErrUnsupportedValue, 0, "unsupported NCC metric");
+    std::shared_ptr<mvamd::NccModel> m = algorithm(op, [&] {
+      return mvamd::create_model(src.pixels, src.pixel_bytes, src.width, src.height, src.width * src.pixel_bytes, p);
+    });
+    put_handle(out[0], std::move(m));
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_get_ncc_model_params(MvTuple ModelID, MvTuple* NumLevels, MvTuple* AngleStart,
+                                            MvTuple* AngleExtent, MvTuple* AngleStep, MvTuple* Metric) {
+  return translate([&] {
+    constexpr auto op = "get_ncc_model_params";
+    Outputs<5> out(op, {NumLevels, AngleStart, AngleExtent, AngleStep, Metric});
+    const auto m = model_of(ModelID, op);
+    push(out[0], int64_t(m->levels.size()));
+    push(out[1], m->params.start);
+    push(out[2], m->params.extent);
+    push(out[3], m->params.extent == 0 ? 0. : m->params.step);
+    push(out[4], m->params.absolute ? "ignore_global_polarity" : "use_polarity");
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple AngleStart, MvTuple AngleExtent,
+                                      MvTuple MinScore, MvTuple NumMatches, MvTuple MaxOverlap, MvTuple SubPixel,
+                                      MvTuple NumLevels, MvTuple* Row, MvTuple* Column, MvTuple* Angle, MvTuple* Score) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    Outputs<4> out(op, {Row, Column, Angle, Score});
+    required(Image, op, "Image");
+    const auto m = model_of(ModelID, op);
+    const mvamd::FindOptions q = controls(op, AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel, NumLevels);
+    if (Image->empty) { out.commit(); return; }
+    if (Image->pixel_bytes != 1)
+      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Image", "uint2 search images are not supported on the device path");
+    const auto found = algorithm(op, [&] {
+      return context().find_host(*m, Image->pixels, Image->width, Image->height, Image->width, q);
+    });
+    // HALCON pixel-edge convention of the translation (ncc_matching_api.cpp:171-176)
+    for (const auto& p : found) {
+      const double c = std::cos(p.angle), s = std::sin(p.angle);
+      push(out[0], p.row + .5 * (1 - c + s));
+      push(out[1], p.col + .5 * (1 - c - s));
+      push(out[2], p.angle);
+      push(out[3], p.score);
+    }
+    out.commit();
+  });
+}
+
+// ============================================================ extensions
+namespace {
+void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
+  if (!s) return;
+  s->n_coarse = int32_t(t.coarse.size());
+  s->n_prepared = int32_t(t.prepared.size());
+  s->top = t.top; s->finest = t.finest; s->angle_stride = t.angle_stride;
+  std::copy(t.storage, t.storage + 4, s->storage);
+  s->t_coarse_us = t.t_coarse_us; s->t_track_us = t.t_track_us; s->t_refine_us = t.t_refine_us;
+  s->t_total_us = t.t_total_us;
+}
+void put_poses(const std::vector<mvamd::Pose>& v, double* dst, size_t cap) {
+  for (size_t i = 0; i < v.size() && i < cap; ++i) {
+    dst[4 * i] = v[i].row; dst[4 * i + 1] = v[i].col; dst[4 * i + 2] = v[i].angle; dst[4 * i + 3] = v[i].score;
+  }
+}
+}  // namespace
+
+extern "C" MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* d_image, int64_t width, int64_t height,
+                                                int64_t pitch, const MvAmdFindOptions* options, double* results,
+                                                size_t cap, size_t* count, MvAmdFindStats* stats) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    required(d_image, op, "Image");
+    required(count, op, "count");
+    if (width <= 0 || height <= 0 || pitch < width)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
+    const auto m = model_of(ModelID, op);
+    const mvamd::FindOptions q = from_ext(options);
+    mvamd::FindTrace trace;
+    const auto found = algorithm(op, [&] {
+      return context().find(*m, d_image, size_t(width), size_t(height), size_t(pitch), q, stats ? &trace : nullptr);
+    });
+    if (results) put_poses(found, results, cap);
+    *count = found.size();
+    fill_stats(trace, stats);
+  });
+}
+
+extern "C" MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* image, int64_t width, int64_t height,
+                                               int64_t pitch, const MvAmdFindOptions* options, double* results,
+                                               size_t cap, size_t* count, double* coarse, size_t cap_coarse,
+                                               size_t* n_coarse, double* prepared, double* refined, size_t cap_prepared,
+                                               size_t* n_prepared, MvAmdFindStats* stats) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    required(image, op, "Image");
+    required(count, op, "count");
+    if (width <= 0 || height <= 0 || pitch < width)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
+    const auto m = model_of(ModelID, op);
+    const mvamd::FindOptions q = from_ext(options);
+    mvamd::FindTrace trace;
+    const auto found = algorithm(op, [&] {
+      return context().find_host(*m, image, size_t(width), size_t(height), size_t(pitch), q, &trace);
+    });
+    if (results) put_poses(found, results, cap);
+    *count = found.size();
+    if (coarse) put_poses(trace.coarse, coarse, cap_coarse);
+    if (n_coarse) *n_coarse = trace.coarse.size();
+    if (prepared) put_poses(trace.prepared, prepared, cap_prepared);
+    if (refined)
+      for (size_t i = 0; i < trace.refined.size() && i < cap_prepared; ++i) {
+        refined[5 * i] = trace.refined[i].row; refined[5 * i + 1] = trace.refined[i].col;
+        refined[5 * i + 2] = trace.refined[i].angle; refined[5 * i + 3] = trace.refined[i].score;
+        refined[5 * i + 4] = trace.verdict[i];
+      }
+    if (n_prepared) *n_prepared = trace.prepared.size();
+    fill_stats(trace, stats);
+  });
+}
+
+extern "C" MvStatus mvamd_model_info(MvTuple ModelID, double* info) {
+  return translate([&] {
+    required(info, "model_info", "info");
+    const auto m = model_of(ModelID, "model_info");
+    info[0] = double(m->levels.size()); info[1] = double(m->n_angles); info[2] = m->params.step;
+    info[3] = m->orow; info[4] = m->ocol; info[5] = double(m->sample_count);
+  });
+}
+
+extern "C" MvStatus mvamd_model_raster(MvTuple ModelID, int32_t level, int32_t angle, double* header, int32_t* rows,
+                                       int32_t* cols, double* weights, size_t cap) {
+  return translate([&] {
+    constexpr auto op = "model_raster";
+    required(header, op, "header");
+    const auto m = model_of(ModelID, op);
+    uint32_t id = m->original;
+    if (level >= 0) {
+      if (size_t(level) >= m->levels.size() || angle < 0 || size_t(angle) >= m->n_angles)
+        fail(mvamd::kOutOfRange, mvamd::kErrIndexOutOfRange, op, "level", "raster index out of range");
+      id = m->raster(size_t(level), size_t(angle));
+    }
+    const mvamd::Raster& r = m->pool.rasters[id];
+    const double h[11] = {double(r.count), r.angle, r.orow, r.ocol, double(r.rmin), double(r.rmax), double(r.cmin),
+                          double(r.cmax), r.wsum, double(r.run_count), double(r.wrun_count)};
+    std::copy(h, h + 11, header);
+    for (size_t i = 0; i < r.count && i < cap; ++i) {
+      const int32_t rc = m->pool.rc[r.first + i];
+      if (rows) rows[i] = rc >> 16;
+      if (cols) cols[i] = int32_t(int16_t(rc & 0xffff));
+      if (weights) weights[i] = m->pool.w[r.first + i];
+    }
+  });
+}

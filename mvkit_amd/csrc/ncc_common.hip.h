@@ -1,0 +1,78 @@
+// Device helpers shared by the NCC kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "device_types.h"
+
+#define NCC_MISSING (-__builtin_inf())
+
+namespace ncc {
+
+constexpr int kWave = 64;
+
+// Integer pixel value (u8 at level 0, block sum S_l above) at (r, c).
+__device__ __forceinline__ uint32_t load_s(const NccLevel& L, int r, int c) {
+  const size_t i = size_t(r) * size_t(L.pitch) + size_t(c);
+  if (L.kind == NCC_U8) return static_cast<const uint8_t*>(L.data)[i];
+  if (L.kind == NCC_U16) return static_cast<const uint16_t*>(L.data)[i];
+  return static_cast<const uint32_t*>(L.data)[i];
+}
+
+// The reference's FP64 level value S * 4^-l - dc (exact in FP64).
+__device__ __forceinline__ double load_q(const NccLevel& L, int r, int c, double dc) {
+  return ldexp(double(load_s(L, r, c)), -L.shift) - dc;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = kWave / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int m = kWave / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+
+// Block-wide sum of N doubles (blockDim.x == 256); every thread gets the
+// totals.  `scratch` holds at least 4*N doubles of LDS.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* scratch) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) scratch[wave * N + i] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = (scratch[i] + scratch[N + i]) + (scratch[2 * N + i] + scratch[3 * N + i]);
+  __syncthreads();
+}
+
+__device__ __forceinline__ int block_any(int flag, int* scratch) {
+  if (threadIdx.x == 0) *scratch = 0;
+  __syncthreads();
+  if (flag) atomicOr(scratch, 1);
+  __syncthreads();
+  const int r = *scratch;
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double clamp1(double x) { return x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x); }
+
+__device__ __forceinline__ int sample_row(int32_t rc) { return rc >> 16; }
+__device__ __forceinline__ int sample_col(int32_t rc) { return int(int16_t(rc & 0xffff)); }
+
+// Exact window moments from integer sums of S over n samples: returns the
+// reference's sum(q) and sum(q^2) with q = S*4^-l - dc, both exact in FP64.
+__device__ __forceinline__ void exact_moments(int64_t sum_s, int64_t sum_s2, int n, int64_t D, int shift,
+                                              double& sum_q, double& sq_q) {
+  const int64_t t1 = sum_s - int64_t(n) * D;
+  const int64_t t2 = sum_s2 - 2 * D * sum_s + int64_t(n) * D * D;
+  sum_q = ldexp(double(t1), -shift);
+  sq_q = ldexp(double(t2), -2 * shift);
+}
+
+}  // namespace ncc

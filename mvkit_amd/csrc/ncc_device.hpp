@@ -1,0 +1,114 @@
+// Device side of the NCC find path: device copies of models and the per-caller
+// find pipeline (search pyramid, coarse correlation, peaks, tracking,
+// speculative refinement) driving the HIP kernels through the ncchip C ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/ncchip.h"
+#include "ncc_model.hpp"
+
+namespace mvamd {
+
+// RAII device allocation.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(size_t n);
+  ~DeviceBuffer();
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Pinned host staging buffer.
+struct PinnedBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  PinnedBuffer() = default;
+  ~PinnedBuffer();
+  void reserve(size_t n);
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+};
+
+void check(hipError_t e, const char* what);
+
+// A raster pool resident on one device.
+struct DevicePool {
+  DeviceBuffer storage;
+  NccPatchPool view{};            // device pointers into `storage`
+  size_t n_rasters = 0;
+  // Uploads a host pool (synchronously on `stream`).
+  void upload(const RasterPool& pool, hipStream_t stream);
+};
+
+class DeviceModel {
+ public:
+  int device = 0;
+  DevicePool pool;
+};
+
+// Returns the model's copy on the current device, uploading it on first use.
+std::shared_ptr<DeviceModel> device_model(const NccModel& m, hipStream_t stream);
+
+struct FindOptions {
+  double start = 0, extent = 0, min_score = .8, max_overlap = .5;
+  size_t num_matches = 1, max_candidates = 10000;
+  bool subpixel = true;
+  int highest = 0, lowest = 1;
+  int form = NCCHIP_FORM_FP32;     // coarse correlation form
+  double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN)
+};
+
+// Intermediate results of one find, for stage-by-stage checks.
+struct FindTrace {
+  std::vector<Pose> coarse;                 // peaks, sorted in the reference's emission order
+  std::vector<std::vector<Pose>> tracked;   // after each tracking level (merged, filtered)
+  std::vector<Pose> prepared;               // full-scored, sorted
+  std::vector<Pose> refined;                // per prepared candidate
+  std::vector<int> verdict;                 // per prepared candidate
+  int top = 0, finest = 0, angle_stride = 1;
+  int storage[4] = {0, 0, 0, 0};
+  double t_coarse_us = 0, t_track_us = 0, t_refine_us = 0, t_total_us = 0;
+};
+
+// One find pipeline: owns a stream and grow-only device/pinned workspaces.
+// Not thread-safe; use one context per host thread.
+class FindContext {
+ public:
+  FindContext();
+  ~FindContext();
+  FindContext(const FindContext&) = delete;
+  FindContext& operator=(const FindContext&) = delete;
+
+  // `image` is a device pointer to a u8 image with `pitch` bytes per row.
+  std::vector<Pose> find(const NccModel& model, const uint8_t* image, size_t width, size_t height, size_t pitch,
+                         const FindOptions& options, FindTrace* trace = nullptr);
+  // Same, from host memory (copied to the device first).
+  std::vector<Pose> find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
+                              size_t pitch, const FindOptions& options, FindTrace* trace = nullptr);
+
+  hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
+
+ private:
+  struct Arena;
+  hipStream_t stream_ = nullptr;
+  int device_ = 0;
+  DeviceBuffer work_;
+  DeviceBuffer image_;
+  PinnedBuffer stage_;
+  PinnedBuffer readback_;
+};
+
+// Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).
+double pose_overlap(const Pose& a, const Pose& b, const LevelGeom& level);
+
+}  // namespace mvamd

@@ -1,0 +1,135 @@
+// Integer passes of the NCC find path: search pyramid (exact block sums),
+// top-level row prefix sums of S and S^2, and the 2D support integral.
+// All results are bit-exact images of the reference's FP64 values
+// (matcher.cpp:401-455, 487-495, 498-503).
+#include "ncc_common.hip.h"
+#include "../../include/ncchip.h"
+
+namespace ncc {
+
+template <class Tin>
+__device__ __forceinline__ uint32_t ld(const Tin* p, size_t i) { return uint32_t(p[i]); }
+
+// One output level from the previous one: S_{l+1}(r,c) = sum of the 2x2 block
+// at (2r, 2c) with the reference's edge clamp (matcher.cpp:442, 448-450).
+// Each thread writes 4 consecutive outputs of one row.
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(256) k_pyramid(const Tin* __restrict__ src, int sp, int sw, int sh,
+                                                 Tout* __restrict__ dst, int dp, int dw, int dh) {
+  const int r = blockIdx.y;
+  const int c0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (r >= dh || c0 >= dw) return;
+  const int r0 = 2 * r, r1 = min(r0 + 1, sh - 1);
+  const Tin* a = src + size_t(r0) * sp;
+  const Tin* b = src + size_t(r1) * sp;
+  Tout out[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int c = c0 + j;
+    if (c < dw) {
+      const int x0 = 2 * c, x1 = min(x0 + 1, sw - 1);
+      out[j] = Tout(ld(a, x0) + ld(a, x1) + ld(b, x0) + ld(b, x1));
+    }
+  }
+  Tout* o = dst + size_t(r) * dp + c0;
+  if (c0 + 3 < dw) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = out[j];
+  } else {
+    for (int j = 0; j < 4 && c0 + j < dw; ++j) o[j] = out[j];
+  }
+}
+
+// Row prefix of S (u32) and S^2 (u64) over one level row, plus the row
+// prefix of the support indicator (S != D, i.e. q != 0), one block per row.
+// Layout: (w+1) entries per row with a leading zero (matcher.cpp:490-494).
+template <class T>
+__global__ void __launch_bounds__(256) k_row_prefix(const T* __restrict__ src, int sp, int w, int h,
+                                                    const uint8_t* __restrict__ dc_src, int shift,
+                                                    uint32_t* __restrict__ p1, uint64_t* __restrict__ p2,
+                                                    uint32_t* __restrict__ nz) {
+  const int r = blockIdx.x;
+  const uint32_t D = uint32_t(*dc_src) << shift;
+  if (r >= h) return;
+  __shared__ uint32_t s1[4];
+  __shared__ uint64_t s2[4];
+  __shared__ uint32_t s3[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const T* row = src + size_t(r) * sp;
+  uint32_t* o1 = p1 + size_t(r) * (w + 1);
+  uint64_t* o2 = p2 + size_t(r) * (w + 1);
+  uint32_t* o3 = nz + size_t(r) * (w + 1);
+  if (threadIdx.x == 0) { o1[0] = 0; o2[0] = 0; o3[0] = 0; }
+  uint32_t carry1 = 0, carry3 = 0;
+  uint64_t carry2 = 0;
+  for (int base = 0; base < w; base += 256) {
+    const int c = base + threadIdx.x;
+    const uint32_t s = c < w ? uint32_t(row[c]) : 0u;
+    uint32_t v1 = s, v3 = (c < w && s != D) ? 1u : 0u;
+    uint64_t v2 = uint64_t(s) * s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t1 = __shfl_up(v1, d, 64);
+      const uint64_t t2 = __shfl_up(v2, d, 64);
+      const uint32_t t3 = __shfl_up(v3, d, 64);
+      if (lane >= d) { v1 += t1; v2 += t2; v3 += t3; }
+    }
+    if (lane == 63) { s1[wave] = v1; s2[wave] = v2; s3[wave] = v3; }
+    __syncthreads();
+    uint32_t add1 = carry1, add3 = carry3;
+    uint64_t add2 = carry2;
+    for (int k = 0; k < wave; ++k) { add1 += s1[k]; add2 += s2[k]; add3 += s3[k]; }
+    if (c < w) { o1[c + 1] = v1 + add1; o2[c + 1] = v2 + add2; o3[c + 1] = v3 + add3; }
+    carry1 += s1[0] + s1[1] + s1[2] + s1[3];
+    carry2 += s2[0] + s2[1] + s2[2] + s2[3];
+    carry3 += s3[0] + s3[1] + s3[2] + s3[3];
+    __syncthreads();
+  }
+}
+
+// Column pass turning row prefixes of the support indicator into the 2D
+// integral support[(r+1)*(w+1) + c] (first row zero), matcher.cpp:498-499.
+__global__ void __launch_bounds__(256) k_support_cols(const uint32_t* __restrict__ nz, int w, int h,
+                                                      uint32_t* __restrict__ sup) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c > w) return;
+  uint32_t acc = 0;
+  sup[c] = 0;
+  for (int r = 0; r < h; ++r) {
+    acc += nz[size_t(r) * (w + 1) + c];
+    sup[size_t(r + 1) * (w + 1) + c] = acc;
+  }
+}
+
+}  // namespace ncc
+
+using namespace ncc;
+
+static inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+extern "C" int ncchip_pyramid_level(const void* src, int src_kind, int src_pitch, int src_w, int src_h, void* dst,
+                                    int dst_kind, int dst_pitch, int dst_w, int dst_h, void* stream) {
+  const dim3 grid((dst_w + 1023) / 1024, dst_h);
+  hipStream_t st = as_stream(stream);
+#define NCC_PYR(TI, TO) \
+  hipLaunchKernelGGL((k_pyramid<TI, TO>), grid, dim3(256), 0, st, static_cast<const TI*>(src), src_pitch, src_w, \
+                     src_h, static_cast<TO*>(dst), dst_pitch, dst_w, dst_h)
+  if (src_kind == NCC_U8 && dst_kind == NCC_U16) NCC_PYR(uint8_t, uint16_t);
+  else if (src_kind == NCC_U16 && dst_kind == NCC_U16) NCC_PYR(uint16_t, uint16_t);
+  else if (src_kind == NCC_U16 && dst_kind == NCC_U32) NCC_PYR(uint16_t, uint32_t);
+  else if (src_kind == NCC_U32 && dst_kind == NCC_U32) NCC_PYR(uint32_t, uint32_t);
+  else if (src_kind == NCC_U8 && dst_kind == NCC_U32) NCC_PYR(uint8_t, uint32_t);
+  else return -1;
+#undef NCC_PYR
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src, int shift,
+                                 uint32_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream) {
+  hipStream_t st = as_stream(stream);
+  if (kind == NCC_U8) hipLaunchKernelGGL((k_row_prefix<uint8_t>), dim3(h), dim3(256), 0, st, static_cast<const uint8_t*>(src), pitch, w, h, dc_src, shift, p1, p2, nz);
+  else if (kind == NCC_U16) hipLaunchKernelGGL((k_row_prefix<uint16_t>), dim3(h), dim3(256), 0, st, static_cast<const uint16_t*>(src), pitch, w, h, dc_src, shift, p1, p2, nz);
+  else hipLaunchKernelGGL((k_row_prefix<uint32_t>), dim3(h), dim3(256), 0, st, static_cast<const uint32_t*>(src), pitch, w, h, dc_src, shift, p1, p2, nz);
+  if (support) hipLaunchKernelGGL(k_support_cols, dim3((w + 1 + 255) / 256), dim3(256), 0, st, nz, w, h, support);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}

@@ -1,0 +1,321 @@
+// Host construction of NCC models.  Compiled with -ffp-contract=off: every
+// FP64 expression below is evaluated in the reference's order, so rotated
+// template weights match mvkit bit for bit (model.cpp:39-242).
+#include "ncc_model.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace mvamd {
+
+namespace {
+constexpr double kTau = 6.283185307179586476925286766559;
+constexpr size_t kMaxPixels = 1048576;   // model.cpp:15
+constexpr size_t kMaxAngles = 4096;
+constexpr size_t kMaxSamples = 16000000;
+const char* const kCreateOp = "ncc_model_create";
+
+[[noreturn]] void raise(Status s, const char* param, const char* msg) { throw Error(s, kCreateOp, param, msg); }
+
+int32_t pack_rc(int32_t r, int32_t c) { return int32_t((uint32_t(r) << 16) | (uint32_t(c) & 0xffffu)); }
+}  // namespace
+
+ErrorCode default_code(Status s) {
+  switch (s) {
+    case kInvalidArgument: return kErrInvalidValue;
+    case kOutOfRange: return kErrIndexOutOfRange;
+    case kTypeMismatch: return kErrInvalidType;
+    case kNotFound: return kErrMissingItem;
+    case kUnsupported: return kErrUnsupportedValue;
+    case kInvalidState: return kErrInvalidObjectState;
+    case kIoError: case kDecodeError: case kEncodeError: return kErrIoFailure;
+    case kInternalError: case kOutOfMemory: return kErrInternalFailure;
+    case kLimitExceeded: return kErrInvalidValue;
+    default: return kErrUnspecified;
+  }
+}
+
+size_t RasterPool::bytes() const {
+  return rasters.size() * sizeof(Raster) + rc.size() * 4 + w.size() * 8 + wf.size() * 4 + runs.size() * 4 +
+         wruns.size() * 4 + wrun_w.size() * 8;
+}
+
+// Samples are appended in raster order as (row, col, value); `finish` centres
+// and scales them and derives bounds and runs (normalize, model.cpp:39-65).
+namespace {
+struct Staging {
+  std::vector<int32_t> r, c;
+  std::vector<double> v;
+};
+
+bool finish(const Staging& s, double angle, double orow, double ocol, RasterPool& pool, const double* pmean,
+            const double* penergy, Raster& out) {
+  const size_t n = s.v.size();
+  if (n < 4) return false;
+  std::vector<double> w(s.v);
+  double mean = 0, energy = 0;
+  if (pmean) {
+    mean = *pmean;
+    energy = *penergy;
+    for (auto& x : w) x -= mean;
+  } else {
+    for (size_t i = 0; i < n; ++i) mean += (w[i] - mean) / double(i + 1);
+    for (auto& x : w) { x -= mean; energy += x * x; }
+  }
+  if (!(energy > 0) || !std::isfinite(energy)) return false;
+  const double scale = 1 / std::sqrt(energy);
+  out = Raster{};
+  out.angle = angle; out.orow = orow; out.ocol = ocol;
+  out.first = uint32_t(pool.w.size());
+  out.count = uint32_t(n);
+  out.run_first = uint32_t(pool.runs.size() / 4);
+  out.wrun_first = uint32_t(pool.wrun_w.size());
+  out.rmin = out.rmax = s.r[0];
+  out.cmin = out.cmax = s.c[0];
+  int32_t run_row = 0, run_end = 0, wrun_row = 0, wrun_end = 0;
+  double wrun_value = 0;
+  bool have_run = false, have_wrun = false;
+  for (size_t i = 0; i < n; ++i) {
+    const double wi = w[i] * scale;
+    const int32_t r = s.r[i], c = s.c[i];
+    pool.rc.push_back(pack_rc(r, c));
+    pool.w.push_back(wi);
+    pool.wf.push_back(float(wi));
+    out.wsum += wi;
+    out.wsum_f += double(float(wi));
+    out.rmin = std::min(out.rmin, r); out.rmax = std::max(out.rmax, r);
+    out.cmin = std::min(out.cmin, c); out.cmax = std::max(out.cmax, c);
+    if (have_run && run_row == r && run_end + 1 == c) {
+      run_end = c;
+      pool.runs[pool.runs.size() - 2] = c;
+    } else {
+      pool.runs.insert(pool.runs.end(), {r, c, c, 0});
+      run_row = r; run_end = c; have_run = true;
+    }
+    if (have_wrun && wrun_row == r && wrun_end + 1 == c && wrun_value == wi) {
+      wrun_end = c;
+      pool.wruns[pool.wruns.size() - 2] = c;
+    } else {
+      pool.wruns.insert(pool.wruns.end(), {r, c, c, 0});
+      pool.wrun_w.push_back(wi);
+      wrun_row = r; wrun_end = c; wrun_value = wi; have_wrun = true;
+    }
+  }
+  out.run_count = uint32_t(pool.runs.size() / 4) - out.run_first;
+  out.wrun_count = uint32_t(pool.wrun_w.size()) - out.wrun_first;
+  return true;
+}
+}  // namespace
+
+int64_t build_raster(const LevelGeom& lv, double angle, RasterPool& pool, const double* pmean, const double* penergy) {
+  Staging s;
+  double orow = 0, ocol = 0;
+  if (std::abs(std::remainder(angle, kTau)) < 1e-12) {
+    // unrotated raster on the level grid, origin at the level centroid
+    orow = lv.orow; ocol = lv.ocol;
+    for (size_t r = 0; r < lv.h; ++r)
+      for (size_t c = 0; c < lv.w; ++c)
+        if (lv.mask[r * lv.w + c]) { s.r.push_back(int32_t(r)); s.c.push_back(int32_t(c)); s.v.push_back(lv.gray[r * lv.w + c]); }
+  } else {
+    // inverse bilinear rotation about the centroid; raster coordinates are
+    // offsets from the centroid (model.cpp:77-96)
+    const double ca = std::cos(angle), sa = std::sin(angle);
+    double lo_r = 0, hi_r = 0, lo_c = 0, hi_c = 0;
+    const double corner_r[2] = {0., double(lv.h - 1)}, corner_c[2] = {0., double(lv.w - 1)};
+    for (double cr : corner_r)
+      for (double cc : corner_c) {
+        const double dr = cr - lv.orow, dc = cc - lv.ocol;
+        const double y = ca * dr - sa * dc, x = sa * dr + ca * dc;
+        lo_r = std::min(lo_r, y); hi_r = std::max(hi_r, y);
+        lo_c = std::min(lo_c, x); hi_c = std::max(hi_c, x);
+      }
+    const double rows = std::ceil(hi_r) - std::floor(lo_r) + 1, cols = std::ceil(hi_c) - std::floor(lo_c) + 1;
+    if (rows > 4 * kMaxPixels / cols) raise(kLimitExceeded, "domain", "rotated bounding box exceeds budget");
+    const int r_lo = int(std::floor(lo_r)), r_hi = int(std::ceil(hi_r));
+    const int c_lo = int(std::floor(lo_c)), c_hi = int(std::ceil(hi_c));
+    const double ymax = double(lv.h - 1), xmax = double(lv.w - 1);
+    for (int r = r_lo; r <= r_hi; ++r)
+      for (int c = c_lo; c <= c_hi; ++c) {
+        const double ty = ca * r + sa * c + lv.orow, tx = -sa * r + ca * c + lv.ocol;
+        if (ty < 0 || tx < 0 || ty > ymax || tx > xmax) continue;
+        const size_t y0 = size_t(ty), x0 = size_t(tx);
+        const size_t y1 = std::min(y0 + 1, lv.h - 1), x1 = std::min(x0 + 1, lv.w - 1);
+        const size_t i00 = y0 * lv.w + x0, i01 = y0 * lv.w + x1, i10 = y1 * lv.w + x0, i11 = y1 * lv.w + x1;
+        if (!lv.mask[i00] || !lv.mask[i01] || !lv.mask[i10] || !lv.mask[i11]) continue;
+        const double fy = ty - y0, fx = tx - x0;
+        const double top = (1 - fx) * lv.gray[i00] + fx * lv.gray[i01];
+        const double bottom = (1 - fx) * lv.gray[i10] + fx * lv.gray[i11];
+        s.r.push_back(r); s.c.push_back(c); s.v.push_back((1 - fy) * top + fy * bottom);
+      }
+  }
+  // the shared first-level mean/energy only replaces the reduction of the
+  // unrotated raster (model.cpp:154-158); rotated rasters normalise themselves
+  const bool unrotated = std::abs(std::remainder(angle, kTau)) < 1e-12;
+  Raster out;
+  if (!finish(s, angle, orow, ocol, pool, unrotated ? pmean : nullptr, unrotated ? penergy : nullptr, out)) return -1;
+  pool.rasters.push_back(out);
+  return int64_t(pool.rasters.size() - 1);
+}
+
+namespace {
+// validate_zero (model.cpp:22-36): sample count and finite positive energy.
+bool usable(const LevelGeom& lv, size_t& count) {
+  double mean = 0;
+  size_t n = 0;
+  for (size_t i = 0; i < lv.gray.size(); ++i)
+    if (lv.mask[i]) mean += (lv.gray[i] - mean) / double(++n);
+  count = n;
+  if (n < 4) return false;
+  double energy = 0;
+  for (size_t i = 0; i < lv.gray.size(); ++i)
+    if (lv.mask[i]) { const double v = lv.gray[i] - mean; energy += v * v; }
+  return energy > 0 && std::isfinite(energy);
+}
+
+// 2x2 mean level (model.cpp:206-220 with gray_pyramid.hpp:8-13).
+LevelGeom halve(const LevelGeom& pv) {
+  LevelGeom nx;
+  nx.w = pv.w / 2; nx.h = pv.h / 2;
+  nx.orow = (pv.orow - .5) * .5; nx.ocol = (pv.ocol - .5) * .5;
+  nx.gray.resize(nx.w * nx.h);
+  nx.mask.resize(nx.w * nx.h);
+  for (size_t r = 0; r < nx.h; ++r)
+    for (size_t c = 0; c < nx.w; ++c) {
+      const size_t top = 2 * r * pv.w, bottom = (2 * r + 1) * pv.w;
+      const size_t c0 = 2 * c, c1 = c0 + 1 < pv.w ? c0 + 1 : c0;
+      nx.mask[r * nx.w + c] = pv.mask[top + c0] && pv.mask[top + c0 + 1] && pv.mask[bottom + c0] && pv.mask[bottom + c0 + 1];
+      nx.gray[r * nx.w + c] = .25 * (pv.gray[top + c0] + pv.gray[top + c1] + pv.gray[bottom + c0] + pv.gray[bottom + c1]);
+    }
+  return nx;
+}
+}  // namespace
+
+NccModel::~NccModel() = default;
+
+std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size_t width, size_t height,
+                                       size_t stride_bytes, const CreateParams& p) {
+  if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
+  if (!std::isfinite(p.start) || !std::isfinite(p.extent) || !std::isfinite(p.step) || p.extent < 0 || p.extent > kTau ||
+      !std::isfinite(p.start + p.extent) || p.step < 0 || p.levels < 0 || p.levels > 16)
+    raise(kInvalidArgument, "options", "invalid levels or angle range");
+  if (width * height < 4) raise(kInvalidArgument, "domain", "template domain requires at least four pixels");
+  if (height > kMaxPixels / width)
+    raise(kLimitExceeded, "domain", "template bounding box exceeds one million pixels");
+
+  auto m = std::make_unique<NccModel>();
+  m->params = p;
+  LevelGeom first;
+  first.w = width; first.h = height;
+  first.gray.resize(width * height);
+  first.mask.assign(width * height, 1);
+  // centroid of the (rectangular) domain, accumulated per row run in long
+  // double as model.cpp:142-146, 158-161
+  long double sum_row = 0, sum_col = 0;
+  int64_t total = 0;
+  const auto* base = static_cast<const uint8_t*>(pixels);
+  for (size_t r = 0; r < height; ++r) {
+    const int64_t count = int64_t(width);
+    sum_row += static_cast<long double>(r) * count;
+    sum_col += (static_cast<long double>(0) + int64_t(width - 1)) * count / 2;
+    const uint8_t* row = base + r * stride_bytes;
+    for (size_t c = 0; c < width; ++c) {
+      uint32_t v;
+      if (pixel_bytes == 1) v = row[c];
+      else { uint16_t t; std::memcpy(&t, row + 2 * c, 2); v = t; }
+      first.gray[r * width + c] = double(v);
+      total += v;
+    }
+  }
+  const long double area = static_cast<long double>(width * height);
+  m->orow = double(sum_row / area);
+  m->ocol = double(sum_col / area);
+  first.orow = double((sum_row - static_cast<long double>(0) * area) / area);
+  first.ocol = double((sum_col - static_cast<long double>(0) * area) / area);
+  const double mean = double(total) / double(width * height);
+  for (auto& g : first.gray) g -= mean;
+
+  // rebuild(): original patch (division by the norm), pyramid, angle grid
+  m->sample_count = width * height;
+  m->levels.push_back(std::move(first));
+  // original patch: centred by the incremental mean, divided by the norm
+  // (model.cpp:182-204); appended after the angle grid
+  double first_mean = 0, first_energy = 0;
+  std::vector<int32_t> orig_rc;
+  std::vector<double> orig_w;
+  {
+    const LevelGeom& lv = m->levels.front();
+    size_t cnt = 0;
+    for (size_t i = 0; i < lv.gray.size(); ++i)
+      if (lv.mask[i]) first_mean += (lv.gray[i] - first_mean) / double(++cnt);
+    for (size_t r = 0; r < lv.h; ++r)
+      for (size_t c = 0; c < lv.w; ++c)
+        if (lv.mask[r * lv.w + c]) {
+          const double v = lv.gray[r * lv.w + c] - first_mean;
+          first_energy += v * v;
+          orig_w.push_back(v);
+          orig_rc.push_back(pack_rc(int32_t(r), int32_t(c)));
+        }
+    if (cnt < 4 || !(first_energy > 0) || !std::isfinite(first_energy))
+      raise(kInvalidArgument, "image", "constant template cannot define an NCC model");
+    const double norm = std::sqrt(first_energy);
+    for (auto& v : orig_w) v /= norm;
+  }
+  // pyramid of the template (model.cpp:205-221)
+  const size_t wanted = p.levels ? size_t(p.levels) : 16;
+  while (m->levels.size() < wanted) {
+    const LevelGeom& pv = m->levels.back();
+    if (pv.w < 8 || pv.h < 8) break;
+    LevelGeom nx = halve(pv);
+    size_t count = 0;
+    const bool ok = usable(nx, count);
+    if (!ok || count < 8 || (m->levels.size() >= 2 && count < 64)) break;
+    m->levels.push_back(std::move(nx));
+  }
+  if (p.levels && m->levels.size() != size_t(p.levels))
+    raise(kInvalidArgument, "levels", "requested pyramid level is degenerate or too small");
+  // angle grid (model.cpp:222-238)
+  const double radius = std::hypot(double(m->levels.front().w), double(m->levels.front().h)) * .5;
+  m->params.step = p.step > 0 ? p.step : std::max(.01, std::min(.15, 1 / std::max(1., radius)));
+  const double intervals = std::ceil(p.extent / m->params.step);
+  if (!std::isfinite(intervals) || intervals >= double(kMaxAngles))
+    raise(kLimitExceeded, "angle_step", "angle grid exceeds limit");
+  m->n_angles = size_t(intervals) + 1;
+  size_t total_samples = 0;
+  for (size_t li = 0; li < m->levels.size(); ++li) {
+    LevelGeom& lv = m->levels[li];
+    lv.first_raster = uint32_t(m->pool.rasters.size());
+    for (size_t i = 0; i < m->n_angles; ++i) {
+      const double angle = p.start + std::min(p.extent, i * m->params.step);
+      const int64_t id = li == 0 ? build_raster(lv, angle, m->pool, &first_mean, &first_energy)
+                                 : build_raster(lv, angle, m->pool);
+      if (id < 0) raise(kInvalidArgument, "domain", "rotation produces a degenerate template");
+      const size_t n = m->pool.rasters[size_t(id)].count;
+      if (n > kMaxSamples - total_samples) raise(kLimitExceeded, "angle_extent", "model raster budget exceeded");
+      total_samples += n;
+    }
+  }
+  Raster original;
+  const LevelGeom& l0 = m->levels.front();
+  original.orow = l0.orow;
+  original.ocol = l0.ocol;
+  original.rmin = 0; original.rmax = int32_t(l0.h - 1);
+  original.cmin = 0; original.cmax = int32_t(l0.w - 1);
+  original.first = uint32_t(m->pool.w.size());
+  original.count = uint32_t(orig_w.size());
+  original.run_first = uint32_t(m->pool.runs.size() / 4);
+  original.wrun_first = uint32_t(m->pool.wrun_w.size());
+  for (size_t i = 0; i < orig_w.size(); ++i) {
+    m->pool.rc.push_back(orig_rc[i]);
+    m->pool.w.push_back(orig_w[i]);
+    m->pool.wf.push_back(float(orig_w[i]));
+  }
+  // like the reference, the original patch carries no weight sum: the
+  // continuous score centres the image samples instead (matcher.cpp:146-156)
+  m->pool.rasters.push_back(original);
+  m->original = uint32_t(m->pool.rasters.size() - 1);
+  return m;
+}
+
+}  // namespace mvamd

@@ -1,0 +1,114 @@
+// NCC model: host FP64 construction of the rotated, zero-mean, unit-energy
+// template rasters of every pyramid level, stored as flat structure-of-arrays
+// pools that upload to the device as-is.
+//
+// Semantics follow mvkit's create/rebuild (src/ncc_matching/model.cpp:120-242)
+// operation for operation, so every weight is bit-identical to the reference.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device_types.h"
+
+namespace mvamd {
+
+// Status codes of the public C ABI (mv_status.h of the reference).
+enum Status : int32_t {
+  kOk = 0, kInvalidArgument = -1, kOutOfRange = -2, kTypeMismatch = -3, kNotFound = -4, kOutOfMemory = -5,
+  kUnsupported = -6, kInvalidState = -7, kIoError = -100, kDecodeError = -101, kEncodeError = -102,
+  kLimitExceeded = -103, kInternalError = -1000
+};
+// Error codes of the public C ABI (mv_api.h of the reference).
+enum ErrorCode : int32_t {
+  kErrUnspecified = 0, kErrNullArgument = 1, kErrInvalidCardinality = 2, kErrInvalidType = 3, kErrInvalidValue = 4,
+  kErrIndexOutOfRange = 5, kErrMissingItem = 6, kErrUnsupportedValue = 7, kErrInvalidObjectState = 8,
+  kErrIoFailure = 9, kErrInternalFailure = 10
+};
+
+ErrorCode default_code(Status s);
+
+struct Error {
+  Status status;
+  ErrorCode code;
+  std::string operation, parameter, message;
+  int64_t index = -1;
+  Error(Status s, std::string op, std::string param, std::string msg, int64_t idx = -1, ErrorCode c = kErrUnspecified)
+      : status(s), code(c == kErrUnspecified ? default_code(s) : c), operation(std::move(op)),
+        parameter(std::move(param)), message(std::move(msg)), index(idx) {}
+};
+
+struct Pose { double row = 0, col = 0, angle = 0, score = 0; };
+
+// One template raster (a rotated patch of one level), indices into a pool.
+struct Raster {
+  double angle = 0, orow = 0, ocol = 0;
+  double wsum = 0;    // sum of the FP64 weights in sample order
+  double wsum_f = 0;  // sum of the FP32-rounded weights (exact in FP64)
+  int32_t rmin = 0, rmax = 0, cmin = 0, cmax = 0;
+  uint32_t first = 0, count = 0;          // samples
+  uint32_t run_first = 0, run_count = 0;  // row runs of consecutive samples
+  uint32_t wrun_first = 0, wrun_count = 0;// runs of equal consecutive weights
+};
+
+// Flat sample storage shared by many rasters.
+struct RasterPool {
+  std::vector<Raster> rasters;
+  std::vector<int32_t> rc;      // (row << 16) | (col & 0xffff)
+  std::vector<double> w;
+  std::vector<float> wf;
+  std::vector<int32_t> runs;    // 4 ints per run: row, begin, end, 0
+  std::vector<int32_t> wruns;   // 4 ints per weight run
+  std::vector<double> wrun_w;
+  size_t bytes() const;
+};
+
+// Dense geometry of one model level (model.hpp:44-53).
+struct LevelGeom {
+  size_t w = 0, h = 0;
+  double orow = 0, ocol = 0;
+  std::vector<double> gray;     // mean-free template values
+  std::vector<uint8_t> mask;
+  uint32_t first_raster = 0;    // index of angle 0 in the pool
+};
+
+struct CreateParams {
+  int levels = 0;               // 0: automatic
+  bool absolute = false;        // "ignore_global_polarity"
+  double start = 0, extent = 0, step = 0;  // step 0: automatic
+};
+
+class DeviceModel;  // device mirror (ncc_device.cpp)
+
+struct NccModel {
+  CreateParams params;          // step holds the resolved angle step
+  size_t sample_count = 0;
+  double orow = 0, ocol = 0;    // centroid in template coordinates
+  std::vector<LevelGeom> levels;
+  size_t n_angles = 0;
+  RasterPool pool;              // level-major rasters, then the original patch
+  uint32_t original = 0;        // unrotated level-0 patch, divided by its norm
+
+  // Lazily created device copies, one per device.
+  mutable std::mutex device_mutex;
+  mutable std::vector<std::shared_ptr<DeviceModel>> device;
+
+  uint32_t raster(size_t level, size_t angle) const { return levels[level].first_raster + uint32_t(angle); }
+  ~NccModel();
+};
+
+// create(): model.cpp:120-171 + rebuild (model.cpp:173-242) for a full-domain
+// single-channel template of u8 or u16 samples.
+std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size_t width, size_t height,
+                                       size_t stride_bytes, const CreateParams& params);
+
+// Builds the raster of one level at `angle` into `pool` (rotate + normalize,
+// model.cpp:39-98).  Returns the raster index, or -1 when degenerate.
+int64_t build_raster(const LevelGeom& level, double angle, RasterPool& pool, const double* prepared_mean = nullptr,
+                     const double* prepared_energy = nullptr);
+
+}  // namespace mvamd

@@ -1,0 +1,220 @@
+"""Deterministic synthetic scenes for the NCC find workloads.
+
+Generator: SplitMix64 -> u64 -> double as (x >> 11) * 2**-53; normals by
+Box-Muller in FP64.  Buffers are row-major u8 with stride == width.
+
+* blurred-noise base: uniform [0,1) per pixel, separable Gaussian sigma=2
+  (13 taps, radius 6, normalised, reflect-101 border) in FP64, linearly
+  rescaled min->40 / max->215 and rounded half-to-even to u8;
+* template: the same base with its own seed, then 6 filled shapes (ellipse or
+  rectangle with p=1/2; centre uniform in the template; semi-axes uniform in
+  [w/10, w/3]; rotation uniform in [0, pi); integer level in [30, 220]);
+* pastes: backward bilinear warp of the template about its centroid
+  ((h-1)/2, (w-1)/2), so an instance pasted at centre (cy, cx) with angle t is
+  found by the reference's find at pose (cy, cx, t) in its internal
+  (centroid) convention.
+
+Each workload seed is 0x6D766B6974 + k (k = workload index).
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SEED_BASE = 0x6D766B6974
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class SplitMix64:
+    """Vectorised SplitMix64 stream."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + k * _GAMMA
+            self.state = self.state + np.uint64(n) * _GAMMA
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n: int) -> np.ndarray:
+        return (self.u64(n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+    def uniform1(self) -> float:
+        return float(self.uniform(1)[0])
+
+    def normal(self, n: int) -> np.ndarray:
+        m = (n + 1) // 2
+        u1 = self.uniform(m)
+        u2 = self.uniform(m)
+        r = np.sqrt(-2.0 * np.log1p(-u1))  # 1-u1 in (0,1]
+        z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return z[:n]
+
+
+def _gauss_kernel(sigma: float = 2.0, radius: int = 6) -> np.ndarray:
+    x = np.arange(-radius, radius + 1, dtype=np.float64)
+    k = np.exp(-0.5 * (x / sigma) ** 2)
+    return k / k.sum()
+
+
+def _blur_axis(a: np.ndarray, k: np.ndarray, axis: int) -> np.ndarray:
+    r = (len(k) - 1) // 2
+    pad = [(0, 0), (0, 0)]
+    pad[axis] = (r, r)
+    p = np.pad(a, pad, mode="reflect")  # numpy 'reflect' == reflect-101
+    out = np.zeros_like(a)
+    n = a.shape[axis]
+    for i, w in enumerate(k):
+        sl = [slice(None), slice(None)]
+        sl[axis] = slice(i, i + n)
+        out += w * p[tuple(sl)]
+    return out
+
+
+def blurred_noise(rng: SplitMix64, h: int, w: int) -> np.ndarray:
+    """Blurred-noise base image as u8 in [40, 215]."""
+    u = rng.uniform(h * w).reshape(h, w)
+    k = _gauss_kernel()
+    b = _blur_axis(_blur_axis(u, k, 1), k, 0)
+    lo, hi = b.min(), b.max()
+    s = 40.0 + (b - lo) * (175.0 / (hi - lo))
+    return np.rint(s).clip(0, 255).astype(np.uint8)
+
+
+def make_template(rng: SplitMix64, h: int, w: int, shapes: int = 6) -> np.ndarray:
+    t = blurred_noise(rng, h, w).astype(np.float64)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    for _ in range(shapes):
+        ellipse = rng.uniform1() < 0.5
+        cy = rng.uniform1() * (h - 1)
+        cx = rng.uniform1() * (w - 1)
+        ay = w / 10 + rng.uniform1() * (w / 3 - w / 10)
+        ax = w / 10 + rng.uniform1() * (w / 3 - w / 10)
+        th = rng.uniform1() * math.pi
+        level = 30 + int(rng.uniform1() * 191)  # integer uniform in [30, 220]
+        c, s = math.cos(th), math.sin(th)
+        u = c * (xx - cx) + s * (yy - cy)
+        v = -s * (xx - cx) + c * (yy - cy)
+        if ellipse:
+            inside = (u / ax) ** 2 + (v / ay) ** 2 <= 1.0
+        else:
+            inside = (np.abs(u) <= ax) & (np.abs(v) <= ay)
+        t[inside] = level
+    return t.astype(np.uint8)
+
+
+def paste(image: np.ndarray, tmpl: np.ndarray, cy: float, cx: float, angle: float) -> None:
+    """Backward bilinear warp of ``tmpl`` rotated by ``angle`` about its centroid
+    to centre (cy, cx); pixels whose template coordinate is outside are kept."""
+    th, tw = tmpl.shape
+    oy, ox = (th - 1) / 2.0, (tw - 1) / 2.0
+    rad = math.hypot(th, tw) / 2 + 2
+    r0, r1 = max(0, int(math.floor(cy - rad))), min(image.shape[0], int(math.ceil(cy + rad)) + 1)
+    c0, c1 = max(0, int(math.floor(cx - rad))), min(image.shape[1], int(math.ceil(cx + rad)) + 1)
+    yy, xx = np.mgrid[r0:r1, c0:c1].astype(np.float64)
+    ca, sa = math.cos(angle), math.sin(angle)
+    dy, dx = yy - cy, xx - cx
+    ty = ca * dy + sa * dx + oy
+    tx = -sa * dy + ca * dx + ox
+    ok = (ty >= 0) & (tx >= 0) & (ty <= th - 1) & (tx <= tw - 1)
+    y0 = np.floor(ty).astype(np.int64).clip(0, th - 1)
+    x0 = np.floor(tx).astype(np.int64).clip(0, tw - 1)
+    y1 = np.minimum(y0 + 1, th - 1)
+    x1 = np.minimum(x0 + 1, tw - 1)
+    fy, fx = ty - y0, tx - x0
+    t = tmpl.astype(np.float64)
+    v = (1 - fy) * ((1 - fx) * t[y0, x0] + fx * t[y0, x1]) + fy * ((1 - fx) * t[y1, x0] + fx * t[y1, x1])
+    region = image[r0:r1, c0:c1]
+    region[ok] = v[ok]
+
+
+@dataclass
+class Scene:
+    image: np.ndarray                  # u8 search image
+    template: np.ndarray               # u8 template
+    truth: list = field(default_factory=list)  # [(row, col, angle)] centroid convention
+    seed: int = 0
+
+    def sha256(self) -> dict:
+        return {"image": hashlib.sha256(self.image.tobytes()).hexdigest(),
+                "template": hashlib.sha256(self.template.tobytes()).hexdigest()}
+
+
+def _finish(img: np.ndarray, rng: SplitMix64, sigma: float, gain: bool) -> np.ndarray:
+    h, w = img.shape
+    if gain:
+        img = img * (0.8 + 0.4 * np.arange(w, dtype=np.float64) / (w - 1))[None, :]
+    img = img + sigma * rng.normal(h * w).reshape(h, w)
+    return np.rint(np.clip(img, 0, 255)).astype(np.uint8)
+
+
+def scene(width: int, height: int, tw: int, th: int, count: int, angle_lo: float, angle_hi: float,
+          noise: float, gain: bool, seed: int, integer_positions: bool = False,
+          min_separation: float | None = None) -> Scene:
+    """Generic scene: ``count`` rotated pastes of one template."""
+    rng = SplitMix64(seed)
+    base = blurred_noise(rng, height, width).astype(np.float64)
+    tmpl = make_template(SplitMix64(seed ^ 0x5EED5EED), th, tw)
+    diag = math.hypot(th, tw)
+    sep = min_separation if min_separation is not None else 1.5 * diag
+    margin = diag / 2 + 2
+    truth = []
+    tries = 0
+    while len(truth) < count and tries < 10000:
+        tries += 1
+        if integer_positions:
+            y0 = int(rng.uniform1() * (height - th + 1))
+            x0 = int(rng.uniform1() * (width - tw + 1))
+            cy, cx = y0 + (th - 1) / 2.0, x0 + (tw - 1) / 2.0
+            ang = 0.0
+        else:
+            cy = margin + rng.uniform1() * (height - 1 - 2 * margin)
+            cx = margin + rng.uniform1() * (width - 1 - 2 * margin)
+            ang = angle_lo + rng.uniform1() * (angle_hi - angle_lo)
+        if any(math.hypot(cy - r, cx - c) < sep for r, c, _ in truth):
+            continue
+        truth.append((cy, cx, ang))
+    for cy, cx, ang in truth:
+        if integer_positions:
+            y0, x0 = int(round(cy - (th - 1) / 2.0)), int(round(cx - (tw - 1) / 2.0))
+            base[y0:y0 + th, x0:x0 + tw] = tmpl
+        else:
+            paste(base, tmpl, cy, cx, ang)
+    img = _finish(base, rng, noise, gain)
+    return Scene(img, tmpl, truth, seed)
+
+
+# ----------------------------------------------------------------- workloads
+def workload_plain_640(seed_k: int = 0) -> Scene:
+    """640x480 image, 64x64 template pasted 3x at integer positions, N(0,5)."""
+    return scene(640, 480, 64, 64, 3, 0.0, 0.0, 5.0, False, SEED_BASE + seed_k, integer_positions=True)
+
+
+def workload_rot_2048(seed_k: int = 1) -> Scene:
+    """2048x2048 image, 128x128 template pasted 5x at subpixel centres with
+    rotation in [-30, 30] degrees, gain ramp 0.8-1.2 and N(0,8)."""
+    return scene(2048, 2048, 128, 128, 5, -math.pi / 6, math.pi / 6, 8.0, True, SEED_BASE + seed_k,
+                 min_separation=1.2 * math.hypot(128, 128))
+
+
+def workload_full_2048(seed_k: int = 2) -> Scene:
+    """2048x2048 image, 256x256 template pasted 5x with rotation in [0, 2pi), N(0,8)."""
+    return scene(2048, 2048, 256, 256, 5, 0.0, 2 * math.pi, 8.0, False, SEED_BASE + seed_k,
+                 min_separation=1.2 * math.hypot(256, 256))
+
+
+def small_rot(width=256, height=192, t=32, count=2, extent_deg=20.0, seed_k=10, noise=4.0) -> Scene:
+    """Small rotated scene for oracle-speed parity tests."""
+    e = math.radians(extent_deg)
+    return scene(width, height, t, t, count, -e, e, noise, True, SEED_BASE + seed_k,
+                 min_separation=1.3 * math.hypot(t, t))

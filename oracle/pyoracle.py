@@ -1,0 +1,176 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Imported by tests/, the smoke check and the benchmark's CPU-baseline leg --
+never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libncc_oracle.so")
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(os.path.join(HERE, "ncc_oracle.cpp")):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return LIB
+
+
+class FindOut(C.Structure):
+    _fields_ = [("n_result", C.c_int), ("n_coarse", C.c_int), ("n_prepared", C.c_int), ("top", C.c_int),
+                ("finest", C.c_int), ("angle_stride", C.c_int), ("coarse_windows", C.c_double),
+                ("coarse_samples", C.c_double), ("track_windows", C.c_double), ("track_samples", C.c_double),
+                ("crop_r", C.c_long), ("crop_c", C.c_long), ("store_r", C.c_long), ("store_c", C.c_long),
+                ("store_h", C.c_long), ("store_w", C.c_long)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        D = C.c_double
+        I = C.c_int
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_create.argtypes = [P, I, I, I, I, D, D, D, I, C.POINTER(P)]
+        L.orc_free.argtypes = [P]
+        L.orc_model_info.argtypes = [P, P]
+        L.orc_level_info.argtypes = [P, I, P]
+        L.orc_patch_info.argtypes = [P, I, I, P]
+        L.orc_patch_samples.argtypes = [P, I, I, P, P, P]
+        L.orc_level_gray.argtypes = [P, I, P]
+        L.orc_pyramid.argtypes = [P, I, I, I, I, P]
+        L.orc_row_prefix.argtypes = [P, I, I, P, P]
+        L.orc_find.argtypes = [P, P, I, I, I, D, D, D, I, D, I, I, I, P, I, P, I, P, I, P, C.POINTER(FindOut)]
+        L.orc_zncc.argtypes = [P, P, I, I]
+        L.orc_zncc.restype = D
+        L.orc_overlap.argtypes = [D, D, D, D, D, D, I, I, D, D]
+        L.orc_overlap.restype = D
+        L.orc_continuous.argtypes = [P, P, I, I, I, D, D, D]
+        L.orc_continuous.restype = D
+        L.orc_refine_one.argtypes = [P, P, I, I, I, D, D, D, I, D, I, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+class Model:
+    def __init__(self, template: np.ndarray, levels: int = 0, start: float = 0.0, extent: float = 0.0,
+                 step: float = 0.0, absolute: bool = False):
+        t = np.ascontiguousarray(template, dtype=np.uint8)
+        h = C.c_void_p()
+        st = lib().orc_create(_ptr(t), t.shape[1], t.shape[0], t.shape[1], levels, start, extent, step,
+                              int(absolute), C.byref(h))
+        if st:
+            raise OracleError(st, lib().orc_last_error().decode())
+        self.h = h
+        info = np.zeros(6)
+        lib().orc_model_info(self.h, _ptr(info))
+        self.n_levels, self.n_angles = int(info[0]), int(info[1])
+        self.step, self.origin_row, self.origin_col, self.sample_count = info[2], info[3], info[4], int(info[5])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    def level(self, l: int) -> dict:
+        out = np.zeros(4)
+        lib().orc_level_info(self.h, l, _ptr(out))
+        w, h = int(out[0]), int(out[1])
+        g = np.zeros(w * h)
+        lib().orc_level_gray(self.h, l, _ptr(g))
+        return {"w": w, "h": h, "orow": out[2], "ocol": out[3], "gray": g.reshape(h, w)}
+
+    def patch(self, level: int, angle: int) -> dict:
+        hdr = np.zeros(11)
+        lib().orc_patch_info(self.h, level, angle, _ptr(hdr))
+        n = int(hdr[0])
+        rows = np.zeros(n, np.int32)
+        cols = np.zeros(n, np.int32)
+        w = np.zeros(n)
+        lib().orc_patch_samples(self.h, level, angle, _ptr(rows), _ptr(cols), _ptr(w))
+        return {"n": n, "angle": hdr[1], "orow": hdr[2], "ocol": hdr[3], "rmin": int(hdr[4]), "rmax": int(hdr[5]),
+                "cmin": int(hdr[6]), "cmax": int(hdr[7]), "wsum": hdr[8], "n_runs": int(hdr[9]),
+                "n_wruns": int(hdr[10]), "rows": rows, "cols": cols, "w": w}
+
+    def find(self, image: np.ndarray, start: float, extent: float, min_score: float, num_matches: int,
+             max_overlap: float, subpixel: bool = True, highest: int = 0, lowest: int = 1, trace: bool = False):
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        cap = 4096
+        res = np.zeros((cap, 4))
+        coarse = np.zeros((20000, 4)) if trace else None
+        prep = np.zeros((20000, 4)) if trace else None
+        refd = np.zeros((20000, 5)) if trace else None
+        info = FindOut()
+        st = lib().orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], start, extent, min_score,
+                            num_matches, max_overlap, int(subpixel), highest, lowest, _ptr(res), cap,
+                            _ptr(coarse) if trace else None, 20000, _ptr(prep) if trace else None, 20000,
+                            _ptr(refd) if trace else None, C.byref(info))
+        if st:
+            raise OracleError(st, lib().orc_last_error().decode())
+        out = res[:info.n_result].copy()
+        if not trace:
+            return out
+        return out, {"coarse": coarse[:info.n_coarse].copy(), "prepared": prep[:info.n_prepared].copy(),
+                     "refined": refd[:info.n_prepared].copy(), "info": info}
+
+    def continuous(self, image: np.ndarray, row: float, col: float, angle: float) -> float:
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        return lib().orc_continuous(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], row, col, angle)
+
+
+def pyramid(image: np.ndarray, levels: int) -> list:
+    img = np.ascontiguousarray(image, dtype=np.uint8)
+    h, w = img.shape
+    dims = []
+    for _ in range(1, levels):
+        w, h = max(1, w // 2), max(1, h // 2)
+        dims.append((h, w))
+    out = np.zeros(sum(a * b for a, b in dims))
+    st = lib().orc_pyramid(_ptr(img), img.shape[1], img.shape[0], img.shape[1], levels, _ptr(out))
+    if st:
+        raise OracleError(st, lib().orc_last_error().decode())
+    res, off = [], 0
+    for a, b in dims:
+        res.append(out[off:off + a * b].reshape(a, b))
+        off += a * b
+    return res
+
+
+def row_prefix(level: np.ndarray):
+    v = np.ascontiguousarray(level, dtype=np.float64)
+    h, w = v.shape
+    s = np.zeros((h, w + 1))
+    q = np.zeros((h, w + 1))
+    lib().orc_row_prefix(_ptr(v), w, h, _ptr(s), _ptr(q))
+    return s, q
+
+
+def zncc(a: np.ndarray, b: np.ndarray, absolute: bool = False) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return lib().orc_zncc(_ptr(a), _ptr(b), len(a), int(absolute))
+
+
+def overlap(a, b, w, h, orow, ocol) -> float:
+    return lib().orc_overlap(a[0], a[1], a[2], b[0], b[1], b[2], w, h, orow, ocol)
