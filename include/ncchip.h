@@ -88,13 +88,18 @@ typedef struct NcchipRefineParams {
   int32_t image_h, image_w;
 } NcchipRefineParams;
 
-/* Full continuous score of every candidate at the finest level, then a
- * speculative refine_candidate of each (Gauss-Newton fit, backtracking and
- * the level-0 recovery search), one workgroup per candidate.  Outputs the
- * full-scored candidate, the refined pose and the verdict.
- * Replaces matcher.cpp:813-822, 853-1061 (kernels_avx2.cpp:327-541). */
+/* Continuous bilinear score of every candidate at the finest level, one
+ * workgroup per candidate.  Replaces matcher.cpp:813-822
+ * (kernels_avx2.cpp:327-477, 517-541). */
+int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
+                      NccMatch* out_full, void* stream);
+
+/* refine_candidate of every given full-scored candidate (Gauss-Newton fit
+ * with backtracking, and the level-0 recovery search), one workgroup per
+ * candidate; outputs the refined pose and the verdict (score >= MinScore
+ * inside the domain).  Replaces matcher.cpp:853-1061. */
 int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
-                  NccMatch* out_full, NccMatch* out_refined, int32_t* verdict, void* stream);
+                  NccMatch* out_refined, int32_t* verdict, void* stream);
 
 #ifdef __cplusplus
 }

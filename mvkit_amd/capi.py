@@ -32,7 +32,7 @@ PUBLIC_SYMBOLS = [
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_coarse", "ncchip_coarse_mfma",
-                  "ncchip_peaks", "ncchip_track", "ncchip_refine"]
+                  "ncchip_peaks", "ncchip_track", "ncchip_full_score", "ncchip_refine"]
 
 
 class StringView(C.Structure):

@@ -75,10 +75,14 @@ struct CoarseArgs {
 constexpr int kTileR = 32, kTileC = 64, kColsPerThread = 8;
 
 // FP32 form: one block per (64x32 anchor tile, coarse angle); each thread
-// scores 8 consecutive anchors of one row with a sliding register window over
-// each template run.
-__global__ void __launch_bounds__(256) k_coarse_fp32(CoarseArgs a) {
-  extern __shared__ float tile[];
+// scores 8 consecutive anchors of one row.  The image tile holds the centred
+// integers T = S - C.  Per template run, a sliding register window feeds the
+// FP32 dot products of the 8 anchors, and the exact integer window moments
+// sum(T) and sum(T^2) are accumulated for the first anchor and slid to the
+// other seven with the run's boundary pixels.
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs a) {
+  extern __shared__ int32_t tile[];
   const double dc = double(*a.dc_src);
   const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
   const int ai = blockIdx.z;
@@ -87,36 +91,49 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(CoarseArgs a) {
   const int W = a.lv.width, H = a.lv.height;
   const int r0 = blockIdx.y * kTileR, c0 = blockIdx.x * kTileC;
   const int LH = kTileR + P.rmax - P.rmin, LW = a.lds_w;
-  // stage (S - C) for rows r0+rmin .. r0+rmin+LH-1, cols c0+cmin .. c0+cmin+LW-1
-  for (int i = threadIdx.x; i < LH * LW; i += 256) {
-    const int y = i / LW, x = i % LW;
-    const int gr = r0 + P.rmin + y, gc = c0 + P.cmin + x;
-    float v = 0.f;
-    if (gr >= 0 && gr < H && gc >= 0 && gc < W) v = float(int(load_s(a.lv, gr, gc)) - int(a.C));
-    tile[i] = v;
+  const int Ci = int(a.C);
+  constexpr int U = 8;  // staging loads issued together per thread
+  for (int base = threadIdx.x; base < LH * LW; base += 256 * U) {
+    int32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + 256 * u;
+      const int gr = r0 + P.rmin + i / LW, gc = c0 + P.cmin + i % LW;
+      v[u] = (i < LH * LW && gr >= 0 && gr < H && gc >= 0 && gc < W)
+                 ? int32_t(static_cast<const T*>(a.lv.data)[size_t(gr) * a.lv.pitch + gc]) - Ci
+                 : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + 256 * u < LH * LW) tile[base + 256 * u] = v[u];
   }
   __syncthreads();
   const int ty = threadIdx.x / (kTileC / kColsPerThread), tx = (threadIdx.x % (kTileC / kColsPerThread)) * kColsPerThread;
   float acc[kColsPerThread];
+  int64_t m1[kColsPerThread], m2[kColsPerThread];
 #pragma unroll
-  for (int j = 0; j < kColsPerThread; ++j) acc[j] = 0.f;
+  for (int j = 0; j < kColsPerThread; ++j) { acc[j] = 0.f; m1[j] = 0; m2[j] = 0; }
   const int32_t* runs = a.pool.runs + 4 * P.run_off;
   const float* wf = a.pool.wf + P.sample_off;
   int k = 0;
   for (int ri = 0; ri < P.n_runs; ++ri) {
     const int4 run = reinterpret_cast<const int4*>(runs)[ri];
-    const float* src = tile + (ty + run.x - P.rmin) * LW + (tx + run.y - P.cmin);
+    const int32_t* src = tile + (ty + run.x - P.rmin) * LW + (tx + run.y - P.cmin);
     const int len = run.z - run.y + 1;
     float win[16];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) win[j] = src[j];
+    for (int j = 0; j < 8; ++j) win[j] = float(src[j]);
+    int64_t s1 = 0, s2 = 0;  // moments of the run for the first anchor
     int s = 0;
     for (; s + 8 <= len; s += 8) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) win[8 + j] = src[s + 8 + j];
+      for (int j = 0; j < 8; ++j) win[8 + j] = float(src[s + 8 + j]);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const float w = wf[k + s + u];
+        const int32_t t = int32_t(win[u]);
+        s1 += t;
+        s2 += int64_t(t) * t;
 #pragma unroll
         for (int j = 0; j < kColsPerThread; ++j) acc[j] = fmaf(w, win[u + j], acc[j]);
       }
@@ -125,26 +142,43 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(CoarseArgs a) {
     }
     for (; s < len; ++s) {
       const float w = wf[k + s];
+      const int32_t t = int32_t(win[0]);
+      s1 += t;
+      s2 += int64_t(t) * t;
 #pragma unroll
       for (int j = 0; j < kColsPerThread; ++j) acc[j] = fmaf(w, win[j], acc[j]);
 #pragma unroll
       for (int j = 0; j < 7; ++j) win[j] = win[j + 1];
-      win[7] = src[s + 8];
+      win[7] = float(src[s + 8]);
+    }
+    // slide the run's moments to anchors 1..7: drop src[j-1], add src[len+j-1]
+    m1[0] += s1;
+    m2[0] += s2;
+#pragma unroll
+    for (int j = 1; j < kColsPerThread; ++j) {
+      const int32_t out = src[j - 1], in = src[len + j - 1];
+      s1 += in - out;
+      s2 += int64_t(in) * in - int64_t(out) * out;
+      m1[j] += s1;
+      m2[j] += s2;
     }
     k += len;
   }
   const int r = r0 + ty;
   if (r >= H) return;
   double* out = a.planes + (size_t(ai) * H + r) * W;
-  const double wsum_f = P.wsum_f;
+  const int64_t Cl = Ci;
+#pragma unroll
   for (int j = 0; j < kColsPerThread; ++j) {
     const int c = c0 + tx + j;
     if (c >= W) break;
     double score = NCC_MISSING;
     if (r >= rg.x && r < rg.y && c >= rg.z && c < rg.w) {
-      int64_t s1, s2;
-      window_moments(a.p1, a.p2, W + 1, runs, P.n_runs, r, c, s1, s2);
-      score = normalise(double(acc[j]), a.C, wsum_f, P.wsum, P.n, s1, s2, D, a.lv.shift, dc, a.exact_lattice,
+      // sums of S from sums of T = S - C
+      const int64_t n = P.n;
+      const int64_t sum_s = m1[j] + n * Cl;
+      const int64_t sum_s2 = m2[j] + 2 * Cl * m1[j] + n * Cl * Cl;
+      score = normalise(double(acc[j]), a.C, P.wsum_f, P.wsum, P.n, sum_s, sum_s2, D, a.lv.shift, dc, a.exact_lattice,
                         a.absolute);
     }
     out[c] = score;
@@ -153,7 +187,8 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(CoarseArgs a) {
 
 // Reference-order FP64 scores of every anchor (one thread per anchor); used
 // when the LDS tile would not fit, and as the FP64 form for error reports.
-__global__ void __launch_bounds__(256) k_coarse_fp64(CoarseArgs a) {
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_fp64(const CoarseArgs a) {
   const double dc = double(*a.dc_src);
   const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
   const int ai = blockIdx.z;
@@ -167,7 +202,10 @@ __global__ void __launch_bounds__(256) k_coarse_fp64(CoarseArgs a) {
     const int32_t* rc = a.pool.rc + P.sample_off;
     const double* w = a.pool.w + P.sample_off;
     double dot = 0;
-    for (int i = 0; i < P.n; ++i) dot += load_q(a.lv, r + sample_row(rc[i]), c + sample_col(rc[i]), dc) * w[i];
+    const T* px = static_cast<const T*>(a.lv.data);
+    const double unit = ldexp(1.0, -a.lv.shift);
+    for (int i = 0; i < P.n; ++i)
+      dot += (double(px[size_t(r + sample_row(rc[i])) * a.lv.pitch + c + sample_col(rc[i])]) * unit - dc) * w[i];
     int64_t s1, s2;
     window_moments(a.p1, a.p2, W + 1, a.pool.runs + 4 * P.run_off, P.n_runs, r, c, s1, s2);
     double sum, sq;
@@ -221,13 +259,17 @@ extern "C" int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int LH = kTileR + max_bbox_h;
   a.lds_w = ((kTileC + max_bbox_w + 16) + 3) & ~3;
-  const size_t lds = size_t(LH) * a.lds_w * sizeof(float);
+  const size_t lds = size_t(LH) * a.lds_w * sizeof(int32_t);
   if (form == NCCHIP_FORM_FP32 && lds <= 64 * 1024) {
     const dim3 grid((lv->width + kTileC - 1) / kTileC, (lv->height + kTileR - 1) / kTileR, n_angles);
-    hipLaunchKernelGGL(k_coarse_fp32, grid, dim3(256), lds, st, a);
+    if (lv->kind == NCC_U8) hipLaunchKernelGGL(k_coarse_fp32<uint8_t>, grid, dim3(256), lds, st, a);
+    else if (lv->kind == NCC_U16) hipLaunchKernelGGL(k_coarse_fp32<uint16_t>, grid, dim3(256), lds, st, a);
+    else hipLaunchKernelGGL(k_coarse_fp32<uint32_t>, grid, dim3(256), lds, st, a);
   } else {
     const dim3 grid((lv->width + 255) / 256, lv->height, n_angles);
-    hipLaunchKernelGGL(k_coarse_fp64, grid, dim3(256), 0, st, a);
+    if (lv->kind == NCC_U8) hipLaunchKernelGGL(k_coarse_fp64<uint8_t>, grid, dim3(256), 0, st, a);
+    else if (lv->kind == NCC_U16) hipLaunchKernelGGL(k_coarse_fp64<uint16_t>, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_coarse_fp64<uint32_t>, grid, dim3(256), 0, st, a);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

@@ -34,11 +34,12 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
   return v;
 }
 
-// Block-wide sum of N doubles (blockDim.x == 256); every thread gets the
-// totals.  `scratch` holds at least 4*N doubles of LDS.
+// Block-wide sum of N doubles (blockDim.x a multiple of 64, at most 1024);
+// every thread gets the totals, summed over waves in a fixed order.
+// `scratch` holds at least 16*N doubles of LDS.
 template <int N>
 __device__ __forceinline__ void block_sum(double (&v)[N], double* scratch) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, n_waves = blockDim.x / kWave;
 #pragma unroll
   for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
   if (lane == 0)
@@ -46,7 +47,11 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* scratch) {
     for (int i = 0; i < N; ++i) scratch[wave * N + i] = v[i];
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < N; ++i) v[i] = (scratch[i] + scratch[N + i]) + (scratch[2 * N + i] + scratch[3 * N + i]);
+  for (int i = 0; i < N; ++i) {
+    double t = 0;
+    for (int w = 0; w < n_waves; ++w) t += scratch[w * N + i];
+    v[i] = t;
+  }
   __syncthreads();
 }
 

@@ -489,67 +489,87 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   }
   const double t2 = now_us();
 
-  // ---- full score + speculative refinement (matcher.cpp:813-827, 853-1061)
-  std::vector<Pose> full(cand.size()), refined(cand.size());
-  std::vector<int> verdict(cand.size(), 0);
+  // ---- full score at the finest level, then candidate order (matcher.cpp:813-827)
+  NcchipRefineParams rp{};
+  rp.level = level_desc(finest);
+  rp.dc_src = crop;
+  if (finest) { rp.storage[0] = 0; rp.storage[1] = 0; rp.storage[2] = lv[finest].h; rp.storage[3] = lv[finest].w; }
+  else std::copy(storage0, storage0 + 4, rp.storage);
+  rp.crop_r = double(crop_r); rp.crop_c = double(crop_c);
+  rp.scale = std::ldexp(1., int(finest)); rp.phase = (rp.scale - 1) * .5;
+  rp.finest = int32_t(finest); rp.top = int32_t(top);
+  rp.orig_ref = finest ? ref(1, uint32_t(coarse_original)) : ref(0, model.original);
+  rp.level_orow = model.levels[finest].orow; rp.level_ocol = model.levels[finest].ocol;
+  rp.refs0 = reinterpret_cast<const int32_t*>(dev + off_refs[0]);
+  rp.n_refs0 = int32_t(refs[0].size());
+  rp.step = model.params.step; rp.angle_start = o.start; rp.angle_end = o.start + o.extent;
+  rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
+  rp.image_h = int32_t(H); rp.image_w = int32_t(W);
+  std::vector<Pose> full(cand.size());
   if (!cand.empty()) {
-    NcchipRefineParams rp{};
-    rp.level = level_desc(finest);
-    rp.dc_src = crop;
-    if (finest) { rp.storage[0] = 0; rp.storage[1] = 0; rp.storage[2] = lv[finest].h; rp.storage[3] = lv[finest].w; }
-    else std::copy(storage0, storage0 + 4, rp.storage);
-    rp.crop_r = double(crop_r); rp.crop_c = double(crop_c);
-    rp.scale = std::ldexp(1., int(finest)); rp.phase = (rp.scale - 1) * .5;
-    rp.finest = int32_t(finest); rp.top = int32_t(top);
-    rp.orig_ref = finest ? ref(1, uint32_t(coarse_original)) : ref(0, model.original);
-    rp.level_orow = model.levels[finest].orow; rp.level_ocol = model.levels[finest].ocol;
-    rp.refs0 = reinterpret_cast<const int32_t*>(dev + off_refs[0]);
-    rp.n_refs0 = int32_t(refs[0].size());
-    rp.step = model.params.step; rp.angle_start = o.start; rp.angle_end = o.start + o.extent;
-    rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
-    rp.image_h = int32_t(H); rp.image_w = int32_t(W);
     upload_cand(cand);
-    auto* drefined = reinterpret_cast<NccMatch*>(dev + off_refined);
-    auto* dverdict = reinterpret_cast<int32_t*>(dev + off_verdict);
-    if (ncchip_refine(&rp, pools, dcand, int(cand.size()), dout, drefined, dverdict, stream_))
-      raise(kInternalError, "device", "refine launch failed");
-    const size_t n = cand.size();
-    check(hipMemcpyAsync(rb, dout, n * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipMemcpyAsync(rb + n * sizeof(NccMatch), drefined, n * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipMemcpyAsync(rb + 2 * n * sizeof(NccMatch), dverdict, n * 4, hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipStreamSynchronize(stream_), "refine");
+    if (ncchip_full_score(&rp, pools, dcand, int(cand.size()), dout, stream_))
+      raise(kInternalError, "device", "full score launch failed");
+    check(hipMemcpyAsync(rb, dout, cand.size() * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
+    check(hipStreamSynchronize(stream_), "full score");
     const auto* f = reinterpret_cast<const NccMatch*>(rb);
-    const auto* g = reinterpret_cast<const NccMatch*>(rb + n * sizeof(NccMatch));
-    const auto* v = reinterpret_cast<const int32_t*>(rb + 2 * n * sizeof(NccMatch));
-    for (size_t i = 0; i < n; ++i) {
-      full[i] = {f[i].row, f[i].col, f[i].angle, f[i].score};
-      refined[i] = {g[i].row, g[i].col, g[i].angle, g[i].score};
-      verdict[i] = v[i];
+    for (size_t i = 0; i < cand.size(); ++i) full[i] = {f[i].row, f[i].col, f[i].angle, f[i].score};
+  }
+  std::sort(full.begin(), full.end(), score_order);
+
+  // ---- greedy selection in score order (matcher.cpp:1063-1076).  Each
+  // candidate's refinement is independent of the others, so candidates are
+  // refined on the device in batches, in the order the greedy walk needs them.
+  const size_t n = full.size();
+  std::vector<Pose> refined(n);
+  std::vector<int> verdict(n, -1);  // -1: not refined
+  std::vector<Pose> result;
+  auto suppressed = [&](const Pose& c) {
+    for (const auto& p : result) if (pose_overlap(c, p, base) > o.max_overlap + 1e-12) return true;
+    return false;
+  };
+  auto* drefined = reinterpret_cast<NccMatch*>(dev + off_refined);
+  auto* dverdict = reinterpret_cast<int32_t*>(dev + off_verdict);
+  size_t pos = 0, batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : n;
+  bool done = false;
+  while (!done && pos < n) {
+    // walk as far as the refined results allow
+    while (pos < n) {
+      const Pose& c = full[pos];
+      if (!std::isfinite(c.score) || suppressed(c)) { ++pos; continue; }
+      if (verdict[pos] < 0) break;
+      if (verdict[pos] && !suppressed(refined[pos])) {
+        result.push_back(refined[pos]);
+        if (o.num_matches && result.size() == o.num_matches) { done = true; break; }
+      }
+      ++pos;
     }
+    if (done || pos >= n) break;
+    // refine the next candidates the walk may need
+    std::vector<size_t> batch;
+    std::vector<Pose> poses;
+    for (size_t i = pos; i < n && batch.size() < batch_size; ++i)
+      if (std::isfinite(full[i].score) && verdict[i] < 0 && !suppressed(full[i])) { batch.push_back(i); poses.push_back(full[i]); }
+    upload_cand(poses);
+    if (ncchip_refine(&rp, pools, dcand, int(poses.size()), drefined, dverdict, stream_))
+      raise(kInternalError, "device", "refine launch failed");
+    const size_t m = poses.size();
+    check(hipMemcpyAsync(rb, drefined, m * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
+    check(hipMemcpyAsync(rb + m * sizeof(NccMatch), dverdict, m * 4, hipMemcpyDeviceToHost, stream_), "readback");
+    check(hipStreamSynchronize(stream_), "refine");
+    const auto* g = reinterpret_cast<const NccMatch*>(rb);
+    const auto* v = reinterpret_cast<const int32_t*>(rb + m * sizeof(NccMatch));
+    for (size_t k = 0; k < m; ++k) {
+      refined[batch[k]] = {g[k].row, g[k].col, g[k].angle, g[k].score};
+      verdict[batch[k]] = v[k];
+    }
+    batch_size *= 2;
   }
   const double t3 = now_us();
-
-  // ---- greedy selection in score order (matcher.cpp:1063-1076)
-  std::vector<size_t> order(full.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return score_order(full[a], full[b]); });
   if (trace) {
-    trace->prepared.clear(); trace->refined.clear(); trace->verdict.clear();
-    for (size_t i : order) { trace->prepared.push_back(full[i]); trace->refined.push_back(refined[i]); trace->verdict.push_back(verdict[i]); }
-  }
-  std::vector<Pose> result;
-  for (size_t i : order) {
-    const Pose& c = full[i];
-    if (!std::isfinite(c.score)) continue;
-    bool dup = false;
-    for (const auto& p : result) if (pose_overlap(c, p, base) > o.max_overlap + 1e-12) { dup = true; break; }
-    if (dup || !verdict[i]) continue;
-    const Pose& r = refined[i];
-    dup = false;
-    for (const auto& p : result) if (pose_overlap(r, p, base) > o.max_overlap + 1e-12) { dup = true; break; }
-    if (dup) continue;
-    result.push_back(r);
-    if (o.num_matches && result.size() == o.num_matches) break;
+    trace->prepared = full;
+    trace->refined = refined;
+    trace->verdict = verdict;
   }
   std::sort(result.begin(), result.end(), score_order);
   if (trace) {

@@ -18,8 +18,17 @@ struct PatchRef {
   const double* wrun_w;
 };
 
-__device__ __forceinline__ PatchRef patch_ref(const NccPatchPool* pools, int32_t ref) {
-  const NccPatchPool& pool = pools[ref >> 24];
+// Raster references are (pool << 24) | index with pool 0 the model and pool 1
+// the per-find rasters; the two pools are separate values so that selecting
+// one never indexes a private array.
+struct Pools {
+  NccPatchPool p0, p1;
+  __device__ __forceinline__ const NccPatchPool& of(int32_t ref) const { return (ref >> 24) ? p1 : p0; }
+  __device__ __forceinline__ const NccPatch& patch(int32_t ref) const { return of(ref).patches[ref & 0xffffff]; }
+};
+
+__device__ __forceinline__ PatchRef patch_ref(const Pools& pools, int32_t ref) {
+  const NccPatchPool& pool = pools.of(ref);
   PatchRef out;
   out.P = pool.patches[ref & 0xffffff];
   out.rc = pool.rc + out.P.sample_off;
@@ -28,9 +37,7 @@ __device__ __forceinline__ PatchRef patch_ref(const NccPatchPool* pools, int32_t
   out.wrun_w = pool.wrun_w + out.P.wrun_off;
   return out;
 }
-__device__ __forceinline__ double ref_angle(const NccPatchPool* pools, int32_t ref) {
-  return pools[ref >> 24].patches[ref & 0xffffff].angle;
-}
+__device__ __forceinline__ double ref_angle(const Pools& pools, int32_t ref) { return pools.patch(ref).angle; }
 
 // ------------------------------------------------------------------ tracking
 struct TrackArgs {
@@ -38,7 +45,7 @@ struct TrackArgs {
   double dc;              // filled in-kernel from dc_src
   int64_t D;
   const uint8_t* dc_src;
-  NccPatchPool pools[2];
+  Pools pools;
   const int32_t* refs;
   int n_refs;
   const NccMatch* cand;
@@ -47,34 +54,74 @@ struct TrackArgs {
   double tol, crop_r, crop_c, step, offset;
   int sr, sc, sh, sw;
   int exact_lattice, absolute;
+  int tile_cap;           // LDS tile capacity in pixels
 };
 
-// One window of the reference's 8/4-lane prefix kernels (kernels_avx2.cpp:
-// 145-238): exact moments, dot in sample order or by weight runs.
-__device__ double track_window(const TrackArgs& a, const PatchRef& p, int row, int col) {
+// Pixel source of a tracking window: an LDS tile of the candidate's
+// neighbourhood, or the level in global memory when the tile is too large.
+struct TileReader {
+  const uint32_t* tile;
+  int r0, c0, tw;
+  __device__ __forceinline__ uint32_t at(int r, int c) const { return tile[(r - r0) * tw + (c - c0)]; }
+};
+template <class T>
+struct LevelReader {
+  const T* data;
+  int pitch;
+  __device__ __forceinline__ uint32_t at(int r, int c) const { return data[size_t(r) * pitch + c]; }
+};
+
+// One window of the reference's 8-lane prefix kernel (kernels_avx2.cpp:
+// 209-238), evaluated by one wave with lanes striding over the samples:
+// exact integer moments, and the dot product from per-sample products (or by
+// weight runs).  The dot product is summed in a different order than the
+// reference's single accumulator, so scores agree to ~1e-15.  Every lane of
+// the wave returns the score.
+template <class Reader>
+__device__ __forceinline__ double track_window(const TrackArgs& a, const PatchRef& p, int row, int col,
+                                               const Reader& rd) {
   const NccPatch& P = p.P;
   const int H = a.lv.height, W = a.lv.width;
   if (row + P.rmin < 0 || col + P.cmin < 0 || row + P.rmax >= H || col + P.cmax >= W) return NCC_MISSING;
   if (row + P.rmin < a.sr || col + P.cmin < a.sc || row + P.rmax >= a.sr + a.sh || col + P.cmax >= a.sc + a.sw)
     return NCC_MISSING;
+  const int lane = threadIdx.x & 63;
+  const double unit = ldexp(1.0, -a.lv.shift);
   int64_t s1 = 0, s2 = 0;
   double dot = 0;
-  for (int i = 0; i < P.n; ++i) {
-    const int32_t rc = p.rc[i];
-    const uint32_t s = load_s(a.lv, row + sample_row(rc), col + sample_col(rc));
-    s1 += s;
-    s2 += int64_t(s) * s;
-    if (!P.use_wruns) dot += (ldexp(double(s), -a.lv.shift) - a.dc) * p.w[i];
+  constexpr int U = 8;  // samples per lane whose loads are issued together
+  for (int base = lane; base < P.n; base += 64 * U) {
+    int32_t rcv[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + 64 * u;
+      rcv[u] = i < P.n ? p.rc[i] : 0;
+      wv[u] = i < P.n && !P.use_wruns ? p.w[i] : 0.0;
+    }
+    uint32_t sv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      sv[u] = base + 64 * u < P.n ? rd.at(row + sample_row(rcv[u]), col + sample_col(rcv[u])) : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + 64 * u < P.n) {
+        s1 += sv[u];
+        s2 += int64_t(sv[u]) * sv[u];
+        if (!P.use_wruns) dot += (double(sv[u]) * unit - a.dc) * wv[u];
+      }
   }
   if (P.use_wruns) {
-    for (int k = 0; k < P.n_wruns; ++k) {
+    for (int k = lane; k < P.n_wruns; k += 64) {
       const int4 run = reinterpret_cast<const int4*>(p.wruns)[k];
       int64_t rs = 0;
-      for (int c = run.y; c <= run.z; ++c) rs += load_s(a.lv, row + run.x, col + c);
-      const double run_q = ldexp(double(rs - int64_t(run.z - run.y + 1) * a.D), -a.lv.shift);
-      dot += run_q * p.wrun_w[k];
+      for (int c = run.y; c <= run.z; ++c) rs += rd.at(row + run.x, col + c);
+      dot += double(rs - int64_t(run.z - run.y + 1) * a.D) * unit * p.wrun_w[k];
     }
   }
+  s1 = wave_sum_i64(s1);
+  s2 = wave_sum_i64(s2);
+  dot = wave_sum(dot);
   double sum, sq;
   exact_moments(s1, s2, P.n, a.D, a.lv.shift, sum, sq);
   const double cnt = double(P.n);
@@ -85,22 +132,23 @@ __device__ double track_window(const TrackArgs& a, const PatchRef& p, int row, i
     const int64_t t2 = s2 - 2 * a.D * s1 + int64_t(P.n) * a.D * a.D;
     if (a.exact_lattice || (__int128)t2 * P.n == (__int128)t1 * t1) return NCC_MISSING;
     const double v = ldexp(double((__int128)t2 * P.n - (__int128)t1 * t1), -2 * a.lv.shift) / cnt;
-    const double s = clamp1(dot / sqrt(v));
-    return a.absolute ? fabs(s) : s;
+    const double sc = clamp1(dot / sqrt(v));
+    return a.absolute ? fabs(sc) : sc;
   }
-  const double s = clamp1(dot / sqrt(var));
-  return a.absolute ? fabs(s) : s;
+  const double sc = clamp1(dot / sqrt(var));
+  return a.absolute ? fabs(sc) : sc;
 }
 
 constexpr int kMaxEligible = 1024;
 
 // Ordered compaction of the patch indices whose angle is within `tol` of
 // `angle` (query order kept: it defines the reference's tie-breaking).
-__device__ int eligible_patches(const NccPatchPool* pools, const int32_t* refs, int n_refs, double angle, double tol,
+// Works for any block size that is a multiple of the wave size.
+__device__ __forceinline__ int eligible_patches(const Pools& pools, const int32_t* refs, int n_refs, double angle, double tol,
                                 int* elig, int* wave_cnt) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
   int n_e = 0;
-  for (int base = 0; base < n_refs; base += 256) {
+  for (int base = 0; base < n_refs; base += blockDim.x) {
     const int k = base + threadIdx.x;
     const bool f = k < n_refs && fabs(ref_angle(pools, refs[k]) - angle) <= tol;
     const uint64_t m = __ballot(f);
@@ -110,41 +158,90 @@ __device__ int eligible_patches(const NccPatchPool* pools, const int32_t* refs, 
     int off = n_e;
     for (int w = 0; w < wave; ++w) off += wave_cnt[w];
     if (f && off + pre < kMaxEligible) elig[off + pre] = k;
-    n_e += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    for (int w = 0; w < n_waves; ++w) n_e += wave_cnt[w];
     __syncthreads();
   }
   return min(n_e, kMaxEligible);
 }
 
+constexpr int kTrackThreads = 256;
+
+__device__ __forceinline__ void anchor(const TrackArgs& a, const NccMatch& cd, const NccPatch& P, int& row, int& col) {
+  row = int(round((cd.row - a.crop_r - a.offset) / a.step - P.orow));
+  col = int(round((cd.col - a.crop_c - a.offset) / a.step - P.ocol));
+}
+
 // One block per candidate: score the 5x5 anchors around the candidate for every
 // query patch within the angle window and keep the first strict maximum in the
-// reference's (patch, dr, dc) order (matcher.cpp:670-756).
-__global__ void __launch_bounds__(256) k_track(TrackArgs a) {
+// reference's (patch, dr, dc) order (matcher.cpp:670-756).  Windows are
+// spread over the waves; the candidate's neighbourhood is staged in LDS.
+template <class T>
+__global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
+  extern __shared__ uint32_t tile[];
+  __shared__ int elig[kMaxEligible];
+  __shared__ int wave_cnt[kTrackThreads / 64];
+  __shared__ int box[4];
+  __shared__ double red_s[kTrackThreads];
+  __shared__ int red_i[kTrackThreads];
+  TrackArgs a = args;
   a.dc = double(*a.dc_src);
   a.D = int64_t(*a.dc_src) << a.lv.shift;
-  __shared__ int elig[kMaxEligible];
-  __shared__ int wave_cnt[4];
-  __shared__ double red_s[256];
-  __shared__ int red_i[256];
   const int ci = blockIdx.x;
   if (ci >= a.n_cand) return;
   const NccMatch cd = a.cand[ci];
   const int n_e = eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt);
-  const int n_win = n_e * 25;
+  // union of the windows' sample rectangles, clipped to the valid region
+  const int lo_r = max(0, a.sr), lo_c = max(0, a.sc);
+  const int hi_r = min(a.lv.height, a.sr + a.sh) - 1, hi_c = min(a.lv.width, a.sc + a.sw) - 1;
+  if (threadIdx.x == 0) { box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
+    const NccPatch& P = a.pools.patch(a.refs[elig[e]]);
+    int row, col;
+    anchor(a, cd, P, row, col);
+    atomicMin(&box[0], max(lo_r, row - 2 + P.rmin));
+    atomicMax(&box[1], min(hi_r, row + 2 + P.rmax));
+    atomicMin(&box[2], max(lo_c, col - 2 + P.cmin));
+    atomicMax(&box[3], min(hi_c, col + 2 + P.cmax));
+  }
+  __syncthreads();
+  const int r0 = box[0], c0 = box[2];
+  const int th = box[1] - box[0] + 1, tw = box[3] - box[2] + 1;
+  const T* lvp = static_cast<const T*>(a.lv.data);
+  const bool use_tile = n_e > 0 && th > 0 && tw > 0 && th * tw <= a.tile_cap;
+  if (use_tile) {
+    constexpr int U = 8;
+    for (int base = threadIdx.x; base < th * tw; base += kTrackThreads * U) {
+      uint32_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + kTrackThreads * u;
+        v[u] = i < th * tw ? uint32_t(lvp[size_t(r0 + i / tw) * a.lv.pitch + c0 + i % tw]) : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (base + kTrackThreads * u < th * tw) tile[base + kTrackThreads * u] = v[u];
+    }
+  }
+  __syncthreads();
   double best = NCC_MISSING;
   int best_i = 0x7fffffff;
-  for (int wi = threadIdx.x; wi < n_win; wi += 256) {
+  const int wave = threadIdx.x >> 6;
+  for (int wi = wave; wi < n_e * 25; wi += kTrackThreads / 64) {
     const PatchRef p = patch_ref(a.pools, a.refs[elig[wi / 25]]);
-    const int dr = (wi % 25) / 5 - 2, dc = wi % 5 - 2;
-    const int row = int(round((cd.row - a.crop_r - a.offset) / a.step - p.P.orow));
-    const int col = int(round((cd.col - a.crop_c - a.offset) / a.step - p.P.ocol));
-    const double s = track_window(a, p, row + dr, col + dc);
+    int row, col;
+    anchor(a, cd, p.P, row, col);
+    row += (wi % 25) / 5 - 2;
+    col += wi % 5 - 2;
+    double s;
+    if (use_tile) s = track_window(a, p, row, col, TileReader{tile, r0, c0, tw});
+    else s = track_window(a, p, row, col, LevelReader<T>{lvp, a.lv.pitch});
     if (s > best) { best = s; best_i = wi; }
   }
   red_s[threadIdx.x] = best;
   red_i[threadIdx.x] = best_i;
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = kTrackThreads / 2; h > 0; h >>= 1) {
     if (threadIdx.x < h) {
       const double o = red_s[threadIdx.x + h];
       const int oi = red_i[threadIdx.x + h];
@@ -160,13 +257,12 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a) {
     res.score = NCC_MISSING;
     if (red_s[0] > NCC_MISSING) {
       const int wi = red_i[0];
-      const PatchRef p = patch_ref(a.pools, a.refs[elig[wi / 25]]);
-      const int dr = (wi % 25) / 5 - 2, dc = wi % 5 - 2;
-      const int row = int(round((cd.row - a.crop_r - a.offset) / a.step - p.P.orow));
-      const int col = int(round((cd.col - a.crop_c - a.offset) / a.step - p.P.ocol));
-      res.row = a.crop_r + a.step * (row + dr + p.P.orow) + a.offset;
-      res.col = a.crop_c + a.step * (col + dc + p.P.ocol) + a.offset;
-      res.angle = p.P.angle;
+      const NccPatch& P = a.pools.patch(a.refs[elig[wi / 25]]);
+      int row, col;
+      anchor(a, cd, P, row, col);
+      res.row = a.crop_r + a.step * (row + (wi % 25) / 5 - 2 + P.orow) + a.offset;
+      res.col = a.crop_c + a.step * (col + wi % 5 - 2 + P.ocol) + a.offset;
+      res.angle = P.angle;
       res.score = red_s[0];
     }
     a.out[ci] = res;
@@ -181,7 +277,7 @@ struct RefineArgs {
   int sr, sc, sh, sw;     // storage rectangle at the finest level
   double crop_r, crop_c, scale, phase;
   int finest, top;
-  NccPatchPool pools[2];
+  Pools pools;
   int32_t orig_ref;       // unrotated patch of the finest level (model.original at level 0)
   double lv_orow, lv_ocol;
   const int32_t* refs0;   // level-0 query patches (recovery search)
@@ -195,12 +291,20 @@ struct RefineArgs {
   NccMatch* out_full;
   NccMatch* out_refined;
   int32_t* verdict;
+  // LDS copy of the candidate's neighbourhood (storage-local rectangle)
+  const void* tile;
+  int tr0, tc0, th, tw;
 };
 
+constexpr int kRefineThreads = 512;
+constexpr int kRefineTileBytes = 64 * 1024;
+
 struct Shared {
-  double red[4 * 18];
+  double red[(kRefineThreads / 64) * 18];
   int flag;
   double bcast[8];
+  double win_s[kRefineThreads / 64];
+  int win_i[kRefineThreads / 64];
 };
 
 __device__ __forceinline__ bool domain_contains(const RefineArgs& a, double row, double col) {
@@ -208,13 +312,52 @@ __device__ __forceinline__ bool domain_contains(const RefineArgs& a, double row,
   return r >= 0 && c >= 0 && r < double(a.img_h) && c < double(a.img_w);
 }
 
+// Integer pixel at storage-local coordinates, from the LDS tile when the
+// point lies inside it.
+template <class T>
+__device__ __forceinline__ uint32_t pix_s(const RefineArgs& a, int r, int c) {
+  const int y = r - a.tr0, x = c - a.tc0;
+  if (unsigned(y) < unsigned(a.th) && unsigned(x) < unsigned(a.tw)) return static_cast<const T*>(a.tile)[y * a.tw + x];
+  return static_cast<const T*>(a.lv.data)[size_t(r + a.sr) * a.lv.pitch + (c + a.sc)];
+}
 // Value at storage-local integer coordinates.
+template <class T>
 __device__ __forceinline__ double pix(const RefineArgs& a, int r, int c) {
-  return load_q(a.lv, r + a.sr, c + a.sc, a.dc);
+  return ldexp(double(pix_s<T>(a, r, c)), -a.lv.shift) - a.dc;
+}
+
+// Stages the storage-local neighbourhood of radius `reach` around (pr, pc)
+// into LDS (u8 or u16 levels); records an empty tile when it does not fit.
+template <class T>
+__device__ __forceinline__ void stage_tile(RefineArgs& a, void* lds, double pr, double pc, double reach) {
+  a.tile = lds;
+  a.th = 0; a.tw = 0; a.tr0 = 0; a.tc0 = 0;
+  if (sizeof(T) > 2 || !isfinite(pr) || !isfinite(pc)) return;
+  const int r0 = max(0, int(floor(pr - reach))), r1 = min(a.sh - 1, int(ceil(pr + reach)));
+  const int c0 = max(0, int(floor(pc - reach))), c1 = min(a.sw - 1, int(ceil(pc + reach)));
+  const int h = r1 - r0 + 1, w = c1 - c0 + 1;
+  if (h <= 0 || w <= 0 || h * w * int(sizeof(T)) > kRefineTileBytes) return;
+  const T* src = static_cast<const T*>(a.lv.data);
+  constexpr int U = 8;
+  for (int base = threadIdx.x; base < h * w; base += blockDim.x * U) {
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + int(blockDim.x) * u;
+      v[u] = i < h * w ? uint32_t(src[size_t(r0 + i / w + a.sr) * a.lv.pitch + (c0 + i % w + a.sc)]) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + int(blockDim.x) * u;
+      if (i < h * w) static_cast<T*>(lds)[i] = T(v[u]);
+    }
+  }
+  __syncthreads();
+  a.tr0 = r0; a.tc0 = c0; a.th = h; a.tw = w;
 }
 
 // strictly_inside: kernels_avx2.cpp:261-277
-__device__ bool strictly_inside(const RefineArgs& a, const NccPatch& P, double pr, double pc, double ca, double sa) {
+__device__ __forceinline__ bool strictly_inside(const RefineArgs& a, const NccPatch& P, double pr, double pc, double ca, double sa) {
   if (P.rmin == P.rmax && P.cmin == P.cmax) return false;
   const double dr[2] = {P.rmin - a.lv_orow, P.rmax - a.lv_orow};
   const double dc[2] = {P.cmin - a.lv_ocol, P.cmax - a.lv_ocol};
@@ -230,12 +373,13 @@ __device__ bool strictly_inside(const RefineArgs& a, const NccPatch& P, double p
 }
 
 // Bilinear sample (and gradients) at storage-local (y, x) known to be inside.
+template <class T>
 __device__ __forceinline__ void bilinear_at(const RefineArgs& a, double y, double x, bool interior, double& q,
                                             double& gy, double& gx) {
   const double yr = floor(y), xc = floor(x), fy = y - yr, fx = x - xc;
   const int r = int(yr), c = int(xc);
   const int r1 = interior ? r + 1 : min(r + 1, a.sh - 1), c1 = interior ? c + 1 : min(c + 1, a.sw - 1);
-  const double va = pix(a, r, c), vb = pix(a, r, c1), vd = pix(a, r1, c), ve = pix(a, r1, c1);
+  const double va = pix<T>(a, r, c), vb = pix<T>(a, r, c1), vd = pix<T>(a, r1, c), ve = pix<T>(a, r1, c1);
   const double ux = 1 - fx, uy = 1 - fy;
   q = uy * (ux * va + fx * vb) + fy * (ux * vd + fx * ve);
   gy = ux * (vd - va) + fx * (ve - vb);
@@ -250,7 +394,8 @@ __device__ __forceinline__ bool inside(const RefineArgs& a, double y, double x, 
 // Block-wide bilinear moments of the unrotated patch at a storage-local pose.
 // out: [0] sum q, [1] sum q^2, [2] sum w q, then gs[3], qg[3], wg[3], and
 // gg00, gg10, gg11, gg20, gg21, gg22.  Returns false if any sample is outside.
-__device__ bool bilinear_pass(const RefineArgs& a, const PatchRef& O, double pr, double pc, double angle, bool deriv,
+template <class T>
+__device__ __forceinline__ bool bilinear_pass(const RefineArgs& a, const PatchRef& O, double pr, double pc, double angle, bool deriv,
                               double* out, Shared& sm) {
   const double ca = cos(angle), sa = sin(angle);
   const NccPatch& P = O.P;
@@ -263,30 +408,42 @@ __device__ bool bilinear_pass(const RefineArgs& a, const PatchRef& O, double pr,
     const double y = pr + ca * dr - sa * dc, x = pc + sa * dr + ca * dc;
     if (!inside(a, y, x, deriv)) return false;
     double gy, gx;
-    bilinear_at(a, y, x, false, off, gy, gx);
+    bilinear_at<T>(a, y, x, false, off, gy, gx);
   }
   double v[18];
 #pragma unroll
   for (int k = 0; k < 18; ++k) v[k] = 0;
   int bad = 0;
-  for (int i = threadIdx.x; i < P.n; i += 256) {
-    const int32_t rc = O.rc[i];
-    const double dr = sample_row(rc) - a.lv_orow, dc = sample_col(rc) - a.lv_ocol;
-    double y, x;
-    if (aligned) { y = pr + dr; x = pc + dc; }
-    else { y = (pr + ca * dr) - sa * dc; x = (pc + sa * dr) + ca * dc; }
-    if (!interior && !inside(a, y, x, deriv)) { bad = 1; continue; }
-    double q, gy, gx;
-    bilinear_at(a, y, x, interior, q, gy, gx);
-    q = q - off;
-    const double w = O.w[i];
-    v[0] += q; v[1] += q * q; v[2] += w * q;
-    if (deriv) {
-      const double g[3] = {gy, gx, gy * ((0 - sa * dr) - ca * dc) + gx * (ca * dr - sa * dc)};
+  constexpr int U = 4;  // samples per thread whose template loads are issued together
+  for (int base = threadIdx.x; base < P.n; base += kRefineThreads * U) {
+    int32_t rcv[U];
+    double wv[U];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) { v[3 + j] += g[j]; v[6 + j] += q * g[j]; v[9 + j] += w * g[j]; }
-      v[12] += g[0] * g[0]; v[13] += g[1] * g[0]; v[14] += g[1] * g[1];
-      v[15] += g[2] * g[0]; v[16] += g[2] * g[1]; v[17] += g[2] * g[2];
+    for (int u = 0; u < U; ++u) {
+      const int i = base + kRefineThreads * u;
+      rcv[u] = i < P.n ? O.rc[i] : 0;
+      wv[u] = i < P.n ? O.w[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (base + kRefineThreads * u >= P.n) continue;
+      const double dr = sample_row(rcv[u]) - a.lv_orow, dc = sample_col(rcv[u]) - a.lv_ocol;
+      double y, x;
+      if (aligned) { y = pr + dr; x = pc + dc; }
+      else { y = (pr + ca * dr) - sa * dc; x = (pc + sa * dr) + ca * dc; }
+      if (!interior && !inside(a, y, x, deriv)) { bad = 1; continue; }
+      double q, gy, gx;
+      bilinear_at<T>(a, y, x, interior, q, gy, gx);
+      q = q - off;
+      const double w = wv[u];
+      v[0] += q; v[1] += q * q; v[2] += w * q;
+      if (deriv) {
+        const double g[3] = {gy, gx, gy * ((0 - sa * dr) - ca * dc) + gx * (ca * dr - sa * dc)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) { v[3 + j] += g[j]; v[6 + j] += q * g[j]; v[9 + j] += w * g[j]; }
+        v[12] += g[0] * g[0]; v[13] += g[1] * g[0]; v[14] += g[1] * g[1];
+        v[15] += g[2] * g[0]; v[16] += g[2] * g[1]; v[17] += g[2] * g[2];
+      }
     }
   }
   if (block_any(bad, &sm.flag)) return false;
@@ -310,30 +467,68 @@ __device__ __forceinline__ void to_local(const RefineArgs& a, double row, double
 }
 
 // continuous(): matcher.cpp:121-157
-__device__ double continuous(const RefineArgs& a, const PatchRef& O, double row, double col, double angle, Shared& sm) {
+template <class T>
+__device__ __forceinline__ double continuous(const RefineArgs& a, const PatchRef& O, double row, double col, double angle, Shared& sm) {
   double pr, pc, m[18];
   to_local(a, row, col, pr, pc);
-  if (!bilinear_pass(a, O, pr, pc, angle, false, m, sm)) return NCC_MISSING;
+  if (!bilinear_pass<T>(a, O, pr, pc, angle, false, m, sm)) return NCC_MISSING;
   const double var = m[1] - m[0] * m[0] / double(O.P.n);
   if (!(var > 0)) return NCC_MISSING;
   const double s = clamp1(m[2] / sqrt(var));
   return a.absolute ? fabs(s) : s;
 }
 
+// Gauss-Jordan elimination with partial pivoting on an AXES x (AXES+1)
+// system, in the reference's operation order (matcher.cpp:231-238); row
+// swaps use selects so the matrix stays in registers.
+template <int AXES>
+__device__ __forceinline__ bool solve(double (&M)[3][4], double* d) {
+#pragma unroll
+  for (int i = 0; i < AXES; ++i) {
+    int pv = i;
+#pragma unroll
+    for (int j = i + 1; j < AXES; ++j) if (fabs(M[j][i]) > fabs(M[pv][i])) pv = j;
+    double piv = M[i][i];
+#pragma unroll
+    for (int j = i + 1; j < AXES; ++j) if (j == pv) piv = M[j][i];
+    if (fabs(piv) < 1e-14) return false;
+#pragma unroll
+    for (int j = i + 1; j < AXES; ++j)
+      if (j == pv)
+#pragma unroll
+        for (int k = 0; k <= AXES; ++k) { const double t = M[i][k]; M[i][k] = M[j][k]; M[j][k] = t; }
+    const double dv = M[i][i];
+#pragma unroll
+    for (int k = 0; k <= AXES; ++k) if (k >= i) M[i][k] /= dv;
+#pragma unroll
+    for (int j = 0; j < AXES; ++j)
+      if (j != i) {
+        const double f = M[j][i];
+#pragma unroll
+        for (int k = 0; k <= AXES; ++k) if (k >= i) M[j][k] -= f * M[i][k];
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < AXES; ++i) d[i] = M[i][AXES];
+  return true;
+}
+
 // refinement_step(): matcher.cpp:170-240.  Returns the verdict; writes delta
 // and (when out_score) the clamped score.
-__device__ bool refinement_step(const RefineArgs& a, const PatchRef& O, double row, double col, double angle,
-                                double* delta, double* out_score, const double* min_score, Shared& sm) {
+template <class T>
+__device__ __forceinline__ bool refinement_step(const RefineArgs& a, const PatchRef& O, double row, double col,
+                                                double angle, double* delta, double* out_score,
+                                                const double* min_score, Shared& sm) {
   double pr, pc, m[18];
   to_local(a, row, col, pr, pc);
-  if (!bilinear_pass(a, O, pr, pc, angle, true, m, sm)) return false;
-  bool ok = false;
+  if (!bilinear_pass<T>(a, O, pr, pc, angle, true, m, sm)) return false;
   if (threadIdx.x == 0) {
     const double n = double(O.P.n);
     const double sum = m[0], dot = m[2];
     double energy = m[1];
-    double gs[3] = {m[3], m[4], m[5]}, qg[3] = {m[6], m[7], m[8]}, wg[3] = {m[9], m[10], m[11]};
-    double gg[3][3] = {{m[12], 0, 0}, {m[13], m[14], 0}, {m[15], m[16], m[17]}};
+    const double gs[3] = {m[3], m[4], m[5]}, wg[3] = {m[9], m[10], m[11]};
+    double qg[3] = {m[6], m[7], m[8]};
+    const double gg[3][3] = {{m[12], 0, 0}, {m[13], m[14], 0}, {m[15], m[16], m[17]}};
     double sc_out = NCC_MISSING;
     energy -= sum * sum / n;
     int st = 0;
@@ -341,41 +536,38 @@ __device__ bool refinement_step(const RefineArgs& a, const PatchRef& O, double r
     if (energy > 0) {
       const double sign = a.absolute && dot < 0 ? -1 : 1, score = sign * dot / sqrt(energy);
       sc_out = clamp1(score);
-      st = 1;
-      if (min_score && clamp1(score) <= *min_score) st = 0;
+      st = !(min_score && clamp1(score) <= *min_score);
       if (st) {
         const int axes = a.rot ? 3 : 2;
-        double M[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-        for (int i = 0; i < axes; ++i) qg[i] -= sum * gs[i] / n;
-        for (int i = 0; i < axes; ++i) {
-          for (int j = 0; j < axes; ++j) {
+        double M[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) qg[i] -= sum * gs[i] / n;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
             const int hi = i > j ? i : j, lo = i > j ? j : i;
             M[i][j] = (gg[hi][lo] - gs[i] * gs[j] / n - qg[i] * qg[j] / energy) / energy;
           }
-          M[i][axes] = sign * wg[i] / sqrt(energy) - score * qg[i] / energy;
+          M[i][3] = sign * wg[i] / sqrt(energy) - score * qg[i] / energy;
         }
-        for (int i = 0; i < axes && st; ++i) {
-          int pv = i;
-          for (int j = i + 1; j < axes; ++j) if (fabs(M[j][i]) > fabs(M[pv][i])) pv = j;
-          if (fabs(M[pv][i]) < 1e-14) { st = 0; break; }
-          for (int k = i; k <= axes; ++k) { const double t = M[i][k]; M[i][k] = M[pv][k]; M[pv][k] = t; }
-          const double dv = M[i][i];
-          for (int k = i; k <= axes; ++k) M[i][k] /= dv;
-          for (int j = 0; j < axes; ++j)
-            if (j != i) { const double f = M[j][i]; for (int k = i; k <= axes; ++k) M[j][k] -= f * M[i][k]; }
+        if (axes == 3) {
+          st = solve<3>(M, d);
+        } else {
+          // the translation-only system: the right-hand side sits in column 2
+#pragma unroll
+          for (int i = 0; i < 2; ++i) M[i][2] = M[i][3];
+          st = solve<2>(M, d);
+          d[2] = 0;
         }
-        if (st) {
-          for (int i = 0; i < axes; ++i) d[i] = M[i][axes];
-          if (!a.rot) d[2] = 0;
-          st = isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2]);
-        }
+        if (st) st = isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2]);
       }
     }
     sm.bcast[0] = st; sm.bcast[1] = d[0]; sm.bcast[2] = d[1]; sm.bcast[3] = d[2]; sm.bcast[4] = sc_out;
     sm.bcast[5] = energy > 0 ? 1 : 0;
   }
   __syncthreads();
-  ok = sm.bcast[0] != 0;
+  const bool ok = sm.bcast[0] != 0;
   if (sm.bcast[5] != 0 && out_score) *out_score = sm.bcast[4];
   delta[0] = sm.bcast[1]; delta[1] = sm.bcast[2]; delta[2] = sm.bcast[3];
   __syncthreads();
@@ -385,14 +577,15 @@ __device__ bool refinement_step(const RefineArgs& a, const PatchRef& O, double r
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
 // fit(): matcher.cpp:903-982 without the multi-model memo or guard.
-__device__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
+template <class T>
+__device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
   if (!a.subpixel) return;
   bool cached = false;
   double next_delta[3] = {0, 0, 0};
   for (int it = 0; it < 6; ++it) {
     double delta[3] = {0, 0, 0};
     if (cached) { delta[0] = next_delta[0]; delta[1] = next_delta[1]; delta[2] = next_delta[2]; }
-    else if (!refinement_step(a, O, c.row, c.col, c.angle, delta, nullptr, nullptr, sm)) break;
+    else if (!refinement_step<T>(a, O, c.row, c.col, c.angle, delta, nullptr, nullptr, sm)) break;
     cached = false;
     const double factor = fmin(fmin(1., .75 / fmax(1e-20, fabs(delta[0]))),
                                fmin(.75 / fmax(1e-20, fabs(delta[1])), a.rot ? a.step / fmax(1e-20, fabs(delta[2])) : 1.));
@@ -408,10 +601,10 @@ __device__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared&
       if (bt == 0 && it + 1 < 6) {
         im.score = NCC_MISSING;
         const double floor_score = c.score;
-        next_valid = refinement_step(a, O, im.row, im.col, im.angle, next_delta, &im.score, &floor_score, sm);
-        if (!isfinite(im.score)) im.score = continuous(a, O, im.row, im.col, im.angle, sm);
+        next_valid = refinement_step<T>(a, O, im.row, im.col, im.angle, next_delta, &im.score, &floor_score, sm);
+        if (!isfinite(im.score)) im.score = continuous<T>(a, O, im.row, im.col, im.angle, sm);
       } else {
-        im.score = continuous(a, O, im.row, im.col, im.angle, sm);
+        im.score = continuous<T>(a, O, im.row, im.col, im.angle, sm);
       }
       if (im.score > c.score) { c = im; accepted = true; cached = next_valid; break; }
     }
@@ -420,11 +613,12 @@ __device__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared&
 }
 
 // refine_coarse_candidate(): matcher.cpp:853-877 (lowest level above 0).
-__device__ void fit_coarse(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
+template <class T>
+__device__ __forceinline__ void fit_coarse(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
   if (!a.subpixel) return;
   for (int it = 0; it < 6; ++it) {
     double delta[3] = {0, 0, 0};
-    if (!refinement_step(a, O, c.row, c.col, c.angle, delta, nullptr, nullptr, sm)) break;
+    if (!refinement_step<T>(a, O, c.row, c.col, c.angle, delta, nullptr, nullptr, sm)) break;
     const double factor = fmin(fmin(1., .75 / fmax(1e-20, fabs(delta[0]))),
                                fmin(.75 / fmax(1e-20, fabs(delta[1])), a.rot ? a.step / fmax(1e-20, fabs(delta[2])) : 1.));
     bool accepted = false;
@@ -435,76 +629,149 @@ __device__ void fit_coarse(const RefineArgs& a, const PatchRef& O, NccMatch& c, 
       nx.col += a.scale * st * delta[1];
       nx.angle = clampd(nx.angle + st * delta[2], a.a_start, a.a_end);
       if (!domain_contains(a, nx.row, nx.col)) continue;
-      nx.score = continuous(a, O, nx.row, nx.col, nx.angle, sm);
+      nx.score = continuous<T>(a, O, nx.row, nx.col, nx.angle, sm);
       if (nx.score > c.score) { c = nx; accepted = true; break; }
     }
     if (!accepted) break;
   }
 }
 
-// Direct window NCC at level 0 (correlation_four_avx2 / correlation_avx2):
-// per-window offset at the first sample, block-wide sums.
-__device__ double direct_window(const RefineArgs& a, const PatchRef& p, int row, int col, Shared& sm) {
+// Direct window NCC at level 0 (correlation_four_avx2 / correlation_avx2,
+// kernels_avx2.cpp:105-117, 239-258): per-window offset at the first sample.
+// Evaluated by one wave; every lane of the wave gets the score.
+template <class T>
+__device__ __forceinline__ double direct_window_wave(const RefineArgs& a, const PatchRef& p, int row, int col) {
   const NccPatch& P = p.P;
   if (row + P.rmin < 0 || col + P.cmin < 0 || row + P.rmax >= a.H0 || col + P.cmax >= a.W0) return NCC_MISSING;
   if (row + P.rmin < a.sr || col + P.cmin < a.sc || row + P.rmax >= a.sr + a.sh || col + P.cmax >= a.sc + a.sw)
     return NCC_MISSING;
-  const double off = load_q(a.lv, row + sample_row(p.rc[0]), col + sample_col(p.rc[0]), a.dc);
-  double v[3] = {0, 0, 0};
-  for (int i = threadIdx.x; i < P.n; i += 256) {
-    const int32_t rc = p.rc[i];
-    const double q = load_q(a.lv, row + sample_row(rc), col + sample_col(rc), a.dc) - off;
-    v[0] += q; v[1] += q * q; v[2] += q * p.w[i];
+  const int lr = row - a.sr, lc = col - a.sc;
+  const double off = pix<T>(a, lr + sample_row(p.rc[0]), lc + sample_col(p.rc[0]));
+  double s = 0, sq = 0, d = 0;
+  constexpr int U = 8;
+  for (int base = threadIdx.x & 63; base < P.n; base += 64 * U) {
+    int32_t rcv[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + 64 * u;
+      rcv[u] = i < P.n ? p.rc[i] : 0;
+      wv[u] = i < P.n ? p.w[i] : 0.0;
+    }
+    double qv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) qv[u] = base + 64 * u < P.n ? pix<T>(a, lr + sample_row(rcv[u]), lc + sample_col(rcv[u])) - off : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + 64 * u < P.n) { s += qv[u]; sq += qv[u] * qv[u]; d += qv[u] * wv[u]; }
   }
-  block_sum<3>(v, sm.red);
-  const double var = v[1] - v[0] * v[0] / double(P.n);
+  s = wave_sum(s); sq = wave_sum(sq); d = wave_sum(d);
+  const double var = sq - s * s / double(P.n);
   if (!(var > 0)) return NCC_MISSING;
-  const double s = clamp1(v[2] / sqrt(var));
-  return a.absolute ? fabs(s) : s;
+  const double sc = clamp1(d / sqrt(var));
+  return a.absolute ? fabs(sc) : sc;
 }
 
-// One block per candidate: full continuous score (matcher.cpp:822), then the
-// speculative refine_candidate (matcher.cpp:878-1061) from that score.
-__global__ void __launch_bounds__(256) k_refine(RefineArgs a) {
+// Recovery search (matcher.cpp:986-1059): the 5x5 level-0 anchors around the
+// seed for every level-0 patch within one angle step.  Windows are spread over
+// the waves; the first strict maximum above `floor_score` in the reference's
+// (patch, dr, dc) order wins.  Returns false when no window beats it.
+template <class T>
+__device__ __forceinline__ bool recovery_search(const RefineArgs& a, const NccMatch& seed, double floor_score, NccMatch& best,
+                                int* elig, int* wave_cnt, Shared& sm) {
+  const int n_e = eligible_patches(a.pools, a.refs0, a.n_refs0, seed.angle, a.step * 1.01, elig, wave_cnt);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = kRefineThreads / 64;
+  double wb = NCC_MISSING;
+  int wi = 0x7fffffff;
+  for (int k = wave; k < n_e * 25; k += n_waves) {
+    const PatchRef p = patch_ref(a.pools, a.refs0[elig[k / 25]]);
+    const int r = int(round(seed.row - a.crop_r - p.P.orow)) + (k % 25) / 5 - 2;
+    const int c = int(round(seed.col - a.crop_c - p.P.ocol)) + k % 5 - 2;
+    if (!domain_contains(a, a.crop_r + r + p.P.orow, a.crop_c + c + p.P.ocol)) continue;
+    const double sc = direct_window_wave<T>(a, p, r, c);
+    if (sc > wb) { wb = sc; wi = k; }
+  }
+  if (lane == 0) { sm.win_s[wave] = wb; sm.win_i[wave] = wi; }
+  __syncthreads();
+  double bs = NCC_MISSING;
+  int bi = 0x7fffffff;
+  for (int w = 0; w < n_waves; ++w)
+    if (sm.win_s[w] > bs || (sm.win_s[w] == bs && sm.win_i[w] < bi)) { bs = sm.win_s[w]; bi = sm.win_i[w]; }
+  __syncthreads();
+  if (!(bs > floor_score)) return false;
+  const PatchRef p = patch_ref(a.pools, a.refs0[elig[bi / 25]]);
+  const int r = int(round(seed.row - a.crop_r - p.P.orow)) + (bi % 25) / 5 - 2;
+  const int c = int(round(seed.col - a.crop_c - p.P.ocol)) + bi % 5 - 2;
+  best.row = a.crop_r + r + p.P.orow;
+  best.col = a.crop_c + c + p.P.ocol;
+  best.angle = p.P.angle;
+  best.score = bs;
+  return true;
+}
+
+// Largest distance of the unrotated patch's bounding box from its origin.
+__device__ __forceinline__ double template_reach(const RefineArgs& a, const NccPatch& P) {
+  const double dy = fmax(a.lv_orow - P.rmin, P.rmax - a.lv_orow), dx = fmax(a.lv_ocol - P.cmin, P.cmax - a.lv_ocol);
+  return sqrt(dy * dy + dx * dx);
+}
+
+// One block per candidate: the continuous score at the finest level
+// (matcher.cpp:813-822).
+template <class T>
+__global__ void __launch_bounds__(kRefineThreads) k_full_score(const RefineArgs args) {
+  RefineArgs a = args;
   a.dc = double(*a.dc_src);
   __shared__ Shared sm;
-  __shared__ int elig[kMaxEligible];
-  __shared__ int wave_cnt[4];
+  extern __shared__ uint8_t lds_tile[];
   const int ci = blockIdx.x;
   if (ci >= a.n_cand) return;
   const PatchRef O = patch_ref(a.pools, a.orig_ref);
   NccMatch cd = a.cand[ci];
-  if (isfinite(cd.score)) cd.score = continuous(a, O, cd.row, cd.col, cd.angle, sm);
+  {
+    double pr, pc;
+    to_local(a, cd.row, cd.col, pr, pc);
+    stage_tile<T>(a, lds_tile, pr, pc, template_reach(a, O.P) + 2);
+  }
+  if (isfinite(cd.score)) cd.score = continuous<T>(a, O, cd.row, cd.col, cd.angle, sm);
   if (threadIdx.x == 0) a.out_full[ci] = cd;
+}
+
+// One block per candidate: refine_candidate (matcher.cpp:853-1061) from a
+// full-scored pose -- Gauss-Newton fit, then for a level-0 fit that stays
+// below the minimum score the recovery search and a second fit.
+template <class T>
+__global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args) {
+  RefineArgs a = args;
+  a.dc = double(*a.dc_src);
+  __shared__ Shared sm;
+  __shared__ int elig[kMaxEligible];
+  __shared__ int wave_cnt[kRefineThreads / 64];
+  extern __shared__ uint8_t lds_tile[];
+  const int ci = blockIdx.x;
+  if (ci >= a.n_cand) return;
+  const PatchRef O = patch_ref(a.pools, a.orig_ref);
+  NccMatch cd = a.cand[ci];
   if (!isfinite(cd.score)) {
     if (threadIdx.x == 0) { a.out_refined[ci] = cd; a.verdict[ci] = 0; }
     return;
   }
+  {
+    // the fit moves at most a few pixels; farther samples fall back to global memory
+    double pr, pc;
+    to_local(a, cd.row, cd.col, pr, pc);
+    stage_tile<T>(a, lds_tile, pr, pc, template_reach(a, O.P) + 12);
+  }
   if (a.finest) {
-    fit_coarse(a, O, cd, sm);
+    fit_coarse<T>(a, O, cd, sm);
   } else {
     const NccMatch seed = cd;
-    fit(a, O, cd, sm);
+    fit<T>(a, O, cd, sm);
     if (a.subpixel && a.top > 0 && cd.score < a.min_score) {
       NccMatch best = cd;
-      const int n_e = eligible_patches(a.pools, a.refs0, a.n_refs0, seed.angle, a.step * 1.01, elig, wave_cnt);
-      for (int e = 0; e < n_e; ++e) {
-        const PatchRef p = patch_ref(a.pools, a.refs0[elig[e]]);
-        const int r = int(round(seed.row - a.crop_r - p.P.orow)), c = int(round(seed.col - a.crop_c - p.P.ocol));
-        for (int dr = -2; dr <= 2; ++dr)
-          for (int dc = -2; dc <= 2; ++dc) {
-            NccMatch ps;
-            ps.row = a.crop_r + r + dr + p.P.orow;
-            ps.col = a.crop_c + c + dc + p.P.ocol;
-            ps.angle = p.P.angle;
-            if (!domain_contains(a, ps.row, ps.col)) continue;
-            ps.score = direct_window(a, p, r + dr, c + dc, sm);
-            if (ps.score > best.score) best = ps;
-          }
-      }
-      if (best.row != cd.row || best.col != cd.col || best.angle != cd.angle) {
-        best.score = continuous(a, O, best.row, best.col, best.angle, sm);
-        fit(a, O, best, sm);
+      if (recovery_search<T>(a, seed, cd.score, best, elig, wave_cnt, sm) &&
+          (best.row != cd.row || best.col != cd.col || best.angle != cd.angle)) {
+        best.score = continuous<T>(a, O, best.row, best.col, best.angle, sm);
+        fit<T>(a, O, best, sm);
         if (best.score > cd.score) cd = best;
       }
     }
@@ -525,26 +792,58 @@ extern "C" int ncchip_track(const NccLevel* lv, const uint8_t* dc_src, const Ncc
                             int absolute, void* stream) {
   if (n_cand <= 0) return 0;
   TrackArgs a;
-  a.lv = *lv; a.dc_src = dc_src; a.dc = 0; a.D = 0; a.pools[0] = pools[0]; a.pools[1] = pools[1]; a.refs = refs; a.n_refs = n_refs;
-  a.cand = cand; a.out = out; a.n_cand = n_cand; a.tol = tol; a.crop_r = crop_r; a.crop_c = crop_c; a.step = step;
-  a.offset = offset; a.sr = storage[0]; a.sc = storage[1]; a.sh = storage[2]; a.sw = storage[3];
-  a.exact_lattice = exact_lattice; a.absolute = absolute;
-  hipLaunchKernelGGL(k_track, dim3(n_cand), dim3(256), 0, static_cast<hipStream_t>(stream), a);
+  a.lv = *lv; a.dc_src = dc_src; a.dc = 0; a.D = 0; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.refs = refs;
+  a.n_refs = n_refs; a.cand = cand; a.out = out; a.n_cand = n_cand; a.tol = tol; a.crop_r = crop_r;
+  a.crop_c = crop_c; a.step = step; a.offset = offset; a.sr = storage[0]; a.sc = storage[1]; a.sh = storage[2];
+  a.sw = storage[3]; a.exact_lattice = exact_lattice; a.absolute = absolute;
+  constexpr size_t kTileBytes = 64 * 1024;
+  a.tile_cap = int(kTileBytes / sizeof(uint32_t));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (lv->kind == NCC_U8) hipLaunchKernelGGL(k_track<uint8_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
+  else if (lv->kind == NCC_U16) hipLaunchKernelGGL(k_track<uint16_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
+  else hipLaunchKernelGGL(k_track<uint32_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 
-extern "C" int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
-                             NccMatch* out_full, NccMatch* out_refined, int32_t* verdict, void* stream) {
-  if (n_cand <= 0) return 0;
+namespace {
+RefineArgs refine_args(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand) {
   RefineArgs a;
   a.lv = p->level; a.dc_src = p->dc_src; a.dc = 0; a.sr = p->storage[0]; a.sc = p->storage[1]; a.sh = p->storage[2];
   a.sw = p->storage[3]; a.crop_r = p->crop_r; a.crop_c = p->crop_c; a.scale = p->scale; a.phase = p->phase;
-  a.finest = p->finest; a.top = p->top; a.pools[0] = pools[0]; a.pools[1] = pools[1]; a.orig_ref = p->orig_ref;
+  a.finest = p->finest; a.top = p->top; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.orig_ref = p->orig_ref;
   a.lv_orow = p->level_orow; a.lv_ocol = p->level_ocol; a.refs0 = p->refs0; a.n_refs0 = p->n_refs0;
   a.step = p->step; a.a_start = p->angle_start; a.a_end = p->angle_end; a.min_score = p->min_score;
   a.subpixel = p->subpixel; a.rot = p->rotate; a.absolute = p->absolute; a.img_h = p->image_h; a.img_w = p->image_w;
   a.H0 = p->level.height; a.W0 = p->level.width;
-  a.cand = cand; a.n_cand = n_cand; a.out_full = out_full; a.out_refined = out_refined; a.verdict = verdict;
-  hipLaunchKernelGGL(k_refine, dim3(n_cand), dim3(256), 0, static_cast<hipStream_t>(stream), a);
+  a.cand = cand; a.n_cand = n_cand; a.out_full = nullptr; a.out_refined = nullptr; a.verdict = nullptr;
+  a.tile = nullptr; a.tr0 = a.tc0 = a.th = a.tw = 0;
+  return a;
+}
+}  // namespace
+
+extern "C" int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
+                                 int n_cand, NccMatch* out_full, void* stream) {
+  if (n_cand <= 0) return 0;
+  RefineArgs a = refine_args(p, pools, cand, n_cand);
+  a.out_full = out_full;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(n_cand), b(kRefineThreads);
+  if (p->level.kind == NCC_U8) hipLaunchKernelGGL(k_full_score<uint8_t>, g, b, kRefineTileBytes, st, a);
+  else if (p->level.kind == NCC_U16) hipLaunchKernelGGL(k_full_score<uint16_t>, g, b, kRefineTileBytes, st, a);
+  else hipLaunchKernelGGL(k_full_score<uint32_t>, g, b, kRefineTileBytes, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
+                             NccMatch* out_refined, int32_t* verdict, void* stream) {
+  if (n_cand <= 0) return 0;
+  RefineArgs a = refine_args(p, pools, cand, n_cand);
+  a.out_refined = out_refined;
+  a.verdict = verdict;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(n_cand), b(kRefineThreads);
+  if (p->level.kind == NCC_U8) hipLaunchKernelGGL(k_refine<uint8_t>, g, b, kRefineTileBytes, st, a);
+  else if (p->level.kind == NCC_U16) hipLaunchKernelGGL(k_refine<uint16_t>, g, b, kRefineTileBytes, st, a);
+  else hipLaunchKernelGGL(k_refine<uint32_t>, g, b, kRefineTileBytes, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
