@@ -1,0 +1,167 @@
+"""NCC find benchmark on one MI355X: the 2048x2048, +-30 degree rotation
+search workload (128x128 template pasted 5 times with bilinear rotation in
+[-30, 30] degrees, N(0,8) noise and a 0.8-1.2 gain ramp; model with 4 levels
+and the automatic angle step; MinScore 0.7, NumMatches 5, MaxOverlap 0.5,
+subpixel refinement).
+
+A step is one find over a batch of `--inflight` frames (default 4), each
+running on its own host thread and HIP stream, with every frame already
+resident in HBM when the timed region starts.  The throughput `value` is
+search megapixels per second over the whole timed region.
+
+Prints one JSON line (rank 0).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FORMS = {"fp32": 0, "bf16x3": 1, "fp64": 2, "fp8": 3}
+WORKLOAD = {"workload": "rot2048_pm30", "image": [2048, 2048], "template": [128, 128], "instances": 5,
+            "angle_range_deg": [-30, 30], "levels": 4, "angle_step": "auto", "min_score": 0.7, "num_matches": 5,
+            "max_overlap": 0.5, "subpixel": True}
+START, EXTENT = -math.pi / 6, math.pi / 3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--form", default="fp32", choices=sorted(FORMS))
+    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--cpu-finds", type=int, default=24, help="finds in the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the NCC find path shards by frame, and only the single-GPU "
+              "configuration is benchmarked here (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mvkit_amd import capi, synth
+
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.set_device(0)
+    F = max(1, args.inflight)
+    log(f"generating {F} frames")
+    scenes = [synth.workload_rot_2048(1 + 100 * k) if k else synth.workload_rot_2048() for k in range(F)]
+    model = capi.NccModel(scenes[0].template, 4, START, EXTENT)
+    frames = [torch.from_numpy(s.image).cuda() for s in scenes]
+    torch.cuda.synchronize()
+    form = FORMS[args.form]
+    opts = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form)
+    prof = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form, profile=True)
+
+    # parity guard: the benchmarked path must reproduce the truth of frame 0
+    res, st = model.find_device(frames[0].data_ptr(), 2048, 2048, 2048, opts)
+    assert len(res) == 5, res
+    for r, c, a, s in res:
+        t = min(scenes[0].truth, key=lambda t: math.hypot(r - t[0], c - t[1]))
+        assert math.hypot(r - t[0], c - t[1]) < 0.2 and abs(a - t[2]) < 2e-3, (res, scenes[0].truth)
+
+    latencies = []
+    kernel = {k: 0.0 for k in capi.STAGES}
+    coarse_flops = [0.0]
+    lock = threading.Lock()
+    barrier = threading.Barrier(F + 1)
+    n_steps = {"n": 0}
+
+    def worker(k, steps, profile):
+        ptr = frames[k].data_ptr()
+        o = prof if profile else opts
+        for _ in range(steps):
+            barrier.wait()
+            t0 = time.perf_counter()
+            r, s = model.find_device(ptr, 2048, 2048, 2048, o)
+            dt = time.perf_counter() - t0
+            with lock:
+                latencies.append(dt)
+                if profile:
+                    for i, name in enumerate(capi.STAGES):
+                        kernel[name] += s.kernel_us[i]
+                    coarse_flops[0] += s.coarse_flops
+            barrier.wait()
+
+    def run(steps, profile):
+        ts = [threading.Thread(target=worker, args=(k, steps, profile)) for k in range(F)]
+        for t in ts:
+            t.start()
+        for _ in range(steps):
+            barrier.wait()   # release the batch
+            barrier.wait()   # batch done
+        for t in ts:
+            t.join()
+
+    log(f"warmup {args.warmup} steps x {F} frames")
+    run(args.warmup, False)
+    latencies.clear()
+    torch.cuda.synchronize()
+    log(f"timed {args.steps} steps x {F} frames, form {args.form}")
+    t0 = time.perf_counter()
+    run(args.steps, True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    frames_done = args.steps * F
+    mpx = 2048 * 2048 * frames_done / elapsed / 1e6
+    lat = np.array(latencies) * 1e3
+
+    # roofline of the coarse correlation (the path's dense contraction)
+    coarse_us = kernel["coarse"] / frames_done
+    flops = coarse_flops[0] / frames_done
+    achieved = flops / (coarse_us * 1e-6) / 1e12 if coarse_us > 0 else 0.0
+    peak = {0: 157.3, 1: 2500.0, 2: 78.6, 3: 5000.0}[form]
+    total_kernel = sum(kernel.values())
+    dominant = max(kernel, key=kernel.get)
+
+    cpu = None
+    if args.cpu_finds > 0:
+        from oracle import pyoracle
+        log(f"CPU baseline: {args.cpu_finds} oracle finds on one core")
+        o = pyoracle.Model(scenes[0].template, levels=4, start=START, extent=EXTENT)
+        o.find(scenes[0].image, START, EXTENT, 0.7, 5, 0.5)
+        t = time.perf_counter()
+        for _ in range(args.cpu_finds):
+            o.find(scenes[0].image, START, EXTENT, 0.7, 5, 0.5)
+        cpu_s = (time.perf_counter() - t) / args.cpu_finds
+        cpu = {"value": round(2048 * 2048 / cpu_s / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_finds} finds of frame 0 (scalar FP64 restatement, one thread)",
+               "ms_per_frame": round(cpu_s * 1e3, 2)}
+
+    line = {
+        "metric": "search Mpx/s & ms/frame, 2048^2 +-30 deg NCC find",
+        "value": round(mpx, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_frame": round(elapsed / frames_done * 1e3, 4),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
+        "frames_in_flight": F, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {0: "fp32", 1: "bf16x3", 2: "f64", 3: "fp8+bf16x3"}[form] + " correlation / f64 refinement",
+        "form": args.form, "data": "synthetic (deterministic SplitMix64 generator)",
+        "config": WORKLOAD,
+        "roofline": {"kernel": "coarse correlation", "bound": "mfma" if form in (1, 3) else "valu",
+                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 5), "traffic": None,
+                     "launch_us": round(coarse_us, 2), "algorithmic_gflop_per_launch": round(flops / 1e9, 4)},
+        "kernel_us_per_frame": {k: round(v / frames_done, 2) for k, v in kernel.items()},
+        "dominant_kernel": {"stage": dominant, "share": round(kernel[dominant] / total_kernel, 3) if total_kernel else 0},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,12 +26,25 @@ typedef struct MvAmdFindOptions {
   int32_t highest_level, lowest_level;  /* 1-based; highest 0 = all */
   int32_t form;                          /* MVAMD_FORM_* */
   double screen;                         /* FP8 screen threshold (form FP8_SCREEN) */
+  int32_t profile;                       /* time each kernel stage with HIP events */
 } MvAmdFindOptions;
+
+/* Kernel stages reported in MvAmdFindStats.kernel_us. */
+#define MVAMD_STAGE_PYRAMID 0
+#define MVAMD_STAGE_PREFIX 1
+#define MVAMD_STAGE_COARSE 2
+#define MVAMD_STAGE_PEAKS 3
+#define MVAMD_STAGE_TRACK 4
+#define MVAMD_STAGE_FULL 5
+#define MVAMD_STAGE_REFINE 6
+#define MVAMD_STAGE_COUNT 7
 
 typedef struct MvAmdFindStats {
   int32_t n_coarse, n_prepared, top, finest, angle_stride;
   int32_t storage[4];
   double t_coarse_us, t_track_us, t_refine_us, t_total_us;
+  double kernel_us[MVAMD_STAGE_COUNT];   /* device time per stage (profile != 0) */
+  double coarse_flops;                   /* algorithmic FLOPs of the coarse correlation */
 } MvAmdFindStats;
 
 /* Find on a u8 image given as a HIP device pointer on the calling thread's

@@ -37,11 +37,12 @@ int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const 
 
 /* Coarse exhaustive NCC over every anchor of the top level for each coarse
  * angle, writing FP64 score planes [n_angles][h][w] (-inf outside each
- * angle's valid anchor range).  `ranges` holds 4 ints per angle: row begin,
- * row end, col begin, col end.  Replaces matcher.cpp:542-579 with
- * kernels_avx2.cpp:105-238. */
+ * angle's valid anchor range).  `pools` points to two raster pools (the
+ * model's and the per-find rasters); `patch_ids` are (pool << 24 | index).
+ * `ranges` holds 4 ints per angle: row begin, row end, col begin, col end.
+ * Replaces matcher.cpp:542-579 with kernels_avx2.cpp:105-238. */
 int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1, const uint64_t* p2,
-                  const NccPatchPool* pool, const int32_t* patch_ids, const int32_t* ranges, int n_angles,
+                  const NccPatchPool* pools, const int32_t* patch_ids, const int32_t* ranges, int n_angles,
                   int max_bbox_h, int max_bbox_w, int form, int exact_lattice, int absolute, double* planes,
                   void* stream);
 
@@ -49,7 +50,7 @@ int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C, const uin
  * `screen` > -inf enables the FP8-E4M3 screen: anchors whose screened score
  * is below it are written -inf, survivors are re-scored in split-BF16. */
 int ncchip_coarse_mfma(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1, const uint64_t* p2,
-                       const NccPatchPool* pool, const int32_t* patch_ids, const int32_t* ranges, int n_angles,
+                       const NccPatchPool* pools, const int32_t* patch_ids, const int32_t* ranges, int n_angles,
                        int bbox_h, int bbox_w, int exact_lattice, int absolute, double screen, double* planes,
                        void* stream);
 

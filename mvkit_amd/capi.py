@@ -51,13 +51,16 @@ class FindOptions(C.Structure):
     _fields_ = [("angle_start", C.c_double), ("angle_extent", C.c_double), ("min_score", C.c_double),
                 ("max_overlap", C.c_double), ("num_matches", C.c_int64), ("subpixel", C.c_int32),
                 ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
-                ("screen", C.c_double)]
+                ("screen", C.c_double), ("profile", C.c_int32)]
 
 
 class FindStats(C.Structure):
     _fields_ = [("n_coarse", C.c_int32), ("n_prepared", C.c_int32), ("top", C.c_int32), ("finest", C.c_int32),
                 ("angle_stride", C.c_int32), ("storage", C.c_int32 * 4), ("t_coarse_us", C.c_double),
-                ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double)]
+                ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double),
+                ("kernel_us", C.c_double * 7), ("coarse_flops", C.c_double)]
+
+STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine"]
 
 
 class MvError(RuntimeError):
@@ -175,8 +178,11 @@ class Tuple:
         return out
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mv_destroy_tuple(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.mv_destroy_tuple(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
 
@@ -195,13 +201,21 @@ class Image:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mv_clear_obj(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.mv_clear_obj(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
 
 def _to_tuple(v):
     return v if isinstance(v, Tuple) else Tuple(v)
+
+
+def _real(v):
+    """Numbers become reals; anything else is passed through unchanged."""
+    return float(v) if isinstance(v, (int, float, np.integer, np.floating)) and not isinstance(v, bool) else v
 
 
 class NccModel:
@@ -244,18 +258,17 @@ class NccModel:
         in the C ABI's HALCON convention."""
         img = image if isinstance(image, Image) else Image(image)
         outs = [C.c_void_p() for _ in range(4)]
-        args = [_to_tuple(float(angle_start)), _to_tuple(float(angle_extent)), _to_tuple(float(min_score)),
-                _to_tuple(int(num_matches)), _to_tuple(float(max_overlap)), _to_tuple(subpixel),
-                _to_tuple(num_levels)]
+        args = [_to_tuple(_real(angle_start)), _to_tuple(_real(angle_extent)), _to_tuple(_real(min_score)),
+                _to_tuple(num_matches), _to_tuple(_real(max_overlap)), _to_tuple(subpixel), _to_tuple(num_levels)]
         _check(lib().mv_find_ncc_model(img.h, self.tuple.h, *[t.h for t in args], *[C.byref(o) for o in outs]))
         cols = [Tuple(handle=o).values() for o in outs]
         return np.array(cols, dtype=np.float64).T.reshape(-1, 4)
 
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
-                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf) -> FindOptions:
+                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=False) -> FindOptions:
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
-                           lowest, form, screen)
+                           lowest, form, screen, int(profile))
 
     def find_device(self, d_image: int, width: int, height: int, pitch: int, opts: FindOptions, cap: int = 4096):
         """Find on a device-resident u8 image (integer device address)."""

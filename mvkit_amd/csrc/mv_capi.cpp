@@ -254,7 +254,7 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
   q.num_matches = size_t(o->num_matches);
   q.subpixel = o->subpixel != 0;
   q.highest = o->highest_level; q.lowest = std::max(1, o->lowest_level);
-  q.form = o->form; q.screen = o->screen;
+  q.form = o->form; q.screen = o->screen; q.profile = o->profile != 0;
   if (q.form < MVAMD_FORM_FP32 || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
   return q;
@@ -560,6 +560,8 @@ void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
   std::copy(t.storage, t.storage + 4, s->storage);
   s->t_coarse_us = t.t_coarse_us; s->t_track_us = t.t_track_us; s->t_refine_us = t.t_refine_us;
   s->t_total_us = t.t_total_us;
+  for (int k = 0; k < MVAMD_STAGE_COUNT; ++k) s->kernel_us[k] = t.kernel_us[k];
+  s->coarse_flops = t.coarse_flops;
 }
 void put_poses(const std::vector<mvamd::Pose>& v, double* dst, size_t cap) {
   for (size_t i = 0; i < v.size() && i < cap; ++i) {

@@ -64,8 +64,8 @@ struct CoarseArgs {
   double C;                // centring constant subtracted from S before the dot product
   const uint32_t* p1;      // row prefix of S, (w+1) per row
   const uint64_t* p2;      // row prefix of S^2
-  NccPatchPool pool;
-  const int32_t* patch_ids;  // coarse patches (indices into pool)
+  Pools pools;
+  const int32_t* patch_ids;  // coarse patches as (pool << 24) | index
   const int4* ranges;        // per coarse patch: valid anchor range [rb, re) x [cb, ce)
   double* planes;            // [A][h][w] scores, -inf outside the valid range
   int exact_lattice, absolute;
@@ -86,7 +86,8 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs a) {
   const double dc = double(*a.dc_src);
   const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
   const int ai = blockIdx.z;
-  const NccPatch P = a.pool.patches[a.patch_ids[ai]];
+  const NccPatchPool& pool = a.pools.of(a.patch_ids[ai]);
+  const NccPatch P = a.pools.patch(a.patch_ids[ai]);
   const int4 rg = a.ranges[ai];
   const int W = a.lv.width, H = a.lv.height;
   const int r0 = blockIdx.y * kTileR, c0 = blockIdx.x * kTileC;
@@ -113,8 +114,8 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs a) {
   int64_t m1[kColsPerThread], m2[kColsPerThread];
 #pragma unroll
   for (int j = 0; j < kColsPerThread; ++j) { acc[j] = 0.f; m1[j] = 0; m2[j] = 0; }
-  const int32_t* runs = a.pool.runs + 4 * P.run_off;
-  const float* wf = a.pool.wf + P.sample_off;
+  const int32_t* runs = pool.runs + 4 * P.run_off;
+  const float* wf = pool.wf + P.sample_off;
   int k = 0;
   for (int ri = 0; ri < P.n_runs; ++ri) {
     const int4 run = reinterpret_cast<const int4*>(runs)[ri];
@@ -192,22 +193,23 @@ __global__ void __launch_bounds__(256) k_coarse_fp64(const CoarseArgs a) {
   const double dc = double(*a.dc_src);
   const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
   const int ai = blockIdx.z;
-  const NccPatch P = a.pool.patches[a.patch_ids[ai]];
+  const NccPatchPool& pool = a.pools.of(a.patch_ids[ai]);
+  const NccPatch P = a.pools.patch(a.patch_ids[ai]);
   const int4 rg = a.ranges[ai];
   const int W = a.lv.width, H = a.lv.height;
   const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
   if (c >= W || r >= H) return;
   double score = NCC_MISSING;
   if (r >= rg.x && r < rg.y && c >= rg.z && c < rg.w) {
-    const int32_t* rc = a.pool.rc + P.sample_off;
-    const double* w = a.pool.w + P.sample_off;
+    const int32_t* rc = pool.rc + P.sample_off;
+    const double* w = pool.w + P.sample_off;
     double dot = 0;
     const T* px = static_cast<const T*>(a.lv.data);
     const double unit = ldexp(1.0, -a.lv.shift);
     for (int i = 0; i < P.n; ++i)
       dot += (double(px[size_t(r + sample_row(rc[i])) * a.lv.pitch + c + sample_col(rc[i])]) * unit - dc) * w[i];
     int64_t s1, s2;
-    window_moments(a.p1, a.p2, W + 1, a.pool.runs + 4 * P.run_off, P.n_runs, r, c, s1, s2);
+    window_moments(a.p1, a.p2, W + 1, pool.runs + 4 * P.run_off, P.n_runs, r, c, s1, s2);
     double sum, sq;
     exact_moments(s1, s2, P.n, D, a.lv.shift, sum, sq);
     const double cnt = double(P.n), var = sq - sum * sum / cnt;
@@ -249,11 +251,11 @@ __global__ void __launch_bounds__(256) k_peaks(const double* __restrict__ planes
 using namespace ncc;
 
 extern "C" int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1,
-                             const uint64_t* p2, const NccPatchPool* pool, const int32_t* patch_ids,
+                             const uint64_t* p2, const NccPatchPool* pools, const int32_t* patch_ids,
                              const int32_t* ranges, int n_angles, int max_bbox_h, int max_bbox_w, int form,
                              int exact_lattice, int absolute, double* planes, void* stream) {
   CoarseArgs a;
-  a.lv = *lv; a.dc_src = dc_src; a.C = C; a.p1 = p1; a.p2 = p2; a.pool = *pool; a.patch_ids = patch_ids;
+  a.lv = *lv; a.dc_src = dc_src; a.C = C; a.p1 = p1; a.p2 = p2; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.patch_ids = patch_ids;
   a.ranges = reinterpret_cast<const int4*>(ranges); a.planes = planes; a.exact_lattice = exact_lattice;
   a.absolute = absolute;
   hipStream_t st = static_cast<hipStream_t>(stream);

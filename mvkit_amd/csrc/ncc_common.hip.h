@@ -67,6 +67,15 @@ __device__ __forceinline__ int block_any(int flag, int* scratch) {
 
 __device__ __forceinline__ double clamp1(double x) { return x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x); }
 
+// Raster references are (pool << 24) | index with pool 0 the model's rasters
+// and pool 1 the per-find rasters; the pools are two separate values so that
+// selecting one never indexes a private array.
+struct Pools {
+  NccPatchPool p0, p1;
+  __device__ __forceinline__ const NccPatchPool& of(int32_t ref) const { return (ref >> 24) ? p1 : p0; }
+  __device__ __forceinline__ const NccPatch& patch(int32_t ref) const { return of(ref).patches[ref & 0xffffff]; }
+};
+
 __device__ __forceinline__ int sample_row(int32_t rc) { return rc >> 16; }
 __device__ __forceinline__ int sample_col(int32_t rc) { return int(int16_t(rc & 0xffff)); }
 

@@ -196,8 +196,37 @@ FindContext::FindContext() {
 FindContext::~FindContext() {
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
+    for (auto& t : timers_) { (void)hipEventDestroy(t.begin); (void)hipEventDestroy(t.end); }
     (void)hipStreamDestroy(stream_);
   }
+}
+
+void FindContext::time_begin(int stage) {
+  if (!profiling_) return;
+  if (timers_used_ == timers_.size()) {
+    Timer t;
+    check(hipEventCreate(&t.begin), "hipEventCreate");
+    check(hipEventCreate(&t.end), "hipEventCreate");
+    timers_.push_back(t);
+  }
+  timers_[timers_used_].stage = stage;
+  check(hipEventRecord(timers_[timers_used_].begin, stream_), "hipEventRecord");
+}
+void FindContext::time_end() {
+  if (!profiling_) return;
+  check(hipEventRecord(timers_[timers_used_].end, stream_), "hipEventRecord");
+  ++timers_used_;
+}
+void FindContext::time_collect(FindTrace* trace) {
+  if (profiling_ && trace) {
+    check(hipStreamSynchronize(stream_), "sync");
+    for (size_t i = 0; i < timers_used_; ++i) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, timers_[i].begin, timers_[i].end), "hipEventElapsedTime");
+      trace->kernel_us[timers_[i].stage] += 1000.0 * ms;
+    }
+  }
+  timers_used_ = 0;
 }
 
 std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
@@ -212,6 +241,9 @@ std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* i
 std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image, size_t W, size_t H, size_t pitch,
                                     const FindOptions& req, FindTrace* trace) {
   const double t0 = now_us();
+  profiling_ = req.profile && trace;
+  timers_used_ = 0;
+  if (trace) *trace = FindTrace{};
   // ---- options and levels (matcher.cpp:345-374)
   FindOptions o = req;
   if (!W || !H) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
@@ -319,6 +351,9 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
     ranges[4 * i + 3] = std::min(lv[top].w - p.cmax, int(std::floor((cmax_box + margin - crop_c - phase) / scale - p.ocol)) + 1);
     max_bh = std::max(max_bh, p.rmax - p.rmin);
     max_bw = std::max(max_bw, p.cmax - p.cmin);
+    if (trace)
+      trace->coarse_flops += 2.0 * double(p.count) * std::max(0, ranges[4 * i + 1] - ranges[4 * i]) *
+                             std::max(0, ranges[4 * i + 3] - ranges[4 * i + 2]);
   }
 
   // ---- workspace layout
@@ -369,22 +404,27 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   };
 
   // ---- search pyramid, top-level statistics, coarse search, peaks
+  time_begin(kStagePyramid);
   for (size_t l = 1; l < level_count; ++l) {
     const NccLevel src = level_desc(l - 1);
     if (ncchip_pyramid_level(src.data, src.kind, src.pitch, src.width, src.height, dev + lv[l].off, lv[l].kind,
                              lv[l].pitch, lv[l].w, lv[l].h, stream_))
       raise(kInternalError, "device", "pyramid launch failed");
   }
+  time_end();
   const NccLevel topd = level_desc(top);
   auto* p1 = reinterpret_cast<uint32_t*>(dev + off_p1);
   auto* p2 = reinterpret_cast<uint64_t*>(dev + off_p2);
+  time_begin(kStagePrefix);
   if (ncchip_row_prefix(topd.data, topd.kind, topd.pitch, Wt, Ht, crop, int(2 * top), p1, p2,
                         reinterpret_cast<uint32_t*>(dev + off_nz), nullptr, stream_))
     raise(kInternalError, "device", "prefix launch failed");
+  time_end();
   auto* planes = reinterpret_cast<double*>(dev + off_planes);
   const double C = std::ldexp(128.0, int(2 * top));
   const int exact_top = top <= 2;
   int rc = 0;
+  time_begin(kStageCoarse);
   if (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN)
     rc = ncchip_coarse_mfma(&topd, crop, C, p1, p2, &pools[0], reinterpret_cast<const int32_t*>(dev + off_refs[top]),
                             reinterpret_cast<const int32_t*>(dev + off_ranges), int(A), max_bh, max_bw, exact_top,
@@ -394,11 +434,14 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
                        reinterpret_cast<const int32_t*>(dev + off_ranges), int(A), max_bh, max_bw, o.form, exact_top,
                        model.params.absolute, planes, stream_);
   if (rc) raise(kInternalError, "device", "coarse launch failed");
+  time_end();
   auto* count = reinterpret_cast<int32_t*>(dev + off_count);
   check(hipMemsetAsync(count, 0, 4, stream_), "memset");
+  time_begin(kStagePeaks);
   if (ncchip_peaks(planes, int(A), Ht, Wt, o.min_score * .4, reinterpret_cast<NccPeak*>(dev + off_peaks), int(cap),
                    count, stream_))
     raise(kInternalError, "device", "peak launch failed");
+  time_end();
   auto* rb = static_cast<uint8_t*>(readback_.ptr);
   check(hipMemcpyAsync(rb, count, 4, hipMemcpyDeviceToHost, stream_), "readback");
   check(hipStreamSynchronize(stream_), "coarse");
@@ -430,6 +473,7 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   }
   const double t1 = now_us();
   if (cand.empty()) {
+    time_collect(trace);
     if (trace) { trace->t_coarse_us = t1 - t0; trace->t_total_us = t1 - t0; }
     return {};
   }
@@ -473,10 +517,12 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
     int32_t storage[4] = {0, 0, tl.height, tl.width};
     if (l - 1 == 0) std::copy(storage0, storage0 + 4, storage);
     upload_cand(cand);
+    time_begin(kStageTrack);
     if (ncchip_track(&tl, crop, pools, reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                      dcand, dout, int(cand.size()), tol, double(crop_r), double(crop_c), step, offset, storage,
                      l - 1 <= 2, model.params.absolute, stream_))
       raise(kInternalError, "device", "tracking launch failed");
+    time_end();
     check(hipMemcpyAsync(rb, dout, cand.size() * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
     check(hipStreamSynchronize(stream_), "tracking");
     const auto* res = reinterpret_cast<const NccMatch*>(rb);
@@ -508,8 +554,10 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   std::vector<Pose> full(cand.size());
   if (!cand.empty()) {
     upload_cand(cand);
+    time_begin(kStageFull);
     if (ncchip_full_score(&rp, pools, dcand, int(cand.size()), dout, stream_))
       raise(kInternalError, "device", "full score launch failed");
+    time_end();
     check(hipMemcpyAsync(rb, dout, cand.size() * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
     check(hipStreamSynchronize(stream_), "full score");
     const auto* f = reinterpret_cast<const NccMatch*>(rb);
@@ -551,8 +599,10 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
     for (size_t i = pos; i < n && batch.size() < batch_size; ++i)
       if (std::isfinite(full[i].score) && verdict[i] < 0 && !suppressed(full[i])) { batch.push_back(i); poses.push_back(full[i]); }
     upload_cand(poses);
+    time_begin(kStageRefine);
     if (ncchip_refine(&rp, pools, dcand, int(poses.size()), drefined, dverdict, stream_))
       raise(kInternalError, "device", "refine launch failed");
+    time_end();
     const size_t m = poses.size();
     check(hipMemcpyAsync(rb, drefined, m * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
     check(hipMemcpyAsync(rb + m * sizeof(NccMatch), dverdict, m * 4, hipMemcpyDeviceToHost, stream_), "readback");
@@ -572,6 +622,7 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
     trace->verdict = verdict;
   }
   std::sort(result.begin(), result.end(), score_order);
+  time_collect(trace);
   if (trace) {
     trace->t_coarse_us = t1 - t0; trace->t_track_us = t2 - t1; trace->t_refine_us = t3 - t2;
     trace->t_total_us = now_us() - t0;

@@ -65,7 +65,12 @@ struct FindOptions {
   int highest = 0, lowest = 1;
   int form = NCCHIP_FORM_FP32;     // coarse correlation form
   double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN)
+  bool profile = false;            // time each kernel with HIP events on the context's stream
 };
+
+// Kernel stages timed in profiling mode.
+enum KernelStage { kStagePyramid = 0, kStagePrefix, kStageCoarse, kStagePeaks, kStageTrack, kStageFull, kStageRefine,
+                   kStageCount };
 
 // Intermediate results of one find, for stage-by-stage checks.
 struct FindTrace {
@@ -77,6 +82,8 @@ struct FindTrace {
   int top = 0, finest = 0, angle_stride = 1;
   int storage[4] = {0, 0, 0, 0};
   double t_coarse_us = 0, t_track_us = 0, t_refine_us = 0, t_total_us = 0;
+  double kernel_us[kStageCount] = {};  // device time per stage (profiling mode)
+  double coarse_flops = 0;             // 2 * sum over evaluated coarse windows of their sample counts
 };
 
 // One find pipeline: owns a stream and grow-only device/pinned workspaces.
@@ -99,7 +106,13 @@ class FindContext {
   int device() const { return device_; }
 
  private:
-  struct Arena;
+  struct Timer { hipEvent_t begin = nullptr, end = nullptr; int stage = 0; };
+  void time_begin(int stage);
+  void time_end();
+  void time_collect(FindTrace* trace);
+  bool profiling_ = false;
+  std::vector<Timer> timers_;
+  size_t timers_used_ = 0;
   hipStream_t stream_ = nullptr;
   int device_ = 0;
   DeviceBuffer work_;

@@ -18,15 +18,6 @@ struct PatchRef {
   const double* wrun_w;
 };
 
-// Raster references are (pool << 24) | index with pool 0 the model and pool 1
-// the per-find rasters; the two pools are separate values so that selecting
-// one never indexes a private array.
-struct Pools {
-  NccPatchPool p0, p1;
-  __device__ __forceinline__ const NccPatchPool& of(int32_t ref) const { return (ref >> 24) ? p1 : p0; }
-  __device__ __forceinline__ const NccPatch& patch(int32_t ref) const { return of(ref).patches[ref & 0xffffff]; }
-};
-
 __device__ __forceinline__ PatchRef patch_ref(const Pools& pools, int32_t ref) {
   const NccPatchPool& pool = pools.of(ref);
   PatchRef out;
@@ -71,57 +62,11 @@ struct LevelReader {
   __device__ __forceinline__ uint32_t at(int r, int c) const { return data[size_t(r) * pitch + c]; }
 };
 
-// One window of the reference's 8-lane prefix kernel (kernels_avx2.cpp:
-// 209-238), evaluated by one wave with lanes striding over the samples:
-// exact integer moments, and the dot product from per-sample products (or by
-// weight runs).  The dot product is summed in a different order than the
-// reference's single accumulator, so scores agree to ~1e-15.  Every lane of
-// the wave returns the score.
-template <class Reader>
-__device__ __forceinline__ double track_window(const TrackArgs& a, const PatchRef& p, int row, int col,
-                                               const Reader& rd) {
-  const NccPatch& P = p.P;
-  const int H = a.lv.height, W = a.lv.width;
-  if (row + P.rmin < 0 || col + P.cmin < 0 || row + P.rmax >= H || col + P.cmax >= W) return NCC_MISSING;
-  if (row + P.rmin < a.sr || col + P.cmin < a.sc || row + P.rmax >= a.sr + a.sh || col + P.cmax >= a.sc + a.sw)
-    return NCC_MISSING;
-  const int lane = threadIdx.x & 63;
-  const double unit = ldexp(1.0, -a.lv.shift);
-  int64_t s1 = 0, s2 = 0;
-  double dot = 0;
-  constexpr int U = 8;  // samples per lane whose loads are issued together
-  for (int base = lane; base < P.n; base += 64 * U) {
-    int32_t rcv[U];
-    double wv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = base + 64 * u;
-      rcv[u] = i < P.n ? p.rc[i] : 0;
-      wv[u] = i < P.n && !P.use_wruns ? p.w[i] : 0.0;
-    }
-    uint32_t sv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      sv[u] = base + 64 * u < P.n ? rd.at(row + sample_row(rcv[u]), col + sample_col(rcv[u])) : 0u;
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (base + 64 * u < P.n) {
-        s1 += sv[u];
-        s2 += int64_t(sv[u]) * sv[u];
-        if (!P.use_wruns) dot += (double(sv[u]) * unit - a.dc) * wv[u];
-      }
-  }
-  if (P.use_wruns) {
-    for (int k = lane; k < P.n_wruns; k += 64) {
-      const int4 run = reinterpret_cast<const int4*>(p.wruns)[k];
-      int64_t rs = 0;
-      for (int c = run.y; c <= run.z; ++c) rs += rd.at(row + run.x, col + c);
-      dot += double(rs - int64_t(run.z - run.y + 1) * a.D) * unit * p.wrun_w[k];
-    }
-  }
-  s1 = wave_sum_i64(s1);
-  s2 = wave_sum_i64(s2);
-  dot = wave_sum(dot);
+// Score of one window from its dot product and exact integer moments, with
+// the reference's prefix-kernel normalisation and low-variance rule
+// (kernels_avx2.cpp:209-238, 128-129).
+__device__ __forceinline__ double window_score(const TrackArgs& a, const NccPatch& P, double dot, int64_t s1,
+                                               int64_t s2) {
   double sum, sq;
   exact_moments(s1, s2, P.n, a.D, a.lv.shift, sum, sq);
   const double cnt = double(P.n);
@@ -137,6 +82,81 @@ __device__ __forceinline__ double track_window(const TrackArgs& a, const PatchRe
   }
   const double sc = clamp1(dot / sqrt(var));
   return a.absolute ? fabs(sc) : sc;
+}
+
+__device__ __forceinline__ bool window_inside(const TrackArgs& a, const NccPatch& P, int row, int col) {
+  return row + P.rmin >= 0 && col + P.cmin >= 0 && row + P.rmax < a.lv.height && col + P.cmax < a.lv.width &&
+         row + P.rmin >= a.sr && col + P.cmin >= a.sc && row + P.rmax < a.sr + a.sh && col + P.cmax < a.sc + a.sw;
+}
+
+// The five windows (row, col-2 .. col+2) of one template row offset, evaluated
+// by one wave with lanes striding over the samples, so each sample's position
+// and weight are loaded once for the five windows.  Exact integer moments;
+// the dot products are summed in a different order than the reference's
+// single accumulator, so scores agree to ~1e-15.  Every lane gets the scores.
+template <class Reader>
+__device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p, int row, int col, const Reader& rd,
+                                          double (&score)[5]) {
+  const NccPatch& P = p.P;
+  bool in[5];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) { in[k] = window_inside(a, P, row, col - 2 + k); any |= in[k]; }
+  if (!any) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) score[k] = NCC_MISSING;
+    return;
+  }
+  // windows partly outside read clamped columns and are discarded below
+  const int lane = threadIdx.x & 63;
+  const double unit = ldexp(1.0, -a.lv.shift);
+  int64_t s1[5] = {0, 0, 0, 0, 0}, s2[5] = {0, 0, 0, 0, 0};
+  double dot[5] = {0, 0, 0, 0, 0};
+  int kmin = 0, kmax = 4;
+  while (!in[kmin]) ++kmin;
+  while (!in[kmax]) --kmax;
+  constexpr int U = 8;
+  for (int base = lane; base < P.n; base += 64 * U) {
+    int32_t rcv[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + 64 * u;
+      rcv[u] = i < P.n ? p.rc[i] : 0;
+      wv[u] = i < P.n && !P.use_wruns ? p.w[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (base + 64 * u >= P.n) break;
+      const int r = row + sample_row(rcv[u]), c = col + sample_col(rcv[u]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const uint32_t v = rd.at(r, c - 2 + min(max(k, kmin), kmax));
+        s1[k] += v;
+        s2[k] += int64_t(v) * v;
+        if (!P.use_wruns) dot[k] += (double(v) * unit - a.dc) * wv[u];
+      }
+    }
+  }
+  if (P.use_wruns) {
+    for (int j = lane; j < P.n_wruns; j += 64) {
+      const int4 run = reinterpret_cast<const int4*>(p.wruns)[j];
+      const double w = p.wrun_w[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int cc = col - 2 + min(max(k, kmin), kmax);
+        int64_t rs = 0;
+        for (int c = run.y; c <= run.z; ++c) rs += rd.at(row + run.x, cc + c);
+        dot[k] += double(rs - int64_t(run.z - run.y + 1) * a.D) * unit * w;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int64_t t1 = wave_sum_i64(s1[k]), t2 = wave_sum_i64(s2[k]);
+    const double d = wave_sum(dot[k]);
+    score[k] = in[k] ? window_score(a, P, d, t1, t2) : NCC_MISSING;
+  }
 }
 
 constexpr int kMaxEligible = 1024;
@@ -227,16 +247,17 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   double best = NCC_MISSING;
   int best_i = 0x7fffffff;
   const int wave = threadIdx.x >> 6;
-  for (int wi = wave; wi < n_e * 25; wi += kTrackThreads / 64) {
-    const PatchRef p = patch_ref(a.pools, a.refs[elig[wi / 25]]);
+  for (int task = wave; task < n_e * 5; task += kTrackThreads / 64) {
+    const PatchRef p = patch_ref(a.pools, a.refs[elig[task / 5]]);
     int row, col;
     anchor(a, cd, p.P, row, col);
-    row += (wi % 25) / 5 - 2;
-    col += wi % 5 - 2;
-    double s;
-    if (use_tile) s = track_window(a, p, row, col, TileReader{tile, r0, c0, tw});
-    else s = track_window(a, p, row, col, LevelReader<T>{lvp, a.lv.pitch});
-    if (s > best) { best = s; best_i = wi; }
+    const int dr = task % 5 - 2;
+    double sc[5];
+    if (use_tile) track_row(a, p, row + dr, col, TileReader{tile, r0, c0, tw}, sc);
+    else track_row(a, p, row + dr, col, LevelReader<T>{lvp, a.lv.pitch}, sc);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (sc[k] > best) { best = sc[k]; best_i = (task / 5) * 25 + (dr + 2) * 5 + k; }
   }
   red_s[threadIdx.x] = best;
   red_i[threadIdx.x] = best_i;
@@ -414,7 +435,7 @@ __device__ __forceinline__ bool bilinear_pass(const RefineArgs& a, const PatchRe
 #pragma unroll
   for (int k = 0; k < 18; ++k) v[k] = 0;
   int bad = 0;
-  constexpr int U = 4;  // samples per thread whose template loads are issued together
+  constexpr int U = 8;  // samples per thread whose template loads are issued together
   for (int base = threadIdx.x; base < P.n; base += kRefineThreads * U) {
     int32_t rcv[U];
     double wv[U];

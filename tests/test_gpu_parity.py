@@ -1,0 +1,224 @@
+"""GPU parity: the HIP find path against the CPU oracle.
+
+Integer passes are bit-exact; find results must have the oracle's instance
+count and order, row/col within 0.02 px, angle within 0.01 degree and score
+within 1e-4.  Everything goes through the library's C ABI (mv_* and the
+mvamd_* / ncchip_* extensions)."""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ANGLE_TOL, POS_TOL, SCORE_TOL
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    torch.cuda.set_device(0)
+    return torch.device("cuda:0")
+
+
+def check_matches(got, want, label=""):
+    assert got.shape == want.shape, f"{label}: count {len(got)} != {len(want)}\n{got}\n{want}"
+    if len(want) == 0:
+        return
+    assert np.abs(got[:, 0] - want[:, 0]).max() <= POS_TOL, label
+    assert np.abs(got[:, 1] - want[:, 1]).max() <= POS_TOL, label
+    assert np.abs(got[:, 2] - want[:, 2]).max() <= ANGLE_TOL, label
+    assert np.abs(got[:, 3] - want[:, 3]).max() <= SCORE_TOL, label
+
+
+# ----------------------------------------------------------- integer passes
+def block_sums(img, levels):
+    out, cur = [], img.astype(np.int64)
+    for _ in range(1, levels):
+        h, w = cur.shape
+        nh, nw = max(1, h // 2), max(1, w // 2)
+        r0 = 2 * np.arange(nh); r1 = np.minimum(r0 + 1, h - 1)
+        c0 = 2 * np.arange(nw); c1 = np.minimum(c0 + 1, w - 1)
+        cur = cur[r0][:, c0] + cur[r0][:, c1] + cur[r1][:, c0] + cur[r1][:, c1]
+        out.append(cur)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(2048, 2048), (480, 640), (37, 53), (1, 9), (3, 1)])
+def test_pyramid_bit_exact(capi, oracle, shape):
+    say(f"pyramid {shape}")
+    rng = np.random.default_rng(11)
+    img = rng.integers(0, 256, size=shape, dtype=np.uint8)
+    levels = 6
+    lib = capi.lib()
+    lib.ncchip_pyramid_level.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_void_p]
+    src = torch.from_numpy(img).cuda()
+    kinds = {0: (0, torch.uint8), 1: (1, torch.int16), 2: (2, torch.int32)}
+    prev, prev_kind = src, 0
+    want = block_sums(img, levels)
+    ref = oracle.pyramid(img, levels)
+    dc = float(img[0, 0])
+    for l in range(1, levels):
+        h, w = want[l - 1].shape
+        kind = 1 if l <= 4 else 2
+        dst = torch.zeros((h, w), dtype=kinds[kind][1], device="cuda")
+        st = lib.ncchip_pyramid_level(prev.data_ptr(), prev_kind, prev.shape[1], prev.shape[1], prev.shape[0],
+                                      dst.data_ptr(), kind, w, w, h, None)
+        assert st == 0
+        torch.cuda.synchronize()
+        got = dst.cpu().numpy().astype(np.int64)
+        if kind == 1:
+            got &= 0xFFFF
+        assert np.array_equal(got, want[l - 1]), f"level {l}"
+        # the reference's FP64 level values are exactly S * 4^-l - dc
+        assert np.array_equal(got * 4.0 ** -l - dc, ref[l - 1]), f"level {l} vs oracle"
+        prev, prev_kind = dst, kind
+
+
+def test_row_prefix_and_support_bit_exact(capi, oracle):
+    say("row prefix")
+    rng = np.random.default_rng(12)
+    img = rng.integers(0, 256, size=(64, 300), dtype=np.uint8)
+    img[10:20, 30:200] = img[0, 0]  # flat region: zero support
+    lib = capi.lib()
+    lib.ncchip_row_prefix.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    h, w = img.shape
+    src = torch.from_numpy(img).cuda()
+    p1 = torch.zeros((h, w + 1), dtype=torch.int32, device="cuda")
+    p2 = torch.zeros((h, w + 1), dtype=torch.int64, device="cuda")
+    nz = torch.zeros((h, w + 1), dtype=torch.int32, device="cuda")
+    sup = torch.zeros((h + 1, w + 1), dtype=torch.int32, device="cuda")
+    assert lib.ncchip_row_prefix(src.data_ptr(), 0, w, w, h, src.data_ptr(), 0, p1.data_ptr(), p2.data_ptr(),
+                                 nz.data_ptr(), sup.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    s = img.astype(np.int64)
+    assert np.array_equal(p1.cpu().numpy()[:, 1:].astype(np.int64), np.cumsum(s, axis=1))
+    assert np.array_equal(p2.cpu().numpy()[:, 1:], np.cumsum(s * s, axis=1))
+    # reference values: prefix of q = S - dc, exact in FP64 (matcher.cpp:487-495)
+    ps, pq = oracle.row_prefix(s - float(img[0, 0]))
+    P1 = p1.cpu().numpy().astype(np.int64)
+    assert np.array_equal(P1 - np.arange(w + 1) * int(img[0, 0]), ps)
+    q = (s != img[0, 0]).astype(np.int64)
+    integral = np.zeros((h + 1, w + 1), np.int64)
+    integral[1:, 1:] = q.cumsum(0).cumsum(1)
+    assert np.array_equal(sup.cpu().numpy().astype(np.int64), integral)
+
+
+# ------------------------------------------------------------------- find
+def run_find(capi, oracle, scene, levels, start, extent, min_score, nm, overlap=0.5, subpixel=True, highest=0,
+             lowest=1, absolute=False, form=0, label=""):
+    say(f"find {label}: {scene.image.shape} levels={levels} extent={extent:.3f} nm={nm} form={form}")
+    metric = "ignore_global_polarity" if absolute else "use_polarity"
+    m = capi.NccModel(scene.template, levels if levels else "auto", start, extent, "auto", metric)
+    o = oracle.Model(scene.template, levels=levels, start=start, extent=extent, absolute=absolute)
+    want, tr = o.find(scene.image, start, extent, min_score, nm, overlap, subpixel, highest, lowest, trace=True)
+    opts = capi.NccModel.options(start, extent, min_score, nm, overlap, subpixel, highest, lowest, form)
+    g = m.find_trace(scene.image, opts)
+    return m, want, tr, g
+
+
+CASES = [
+    ("plain640", "scene_plain", 3, 0.0, 0.0, 0.8, 3),
+    ("small_rot", "scene_small", 0, -math.radians(20), math.radians(40), 0.7, 2),
+    ("small_rot_all", "scene_small", 0, -math.radians(20), math.radians(40), 0.5, 0),
+    ("rot2048", "scene_rot", 4, -math.pi / 6, math.pi / 3, 0.7, 5),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_find_matches_oracle(capi, oracle, request, case):
+    label, fixture, levels, start, extent, ms, nm = case
+    scene = request.getfixturevalue(fixture)
+    m, want, tr, g = run_find(capi, oracle, scene, levels, start, extent, ms, nm, label=label)
+    check_matches(g["results"], want, label)
+    # stage by stage: identical coarse peaks and full-scored candidate list
+    assert len(g["coarse"]) == len(tr["coarse"])
+    assert np.array_equal(g["coarse"][:, :3], tr["coarse"][:, :3])
+    assert np.abs(g["coarse"][:, 3] - tr["coarse"][:, 3]).max() <= 1e-5
+    assert len(g["prepared"]) == len(tr["prepared"])
+    fin = np.isfinite(tr["prepared"][:, 3])
+    assert np.array_equal(np.isfinite(g["prepared"][:, 3]), fin)
+    assert np.array_equal(g["prepared"][fin, :3], tr["prepared"][fin, :3])
+    assert np.abs(g["prepared"][fin, 3] - tr["prepared"][fin, 3]).max() <= 1e-12
+    # every candidate the oracle refined was refined to the same verdict and pose
+    ro, rg = tr["refined"], g["refined"]
+    done = ro[:, 4] >= 0
+    assert np.all(rg[done, 4] >= 0)
+    assert np.array_equal(ro[done, 4], rg[done, 4])
+    assert np.abs(ro[done, :2] - rg[done, :2]).max() <= POS_TOL
+    # the C ABI (HALCON output convention) returns the same matches
+    shifted = capi.halcon_shift(want)
+    c = m.find(scene.image, start, extent, ms, nm, 0.5, "true", 0)
+    check_matches(c, shifted, label + " via C ABI")
+
+
+def test_truth_recovered_rot2048(capi, scene_rot):
+    """All five pasted instances of the 2048x2048 workload are found."""
+    m = capi.NccModel(scene_rot.template, 4, -math.pi / 6, math.pi / 3)
+    res = m.find_trace(scene_rot.image, capi.NccModel.options(-math.pi / 6, math.pi / 3, 0.7, 5, 0.5))["results"]
+    assert len(res) == 5
+    for r, c, a, s in res:
+        t = min(scene_rot.truth, key=lambda t: math.hypot(r - t[0], c - t[1]))
+        assert math.hypot(r - t[0], c - t[1]) < 0.2 and abs(a - t[2]) < 2e-3 and s > 0.9
+
+
+@pytest.mark.parametrize("variant", ["no_subpixel", "lowest2", "absolute", "highest2", "narrow_query"])
+def test_find_variants(capi, oracle, scene_small, variant):
+    start, extent = -math.radians(20), math.radians(40)
+    kw = dict(levels=0, start=start, extent=extent, min_score=0.6, nm=0)
+    qstart, qextent = start, extent
+    opts = {}
+    if variant == "no_subpixel":
+        opts["subpixel"] = False
+    elif variant == "lowest2":
+        opts["lowest"] = 2
+    elif variant == "absolute":
+        opts["absolute"] = True
+    elif variant == "highest2":
+        opts["highest"] = 2
+    elif variant == "narrow_query":
+        qstart, qextent = -0.1234, 0.3  # endpoints off the trained grid
+    metric = "ignore_global_polarity" if opts.get("absolute") else "use_polarity"
+    say(f"variant {variant}")
+    m = capi.NccModel(scene_small.template, "auto", start, extent, "auto", metric)
+    o = oracle.Model(scene_small.template, levels=0, start=start, extent=extent, absolute=opts.get("absolute", False))
+    want = o.find(scene_small.image, qstart, qextent, 0.6, 0, 0.5, opts.get("subpixel", True), opts.get("highest", 0),
+                  opts.get("lowest", 1))
+    got = m.find_trace(scene_small.image, capi.NccModel.options(qstart, qextent, 0.6, 0, 0.5, opts.get("subpixel", True),
+                                                                opts.get("highest", 0), opts.get("lowest", 1)))
+    check_matches(got["results"], want, variant)
+
+
+def test_no_candidates_and_out_of_range_query(capi, scene_small):
+    say("no candidates")
+    m = capi.NccModel(scene_small.template, "auto", 0.0, 0.3)
+    flat = np.full((120, 160), 77, np.uint8)
+    assert m.find(flat, 0.0, 0.3, 0.5, 0).shape == (0, 4)
+    # a query range disjoint from the trained range finds nothing
+    assert m.find(scene_small.image, 1.0, 0.2, 0.5, 0).shape == (0, 4)
+
+
+def test_device_resident_find_matches_host(capi, scene_rot):
+    say("device-resident find")
+    m = capi.NccModel(scene_rot.template, 4, -math.pi / 6, math.pi / 3)
+    opts = capi.NccModel.options(-math.pi / 6, math.pi / 3, 0.7, 5, 0.5)
+    img = torch.from_numpy(scene_rot.image).cuda()
+    a, _ = m.find_device(img.data_ptr(), 2048, 2048, 2048, opts)
+    b = m.find_trace(scene_rot.image, opts)["results"]
+    assert np.array_equal(a, b)
+    # a pitched view (every other column block) gives the same result as a packed copy
+    wide = torch.zeros((2048, 2304), dtype=torch.uint8, device="cuda")
+    wide[:, :2048] = img
+    c, _ = m.find_device(wide.data_ptr(), 2048, 2048, 2304, opts)
+    assert np.array_equal(a, c)
