@@ -55,6 +55,17 @@ MV_API MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* d_im
                                             int64_t pitch, const MvAmdFindOptions* options, double* results,
                                             size_t cap, size_t* count, MvAmdFindStats* stats);
 
+/* Finds the model in n u8 images given as HIP device pointers, all with the
+ * same options, stage by stage over the whole batch (one launch per stage for
+ * the candidates of every frame).  Frame k's matches go to
+ * results + 4*cap*k (row, col, angle, score) and their number to counts[k];
+ * each frame's result is the one mvamd_find_ncc_model_device returns for it.
+ * stats (optional) receives the batch totals of frame 0's trace. */
+MV_API MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* const* d_images, const int64_t* widths,
+                                           const int64_t* heights, const int64_t* pitches, size_t n,
+                                           const MvAmdFindOptions* options, double* results, size_t cap,
+                                           size_t* counts, MvAmdFindStats* stats);
+
 /* Find on a host u8 image returning the stages: coarse peaks (cap_coarse x 4),
  * full-scored candidates in score order (cap_prepared x 4) and, per such
  * candidate, the refined pose and verdict (cap_prepared x 5). */

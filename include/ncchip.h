@@ -67,18 +67,19 @@ int ncchip_peaks(const double* planes, int n_angles, int h, int w, double thresh
 
 /* Coarse-to-fine tracking of every candidate at one level: the first strict
  * maximum over (query patch within `tol` of the candidate's angle, 5x5
- * anchors).  `refs` lists the level's query patches as (pool << 24 | index).
- * `storage` is the level's valid rectangle {row, col, height, width}.
+ * anchors).  Candidates may come from several frames: `frames` (device) holds
+ * one NccFrame per frame and `cand_frame` (device) each candidate's frame;
+ * `level` is the tracked level and `level_kind` its pixel kind.  `refs` lists
+ * the level's query patches as (pool << 24 | index).
  * Replaces matcher.cpp:670-756 (kernels_avx2.cpp:118-238). */
-int ncchip_track(const NccLevel* lv, const uint8_t* dc_src, const NccPatchPool* pools, const int32_t* refs, int n_refs,
-                 const NccMatch* cand, NccMatch* out, int n_cand, double tol, double crop_r, double crop_c,
-                 double step, double offset, const int32_t* storage, int exact_lattice, int absolute, void* stream);
+int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, int level, int level_kind,
+                 const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* cand, NccMatch* out,
+                 int n_cand, double tol, double step, double offset, int exact_lattice, int absolute, void* stream);
 
 typedef struct NcchipRefineParams {
-  NccLevel level;          /* finest search level */
-  const uint8_t* dc_src;   /* device pointer to the reference pixel (first crop pixel) */
-  int32_t storage[4];      /* {row, col, height, width} at the finest level */
-  double crop_r, crop_c, scale, phase;
+  const NccFrame* frames;  /* per-frame descriptors (device) */
+  int32_t level_kind;      /* pixel kind of the finest level */
+  double scale, phase;     /* 2^finest and (scale - 1) / 2 */
   int32_t finest, top;
   int32_t orig_ref;        /* unrotated patch at the finest level */
   double level_orow, level_ocol;
@@ -86,21 +87,20 @@ typedef struct NcchipRefineParams {
   int32_t n_refs0;
   double step, angle_start, angle_end, min_score;
   int32_t subpixel, rotate, absolute;
-  int32_t image_h, image_w;
 } NcchipRefineParams;
 
-/* Continuous bilinear score of every candidate at the finest level, one
- * workgroup per candidate.  Replaces matcher.cpp:813-822
+/* Continuous bilinear score of every candidate at the finest level of its
+ * frame, one workgroup per candidate.  Replaces matcher.cpp:813-822
  * (kernels_avx2.cpp:327-477, 517-541). */
-int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
-                      NccMatch* out_full, void* stream);
+int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
+                      const int32_t* cand_frame, int n_cand, NccMatch* out_full, void* stream);
 
 /* refine_candidate of every given full-scored candidate (Gauss-Newton fit
  * with backtracking, and the level-0 recovery search), one workgroup per
  * candidate; outputs the refined pose and the verdict (score >= MinScore
  * inside the domain).  Replaces matcher.cpp:853-1061. */
-int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
-                  NccMatch* out_refined, int32_t* verdict, void* stream);
+int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
+                  const int32_t* cand_frame, int n_cand, NccMatch* out_refined, int32_t* verdict, void* stream);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ PUBLIC_SYMBOLS = [
     "mv_length_tuple", "mv_get_type", "mv_get_i", "mv_get_d", "mv_get_s", "mv_get_h", "mv_resize_tuple", "mv_set_i",
     "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
     "mv_clear_handle", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model",
-    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
+    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_coarse", "ncchip_coarse_mfma",
@@ -110,6 +110,8 @@ def lib() -> C.CDLL:
         L.mv_find_ncc_model.argtypes = [P] * 9 + [C.POINTER(P)] * 4
         L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
                                                   C.POINTER(FindStats)]
+        L.mvamd_find_ncc_model_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), P, S, P,
+                                                 C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_trace.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S), P, S,
                                                  C.POINTER(S), P, P, S, C.POINTER(S), C.POINTER(FindStats)]
         L.mvamd_model_info.argtypes = [P, P]
@@ -278,6 +280,24 @@ class NccModel:
         _check(lib().mvamd_find_ncc_model_device(self.tuple.h, C.c_void_p(d_image), width, height, pitch,
                                                  C.byref(opts), _ptr(res), cap, C.byref(n), C.byref(st)))
         return res[:n.value].copy(), st
+
+    def find_batch(self, d_images: list, width, height, pitch, opts: FindOptions, cap: int = 256):
+        """Find in several device-resident u8 images in one batch; width,
+        height and pitch are one value for all frames or one per frame."""
+        n = len(d_images)
+
+        def per_frame(v):
+            return list(v) if isinstance(v, (list, tuple)) else [v] * n
+        ptrs = (C.c_void_p * n)(*d_images)
+        ws = (C.c_int64 * n)(*per_frame(width))
+        hs = (C.c_int64 * n)(*per_frame(height))
+        ps = (C.c_int64 * n)(*per_frame(pitch))
+        res = np.zeros((n, cap, 4))
+        counts = np.zeros(n, np.uint64)
+        st = FindStats()
+        _check(lib().mvamd_find_ncc_model_batch(self.tuple.h, ptrs, ws, hs, ps, n, C.byref(opts), _ptr(res), cap,
+                                                _ptr(counts), C.byref(st)))
+        return [res[k, :int(counts[k])].copy() for k in range(n)], st
 
     def find_trace(self, image: np.ndarray, opts: FindOptions, cap: int = 20000):
         img = np.ascontiguousarray(image, dtype=np.uint8)

@@ -56,6 +56,18 @@ typedef struct NccPatchPool {
 // A pose in the reference's internal (centroid) convention (model.hpp:26).
 typedef struct NccMatch { double row, col, angle, score; } NccMatch;
 
+// Everything a per-candidate kernel needs to know about the candidate's
+// frame: its search pyramid, reference pixel, level-0 storage rectangle,
+// crop origin and search domain.
+#define NCC_MAX_LEVELS 16
+typedef struct NccFrame {
+  NccLevel levels[NCC_MAX_LEVELS];
+  const uint8_t* dc_src;     // reference pixel dc (first crop pixel), device memory
+  int32_t storage0[4];       // level-0 valid rectangle {row, col, height, width}
+  double crop_r, crop_c;
+  int32_t image_h, image_w;  // search domain (full image)
+} NccFrame;
+
 // A coarse peak: angle index into the coarse patch list, anchor and score.
 typedef struct NccPeak { int32_t angle, row, col, pad; double score; } NccPeak;
 

@@ -591,6 +591,42 @@ extern "C" MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* 
   });
 }
 
+extern "C" MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* const* d_images, const int64_t* widths,
+                                               const int64_t* heights, const int64_t* pitches, size_t n,
+                                               const MvAmdFindOptions* options, double* results, size_t cap,
+                                               size_t* counts, MvAmdFindStats* stats) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    required(d_images, op, "Images");
+    required(widths, op, "widths");
+    required(heights, op, "heights");
+    required(pitches, op, "pitches");
+    required(counts, op, "counts");
+    std::vector<mvamd::FrameInput> frames(n);
+    for (size_t k = 0; k < n; ++k) {
+      if (!d_images[k] || widths[k] <= 0 || heights[k] <= 0 || pitches[k] < widths[k])
+        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Images", "invalid image geometry", int64_t(k));
+      frames[k] = {d_images[k], size_t(widths[k]), size_t(heights[k]), size_t(pitches[k])};
+    }
+    const auto m = model_of(ModelID, op);
+    const mvamd::FindOptions q = from_ext(options);
+    std::vector<mvamd::FindTrace> traces(stats ? n : 0);
+    const auto found = algorithm(op, [&] {
+      return context().find_batch(*m, frames.data(), n, q, stats ? traces.data() : nullptr);
+    });
+    for (size_t k = 0; k < n; ++k) {
+      if (results) put_poses(found[k], results + 4 * cap * k, cap);
+      counts[k] = found[k].size();
+    }
+    if (stats && n) {
+      fill_stats(traces[0], stats);
+      double flops = 0;
+      for (const auto& t : traces) flops += t.coarse_flops;
+      stats->coarse_flops = flops;
+    }
+  });
+}
+
 extern "C" MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* image, int64_t width, int64_t height,
                                                int64_t pitch, const MvAmdFindOptions* options, double* results,
                                                size_t cap, size_t* count, double* coarse, size_t cap_coarse,

@@ -240,13 +240,42 @@ std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* i
 
 std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image, size_t W, size_t H, size_t pitch,
                                     const FindOptions& req, FindTrace* trace) {
+  FrameInput in{image, W, H, pitch};
+  auto out = find_batch(model, &in, 1, req, trace);
+  return std::move(out.front());
+}
+
+namespace {
+// Host state of one frame of a batch.
+struct Job {
+  FrameInput in;
+  FindTrace* trace = nullptr;
+  int64_t crop_r = 0, crop_c = 0, crop_b = 0, crop_e = 0;
+  struct Lv { int w, h, kind, pitch; size_t off, bytes; };
+  std::vector<Lv> lv;
+  std::vector<int32_t> ranges;
+  size_t off_p1 = 0, off_p2 = 0, off_nz = 0, off_planes = 0, off_peaks = 0, off_ranges = 0;
+  std::vector<Pose> cand, full, refined, results;
+  std::vector<int> verdict;
+  int32_t storage0[4] = {0, 0, 0, 0};
+  size_t pos = 0, batch_size = 0;
+  bool done = false;
+  double coarse_flops = 0;
+};
+}  // namespace
+
+std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, const FrameInput* inputs, size_t n_frames,
+                                                       const FindOptions& req, FindTrace* traces) {
   const double t0 = now_us();
-  profiling_ = req.profile && trace;
+  profiling_ = req.profile && traces;
   timers_used_ = 0;
-  if (trace) *trace = FindTrace{};
-  // ---- options and levels (matcher.cpp:345-374)
+  for (size_t f = 0; f < n_frames && traces; ++f) traces[f] = FindTrace{};
+  std::vector<std::vector<Pose>> results(n_frames);
+  if (!n_frames) return results;
+  // ---- options and levels, shared by the batch (matcher.cpp:345-374)
   FindOptions o = req;
-  if (!W || !H) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
+  for (size_t f = 0; f < n_frames; ++f)
+    if (!inputs[f].width || !inputs[f].height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
   if (model.levels.empty()) raise(kInvalidState, "model", "model is empty");
   if (o.highest < 0 || o.lowest < 1) raise(kInvalidArgument, "NumLevels", "invalid pyramid levels");
   const size_t level_count = o.highest ? std::min(model.levels.size(), size_t(o.highest)) : model.levels.size();
@@ -266,38 +295,17 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   if (!std::isfinite(angle_end)) raise(kInvalidArgument, "angle_extent", "query angle range must be finite");
   const double cs = std::max(o.start, model.params.start);
   const double ce = std::min(angle_end, model.params.start + model.params.extent);
-  if (cs > ce + 1e-12) return {};
+  if (cs > ce + 1e-12) return results;
   o.start = cs;
   o.extent = std::max(0., ce - cs);
-  if (W > size_t(std::numeric_limits<int>::max()) || H > size_t(std::numeric_limits<int>::max()) || H > 32000000 / W)
-    raise(kLimitExceeded, "image", "search image exceeds pixel budget");
+  for (size_t f = 0; f < n_frames; ++f) {
+    const size_t W = inputs[f].width, H = inputs[f].height;
+    if (W > size_t(std::numeric_limits<int>::max()) || H > size_t(std::numeric_limits<int>::max()) || H > 32000000 / W)
+      raise(kLimitExceeded, "image", "search image exceeds pixel budget");
+  }
   auto dmodel = device_model(model, stream_);
 
-  // ---- crop (matcher.cpp:383-393; full-image domain)
-  const LevelGeom& base = model.levels.front();
-  const double radius = std::hypot(std::max(base.orow, double(base.h - 1) - base.orow),
-                                   std::max(base.ocol, double(base.w - 1) - base.ocol));
-  const int64_t lattice = int64_t(size_t(1) << (level_count - 1));
-  const int64_t halo = int64_t(std::ceil(radius)) + 4 * lattice;
-  const int64_t crop_r = std::max<int64_t>(0, 0 - halo) / lattice * lattice;
-  const int64_t crop_c = std::max<int64_t>(0, 0 - halo) / lattice * lattice;
-  const int64_t crop_b = std::min<int64_t>(int64_t(H), ((int64_t(H) - 1 + halo) / lattice + 1) * lattice);
-  const int64_t crop_e = std::min<int64_t>(int64_t(W), ((int64_t(W) - 1 + halo) / lattice + 1) * lattice);
-  const uint8_t* crop = image + crop_r * int64_t(pitch) + crop_c;
-
-  // ---- level geometry of the search pyramid (matcher.cpp:407, 437)
-  struct Lv { int w, h, kind, pitch; size_t off, bytes; };
-  std::vector<Lv> lv(level_count);
-  lv[0] = {int(crop_e - crop_c), int(crop_b - crop_r), NCC_U8, int(pitch), 0, 0};
-  for (size_t l = 1; l < level_count; ++l) {
-    lv[l].w = std::max(1, lv[l - 1].w / 2);
-    lv[l].h = std::max(1, lv[l - 1].h / 2);
-    lv[l].kind = l <= 4 ? NCC_U16 : NCC_U32;
-    lv[l].pitch = lv[l].w;
-    lv[l].bytes = size_t(lv[l].w) * lv[l].h * (l <= 4 ? 2 : 4);
-  }
-
-  // ---- query patches per level (matcher.cpp:463-482)
+  // ---- query patches per level (matcher.cpp:463-482), shared by the batch
   std::vector<std::vector<int32_t>> refs(level_count);
   std::vector<std::vector<double>> ref_angles(level_count);
   for (size_t li = 0; li < level_count; ++li) {
@@ -335,49 +343,79 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   auto raster_of = [&](int32_t r) -> const Raster& {
     return (r >> 24) ? qpool.rasters[size_t(r & 0xffffff)] : model.pool.rasters[size_t(r & 0xffffff)];
   };
-
-  // ---- coarse anchor ranges (matcher.cpp:549-553; full-image domain)
+  const LevelGeom& base = model.levels.front();
+  const double radius = std::hypot(std::max(base.orow, double(base.h - 1) - base.orow),
+                                   std::max(base.ocol, double(base.w - 1) - base.ocol));
   const double scale = std::ldexp(1., int(top)), phase = (scale - 1) * .5;
   const size_t A = refs[top].size();
-  std::vector<int32_t> ranges(4 * A);
   int max_bh = 0, max_bw = 0;
   for (size_t i = 0; i < A; ++i) {
     const Raster& p = raster_of(refs[top][i]);
-    const double margin = scale == 1 ? .5 : 2 * scale;
-    const int64_t rmin_box = 0, rmax_box = int64_t(H) - 1, cmin_box = 0, cmax_box = int64_t(W) - 1;
-    ranges[4 * i + 0] = std::max({0, -p.rmin, int(std::ceil((rmin_box - margin - crop_r - phase) / scale - p.orow))});
-    ranges[4 * i + 1] = std::min(lv[top].h - p.rmax, int(std::floor((rmax_box + margin - crop_r - phase) / scale - p.orow)) + 1);
-    ranges[4 * i + 2] = std::max({0, -p.cmin, int(std::ceil((cmin_box - margin - crop_c - phase) / scale - p.ocol))});
-    ranges[4 * i + 3] = std::min(lv[top].w - p.cmax, int(std::floor((cmax_box + margin - crop_c - phase) / scale - p.ocol)) + 1);
     max_bh = std::max(max_bh, p.rmax - p.rmin);
     max_bw = std::max(max_bw, p.cmax - p.cmin);
-    if (trace)
-      trace->coarse_flops += 2.0 * double(p.count) * std::max(0, ranges[4 * i + 1] - ranges[4 * i]) *
-                             std::max(0, ranges[4 * i + 3] - ranges[4 * i + 2]);
+  }
+
+  // ---- per-frame geometry: crop (matcher.cpp:383-393), pyramid levels, coarse anchor ranges (:549-553)
+  std::vector<Job> jobs(n_frames);
+  const int64_t lattice = int64_t(size_t(1) << (level_count - 1));
+  const int64_t halo = int64_t(std::ceil(radius)) + 4 * lattice;
+  for (size_t f = 0; f < n_frames; ++f) {
+    Job& J = jobs[f];
+    J.in = inputs[f];
+    J.trace = traces ? &traces[f] : nullptr;
+    const int64_t W = int64_t(J.in.width), H = int64_t(J.in.height);
+    J.crop_r = std::max<int64_t>(0, 0 - halo) / lattice * lattice;
+    J.crop_c = std::max<int64_t>(0, 0 - halo) / lattice * lattice;
+    J.crop_b = std::min<int64_t>(H, ((H - 1 + halo) / lattice + 1) * lattice);
+    J.crop_e = std::min<int64_t>(W, ((W - 1 + halo) / lattice + 1) * lattice);
+    J.lv.resize(level_count);
+    J.lv[0] = {int(J.crop_e - J.crop_c), int(J.crop_b - J.crop_r), NCC_U8, int(J.in.pitch), 0, 0};
+    for (size_t l = 1; l < level_count; ++l) {
+      J.lv[l].w = std::max(1, J.lv[l - 1].w / 2);
+      J.lv[l].h = std::max(1, J.lv[l - 1].h / 2);
+      J.lv[l].kind = l <= 4 ? NCC_U16 : NCC_U32;
+      J.lv[l].pitch = J.lv[l].w;
+      J.lv[l].bytes = size_t(J.lv[l].w) * J.lv[l].h * (l <= 4 ? 2 : 4);
+    }
+    J.ranges.resize(4 * A);
+    for (size_t i = 0; i < A; ++i) {
+      const Raster& p = raster_of(refs[top][i]);
+      const double margin = scale == 1 ? .5 : 2 * scale;
+      int32_t* rg = &J.ranges[4 * i];
+      rg[0] = std::max({0, -p.rmin, int(std::ceil((0 - margin - J.crop_r - phase) / scale - p.orow))});
+      rg[1] = std::min(J.lv[top].h - p.rmax, int(std::floor((H - 1 + margin - J.crop_r - phase) / scale - p.orow)) + 1);
+      rg[2] = std::max({0, -p.cmin, int(std::ceil((0 - margin - J.crop_c - phase) / scale - p.ocol))});
+      rg[3] = std::min(J.lv[top].w - p.cmax, int(std::floor((W - 1 + margin - J.crop_c - phase) / scale - p.ocol)) + 1);
+      J.coarse_flops += 2.0 * double(p.count) * std::max(0, rg[1] - rg[0]) * std::max(0, rg[3] - rg[2]);
+    }
   }
 
   // ---- workspace layout
-  const size_t cap = o.max_candidates + 1;
-  const int Wt = lv[top].w, Ht = lv[top].h;
+  const size_t cap = o.max_candidates + 1;       // per frame
+  const size_t cap_all = cap * n_frames;
   const PoolLayout QL(qpool);
   size_t off = 0;
-  for (size_t l = 1; l < level_count; ++l) { lv[l].off = off; off = align_up(off + lv[l].bytes); }
-  const size_t off_p1 = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
-  const size_t off_p2 = off; off = align_up(off + size_t(Wt + 1) * Ht * 8);
-  const size_t off_nz = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
-  const size_t off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
-  const size_t off_peaks = off; off = align_up(off + cap * sizeof(NccPeak));
-  const size_t off_count = off; off = align_up(off + 16);
-  const size_t off_cand = off; off = align_up(off + cap * sizeof(NccMatch));
-  const size_t off_out = off; off = align_up(off + cap * sizeof(NccMatch));
-  const size_t off_refined = off; off = align_up(off + cap * sizeof(NccMatch));
-  const size_t off_verdict = off; off = align_up(off + cap * 4);
-  // small per-find uploads: refs of every level, coarse ranges, query pool
+  for (auto& J : jobs) {
+    const int Wt = J.lv[top].w, Ht = J.lv[top].h;
+    for (size_t l = 1; l < level_count; ++l) { J.lv[l].off = off; off = align_up(off + J.lv[l].bytes); }
+    J.off_p1 = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
+    J.off_p2 = off; off = align_up(off + size_t(Wt + 1) * Ht * 8);
+    J.off_nz = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
+    J.off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
+    J.off_peaks = off; off = align_up(off + cap * sizeof(NccPeak));
+  }
+  const size_t off_count = off; off = align_up(off + n_frames * 4);
+  const size_t off_cand = off; off = align_up(off + cap_all * sizeof(NccMatch));
+  const size_t off_cand_frame = off; off = align_up(off + cap_all * 4);
+  const size_t off_out = off; off = align_up(off + cap_all * sizeof(NccMatch));
+  const size_t off_verdict = off; off = align_up(off + cap_all * 4);
+  const size_t off_frames = off; off = align_up(off + n_frames * sizeof(NccFrame));
+  // small uploads: refs of every level, per-frame coarse ranges, query pool
   std::vector<size_t> off_refs(level_count);
   const size_t off_small = off;
   size_t small = 0;
   for (size_t l = 0; l < level_count; ++l) { off_refs[l] = off_small + small; small = align_up(small + refs[l].size() * 4 + 4); }
-  const size_t off_ranges = off_small + small; small = align_up(small + ranges.size() * 4);
+  for (auto& J : jobs) { J.off_ranges = off_small + small; small = align_up(small + J.ranges.size() * 4); }
   const size_t off_qpool = off_small + small; small = align_up(small + QL.total);
   off = off_small + small;
   if (work_.bytes < off) {
@@ -385,163 +423,203 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
     work_ = DeviceBuffer(align_up(off + off / 4, 1 << 20));
   }
   uint8_t* dev = work_.as<uint8_t>();
-  stage_.reserve(std::max<size_t>(small, cap * sizeof(NccMatch)));
-  readback_.reserve(cap * (2 * sizeof(NccMatch) + 4) + cap * sizeof(NccPeak) + 64);
+  stage_.reserve(std::max({small, cap_all * (sizeof(NccMatch) + 4), n_frames * sizeof(NccFrame)}) + 256);
+  readback_.reserve(cap_all * (sizeof(NccMatch) + 4) + cap * sizeof(NccPeak) + n_frames * 4 + 64);
   {
-    uint8_t* s = static_cast<uint8_t*>(stage_.ptr);
+    uint8_t* st = static_cast<uint8_t*>(stage_.ptr);
     for (size_t l = 0; l < level_count; ++l)
-      std::memcpy(s + (off_refs[l] - off_small), refs[l].data(), refs[l].size() * 4);
-    std::memcpy(s + (off_ranges - off_small), ranges.data(), ranges.size() * 4);
-    serialize(qpool, QL, s + (off_qpool - off_small));
-    check(hipMemcpyAsync(dev + off_small, s, small, hipMemcpyHostToDevice, stream_), "upload");
+      std::memcpy(st + (off_refs[l] - off_small), refs[l].data(), refs[l].size() * 4);
+    for (auto& J : jobs) std::memcpy(st + (J.off_ranges - off_small), J.ranges.data(), J.ranges.size() * 4);
+    serialize(qpool, QL, st + (off_qpool - off_small));
+    check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, stream_), "upload");
   }
-  NccPatchPool pools[2] = {dmodel->pool.view, view_at(dev + off_qpool, QL)};
-  auto level_desc = [&](size_t l) {
+  const NccPatchPool pools[2] = {dmodel->pool.view, view_at(dev + off_qpool, QL)};
+  auto level_desc = [&](const Job& J, size_t l) {
     NccLevel d{};
-    d.data = l ? static_cast<const void*>(dev + lv[l].off) : static_cast<const void*>(crop);
-    d.kind = lv[l].kind; d.pitch = lv[l].pitch; d.width = lv[l].w; d.height = lv[l].h; d.shift = int32_t(2 * l);
+    d.data = l ? static_cast<const void*>(dev + J.lv[l].off)
+               : static_cast<const void*>(J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c);
+    d.kind = J.lv[l].kind; d.pitch = J.lv[l].pitch; d.width = J.lv[l].w; d.height = J.lv[l].h; d.shift = int32_t(2 * l);
     return d;
   };
+  auto crop_of = [&](const Job& J) { return J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c; };
 
-  // ---- search pyramid, top-level statistics, coarse search, peaks
-  time_begin(kStagePyramid);
-  for (size_t l = 1; l < level_count; ++l) {
-    const NccLevel src = level_desc(l - 1);
-    if (ncchip_pyramid_level(src.data, src.kind, src.pitch, src.width, src.height, dev + lv[l].off, lv[l].kind,
-                             lv[l].pitch, lv[l].w, lv[l].h, stream_))
-      raise(kInternalError, "device", "pyramid launch failed");
-  }
-  time_end();
-  const NccLevel topd = level_desc(top);
-  auto* p1 = reinterpret_cast<uint32_t*>(dev + off_p1);
-  auto* p2 = reinterpret_cast<uint64_t*>(dev + off_p2);
-  time_begin(kStagePrefix);
-  if (ncchip_row_prefix(topd.data, topd.kind, topd.pitch, Wt, Ht, crop, int(2 * top), p1, p2,
-                        reinterpret_cast<uint32_t*>(dev + off_nz), nullptr, stream_))
-    raise(kInternalError, "device", "prefix launch failed");
-  time_end();
-  auto* planes = reinterpret_cast<double*>(dev + off_planes);
+  // ---- search pyramid, top-level statistics, coarse search and peaks of every frame
+  auto* counts = reinterpret_cast<int32_t*>(dev + off_count);
+  check(hipMemsetAsync(counts, 0, n_frames * 4, stream_), "memset");
   const double C = std::ldexp(128.0, int(2 * top));
   const int exact_top = top <= 2;
-  int rc = 0;
-  time_begin(kStageCoarse);
-  if (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN)
-    rc = ncchip_coarse_mfma(&topd, crop, C, p1, p2, &pools[0], reinterpret_cast<const int32_t*>(dev + off_refs[top]),
-                            reinterpret_cast<const int32_t*>(dev + off_ranges), int(A), max_bh, max_bw, exact_top,
-                            model.params.absolute, o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, planes, stream_);
-  else
-    rc = ncchip_coarse(&topd, crop, C, p1, p2, &pools[0], reinterpret_cast<const int32_t*>(dev + off_refs[top]),
-                       reinterpret_cast<const int32_t*>(dev + off_ranges), int(A), max_bh, max_bw, o.form, exact_top,
-                       model.params.absolute, planes, stream_);
-  if (rc) raise(kInternalError, "device", "coarse launch failed");
-  time_end();
-  auto* count = reinterpret_cast<int32_t*>(dev + off_count);
-  check(hipMemsetAsync(count, 0, 4, stream_), "memset");
-  time_begin(kStagePeaks);
-  if (ncchip_peaks(planes, int(A), Ht, Wt, o.min_score * .4, reinterpret_cast<NccPeak*>(dev + off_peaks), int(cap),
-                   count, stream_))
-    raise(kInternalError, "device", "peak launch failed");
-  time_end();
+  for (size_t f = 0; f < n_frames; ++f) {
+    Job& J = jobs[f];
+    time_begin(kStagePyramid);
+    for (size_t l = 1; l < level_count; ++l) {
+      const NccLevel src = level_desc(J, l - 1);
+      if (ncchip_pyramid_level(src.data, src.kind, src.pitch, src.width, src.height, dev + J.lv[l].off, J.lv[l].kind,
+                               J.lv[l].pitch, J.lv[l].w, J.lv[l].h, stream_))
+        raise(kInternalError, "device", "pyramid launch failed");
+    }
+    time_end();
+    const NccLevel topd = level_desc(J, top);
+    const int Wt = J.lv[top].w, Ht = J.lv[top].h;
+    auto* p1 = reinterpret_cast<uint32_t*>(dev + J.off_p1);
+    auto* p2 = reinterpret_cast<uint64_t*>(dev + J.off_p2);
+    time_begin(kStagePrefix);
+    if (ncchip_row_prefix(topd.data, topd.kind, topd.pitch, Wt, Ht, crop_of(J), int(2 * top), p1, p2,
+                          reinterpret_cast<uint32_t*>(dev + J.off_nz), nullptr, stream_))
+      raise(kInternalError, "device", "prefix launch failed");
+    time_end();
+    auto* planes = reinterpret_cast<double*>(dev + J.off_planes);
+    const auto* ranges = reinterpret_cast<const int32_t*>(dev + J.off_ranges);
+    const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
+    int rc = 0;
+    time_begin(kStageCoarse);
+    if (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN)
+      rc = ncchip_coarse_mfma(&topd, crop_of(J), C, p1, p2, pools, coarse_refs, ranges, int(A), max_bh, max_bw, exact_top,
+                              model.params.absolute, o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, planes,
+                              stream_);
+    else
+      rc = ncchip_coarse(&topd, crop_of(J), C, p1, p2, pools, coarse_refs, ranges, int(A), max_bh, max_bw, o.form,
+                         exact_top, model.params.absolute, planes, stream_);
+    if (rc) raise(kInternalError, "device", "coarse launch failed");
+    time_end();
+    time_begin(kStagePeaks);
+    if (ncchip_peaks(planes, int(A), Ht, Wt, o.min_score * .4, reinterpret_cast<NccPeak*>(dev + J.off_peaks), int(cap),
+                     counts + f, stream_))
+      raise(kInternalError, "device", "peak launch failed");
+    time_end();
+  }
   auto* rb = static_cast<uint8_t*>(readback_.ptr);
-  check(hipMemcpyAsync(rb, count, 4, hipMemcpyDeviceToHost, stream_), "readback");
+  check(hipMemcpyAsync(rb, counts, n_frames * 4, hipMemcpyDeviceToHost, stream_), "readback");
   check(hipStreamSynchronize(stream_), "coarse");
-  int32_t n_peaks = *reinterpret_cast<int32_t*>(rb);
-  if (size_t(n_peaks) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
-  std::vector<Pose> cand;
-  if (n_peaks > 0) {
-    check(hipMemcpyAsync(rb, dev + off_peaks, size_t(n_peaks) * sizeof(NccPeak), hipMemcpyDeviceToHost, stream_),
+  std::vector<int32_t> n_peaks(reinterpret_cast<int32_t*>(rb), reinterpret_cast<int32_t*>(rb) + n_frames);
+  for (size_t f = 0; f < n_frames; ++f)
+    if (size_t(n_peaks[f]) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
+  for (size_t f = 0; f < n_frames; ++f) {
+    Job& J = jobs[f];
+    if (n_peaks[f] <= 0) continue;
+    check(hipMemcpyAsync(rb, dev + J.off_peaks, size_t(n_peaks[f]) * sizeof(NccPeak), hipMemcpyDeviceToHost, stream_),
           "readback");
     check(hipStreamSynchronize(stream_), "coarse");
-    std::vector<NccPeak> pk(static_cast<size_t>(n_peaks));
-    std::memcpy(pk.data(), rb, size_t(n_peaks) * sizeof(NccPeak));
+    std::vector<NccPeak> pk(static_cast<size_t>(n_peaks[f]));
+    std::memcpy(pk.data(), rb, pk.size() * sizeof(NccPeak));
     // the reference emits peaks in (angle, row, col) order
     std::sort(pk.begin(), pk.end(), [](const NccPeak& a, const NccPeak& b) {
       if (a.angle != b.angle) return a.angle < b.angle;
       if (a.row != b.row) return a.row < b.row;
       return a.col < b.col;
     });
-    cand.reserve(pk.size());
+    J.cand.reserve(pk.size());
     for (const auto& k : pk) {
       const Raster& p = raster_of(refs[top][size_t(k.angle)]);
-      cand.push_back({double(crop_r) + scale * (k.row + p.orow) + phase, double(crop_c) + scale * (k.col + p.ocol) + phase,
-                      p.angle, k.score});
+      J.cand.push_back({double(J.crop_r) + scale * (k.row + p.orow) + phase,
+                        double(J.crop_c) + scale * (k.col + p.ocol) + phase, p.angle, k.score});
     }
-  }
-  if (trace) {
-    trace->coarse = cand; trace->top = int(top); trace->finest = int(finest); trace->angle_stride = int(stride);
-    trace->tracked.clear();
   }
   const double t1 = now_us();
-  if (cand.empty()) {
-    time_collect(trace);
-    if (trace) { trace->t_coarse_us = t1 - t0; trace->t_total_us = t1 - t0; }
-    return {};
-  }
-
-  // ---- level-0 storage of the candidate region (matcher.cpp:612-629)
-  int32_t storage0[4] = {0, 0, lv[0].h, lv[0].w};
-  if (!finest) {
-    double rmin = lv[0].h, cmin = lv[0].w, rmax = 0, cmax = 0;
-    for (const auto& p : cand) {
-      rmin = std::min(rmin, p.row - crop_r); rmax = std::max(rmax, p.row - crop_r);
-      cmin = std::min(cmin, p.col - crop_c); cmax = std::max(cmax, p.col - crop_c);
+  for (auto& J : jobs)
+    if (J.trace) J.trace->coarse = J.cand;
+  // level-0 storage of the candidate region (matcher.cpp:612-629)
+  for (auto& J : jobs) {
+    J.storage0[2] = J.lv[0].h;
+    J.storage0[3] = J.lv[0].w;
+    if (finest || J.cand.empty()) continue;
+    double rmin = J.lv[0].h, cmin = J.lv[0].w, rmax = 0, cmax = 0;
+    for (const auto& p : J.cand) {
+      rmin = std::min(rmin, p.row - J.crop_r); rmax = std::max(rmax, p.row - J.crop_r);
+      cmin = std::min(cmin, p.col - J.crop_c); cmax = std::max(cmax, p.col - J.crop_c);
     }
     const double margin = radius + 4 * scale + 8;
-    storage0[0] = int32_t(size_t(std::max(0., std::floor(rmin - margin))));
-    storage0[1] = int32_t(size_t(std::max(0., std::floor(cmin - margin))));
-    storage0[2] = int32_t(size_t(std::min(double(lv[0].h), std::ceil(rmax + margin) + 1)) - size_t(storage0[0]));
-    storage0[3] = int32_t(size_t(std::min(double(lv[0].w), std::ceil(cmax + margin) + 1)) - size_t(storage0[1]));
+    J.storage0[0] = int32_t(size_t(std::max(0., std::floor(rmin - margin))));
+    J.storage0[1] = int32_t(size_t(std::max(0., std::floor(cmin - margin))));
+    J.storage0[2] = int32_t(size_t(std::min(double(J.lv[0].h), std::ceil(rmax + margin) + 1)) - size_t(J.storage0[0]));
+    J.storage0[3] = int32_t(size_t(std::min(double(J.lv[0].w), std::ceil(cmax + margin) + 1)) - size_t(J.storage0[1]));
   }
-  if (trace) std::copy(storage0, storage0 + 4, trace->storage);
+  // frame descriptors for the per-candidate kernels
+  {
+    auto* fr = static_cast<NccFrame*>(stage_.ptr);
+    for (size_t f = 0; f < n_frames; ++f) {
+      const Job& J = jobs[f];
+      NccFrame d{};
+      for (size_t l = 0; l < level_count; ++l) d.levels[l] = level_desc(J, l);
+      d.dc_src = crop_of(J);
+      std::copy(J.storage0, J.storage0 + 4, d.storage0);
+      d.crop_r = double(J.crop_r); d.crop_c = double(J.crop_c);
+      d.image_h = int32_t(J.in.height); d.image_w = int32_t(J.in.width);
+      fr[f] = d;
+    }
+    check(hipMemcpyAsync(dev + off_frames, fr, n_frames * sizeof(NccFrame), hipMemcpyHostToDevice, stream_), "upload");
+  }
+  const auto* dframes = reinterpret_cast<const NccFrame*>(dev + off_frames);
   auto* dcand = reinterpret_cast<NccMatch*>(dev + off_cand);
+  auto* dcand_frame = reinterpret_cast<int32_t*>(dev + off_cand_frame);
   auto* dout = reinterpret_cast<NccMatch*>(dev + off_out);
-  auto upload_cand = [&](const std::vector<Pose>& c) {
-    auto* s = static_cast<NccMatch*>(stage_.ptr);
-    for (size_t i = 0; i < c.size(); ++i) s[i] = {c[i].row, c[i].col, c[i].angle, c[i].score};
-    check(hipMemcpyAsync(dcand, s, c.size() * sizeof(NccMatch), hipMemcpyHostToDevice, stream_), "upload");
+  auto* dverdict = reinterpret_cast<int32_t*>(dev + off_verdict);
+  // gathers (frame, candidate) lists into one upload; the staging copy is
+  // synchronous with respect to the next readback sync of this stream
+  auto upload = [&](const std::vector<std::pair<uint32_t, Pose>>& list) {
+    check(hipStreamSynchronize(stream_), "sync");  // the staging buffer is reused
+    auto* sm = static_cast<NccMatch*>(stage_.ptr);
+    auto* sf = reinterpret_cast<int32_t*>(sm + list.size());
+    for (size_t i = 0; i < list.size(); ++i) {
+      const Pose& p = list[i].second;
+      sm[i] = {p.row, p.col, p.angle, p.score};
+      sf[i] = int32_t(list[i].first);
+    }
+    check(hipMemcpyAsync(dcand, sm, list.size() * sizeof(NccMatch), hipMemcpyHostToDevice, stream_), "upload");
+    check(hipMemcpyAsync(dcand_frame, sf, list.size() * 4, hipMemcpyHostToDevice, stream_), "upload");
+  };
+  auto readback = [&](size_t n, bool with_verdict) {
+    check(hipMemcpyAsync(rb, dout, n * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
+    if (with_verdict)
+      check(hipMemcpyAsync(rb + n * sizeof(NccMatch), dverdict, n * 4, hipMemcpyDeviceToHost, stream_), "readback");
+    check(hipStreamSynchronize(stream_), "readback");
+  };
+  auto result_at = [&](size_t i) {
+    const NccMatch& m = reinterpret_cast<const NccMatch*>(rb)[i];
+    return Pose{m.row, m.col, m.angle, m.score};
   };
 
-  // ---- coarse-to-fine tracking (matcher.cpp:630-761)
+  // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
   for (size_t l = top; l > finest; --l) {
     if (l == 1 && o.subpixel) {
-      for (auto& p : cand) {
-        const double r = std::floor(p.row + .5), c = std::floor(p.col + .5);
-        if (!(r >= 0 && c >= 0 && r < double(H) && c < double(W))) p.score = kMissing;
+      for (auto& J : jobs) {
+        for (auto& p : J.cand) {
+          const double r = std::floor(p.row + .5), c = std::floor(p.col + .5);
+          if (!(r >= 0 && c >= 0 && r < double(J.in.height) && c < double(J.in.width))) p.score = kMissing;
+        }
+        merge_identical(J.cand);
       }
-      merge_identical(cand);
       break;
     }
-    const NccLevel tl = level_desc(l - 1);
+    std::vector<std::pair<uint32_t, Pose>> list;
+    for (size_t f = 0; f < n_frames; ++f)
+      for (const auto& p : jobs[f].cand) list.push_back({uint32_t(f), p});
+    if (list.empty()) break;
     const double step = std::ldexp(1., int(l - 1)), offset = (step - 1) * .5;
     const double tol = model.params.step * (l == top ? double(stride) : 1.) * 1.01;
-    int32_t storage[4] = {0, 0, tl.height, tl.width};
-    if (l - 1 == 0) std::copy(storage0, storage0 + 4, storage);
-    upload_cand(cand);
+    upload(list);
     time_begin(kStageTrack);
-    if (ncchip_track(&tl, crop, pools, reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
-                     dcand, dout, int(cand.size()), tol, double(crop_r), double(crop_c), step, offset, storage,
-                     l - 1 <= 2, model.params.absolute, stream_))
+    if (ncchip_track(dframes, dcand_frame, int(l - 1), jobs[0].lv[l - 1].kind, pools,
+                     reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()), dcand, dout,
+                     int(list.size()), tol, step, offset, l - 1 <= 2, model.params.absolute, stream_))
       raise(kInternalError, "device", "tracking launch failed");
     time_end();
-    check(hipMemcpyAsync(rb, dout, cand.size() * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipStreamSynchronize(stream_), "tracking");
-    const auto* res = reinterpret_cast<const NccMatch*>(rb);
-    for (size_t i = 0; i < cand.size(); ++i) cand[i] = {res[i].row, res[i].col, res[i].angle, res[i].score};
-    merge_identical(cand);
-    cand.erase(std::remove_if(cand.begin(), cand.end(),
-                              [&](const Pose& p) { return !std::isfinite(p.score) || p.score < o.min_score * .4; }),
-               cand.end());
-    if (trace) trace->tracked.push_back(cand);
+    readback(list.size(), false);
+    for (auto& J : jobs) J.cand.clear();
+    for (size_t i = 0; i < list.size(); ++i) jobs[list[i].first].cand.push_back(result_at(i));
+    for (auto& J : jobs) {
+      merge_identical(J.cand);
+      J.cand.erase(std::remove_if(J.cand.begin(), J.cand.end(),
+                                  [&](const Pose& p) { return !std::isfinite(p.score) || p.score < o.min_score * .4; }),
+                   J.cand.end());
+      if (J.trace) J.trace->tracked.push_back(J.cand);
+    }
   }
   const double t2 = now_us();
 
   // ---- full score at the finest level, then candidate order (matcher.cpp:813-827)
   NcchipRefineParams rp{};
-  rp.level = level_desc(finest);
-  rp.dc_src = crop;
-  if (finest) { rp.storage[0] = 0; rp.storage[1] = 0; rp.storage[2] = lv[finest].h; rp.storage[3] = lv[finest].w; }
-  else std::copy(storage0, storage0 + 4, rp.storage);
-  rp.crop_r = double(crop_r); rp.crop_c = double(crop_c);
+  rp.frames = dframes;
+  rp.level_kind = jobs[0].lv[finest].kind;
   rp.scale = std::ldexp(1., int(finest)); rp.phase = (rp.scale - 1) * .5;
   rp.finest = int32_t(finest); rp.top = int32_t(top);
   rp.orig_ref = finest ? ref(1, uint32_t(coarse_original)) : ref(0, model.original);
@@ -550,84 +628,92 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   rp.n_refs0 = int32_t(refs[0].size());
   rp.step = model.params.step; rp.angle_start = o.start; rp.angle_end = o.start + o.extent;
   rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
-  rp.image_h = int32_t(H); rp.image_w = int32_t(W);
-  std::vector<Pose> full(cand.size());
-  if (!cand.empty()) {
-    upload_cand(cand);
-    time_begin(kStageFull);
-    if (ncchip_full_score(&rp, pools, dcand, int(cand.size()), dout, stream_))
-      raise(kInternalError, "device", "full score launch failed");
-    time_end();
-    check(hipMemcpyAsync(rb, dout, cand.size() * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipStreamSynchronize(stream_), "full score");
-    const auto* f = reinterpret_cast<const NccMatch*>(rb);
-    for (size_t i = 0; i < cand.size(); ++i) full[i] = {f[i].row, f[i].col, f[i].angle, f[i].score};
+  {
+    std::vector<std::pair<uint32_t, Pose>> list;
+    for (size_t f = 0; f < n_frames; ++f)
+      for (const auto& p : jobs[f].cand) list.push_back({uint32_t(f), p});
+    if (!list.empty()) {
+      upload(list);
+      time_begin(kStageFull);
+      if (ncchip_full_score(&rp, pools, dcand, dcand_frame, int(list.size()), dout, stream_))
+        raise(kInternalError, "device", "full score launch failed");
+      time_end();
+      readback(list.size(), false);
+      for (size_t i = 0; i < list.size(); ++i) jobs[list[i].first].full.push_back(result_at(i));
+    }
+    for (auto& J : jobs) {
+      std::sort(J.full.begin(), J.full.end(), score_order);
+      J.refined.assign(J.full.size(), Pose{});
+      J.verdict.assign(J.full.size(), -1);  // -1: not refined
+      J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
+    }
   }
-  std::sort(full.begin(), full.end(), score_order);
 
-  // ---- greedy selection in score order (matcher.cpp:1063-1076).  Each
-  // candidate's refinement is independent of the others, so candidates are
-  // refined on the device in batches, in the order the greedy walk needs them.
-  const size_t n = full.size();
-  std::vector<Pose> refined(n);
-  std::vector<int> verdict(n, -1);  // -1: not refined
-  std::vector<Pose> result;
-  auto suppressed = [&](const Pose& c) {
-    for (const auto& p : result) if (pose_overlap(c, p, base) > o.max_overlap + 1e-12) return true;
+  // ---- greedy selection in score order (matcher.cpp:1063-1076).  A
+  // candidate's refinement does not depend on the other candidates, so each
+  // frame's walk asks for the next candidates it may need and every frame's
+  // requests are refined together in one launch.
+  auto suppressed = [&](const Job& J, const Pose& c) {
+    for (const auto& p : J.results) if (pose_overlap(c, p, base) > o.max_overlap + 1e-12) return true;
     return false;
   };
-  auto* drefined = reinterpret_cast<NccMatch*>(dev + off_refined);
-  auto* dverdict = reinterpret_cast<int32_t*>(dev + off_verdict);
-  size_t pos = 0, batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : n;
-  bool done = false;
-  while (!done && pos < n) {
-    // walk as far as the refined results allow
-    while (pos < n) {
-      const Pose& c = full[pos];
-      if (!std::isfinite(c.score) || suppressed(c)) { ++pos; continue; }
-      if (verdict[pos] < 0) break;
-      if (verdict[pos] && !suppressed(refined[pos])) {
-        result.push_back(refined[pos]);
-        if (o.num_matches && result.size() == o.num_matches) { done = true; break; }
+  for (;;) {
+    std::vector<std::pair<uint32_t, Pose>> list;
+    std::vector<size_t> index;  // candidate index within its frame
+    for (size_t f = 0; f < n_frames; ++f) {
+      Job& J = jobs[f];
+      const size_t n = J.full.size();
+      while (!J.done && J.pos < n) {
+        const Pose& c = J.full[J.pos];
+        if (!std::isfinite(c.score) || suppressed(J, c)) { ++J.pos; continue; }
+        if (J.verdict[J.pos] < 0) break;
+        if (J.verdict[J.pos] && !suppressed(J, J.refined[J.pos])) {
+          J.results.push_back(J.refined[J.pos]);
+          if (o.num_matches && J.results.size() == o.num_matches) { J.done = true; break; }
+        }
+        ++J.pos;
       }
-      ++pos;
+      if (J.done || J.pos >= n) { J.done = true; continue; }
+      size_t taken = 0;
+      for (size_t i = J.pos; i < n && taken < J.batch_size; ++i)
+        if (std::isfinite(J.full[i].score) && J.verdict[i] < 0 && !suppressed(J, J.full[i])) {
+          list.push_back({uint32_t(f), J.full[i]});
+          index.push_back(i);
+          ++taken;
+        }
+      J.batch_size *= 2;
     }
-    if (done || pos >= n) break;
-    // refine the next candidates the walk may need
-    std::vector<size_t> batch;
-    std::vector<Pose> poses;
-    for (size_t i = pos; i < n && batch.size() < batch_size; ++i)
-      if (std::isfinite(full[i].score) && verdict[i] < 0 && !suppressed(full[i])) { batch.push_back(i); poses.push_back(full[i]); }
-    upload_cand(poses);
+    if (list.empty()) break;
+    upload(list);
     time_begin(kStageRefine);
-    if (ncchip_refine(&rp, pools, dcand, int(poses.size()), drefined, dverdict, stream_))
+    if (ncchip_refine(&rp, pools, dcand, dcand_frame, int(list.size()), dout, dverdict, stream_))
       raise(kInternalError, "device", "refine launch failed");
     time_end();
-    const size_t m = poses.size();
-    check(hipMemcpyAsync(rb, drefined, m * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipMemcpyAsync(rb + m * sizeof(NccMatch), dverdict, m * 4, hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipStreamSynchronize(stream_), "refine");
-    const auto* g = reinterpret_cast<const NccMatch*>(rb);
-    const auto* v = reinterpret_cast<const int32_t*>(rb + m * sizeof(NccMatch));
-    for (size_t k = 0; k < m; ++k) {
-      refined[batch[k]] = {g[k].row, g[k].col, g[k].angle, g[k].score};
-      verdict[batch[k]] = v[k];
+    readback(list.size(), true);
+    const auto* v = reinterpret_cast<const int32_t*>(rb + list.size() * sizeof(NccMatch));
+    for (size_t k = 0; k < list.size(); ++k) {
+      Job& J = jobs[list[k].first];
+      J.refined[index[k]] = result_at(k);
+      J.verdict[index[k]] = v[k];
     }
-    batch_size *= 2;
   }
   const double t3 = now_us();
-  if (trace) {
-    trace->prepared = full;
-    trace->refined = refined;
-    trace->verdict = verdict;
+  time_collect(traces);  // event timings of the whole batch, into the first trace
+  for (size_t f = 0; f < n_frames; ++f) {
+    Job& J = jobs[f];
+    std::sort(J.results.begin(), J.results.end(), score_order);
+    results[f] = J.results;
+    if (J.trace) {
+      FindTrace& tr = *J.trace;
+      tr.top = int(top); tr.finest = int(finest); tr.angle_stride = int(stride);
+      std::copy(J.storage0, J.storage0 + 4, tr.storage);
+      tr.prepared = J.full; tr.refined = J.refined; tr.verdict = J.verdict;
+      tr.coarse_flops = J.coarse_flops;
+      tr.t_coarse_us = t1 - t0; tr.t_track_us = t2 - t1; tr.t_refine_us = t3 - t2; tr.t_total_us = now_us() - t0;
+      if (f > 0 && traces) std::copy(traces[0].kernel_us, traces[0].kernel_us + kStageCount, tr.kernel_us);
+    }
   }
-  std::sort(result.begin(), result.end(), score_order);
-  time_collect(trace);
-  if (trace) {
-    trace->t_coarse_us = t1 - t0; trace->t_track_us = t2 - t1; trace->t_refine_us = t3 - t2;
-    trace->t_total_us = now_us() - t0;
-  }
-  return result;
+  return results;
 }
 
 }  // namespace mvamd

@@ -86,6 +86,12 @@ struct FindTrace {
   double coarse_flops = 0;             // 2 * sum over evaluated coarse windows of their sample counts
 };
 
+// A u8 search image in device memory (`pitch` bytes per row).
+struct FrameInput {
+  const uint8_t* image;
+  size_t width, height, pitch;
+};
+
 // One find pipeline: owns a stream and grow-only device/pinned workspaces.
 // Not thread-safe; use one context per host thread.
 class FindContext {
@@ -98,6 +104,13 @@ class FindContext {
   // `image` is a device pointer to a u8 image with `pitch` bytes per row.
   std::vector<Pose> find(const NccModel& model, const uint8_t* image, size_t width, size_t height, size_t pitch,
                          const FindOptions& options, FindTrace* trace = nullptr);
+  // Finds the model in several frames with the same options.  Every stage
+  // runs for all frames before the next, and the per-candidate kernels take
+  // the candidates of every frame in one launch; each frame's result is the
+  // one find() returns for it.  traces (optional) has n_frames entries; the
+  // kernel timings of the batch are reported in every trace.
+  std::vector<std::vector<Pose>> find_batch(const NccModel& model, const FrameInput* frames, size_t n_frames,
+                                            const FindOptions& options, FindTrace* traces = nullptr);
   // Same, from host memory (copied to the device first).
   std::vector<Pose> find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
                               size_t pitch, const FindOptions& options, FindTrace* trace = nullptr);

@@ -32,21 +32,36 @@ __device__ __forceinline__ double ref_angle(const Pools& pools, int32_t ref) { r
 
 // ------------------------------------------------------------------ tracking
 struct TrackArgs {
-  NccLevel lv;
-  double dc;              // filled in-kernel from dc_src
-  int64_t D;
-  const uint8_t* dc_src;
+  const NccFrame* frames;     // per-frame descriptors (device)
+  const int32_t* cand_frame;  // frame index of every candidate
+  int level;                  // tracked level l-1
   Pools pools;
   const int32_t* refs;
   int n_refs;
   const NccMatch* cand;
   NccMatch* out;
   int n_cand;
-  double tol, crop_r, crop_c, step, offset;
-  int sr, sc, sh, sw;
+  double tol, step, offset;
   int exact_lattice, absolute;
-  int tile_cap;           // LDS tile capacity in pixels
+  int tile_cap;               // LDS tile capacity in pixels
+  // the candidate's frame, filled in-kernel
+  NccLevel lv;
+  double dc, crop_r, crop_c;
+  int64_t D;
+  int sr, sc, sh, sw;
 };
+
+// Loads the candidate's frame into the per-block copy of the arguments.
+__device__ __forceinline__ void bind_frame(TrackArgs& a, int ci) {
+  const NccFrame& F = a.frames[a.cand_frame[ci]];
+  a.lv = F.levels[a.level];
+  a.dc = double(*F.dc_src);
+  a.D = int64_t(*F.dc_src) << a.lv.shift;
+  a.crop_r = F.crop_r;
+  a.crop_c = F.crop_c;
+  if (a.level == 0) { a.sr = F.storage0[0]; a.sc = F.storage0[1]; a.sh = F.storage0[2]; a.sw = F.storage0[3]; }
+  else { a.sr = 0; a.sc = 0; a.sh = a.lv.height; a.sw = a.lv.width; }
+}
 
 // Pixel source of a tracking window: an LDS tile of the candidate's
 // neighbourhood, or the level in global memory when the tile is too large.
@@ -204,10 +219,9 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   __shared__ double red_s[kTrackThreads];
   __shared__ int red_i[kTrackThreads];
   TrackArgs a = args;
-  a.dc = double(*a.dc_src);
-  a.D = int64_t(*a.dc_src) << a.lv.shift;
   const int ci = blockIdx.x;
   if (ci >= a.n_cand) return;
+  bind_frame(a, ci);
   const NccMatch cd = a.cand[ci];
   const int n_e = eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt);
   // union of the windows' sample rectangles, clipped to the valid region
@@ -292,11 +306,9 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
 
 // ------------------------------------------------------- continuous / refine
 struct RefineArgs {
-  NccLevel lv;            // finest search level
-  double dc;              // filled in-kernel from dc_src
-  const uint8_t* dc_src;
-  int sr, sc, sh, sw;     // storage rectangle at the finest level
-  double crop_r, crop_c, scale, phase;
+  const NccFrame* frames;     // per-frame descriptors (device)
+  const int32_t* cand_frame;  // frame index of every candidate
+  double scale, phase;        // 2^finest, (scale - 1) / 2
   int finest, top;
   Pools pools;
   int32_t orig_ref;       // unrotated patch of the finest level (model.original at level 0)
@@ -305,17 +317,35 @@ struct RefineArgs {
   int n_refs0;
   double step, a_start, a_end, min_score;
   int subpixel, rot, absolute;
-  int img_h, img_w;       // search domain (full image)
-  int H0, W0;             // level-0 dims (crop)
   const NccMatch* cand;
   int n_cand;
   NccMatch* out_full;
   NccMatch* out_refined;
   int32_t* verdict;
+  // the candidate's frame, filled in-kernel
+  NccLevel lv;            // finest search level
+  double dc, crop_r, crop_c;
+  int sr, sc, sh, sw;     // storage rectangle at the finest level
+  int img_h, img_w;       // search domain (full image)
+  int H0, W0;             // level-0 dims (crop)
   // LDS copy of the candidate's neighbourhood (storage-local rectangle)
   const void* tile;
   int tr0, tc0, th, tw;
 };
+
+__device__ __forceinline__ void bind_frame(RefineArgs& a, int ci) {
+  const NccFrame& F = a.frames[a.cand_frame[ci]];
+  a.lv = F.levels[a.finest];
+  a.dc = double(*F.dc_src);
+  a.crop_r = F.crop_r;
+  a.crop_c = F.crop_c;
+  if (a.finest == 0) { a.sr = F.storage0[0]; a.sc = F.storage0[1]; a.sh = F.storage0[2]; a.sw = F.storage0[3]; }
+  else { a.sr = 0; a.sc = 0; a.sh = a.lv.height; a.sw = a.lv.width; }
+  a.img_h = F.image_h;
+  a.img_w = F.image_w;
+  a.H0 = F.levels[0].height;
+  a.W0 = F.levels[0].width;
+}
 
 constexpr int kRefineThreads = 512;
 constexpr int kRefineTileBytes = 64 * 1024;
@@ -741,11 +771,11 @@ __device__ __forceinline__ double template_reach(const RefineArgs& a, const NccP
 template <class T>
 __global__ void __launch_bounds__(kRefineThreads) k_full_score(const RefineArgs args) {
   RefineArgs a = args;
-  a.dc = double(*a.dc_src);
   __shared__ Shared sm;
   extern __shared__ uint8_t lds_tile[];
   const int ci = blockIdx.x;
   if (ci >= a.n_cand) return;
+  bind_frame(a, ci);
   const PatchRef O = patch_ref(a.pools, a.orig_ref);
   NccMatch cd = a.cand[ci];
   {
@@ -763,13 +793,13 @@ __global__ void __launch_bounds__(kRefineThreads) k_full_score(const RefineArgs 
 template <class T>
 __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args) {
   RefineArgs a = args;
-  a.dc = double(*a.dc_src);
   __shared__ Shared sm;
   __shared__ int elig[kMaxEligible];
   __shared__ int wave_cnt[kRefineThreads / 64];
   extern __shared__ uint8_t lds_tile[];
   const int ci = blockIdx.x;
   if (ci >= a.n_cand) return;
+  bind_frame(a, ci);
   const PatchRef O = patch_ref(a.pools, a.orig_ref);
   NccMatch cd = a.cand[ci];
   if (!isfinite(cd.score)) {
@@ -807,64 +837,62 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
 
 using namespace ncc;
 
-extern "C" int ncchip_track(const NccLevel* lv, const uint8_t* dc_src, const NccPatchPool* pools, const int32_t* refs,
-                            int n_refs, const NccMatch* cand, NccMatch* out, int n_cand, double tol, double crop_r,
-                            double crop_c, double step, double offset, const int32_t* storage, int exact_lattice,
+extern "C" int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, int level, int level_kind,
+                            const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* cand,
+                            NccMatch* out, int n_cand, double tol, double step, double offset, int exact_lattice,
                             int absolute, void* stream) {
   if (n_cand <= 0) return 0;
-  TrackArgs a;
-  a.lv = *lv; a.dc_src = dc_src; a.dc = 0; a.D = 0; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.refs = refs;
-  a.n_refs = n_refs; a.cand = cand; a.out = out; a.n_cand = n_cand; a.tol = tol; a.crop_r = crop_r;
-  a.crop_c = crop_c; a.step = step; a.offset = offset; a.sr = storage[0]; a.sc = storage[1]; a.sh = storage[2];
-  a.sw = storage[3]; a.exact_lattice = exact_lattice; a.absolute = absolute;
+  TrackArgs a{};
+  a.frames = frames; a.cand_frame = cand_frame; a.level = level; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
+  a.refs = refs; a.n_refs = n_refs; a.cand = cand; a.out = out; a.n_cand = n_cand; a.tol = tol; a.step = step;
+  a.offset = offset; a.exact_lattice = exact_lattice; a.absolute = absolute;
   constexpr size_t kTileBytes = 64 * 1024;
   a.tile_cap = int(kTileBytes / sizeof(uint32_t));
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (lv->kind == NCC_U8) hipLaunchKernelGGL(k_track<uint8_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
-  else if (lv->kind == NCC_U16) hipLaunchKernelGGL(k_track<uint16_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
+  if (level_kind == NCC_U8) hipLaunchKernelGGL(k_track<uint8_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
+  else if (level_kind == NCC_U16) hipLaunchKernelGGL(k_track<uint16_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
   else hipLaunchKernelGGL(k_track<uint32_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 
 namespace {
-RefineArgs refine_args(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand) {
-  RefineArgs a;
-  a.lv = p->level; a.dc_src = p->dc_src; a.dc = 0; a.sr = p->storage[0]; a.sc = p->storage[1]; a.sh = p->storage[2];
-  a.sw = p->storage[3]; a.crop_r = p->crop_r; a.crop_c = p->crop_c; a.scale = p->scale; a.phase = p->phase;
+RefineArgs refine_args(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
+                       const int32_t* cand_frame, int n_cand) {
+  RefineArgs a{};
+  a.frames = p->frames; a.cand_frame = cand_frame; a.scale = p->scale; a.phase = p->phase;
   a.finest = p->finest; a.top = p->top; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.orig_ref = p->orig_ref;
   a.lv_orow = p->level_orow; a.lv_ocol = p->level_ocol; a.refs0 = p->refs0; a.n_refs0 = p->n_refs0;
   a.step = p->step; a.a_start = p->angle_start; a.a_end = p->angle_end; a.min_score = p->min_score;
-  a.subpixel = p->subpixel; a.rot = p->rotate; a.absolute = p->absolute; a.img_h = p->image_h; a.img_w = p->image_w;
-  a.H0 = p->level.height; a.W0 = p->level.width;
-  a.cand = cand; a.n_cand = n_cand; a.out_full = nullptr; a.out_refined = nullptr; a.verdict = nullptr;
-  a.tile = nullptr; a.tr0 = a.tc0 = a.th = a.tw = 0;
+  a.subpixel = p->subpixel; a.rot = p->rotate; a.absolute = p->absolute;
+  a.cand = cand; a.n_cand = n_cand;
   return a;
 }
 }  // namespace
 
 extern "C" int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
-                                 int n_cand, NccMatch* out_full, void* stream) {
+                                 const int32_t* cand_frame, int n_cand, NccMatch* out_full, void* stream) {
   if (n_cand <= 0) return 0;
-  RefineArgs a = refine_args(p, pools, cand, n_cand);
+  RefineArgs a = refine_args(p, pools, cand, cand_frame, n_cand);
   a.out_full = out_full;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 g(n_cand), b(kRefineThreads);
-  if (p->level.kind == NCC_U8) hipLaunchKernelGGL(k_full_score<uint8_t>, g, b, kRefineTileBytes, st, a);
-  else if (p->level.kind == NCC_U16) hipLaunchKernelGGL(k_full_score<uint16_t>, g, b, kRefineTileBytes, st, a);
+  if (p->level_kind == NCC_U8) hipLaunchKernelGGL(k_full_score<uint8_t>, g, b, kRefineTileBytes, st, a);
+  else if (p->level_kind == NCC_U16) hipLaunchKernelGGL(k_full_score<uint16_t>, g, b, kRefineTileBytes, st, a);
   else hipLaunchKernelGGL(k_full_score<uint32_t>, g, b, kRefineTileBytes, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 
-extern "C" int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand, int n_cand,
-                             NccMatch* out_refined, int32_t* verdict, void* stream) {
+extern "C" int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
+                             const int32_t* cand_frame, int n_cand, NccMatch* out_refined, int32_t* verdict,
+                             void* stream) {
   if (n_cand <= 0) return 0;
-  RefineArgs a = refine_args(p, pools, cand, n_cand);
+  RefineArgs a = refine_args(p, pools, cand, cand_frame, n_cand);
   a.out_refined = out_refined;
   a.verdict = verdict;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 g(n_cand), b(kRefineThreads);
-  if (p->level.kind == NCC_U8) hipLaunchKernelGGL(k_refine<uint8_t>, g, b, kRefineTileBytes, st, a);
-  else if (p->level.kind == NCC_U16) hipLaunchKernelGGL(k_refine<uint16_t>, g, b, kRefineTileBytes, st, a);
+  if (p->level_kind == NCC_U8) hipLaunchKernelGGL(k_refine<uint8_t>, g, b, kRefineTileBytes, st, a);
+  else if (p->level_kind == NCC_U16) hipLaunchKernelGGL(k_refine<uint16_t>, g, b, kRefineTileBytes, st, a);
   else hipLaunchKernelGGL(k_refine<uint32_t>, g, b, kRefineTileBytes, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

@@ -160,11 +160,13 @@ def _finish(img: np.ndarray, rng: SplitMix64, sigma: float, gain: bool) -> np.nd
 
 def scene(width: int, height: int, tw: int, th: int, count: int, angle_lo: float, angle_hi: float,
           noise: float, gain: bool, seed: int, integer_positions: bool = False,
-          min_separation: float | None = None) -> Scene:
-    """Generic scene: ``count`` rotated pastes of one template."""
+          min_separation: float | None = None, template_seed: int | None = None) -> Scene:
+    """Generic scene: ``count`` rotated pastes of one template (drawn from
+    ``template_seed``, by default the scene seed)."""
     rng = SplitMix64(seed)
     base = blurred_noise(rng, height, width).astype(np.float64)
-    tmpl = make_template(SplitMix64(seed ^ 0x5EED5EED), th, tw)
+    tseed = seed if template_seed is None else template_seed
+    tmpl = make_template(SplitMix64(tseed ^ 0x5EED5EED), th, tw)
     diag = math.hypot(th, tw)
     sep = min_separation if min_separation is not None else 1.5 * diag
     margin = diag / 2 + 2
@@ -200,11 +202,12 @@ def workload_plain_640(seed_k: int = 0) -> Scene:
     return scene(640, 480, 64, 64, 3, 0.0, 0.0, 5.0, False, SEED_BASE + seed_k, integer_positions=True)
 
 
-def workload_rot_2048(seed_k: int = 1) -> Scene:
+def workload_rot_2048(seed_k: int = 1, frame: int = 0) -> Scene:
     """2048x2048 image, 128x128 template pasted 5x at subpixel centres with
-    rotation in [-30, 30] degrees, gain ramp 0.8-1.2 and N(0,8)."""
-    return scene(2048, 2048, 128, 128, 5, -math.pi / 6, math.pi / 6, 8.0, True, SEED_BASE + seed_k,
-                 min_separation=1.2 * math.hypot(128, 128))
+    rotation in [-30, 30] degrees, gain ramp 0.8-1.2 and N(0,8).  Frames
+    k > 0 are further scenes of the same template (other placements, noise)."""
+    return scene(2048, 2048, 128, 128, 5, -math.pi / 6, math.pi / 6, 8.0, True, SEED_BASE + seed_k + 1000 * frame,
+                 min_separation=1.2 * math.hypot(128, 128), template_seed=SEED_BASE + seed_k)
 
 
 def workload_full_2048(seed_k: int = 2) -> Scene:
@@ -213,8 +216,9 @@ def workload_full_2048(seed_k: int = 2) -> Scene:
                  min_separation=1.2 * math.hypot(256, 256))
 
 
-def small_rot(width=256, height=192, t=32, count=2, extent_deg=20.0, seed_k=10, noise=4.0) -> Scene:
-    """Small rotated scene for oracle-speed parity tests."""
+def small_rot(width=256, height=192, t=32, count=2, extent_deg=20.0, seed_k=10, noise=4.0, frame: int = 0) -> Scene:
+    """Small rotated scene for oracle-speed parity tests (frames k > 0: same
+    template, other placements and noise)."""
     e = math.radians(extent_deg)
-    return scene(width, height, t, t, count, -e, e, noise, True, SEED_BASE + seed_k,
-                 min_separation=1.3 * math.hypot(t, t))
+    return scene(width, height, t, t, count, -e, e, noise, True, SEED_BASE + seed_k + 1000 * frame,
+                 min_separation=1.3 * math.hypot(t, t), template_seed=SEED_BASE + seed_k)

@@ -222,3 +222,28 @@ def test_device_resident_find_matches_host(capi, scene_rot):
     wide[:, :2048] = img
     c, _ = m.find_device(wide.data_ptr(), 2048, 2048, 2304, opts)
     assert np.array_equal(a, c)
+
+
+def test_batched_find_equals_single_finds(capi, oracle):
+    """A batch of frames of different sizes (and one without matches) returns,
+    frame by frame, exactly what single finds return, and matches the oracle."""
+    say("batched find: 4 frames")
+    from mvkit_amd import synth
+    scenes = [synth.small_rot(frame=k) for k in range(3)]
+    m = capi.NccModel(scenes[0].template, "auto", -math.radians(20), math.radians(40))
+    opts = capi.NccModel.options(-math.radians(20), math.radians(40), 0.6, 0, 0.5)
+    imgs = [s.image for s in scenes] + [np.full((100, 140), 90, np.uint8)]
+    imgs[1] = np.ascontiguousarray(imgs[1][:170, :230])  # another size
+    dev = [torch.from_numpy(i).cuda() for i in imgs]
+    h = [i.shape[0] for i in imgs]
+    w = [i.shape[1] for i in imgs]
+    batch, _ = m.find_batch([d.data_ptr() for d in dev], w, h, w, opts)
+    assert len(batch) == len(imgs)
+    for k, d in enumerate(dev):
+        single, _ = m.find_device(d.data_ptr(), w[k], h[k], w[k], opts)
+        assert np.array_equal(batch[k], single), (k, batch[k], single)
+    assert len(batch[3]) == 0
+    om = oracle.Model(scenes[0].template, levels=0, start=-math.radians(20), extent=math.radians(40))
+    for k in range(3):
+        want = om.find(imgs[k], -math.radians(20), math.radians(40), 0.6, 0, 0.5)
+        check_matches(batch[k], np.asarray(want).reshape(-1, 4), f"frame {k}")
