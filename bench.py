@@ -65,7 +65,6 @@ def main():
     torch.cuda.synchronize()
     form = FORMS[args.form]
     opts = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form)
-    prof = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form, profile=True)
 
     # parity guard: the benchmarked path must reproduce the truth of frame 0
     res, st = model.find_device(frames[0].data_ptr(), 2048, 2048, 2048, opts)
@@ -76,45 +75,62 @@ def main():
 
     ptrs = [f.data_ptr() for f in frames]
     latencies = []
-    kernel = {k: 0.0 for k in capi.STAGES}
-    coarse_flops = [0.0]
 
-    def step(o, profile):
+    def step(o):
         t = time.perf_counter()
         out, st = model.find_batch(ptrs, 2048, 2048, 2048, o)
         latencies.append(time.perf_counter() - t)
-        if profile:
-            for i, name in enumerate(capi.STAGES):
-                kernel[name] += st.kernel_us[i]
-            coarse_flops[0] += st.coarse_flops
-        return out
+        return out, st
 
     # the batch must return, frame by frame, what a single find returns
-    batch0 = step(opts, False)
+    batch0, _ = step(opts)
     assert len(batch0) == F and np.array_equal(batch0[0], res), (batch0[0], res)
 
     log(f"warmup {args.warmup} steps x {F} frames")
     for _ in range(args.warmup):
-        step(opts, False)
+        step(opts)
+
+    # per-stage device time from a short pass with every stage timed
+    # (an event pair around each stage's launches on the library's stream)
+    kernel = {k: 0.0 for k in capi.STAGES}
+    prof_steps = max(1, min(args.steps, 5))
+    for _ in range(prof_steps):
+        _, st = step(capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form, profile=True))
+        for i, name in enumerate(capi.STAGES):
+            kernel[name] += st.kernel_us[i] / (prof_steps * F)
+    dominant = max(kernel, key=kernel.get)
+    dom_bit = 1 << capi.STAGES.index(dominant)
+
+    # timed region: only the dominant stage carries events, for its roofline
     latencies.clear()
     torch.cuda.synchronize()
-    log(f"timed {args.steps} steps x {F} frames, form {args.form}")
+    log(f"timed {args.steps} steps x {F} frames, form {args.form}, events on {dominant}")
+    timed = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form, profile=dom_bit)
+    dom_us, flops = 0.0, 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(prof, True)
+        _, st = step(timed)
+        dom_us += st.kernel_us[capi.STAGES.index(dominant)]
+        flops += st.track_flops if dominant == "track" else st.coarse_flops
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     frames_done = args.steps * F
     mpx = 2048 * 2048 * frames_done / elapsed / 1e6
     lat = np.array(latencies) * 1e3
 
-    # roofline of the coarse correlation (the path's dense contraction)
-    coarse_us = kernel["coarse"] / frames_done
-    flops = coarse_flops[0] / frames_done
-    achieved = flops / (coarse_us * 1e-6) / 1e12 if coarse_us > 0 else 0.0
-    peak = {0: 157.3, 1: 2500.0, 2: 78.6, 3: 5000.0}[form]
+    # roofline of the dominant stage: the coarse correlation (FP32 VALU or
+    # BF16/FP8 MFMA) or tracking (FP64 VALU); per frame, device time from the
+    # timed region's events
+    per_frame_us = dom_us / frames_done
+    per_frame_flop = flops / frames_done
+    achieved = per_frame_flop / (per_frame_us * 1e-6) / 1e12 if per_frame_us > 0 else 0.0
+    if dominant == "track":
+        bound, peak = "valu-fp64", 78.6
+    elif dominant == "coarse":
+        bound, peak = ("mfma" if form in (1, 3) else "valu"), {0: 157.3, 1: 2500.0, 2: 78.6, 3: 5000.0}[form]
+    else:
+        bound, peak = "latency", None
     total_kernel = sum(kernel.values())
-    dominant = max(kernel, key=kernel.get)
 
     cpu = None
     if args.cpu_finds > 0:
@@ -141,11 +157,10 @@ def main():
         "dtype": {0: "fp32", 1: "bf16x3", 2: "f64", 3: "fp8+bf16x3"}[form] + " correlation / f64 refinement",
         "form": args.form, "data": "synthetic (deterministic SplitMix64 generator)",
         "config": WORKLOAD,
-        "roofline": {"kernel": "coarse correlation", "bound": "mfma" if form in (1, 3) else "valu",
-                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 5), "traffic": None,
-                     "launch_us": round(coarse_us, 2), "algorithmic_gflop_per_launch": round(flops / 1e9, 4)},
-        "kernel_us_per_frame": {k: round(v / frames_done, 2) for k, v in kernel.items()},
+        "roofline": {"kernel": dominant, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 5) if peak else None, "traffic": None,
+                     "us_per_frame": round(per_frame_us, 2), "algorithmic_gflop_per_frame": round(per_frame_flop / 1e9, 4)},
+        "kernel_us_per_frame": {k: round(v, 2) for k, v in kernel.items()},
         "dominant_kernel": {"stage": dominant, "share": round(kernel[dominant] / total_kernel, 3) if total_kernel else 0},
         "cpu_baseline": cpu,
     }

@@ -26,7 +26,8 @@ typedef struct MvAmdFindOptions {
   int32_t highest_level, lowest_level;  /* 1-based; highest 0 = all */
   int32_t form;                          /* MVAMD_FORM_* */
   double screen;                         /* FP8 screen threshold (form FP8_SCREEN) */
-  int32_t profile;                       /* time each kernel stage with HIP events */
+  int32_t profile;                       /* bit s: time stage s (MVAMD_STAGE_*) with HIP events;
+                                            MVAMD_PROFILE_ALL times every stage */
 } MvAmdFindOptions;
 
 /* Kernel stages reported in MvAmdFindStats.kernel_us. */
@@ -38,13 +39,15 @@ typedef struct MvAmdFindOptions {
 #define MVAMD_STAGE_FULL 5
 #define MVAMD_STAGE_REFINE 6
 #define MVAMD_STAGE_COUNT 7
+#define MVAMD_PROFILE_ALL 0x7f
 
 typedef struct MvAmdFindStats {
   int32_t n_coarse, n_prepared, top, finest, angle_stride;
   int32_t storage[4];
   double t_coarse_us, t_track_us, t_refine_us, t_total_us;
-  double kernel_us[MVAMD_STAGE_COUNT];   /* device time per stage (profile != 0) */
+  double kernel_us[MVAMD_STAGE_COUNT];   /* device time of each profiled stage */
   double coarse_flops;                   /* algorithmic FLOPs of the coarse correlation */
+  double track_flops;                    /* algorithmic FLOPs of tracking (when its stage is profiled) */
 } MvAmdFindStats;
 
 /* Find on a u8 image given as a HIP device pointer on the calling thread's

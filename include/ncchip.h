@@ -65,16 +65,27 @@ int ncchip_coarse_mfma(const NccLevel* lv, const uint8_t* dc_src, double C, cons
 int ncchip_peaks(const double* planes, int n_angles, int h, int w, double threshold, NccPeak* out, int cap,
                  int32_t* count, void* stream);
 
+/* Copies the peak counts of n_frames frames and each frame's first
+ * min(count, cap) peaks (frame f's at peaks + f * cap) into host-visible
+ * memory (pinned, coherent host allocations), laid out the same way. */
+int ncchip_export_peaks(const int32_t* counts, const NccPeak* peaks, int n_frames, int cap, int32_t* host_counts,
+                        NccPeak* host_peaks, void* stream);
+
 /* Coarse-to-fine tracking of every candidate at one level: the first strict
  * maximum over (query patch within `tol` of the candidate's angle, 5x5
  * anchors).  Candidates may come from several frames: `frames` (device) holds
  * one NccFrame per frame and `cand_frame` (device) each candidate's frame;
  * `level` is the tracked level and `level_kind` its pixel kind.  `refs` lists
- * the level's query patches as (pool << 24 | index).
+ * the level's query patches as (pool << 24 | index).  `tile_pixels` bounds
+ * the pixels of a candidate's neighbourhood (the union of its windows),
+ * staged in LDS when it fits in 64 KiB (0: read the level from HBM);
+ * exact_f64 != 0 asserts n * (max |S - D|)^2 < 2^53 for every query patch,
+ * which lets the moments accumulate exactly in FP64.
  * Replaces matcher.cpp:670-756 (kernels_avx2.cpp:118-238). */
 int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, int level, int level_kind,
                  const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* cand, NccMatch* out,
-                 int n_cand, double tol, double step, double offset, int exact_lattice, int absolute, void* stream);
+                 int n_cand, double tol, double step, double offset, int exact_lattice, int absolute,
+                 int tile_pixels, int exact_f64, void* stream);
 
 typedef struct NcchipRefineParams {
   const NccFrame* frames;  /* per-frame descriptors (device) */

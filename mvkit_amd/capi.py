@@ -32,7 +32,7 @@ PUBLIC_SYMBOLS = [
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_coarse", "ncchip_coarse_mfma",
-                  "ncchip_peaks", "ncchip_track", "ncchip_full_score", "ncchip_refine"]
+                  "ncchip_peaks", "ncchip_export_peaks", "ncchip_track", "ncchip_full_score", "ncchip_refine"]
 
 
 class StringView(C.Structure):
@@ -58,8 +58,10 @@ class FindStats(C.Structure):
     _fields_ = [("n_coarse", C.c_int32), ("n_prepared", C.c_int32), ("top", C.c_int32), ("finest", C.c_int32),
                 ("angle_stride", C.c_int32), ("storage", C.c_int32 * 4), ("t_coarse_us", C.c_double),
                 ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double),
-                ("kernel_us", C.c_double * 7), ("coarse_flops", C.c_double)]
+                ("kernel_us", C.c_double * 7), ("coarse_flops", C.c_double),
+                ("track_flops", C.c_double)]
 
+PROFILE_ALL = 0x7f
 STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine"]
 
 
@@ -268,9 +270,12 @@ class NccModel:
 
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
-                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=False) -> FindOptions:
+                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0) -> FindOptions:
+        """profile: True times every stage, an int is a mask of stage bits
+        (bit i = STAGES[i])."""
+        mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
-                           lowest, form, screen, int(profile))
+                           lowest, form, screen, mask)
 
     def find_device(self, d_image: int, width: int, height: int, pitch: int, opts: FindOptions, cap: int = 4096):
         """Find on a device-resident u8 image (integer device address)."""

@@ -254,7 +254,7 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
   q.num_matches = size_t(o->num_matches);
   q.subpixel = o->subpixel != 0;
   q.highest = o->highest_level; q.lowest = std::max(1, o->lowest_level);
-  q.form = o->form; q.screen = o->screen; q.profile = o->profile != 0;
+  q.form = o->form; q.screen = o->screen; q.profile = o->profile;
   if (q.form < MVAMD_FORM_FP32 || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
   return q;
@@ -562,6 +562,7 @@ void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
   s->t_total_us = t.t_total_us;
   for (int k = 0; k < MVAMD_STAGE_COUNT; ++k) s->kernel_us[k] = t.kernel_us[k];
   s->coarse_flops = t.coarse_flops;
+  s->track_flops = t.track_flops;
 }
 void put_poses(const std::vector<mvamd::Pose>& v, double* dst, size_t cap) {
   for (size_t i = 0; i < v.size() && i < cap; ++i) {
@@ -621,8 +622,10 @@ extern "C" MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* c
     if (stats && n) {
       fill_stats(traces[0], stats);
       double flops = 0;
-      for (const auto& t : traces) flops += t.coarse_flops;
+      double tflops = 0;
+      for (const auto& t : traces) { flops += t.coarse_flops; tflops += t.track_flops; }
       stats->coarse_flops = flops;
+      stats->track_flops = tflops;
     }
   });
 }

@@ -246,9 +246,30 @@ __global__ void __launch_bounds__(256) k_peaks(const double* __restrict__ planes
   }
 }
 
+// Copies every frame's peak count and its first min(count, cap) peaks to
+// host-visible memory (frame f's peaks at f * cap), so one stream
+// synchronisation returns all frames' peaks.
+__global__ void __launch_bounds__(256) k_export_peaks(const int32_t* __restrict__ counts,
+                                                      const NccPeak* __restrict__ peaks, int cap,
+                                                      int32_t* __restrict__ out_counts, NccPeak* __restrict__ out) {
+  const int f = blockIdx.y;
+  const int n = min(counts[f], cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) out_counts[f] = counts[f];
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    out[size_t(f) * cap + i] = peaks[size_t(f) * cap + i];
+}
+
 }  // namespace ncc
 
 using namespace ncc;
+
+extern "C" int ncchip_export_peaks(const int32_t* counts, const NccPeak* peaks, int n_frames, int cap,
+                                   int32_t* host_counts, NccPeak* host_peaks, void* stream) {
+  if (n_frames <= 0) return 0;
+  hipLaunchKernelGGL(k_export_peaks, dim3(8, n_frames), dim3(256), 0, static_cast<hipStream_t>(stream), counts, peaks,
+                     cap, host_counts, host_peaks);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
 
 extern "C" int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1,
                              const uint64_t* p2, const NccPatchPool* pools, const int32_t* patch_ids,

@@ -124,7 +124,7 @@ void PinnedBuffer::reserve(size_t n) {
   if (ptr) (void)hipHostFree(ptr);
   ptr = nullptr; bytes = 0;
   n = std::max<size_t>(align_up(n, 1 << 16), 1 << 16);
-  check(hipHostMalloc(&ptr, n, 0), "hipHostMalloc");
+  check(hipHostMalloc(&ptr, n, hipHostMallocCoherent | hipHostMallocMapped), "hipHostMalloc");
   bytes = n;
 }
 
@@ -202,7 +202,8 @@ FindContext::~FindContext() {
 }
 
 void FindContext::time_begin(int stage) {
-  if (!profiling_) return;
+  timing_ = profiling_ & (1 << stage);
+  if (!timing_) return;
   if (timers_used_ == timers_.size()) {
     Timer t;
     check(hipEventCreate(&t.begin), "hipEventCreate");
@@ -213,7 +214,7 @@ void FindContext::time_begin(int stage) {
   check(hipEventRecord(timers_[timers_used_].begin, stream_), "hipEventRecord");
 }
 void FindContext::time_end() {
-  if (!profiling_) return;
+  if (!timing_) return;
   check(hipEventRecord(timers_[timers_used_].end, stream_), "hipEventRecord");
   ++timers_used_;
 }
@@ -254,20 +255,20 @@ struct Job {
   struct Lv { int w, h, kind, pitch; size_t off, bytes; };
   std::vector<Lv> lv;
   std::vector<int32_t> ranges;
-  size_t off_p1 = 0, off_p2 = 0, off_nz = 0, off_planes = 0, off_peaks = 0, off_ranges = 0;
+  size_t off_p1 = 0, off_p2 = 0, off_nz = 0, off_planes = 0, off_ranges = 0;
   std::vector<Pose> cand, full, refined, results;
   std::vector<int> verdict;
   int32_t storage0[4] = {0, 0, 0, 0};
   size_t pos = 0, batch_size = 0;
   bool done = false;
-  double coarse_flops = 0;
+  double coarse_flops = 0, track_flops = 0;
 };
 }  // namespace
 
 std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, const FrameInput* inputs, size_t n_frames,
                                                        const FindOptions& req, FindTrace* traces) {
   const double t0 = now_us();
-  profiling_ = req.profile && traces;
+  profiling_ = traces ? req.profile : 0;
   timers_used_ = 0;
   for (size_t f = 0; f < n_frames && traces; ++f) traces[f] = FindTrace{};
   std::vector<std::vector<Pose>> results(n_frames);
@@ -402,13 +403,9 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     J.off_p2 = off; off = align_up(off + size_t(Wt + 1) * Ht * 8);
     J.off_nz = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
     J.off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
-    J.off_peaks = off; off = align_up(off + cap * sizeof(NccPeak));
   }
+  const size_t off_peaks = off; off = align_up(off + cap_all * sizeof(NccPeak));
   const size_t off_count = off; off = align_up(off + n_frames * 4);
-  const size_t off_cand = off; off = align_up(off + cap_all * sizeof(NccMatch));
-  const size_t off_cand_frame = off; off = align_up(off + cap_all * 4);
-  const size_t off_out = off; off = align_up(off + cap_all * sizeof(NccMatch));
-  const size_t off_verdict = off; off = align_up(off + cap_all * 4);
   const size_t off_frames = off; off = align_up(off + n_frames * sizeof(NccFrame));
   // small uploads: refs of every level, per-frame coarse ranges, query pool
   std::vector<size_t> off_refs(level_count);
@@ -423,8 +420,14 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     work_ = DeviceBuffer(align_up(off + off / 4, 1 << 20));
   }
   uint8_t* dev = work_.as<uint8_t>();
-  stage_.reserve(std::max({small, cap_all * (sizeof(NccMatch) + 4), n_frames * sizeof(NccFrame)}) + 256);
-  readback_.reserve(cap_all * (sizeof(NccMatch) + 4) + cap * sizeof(NccPeak) + n_frames * 4 + 64);
+  // Host-visible (pinned, coherent) staging: candidate lists are read by the
+  // kernels in place and per-candidate results are written back in place, so
+  // a stage costs one launch and one synchronisation.  A list is rewritten
+  // only after the synchronisation that ends the launch reading it.
+  const size_t stage_frames = align_up(cap_all * (sizeof(NccMatch) + 4));
+  stage_.reserve(std::max(small, stage_frames + n_frames * sizeof(NccFrame)) + 256);
+  const size_t rb_peaks = align_up(cap_all * (sizeof(NccMatch) + 4));
+  readback_.reserve(rb_peaks + cap_all * sizeof(NccPeak) + n_frames * 4 + 64);
   {
     uint8_t* st = static_cast<uint8_t*>(stage_.ptr);
     for (size_t l = 0; l < level_count; ++l)
@@ -482,25 +485,25 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     if (rc) raise(kInternalError, "device", "coarse launch failed");
     time_end();
     time_begin(kStagePeaks);
-    if (ncchip_peaks(planes, int(A), Ht, Wt, o.min_score * .4, reinterpret_cast<NccPeak*>(dev + J.off_peaks), int(cap),
-                     counts + f, stream_))
+    if (ncchip_peaks(planes, int(A), Ht, Wt, o.min_score * .4,
+                     reinterpret_cast<NccPeak*>(dev + off_peaks) + f * cap, int(cap), counts + f, stream_))
       raise(kInternalError, "device", "peak launch failed");
     time_end();
   }
   auto* rb = static_cast<uint8_t*>(readback_.ptr);
-  check(hipMemcpyAsync(rb, counts, n_frames * 4, hipMemcpyDeviceToHost, stream_), "readback");
+  auto* hpeaks = reinterpret_cast<NccPeak*>(rb + rb_peaks);
+  auto* hcounts = reinterpret_cast<int32_t*>(rb + rb_peaks + cap_all * sizeof(NccPeak));
+  if (ncchip_export_peaks(counts, reinterpret_cast<const NccPeak*>(dev + off_peaks), int(n_frames), int(cap), hcounts,
+                          hpeaks, stream_))
+    raise(kInternalError, "device", "peak export launch failed");
   check(hipStreamSynchronize(stream_), "coarse");
-  std::vector<int32_t> n_peaks(reinterpret_cast<int32_t*>(rb), reinterpret_cast<int32_t*>(rb) + n_frames);
+  const std::vector<int32_t> n_peaks(hcounts, hcounts + n_frames);
   for (size_t f = 0; f < n_frames; ++f)
     if (size_t(n_peaks[f]) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
   for (size_t f = 0; f < n_frames; ++f) {
     Job& J = jobs[f];
     if (n_peaks[f] <= 0) continue;
-    check(hipMemcpyAsync(rb, dev + J.off_peaks, size_t(n_peaks[f]) * sizeof(NccPeak), hipMemcpyDeviceToHost, stream_),
-          "readback");
-    check(hipStreamSynchronize(stream_), "coarse");
-    std::vector<NccPeak> pk(static_cast<size_t>(n_peaks[f]));
-    std::memcpy(pk.data(), rb, pk.size() * sizeof(NccPeak));
+    std::vector<NccPeak> pk(hpeaks + f * cap, hpeaks + f * cap + size_t(n_peaks[f]));
     // the reference emits peaks in (angle, row, col) order
     std::sort(pk.begin(), pk.end(), [](const NccPeak& a, const NccPeak& b) {
       if (a.angle != b.angle) return a.angle < b.angle;
@@ -535,7 +538,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   }
   // frame descriptors for the per-candidate kernels
   {
-    auto* fr = static_cast<NccFrame*>(stage_.ptr);
+    auto* fr = reinterpret_cast<NccFrame*>(static_cast<uint8_t*>(stage_.ptr) + stage_frames);
     for (size_t f = 0; f < n_frames; ++f) {
       const Job& J = jobs[f];
       NccFrame d{};
@@ -549,30 +552,21 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     check(hipMemcpyAsync(dev + off_frames, fr, n_frames * sizeof(NccFrame), hipMemcpyHostToDevice, stream_), "upload");
   }
   const auto* dframes = reinterpret_cast<const NccFrame*>(dev + off_frames);
-  auto* dcand = reinterpret_cast<NccMatch*>(dev + off_cand);
-  auto* dcand_frame = reinterpret_cast<int32_t*>(dev + off_cand_frame);
-  auto* dout = reinterpret_cast<NccMatch*>(dev + off_out);
-  auto* dverdict = reinterpret_cast<int32_t*>(dev + off_verdict);
-  // gathers (frame, candidate) lists into one upload; the staging copy is
-  // synchronous with respect to the next readback sync of this stream
+  // candidate lists and per-candidate results live in host-visible memory
+  auto* dcand = static_cast<NccMatch*>(stage_.ptr);
+  int32_t* dcand_frame = nullptr;
+  auto* dout = reinterpret_cast<NccMatch*>(rb);
+  int32_t* dverdict = nullptr;
   auto upload = [&](const std::vector<std::pair<uint32_t, Pose>>& list) {
-    check(hipStreamSynchronize(stream_), "sync");  // the staging buffer is reused
-    auto* sm = static_cast<NccMatch*>(stage_.ptr);
-    auto* sf = reinterpret_cast<int32_t*>(sm + list.size());
+    dcand_frame = reinterpret_cast<int32_t*>(dcand + list.size());
+    dverdict = reinterpret_cast<int32_t*>(dout + list.size());
     for (size_t i = 0; i < list.size(); ++i) {
       const Pose& p = list[i].second;
-      sm[i] = {p.row, p.col, p.angle, p.score};
-      sf[i] = int32_t(list[i].first);
+      dcand[i] = {p.row, p.col, p.angle, p.score};
+      dcand_frame[i] = int32_t(list[i].first);
     }
-    check(hipMemcpyAsync(dcand, sm, list.size() * sizeof(NccMatch), hipMemcpyHostToDevice, stream_), "upload");
-    check(hipMemcpyAsync(dcand_frame, sf, list.size() * 4, hipMemcpyHostToDevice, stream_), "upload");
   };
-  auto readback = [&](size_t n, bool with_verdict) {
-    check(hipMemcpyAsync(rb, dout, n * sizeof(NccMatch), hipMemcpyDeviceToHost, stream_), "readback");
-    if (with_verdict)
-      check(hipMemcpyAsync(rb + n * sizeof(NccMatch), dverdict, n * 4, hipMemcpyDeviceToHost, stream_), "readback");
-    check(hipStreamSynchronize(stream_), "readback");
-  };
+  auto readback = [&](size_t, bool) { check(hipStreamSynchronize(stream_), "readback"); };
   auto result_at = [&](size_t i) {
     const NccMatch& m = reinterpret_cast<const NccMatch*>(rb)[i];
     return Pose{m.row, m.col, m.angle, m.score};
@@ -596,11 +590,33 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     if (list.empty()) break;
     const double step = std::ldexp(1., int(l - 1)), offset = (step - 1) * .5;
     const double tol = model.params.step * (l == top ? double(stride) : 1.) * 1.01;
+    // bound of a candidate's neighbourhood (the union of the 5x5 windows of
+    // every query patch) and of the FP64-exact moment sums
+    double lo_r = 1e300, hi_r = -1e300, lo_c = 1e300, hi_c = -1e300, n_max = 0;
+    for (const int32_t r : refs[l - 1]) {
+      const Raster& p = raster_of(r);
+      lo_r = std::min(lo_r, p.rmin - p.orow); hi_r = std::max(hi_r, p.rmax - p.orow);
+      lo_c = std::min(lo_c, p.cmin - p.ocol); hi_c = std::max(hi_c, p.cmax - p.ocol);
+      n_max = std::max(n_max, double(p.count));
+    }
+    const double tile_px = (std::ceil(hi_r - lo_r) + 7) * (std::ceil(hi_c - lo_c) + 7);
+    const double xmax = 255. * std::ldexp(1., 2 * int(l - 1));
+    const int exact_f64 = n_max * xmax * xmax < 0x1p53;
+    if (profiling_ & (1 << kStageTrack)) {
+      // algorithmic work: 2 * samples over the 5x5 windows of every query
+      // patch within the angle window (windows off the level included)
+      std::vector<std::pair<double, double>> an;
+      for (const int32_t r : refs[l - 1]) an.push_back({raster_of(r).angle, double(raster_of(r).count)});
+      for (const auto& [f, p] : list)
+        for (const auto& [ang, n] : an)
+          if (std::abs(ang - p.angle) <= tol) jobs[f].track_flops += 2.0 * 25 * n;
+    }
     upload(list);
     time_begin(kStageTrack);
     if (ncchip_track(dframes, dcand_frame, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                      reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()), dcand, dout,
-                     int(list.size()), tol, step, offset, l - 1 <= 2, model.params.absolute, stream_))
+                     int(list.size()), tol, step, offset, l - 1 <= 2, model.params.absolute,
+                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, stream_))
       raise(kInternalError, "device", "tracking launch failed");
     time_end();
     readback(list.size(), false);
@@ -709,6 +725,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
       std::copy(J.storage0, J.storage0 + 4, tr.storage);
       tr.prepared = J.full; tr.refined = J.refined; tr.verdict = J.verdict;
       tr.coarse_flops = J.coarse_flops;
+      tr.track_flops = J.track_flops;
       tr.t_coarse_us = t1 - t0; tr.t_track_us = t2 - t1; tr.t_refine_us = t3 - t2; tr.t_total_us = now_us() - t0;
       if (f > 0 && traces) std::copy(traces[0].kernel_us, traces[0].kernel_us + kStageCount, tr.kernel_us);
     }

@@ -65,7 +65,7 @@ struct FindOptions {
   int highest = 0, lowest = 1;
   int form = NCCHIP_FORM_FP32;     // coarse correlation form
   double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN)
-  bool profile = false;            // time each kernel with HIP events on the context's stream
+  int profile = 0;                 // bit s: time stage s (KernelStage) with HIP events on the context's stream
 };
 
 // Kernel stages timed in profiling mode.
@@ -84,6 +84,7 @@ struct FindTrace {
   double t_coarse_us = 0, t_track_us = 0, t_refine_us = 0, t_total_us = 0;
   double kernel_us[kStageCount] = {};  // device time per stage (profiling mode)
   double coarse_flops = 0;             // 2 * sum over evaluated coarse windows of their sample counts
+  double track_flops = 0;              // 2 * samples of the tracked 5x5 windows (counted when tracking is profiled)
 };
 
 // A u8 search image in device memory (`pitch` bytes per row).
@@ -123,7 +124,8 @@ class FindContext {
   void time_begin(int stage);
   void time_end();
   void time_collect(FindTrace* trace);
-  bool profiling_ = false;
+  int profiling_ = 0;   // stage mask of this find
+  bool timing_ = false; // the open stage is timed
   std::vector<Timer> timers_;
   size_t timers_used_ = 0;
   hipStream_t stream_ = nullptr;

@@ -5,6 +5,8 @@
 // coordinate, bilinear value and window moment is the reference's value.
 // Sums over samples are block tree reductions (the reference sums in 4 lanes),
 // so reduced totals may differ from the reference in the last bits.
+#include <type_traits>
+
 #include "ncc_common.hip.h"
 #include "../../include/ncchip.h"
 
@@ -65,8 +67,9 @@ __device__ __forceinline__ void bind_frame(TrackArgs& a, int ci) {
 
 // Pixel source of a tracking window: an LDS tile of the candidate's
 // neighbourhood, or the level in global memory when the tile is too large.
+template <class T>
 struct TileReader {
-  const uint32_t* tile;
+  const T* tile;
   int r0, c0, tw;
   __device__ __forceinline__ uint32_t at(int r, int c) const { return tile[(r - r0) * tw + (c - c0)]; }
 };
@@ -77,19 +80,16 @@ struct LevelReader {
   __device__ __forceinline__ uint32_t at(int r, int c) const { return data[size_t(r) * pitch + c]; }
 };
 
-// Score of one window from its dot product and exact integer moments, with
-// the reference's prefix-kernel normalisation and low-variance rule
-// (kernels_avx2.cpp:209-238, 128-129).
-__device__ __forceinline__ double window_score(const TrackArgs& a, const NccPatch& P, double dot, int64_t s1,
-                                               int64_t s2) {
-  double sum, sq;
-  exact_moments(s1, s2, P.n, a.D, a.lv.shift, sum, sq);
+// Score of one window from its dot product and the exact centred moments
+// t1 = sum(S - D), t2 = sum((S - D)^2), with the reference's prefix-kernel
+// normalisation and low-variance rule (kernels_avx2.cpp:209-238, 128-129).
+__device__ __forceinline__ double window_score(const TrackArgs& a, const NccPatch& P, double dot, int64_t t1,
+                                               int64_t t2) {
+  const double sum = ldexp(double(t1), -a.lv.shift), sq = ldexp(double(t2), -2 * a.lv.shift);
   const double cnt = double(P.n);
   const double var = sq - sum * sum / cnt;
   dot = dot - sum / cnt * P.wsum;
   if (!(var > 128 * __DBL_EPSILON__ * sq)) {
-    const int64_t t1 = s1 - int64_t(P.n) * a.D;
-    const int64_t t2 = s2 - 2 * a.D * s1 + int64_t(P.n) * a.D * a.D;
     if (a.exact_lattice || (__int128)t2 * P.n == (__int128)t1 * t1) return NCC_MISSING;
     const double v = ldexp(double((__int128)t2 * P.n - (__int128)t1 * t1), -2 * a.lv.shift) / cnt;
     const double sc = clamp1(dot / sqrt(v));
@@ -106,10 +106,14 @@ __device__ __forceinline__ bool window_inside(const TrackArgs& a, const NccPatch
 
 // The five windows (row, col-2 .. col+2) of one template row offset, evaluated
 // by one wave with lanes striding over the samples, so each sample's position
-// and weight are loaded once for the five windows.  Exact integer moments;
-// the dot products are summed in a different order than the reference's
-// single accumulator, so scores agree to ~1e-15.  Every lane gets the scores.
-template <class Reader>
+// and weight are loaded once for the five windows.  Per sample and window the
+// centred value x = S - D (an integer) feeds the dot product and both moments.
+// Acc = double keeps the moments exact while every partial sum stays below
+// 2^53 (the host checks n * max|x|^2 for the level); Acc = int64_t otherwise.
+// The dot product is sum(x * w) scaled by 4^-l at the end, the reference's
+// per-sample products up to a power of two; the sums run in a different order
+// than the reference's single accumulator, so scores agree to ~1e-15.
+template <class Acc, class Reader>
 __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p, int row, int col, const Reader& rd,
                                           double (&score)[5]) {
   const NccPatch& P = p.P;
@@ -124,32 +128,52 @@ __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p,
   }
   // windows partly outside read clamped columns and are discarded below
   const int lane = threadIdx.x & 63;
-  const double unit = ldexp(1.0, -a.lv.shift);
-  int64_t s1[5] = {0, 0, 0, 0, 0}, s2[5] = {0, 0, 0, 0, 0};
+  Acc t1[5] = {0, 0, 0, 0, 0}, t2[5] = {0, 0, 0, 0, 0};
   double dot[5] = {0, 0, 0, 0, 0};
   int kmin = 0, kmax = 4;
   while (!in[kmin]) ++kmin;
   while (!in[kmax]) --kmax;
-  constexpr int U = 8;
+  const int32_t Di = int32_t(a.D);
+  constexpr int U = 4;
+  // Branch-free over the U samples of a chunk: samples past the end re-read
+  // the chunk's first sample with x forced to 0, so all 5U loads issue before
+  // the first wait.  Weights are 0 when the dot product uses weight runs.
+  const bool dense_w = !P.use_wruns;
   for (int base = lane; base < P.n; base += 64 * U) {
     int32_t rcv[U];
     double wv[U];
+    bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = base + 64 * u;
-      rcv[u] = i < P.n ? p.rc[i] : 0;
-      wv[u] = i < P.n && !P.use_wruns ? p.w[i] : 0.0;
+      ok[u] = i < P.n;
+      const int j = ok[u] ? i : base;
+      rcv[u] = p.rc[j];
+      const double w = p.w[j];
+      wv[u] = ok[u] && dense_w ? w : 0.0;
+    }
+    uint32_t v[U][5];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = row + sample_row(rcv[u]), c = col + sample_col(rcv[u]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v[u][k] = rd.at(r, c - 2 + min(max(k, kmin), kmax));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (base + 64 * u >= P.n) break;
-      const int r = row + sample_row(rcv[u]), c = col + sample_col(rcv[u]);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        const uint32_t v = rd.at(r, c - 2 + min(max(k, kmin), kmax));
-        s1[k] += v;
-        s2[k] += int64_t(v) * v;
-        if (!P.use_wruns) dot[k] += (double(v) * unit - a.dc) * wv[u];
+        if constexpr (std::is_same<Acc, double>::value) {
+          const double x = ok[u] ? double(int32_t(v[u][k]) - Di) : 0.0;
+          t1[k] += x;
+          t2[k] = fma(x, x, t2[k]);
+          dot[k] = fma(x, wv[u], dot[k]);
+        } else {
+          const int64_t x = ok[u] ? int64_t(v[u][k]) - a.D : 0;
+          t1[k] += x;
+          t2[k] += x * x;
+          dot[k] = fma(double(x), wv[u], dot[k]);
+        }
       }
     }
   }
@@ -162,15 +186,23 @@ __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p,
         const int cc = col - 2 + min(max(k, kmin), kmax);
         int64_t rs = 0;
         for (int c = run.y; c <= run.z; ++c) rs += rd.at(row + run.x, cc + c);
-        dot[k] += double(rs - int64_t(run.z - run.y + 1) * a.D) * unit * w;
+        dot[k] += double(rs - int64_t(run.z - run.y + 1) * a.D) * w;
       }
     }
   }
+  const double unit = ldexp(1.0, -a.lv.shift);
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    const int64_t t1 = wave_sum_i64(s1[k]), t2 = wave_sum_i64(s2[k]);
-    const double d = wave_sum(dot[k]);
-    score[k] = in[k] ? window_score(a, P, d, t1, t2) : NCC_MISSING;
+    int64_t T1, T2;
+    if constexpr (std::is_same<Acc, double>::value) {
+      T1 = int64_t(wave_sum(t1[k]));
+      T2 = int64_t(wave_sum(t2[k]));
+    } else {
+      T1 = wave_sum_i64(t1[k]);
+      T2 = wave_sum_i64(t2[k]);
+    }
+    const double d = wave_sum(dot[k]) * unit;
+    score[k] = in[k] ? window_score(a, P, d, T1, T2) : NCC_MISSING;
   }
 }
 
@@ -210,9 +242,10 @@ __device__ __forceinline__ void anchor(const TrackArgs& a, const NccMatch& cd, c
 // query patch within the angle window and keep the first strict maximum in the
 // reference's (patch, dr, dc) order (matcher.cpp:670-756).  Windows are
 // spread over the waves; the candidate's neighbourhood is staged in LDS.
-template <class T>
+template <class T, class Acc>
 __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
-  extern __shared__ uint32_t tile[];
+  extern __shared__ uint32_t tile_words[];
+  T* tile = reinterpret_cast<T*>(tile_words);
   __shared__ int elig[kMaxEligible];
   __shared__ int wave_cnt[kTrackThreads / 64];
   __shared__ int box[4];
@@ -246,11 +279,11 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   if (use_tile) {
     constexpr int U = 8;
     for (int base = threadIdx.x; base < th * tw; base += kTrackThreads * U) {
-      uint32_t v[U];
+      T v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int i = base + kTrackThreads * u;
-        v[u] = i < th * tw ? uint32_t(lvp[size_t(r0 + i / tw) * a.lv.pitch + c0 + i % tw]) : 0u;
+        v[u] = i < th * tw ? lvp[size_t(r0 + i / tw) * a.lv.pitch + c0 + i % tw] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -260,15 +293,19 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   __syncthreads();
   double best = NCC_MISSING;
   int best_i = 0x7fffffff;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (int task = wave; task < n_e * 5; task += kTrackThreads / 64) {
-    const PatchRef p = patch_ref(a.pools, a.refs[elig[task / 5]]);
+    // the task is wave-uniform: keep the patch descriptor in scalar registers
+    const int32_t ref = __builtin_amdgcn_readfirstlane(a.refs[__builtin_amdgcn_readfirstlane(elig[task / 5])]);
+    const PatchRef p = patch_ref(a.pools, ref);
     int row, col;
     anchor(a, cd, p.P, row, col);
+    row = __builtin_amdgcn_readfirstlane(row);
+    col = __builtin_amdgcn_readfirstlane(col);
     const int dr = task % 5 - 2;
     double sc[5];
-    if (use_tile) track_row(a, p, row + dr, col, TileReader{tile, r0, c0, tw}, sc);
-    else track_row(a, p, row + dr, col, LevelReader<T>{lvp, a.lv.pitch}, sc);
+    if (use_tile) track_row<Acc>(a, p, row + dr, col, TileReader<T>{tile, r0, c0, tw}, sc);
+    else track_row<Acc>(a, p, row + dr, col, LevelReader<T>{lvp, a.lv.pitch}, sc);
 #pragma unroll
     for (int k = 0; k < 5; ++k)
       if (sc[k] > best) { best = sc[k]; best_i = (task / 5) * 25 + (dr + 2) * 5 + k; }
@@ -840,18 +877,28 @@ using namespace ncc;
 extern "C" int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, int level, int level_kind,
                             const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* cand,
                             NccMatch* out, int n_cand, double tol, double step, double offset, int exact_lattice,
-                            int absolute, void* stream) {
+                            int absolute, int tile_pixels, int exact_f64, void* stream) {
   if (n_cand <= 0) return 0;
   TrackArgs a{};
   a.frames = frames; a.cand_frame = cand_frame; a.level = level; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
   a.refs = refs; a.n_refs = n_refs; a.cand = cand; a.out = out; a.n_cand = n_cand; a.tol = tol; a.step = step;
   a.offset = offset; a.exact_lattice = exact_lattice; a.absolute = absolute;
-  constexpr size_t kTileBytes = 64 * 1024;
-  a.tile_cap = int(kTileBytes / sizeof(uint32_t));
+  const size_t px = level_kind == NCC_U8 ? 1 : level_kind == NCC_U16 ? 2 : 4;
+  constexpr size_t kMaxTileBytes = 64 * 1024;
+  a.tile_cap = tile_pixels > 0 && size_t(tile_pixels) * px <= kMaxTileBytes ? tile_pixels : 0;
+  const size_t lds = (size_t(a.tile_cap) * px + 15) & ~size_t(15);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (level_kind == NCC_U8) hipLaunchKernelGGL(k_track<uint8_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
-  else if (level_kind == NCC_U16) hipLaunchKernelGGL(k_track<uint16_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
-  else hipLaunchKernelGGL(k_track<uint32_t>, dim3(n_cand), dim3(kTrackThreads), kTileBytes, st, a);
+  const dim3 g(n_cand), b(kTrackThreads);
+  if (level_kind == NCC_U8) {
+    if (exact_f64) hipLaunchKernelGGL((k_track<uint8_t, double>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_track<uint8_t, int64_t>), g, b, lds, st, a);
+  } else if (level_kind == NCC_U16) {
+    if (exact_f64) hipLaunchKernelGGL((k_track<uint16_t, double>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_track<uint16_t, int64_t>), g, b, lds, st, a);
+  } else {
+    if (exact_f64) hipLaunchKernelGGL((k_track<uint32_t, double>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_track<uint32_t, int64_t>), g, b, lds, st, a);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 
