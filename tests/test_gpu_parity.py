@@ -5,7 +5,9 @@ count and order, row/col within 0.02 px, angle within 0.01 degree and score
 within 1e-4.  Everything goes through the library's C ABI (mv_* and the
 mvamd_* / ncchip_* extensions)."""
 import ctypes as C
+import json
 import math
+import os
 import sys
 
 import numpy as np
@@ -247,3 +249,23 @@ def test_batched_find_equals_single_finds(capi, oracle):
     for k in range(3):
         want = om.find(imgs[k], -math.radians(20), math.radians(40), 0.6, 0, 0.5)
         check_matches(batch[k], np.asarray(want).reshape(-1, 4), f"frame {k}")
+
+
+GOLDEN = sorted(p for p in os.listdir(os.path.join(os.path.dirname(__file__), "golden")) if p.endswith(".json"))
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_find_matches_golden(capi, name):
+    """The C ABI find against the committed fixtures (oracle results on
+    generated scenes whose buffers are pinned by SHA-256)."""
+    from mvkit_amd import synth
+    say(f"golden {name}")
+    with open(os.path.join(os.path.dirname(__file__), "golden", name)) as f:
+        g = json.load(f)
+    s = getattr(synth, g["scene"]["factory"])(**g["scene"]["kwargs"])
+    assert s.sha256() == g["scene"]["sha256"]
+    md, fd = g["model"], g["find"]
+    m = capi.NccModel(s.template, md["levels"] if md["levels"] else "auto", md["angle_start"], md["angle_extent"])
+    got = m.find(s.image, md["angle_start"], md["angle_extent"], fd["min_score"], fd["num_matches"], fd["max_overlap"])
+    want = capi.halcon_shift(np.array(g["results"]).reshape(-1, 4))
+    check_matches(got, want, name)

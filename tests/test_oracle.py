@@ -4,7 +4,9 @@ The reference ships no tests or golden vectors for the NCC path and its NCC
 sources need a CMake-generated header, so each oracle function is pinned by
 mathematics a plausible mistake would violate (see the docstring of each).
 """
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -221,3 +223,32 @@ def test_failures(oracle):
     with pytest.raises(oracle.OracleError) as e:
         m.find(img, 0, 0, 0.0, 0, 1.0)
     assert e.value.status == -103
+
+
+# ------------------------------------------------------------ golden fixtures
+GOLDEN = sorted(p for p in os.listdir(os.path.join(os.path.dirname(__file__), "golden")) if p.endswith(".json"))
+
+
+def load_golden(name):
+    with open(os.path.join(os.path.dirname(__file__), "golden", name)) as f:
+        return json.load(f)
+
+
+def golden_scene(g):
+    from mvkit_amd import synth
+    s = getattr(synth, g["scene"]["factory"])(**g["scene"]["kwargs"])
+    assert s.sha256() == g["scene"]["sha256"], "synthetic generator drifted"
+    return s
+
+
+@pytest.mark.parametrize("name", [n for n in GOLDEN if "2048" not in n])
+def test_oracle_reproduces_golden(oracle, name):
+    g = load_golden(name)
+    s = golden_scene(g)
+    m = oracle.Model(s.template, levels=g["model"]["levels"], start=g["model"]["angle_start"],
+                     extent=g["model"]["angle_extent"])
+    f = g["find"]
+    res, tr = m.find(s.image, g["model"]["angle_start"], g["model"]["angle_extent"], f["min_score"],
+                     f["num_matches"], f["max_overlap"], trace=True)
+    assert tr["info"].n_coarse == g["n_coarse"] and tr["info"].n_prepared == g["n_prepared"]
+    assert np.array_equal(res, np.array(g["results"]).reshape(-1, 4))
