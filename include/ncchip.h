@@ -35,57 +35,89 @@ int ncchip_pyramid_level(const void* src, int src_kind, int src_pitch, int src_w
 int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src, int shift,
                       uint32_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream);
 
-/* Coarse exhaustive NCC over every anchor of the top level for each coarse
- * angle, writing FP64 score planes [n_angles][h][w] (-inf outside each
- * angle's valid anchor range).  `pools` points to two raster pools (the
- * model's and the per-find rasters); `patch_ids` are (pool << 24 | index).
- * `ranges` holds 4 ints per angle: row begin, row end, col begin, col end.
+/* Batched front end.  `frames` (device) holds one NccFrame per frame of a
+ * batch (at most NCC_MAX_BATCH); each batched kernel covers every frame in
+ * one launch, sized by the largest frame (max_w x max_h of the level
+ * concerned); blocks past a smaller frame's edge exit. */
+
+/* Pyramid level `level` from level - 1 for every frame (the levels' data
+ * pointers are in the descriptors).  Replaces matcher.cpp:436-454. */
+int ncchip_pyramid_batch(const NccFrame* frames, int n_frames, int level, int src_kind, int dst_kind, int max_w,
+                         int max_h, void* stream);
+
+/* Row prefixes of `level` (p1, p2, nz of each descriptor).
+ * Replaces matcher.cpp:487-495. */
+int ncchip_row_prefix_batch(const NccFrame* frames, int n_frames, int level, int kind, int max_h, void* stream);
+
+/* Fills n u32 words of device memory with `value` (a kernel, not a memset). */
+int ncchip_fill_u32(uint32_t* p, size_t n, uint32_t value, void* stream);
+
+/* Copies n u32 words from device memory to device or host-visible memory. */
+int ncchip_copy_u32(const uint32_t* src, uint32_t* dst, size_t n, void* stream);
+
+/* Coarse exhaustive NCC at the top level `level` of every frame for each
+ * coarse angle, writing each frame's FP64 score planes [n_angles][h][w]
+ * (-inf outside each angle's valid anchor range, descriptor `ranges`).
+ * `pools` points to two raster pools (the model's and the per-find rasters);
+ * `patch_ids` are (pool << 24 | index).  C = 128 * 4^level centres S.
  * Replaces matcher.cpp:542-579 with kernels_avx2.cpp:105-238. */
-int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1, const uint64_t* p2,
-                  const NccPatchPool* pools, const int32_t* patch_ids, const int32_t* ranges, int n_angles,
-                  int max_bbox_h, int max_bbox_w, int form, int exact_lattice, int absolute, double* planes,
-                  void* stream);
+int ncchip_coarse(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h, double C,
+                  const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, int max_bbox_h, int max_bbox_w,
+                  int form, int exact_lattice, int absolute, void* stream);
 
 /* Split-BF16 / FP8-screen MFMA coarse search (same planes as ncchip_coarse).
  * `screen` > -inf enables the FP8-E4M3 screen: anchors whose screened score
  * is below it are written -inf, survivors are re-scored in split-BF16. */
-int ncchip_coarse_mfma(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1, const uint64_t* p2,
-                       const NccPatchPool* pools, const int32_t* patch_ids, const int32_t* ranges, int n_angles,
-                       int bbox_h, int bbox_w, int exact_lattice, int absolute, double screen, double* planes,
-                       void* stream);
+int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
+                       double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, int bbox_h,
+                       int bbox_w, int exact_lattice, int absolute, double screen, void* stream);
+
+/* Per-angle 3x3 peaks of every frame's planes with the reference's tie rule
+ * (matcher.cpp:580-594), emitted as candidate poses (matcher.cpp:590-592:
+ * row = crop_r + scale * (r + origin_r) + phase) into frame f's list at
+ * cand + f * cap (unordered), with the angle index in key; count[f] (zeroed
+ * by the caller) may exceed cap.  extent[4f..4f+3] (min row, max row,
+ * min col, max col of the candidates' offsets from the crop origin, as FP64
+ * bit patterns; the caller initialises them to h, 0, w, 0) feeds
+ * ncchip_storage0. */
+int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int max_w, int max_h, const NccPatchPool* pools,
+                 const int32_t* patch_ids, int n_angles, double threshold, double scale, double phase, NccMatch* cand,
+                 int32_t* key, int cap, int32_t* count, unsigned long long* extent, void* stream);
+
+/* Level-0 storage rectangle of each frame (matcher.cpp:612-629) from its
+ * candidate extent and margin = radius + 4 * 2^top + 8, written into the
+ * descriptors' storage0 (frames without candidates: the whole level). */
+int ncchip_storage0(NccFrame* frames, int n_frames, const int32_t* count, const unsigned long long* extent,
+                    double margin, void* stream);
 
 /* Every kernel taking `dc_src` reads the reference pixel dc (the first
  * pixel of the search crop, matcher.cpp:410-413) from that device address;
  * integer levels use D = dc << shift. */
 
-/* Per-angle 3x3 peak extraction with the reference's tie rule and atomic
- * compaction into `out` (unordered); `count` (zeroed by the caller) receives
- * the number of peaks found, which may exceed cap.
- * Replaces matcher.cpp:580-594. */
-int ncchip_peaks(const double* planes, int n_angles, int h, int w, double threshold, NccPeak* out, int cap,
-                 int32_t* count, void* stream);
-
-/* Copies the peak counts of n_frames frames and each frame's first
- * min(count, cap) peaks (frame f's at peaks + f * cap) into host-visible
- * memory (pinned, coherent host allocations), laid out the same way. */
-int ncchip_export_peaks(const int32_t* counts, const NccPeak* peaks, int n_frames, int cap, int32_t* host_counts,
-                        NccPeak* host_peaks, void* stream);
-
-/* Coarse-to-fine tracking of every candidate at one level: the first strict
- * maximum over (query patch within `tol` of the candidate's angle, 5x5
- * anchors).  Candidates may come from several frames: `frames` (device) holds
- * one NccFrame per frame and `cand_frame` (device) each candidate's frame;
- * `level` is the tracked level and `level_kind` its pixel kind.  `refs` lists
- * the level's query patches as (pool << 24 | index).  `tile_pixels` bounds
- * the pixels of a candidate's neighbourhood (the union of its windows),
- * staged in LDS when it fits in 64 KiB (0: read the level from HBM);
- * exact_f64 != 0 asserts n * (max |S - D|)^2 < 2^53 for every query patch,
- * which lets the moments accumulate exactly in FP64.
- * Replaces matcher.cpp:670-756 (kernels_avx2.cpp:118-238). */
-int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, int level, int level_kind,
-                 const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* cand, NccMatch* out,
-                 int n_cand, double tol, double step, double offset, int exact_lattice, int absolute,
-                 int tile_pixels, int exact_f64, void* stream);
+/* Coarse-to-fine tracking of every candidate of a batch at one level: the
+ * first strict maximum over (query patch within `tol` of the candidate's
+ * angle, 5x5 anchors), then the reference's merge and filter.  Frame f's
+ * candidates are in + f*cap, in_count[f] of them (clamped to cap); the
+ * tracked pose of each goes to the same slot of `raw`; those with a finite
+ * score >= keep (0.4 * MinScore) and no identical (row, col, angle) kept
+ * before are appended to out + f*cap (unordered), counted in out_count[f]
+ * (zeroed by the caller).  `table` holds table_size int32 per frame (a power
+ * of two >= 2*cap, filled with -1 by the caller).  `level` is the tracked
+ * level and `level_kind` its pixel kind; `refs` lists the level's query
+ * patches as (pool << 24 | index).  `tile_pixels` bounds the pixels of a
+ * candidate's neighbourhood (the union of its windows), staged in LDS when
+ * it fits in 64 KiB (0: read the level from HBM); exact_f64 != 0 asserts
+ * n * (max |S - D|)^2 < 2^53 for every query patch, which lets the moments
+ * accumulate exactly in FP64.  `work` (optional, device) accumulates the
+ * samples of every evaluated window (algorithmic work = 2 FLOP per sample).
+ * `grid` workgroups loop over the candidates.
+ * Replaces matcher.cpp:670-760 (kernels_avx2.cpp:118-238) and
+ * merge_identical (matcher.cpp:241-248). */
+int ncchip_track(const NccFrame* frames, int n_frames, int level, int level_kind, const NccPatchPool* pools,
+                 const int32_t* refs, int n_refs, const NccMatch* in, const int32_t* in_count, int cap, NccMatch* raw,
+                 NccMatch* out, int32_t* out_count, int32_t* table, int table_size, double keep, double tol,
+                 double step, double offset, int exact_lattice, int absolute, int tile_pixels, int exact_f64,
+                 unsigned long long* work, int grid, void* stream);
 
 typedef struct NcchipRefineParams {
   const NccFrame* frames;  /* per-frame descriptors (device) */
@@ -100,11 +132,13 @@ typedef struct NcchipRefineParams {
   int32_t subpixel, rotate, absolute;
 } NcchipRefineParams;
 
-/* Continuous bilinear score of every candidate at the finest level of its
- * frame, one workgroup per candidate.  Replaces matcher.cpp:813-822
- * (kernels_avx2.cpp:327-477, 517-541). */
-int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
-                      const int32_t* cand_frame, int n_cand, NccMatch* out_full, void* stream);
+/* Continuous bilinear score at the finest level of every candidate of a
+ * batch (list form as ncchip_track: frame f's at in + f*cap, in_count[f]
+ * of them), written to the same slot of out_full; non-finite scores are
+ * kept.  `grid` workgroups loop over the candidates.  Replaces
+ * matcher.cpp:813-822 (kernels_avx2.cpp:327-477, 517-541). */
+int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, int n_frames, const NccMatch* in,
+                      const int32_t* in_count, int cap, NccMatch* out_full, int grid, void* stream);
 
 /* refine_candidate of every given full-scored candidate (Gauss-Newton fit
  * with backtracking, and the level-0 recovery search), one workgroup per

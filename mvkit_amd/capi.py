@@ -31,8 +31,9 @@ PUBLIC_SYMBOLS = [
     "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
-NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_coarse", "ncchip_coarse_mfma",
-                  "ncchip_peaks", "ncchip_export_peaks", "ncchip_track", "ncchip_full_score", "ncchip_refine"]
+NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
+                  "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma",
+                  "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_full_score", "ncchip_refine"]
 
 
 class StringView(C.Structure):

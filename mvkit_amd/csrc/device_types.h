@@ -59,17 +59,22 @@ typedef struct NccMatch { double row, col, angle, score; } NccMatch;
 // Everything a per-candidate kernel needs to know about the candidate's
 // frame: its search pyramid, reference pixel, level-0 storage rectangle,
 // crop origin and search domain.
+// The front-end fields point at the frame's top-level statistics and score
+// planes, written by the batched pyramid / prefix / coarse kernels.
 #define NCC_MAX_LEVELS 16
+#define NCC_MAX_BATCH 256
 typedef struct NccFrame {
-  NccLevel levels[NCC_MAX_LEVELS];
+  NccLevel levels[NCC_MAX_LEVELS];  // level 0: the caller's u8 image (crop origin); above: workspace
   const uint8_t* dc_src;     // reference pixel dc (first crop pixel), device memory
   int32_t storage0[4];       // level-0 valid rectangle {row, col, height, width}
   double crop_r, crop_c;
   int32_t image_h, image_w;  // search domain (full image)
+  uint32_t* p1;              // top-level row prefix of S, (w+1) per row
+  uint64_t* p2;              // top-level row prefix of S^2
+  uint32_t* nz;              // top-level row prefix of [S != D]
+  double* planes;            // coarse score planes [angle][h][w]
+  const int32_t* ranges;     // per coarse angle: valid anchor range {rb, re, cb, ce}
 } NccFrame;
-
-// A coarse peak: angle index into the coarse patch list, anchor and score.
-typedef struct NccPeak { int32_t angle, row, col, pad; double score; } NccPeak;
 
 #ifdef __cplusplus
 }

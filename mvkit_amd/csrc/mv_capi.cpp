@@ -642,7 +642,8 @@ extern "C" MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* i
     if (width <= 0 || height <= 0 || pitch < width)
       fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
     const auto m = model_of(ModelID, op);
-    const mvamd::FindOptions q = from_ext(options);
+    mvamd::FindOptions q = from_ext(options);
+    q.trace_stages = true;
     mvamd::FindTrace trace;
     const auto found = algorithm(op, [&] {
       return context().find_host(*m, image, size_t(width), size_t(height), size_t(pitch), q, &trace);

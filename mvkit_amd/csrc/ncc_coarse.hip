@@ -59,18 +59,36 @@ __device__ __forceinline__ void window_moments(const uint32_t* __restrict__ p1, 
 }
 
 struct CoarseArgs {
-  NccLevel lv;
-  const uint8_t* dc_src;   // reference pixel (first crop pixel), device memory
+  const NccFrame* frames;  // per-frame descriptors (device)
+  int level;               // the top level
+  int n_angles;            // coarse angles; blockIdx.z = frame * n_angles + angle
   double C;                // centring constant subtracted from S before the dot product
-  const uint32_t* p1;      // row prefix of S, (w+1) per row
-  const uint64_t* p2;      // row prefix of S^2
   Pools pools;
   const int32_t* patch_ids;  // coarse patches as (pool << 24) | index
-  const int4* ranges;        // per coarse patch: valid anchor range [rb, re) x [cb, ce)
-  double* planes;            // [A][h][w] scores, -inf outside the valid range
   int exact_lattice, absolute;
   int lds_w;               // LDS row length (floats)
 };
+
+// The frame-dependent part of a coarse block's work.
+struct CoarseFrame {
+  NccLevel lv;
+  const uint8_t* dc_src;
+  const uint32_t* p1;
+  const uint64_t* p2;
+  int4 rg;
+  double* planes;
+};
+__device__ __forceinline__ CoarseFrame coarse_frame(const CoarseArgs& a, int f, int ai) {
+  const NccFrame& F = a.frames[f];
+  CoarseFrame c;
+  c.lv = F.levels[a.level];
+  c.dc_src = F.dc_src;
+  c.p1 = F.p1;
+  c.p2 = F.p2;
+  c.rg = reinterpret_cast<const int4*>(F.ranges)[ai];
+  c.planes = F.planes;
+  return c;
+}
 
 constexpr int kTileR = 32, kTileC = 64, kColsPerThread = 8;
 
@@ -81,18 +99,20 @@ constexpr int kTileR = 32, kTileC = 64, kColsPerThread = 8;
 // sum(T) and sum(T^2) are accumulated for the first anchor and slid to the
 // other seven with the run's boundary pixels.
 template <class T>
-__global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs a) {
+__global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs args) {
   extern __shared__ int32_t tile[];
-  const double dc = double(*a.dc_src);
-  const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
-  const int ai = blockIdx.z;
-  const NccPatchPool& pool = a.pools.of(a.patch_ids[ai]);
-  const NccPatch P = a.pools.patch(a.patch_ids[ai]);
-  const int4 rg = a.ranges[ai];
+  const int f = blockIdx.z / args.n_angles, ai = blockIdx.z % args.n_angles;
+  const CoarseFrame a = coarse_frame(args, f, ai);
   const int W = a.lv.width, H = a.lv.height;
   const int r0 = blockIdx.y * kTileR, c0 = blockIdx.x * kTileC;
-  const int LH = kTileR + P.rmax - P.rmin, LW = a.lds_w;
-  const int Ci = int(a.C);
+  if (r0 >= H || c0 >= W) return;
+  const double dc = double(*a.dc_src);
+  const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
+  const NccPatchPool& pool = args.pools.of(args.patch_ids[ai]);
+  const NccPatch P = args.pools.patch(args.patch_ids[ai]);
+  const int4 rg = a.rg;
+  const int LH = kTileR + P.rmax - P.rmin, LW = args.lds_w;
+  const int Ci = int(args.C);
   constexpr int U = 8;  // staging loads issued together per thread
   for (int base = threadIdx.x; base < LH * LW; base += 256 * U) {
     int32_t v[U];
@@ -179,8 +199,8 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs a) {
       const int64_t n = P.n;
       const int64_t sum_s = m1[j] + n * Cl;
       const int64_t sum_s2 = m2[j] + 2 * Cl * m1[j] + n * Cl * Cl;
-      score = normalise(double(acc[j]), a.C, P.wsum_f, P.wsum, P.n, sum_s, sum_s2, D, a.lv.shift, dc, a.exact_lattice,
-                        a.absolute);
+      score = normalise(double(acc[j]), args.C, P.wsum_f, P.wsum, P.n, sum_s, sum_s2, D, a.lv.shift, dc,
+                        args.exact_lattice, args.absolute);
     }
     out[c] = score;
   }
@@ -189,16 +209,17 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs a) {
 // Reference-order FP64 scores of every anchor (one thread per anchor); used
 // when the LDS tile would not fit, and as the FP64 form for error reports.
 template <class T>
-__global__ void __launch_bounds__(256) k_coarse_fp64(const CoarseArgs a) {
-  const double dc = double(*a.dc_src);
-  const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
-  const int ai = blockIdx.z;
-  const NccPatchPool& pool = a.pools.of(a.patch_ids[ai]);
-  const NccPatch P = a.pools.patch(a.patch_ids[ai]);
-  const int4 rg = a.ranges[ai];
+__global__ void __launch_bounds__(256) k_coarse_fp64(const CoarseArgs args) {
+  const int f = blockIdx.z / args.n_angles, ai = blockIdx.z % args.n_angles;
+  const CoarseFrame a = coarse_frame(args, f, ai);
   const int W = a.lv.width, H = a.lv.height;
   const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
   if (c >= W || r >= H) return;
+  const double dc = double(*a.dc_src);
+  const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
+  const NccPatchPool& pool = args.pools.of(args.patch_ids[ai]);
+  const NccPatch P = args.pools.patch(args.patch_ids[ai]);
+  const int4 rg = a.rg;
   double score = NCC_MISSING;
   if (r >= rg.x && r < rg.y && c >= rg.z && c < rg.w) {
     const int32_t* rc = pool.rc + P.sample_off;
@@ -215,92 +236,169 @@ __global__ void __launch_bounds__(256) k_coarse_fp64(const CoarseArgs a) {
     const double cnt = double(P.n), var = sq - sum * sum / cnt;
     const double num = dot - sum / cnt * P.wsum;
     if (!(var > 128 * __DBL_EPSILON__ * sq)) score = NCC_MISSING;
-    else { score = clamp1(num / sqrt(var)); if (a.absolute) score = fabs(score); }
+    else { score = clamp1(num / sqrt(var)); if (args.absolute) score = fabs(score); }
   }
   a.planes[(size_t(ai) * H + r) * W + c] = score;
 }
 
 // Per-angle 3x3 peak extraction (matcher.cpp:580-594): keep finite v >= thr
 // with no neighbour above it and no equal neighbour above or to the left.
-__global__ void __launch_bounds__(256) k_peaks(const double* __restrict__ planes, int A, int H, int W, double thr,
-                                               NccPeak* __restrict__ out, int cap, int* __restrict__ count) {
-  const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y, ai = blockIdx.z;
-  if (c >= W || r >= H || ai >= A) return;
-  const double* pl = planes + size_t(ai) * H * W;
-  const double v = pl[size_t(r) * W + c];
-  if (v < thr || !isfinite(v)) return;
-  bool mx = true;
-  for (int dr = -1; dr <= 1 && mx; ++dr)
-    for (int dc = -1; dc <= 1; ++dc) {
-      const int rr = r + dr, cc = c + dc;
-      if (rr < 0 || cc < 0 || rr >= H || cc >= W || (!dr && !dc)) continue;
-      const double o = pl[size_t(rr) * W + cc];
-      if (o > v || (o == v && (dr < 0 || (!dr && dc < 0)))) { mx = false; break; }
+// Batched over frames (blockIdx.z = frame * A + angle).  A peak becomes a
+// candidate pose in the reference's internal convention (matcher.cpp:590-592)
+// appended to its frame's list (unordered; `key` keeps the angle index so the
+// reference's (angle, row, col) emission order can be restored), and widens
+// the frame's candidate extent (row and column offsets from the crop origin,
+// non-negative doubles kept as ordered bit patterns) used for the level-0
+// storage region.
+struct PeakArgs {
+  const NccFrame* frames;
+  int level, n_angles;
+  Pools pools;
+  const int32_t* patch_ids;
+  double thr, scale, phase;
+  NccMatch* cand;        // frame f's candidates at f * cap
+  int32_t* key;          // angle index of each candidate
+  int cap;
+  int32_t* count;        // per frame (may exceed cap)
+  unsigned long long* extent;  // per frame: min row, max row, min col, max col (FP64 bits)
+};
+
+__global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
+  const int f = blockIdx.z / a.n_angles, ai = blockIdx.z % a.n_angles;
+  const NccFrame& F = a.frames[f];
+  const int H = F.levels[a.level].height, W = F.levels[a.level].width;
+  const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+  if (r >= H || int(blockIdx.x) * 256 >= W) return;  // uniform per block
+  bool peak = false;
+  double v = 0;
+  if (c < W) {
+    const double* pl = F.planes + size_t(ai) * H * W;
+    v = pl[size_t(r) * W + c];
+    if (v >= a.thr && isfinite(v)) {
+      peak = true;
+      for (int dr = -1; dr <= 1 && peak; ++dr)
+        for (int dc = -1; dc <= 1; ++dc) {
+          const int rr = r + dr, cc = c + dc;
+          if (rr < 0 || cc < 0 || rr >= H || cc >= W || (!dr && !dc)) continue;
+          const double o = pl[size_t(rr) * W + cc];
+          if (o > v || (o == v && (dr < 0 || (!dr && dc < 0)))) { peak = false; break; }
+        }
     }
-  if (!mx) return;
-  const int k = atomicAdd(count, 1);
-  if (k < cap) {
-    NccPeak p;
-    p.angle = ai; p.row = r; p.col = c; p.pad = 0; p.score = v;
-    out[k] = p;
+  }
+  // one atomic per wave for the slots and the extent
+  const uint64_t m = __ballot(peak);
+  if (!m) return;
+  const int lane = threadIdx.x & 63, leader = __builtin_ctzll(m);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(a.count + f, __popcll(m));
+  base = __shfl(base, leader, 64);
+  double orow = __builtin_inf(), ocol = __builtin_inf(), xrow = 0, xcol = 0;
+  if (peak) {
+    const int k = base + __popcll(m & ((uint64_t(1) << lane) - 1));
+    const NccPatch& P = a.pools.patch(a.patch_ids[ai]);
+    NccMatch q;
+    const double dr = a.scale * (r + P.orow) + a.phase, dcol = a.scale * (c + P.ocol) + a.phase;
+    q.row = F.crop_r + dr;
+    q.col = F.crop_c + dcol;
+    q.angle = P.angle;
+    q.score = v;
+    if (k < a.cap) {
+      a.cand[size_t(f) * a.cap + k] = q;
+      a.key[size_t(f) * a.cap + k] = ai;
+      // offsets from the crop origin, as the host computes them (row - crop_r)
+      orow = xrow = q.row - F.crop_r;
+      ocol = xcol = q.col - F.crop_c;
+    }
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    orow = fmin(orow, __shfl_xor(orow, s, 64));
+    ocol = fmin(ocol, __shfl_xor(ocol, s, 64));
+    xrow = fmax(xrow, __shfl_xor(xrow, s, 64));
+    xcol = fmax(xcol, __shfl_xor(xcol, s, 64));
+  }
+  if (lane == leader && isfinite(orow)) {
+    unsigned long long* e = a.extent + 4 * size_t(f);
+    atomicMin(e + 0, __double_as_longlong(orow));
+    atomicMax(e + 1, __double_as_longlong(xrow));
+    atomicMin(e + 2, __double_as_longlong(ocol));
+    atomicMax(e + 3, __double_as_longlong(xcol));
   }
 }
 
-// Copies every frame's peak count and its first min(count, cap) peaks to
-// host-visible memory (frame f's peaks at f * cap), so one stream
-// synchronisation returns all frames' peaks.
-__global__ void __launch_bounds__(256) k_export_peaks(const int32_t* __restrict__ counts,
-                                                      const NccPeak* __restrict__ peaks, int cap,
-                                                      int32_t* __restrict__ out_counts, NccPeak* __restrict__ out) {
-  const int f = blockIdx.y;
-  const int n = min(counts[f], cap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) out_counts[f] = counts[f];
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-    out[size_t(f) * cap + i] = peaks[size_t(f) * cap + i];
+// Level-0 storage rectangle of every frame from its candidate extent
+// (matcher.cpp:612-629), written into the frame descriptors.  Frames without
+// candidates keep the full level.
+__global__ void k_storage0(NccFrame* __restrict__ frames, int n_frames, const int32_t* __restrict__ count,
+                           const unsigned long long* __restrict__ extent, double margin) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_frames) return;
+  NccFrame& F = frames[f];
+  const int h = F.levels[0].height, w = F.levels[0].width;
+  int32_t s[4] = {0, 0, h, w};
+  if (count[f] > 0) {
+    const double rmin = __longlong_as_double(extent[4 * f + 0]), rmax = __longlong_as_double(extent[4 * f + 1]);
+    const double cmin = __longlong_as_double(extent[4 * f + 2]), cmax = __longlong_as_double(extent[4 * f + 3]);
+    s[0] = int32_t(size_t(fmax(0., floor(rmin - margin))));
+    s[1] = int32_t(size_t(fmax(0., floor(cmin - margin))));
+    s[2] = int32_t(size_t(fmin(double(h), ceil(rmax + margin) + 1)) - size_t(s[0]));
+    s[3] = int32_t(size_t(fmin(double(w), ceil(cmax + margin) + 1)) - size_t(s[1]));
+  }
+  for (int i = 0; i < 4; ++i) F.storage0[i] = s[i];
 }
 
 }  // namespace ncc
 
 using namespace ncc;
 
-extern "C" int ncchip_export_peaks(const int32_t* counts, const NccPeak* peaks, int n_frames, int cap,
-                                   int32_t* host_counts, NccPeak* host_peaks, void* stream) {
-  if (n_frames <= 0) return 0;
-  hipLaunchKernelGGL(k_export_peaks, dim3(8, n_frames), dim3(256), 0, static_cast<hipStream_t>(stream), counts, peaks,
-                     cap, host_counts, host_peaks);
-  return hipGetLastError() == hipSuccess ? 0 : -1000;
-}
-
-extern "C" int ncchip_coarse(const NccLevel* lv, const uint8_t* dc_src, double C, const uint32_t* p1,
-                             const uint64_t* p2, const NccPatchPool* pools, const int32_t* patch_ids,
-                             const int32_t* ranges, int n_angles, int max_bbox_h, int max_bbox_w, int form,
-                             int exact_lattice, int absolute, double* planes, void* stream) {
+extern "C" int ncchip_coarse(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
+                             double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles,
+                             int max_bbox_h, int max_bbox_w, int form, int exact_lattice, int absolute,
+                             void* stream) {
+  if (n_frames <= 0 || n_angles <= 0) return 0;
+  if (size_t(n_frames) * n_angles > 65535) return -1;
   CoarseArgs a;
-  a.lv = *lv; a.dc_src = dc_src; a.C = C; a.p1 = p1; a.p2 = p2; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.patch_ids = patch_ids;
-  a.ranges = reinterpret_cast<const int4*>(ranges); a.planes = planes; a.exact_lattice = exact_lattice;
-  a.absolute = absolute;
+  a.frames = frames; a.level = level; a.n_angles = n_angles; a.C = C; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
+  a.patch_ids = patch_ids; a.exact_lattice = exact_lattice; a.absolute = absolute;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int LH = kTileR + max_bbox_h;
   a.lds_w = ((kTileC + max_bbox_w + 16) + 3) & ~3;
   const size_t lds = size_t(LH) * a.lds_w * sizeof(int32_t);
+  const unsigned z = unsigned(n_frames * n_angles);
   if (form == NCCHIP_FORM_FP32 && lds <= 64 * 1024) {
-    const dim3 grid((lv->width + kTileC - 1) / kTileC, (lv->height + kTileR - 1) / kTileR, n_angles);
-    if (lv->kind == NCC_U8) hipLaunchKernelGGL(k_coarse_fp32<uint8_t>, grid, dim3(256), lds, st, a);
-    else if (lv->kind == NCC_U16) hipLaunchKernelGGL(k_coarse_fp32<uint16_t>, grid, dim3(256), lds, st, a);
+    const dim3 grid((max_w + kTileC - 1) / kTileC, (max_h + kTileR - 1) / kTileR, z);
+    if (level_kind == NCC_U8) hipLaunchKernelGGL(k_coarse_fp32<uint8_t>, grid, dim3(256), lds, st, a);
+    else if (level_kind == NCC_U16) hipLaunchKernelGGL(k_coarse_fp32<uint16_t>, grid, dim3(256), lds, st, a);
     else hipLaunchKernelGGL(k_coarse_fp32<uint32_t>, grid, dim3(256), lds, st, a);
   } else {
-    const dim3 grid((lv->width + 255) / 256, lv->height, n_angles);
-    if (lv->kind == NCC_U8) hipLaunchKernelGGL(k_coarse_fp64<uint8_t>, grid, dim3(256), 0, st, a);
-    else if (lv->kind == NCC_U16) hipLaunchKernelGGL(k_coarse_fp64<uint16_t>, grid, dim3(256), 0, st, a);
+    if (max_h > 65535) return -1;
+    const dim3 grid((max_w + 255) / 256, max_h, z);
+    if (level_kind == NCC_U8) hipLaunchKernelGGL(k_coarse_fp64<uint8_t>, grid, dim3(256), 0, st, a);
+    else if (level_kind == NCC_U16) hipLaunchKernelGGL(k_coarse_fp64<uint16_t>, grid, dim3(256), 0, st, a);
     else hipLaunchKernelGGL(k_coarse_fp64<uint32_t>, grid, dim3(256), 0, st, a);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 
-extern "C" int ncchip_peaks(const double* planes, int n_angles, int h, int w, double threshold, NccPeak* out,
-                            int cap, int32_t* count, void* stream) {
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 grid((w + 255) / 256, h, n_angles);
-  hipLaunchKernelGGL(k_peaks, grid, dim3(256), 0, st, planes, n_angles, h, w, threshold, out, cap, count);
+extern "C" int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int max_w, int max_h,
+                            const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, double threshold,
+                            double scale, double phase, NccMatch* cand, int32_t* key, int cap, int32_t* count,
+                            unsigned long long* extent, void* stream) {
+  if (n_frames <= 0 || n_angles <= 0) return 0;
+  if (size_t(n_frames) * n_angles > 65535 || max_h > 65535) return -1;
+  PeakArgs a;
+  a.frames = frames; a.level = level; a.n_angles = n_angles; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
+  a.patch_ids = patch_ids; a.thr = threshold; a.scale = scale; a.phase = phase; a.cand = cand; a.key = key;
+  a.cap = cap; a.count = count; a.extent = extent;
+  const dim3 grid((max_w + 255) / 256, max_h, unsigned(n_frames * n_angles));
+  hipLaunchKernelGGL(k_peaks, grid, dim3(256), 0, static_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_storage0(NccFrame* frames, int n_frames, const int32_t* count, const unsigned long long* extent,
+                               double margin, void* stream) {
+  if (n_frames <= 0) return 0;
+  hipLaunchKernelGGL(k_storage0, dim3((n_frames + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream), frames,
+                     n_frames, count, extent, margin);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

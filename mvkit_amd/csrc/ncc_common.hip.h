@@ -76,6 +76,27 @@ struct Pools {
   __device__ __forceinline__ const NccPatch& patch(int32_t ref) const { return of(ref).patches[ref & 0xffffff]; }
 };
 
+// Candidate lists of a batch: frame f's entries at f * cap, counts[f] of
+// them (clamped to cap).  A block's prologue builds the exclusive offsets of
+// the per-frame counts in LDS so that a flat index g in [0, off[n]) maps to
+// (frame, entry).  Every thread of the block calls it.
+__device__ __forceinline__ void list_offsets(const int32_t* counts, int n_frames, int cap, int* off) {
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int f = 0; f < n_frames; ++f) { off[f] = t; t += min(max(counts[f], 0), cap); }
+    off[n_frames] = t;
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ int list_frame(const int* off, int n_frames, int g) {
+  int lo = 0, hi = n_frames - 1;  // last f with off[f] <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ int sample_row(int32_t rc) { return rc >> 16; }
 __device__ __forceinline__ int sample_col(int32_t rc) { return int(int16_t(rc & 0xffff)); }
 

@@ -88,12 +88,6 @@ bool score_order(const Pose& a, const Pose& b) {  // matcher.cpp:824-827, 1072-1
   if (a.score != b.score) return a.score > b.score;
   return pose_less(a, b);
 }
-void merge_identical(std::vector<Pose>& c) {
-  std::sort(c.begin(), c.end(), pose_less);
-  c.erase(std::unique(c.begin(), c.end(),
-                      [](const Pose& a, const Pose& b) { return a.row == b.row && a.col == b.col && a.angle == b.angle; }),
-          c.end());
-}
 }  // namespace
 
 void check(hipError_t e, const char* what) {
@@ -267,6 +261,14 @@ struct Job {
 
 std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, const FrameInput* inputs, size_t n_frames,
                                                        const FindOptions& req, FindTrace* traces) {
+  if (n_frames > NCC_MAX_BATCH) {  // larger batches run as consecutive batches
+    std::vector<std::vector<Pose>> all;
+    for (size_t b = 0; b < n_frames; b += NCC_MAX_BATCH) {
+      const size_t n = std::min<size_t>(NCC_MAX_BATCH, n_frames - b);
+      for (auto& part : find_batch(model, inputs + b, n, req, traces ? traces + b : nullptr)) all.push_back(std::move(part));
+    }
+    return all;
+  }
   const double t0 = now_us();
   profiling_ = traces ? req.profile : 0;
   timers_used_ = 0;
@@ -391,9 +393,21 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     }
   }
 
-  // ---- workspace layout
-  const size_t cap = o.max_candidates + 1;       // per frame
+  // ---- tracking levels (matcher.cpp:630-645): l = top .. finest+1, ending at
+  // l == 1 with subpixel (the fit takes over from level 1)
+  std::vector<size_t> track_levels;
+  for (size_t l = top; l > finest; --l) {
+    if (l == 1 && o.subpixel) break;
+    track_levels.push_back(l);
+  }
+  const size_t n_track = track_levels.size();
+
+  // ---- workspace layout.  Candidate lists: list 0 = coarse peaks, list k =
+  // the kept poses after tracking level k; frame f's entries at f * cap.
+  const size_t cap = o.max_candidates + 1;  // one past the budget detects overflow
   const size_t cap_all = cap * n_frames;
+  size_t table_size = 2;
+  while (table_size < 2 * cap) table_size *= 2;
   const PoolLayout QL(qpool);
   size_t off = 0;
   for (auto& J : jobs) {
@@ -404,190 +418,128 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     J.off_nz = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
     J.off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
   }
-  const size_t off_peaks = off; off = align_up(off + cap_all * sizeof(NccPeak));
-  const size_t off_count = off; off = align_up(off + n_frames * 4);
-  const size_t off_frames = off; off = align_up(off + n_frames * sizeof(NccFrame));
-  // small uploads: refs of every level, per-frame coarse ranges, query pool
+  const size_t off_lists = off; off = align_up(off + (n_track + 1) * cap_all * sizeof(NccMatch));
+  const size_t off_raw = off; off = align_up(off + cap_all * sizeof(NccMatch));
+  const size_t off_key = off; off = align_up(off + cap_all * 4);
+  // stage counts, then one u64 work counter per tracking level (copied together)
+  const size_t n_count_words = align_up((n_track + 1) * n_frames * 4, 8) / 4 + 2 * std::max<size_t>(n_track, 1);
+  const size_t off_counts = off; off = align_up(off + n_count_words * 4);
+  const size_t off_table = off; off = align_up(off + std::max<size_t>(n_track, 1) * n_frames * table_size * 4);
+  // small uploads: refs of every level, per-frame coarse ranges, query pool,
+  // candidate extents, frame descriptors
   std::vector<size_t> off_refs(level_count);
   const size_t off_small = off;
   size_t small = 0;
   for (size_t l = 0; l < level_count; ++l) { off_refs[l] = off_small + small; small = align_up(small + refs[l].size() * 4 + 4); }
   for (auto& J : jobs) { J.off_ranges = off_small + small; small = align_up(small + J.ranges.size() * 4); }
   const size_t off_qpool = off_small + small; small = align_up(small + QL.total);
+  const size_t off_extent = off_small + small; small = align_up(small + n_frames * 32);
+  const size_t off_frames = off_small + small; small = align_up(small + n_frames * sizeof(NccFrame));
   off = off_small + small;
   if (work_.bytes < off) {
     check(hipStreamSynchronize(stream_), "sync");
     work_ = DeviceBuffer(align_up(off + off / 4, 1 << 20));
   }
   uint8_t* dev = work_.as<uint8_t>();
-  // Host-visible (pinned, coherent) staging: candidate lists are read by the
-  // kernels in place and per-candidate results are written back in place, so
-  // a stage costs one launch and one synchronisation.  A list is rewritten
-  // only after the synchronisation that ends the launch reading it.
-  const size_t stage_frames = align_up(cap_all * (sizeof(NccMatch) + 4));
-  stage_.reserve(std::max(small, stage_frames + n_frames * sizeof(NccFrame)) + 256);
-  const size_t rb_peaks = align_up(cap_all * (sizeof(NccMatch) + 4));
-  readback_.reserve(rb_peaks + cap_all * sizeof(NccPeak) + n_frames * 4 + 64);
+  // Host-visible (pinned, coherent) staging: the refinement rounds' candidate
+  // lists are read by the kernels in place and the full scores and refined
+  // poses are written back in place.  A staging area is rewritten only after
+  // the synchronisation that ends the launch reading it.
+  stage_.reserve(std::max(small, cap_all * (sizeof(NccMatch) + 4)) + 256);
+  const size_t rb_full = align_up(cap_all * (sizeof(NccMatch) + 4));
+  const size_t rb_counts = rb_full + align_up(cap_all * sizeof(NccMatch));
+  readback_.reserve(rb_counts + n_count_words * 4 + 64);
+  auto* rb = static_cast<uint8_t*>(readback_.ptr);
+  const NccPatchPool pools[2] = {dmodel->pool.view, view_at(dev + off_qpool, QL)};
+  auto* dframes = reinterpret_cast<NccFrame*>(dev + off_frames);
+  auto* lists = reinterpret_cast<NccMatch*>(dev + off_lists);
+  auto* counts = reinterpret_cast<int32_t*>(dev + off_counts);
+  auto* work = reinterpret_cast<unsigned long long*>(dev + off_counts + align_up((n_track + 1) * n_frames * 4, 8));
+  auto* extent = reinterpret_cast<unsigned long long*>(dev + off_extent);
   {
     uint8_t* st = static_cast<uint8_t*>(stage_.ptr);
     for (size_t l = 0; l < level_count; ++l)
       std::memcpy(st + (off_refs[l] - off_small), refs[l].data(), refs[l].size() * 4);
     for (auto& J : jobs) std::memcpy(st + (J.off_ranges - off_small), J.ranges.data(), J.ranges.size() * 4);
     serialize(qpool, QL, st + (off_qpool - off_small));
-    check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, stream_), "upload");
-  }
-  const NccPatchPool pools[2] = {dmodel->pool.view, view_at(dev + off_qpool, QL)};
-  auto level_desc = [&](const Job& J, size_t l) {
-    NccLevel d{};
-    d.data = l ? static_cast<const void*>(dev + J.lv[l].off)
-               : static_cast<const void*>(J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c);
-    d.kind = J.lv[l].kind; d.pitch = J.lv[l].pitch; d.width = J.lv[l].w; d.height = J.lv[l].h; d.shift = int32_t(2 * l);
-    return d;
-  };
-  auto crop_of = [&](const Job& J) { return J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c; };
-
-  // ---- search pyramid, top-level statistics, coarse search and peaks of every frame
-  auto* counts = reinterpret_cast<int32_t*>(dev + off_count);
-  check(hipMemsetAsync(counts, 0, n_frames * 4, stream_), "memset");
-  const double C = std::ldexp(128.0, int(2 * top));
-  const int exact_top = top <= 2;
-  for (size_t f = 0; f < n_frames; ++f) {
-    Job& J = jobs[f];
-    time_begin(kStagePyramid);
-    for (size_t l = 1; l < level_count; ++l) {
-      const NccLevel src = level_desc(J, l - 1);
-      if (ncchip_pyramid_level(src.data, src.kind, src.pitch, src.width, src.height, dev + J.lv[l].off, J.lv[l].kind,
-                               J.lv[l].pitch, J.lv[l].w, J.lv[l].h, stream_))
-        raise(kInternalError, "device", "pyramid launch failed");
-    }
-    time_end();
-    const NccLevel topd = level_desc(J, top);
-    const int Wt = J.lv[top].w, Ht = J.lv[top].h;
-    auto* p1 = reinterpret_cast<uint32_t*>(dev + J.off_p1);
-    auto* p2 = reinterpret_cast<uint64_t*>(dev + J.off_p2);
-    time_begin(kStagePrefix);
-    if (ncchip_row_prefix(topd.data, topd.kind, topd.pitch, Wt, Ht, crop_of(J), int(2 * top), p1, p2,
-                          reinterpret_cast<uint32_t*>(dev + J.off_nz), nullptr, stream_))
-      raise(kInternalError, "device", "prefix launch failed");
-    time_end();
-    auto* planes = reinterpret_cast<double*>(dev + J.off_planes);
-    const auto* ranges = reinterpret_cast<const int32_t*>(dev + J.off_ranges);
-    const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
-    int rc = 0;
-    time_begin(kStageCoarse);
-    if (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN)
-      rc = ncchip_coarse_mfma(&topd, crop_of(J), C, p1, p2, pools, coarse_refs, ranges, int(A), max_bh, max_bw, exact_top,
-                              model.params.absolute, o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, planes,
-                              stream_);
-    else
-      rc = ncchip_coarse(&topd, crop_of(J), C, p1, p2, pools, coarse_refs, ranges, int(A), max_bh, max_bw, o.form,
-                         exact_top, model.params.absolute, planes, stream_);
-    if (rc) raise(kInternalError, "device", "coarse launch failed");
-    time_end();
-    time_begin(kStagePeaks);
-    if (ncchip_peaks(planes, int(A), Ht, Wt, o.min_score * .4,
-                     reinterpret_cast<NccPeak*>(dev + off_peaks) + f * cap, int(cap), counts + f, stream_))
-      raise(kInternalError, "device", "peak launch failed");
-    time_end();
-  }
-  auto* rb = static_cast<uint8_t*>(readback_.ptr);
-  auto* hpeaks = reinterpret_cast<NccPeak*>(rb + rb_peaks);
-  auto* hcounts = reinterpret_cast<int32_t*>(rb + rb_peaks + cap_all * sizeof(NccPeak));
-  if (ncchip_export_peaks(counts, reinterpret_cast<const NccPeak*>(dev + off_peaks), int(n_frames), int(cap), hcounts,
-                          hpeaks, stream_))
-    raise(kInternalError, "device", "peak export launch failed");
-  check(hipStreamSynchronize(stream_), "coarse");
-  const std::vector<int32_t> n_peaks(hcounts, hcounts + n_frames);
-  for (size_t f = 0; f < n_frames; ++f)
-    if (size_t(n_peaks[f]) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
-  for (size_t f = 0; f < n_frames; ++f) {
-    Job& J = jobs[f];
-    if (n_peaks[f] <= 0) continue;
-    std::vector<NccPeak> pk(hpeaks + f * cap, hpeaks + f * cap + size_t(n_peaks[f]));
-    // the reference emits peaks in (angle, row, col) order
-    std::sort(pk.begin(), pk.end(), [](const NccPeak& a, const NccPeak& b) {
-      if (a.angle != b.angle) return a.angle < b.angle;
-      if (a.row != b.row) return a.row < b.row;
-      return a.col < b.col;
-    });
-    J.cand.reserve(pk.size());
-    for (const auto& k : pk) {
-      const Raster& p = raster_of(refs[top][size_t(k.angle)]);
-      J.cand.push_back({double(J.crop_r) + scale * (k.row + p.orow) + phase,
-                        double(J.crop_c) + scale * (k.col + p.ocol) + phase, p.angle, k.score});
-    }
-  }
-  const double t1 = now_us();
-  for (auto& J : jobs)
-    if (J.trace) J.trace->coarse = J.cand;
-  // level-0 storage of the candidate region (matcher.cpp:612-629)
-  for (auto& J : jobs) {
-    J.storage0[2] = J.lv[0].h;
-    J.storage0[3] = J.lv[0].w;
-    if (finest || J.cand.empty()) continue;
-    double rmin = J.lv[0].h, cmin = J.lv[0].w, rmax = 0, cmax = 0;
-    for (const auto& p : J.cand) {
-      rmin = std::min(rmin, p.row - J.crop_r); rmax = std::max(rmax, p.row - J.crop_r);
-      cmin = std::min(cmin, p.col - J.crop_c); cmax = std::max(cmax, p.col - J.crop_c);
-    }
-    const double margin = radius + 4 * scale + 8;
-    J.storage0[0] = int32_t(size_t(std::max(0., std::floor(rmin - margin))));
-    J.storage0[1] = int32_t(size_t(std::max(0., std::floor(cmin - margin))));
-    J.storage0[2] = int32_t(size_t(std::min(double(J.lv[0].h), std::ceil(rmax + margin) + 1)) - size_t(J.storage0[0]));
-    J.storage0[3] = int32_t(size_t(std::min(double(J.lv[0].w), std::ceil(cmax + margin) + 1)) - size_t(J.storage0[1]));
-  }
-  // frame descriptors for the per-candidate kernels
-  {
-    auto* fr = reinterpret_cast<NccFrame*>(static_cast<uint8_t*>(stage_.ptr) + stage_frames);
+    auto* ext = reinterpret_cast<double*>(st + (off_extent - off_small));
+    auto* fr = reinterpret_cast<NccFrame*>(st + (off_frames - off_small));
     for (size_t f = 0; f < n_frames; ++f) {
       const Job& J = jobs[f];
+      // the host's running min/max start at (h, 0, w, 0) (matcher.cpp:612-618)
+      ext[4 * f + 0] = J.lv[0].h; ext[4 * f + 1] = 0; ext[4 * f + 2] = J.lv[0].w; ext[4 * f + 3] = 0;
       NccFrame d{};
-      for (size_t l = 0; l < level_count; ++l) d.levels[l] = level_desc(J, l);
-      d.dc_src = crop_of(J);
-      std::copy(J.storage0, J.storage0 + 4, d.storage0);
+      for (size_t l = 0; l < level_count; ++l) {
+        NccLevel& L = d.levels[l];
+        L.data = l ? static_cast<const void*>(dev + J.lv[l].off)
+                   : static_cast<const void*>(J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c);
+        L.kind = J.lv[l].kind; L.pitch = J.lv[l].pitch; L.width = J.lv[l].w; L.height = J.lv[l].h;
+        L.shift = int32_t(2 * l);
+      }
+      d.dc_src = J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c;
+      d.storage0[0] = 0; d.storage0[1] = 0; d.storage0[2] = J.lv[0].h; d.storage0[3] = J.lv[0].w;
       d.crop_r = double(J.crop_r); d.crop_c = double(J.crop_c);
       d.image_h = int32_t(J.in.height); d.image_w = int32_t(J.in.width);
+      d.p1 = reinterpret_cast<uint32_t*>(dev + J.off_p1);
+      d.p2 = reinterpret_cast<uint64_t*>(dev + J.off_p2);
+      d.nz = reinterpret_cast<uint32_t*>(dev + J.off_nz);
+      d.planes = reinterpret_cast<double*>(dev + J.off_planes);
+      d.ranges = reinterpret_cast<const int32_t*>(dev + J.off_ranges);
       fr[f] = d;
     }
-    check(hipMemcpyAsync(dev + off_frames, fr, n_frames * sizeof(NccFrame), hipMemcpyHostToDevice, stream_), "upload");
+    check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, stream_), "upload");
   }
-  const auto* dframes = reinterpret_cast<const NccFrame*>(dev + off_frames);
-  // candidate lists and per-candidate results live in host-visible memory
-  auto* dcand = static_cast<NccMatch*>(stage_.ptr);
-  int32_t* dcand_frame = nullptr;
-  auto* dout = reinterpret_cast<NccMatch*>(rb);
-  int32_t* dverdict = nullptr;
-  auto upload = [&](const std::vector<std::pair<uint32_t, Pose>>& list) {
-    dcand_frame = reinterpret_cast<int32_t*>(dcand + list.size());
-    dverdict = reinterpret_cast<int32_t*>(dout + list.size());
-    for (size_t i = 0; i < list.size(); ++i) {
-      const Pose& p = list[i].second;
-      dcand[i] = {p.row, p.col, p.angle, p.score};
-      dcand_frame[i] = int32_t(list[i].first);
-    }
+  auto max_dims = [&](size_t l) {
+    int w = 0, h = 0;
+    for (const auto& J : jobs) { w = std::max(w, J.lv[l].w); h = std::max(h, J.lv[l].h); }
+    return std::pair<int, int>(w, h);
   };
-  auto readback = [&](size_t, bool) { check(hipStreamSynchronize(stream_), "readback"); };
-  auto result_at = [&](size_t i) {
-    const NccMatch& m = reinterpret_cast<const NccMatch*>(rb)[i];
-    return Pose{m.row, m.col, m.angle, m.score};
-  };
+  const int F = int(n_frames);
 
+  // ---- the device-resident part: pyramid, top-level statistics, coarse
+  // search, peaks, tracking, full score; one synchronisation at the end
+  if (ncchip_fill_u32(reinterpret_cast<uint32_t*>(counts), n_count_words, 0u, stream_) ||
+      ncchip_fill_u32(reinterpret_cast<uint32_t*>(dev + off_table), std::max<size_t>(n_track, 1) * n_frames * table_size,
+                      0xffffffffu, stream_))
+    raise(kInternalError, "device", "fill launch failed");
+  time_begin(kStagePyramid);
+  for (size_t l = 1; l < level_count; ++l) {
+    const auto [w, h] = max_dims(l);
+    if (ncchip_pyramid_batch(dframes, F, int(l), jobs[0].lv[l - 1].kind, jobs[0].lv[l].kind, w, h, stream_))
+      raise(kInternalError, "device", "pyramid launch failed");
+  }
+  time_end();
+  const auto [Wt_max, Ht_max] = max_dims(top);
+  time_begin(kStagePrefix);
+  if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, stream_))
+    raise(kInternalError, "device", "prefix launch failed");
+  time_end();
+  const double C = std::ldexp(128.0, int(2 * top));
+  const int exact_top = top <= 2;
+  const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
+  time_begin(kStageCoarse);
+  int rc = 0;
+  if (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN)
+    rc = ncchip_coarse_mfma(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
+                            max_bh, max_bw, exact_top, model.params.absolute,
+                            o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, stream_);
+  else
+    rc = ncchip_coarse(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
+                       max_bh, max_bw, o.form, exact_top, model.params.absolute, stream_);
+  if (rc == -6) raise(kUnsupported, "form", "correlation form not available in this build");
+  if (rc) raise(kInternalError, "device", "coarse launch failed");
+  time_end();
+  time_begin(kStagePeaks);
+  if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
+                   lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, stream_))
+    raise(kInternalError, "device", "peak launch failed");
+  if (!finest && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
+    raise(kInternalError, "device", "storage launch failed");
+  time_end();
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
-  for (size_t l = top; l > finest; --l) {
-    if (l == 1 && o.subpixel) {
-      for (auto& J : jobs) {
-        for (auto& p : J.cand) {
-          const double r = std::floor(p.row + .5), c = std::floor(p.col + .5);
-          if (!(r >= 0 && c >= 0 && r < double(J.in.height) && c < double(J.in.width))) p.score = kMissing;
-        }
-        merge_identical(J.cand);
-      }
-      break;
-    }
-    std::vector<std::pair<uint32_t, Pose>> list;
-    for (size_t f = 0; f < n_frames; ++f)
-      for (const auto& p : jobs[f].cand) list.push_back({uint32_t(f), p});
-    if (list.empty()) break;
+  constexpr int kListGrid = 2048;
+  for (size_t k = 0; k < n_track; ++k) {
+    const size_t l = track_levels[k];
     const double step = std::ldexp(1., int(l - 1)), offset = (step - 1) * .5;
     const double tol = model.params.step * (l == top ? double(stride) : 1.) * 1.01;
     // bound of a candidate's neighbourhood (the union of the 5x5 windows of
@@ -602,37 +554,18 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     const double tile_px = (std::ceil(hi_r - lo_r) + 7) * (std::ceil(hi_c - lo_c) + 7);
     const double xmax = 255. * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
-    if (profiling_ & (1 << kStageTrack)) {
-      // algorithmic work: 2 * samples over the 5x5 windows of every query
-      // patch within the angle window (windows off the level included)
-      std::vector<std::pair<double, double>> an;
-      for (const int32_t r : refs[l - 1]) an.push_back({raster_of(r).angle, double(raster_of(r).count)});
-      for (const auto& [f, p] : list)
-        for (const auto& [ang, n] : an)
-          if (std::abs(ang - p.angle) <= tol) jobs[f].track_flops += 2.0 * 25 * n;
-    }
-    upload(list);
     time_begin(kStageTrack);
-    if (ncchip_track(dframes, dcand_frame, int(l - 1), jobs[0].lv[l - 1].kind, pools,
-                     reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()), dcand, dout,
-                     int(list.size()), tol, step, offset, l - 1 <= 2, model.params.absolute,
-                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, stream_))
+    if (ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
+                     reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
+                     lists + k * cap_all, counts + k * n_frames, int(cap), reinterpret_cast<NccMatch*>(dev + off_raw),
+                     lists + (k + 1) * cap_all, counts + (k + 1) * n_frames,
+                     reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size, int(table_size),
+                     o.min_score * .4, tol, step, offset, l - 1 <= 2, model.params.absolute,
+                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, stream_))
       raise(kInternalError, "device", "tracking launch failed");
     time_end();
-    readback(list.size(), false);
-    for (auto& J : jobs) J.cand.clear();
-    for (size_t i = 0; i < list.size(); ++i) jobs[list[i].first].cand.push_back(result_at(i));
-    for (auto& J : jobs) {
-      merge_identical(J.cand);
-      J.cand.erase(std::remove_if(J.cand.begin(), J.cand.end(),
-                                  [&](const Pose& p) { return !std::isfinite(p.score) || p.score < o.min_score * .4; }),
-                   J.cand.end());
-      if (J.trace) J.trace->tracked.push_back(J.cand);
-    }
   }
-  const double t2 = now_us();
-
-  // ---- full score at the finest level, then candidate order (matcher.cpp:813-827)
+  // ---- full score at the finest level (matcher.cpp:813-822)
   NcchipRefineParams rp{};
   rp.frames = dframes;
   rp.level_kind = jobs[0].lv[finest].kind;
@@ -644,26 +577,96 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   rp.n_refs0 = int32_t(refs[0].size());
   rp.step = model.params.step; rp.angle_start = o.start; rp.angle_end = o.start + o.extent;
   rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
-  {
-    std::vector<std::pair<uint32_t, Pose>> list;
-    for (size_t f = 0; f < n_frames; ++f)
-      for (const auto& p : jobs[f].cand) list.push_back({uint32_t(f), p});
-    if (!list.empty()) {
-      upload(list);
-      time_begin(kStageFull);
-      if (ncchip_full_score(&rp, pools, dcand, dcand_frame, int(list.size()), dout, stream_))
-        raise(kInternalError, "device", "full score launch failed");
-      time_end();
-      readback(list.size(), false);
-      for (size_t i = 0; i < list.size(); ++i) jobs[list[i].first].full.push_back(result_at(i));
-    }
-    for (auto& J : jobs) {
-      std::sort(J.full.begin(), J.full.end(), score_order);
-      J.refined.assign(J.full.size(), Pose{});
-      J.verdict.assign(J.full.size(), -1);  // -1: not refined
-      J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
+  auto* hfull = reinterpret_cast<NccMatch*>(rb + rb_full);
+  auto* hcounts = reinterpret_cast<int32_t*>(rb + rb_counts);
+  time_begin(kStageFull);
+  if (ncchip_full_score(&rp, pools, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), hfull, 1024,
+                        stream_))
+    raise(kInternalError, "device", "full score launch failed");
+  time_end();
+  if (ncchip_copy_u32(reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<uint32_t*>(hcounts), n_count_words,
+                      stream_))
+    raise(kInternalError, "device", "copy launch failed");
+  check(hipStreamSynchronize(stream_), "find");
+  const double t1 = now_us();
+  for (size_t f = 0; f < n_frames; ++f)
+    if (size_t(hcounts[f]) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
+  auto list_count = [&](size_t k, size_t f) { return std::min<size_t>(size_t(std::max(hcounts[k * n_frames + f], 0)), cap); };
+
+  // tracking work of the batch, reported with the first frame's trace
+  if (traces) {
+    const auto* hwork = reinterpret_cast<const unsigned long long*>(
+        reinterpret_cast<const uint8_t*>(hcounts) + align_up((n_track + 1) * n_frames * 4, 8));
+    for (size_t k = 0; k < n_track; ++k) jobs[0].track_flops += 2.0 * double(hwork[k]);
+  }
+  // ---- stage dumps for traces (not on the benchmarked path)
+  if (traces && o.trace_stages) {
+    std::vector<NccMatch> all((n_track + 1) * cap_all);
+    std::vector<int32_t> keys(cap_all);
+    std::vector<NccFrame> frs(n_frames);
+    check(hipMemcpy(all.data(), lists, all.size() * sizeof(NccMatch), hipMemcpyDeviceToHost), "trace");
+    check(hipMemcpy(keys.data(), dev + off_key, keys.size() * 4, hipMemcpyDeviceToHost), "trace");
+    check(hipMemcpy(frs.data(), dframes, n_frames * sizeof(NccFrame), hipMemcpyDeviceToHost), "trace");
+    for (size_t f = 0; f < n_frames; ++f) {
+      Job& J = jobs[f];
+      if (!J.trace) continue;
+      std::copy(frs[f].storage0, frs[f].storage0 + 4, J.storage0);
+      // the reference emits peaks in (angle, row, col) order (matcher.cpp:580-594)
+      std::vector<size_t> order(list_count(0, f));
+      for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+      const NccMatch* c0 = &all[f * cap];
+      const int32_t* k0 = &keys[f * cap];
+      std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        if (k0[a] != k0[b]) return k0[a] < k0[b];
+        if (c0[a].row != c0[b].row) return c0[a].row < c0[b].row;
+        return c0[a].col < c0[b].col;
+      });
+      for (size_t i : order) J.trace->coarse.push_back({c0[i].row, c0[i].col, c0[i].angle, c0[i].score});
+      for (size_t k = 1; k <= n_track; ++k) {
+        std::vector<Pose> v;
+        for (size_t i = 0; i < list_count(k, f); ++i) {
+          const NccMatch& m = all[k * cap_all + f * cap + i];
+          v.push_back({m.row, m.col, m.angle, m.score});
+        }
+        std::sort(v.begin(), v.end(), pose_less);  // merge_identical's order
+        J.trace->tracked.push_back(std::move(v));
+      }
     }
   }
+  const double t2 = now_us();
+
+  // ---- candidate order (matcher.cpp:824-827)
+  for (size_t f = 0; f < n_frames; ++f) {
+    Job& J = jobs[f];
+    const size_t n = list_count(n_track, f);
+    J.full.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+      const NccMatch& m = hfull[f * cap + i];
+      J.full.push_back({m.row, m.col, m.angle, m.score});
+    }
+    std::sort(J.full.begin(), J.full.end(), score_order);
+    J.refined.assign(J.full.size(), Pose{});
+    J.verdict.assign(J.full.size(), -1);  // -1: not refined
+    J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
+  }
+  // refinement rounds: candidate lists and results in host-visible memory
+  auto* dcand = static_cast<NccMatch*>(stage_.ptr);
+  int32_t* dcand_frame = nullptr;
+  auto* dout = reinterpret_cast<NccMatch*>(rb);
+  int32_t* dverdict = nullptr;
+  auto upload = [&](const std::vector<std::pair<uint32_t, Pose>>& list) {
+    dcand_frame = reinterpret_cast<int32_t*>(dcand + list.size());
+    dverdict = reinterpret_cast<int32_t*>(dout + list.size());
+    for (size_t i = 0; i < list.size(); ++i) {
+      const Pose& p = list[i].second;
+      dcand[i] = {p.row, p.col, p.angle, p.score};
+      dcand_frame[i] = int32_t(list[i].first);
+    }
+  };
+  auto result_at = [&](size_t i) {
+    const NccMatch& m = dout[i];
+    return Pose{m.row, m.col, m.angle, m.score};
+  };
 
   // ---- greedy selection in score order (matcher.cpp:1063-1076).  A
   // candidate's refinement does not depend on the other candidates, so each
@@ -705,8 +708,8 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     if (ncchip_refine(&rp, pools, dcand, dcand_frame, int(list.size()), dout, dverdict, stream_))
       raise(kInternalError, "device", "refine launch failed");
     time_end();
-    readback(list.size(), true);
-    const auto* v = reinterpret_cast<const int32_t*>(rb + list.size() * sizeof(NccMatch));
+    check(hipStreamSynchronize(stream_), "refine");
+    const int32_t* v = dverdict;
     for (size_t k = 0; k < list.size(); ++k) {
       Job& J = jobs[list[k].first];
       J.refined[index[k]] = result_at(k);

@@ -66,6 +66,7 @@ struct FindOptions {
   int form = NCCHIP_FORM_FP32;     // coarse correlation form
   double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN)
   int profile = 0;                 // bit s: time stage s (KernelStage) with HIP events on the context's stream
+  bool trace_stages = false;       // fill FindTrace's stage lists (coarse, tracked, storage)
 };
 
 // Kernel stages timed in profiling mode.

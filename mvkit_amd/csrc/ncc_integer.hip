@@ -2,6 +2,8 @@
 // top-level row prefix sums of S and S^2, and the 2D support integral.
 // All results are bit-exact images of the reference's FP64 values
 // (matcher.cpp:401-455, 487-495, 498-503).
+#include <algorithm>
+
 #include "ncc_common.hip.h"
 #include "../../include/ncchip.h"
 
@@ -12,12 +14,10 @@ __device__ __forceinline__ uint32_t ld(const Tin* p, size_t i) { return uint32_t
 
 // One output level from the previous one: S_{l+1}(r,c) = sum of the 2x2 block
 // at (2r, 2c) with the reference's edge clamp (matcher.cpp:442, 448-450).
-// Each thread writes 4 consecutive outputs of one row.
+// Each thread writes 4 consecutive outputs of row r starting at column c0.
 template <class Tin, class Tout>
-__global__ void __launch_bounds__(256) k_pyramid(const Tin* __restrict__ src, int sp, int sw, int sh,
-                                                 Tout* __restrict__ dst, int dp, int dw, int dh) {
-  const int r = blockIdx.y;
-  const int c0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+__device__ __forceinline__ void pyramid_row4(const Tin* __restrict__ src, int sp, int sw, int sh,
+                                             Tout* __restrict__ dst, int dp, int dw, int dh, int r, int c0) {
   if (r >= dh || c0 >= dw) return;
   const int r0 = 2 * r, r1 = min(r0 + 1, sh - 1);
   const Tin* a = src + size_t(r0) * sp;
@@ -40,17 +40,31 @@ __global__ void __launch_bounds__(256) k_pyramid(const Tin* __restrict__ src, in
   }
 }
 
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(256) k_pyramid(const Tin* __restrict__ src, int sp, int sw, int sh,
+                                                 Tout* __restrict__ dst, int dp, int dw, int dh) {
+  for (int r = blockIdx.y; r < dh; r += gridDim.y)
+    pyramid_row4(src, sp, sw, sh, dst, dp, dw, dh, r, (blockIdx.x * 256 + threadIdx.x) * 4);
+}
+
+// Batched form: frame blockIdx.z builds `level` from level - 1.
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(256) k_pyramid_batch(const NccFrame* __restrict__ frames, int level) {
+  const NccLevel& s = frames[blockIdx.z].levels[level - 1];
+  const NccLevel& d = frames[blockIdx.z].levels[level];
+  for (int r = blockIdx.y; r < d.height; r += gridDim.y)
+    pyramid_row4(static_cast<const Tin*>(s.data), s.pitch, s.width, s.height,
+                 static_cast<Tout*>(const_cast<void*>(d.data)), d.pitch, d.width, d.height, r,
+                 (blockIdx.x * 256 + threadIdx.x) * 4);
+}
+
 // Row prefix of S (u32) and S^2 (u64) over one level row, plus the row
 // prefix of the support indicator (S != D, i.e. q != 0), one block per row.
 // Layout: (w+1) entries per row with a leading zero (matcher.cpp:490-494).
 template <class T>
-__global__ void __launch_bounds__(256) k_row_prefix(const T* __restrict__ src, int sp, int w, int h,
-                                                    const uint8_t* __restrict__ dc_src, int shift,
-                                                    uint32_t* __restrict__ p1, uint64_t* __restrict__ p2,
-                                                    uint32_t* __restrict__ nz) {
-  const int r = blockIdx.x;
-  const uint32_t D = uint32_t(*dc_src) << shift;
-  if (r >= h) return;
+__device__ __forceinline__ void row_prefix(const T* __restrict__ src, int sp, int w, int h, uint32_t D, int r,
+                                           uint32_t* __restrict__ p1, uint64_t* __restrict__ p2,
+                                           uint32_t* __restrict__ nz) {
   __shared__ uint32_t s1[4];
   __shared__ uint64_t s2[4];
   __shared__ uint32_t s3[4];
@@ -87,6 +101,37 @@ __global__ void __launch_bounds__(256) k_row_prefix(const T* __restrict__ src, i
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_row_prefix(const T* __restrict__ src, int sp, int w, int h,
+                                                    const uint8_t* __restrict__ dc_src, int shift,
+                                                    uint32_t* __restrict__ p1, uint64_t* __restrict__ p2,
+                                                    uint32_t* __restrict__ nz) {
+  if (int(blockIdx.x) >= h) return;
+  row_prefix(src, sp, w, h, uint32_t(*dc_src) << shift, blockIdx.x, p1, p2, nz);
+}
+
+// Batched form: row blockIdx.x of frame blockIdx.y's `level`.
+template <class T>
+__global__ void __launch_bounds__(256) k_row_prefix_batch(const NccFrame* __restrict__ frames, int level) {
+  const NccFrame& F = frames[blockIdx.y];
+  const NccLevel& L = F.levels[level];
+  if (int(blockIdx.x) >= L.height) return;
+  row_prefix(static_cast<const T*>(L.data), L.pitch, L.width, L.height, uint32_t(*F.dc_src) << L.shift, blockIdx.x,
+             F.p1, F.p2, F.nz);
+}
+
+// Copies n 32-bit words (device to device or to host-visible memory).
+__global__ void __launch_bounds__(256) k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                  size_t n) {
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+}
+
+// Fills n 32-bit words (a kernel rather than a memset, so that the sequence
+// can be captured in a HIP graph).
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) p[i] = v;
+}
+
 // Column pass turning row prefixes of the support indicator into the 2D
 // integral support[(r+1)*(w+1) + c] (first row zero), matcher.cpp:498-499.
 __global__ void __launch_bounds__(256) k_support_cols(const uint32_t* __restrict__ nz, int w, int h,
@@ -109,7 +154,7 @@ static inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s
 
 extern "C" int ncchip_pyramid_level(const void* src, int src_kind, int src_pitch, int src_w, int src_h, void* dst,
                                     int dst_kind, int dst_pitch, int dst_w, int dst_h, void* stream) {
-  const dim3 grid((dst_w + 1023) / 1024, dst_h);
+  const dim3 grid((dst_w + 1023) / 1024, std::min(dst_h, 65535));
   hipStream_t st = as_stream(stream);
 #define NCC_PYR(TI, TO) \
   hipLaunchKernelGGL((k_pyramid<TI, TO>), grid, dim3(256), 0, st, static_cast<const TI*>(src), src_pitch, src_w, \
@@ -121,6 +166,47 @@ extern "C" int ncchip_pyramid_level(const void* src, int src_kind, int src_pitch
   else if (src_kind == NCC_U8 && dst_kind == NCC_U32) NCC_PYR(uint8_t, uint32_t);
   else return -1;
 #undef NCC_PYR
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_pyramid_batch(const NccFrame* frames, int n_frames, int level, int src_kind, int dst_kind,
+                                    int max_w, int max_h, void* stream) {
+  if (n_frames <= 0 || n_frames > 65535) return n_frames ? -1 : 0;
+  const dim3 grid((max_w + 1023) / 1024, std::min(max_h, 65535), n_frames);
+  hipStream_t st = as_stream(stream);
+#define NCC_PYR(TI, TO) hipLaunchKernelGGL((k_pyramid_batch<TI, TO>), grid, dim3(256), 0, st, frames, level)
+  if (src_kind == NCC_U8 && dst_kind == NCC_U16) NCC_PYR(uint8_t, uint16_t);
+  else if (src_kind == NCC_U16 && dst_kind == NCC_U16) NCC_PYR(uint16_t, uint16_t);
+  else if (src_kind == NCC_U16 && dst_kind == NCC_U32) NCC_PYR(uint16_t, uint32_t);
+  else if (src_kind == NCC_U32 && dst_kind == NCC_U32) NCC_PYR(uint32_t, uint32_t);
+  else if (src_kind == NCC_U8 && dst_kind == NCC_U32) NCC_PYR(uint8_t, uint32_t);
+  else return -1;
+#undef NCC_PYR
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_row_prefix_batch(const NccFrame* frames, int n_frames, int level, int kind, int max_h,
+                                       void* stream) {
+  if (n_frames <= 0 || n_frames > 65535) return n_frames ? -1 : 0;
+  const dim3 grid(max_h, n_frames);
+  hipStream_t st = as_stream(stream);
+  if (kind == NCC_U8) hipLaunchKernelGGL((k_row_prefix_batch<uint8_t>), grid, dim3(256), 0, st, frames, level);
+  else if (kind == NCC_U16) hipLaunchKernelGGL((k_row_prefix_batch<uint16_t>), grid, dim3(256), 0, st, frames, level);
+  else hipLaunchKernelGGL((k_row_prefix_batch<uint32_t>), grid, dim3(256), 0, st, frames, level);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_fill_u32(uint32_t* p, size_t n, uint32_t value, void* stream) {
+  if (!n) return 0;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(unsigned(blocks)), dim3(256), 0, as_stream(stream), p, n, value);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_copy_u32(const uint32_t* src, uint32_t* dst, size_t n, void* stream) {
+  if (!n) return 0;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_copy_u32, dim3(unsigned(blocks)), dim3(256), 0, as_stream(stream), src, dst, n);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 

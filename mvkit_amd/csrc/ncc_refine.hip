@@ -35,14 +35,21 @@ __device__ __forceinline__ double ref_angle(const Pools& pools, int32_t ref) { r
 // ------------------------------------------------------------------ tracking
 struct TrackArgs {
   const NccFrame* frames;     // per-frame descriptors (device)
-  const int32_t* cand_frame;  // frame index of every candidate
+  int n_frames;
   int level;                  // tracked level l-1
   Pools pools;
   const int32_t* refs;
   int n_refs;
-  const NccMatch* cand;
-  NccMatch* out;
-  int n_cand;
+  const NccMatch* in;         // frame f's candidates at f * cap
+  const int32_t* in_count;
+  int cap;
+  NccMatch* raw;              // tracked pose of every input candidate (same slot)
+  NccMatch* out;              // kept, distinct tracked poses (unordered)
+  int32_t* out_count;
+  int32_t* table;             // per frame: open-addressing set of kept raw slots (-1: empty)
+  int table_size;             // power of two
+  double keep;                // 0.4 * MinScore
+  unsigned long long* work;   // algorithmic work counter: samples over the evaluated 5x5 windows
   double tol, step, offset;
   int exact_lattice, absolute;
   int tile_cap;               // LDS tile capacity in pixels
@@ -53,9 +60,9 @@ struct TrackArgs {
   int sr, sc, sh, sw;
 };
 
-// Loads the candidate's frame into the per-block copy of the arguments.
-__device__ __forceinline__ void bind_frame(TrackArgs& a, int ci) {
-  const NccFrame& F = a.frames[a.cand_frame[ci]];
+// Loads frame f into the per-block copy of the arguments.
+__device__ __forceinline__ void bind_frame(TrackArgs& a, int f) {
+  const NccFrame& F = a.frames[f];
   a.lv = F.levels[a.level];
   a.dc = double(*F.dc_src);
   a.D = int64_t(*F.dc_src) << a.lv.shift;
@@ -231,6 +238,48 @@ __device__ __forceinline__ int eligible_patches(const Pools& pools, const int32_
   return min(n_e, kMaxEligible);
 }
 
+// Post-tracking merge and filter of one frame (matcher.cpp:241-248, 758-760):
+// a tracked pose with a finite score >= keep is appended to the frame's
+// output list unless an identical (row, col, angle) is already there.
+// Identical tracked poses come from the same window, so they carry the same
+// score, and the kept set equals the reference's merged list (its order is
+// restored by the host where it matters).  Called by one thread.
+__device__ __forceinline__ uint32_t pose_hash(double row, double col, double angle) {
+  uint64_t h = uint64_t(__double_as_longlong(row + 0.0)) * 0x9E3779B97F4A7C15ull;
+  h ^= uint64_t(__double_as_longlong(col + 0.0)) + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2);
+  h ^= uint64_t(__double_as_longlong(angle + 0.0)) * 0xC2B2AE3D27D4EB4Full;
+  h ^= h >> 29;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 32;
+  return uint32_t(h);
+}
+__device__ __forceinline__ double load_shared_double(const double* p) {
+  return __longlong_as_double(
+      __hip_atomic_load(reinterpret_cast<const long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void keep_distinct(const TrackArgs& a, int f, int ci, const NccMatch& res) {
+  const size_t base = size_t(f) * a.cap;
+  a.raw[base + ci] = res;
+  if (!isfinite(res.score) || res.score < a.keep) return;
+  __threadfence();  // the raw pose is visible before its slot is published
+  int32_t* T = a.table + size_t(f) * a.table_size;
+  const uint32_t mask = uint32_t(a.table_size - 1);
+  uint32_t h = pose_hash(res.row, res.col, res.angle) & mask;
+  for (int probe = 0; probe < a.table_size; ++probe, h = (h + 1) & mask) {
+    const int prev = atomicCAS(T + h, -1, ci);
+    if (prev == -1) {
+      const int k = atomicAdd(a.out_count + f, 1);
+      if (k < a.cap) a.out[base + k] = res;
+      return;
+    }
+    __threadfence();
+    const NccMatch* o = a.raw + base + prev;
+    if (load_shared_double(&o->row) == res.row && load_shared_double(&o->col) == res.col &&
+        load_shared_double(&o->angle) == res.angle)
+      return;  // duplicate of a kept pose
+  }
+}
+
 constexpr int kTrackThreads = 256;
 
 __device__ __forceinline__ void anchor(const TrackArgs& a, const NccMatch& cd, const NccPatch& P, int& row, int& col) {
@@ -251,12 +300,20 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   __shared__ int box[4];
   __shared__ double red_s[kTrackThreads];
   __shared__ int red_i[kTrackThreads];
+  __shared__ int off[NCC_MAX_BATCH + 1];
+  list_offsets(args.in_count, args.n_frames, args.cap, off);
+  const int total = off[args.n_frames];
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
   TrackArgs a = args;
-  const int ci = blockIdx.x;
-  if (ci >= a.n_cand) return;
-  bind_frame(a, ci);
-  const NccMatch cd = a.cand[ci];
+  const int f = list_frame(off, a.n_frames, g), ci = g - off[f];
+  bind_frame(a, f);
+  const NccMatch cd = a.in[size_t(f) * a.cap + ci];
   const int n_e = eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt);
+  if (threadIdx.x == 0 && a.work) {
+    unsigned long long samples = 0;
+    for (int e = 0; e < n_e; ++e) samples += 25ull * uint32_t(a.pools.patch(a.refs[elig[e]]).n);
+    atomicAdd(a.work, samples);
+  }
   // union of the windows' sample rectangles, clipped to the valid region
   const int lo_r = max(0, a.sr), lo_c = max(0, a.sc);
   const int hi_r = min(a.lv.height, a.sr + a.sh) - 1, hi_c = min(a.lv.width, a.sc + a.sw) - 1;
@@ -337,7 +394,9 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
       res.angle = P.angle;
       res.score = red_s[0];
     }
-    a.out[ci] = res;
+    keep_distinct(a, f, ci, res);
+  }
+  __syncthreads();  // the LDS of this candidate is reused by the next
   }
 }
 
@@ -356,6 +415,9 @@ struct RefineArgs {
   int subpixel, rot, absolute;
   const NccMatch* cand;
   int n_cand;
+  // list form (full score): frame f's candidates at cand + f * cap
+  int n_frames, cap;
+  const int32_t* in_count;
   NccMatch* out_full;
   NccMatch* out_refined;
   int32_t* verdict;
@@ -370,8 +432,8 @@ struct RefineArgs {
   int tr0, tc0, th, tw;
 };
 
-__device__ __forceinline__ void bind_frame(RefineArgs& a, int ci) {
-  const NccFrame& F = a.frames[a.cand_frame[ci]];
+__device__ __forceinline__ void bind_frame(RefineArgs& a, int f) {
+  const NccFrame& F = a.frames[f];
   a.lv = F.levels[a.finest];
   a.dc = double(*F.dc_src);
   a.crop_r = F.crop_r;
@@ -805,23 +867,30 @@ __device__ __forceinline__ double template_reach(const RefineArgs& a, const NccP
 
 // One block per candidate: the continuous score at the finest level
 // (matcher.cpp:813-822).
+// Candidates in list form (frame f's at f * cap); results in the same slots.
 template <class T>
 __global__ void __launch_bounds__(kRefineThreads) k_full_score(const RefineArgs args) {
-  RefineArgs a = args;
   __shared__ Shared sm;
+  __shared__ int off[NCC_MAX_BATCH + 1];
   extern __shared__ uint8_t lds_tile[];
-  const int ci = blockIdx.x;
-  if (ci >= a.n_cand) return;
-  bind_frame(a, ci);
-  const PatchRef O = patch_ref(a.pools, a.orig_ref);
-  NccMatch cd = a.cand[ci];
-  {
-    double pr, pc;
-    to_local(a, cd.row, cd.col, pr, pc);
-    stage_tile<T>(a, lds_tile, pr, pc, template_reach(a, O.P) + 2);
+  list_offsets(args.in_count, args.n_frames, args.cap, off);
+  const int total = off[args.n_frames];
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    RefineArgs a = args;
+    const int f = list_frame(off, a.n_frames, g);
+    const size_t slot = size_t(f) * a.cap + (g - off[f]);
+    bind_frame(a, f);
+    const PatchRef O = patch_ref(a.pools, a.orig_ref);
+    NccMatch cd = a.cand[slot];
+    {
+      double pr, pc;
+      to_local(a, cd.row, cd.col, pr, pc);
+      stage_tile<T>(a, lds_tile, pr, pc, template_reach(a, O.P) + 2);
+    }
+    if (isfinite(cd.score)) cd.score = continuous<T>(a, O, cd.row, cd.col, cd.angle, sm);
+    if (threadIdx.x == 0) a.out_full[slot] = cd;
+    __syncthreads();  // the LDS tile is restaged for the next candidate
   }
-  if (isfinite(cd.score)) cd.score = continuous<T>(a, O, cd.row, cd.col, cd.angle, sm);
-  if (threadIdx.x == 0) a.out_full[ci] = cd;
 }
 
 // One block per candidate: refine_candidate (matcher.cpp:853-1061) from a
@@ -836,7 +905,7 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
   extern __shared__ uint8_t lds_tile[];
   const int ci = blockIdx.x;
   if (ci >= a.n_cand) return;
-  bind_frame(a, ci);
+  bind_frame(a, a.cand_frame[ci]);
   const PatchRef O = patch_ref(a.pools, a.orig_ref);
   NccMatch cd = a.cand[ci];
   if (!isfinite(cd.score)) {
@@ -874,21 +943,24 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
 
 using namespace ncc;
 
-extern "C" int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, int level, int level_kind,
-                            const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* cand,
-                            NccMatch* out, int n_cand, double tol, double step, double offset, int exact_lattice,
-                            int absolute, int tile_pixels, int exact_f64, void* stream) {
-  if (n_cand <= 0) return 0;
+extern "C" int ncchip_track(const NccFrame* frames, int n_frames, int level, int level_kind, const NccPatchPool* pools,
+                            const int32_t* refs, int n_refs, const NccMatch* in, const int32_t* in_count, int cap,
+                            NccMatch* raw, NccMatch* out, int32_t* out_count, int32_t* table, int table_size,
+                            double keep, double tol, double step, double offset, int exact_lattice, int absolute,
+                            int tile_pixels, int exact_f64, unsigned long long* work, int grid, void* stream) {
+  if (n_frames <= 0 || grid <= 0) return 0;
+  if (n_frames > NCC_MAX_BATCH || table_size < 2 || (table_size & (table_size - 1))) return -1;
   TrackArgs a{};
-  a.frames = frames; a.cand_frame = cand_frame; a.level = level; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
-  a.refs = refs; a.n_refs = n_refs; a.cand = cand; a.out = out; a.n_cand = n_cand; a.tol = tol; a.step = step;
-  a.offset = offset; a.exact_lattice = exact_lattice; a.absolute = absolute;
+  a.frames = frames; a.n_frames = n_frames; a.level = level; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
+  a.refs = refs; a.n_refs = n_refs; a.in = in; a.in_count = in_count; a.cap = cap; a.raw = raw; a.out = out;
+  a.out_count = out_count; a.table = table; a.table_size = table_size; a.keep = keep; a.work = work;
+  a.tol = tol; a.step = step; a.offset = offset; a.exact_lattice = exact_lattice; a.absolute = absolute;
   const size_t px = level_kind == NCC_U8 ? 1 : level_kind == NCC_U16 ? 2 : 4;
   constexpr size_t kMaxTileBytes = 64 * 1024;
   a.tile_cap = tile_pixels > 0 && size_t(tile_pixels) * px <= kMaxTileBytes ? tile_pixels : 0;
   const size_t lds = (size_t(a.tile_cap) * px + 15) & ~size_t(15);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 g(n_cand), b(kTrackThreads);
+  const dim3 g(grid), b(kTrackThreads);
   if (level_kind == NCC_U8) {
     if (exact_f64) hipLaunchKernelGGL((k_track<uint8_t, double>), g, b, lds, st, a);
     else hipLaunchKernelGGL((k_track<uint8_t, int64_t>), g, b, lds, st, a);
@@ -903,26 +975,26 @@ extern "C" int ncchip_track(const NccFrame* frames, const int32_t* cand_frame, i
 }
 
 namespace {
-RefineArgs refine_args(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
-                       const int32_t* cand_frame, int n_cand) {
+RefineArgs refine_args(const NcchipRefineParams* p, const NccPatchPool* pools) {
   RefineArgs a{};
-  a.frames = p->frames; a.cand_frame = cand_frame; a.scale = p->scale; a.phase = p->phase;
+  a.frames = p->frames; a.scale = p->scale; a.phase = p->phase;
   a.finest = p->finest; a.top = p->top; a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.orig_ref = p->orig_ref;
   a.lv_orow = p->level_orow; a.lv_ocol = p->level_ocol; a.refs0 = p->refs0; a.n_refs0 = p->n_refs0;
   a.step = p->step; a.a_start = p->angle_start; a.a_end = p->angle_end; a.min_score = p->min_score;
   a.subpixel = p->subpixel; a.rot = p->rotate; a.absolute = p->absolute;
-  a.cand = cand; a.n_cand = n_cand;
   return a;
 }
 }  // namespace
 
-extern "C" int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
-                                 const int32_t* cand_frame, int n_cand, NccMatch* out_full, void* stream) {
-  if (n_cand <= 0) return 0;
-  RefineArgs a = refine_args(p, pools, cand, cand_frame, n_cand);
-  a.out_full = out_full;
+extern "C" int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, int n_frames,
+                                 const NccMatch* in, const int32_t* in_count, int cap, NccMatch* out_full, int grid,
+                                 void* stream) {
+  if (n_frames <= 0 || grid <= 0) return 0;
+  if (n_frames > NCC_MAX_BATCH) return -1;
+  RefineArgs a = refine_args(p, pools);
+  a.cand = in; a.n_frames = n_frames; a.in_count = in_count; a.cap = cap; a.out_full = out_full;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 g(n_cand), b(kRefineThreads);
+  const dim3 g(grid), b(kRefineThreads);
   if (p->level_kind == NCC_U8) hipLaunchKernelGGL(k_full_score<uint8_t>, g, b, kRefineTileBytes, st, a);
   else if (p->level_kind == NCC_U16) hipLaunchKernelGGL(k_full_score<uint16_t>, g, b, kRefineTileBytes, st, a);
   else hipLaunchKernelGGL(k_full_score<uint32_t>, g, b, kRefineTileBytes, st, a);
@@ -933,9 +1005,8 @@ extern "C" int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* po
                              const int32_t* cand_frame, int n_cand, NccMatch* out_refined, int32_t* verdict,
                              void* stream) {
   if (n_cand <= 0) return 0;
-  RefineArgs a = refine_args(p, pools, cand, cand_frame, n_cand);
-  a.out_refined = out_refined;
-  a.verdict = verdict;
+  RefineArgs a = refine_args(p, pools);
+  a.cand = cand; a.cand_frame = cand_frame; a.n_cand = n_cand; a.out_refined = out_refined; a.verdict = verdict;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 g(n_cand), b(kRefineThreads);
   if (p->level_kind == NCC_U8) hipLaunchKernelGGL(k_refine<uint8_t>, g, b, kRefineTileBytes, st, a);
