@@ -263,12 +263,15 @@ struct PeakArgs {
   unsigned long long* extent;  // per frame: min row, max row, min col, max col (FP64 bits)
 };
 
+constexpr int kPeakRows = 8;  // plane rows per block
+
 __global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
   const int f = blockIdx.z / a.n_angles, ai = blockIdx.z % a.n_angles;
   const NccFrame& F = a.frames[f];
   const int H = F.levels[a.level].height, W = F.levels[a.level].width;
-  const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
-  if (r >= H || int(blockIdx.x) * 256 >= W) return;  // uniform per block
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (int(blockIdx.x) * 256 >= W) return;  // uniform per block
+  for (int r = blockIdx.y * kPeakRows; r < min(H, int(blockIdx.y + 1) * kPeakRows); ++r) {
   bool peak = false;
   double v = 0;
   if (c < W) {
@@ -287,7 +290,7 @@ __global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
   }
   // one atomic per wave for the slots and the extent
   const uint64_t m = __ballot(peak);
-  if (!m) return;
+  if (!m) continue;
   const int lane = threadIdx.x & 63, leader = __builtin_ctzll(m);
   int base = 0;
   if (lane == leader) base = atomicAdd(a.count + f, __popcll(m));
@@ -323,6 +326,7 @@ __global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
     atomicMax(e + 1, __double_as_longlong(xrow));
     atomicMin(e + 2, __double_as_longlong(ocol));
     atomicMax(e + 3, __double_as_longlong(xcol));
+  }
   }
 }
 
@@ -390,7 +394,7 @@ extern "C" int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int
   a.frames = frames; a.level = level; a.n_angles = n_angles; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
   a.patch_ids = patch_ids; a.thr = threshold; a.scale = scale; a.phase = phase; a.cand = cand; a.key = key;
   a.cap = cap; a.count = count; a.extent = extent;
-  const dim3 grid((max_w + 255) / 256, max_h, unsigned(n_frames * n_angles));
+  const dim3 grid((max_w + 255) / 256, (max_h + kPeakRows - 1) / kPeakRows, unsigned(n_frames * n_angles));
   hipLaunchKernelGGL(k_peaks, grid, dim3(256), 0, static_cast<hipStream_t>(stream), a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

@@ -96,19 +96,65 @@ __device__ __forceinline__ double window_score(const TrackArgs& a, const NccPatc
   const double cnt = double(P.n);
   const double var = sq - sum * sum / cnt;
   dot = dot - sum / cnt * P.wsum;
+  double den = var;
   if (!(var > 128 * __DBL_EPSILON__ * sq)) {
     if (a.exact_lattice || (__int128)t2 * P.n == (__int128)t1 * t1) return NCC_MISSING;
-    const double v = ldexp(double((__int128)t2 * P.n - (__int128)t1 * t1), -2 * a.lv.shift) / cnt;
-    const double sc = clamp1(dot / sqrt(v));
-    return a.absolute ? fabs(sc) : sc;
+    den = ldexp(double((__int128)t2 * P.n - (__int128)t1 * t1), -2 * a.lv.shift) / cnt;
   }
-  const double sc = clamp1(dot / sqrt(var));
+  const double sc = clamp1(dot / sqrt(den));
   return a.absolute ? fabs(sc) : sc;
 }
 
 __device__ __forceinline__ bool window_inside(const TrackArgs& a, const NccPatch& P, int row, int col) {
   return row + P.rmin >= 0 && col + P.cmin >= 0 && row + P.rmax < a.lv.height && col + P.cmax < a.lv.width &&
          row + P.rmin >= a.sr && col + P.cmin >= a.sc && row + P.rmax < a.sr + a.sh && col + P.cmax < a.sc + a.sw;
+}
+
+// Exchanges of doubles for the transposed butterfly, as pairs of dwords:
+// permlane32_swap gives x' = (x lanes 0-31, y lanes 0-31) and
+// y' = (x lanes 32-63, y lanes 32-63); permlane16_swap does the same per
+// pair of 16-lane rows.
+__device__ __forceinline__ double sum_swap32(double x, double y) {
+  const uint64_t a = uint64_t(__double_as_longlong(x)), b = uint64_t(__double_as_longlong(y));
+  const auto lo = __builtin_amdgcn_permlane32_swap(uint32_t(a), uint32_t(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(uint32_t(a >> 32), uint32_t(b >> 32), false, false);
+  const double x2 = __longlong_as_double(int64_t((uint64_t(hi[0]) << 32) | lo[0]));
+  const double y2 = __longlong_as_double(int64_t((uint64_t(hi[1]) << 32) | lo[1]));
+  return x2 + y2;
+}
+__device__ __forceinline__ double sum_swap16(double x, double y) {
+  const uint64_t a = uint64_t(__double_as_longlong(x)), b = uint64_t(__double_as_longlong(y));
+  const auto lo = __builtin_amdgcn_permlane16_swap(uint32_t(a), uint32_t(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(uint32_t(a >> 32), uint32_t(b >> 32), false, false);
+  const double x2 = __longlong_as_double(int64_t((uint64_t(hi[0]) << 32) | lo[0]));
+  const double y2 = __longlong_as_double(int64_t((uint64_t(hi[1]) << 32) | lo[1]));
+  return x2 + y2;
+}
+
+// Sums 16 values over the wave with one transposed butterfly: at each of the
+// first four halvings a lane keeps half of its values and adds its partner's
+// copies of them, so the totals cost 17 exchanges instead of 96.  On return
+// lane L holds the total of value (L >> 2) & 15.
+__device__ __forceinline__ double wave_transpose_sum16(double (&v)[16]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = sum_swap32(v[j], v[j + 8]);  // lanes 32-63 keep j + 8
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = sum_swap16(v[j], v[j + 4]);  // odd rows keep j + 4
+#pragma unroll
+  for (int half = 2, m = 8; half >= 1; half >>= 1, m >>= 1) {
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const double send = hi ? v[j] : v[j + half];
+      const double keep = hi ? v[j + half] : v[j];
+      v[j] = keep + __shfl_xor(send, m, 64);
+    }
+  }
+  double t = v[0];
+  t += __shfl_xor(t, 2, 64);
+  t += __shfl_xor(t, 1, 64);
+  return t;
 }
 
 // The five windows (row, col-2 .. col+2) of one template row offset, evaluated
@@ -119,22 +165,23 @@ __device__ __forceinline__ bool window_inside(const TrackArgs& a, const NccPatch
 // 2^53 (the host checks n * max|x|^2 for the level); Acc = int64_t otherwise.
 // The dot product is sum(x * w) scaled by 4^-l at the end, the reference's
 // per-sample products up to a power of two; the sums run in a different order
-// than the reference's single accumulator, so scores agree to ~1e-15.
+// than the reference's single accumulator, so scores agree to ~1e-15.  Lane L
+// reports window k = (L >> 2) & 15 when k < 5 (score, window = k), and
+// window = -1 otherwise; the window scores are computed lane-parallel.
 template <class Acc, class Reader>
 __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p, int row, int col, const Reader& rd,
-                                          double (&score)[5]) {
+                                          double& score, int& window) {
   const NccPatch& P = p.P;
   bool in[5];
   bool any = false;
 #pragma unroll
   for (int k = 0; k < 5; ++k) { in[k] = window_inside(a, P, row, col - 2 + k); any |= in[k]; }
-  if (!any) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) score[k] = NCC_MISSING;
-    return;
-  }
-  // windows partly outside read clamped columns and are discarded below
   const int lane = threadIdx.x & 63;
+  const int q = (lane >> 2) & 15;  // the value (and window) this lane reports
+  window = q < 5 ? q : -1;
+  score = NCC_MISSING;
+  if (!any) return;
+  // windows partly outside read clamped columns and are discarded below
   Acc t1[5] = {0, 0, 0, 0, 0}, t2[5] = {0, 0, 0, 0, 0};
   double dot[5] = {0, 0, 0, 0, 0};
   int kmin = 0, kmax = 4;
@@ -198,19 +245,28 @@ __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p,
     }
   }
   const double unit = ldexp(1.0, -a.lv.shift);
+  double d = 0;
+  int64_t T1 = 0, T2 = 0;
+  if constexpr (std::is_same<Acc, double>::value) {
+    // one transposed reduction of the 15 sums; lane L ends with value
+    // (L >> 2) & 15: dot products 0-4, sum(x) 5-9, sum(x^2) 10-14
+    double v[16];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    int64_t T1, T2;
-    if constexpr (std::is_same<Acc, double>::value) {
-      T1 = int64_t(wave_sum(t1[k]));
-      T2 = int64_t(wave_sum(t2[k]));
-    } else {
-      T1 = wave_sum_i64(t1[k]);
-      T2 = wave_sum_i64(t2[k]);
+    for (int k = 0; k < 5; ++k) { v[k] = dot[k]; v[5 + k] = t1[k]; v[10 + k] = t2[k]; }
+    v[15] = 0;
+    const double tot = wave_transpose_sum16(v);
+    d = tot;
+    T1 = int64_t(__shfl(tot, ((q + 5) & 15) * 4 + (lane & 3), 64));
+    T2 = int64_t(__shfl(tot, ((q + 10) & 15) * 4 + (lane & 3), 64));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int64_t s1 = wave_sum_i64(t1[k]), s2 = wave_sum_i64(t2[k]);
+      const double dk = wave_sum(dot[k]);
+      if (k == q) { d = dk; T1 = s1; T2 = s2; }
     }
-    const double d = wave_sum(dot[k]) * unit;
-    score[k] = in[k] ? window_score(a, P, d, T1, T2) : NCC_MISSING;
   }
+  if (q < 5 && window_inside(a, P, row, col - 2 + q)) score = window_score(a, P, d * unit, T1, T2);
 }
 
 constexpr int kMaxEligible = 1024;
@@ -305,7 +361,10 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   const int total = off[args.n_frames];
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
   TrackArgs a = args;
-  const int f = list_frame(off, a.n_frames, g), ci = g - off[f];
+  // (frame, entry) are block-uniform: keep them, and what is loaded through
+  // them, in scalar registers
+  const int f = __builtin_amdgcn_readfirstlane(list_frame(off, a.n_frames, g));
+  const int ci = __builtin_amdgcn_readfirstlane(g - off[f]);
   bind_frame(a, f);
   const NccMatch cd = a.in[size_t(f) * a.cap + ci];
   const int n_e = eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt);
@@ -360,12 +419,11 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
     row = __builtin_amdgcn_readfirstlane(row);
     col = __builtin_amdgcn_readfirstlane(col);
     const int dr = task % 5 - 2;
-    double sc[5];
-    if (use_tile) track_row<Acc>(a, p, row + dr, col, TileReader<T>{tile, r0, c0, tw}, sc);
-    else track_row<Acc>(a, p, row + dr, col, LevelReader<T>{lvp, a.lv.pitch}, sc);
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-      if (sc[k] > best) { best = sc[k]; best_i = (task / 5) * 25 + (dr + 2) * 5 + k; }
+    double sc;
+    int k;
+    if (use_tile) track_row<Acc>(a, p, row + dr, col, TileReader<T>{tile, r0, c0, tw}, sc, k);
+    else track_row<Acc>(a, p, row + dr, col, LevelReader<T>{lvp, a.lv.pitch}, sc, k);
+    if (k >= 0 && sc > best) { best = sc; best_i = (task / 5) * 25 + (dr + 2) * 5 + k; }
   }
   red_s[threadIdx.x] = best;
   red_i[threadIdx.x] = best_i;
