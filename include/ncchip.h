@@ -65,12 +65,38 @@ int ncchip_coarse(const NccFrame* frames, int n_frames, int level, int level_kin
                   const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, int max_bbox_h, int max_bbox_w,
                   int form, int exact_lattice, int absolute, void* stream);
 
-/* Split-BF16 / FP8-screen MFMA coarse search (same planes as ncchip_coarse).
- * `screen` > -inf enables the FP8-E4M3 screen: anchors whose screened score
- * is below it are written -inf, survivors are re-scored in split-BF16. */
+/* Dense split-BF16 weights of the coarse patches over their union bounding
+ * box (rows ymin .. ymin + yspan - 1, columns xmin .. xmin + 16 * xchunks - 1),
+ * laid out as MFMA B fragments: for angle tile t (32 angles), template row
+ * index yi and 16-column chunk ci, 64 lanes x 8 bf16 (lane L: angle
+ * 32 t + L % 32, columns 16 ci + 8 (L / 32) .. + 7).  wsum_eff[a] is the FP64
+ * sum of w_hi + w_lo over angle a's samples; max_run the longest row run. */
+typedef struct NcchipDenseWeights {
+  const void* wh;          /* device, 16-byte aligned */
+  const void* wl;
+  const double* wsum_eff;  /* device */
+  int32_t ymin, xmin, yspan, xchunks, max_run;
+  /* Runs (4 ints: row, first col, last col, 0) of the support common to
+   * every coarse patch, and its sample count.  Its variance bounds every
+   * patch's window variance from below, so |score| <= |num| / sqrt(var_common);
+   * windows whose bound is below prune_below are written -inf without exact
+   * moments (they can neither be peaks nor suppress one when prune_below is
+   * under the peak threshold).  n_common = 0 or prune_below <= 0 disables it. */
+  const int32_t* common_runs;
+  int32_t n_common_runs, n_common;
+  double prune_below;
+} NcchipDenseWeights;
+
+/* Split-BF16 MFMA coarse search (same planes as ncchip_coarse): the dot
+ * products on v_mfma_f32_32x32x16_bf16 (image split T_hi + T_lo, weights
+ * w_hi + w_lo, three products in FP32), exact integer window moments and the
+ * shared FP64 normalisation.  Top levels of kind NCC_U8 / NCC_U16 up to level 4.
+ * `screen` > -inf asks for the FP8-E4M3 screen, which this build does not
+ * provide (returns -6 = MV_STATUS_UNSUPPORTED). */
 int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
-                       double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, int bbox_h,
-                       int bbox_w, int exact_lattice, int absolute, double screen, void* stream);
+                       double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles,
+                       const NcchipDenseWeights* weights, int exact_lattice, int absolute, double screen,
+                       void* stream);
 
 /* Per-angle 3x3 peaks of every frame's planes with the reference's tie rule
  * (matcher.cpp:580-594), emitted as candidate poses (matcher.cpp:590-592:

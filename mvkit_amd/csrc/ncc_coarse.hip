@@ -13,35 +13,6 @@
 
 namespace ncc {
 
-// FP64 normalisation of one window from its dot product with the centred
-// pixels.  dot_c = sum_i wf_i * (S_i - C) (any precision), wsum_f = sum_i wf_i.
-// The reference's numerator is sum_i w_i q_i - (sum q / n) * wsum with
-// q = S 4^-l - dc; the window moments are exact.
-__device__ __forceinline__ double normalise(double dot_c, double C, double wsum_f, double wsum, int n, int64_t sum_s,
-                                            int64_t sum_s2, int64_t D, int shift, double dc, bool exact_lattice,
-                                            bool absolute) {
-  double sum, sq;
-  exact_moments(sum_s, sum_s2, n, D, shift, sum, sq);
-  const double cnt = double(n);
-  const double var = sq - sum * sum / cnt;
-  const double dot_q = ldexp(dot_c + C * wsum_f, -shift) - dc * wsum;  // sum_i w_i q_i
-  const double num = dot_q - sum / cnt * wsum;
-  if (!(var > 128 * __DBL_EPSILON__ * sq)) {
-    // Low-variance rule (kernels_avx2.cpp:128-129, 166-167): -inf on the exact
-    // byte lattice; elsewhere the reference recomputes directly, which only
-    // differs from -inf when the window is not constant.
-    const int64_t t1 = sum_s - int64_t(n) * D;
-    const int64_t t2 = sum_s2 - 2 * D * sum_s + int64_t(n) * D * D;
-    const bool constant = (__int128)t2 * n == (__int128)t1 * t1;
-    if (exact_lattice || constant) return NCC_MISSING;
-    const double v = ldexp(double((__int128)t2 * n - (__int128)t1 * t1), -2 * shift) / cnt;
-    const double s = clamp1(num / sqrt(v));
-    return absolute ? fabs(s) : s;
-  }
-  const double s = clamp1(num / sqrt(var));
-  return absolute ? fabs(s) : s;
-}
-
 // Exact window moments over a patch's runs from the row prefixes.
 __device__ __forceinline__ void window_moments(const uint32_t* __restrict__ p1, const uint64_t* __restrict__ p2,
                                                int pitch, const int32_t* __restrict__ runs, int n_runs, int r, int c,

@@ -79,6 +79,90 @@ NccPatchPool view_at(uint8_t* base, const PoolLayout& L) {
 
 int32_t ref(int pool, uint32_t index) { return int32_t((uint32_t(pool) << 24) | index); }
 
+// Round-to-nearest-even BF16 bits of a double (via FP32).
+uint16_t bf16_bits(double v) {
+  const float f = float(v);
+  uint32_t b;
+  std::memcpy(&b, &f, 4);
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return uint16_t(b >> 16);
+}
+double bf16_value(uint16_t h) {
+  const uint32_t b = uint32_t(h) << 16;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+
+// Split-BF16 dense weights of the coarse patches, laid out as MFMA B
+// fragments (see NcchipDenseWeights).
+struct DenseWeights {
+  std::vector<uint16_t> wh, wl;  // [tile][yspan][xchunks][64 lanes][8]
+  std::vector<double> wsum_eff;  // per angle
+  std::vector<int32_t> common_runs;  // 4 ints per run of the common support
+  int n_common = 0;
+  int ymin = 0, xmin = 0, yspan = 0, xchunks = 0, max_run = 0;
+};
+DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std::vector<const RasterPool*>& pools) {
+  DenseWeights d;
+  int ymax = 0, xmax = 0;
+  d.ymin = 1 << 30; d.xmin = 1 << 30; ymax = -(1 << 30); xmax = -(1 << 30);
+  for (const Raster* p : patches) {
+    d.ymin = std::min(d.ymin, p->rmin); ymax = std::max(ymax, p->rmax);
+    d.xmin = std::min(d.xmin, p->cmin); xmax = std::max(xmax, p->cmax);
+  }
+  d.yspan = ymax - d.ymin + 1;
+  d.xchunks = (xmax - d.xmin + 16) / 16;
+  const size_t tiles = (patches.size() + 31) / 32;
+  d.wh.assign(tiles * size_t(d.yspan) * d.xchunks * 64 * 8, 0);
+  d.wl.assign(d.wh.size(), 0);
+  d.wsum_eff.assign(patches.size(), 0.0);
+  for (size_t a = 0; a < patches.size(); ++a) {
+    const Raster& p = *patches[a];
+    const RasterPool& pool = *pools[a];
+    const size_t t = a / 32, slot = a % 32;
+    double ws = 0;
+    for (uint32_t i = 0; i < p.count; ++i) {
+      const int32_t rc = pool.rc[p.first + i];
+      const int y = rc >> 16, x = int(int16_t(rc & 0xffff));
+      const double w = pool.w[p.first + i];
+      const uint16_t h = bf16_bits(w), l = bf16_bits(w - bf16_value(h));
+      ws += bf16_value(h) + bf16_value(l);
+      const int yi = y - d.ymin, xi = x - d.xmin;
+      const int ci = xi / 16, g = (xi % 16) / 8, j = xi % 8;
+      const size_t lane = size_t(g) * 32 + slot;
+      const size_t e = ((((t * d.yspan + size_t(yi)) * d.xchunks + size_t(ci)) * 64 + lane) * 8) + size_t(j);
+      d.wh[e] = h;
+      d.wl[e] = l;
+    }
+    d.wsum_eff[a] = ws;
+    for (uint32_t k = 0; k < p.run_count; ++k) {
+      const int32_t* run = &pool.runs[4 * size_t(p.run_first + k)];
+      d.max_run = std::max(d.max_run, run[2] - run[1] + 1);
+    }
+  }
+  // support common to every patch, as row runs
+  const int xspan = xmax - d.xmin + 1;
+  std::vector<int> hits(size_t(d.yspan) * xspan, 0);
+  for (size_t a = 0; a < patches.size(); ++a) {
+    const Raster& p = *patches[a];
+    const RasterPool& pool = *pools[a];
+    for (uint32_t i = 0; i < p.count; ++i) {
+      const int32_t rc = pool.rc[p.first + i];
+      ++hits[size_t((rc >> 16) - d.ymin) * xspan + (int(int16_t(rc & 0xffff)) - d.xmin)];
+    }
+  }
+  for (int y = 0; y < d.yspan; ++y)
+    for (int x = 0; x < xspan;) {
+      if (hits[size_t(y) * xspan + x] != int(patches.size())) { ++x; continue; }
+      const int b = x;
+      while (x < xspan && hits[size_t(y) * xspan + x] == int(patches.size())) ++x;
+      d.common_runs.insert(d.common_runs.end(), {y + d.ymin, b + d.xmin, x - 1 + d.xmin, 0});
+      d.n_common += x - b;
+    }
+  return d;
+}
+
 bool pose_less(const Pose& a, const Pose& b) {  // merge_identical order (matcher.cpp:242-244)
   if (a.row != b.row) return a.row < b.row;
   if (a.col != b.col) return a.col < b.col;
@@ -358,6 +442,20 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     max_bw = std::max(max_bw, p.cmax - p.cmin);
   }
 
+  if (o.form == NCCHIP_FORM_FP8_SCREEN) raise(kUnsupported, "form", "the FP8 screen form is not available in this build");
+  if (o.form < NCCHIP_FORM_FP32 || o.form > NCCHIP_FORM_FP8_SCREEN) raise(kInvalidArgument, "form", "unknown correlation form");
+  const bool use_mfma = o.form == NCCHIP_FORM_BF16X3;
+  DenseWeights dense;
+  if (use_mfma) {
+    std::vector<const Raster*> ps;
+    std::vector<const RasterPool*> pp;
+    for (size_t i = 0; i < A; ++i) {
+      ps.push_back(&raster_of(refs[top][i]));
+      pp.push_back((refs[top][i] >> 24) ? &qpool : &model.pool);
+    }
+    dense = dense_weights(ps, pp);
+  }
+
   // ---- per-frame geometry: crop (matcher.cpp:383-393), pyramid levels, coarse anchor ranges (:549-553)
   std::vector<Job> jobs(n_frames);
   const int64_t lattice = int64_t(size_t(1) << (level_count - 1));
@@ -434,6 +532,10 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   for (auto& J : jobs) { J.off_ranges = off_small + small; small = align_up(small + J.ranges.size() * 4); }
   const size_t off_qpool = off_small + small; small = align_up(small + QL.total);
   const size_t off_extent = off_small + small; small = align_up(small + n_frames * 32);
+  const size_t off_wh = off_small + small; small = align_up(small + dense.wh.size() * 2);
+  const size_t off_wl = off_small + small; small = align_up(small + dense.wl.size() * 2);
+  const size_t off_wsum = off_small + small; small = align_up(small + dense.wsum_eff.size() * 8);
+  const size_t off_common = off_small + small; small = align_up(small + dense.common_runs.size() * 4);
   const size_t off_frames = off_small + small; small = align_up(small + n_frames * sizeof(NccFrame));
   off = off_small + small;
   if (work_.bytes < off) {
@@ -462,6 +564,12 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
       std::memcpy(st + (off_refs[l] - off_small), refs[l].data(), refs[l].size() * 4);
     for (auto& J : jobs) std::memcpy(st + (J.off_ranges - off_small), J.ranges.data(), J.ranges.size() * 4);
     serialize(qpool, QL, st + (off_qpool - off_small));
+    if (use_mfma) {
+      std::memcpy(st + (off_wh - off_small), dense.wh.data(), dense.wh.size() * 2);
+      std::memcpy(st + (off_wl - off_small), dense.wl.data(), dense.wl.size() * 2);
+      std::memcpy(st + (off_wsum - off_small), dense.wsum_eff.data(), dense.wsum_eff.size() * 8);
+      std::memcpy(st + (off_common - off_small), dense.common_runs.data(), dense.common_runs.size() * 4);
+    }
     auto* ext = reinterpret_cast<double*>(st + (off_extent - off_small));
     auto* fr = reinterpret_cast<NccFrame*>(st + (off_frames - off_small));
     for (size_t f = 0; f < n_frames; ++f) {
@@ -519,10 +627,19 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
   time_begin(kStageCoarse);
   int rc = 0;
-  if (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN)
+  if (use_mfma) {
+    NcchipDenseWeights dw{};
+    dw.wh = dev + off_wh; dw.wl = dev + off_wl; dw.wsum_eff = reinterpret_cast<const double*>(dev + off_wsum);
+    dw.ymin = dense.ymin; dw.xmin = dense.xmin; dw.yspan = dense.yspan; dw.xchunks = dense.xchunks;
+    dw.max_run = dense.max_run;
+    dw.common_runs = reinterpret_cast<const int32_t*>(dev + off_common);
+    dw.n_common_runs = int32_t(dense.common_runs.size() / 4);
+    dw.n_common = dense.n_common;
+    dw.prune_below = 0.9 * (o.min_score * .4);
     rc = ncchip_coarse_mfma(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
-                            max_bh, max_bw, exact_top, model.params.absolute,
+                            &dw, exact_top, model.params.absolute,
                             o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, stream_);
+  }
   else
     rc = ncchip_coarse(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
                        max_bh, max_bw, o.form, exact_top, model.params.absolute, stream_);

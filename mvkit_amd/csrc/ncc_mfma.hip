@@ -1,9 +1,354 @@
-// Split-BF16 MFMA coarse correlation (placeholder until the MFMA kernel lands:
-// returns UNSUPPORTED so callers fail loudly instead of falling back).
+// Split-BF16 MFMA coarse correlation (matcher.cpp:542-579 with the window
+// statistics of kernels_avx2.cpp:145-171) on gfx950 matrix cores.
+//
+// The coarse score planes are an implicit GEMM per output row:
+//   D[anchor c][angle a] = sum over taps (y, x) of W_a[y][x] * T[r + y][c + x]
+// with T = S - C the centred top-level block sums and W_a the dense weights of
+// coarse patch a over the union bounding box of all coarse patches (zero
+// where the patch has no sample).  v_mfma_f32_32x32x16_bf16 takes 32 anchors
+// (M), 32 angle slots (N) and 16 consecutive template columns of one template
+// row (K).  Precision: T = T_hi + T_lo with T_hi = 4^l * (floor(S / 4^l) - 128)
+// and T_lo = S mod 4^l, both exact in BF16 for levels <= 4; each weight is
+// split into BF16 w_hi + w_lo (w_lo = bf16(w - w_hi)); the products
+// T_hi*w_hi, T_hi*w_lo and T_lo*w_hi are exact in FP32 and accumulate in FP32
+// (T_lo*w_lo, below 2^-8 of the smallest kept term, is dropped).
+//
+// The window moments sum(S) and sum(S^2) over each patch's exact support are
+// exact integers, slid along each row run in 32-bit (64-bit where a run could
+// overflow) arithmetic on the vector ALUs, and the FP64 normalisation is the
+// one every correlation form shares, so only the dot product carries the
+// split-BF16 error.
+#include <type_traits>
+
 #include "ncc_common.hip.h"
 #include "../../include/ncchip.h"
 
-extern "C" int ncchip_coarse_mfma(const NccFrame*, int, int, int, int, int, double, const NccPatchPool*,
-                                  const int32_t*, int, int, int, int, int, double, void*) {
-  return -6;
+namespace ncc {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+
+constexpr int kThreads = 512;              // 8 waves: 2 row groups x 4 column groups
+constexpr int kWaveRows = 4;               // output rows per wave
+constexpr int kMRows = 2 * kWaveRows;      // output rows per block
+constexpr int kWaveCols = 32;              // anchors per wave (MFMA M)
+constexpr int kBlockCols = 4 * kWaveCols;
+
+struct MfmaArgs {
+  const NccFrame* frames;
+  int level;
+  int n_angles;              // coarse angles; slots beyond are zero weights
+  int n_tiles;               // angle tiles of 32; blockIdx.z = frame * n_tiles + tile
+  double C;                  // 128 * 4^level
+  Pools pools;
+  const int32_t* patch_ids;  // coarse patches as (pool << 24) | index
+  const uint4* wh;           // [tile][Yspan][XC][64 lanes] of 8 bf16 (w_hi)
+  const uint4* wl;           // same layout (w_lo)
+  const double* wsum_eff;    // per angle: sum of (w_hi + w_lo) in FP64
+  int ymin, xmin, yspan, xc; // union bounding box (rows, first column) and 16-column chunks
+  int pitch;                 // LDS row pitch (elements, even)
+  int exact_lattice, absolute;
+  const int4* common_runs;   // support shared by every coarse patch
+  int n_common_runs, n_common;
+  double prune_below;        // score bound under which windows are written -inf
+};
+
+// Calls fn(integral_constant<int, i>) for i = 0 .. N-1 (compile-time indices).
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  if constexpr (N > 0) {
+    static_for<N - 1>(fn);
+    fn(std::integral_constant<int, N - 1>{});
+  }
+}
+
+// Exact BF16 bits of an integer |v| <= 2^15 with at most 8 significant bits.
+__device__ __forceinline__ uint16_t bf16_bits_exact(int v) {
+  return uint16_t(__float_as_uint(float(v)) >> 16);
+}
+
+template <class Tpix, class RunAcc, bool kLdsB>
+__global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
+  extern __shared__ uint16_t lds[];
+  const int f = blockIdx.z / args.n_tiles, tile = blockIdx.z % args.n_tiles;
+  const NccFrame& F = args.frames[f];
+  const NccLevel lv = F.levels[args.level];
+  const int H = lv.height, W = lv.width;
+  const int r0 = blockIdx.y * kMRows, c0 = blockIdx.x * kBlockCols;
+  if (r0 >= H || c0 >= W) return;  // uniform per block
+  const int P = args.pitch;
+  const int rows = kMRows + args.yspan - 1;
+  const int cols = kBlockCols + 16 * args.xc + 1;
+  uint16_t* th0 = lds;                // T_hi bits
+  uint16_t* th1 = th0 + rows * P;     // T_hi shifted left by one column
+  uint16_t* tl0 = th1 + rows * P;     // T_lo bits
+  uint16_t* tl1 = tl0 + rows * P;
+  uint16_t* su = tl1 + rows * P;      // S
+  // the angle tile's split weights, staged once per block (16-byte aligned)
+  // when they fit beside the tile, else read from L2 in the loop
+  const int nb = args.yspan * args.xc * 64;  // B fragments (uint4) per half
+  uint4* bw = reinterpret_cast<uint4*>(lds + ((5 * rows * P + 7) & ~7));
+  const uint4* gwh = args.wh + size_t(tile) * nb;
+  const uint4* gwl = args.wl + size_t(tile) * nb;
+  const int Ci = int(args.C);
+  const int shift = lv.shift;
+  const int lo_mask = (1 << shift) - 1;
+  // ---- stage the tile: image rows r0 + ymin .., columns c0 + xmin ..
+  // (U loads in flight per thread before the first use)
+  {
+    constexpr int U = 8;
+    const int tr0 = r0 + args.ymin, tc0 = c0 + args.xmin;
+    const Tpix* img = static_cast<const Tpix*>(lv.data);
+    const int total = rows * cols;
+    for (int base = threadIdx.x; base < total; base += kThreads * U) {
+      int Sv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + kThreads * u;
+        const int tr = i / cols, tc = i % cols;
+        const int gr = tr0 + tr, gc = tc0 + tc;
+        Sv[u] = (i < total && gr >= 0 && gr < H && gc >= 0 && gc < W) ? int(img[size_t(gr) * lv.pitch + gc]) : Ci;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + kThreads * u;
+        if (i >= total) break;
+        const int tr = i / cols, tc = i % cols;
+        const int S = Sv[u];
+        const int lo = S & lo_mask;
+        const uint16_t hb = bf16_bits_exact(S - Ci - lo), lb = bf16_bits_exact(lo);
+        th0[tr * P + tc] = hb;
+        tl0[tr * P + tc] = lb;
+        if (tc > 0) { th1[tr * P + tc - 1] = hb; tl1[tr * P + tc - 1] = lb; }
+        su[tr * P + tc] = uint16_t(S);
+      }
+    }
+    if (kLdsB)
+      for (int base = threadIdx.x; base < 2 * nb; base += kThreads * U) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = base + kThreads * u;
+          v[u] = i < nb ? gwh[i] : (i < 2 * nb ? gwl[i - nb] : make_uint4(0, 0, 0, 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (base + kThreads * u < 2 * nb) bw[base + kThreads * u] = v[u];
+      }
+  }
+  __syncthreads();
+  // ---- MFMA: this wave's 32 anchors x 32 angle slots x kMRows rows
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wc = wave & 3, wr0 = (wave >> 2) * kWaveRows;  // column group, first block row of this wave
+  const int m = lane & 31, g = lane >> 5;
+  floatx16 acc[kWaveRows];
+#pragma unroll
+  for (int r = 0; r < kWaveRows; ++r)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[r][v] = 0.f;
+  const int col_base = wc * kWaveCols + m + 8 * g;  // tile column of this lane's first A element for chunk 0
+  for (int yi = 0; yi < args.yspan; ++yi) {
+    for (int ci = 0; ci < args.xc; ++ci) {
+      const int bi = (yi * args.xc + ci) * 64 + lane;
+      const uint4 bh4 = kLdsB ? bw[bi] : gwh[bi], bl4 = kLdsB ? bw[nb + bi] : gwl[bi];
+      const bf16x8 bh = __builtin_bit_cast(bf16x8, bh4), bl = __builtin_bit_cast(bf16x8, bl4);
+      const int cs = col_base + 16 * ci;
+      const bool odd = cs & 1;
+      const int ce = cs - (odd ? 1 : 0);
+      const uint16_t* ph = odd ? th1 : th0;
+      const uint16_t* pl = odd ? tl1 : tl0;
+      bf16x8 ah[kWaveRows], al[kWaveRows];
+#pragma unroll
+      for (int r = 0; r < kWaveRows; ++r) {
+        // 4-byte aligned dword reads (ce is even)
+        const uint32_t* h = reinterpret_cast<const uint32_t*>(ph + (wr0 + r + yi) * P + ce);
+        const uint32_t* l = reinterpret_cast<const uint32_t*>(pl + (wr0 + r + yi) * P + ce);
+        ah[r] = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+        al[r] = __builtin_bit_cast(bf16x8, make_uint4(l[0], l[1], l[2], l[3]));
+      }
+#pragma unroll
+      for (int r = 0; r < kWaveRows; ++r) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bh, acc[r], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < kWaveRows; ++r) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[r], bl, acc[r], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < kWaveRows; ++r) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh, acc[r], 0, 0, 0);
+    }
+  }
+  // ---- epilogue, part 1: for each of the wave's 8 x 32 windows, the inverse
+  // standard deviation of the support shared by every patch.  Its variance is
+  // at most any patch's window variance (a subset's squared deviations about
+  // its own mean), so by Cauchy-Schwarz (unit-energy weights)
+  //   |score| <= (|dot| + 128 * 4^l * |wsum|) * inv_sd_common.
+  // Lane l: wave row l >> 4, anchors 2 (l & 15) .. + 1, slid along the shared runs.
+  __shared__ double s_inv_sd[kThreads / 64][kWaveRows][kWaveCols];
+  const bool prune = args.prune_below > 0 && args.n_common > 0;
+  if (prune) {
+    const int rr = lane >> 4, ca = wc * kWaveCols + 2 * (lane & 15);
+    int64_t m1[2] = {0, 0}, m2[2] = {0, 0};
+    for (int k = 0; k < args.n_common_runs; ++k) {
+      const int4 run = args.common_runs[k];
+      const uint16_t* src = su + (wr0 + rr + run.x - args.ymin) * P + (ca - args.xmin);
+      int32_t s1 = 0;
+      RunAcc s2 = 0;
+      for (int x = run.y; x <= run.z; ++x) {
+        const int32_t t = int32_t(src[x]) - Ci;
+        s1 += t;
+        s2 += RunAcc(uint32_t(t * t));
+      }
+      m1[0] += s1;
+      m2[0] += int64_t(s2);
+      {
+        const int32_t out = int32_t(src[run.y]) - Ci, in = int32_t(src[run.z + 1]) - Ci;
+        s1 += in - out;
+        s2 += RunAcc(uint32_t(in * in)) - RunAcc(uint32_t(out * out));
+        m1[1] += s1;
+        m2[1] += int64_t(s2);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      // n * sum(t^2) - sum(t)^2 = n^2 * variance, exact in 128 bits
+      const __int128 vn = (__int128)args.n_common * m2[j] - (__int128)m1[j] * m1[j];
+      s_inv_sd[wave][rr][2 * (lane & 15) + j] = vn > 0 ? sqrt(double(args.n_common) / double(vn)) : -1.0;
+    }
+  }
+  __syncthreads();
+  // part 2.  Lane (slot m = lane & 31, half g = lane >> 5) holds the anchors
+  // 8 * (v / 4) + 4 * g + (v % 4) of its wave's 32; one exchange with the
+  // partner half gives it the 16 consecutive anchors 16 g .. 16 g + 15.
+  // Windows outside the valid range or under the bound are written -inf; the
+  // others are queued per wave (at most 64) and scored lane-parallel: exact
+  // moments over the patch's runs and the FP64 normalisation.
+  __shared__ int s_code[kThreads / 64][64];
+  __shared__ float s_dot[kThreads / 64][64];
+  int queued = 0;  // wave-uniform
+  const double dc = double(*F.dc_src);
+  const int64_t D = int64_t(*F.dc_src) << shift;
+  const int64_t Cl = Ci;
+  auto flush = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_wave_barrier();
+    if (lane < queued) {
+      const int code = s_code[wave][lane];
+      const float dv = s_dot[wave][lane];
+      const int r = code >> 16, col = (code >> 8) & 0xff, slot = code & 0xff;
+      const int aa = tile * 32 + slot;
+      const int32_t rf = args.patch_ids[aa];
+      const NccPatch Pq = args.pools.patch(rf);
+      const int4* rq = reinterpret_cast<const int4*>(args.pools.of(rf).runs) + Pq.run_off;
+      int64_t m1 = 0, m2 = 0;
+      for (int k = 0; k < Pq.n_runs; ++k) {
+        const int4 run = rq[k];
+        const uint16_t* src = su + (r + run.x - args.ymin) * P + (wc * kWaveCols + col - args.xmin);
+        int32_t s1 = 0;
+        RunAcc s2 = 0;
+        for (int x = run.y; x <= run.z; ++x) {
+          const int32_t t = int32_t(src[x]) - Ci;
+          s1 += t;
+          s2 += RunAcc(uint32_t(t * t));
+        }
+        m1 += s1;
+        m2 += int64_t(s2);
+      }
+      const int64_t nq = Pq.n;
+      const int64_t sum_s = m1 + nq * Cl;
+      const int64_t sum_s2 = m2 + 2 * Cl * m1 + nq * Cl * Cl;
+      const double score = normalise(double(dv), args.C, args.wsum_eff[aa], Pq.wsum, Pq.n, sum_s, sum_s2, D, shift,
+                                     dc, args.exact_lattice, args.absolute);
+      F.planes[(size_t(aa) * H + (r0 + r)) * W + c0 + wc * kWaveCols + col] = score;
+    }
+    __builtin_amdgcn_wave_barrier();
+    queued = 0;
+  };
+  const int a = tile * 32 + m;
+  const bool active = a < args.n_angles;
+  const int4 rg = reinterpret_cast<const int4*>(F.ranges)[active ? a : 0];
+  const double wsum_eff = args.wsum_eff[active ? a : 0];
+  const double num_slack = double(Ci) * fabs(wsum_eff);  // |(mean(S) - C) * wsum| <= C * |wsum|
+  double* plane = F.planes + size_t(active ? a : 0) * H * W;
+  const int cw = 16 * g;  // first of this lane's 16 anchors, relative to the wave's 32
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  static_for<kWaveRows>([&](auto rc) __attribute__((always_inline)) {
+    constexpr int r = decltype(rc)::value;  // wave row; block row wr0 + r
+    float dot[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[r][i]), __float_as_uint(acc[r][8 + i]),
+                                                       false, false);
+      // first result: anchors 16 g + 8 (i / 4) + (i % 4); second: the same + 4
+      dot[8 * (i / 4) + (i % 4)] = __uint_as_float(sw[0]);
+      dot[8 * (i / 4) + 4 + (i % 4)] = __uint_as_float(sw[1]);
+    }
+    const int row = r0 + wr0 + r;
+    const bool row_in = active && row < H && row >= rg.x && row < rg.y;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int c = c0 + wc * kWaveCols + cw + j;
+      const bool exists = active && row < H && c < W;
+      bool keep = row_in && c < W && c >= rg.z && c < rg.w;
+      if (keep && prune) {
+        const double isd = s_inv_sd[wave][r][cw + j];
+        keep = isd < 0 || (fabs(double(dot[j])) + num_slack) * isd >= args.prune_below;
+      }
+      if (exists && !keep) plane[size_t(row) * W + c] = NCC_MISSING;
+      const uint64_t mask = __ballot(keep);
+      const int cnt = __popcll(mask);
+      if (cnt) {
+        if (queued + cnt > 64) flush();
+        if (keep) {
+          const int pos = queued + __popcll(mask & below);
+          s_code[wave][pos] = ((wr0 + r) << 16) | ((cw + j) << 8) | m;
+          s_dot[wave][pos] = dot[j];
+        }
+        queued += cnt;
+      }
+    }
+  });
+  if (queued) flush();
+}
+
+}  // namespace ncc
+
+using namespace ncc;
+
+extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w,
+                                  int max_h, double C, const NccPatchPool* pools, const int32_t* patch_ids,
+                                  int n_angles, const NcchipDenseWeights* dw, int exact_lattice, int absolute,
+                                  double screen, void* stream) {
+  if (n_frames <= 0 || n_angles <= 0) return 0;
+  if (screen > -__builtin_inf()) return -6;            // FP8 screen: not in this build
+  if (level_kind == NCC_U32 || level > 4) return -6;   // T_lo exact in BF16 only up to level 4
+  MfmaArgs a{};
+  a.frames = frames; a.level = level; a.n_angles = n_angles; a.n_tiles = (n_angles + 31) / 32; a.C = C;
+  a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.patch_ids = patch_ids;
+  a.wh = reinterpret_cast<const uint4*>(dw->wh); a.wl = reinterpret_cast<const uint4*>(dw->wl);
+  a.wsum_eff = dw->wsum_eff; a.ymin = dw->ymin; a.xmin = dw->xmin; a.yspan = dw->yspan; a.xc = dw->xchunks;
+  a.exact_lattice = exact_lattice; a.absolute = absolute;
+  a.common_runs = reinterpret_cast<const int4*>(dw->common_runs); a.n_common_runs = dw->n_common_runs;
+  a.n_common = dw->n_common; a.prune_below = dw->prune_below;
+  if (size_t(n_frames) * a.n_tiles > 65535) return -1;
+  const int cols = kBlockCols + 16 * a.xc + 1;
+  a.pitch = (cols + 1) & ~1;
+  const int rows = kMRows + a.yspan - 1;
+  const size_t tile_bytes = ((size_t(5) * rows * a.pitch + 7) & ~size_t(7)) * sizeof(uint16_t);
+  const size_t b_bytes = size_t(2) * a.yspan * a.xc * 64 * 16;
+  // beside the static tables (bounds, survivor queue)
+  constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(double) + 64 * 8);
+  if (tile_bytes > kLdsMax) return -6;
+  const bool lds_b = tile_bytes + b_bytes <= kLdsMax;
+  const size_t lds = lds_b ? tile_bytes + b_bytes : tile_bytes;
+  const dim3 grid((max_w + kBlockCols - 1) / kBlockCols, (max_h + kMRows - 1) / kMRows, unsigned(n_frames * a.n_tiles));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // per-run sums of t^2 fit 32 bits when (longest run) * (128 * 4^level)^2 < 2^32
+  const double tmax = 128.0 * double(1 << (2 * level));
+  const bool narrow = double(dw->max_run) * tmax * tmax < 4294967296.0;
+#define NCC_MFMA(TP, RA, LB) hipLaunchKernelGGL((k_coarse_mfma<TP, RA, LB>), grid, dim3(kThreads), lds, st, a)
+  if (level_kind == NCC_U8) {
+    if (narrow) { if (lds_b) NCC_MFMA(uint8_t, uint32_t, true); else NCC_MFMA(uint8_t, uint32_t, false); }
+    else { if (lds_b) NCC_MFMA(uint8_t, uint64_t, true); else NCC_MFMA(uint8_t, uint64_t, false); }
+  } else {
+    if (narrow) { if (lds_b) NCC_MFMA(uint16_t, uint32_t, true); else NCC_MFMA(uint16_t, uint32_t, false); }
+    else { if (lds_b) NCC_MFMA(uint16_t, uint64_t, true); else NCC_MFMA(uint16_t, uint64_t, false); }
+  }
+#undef NCC_MFMA
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

@@ -269,3 +269,66 @@ def test_find_matches_golden(capi, name):
     got = m.find(s.image, md["angle_start"], md["angle_extent"], fd["min_score"], fd["num_matches"], fd["max_overlap"])
     want = capi.halcon_shift(np.array(g["results"]).reshape(-1, 4))
     check_matches(got, want, name)
+
+
+FORM_CASES = [
+    ("plain640", "scene_plain", 3, 0.0, 0.0, 0.8, 3),
+    ("plain640_1level", "scene_plain", 1, 0.0, 0.0, 0.8, 3),
+    ("small_rot", "scene_small", 0, -math.radians(20), math.radians(40), 0.7, 2),
+    ("small_rot_all", "scene_small", 0, -math.radians(20), math.radians(40), 0.5, 0),
+    ("rot2048", "scene_rot", 4, -math.pi / 6, math.pi / 3, 0.7, 5),
+]
+
+
+def coarse_errors(got, want, thr):
+    """Score errors of the GPU coarse peaks against the FP64 oracle's at the
+    same (row, col, angle); oracle peaks clear of the threshold must be found."""
+    index = {(r, c, a): s for r, c, a, s in got}
+    errs = []
+    for r, c, a, s in want:
+        g = index.get((r, c, a))
+        if g is None:
+            assert s < thr + 2e-4, f"oracle coarse peak {(r, c, a, s)} missing"
+            continue
+        errs.append(abs(g - s))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("case", FORM_CASES, ids=[c[0] for c in FORM_CASES])
+def test_split_bf16_find_matches_oracle(capi, oracle, request, case):
+    """Split-BF16 MFMA coarse correlation: final matches within the parity
+    tolerances and every coarse score within 1e-4 of the FP64 oracle."""
+    label, fixture, levels, start, extent, ms, nm = case
+    scene = request.getfixturevalue(fixture)
+    m, want, tr, g = run_find(capi, oracle, scene, levels, start, extent, ms, nm, form=capi.FORM_BF16X3,
+                              label=label + " bf16x3")
+    check_matches(g["results"], want, label)
+    errs = coarse_errors(g["coarse"], tr["coarse"], ms * .4)
+    say(f"  bf16x3 coarse score error vs FP64: max {errs.max():.3g} rms {np.sqrt((errs ** 2).mean()):.3g} "
+        f"over {len(errs)} peaks")
+    assert errs.max() <= 1e-4
+
+
+def test_coarse_score_error_per_form(capi, oracle, scene_rot):
+    """Each correlation form's coarse score error against the FP64 oracle on
+    the 2048^2 +-30 degree workload (FP64 form: identical to rounding)."""
+    o = oracle.Model(scene_rot.template, levels=4, start=-math.pi / 6, extent=math.pi / 3)
+    _, tr = o.find(scene_rot.image, -math.pi / 6, math.pi / 3, 0.7, 5, 0.5, trace=True)
+    m = capi.NccModel(scene_rot.template, 4, -math.pi / 6, math.pi / 3)
+    report = {}
+    for name, form in (("fp64", capi.FORM_FP64), ("fp32", capi.FORM_FP32), ("bf16x3", capi.FORM_BF16X3)):
+        opts = capi.NccModel.options(-math.pi / 6, math.pi / 3, 0.7, 5, 0.5, form=form)
+        g = m.find_trace(scene_rot.image, opts)
+        errs = coarse_errors(g["coarse"], tr["coarse"], 0.28)
+        report[name] = (errs.max(), float(np.sqrt((errs ** 2).mean())), len(errs))
+        assert errs.max() <= (1e-12 if name == "fp64" else 1e-4), (name, errs.max())
+    say("coarse score error vs FP64 oracle (max, rms, peaks): " +
+        ", ".join(f"{k}: {v[0]:.3g} / {v[1]:.3g} / {v[2]}" for k, v in report.items()))
+
+
+def test_fp8_screen_form_is_unsupported(capi, scene_small):
+    m = capi.NccModel(scene_small.template, "auto", 0.0, 0.3)
+    opts = capi.NccModel.options(0.0, 0.3, 0.5, 0, 0.5, form=capi.FORM_FP8_SCREEN)
+    with pytest.raises(capi.MvError) as e:
+        m.find_trace(scene_small.image, opts)
+    assert e.value.status == -6
