@@ -87,7 +87,8 @@ typedef struct NcchipDenseWeights {
   double prune_below;
 } NcchipDenseWeights;
 
-/* Split-BF16 MFMA coarse search (same planes as ncchip_coarse): the dot
+/* Split-BF16 MFMA coarse search (the planes of ncchip_coarse, stored in
+ * [row][col][angle] order so that the angles of one anchor are adjacent): the dot
  * products on v_mfma_f32_32x32x16_bf16 (image split T_hi + T_lo, weights
  * w_hi + w_lo, three products in FP32), exact integer window moments and the
  * shared FP64 normalisation.  Top levels of kind NCC_U8 / NCC_U16 up to level 4.
@@ -105,10 +106,11 @@ int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int leve
  * by the caller) may exceed cap.  extent[4f..4f+3] (min row, max row,
  * min col, max col of the candidates' offsets from the crop origin, as FP64
  * bit patterns; the caller initialises them to h, 0, w, 0) feeds
- * ncchip_storage0. */
+ * ncchip_storage0.  hwa != 0: the planes are in [row][col][angle] order
+ * (as ncchip_coarse_mfma writes them), else [angle][row][col]. */
 int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int max_w, int max_h, const NccPatchPool* pools,
                  const int32_t* patch_ids, int n_angles, double threshold, double scale, double phase, NccMatch* cand,
-                 int32_t* key, int cap, int32_t* count, unsigned long long* extent, void* stream);
+                 int32_t* key, int cap, int32_t* count, unsigned long long* extent, int hwa, void* stream);
 
 /* Level-0 storage rectangle of each frame (matcher.cpp:612-629) from its
  * candidate extent and margin = radius + 4 * 2^top + 8, written into the

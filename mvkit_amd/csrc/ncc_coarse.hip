@@ -232,29 +232,37 @@ struct PeakArgs {
   int cap;
   int32_t* count;        // per frame (may exceed cap)
   unsigned long long* extent;  // per frame: min row, max row, min col, max col (FP64 bits)
+  int hwa;               // planes in [row][col][angle] order (else [angle][row][col])
 };
 
 constexpr int kPeakRows = 8;  // plane rows per block
 
 __global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
-  const int f = blockIdx.z / a.n_angles, ai = blockIdx.z % a.n_angles;
+  // [angle][row][col] planes: blockIdx.z = frame * A + angle, threads over
+  // columns; [row][col][angle] planes: blockIdx.z = frame, threads over the
+  // (column, angle) pairs of a row, angles innermost (coalesced either way)
+  const int f = a.hwa ? blockIdx.z : blockIdx.z / a.n_angles;
   const NccFrame& F = a.frames[f];
   const int H = F.levels[a.level].height, W = F.levels[a.level].width;
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (int(blockIdx.x) * 256 >= W) return;  // uniform per block
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int c = a.hwa ? e / a.n_angles : e;
+  const int ai = a.hwa ? e % a.n_angles : blockIdx.z % a.n_angles;
+  if (int(blockIdx.x) * 256 >= (a.hwa ? W * a.n_angles : W)) return;  // uniform per block
+  // element (r, c) of this angle's plane
+  const size_t sr = a.hwa ? size_t(W) * a.n_angles : size_t(W), sc = a.hwa ? size_t(a.n_angles) : 1;
+  const double* pl = F.planes + (a.hwa ? size_t(ai) : size_t(ai) * H * W);
   for (int r = blockIdx.y * kPeakRows; r < min(H, int(blockIdx.y + 1) * kPeakRows); ++r) {
   bool peak = false;
   double v = 0;
   if (c < W) {
-    const double* pl = F.planes + size_t(ai) * H * W;
-    v = pl[size_t(r) * W + c];
+    v = pl[r * sr + c * sc];
     if (v >= a.thr && isfinite(v)) {
       peak = true;
       for (int dr = -1; dr <= 1 && peak; ++dr)
         for (int dc = -1; dc <= 1; ++dc) {
           const int rr = r + dr, cc = c + dc;
           if (rr < 0 || cc < 0 || rr >= H || cc >= W || (!dr && !dc)) continue;
-          const double o = pl[size_t(rr) * W + cc];
+          const double o = pl[rr * sr + cc * sc];
           if (o > v || (o == v && (dr < 0 || (!dr && dc < 0)))) { peak = false; break; }
         }
     }
@@ -358,14 +366,15 @@ extern "C" int ncchip_coarse(const NccFrame* frames, int n_frames, int level, in
 extern "C" int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int max_w, int max_h,
                             const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, double threshold,
                             double scale, double phase, NccMatch* cand, int32_t* key, int cap, int32_t* count,
-                            unsigned long long* extent, void* stream) {
+                            unsigned long long* extent, int hwa, void* stream) {
   if (n_frames <= 0 || n_angles <= 0) return 0;
   if (size_t(n_frames) * n_angles > 65535 || max_h > 65535) return -1;
   PeakArgs a;
   a.frames = frames; a.level = level; a.n_angles = n_angles; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
   a.patch_ids = patch_ids; a.thr = threshold; a.scale = scale; a.phase = phase; a.cand = cand; a.key = key;
-  a.cap = cap; a.count = count; a.extent = extent;
-  const dim3 grid((max_w + 255) / 256, (max_h + kPeakRows - 1) / kPeakRows, unsigned(n_frames * n_angles));
+  a.cap = cap; a.count = count; a.extent = extent; a.hwa = hwa;
+  const dim3 grid(hwa ? (max_w * n_angles + 255) / 256 : (max_w + 255) / 256, (max_h + kPeakRows - 1) / kPeakRows,
+                  unsigned(hwa ? n_frames : n_frames * n_angles));
   hipLaunchKernelGGL(k_peaks, grid, dim3(256), 0, static_cast<hipStream_t>(stream), a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

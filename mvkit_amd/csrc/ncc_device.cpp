@@ -648,7 +648,8 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   time_end();
   time_begin(kStagePeaks);
   if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
-                   lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, stream_))
+                   lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 1 : 0,
+                   stream_))
     raise(kInternalError, "device", "peak launch failed");
   if (!finest && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
     raise(kInternalError, "device", "storage launch failed");

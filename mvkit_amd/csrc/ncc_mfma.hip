@@ -28,9 +28,9 @@ namespace ncc {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
-constexpr int kThreads = 512;              // 8 waves: 2 row groups x 4 column groups
+constexpr int kThreads = 256;              // 4 waves, one per column group
 constexpr int kWaveRows = 4;               // output rows per wave
-constexpr int kMRows = 2 * kWaveRows;      // output rows per block
+constexpr int kMRows = kWaveRows;          // output rows per block
 constexpr int kWaveCols = 32;              // anchors per wave (MFMA M)
 constexpr int kBlockCols = 4 * kWaveCols;
 
@@ -67,8 +67,8 @@ __device__ __forceinline__ uint16_t bf16_bits_exact(int v) {
   return uint16_t(__float_as_uint(float(v)) >> 16);
 }
 
-template <class Tpix, class RunAcc, bool kLdsB>
-__global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
+template <class Tpix, class RunAcc>
+__global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args) {
   extern __shared__ uint16_t lds[];
   const int f = blockIdx.z / args.n_tiles, tile = blockIdx.z % args.n_tiles;
   const NccFrame& F = args.frames[f];
@@ -84,10 +84,9 @@ __global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
   uint16_t* tl0 = th1 + rows * P;     // T_lo bits
   uint16_t* tl1 = tl0 + rows * P;
   uint16_t* su = tl1 + rows * P;      // S
-  // the angle tile's split weights, staged once per block (16-byte aligned)
-  // when they fit beside the tile, else read from L2 in the loop
+  // the angle tile's split weights (MFMA B fragments), read from L2 one
+  // iteration ahead of their use
   const int nb = args.yspan * args.xc * 64;  // B fragments (uint4) per half
-  uint4* bw = reinterpret_cast<uint4*>(lds + ((5 * rows * P + 7) & ~7));
   const uint4* gwh = args.wh + size_t(tile) * nb;
   const uint4* gwl = args.wl + size_t(tile) * nb;
   const int Ci = int(args.C);
@@ -123,18 +122,6 @@ __global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
         su[tr * P + tc] = uint16_t(S);
       }
     }
-    if (kLdsB)
-      for (int base = threadIdx.x; base < 2 * nb; base += kThreads * U) {
-        uint4 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int i = base + kThreads * u;
-          v[u] = i < nb ? gwh[i] : (i < 2 * nb ? gwl[i - nb] : make_uint4(0, 0, 0, 0));
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (base + kThreads * u < 2 * nb) bw[base + kThreads * u] = v[u];
-      }
   }
   __syncthreads();
   // ---- MFMA: this wave's 32 anchors x 32 angle slots x kMRows rows
@@ -147,10 +134,16 @@ __global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[r][v] = 0.f;
   const int col_base = wc * kWaveCols + m + 8 * g;  // tile column of this lane's first A element for chunk 0
-  for (int yi = 0; yi < args.yspan; ++yi) {
-    for (int ci = 0; ci < args.xc; ++ci) {
-      const int bi = (yi * args.xc + ci) * 64 + lane;
-      const uint4 bh4 = kLdsB ? bw[bi] : gwh[bi], bl4 = kLdsB ? bw[nb + bi] : gwl[bi];
+  const int n_iter = args.yspan * args.xc;
+  uint4 bh_next = gwh[lane], bl_next = gwl[lane];
+  for (int it = 0; it < n_iter; ++it) {
+    const int yi = it / args.xc, ci = it - yi * args.xc;
+    {
+      const uint4 bh4 = bh_next, bl4 = bl_next;
+      if (it + 1 < n_iter) {
+        bh_next = gwh[(it + 1) * 64 + lane];
+        bl_next = gwl[(it + 1) * 64 + lane];
+      }
       const bf16x8 bh = __builtin_bit_cast(bf16x8, bh4), bl = __builtin_bit_cast(bf16x8, bl4);
       const int cs = col_base + 16 * ci;
       const bool odd = cs & 1;
@@ -254,7 +247,7 @@ __global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
       const int64_t sum_s2 = m2 + 2 * Cl * m1 + nq * Cl * Cl;
       const double score = normalise(double(dv), args.C, args.wsum_eff[aa], Pq.wsum, Pq.n, sum_s, sum_s2, D, shift,
                                      dc, args.exact_lattice, args.absolute);
-      F.planes[(size_t(aa) * H + (r0 + r)) * W + c0 + wc * kWaveCols + col] = score;
+      F.planes[(size_t(r0 + r) * W + c0 + wc * kWaveCols + col) * args.n_angles + aa] = score;
     }
     __builtin_amdgcn_wave_barrier();
     queued = 0;
@@ -264,7 +257,8 @@ __global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
   const int4 rg = reinterpret_cast<const int4*>(F.ranges)[active ? a : 0];
   const double wsum_eff = args.wsum_eff[active ? a : 0];
   const double num_slack = double(Ci) * fabs(wsum_eff);  // |(mean(S) - C) * wsum| <= C * |wsum|
-  double* plane = F.planes + size_t(active ? a : 0) * H * W;
+  // planes in [row][col][angle] order: the 32 slots of one anchor are adjacent
+  double* plane = F.planes + (active ? a : 0);
   const int cw = 16 * g;  // first of this lane's 16 anchors, relative to the wave's 32
   const uint64_t below = (uint64_t(1) << lane) - 1;
   static_for<kWaveRows>([&](auto rc) __attribute__((always_inline)) {
@@ -289,7 +283,7 @@ __global__ void __launch_bounds__(kThreads) k_coarse_mfma(const MfmaArgs args) {
         const double isd = s_inv_sd[wave][r][cw + j];
         keep = isd < 0 || (fabs(double(dot[j])) + num_slack) * isd >= args.prune_below;
       }
-      if (exists && !keep) plane[size_t(row) * W + c] = NCC_MISSING;
+      if (exists && !keep) plane[(size_t(row) * W + c) * args.n_angles] = NCC_MISSING;
       const uint64_t mask = __ballot(keep);
       const int cnt = __popcll(mask);
       if (cnt) {
@@ -329,25 +323,20 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   const int cols = kBlockCols + 16 * a.xc + 1;
   a.pitch = (cols + 1) & ~1;
   const int rows = kMRows + a.yspan - 1;
-  const size_t tile_bytes = ((size_t(5) * rows * a.pitch + 7) & ~size_t(7)) * sizeof(uint16_t);
-  const size_t b_bytes = size_t(2) * a.yspan * a.xc * 64 * 16;
+  const size_t lds = size_t(5) * rows * a.pitch * sizeof(uint16_t);
   // beside the static tables (bounds, survivor queue)
   constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(double) + 64 * 8);
-  if (tile_bytes > kLdsMax) return -6;
-  const bool lds_b = tile_bytes + b_bytes <= kLdsMax;
-  const size_t lds = lds_b ? tile_bytes + b_bytes : tile_bytes;
+  if (lds > kLdsMax) return -6;
   const dim3 grid((max_w + kBlockCols - 1) / kBlockCols, (max_h + kMRows - 1) / kMRows, unsigned(n_frames * a.n_tiles));
   hipStream_t st = static_cast<hipStream_t>(stream);
   // per-run sums of t^2 fit 32 bits when (longest run) * (128 * 4^level)^2 < 2^32
   const double tmax = 128.0 * double(1 << (2 * level));
   const bool narrow = double(dw->max_run) * tmax * tmax < 4294967296.0;
-#define NCC_MFMA(TP, RA, LB) hipLaunchKernelGGL((k_coarse_mfma<TP, RA, LB>), grid, dim3(kThreads), lds, st, a)
+#define NCC_MFMA(TP, RA) hipLaunchKernelGGL((k_coarse_mfma<TP, RA>), grid, dim3(kThreads), lds, st, a)
   if (level_kind == NCC_U8) {
-    if (narrow) { if (lds_b) NCC_MFMA(uint8_t, uint32_t, true); else NCC_MFMA(uint8_t, uint32_t, false); }
-    else { if (lds_b) NCC_MFMA(uint8_t, uint64_t, true); else NCC_MFMA(uint8_t, uint64_t, false); }
+    if (narrow) NCC_MFMA(uint8_t, uint32_t); else NCC_MFMA(uint8_t, uint64_t);
   } else {
-    if (narrow) { if (lds_b) NCC_MFMA(uint16_t, uint32_t, true); else NCC_MFMA(uint16_t, uint32_t, false); }
-    else { if (lds_b) NCC_MFMA(uint16_t, uint64_t, true); else NCC_MFMA(uint16_t, uint64_t, false); }
+    if (narrow) NCC_MFMA(uint16_t, uint32_t); else NCC_MFMA(uint16_t, uint64_t);
   }
 #undef NCC_MFMA
   return hipGetLastError() == hipSuccess ? 0 : -1000;
