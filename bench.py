@@ -4,12 +4,14 @@ search workload (128x128 template pasted 5 times with bilinear rotation in
 and the automatic angle step; MinScore 0.7, NumMatches 5, MaxOverlap 0.5,
 subpixel refinement).
 
-A step is one batched find over `--inflight` frames (default 4) already
+A step is one batched find over `--inflight` frames (default 16) already
 resident in HBM: every stage runs for all frames before the next, and the
 per-candidate kernels (tracking, full score, refinement) take the candidates
 of every frame in one launch.  The throughput `value` is search megapixels
 per second over the whole timed region; the reported latency is that of one
-batch.
+batch.  The coarse correlation runs in the split-BF16 MFMA form by default;
+the plain FP32 form is timed on the same frames for the comparison, and the
+coarse scores of both are compared with the FP64 form on frame 0.
 
 Prints one JSON line (rank 0).
 """
@@ -43,8 +45,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--form", default="fp32", choices=sorted(FORMS))
-    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--form", default="bf16x3", choices=sorted(FORMS))
+    ap.add_argument("--inflight", type=int, default=16)
+    ap.add_argument("--compare-steps", type=int, default=20,
+                    help="timed steps of the plain FP32 form for the comparison (0: skip)")
     ap.add_argument("--cpu-finds", type=int, default=24, help="finds in the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -132,6 +136,34 @@ def main():
         bound, peak = "latency", None
     total_kernel = sum(kernel.values())
 
+    # the plain FP32 form on the same frames, for the split-BF16 comparison
+    fp32 = None
+    if args.compare_steps > 0 and form != FORMS["fp32"]:
+        o32 = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=FORMS["fp32"])
+        for _ in range(2):
+            step(o32)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.compare_steps):
+            step(o32)
+        torch.cuda.synchronize()
+        e32 = time.perf_counter() - t
+        fp32 = {"value": round(2048 * 2048 * args.compare_steps * F / e32 / 1e6, 2), "unit": "Mpx/s",
+                "ms_per_frame": round(e32 / (args.compare_steps * F) * 1e3, 4), "steps": args.compare_steps}
+
+    # coarse score error of each form against this build's FP64 form (the
+    # reference's sample order in FP64) on frame 0, matched by (row, col, angle)
+    def coarse(f):
+        o = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=f)
+        return {(r, c, a): sc for r, c, a, sc in model.find_trace(scenes[0].image, o)["coarse"]}
+    ref = coarse(FORMS["fp64"])
+    score_err = {}
+    for name in ("fp32", "bf16x3"):
+        got = coarse(FORMS[name])
+        errs = [abs(got[k] - v) for k, v in ref.items() if k in got]
+        score_err[name] = {"max": float(max(errs)), "rms": float(np.sqrt(np.mean(np.square(errs)))),
+                           "peaks": len(errs), "missing": len(ref) - len(errs)}
+
     cpu = None
     if args.cpu_finds > 0:
         from oracle import pyoracle
@@ -147,7 +179,7 @@ def main():
                "ms_per_frame": round(cpu_s * 1e3, 2)}
 
     line = {
-        "metric": "search Mpx/s & ms/frame, 2048^2 +-30 deg NCC find",
+        "metric": "search Mpx/s & ms/frame, 2048\u00b2 \u00b130\u00b0: split-BF16 vs FP32 at \u22641e-4 score err",
         "value": round(mpx, 2), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1e3, 4),
         "ms_per_frame": round(elapsed / frames_done * 1e3, 4),
@@ -162,6 +194,9 @@ def main():
                      "us_per_frame": round(per_frame_us, 2), "algorithmic_gflop_per_frame": round(per_frame_flop / 1e9, 4)},
         "kernel_us_per_frame": {k: round(v, 2) for k, v in kernel.items()},
         "dominant_kernel": {"stage": dominant, "share": round(kernel[dominant] / total_kernel, 3) if total_kernel else 0},
+        "fp32_form": fp32,
+        "speedup_vs_fp32": round(mpx / fp32["value"], 3) if fp32 else None,
+        "coarse_score_error_vs_fp64": score_err,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
