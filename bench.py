@@ -7,9 +7,11 @@ subpixel refinement).
 A step is one batched find over `--inflight` frames (default 16) already
 resident in HBM: every stage runs for all frames before the next, and the
 per-candidate kernels (tracking, full score, refinement) take the candidates
-of every frame in one launch.  The throughput `value` is search megapixels
+of every frame in one launch.  Consecutive steps are overlapped the way a
+serving loop would run them: step k+1 is submitted before step k is waited
+for, so step k's refinement and host work overlap step k+1's device work.  The throughput `value` is search megapixels
 per second over the whole timed region; the reported latency is that of one
-batch.  The coarse correlation runs in the split-BF16 MFMA form by default;
+batch, from its submission to the end of its wait.  The coarse correlation runs in the split-BF16 MFMA form by default;
 the plain FP32 form is timed on the same frames for the comparison, and the
 coarse scores of both are compared with the FP64 form on frame 0.
 
@@ -86,13 +88,25 @@ def main():
         latencies.append(time.perf_counter() - t)
         return out, st
 
+    def pipelined(o, steps, on_stats=None):
+        """`steps` batches with the next batch submitted before the previous
+        one is waited for (its refinement overlaps the next device part)."""
+        pending = None
+        for i in range(steps + 1):
+            nxt = (model.submit_batch(ptrs, 2048, 2048, 2048, o), time.perf_counter()) if i < steps else None
+            if pending is not None:
+                _, st = model.wait_batch(pending[0])
+                latencies.append(time.perf_counter() - pending[1])
+                if on_stats:
+                    on_stats(st)
+            pending = nxt
+
     # the batch must return, frame by frame, what a single find returns
     batch0, _ = step(opts)
     assert len(batch0) == F and np.array_equal(batch0[0], res), (batch0[0], res)
 
     log(f"warmup {args.warmup} steps x {F} frames")
-    for _ in range(args.warmup):
-        step(opts)
+    pipelined(opts, args.warmup)
 
     # per-stage device time from a short pass with every stage timed
     # (an event pair around each stage's launches on the library's stream)
@@ -110,14 +124,16 @@ def main():
     torch.cuda.synchronize()
     log(f"timed {args.steps} steps x {F} frames, form {args.form}, events on {dominant}")
     timed = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=form, profile=dom_bit)
-    dom_us, flops = 0.0, 0.0
+    acc = {"us": 0.0, "flops": 0.0}
+
+    def take(st):
+        acc["us"] += st.kernel_us[capi.STAGES.index(dominant)]
+        acc["flops"] += st.track_flops if dominant == "track" else st.coarse_flops
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, st = step(timed)
-        dom_us += st.kernel_us[capi.STAGES.index(dominant)]
-        flops += st.track_flops if dominant == "track" else st.coarse_flops
+    pipelined(timed, args.steps, take)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    dom_us, flops = acc["us"], acc["flops"]
     frames_done = args.steps * F
     mpx = 2048 * 2048 * frames_done / elapsed / 1e6
     lat = np.array(latencies) * 1e3
@@ -140,12 +156,10 @@ def main():
     fp32 = None
     if args.compare_steps > 0 and form != FORMS["fp32"]:
         o32 = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=FORMS["fp32"])
-        for _ in range(2):
-            step(o32)
+        pipelined(o32, 2)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.compare_steps):
-            step(o32)
+        pipelined(o32, args.compare_steps)
         torch.cuda.synchronize()
         e32 = time.perf_counter() - t
         fp32 = {"value": round(2048 * 2048 * args.compare_steps * F / e32 / 1e6, 2), "unit": "Mpx/s",

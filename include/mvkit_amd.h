@@ -69,6 +69,22 @@ MV_API MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* const
                                            const MvAmdFindOptions* options, double* results, size_t cap,
                                            size_t* counts, MvAmdFindStats* stats);
 
+/* The two halves of mvamd_find_ncc_model_batch, for overlapping consecutive
+ * batches on the calling thread's current device: submit enqueues the
+ * device-resident part of the find (at most 256 frames) and returns at once
+ * with a ticket; wait completes it (refinement rounds, results as in
+ * mvamd_find_ncc_model_batch).  At most two batches may be pending per
+ * thread and device; a batch submitted after another overlaps the other's
+ * refinement and host work.  The images must stay valid until the wait
+ * returns; the model stays alive as long as the batch is pending.
+ * with_stats != 0 fills stats at the wait. */
+MV_API MvStatus mvamd_find_ncc_model_batch_submit(MvTuple ModelID, const uint8_t* const* d_images,
+                                                  const int64_t* widths, const int64_t* heights, const int64_t* pitches,
+                                                  size_t n, const MvAmdFindOptions* options, int32_t with_stats,
+                                                  uint64_t* ticket);
+MV_API MvStatus mvamd_find_ncc_model_batch_wait(uint64_t ticket, double* results, size_t cap, size_t* counts,
+                                                MvAmdFindStats* stats);
+
 /* Find on a host u8 image returning the stages: coarse peaks (cap_coarse x 4),
  * full-scored candidates in score order (cap_prepared x 4) and, per such
  * candidate, the refined pose and verdict (cap_prepared x 5). */

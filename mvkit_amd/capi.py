@@ -28,7 +28,8 @@ PUBLIC_SYMBOLS = [
     "mv_length_tuple", "mv_get_type", "mv_get_i", "mv_get_d", "mv_get_s", "mv_get_h", "mv_resize_tuple", "mv_set_i",
     "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
     "mv_clear_handle", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model",
-    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
+    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
+                  "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
@@ -115,6 +116,9 @@ def lib() -> C.CDLL:
                                                   C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), P, S, P,
                                                  C.POINTER(FindStats)]
+        L.mvamd_find_ncc_model_batch_submit.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), C.c_int32,
+                                                        C.POINTER(C.c_uint64)]
+        L.mvamd_find_ncc_model_batch_wait.argtypes = [C.c_uint64, P, S, P, C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_trace.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S), P, S,
                                                  C.POINTER(S), P, P, S, C.POINTER(S), C.POINTER(FindStats)]
         L.mvamd_model_info.argtypes = [P, P]
@@ -303,6 +307,31 @@ class NccModel:
         st = FindStats()
         _check(lib().mvamd_find_ncc_model_batch(self.tuple.h, ptrs, ws, hs, ps, n, C.byref(opts), _ptr(res), cap,
                                                 _ptr(counts), C.byref(st)))
+        return [res[k, :int(counts[k])].copy() for k in range(n)], st
+
+    def submit_batch(self, d_images: list, width, height, pitch, opts: FindOptions, stats: bool = True) -> tuple:
+        """Submits a batch (see find_batch) without waiting; returns a handle
+        for wait_batch.  At most two batches may be pending."""
+        n = len(d_images)
+
+        def per_frame(v):
+            return list(v) if isinstance(v, (list, tuple)) else [v] * n
+        ptrs = (C.c_void_p * n)(*d_images)
+        ws = (C.c_int64 * n)(*per_frame(width))
+        hs = (C.c_int64 * n)(*per_frame(height))
+        ps = (C.c_int64 * n)(*per_frame(pitch))
+        ticket = C.c_uint64()
+        _check(lib().mvamd_find_ncc_model_batch_submit(self.tuple.h, ptrs, ws, hs, ps, n, C.byref(opts), int(stats),
+                                                       C.byref(ticket)))
+        return ticket.value, n
+
+    @staticmethod
+    def wait_batch(handle: tuple, cap: int = 256):
+        ticket, n = handle
+        res = np.zeros((max(n, 1), cap, 4))
+        counts = np.zeros(max(n, 1), np.uint64)
+        st = FindStats()
+        _check(lib().mvamd_find_ncc_model_batch_wait(ticket, _ptr(res), cap, _ptr(counts), C.byref(st)))
         return [res[k, :int(counts[k])].copy() for k in range(n)], st
 
     def find_trace(self, image: np.ndarray, opts: FindOptions, cap: int = 20000):

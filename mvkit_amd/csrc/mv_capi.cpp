@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -592,6 +593,28 @@ extern "C" MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* 
   });
 }
 
+namespace {
+// Submitted batches of this thread, by ticket (device << 56 | context ticket).
+struct PendingBatch {
+  std::shared_ptr<mvamd::NccModel> model;  // kept alive until the wait
+  std::vector<mvamd::FindTrace> traces;
+  size_t n = 0;
+  int device = 0;
+  uint64_t local = 0;
+};
+std::map<uint64_t, PendingBatch>& pending_batches() {
+  thread_local std::map<uint64_t, PendingBatch> pending;
+  return pending;
+}
+void fill_batch_stats(const std::vector<mvamd::FindTrace>& traces, MvAmdFindStats* stats) {
+  fill_stats(traces[0], stats);
+  double flops = 0, tflops = 0;
+  for (const auto& t : traces) { flops += t.coarse_flops; tflops += t.track_flops; }
+  stats->coarse_flops = flops;
+  stats->track_flops = tflops;
+}
+}  // namespace
+
 extern "C" MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* const* d_images, const int64_t* widths,
                                                const int64_t* heights, const int64_t* pitches, size_t n,
                                                const MvAmdFindOptions* options, double* results, size_t cap,
@@ -619,14 +642,71 @@ extern "C" MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* c
       if (results) put_poses(found[k], results + 4 * cap * k, cap);
       counts[k] = found[k].size();
     }
-    if (stats && n) {
-      fill_stats(traces[0], stats);
-      double flops = 0;
-      double tflops = 0;
-      for (const auto& t : traces) { flops += t.coarse_flops; tflops += t.track_flops; }
-      stats->coarse_flops = flops;
-      stats->track_flops = tflops;
+    if (stats && n) fill_batch_stats(traces, stats);
+  });
+}
+
+
+extern "C" MvStatus mvamd_find_ncc_model_batch_submit(MvTuple ModelID, const uint8_t* const* d_images,
+                                                      const int64_t* widths, const int64_t* heights,
+                                                      const int64_t* pitches, size_t n, const MvAmdFindOptions* options,
+                                                      int32_t with_stats, uint64_t* ticket) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    required(d_images, op, "Images");
+    required(widths, op, "widths");
+    required(heights, op, "heights");
+    required(pitches, op, "pitches");
+    required(ticket, op, "ticket");
+    if (n > NCC_MAX_BATCH)
+      fail(mvamd::kLimitExceeded, mvamd::kErrInvalidValue, op, "Images", "too many frames for one submitted batch");
+    std::vector<mvamd::FrameInput> frames(n);
+    for (size_t k = 0; k < n; ++k) {
+      if (!d_images[k] || widths[k] <= 0 || heights[k] <= 0 || pitches[k] < widths[k])
+        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Images", "invalid image geometry", int64_t(k));
+      frames[k] = {d_images[k], size_t(widths[k]), size_t(heights[k]), size_t(pitches[k])};
     }
+    PendingBatch b;
+    b.model = model_of(ModelID, op);
+    b.n = n;
+    mvamd::check(hipGetDevice(&b.device), "hipGetDevice");
+    b.traces.resize(with_stats ? n : 0);
+    const mvamd::FindOptions q = from_ext(options);
+    auto& pend = pending_batches();
+    // the context keeps pointers into the traces' buffer, which moves with
+    // the vector into the map
+    auto node = std::make_unique<PendingBatch>(std::move(b));
+    const uint64_t local = algorithm(op, [&] {
+      return context().submit_batch(*node->model, frames.data(), n, q, with_stats && n ? node->traces.data() : nullptr);
+    });
+    node->local = local;
+    const uint64_t key = (uint64_t(node->device) << 56) | local;
+    pend[key] = std::move(*node);
+    *ticket = key;
+  });
+}
+
+extern "C" MvStatus mvamd_find_ncc_model_batch_wait(uint64_t ticket, double* results, size_t cap, size_t* counts,
+                                                    MvAmdFindStats* stats) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    required(counts, op, "counts");
+    auto& pend = pending_batches();
+    const auto it = pend.find(ticket);
+    if (it == pend.end()) fail(mvamd::kInvalidState, mvamd::kErrInvalidValue, op, "ticket", "no such pending batch");
+    int dev = 0;
+    mvamd::check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != it->second.device) {
+      fail(mvamd::kInvalidState, mvamd::kErrInvalidValue, op, "ticket", "waited on another device than submitted");
+    }
+    PendingBatch b = std::move(it->second);
+    pend.erase(it);
+    const auto found = algorithm(op, [&] { return context().complete_batch(b.local); });
+    for (size_t k = 0; k < b.n; ++k) {
+      if (results) put_poses(found[k], results + 4 * cap * k, cap);
+      counts[k] = found[k].size();
+    }
+    if (stats && !b.traces.empty()) fill_batch_stats(b.traces, stats);
   });
 }
 

@@ -267,45 +267,111 @@ double pose_overlap(const Pose& a, const Pose& b, const LevelGeom& lv) {
   return std::abs(area) * .5 / (double(lv.w) * lv.h);
 }
 
+namespace {
+// Host state of one frame of a batch.
+struct Job {
+  FrameInput in;
+  FindTrace* trace = nullptr;
+  int64_t crop_r = 0, crop_c = 0, crop_b = 0, crop_e = 0;
+  struct Lv { int w, h, kind, pitch; size_t off, bytes; };
+  std::vector<Lv> lv;
+  std::vector<int32_t> ranges;
+  size_t off_p1 = 0, off_p2 = 0, off_nz = 0, off_planes = 0, off_ranges = 0;
+  std::vector<Pose> cand, full, refined, results;
+  std::vector<int> verdict;
+  int32_t storage0[4] = {0, 0, 0, 0};
+  size_t pos = 0, batch_size = 0;
+  bool done = false;
+  double coarse_flops = 0, track_flops = 0;
+};
+struct Timer { hipEvent_t begin = nullptr, end = nullptr; int stage = 0; };
+}  // namespace
+
+// A submitted batch: its buffers (reused by later batches in the same slot)
+// and what the completion needs.
+struct FindContext::Slot {
+  DeviceBuffer work;
+  PinnedBuffer stage, readback;
+  hipEvent_t done = nullptr;  // the device-resident part has finished
+  std::vector<Timer> timers;
+  size_t timers_used = 0;
+  int profiling = 0;          // stage mask
+  bool timing = false;        // the open stage is timed
+  bool busy = false;
+  uint64_t ticket = 0;
+  // the batch
+  const NccModel* model = nullptr;
+  std::shared_ptr<DeviceModel> dmodel;
+  FindOptions o;
+  FindTrace* traces = nullptr;
+  size_t n_frames = 0;
+  bool empty = false;  // the query range misses the model's angles
+  std::vector<Job> jobs;
+  RasterPool qpool;
+  size_t cap = 0, cap_all = 0, n_track = 0, top = 0, finest = 0, stride = 1, off_key = 0;
+  uint8_t* dev = nullptr;
+  NccFrame* dframes = nullptr;
+  NccMatch* lists = nullptr;
+  NccMatch* hfull = nullptr;
+  int32_t* hcounts = nullptr;
+  uint8_t* rb = nullptr;
+  NccPatchPool pools[2] = {};
+  NcchipRefineParams rp{};
+  double t0 = 0;
+};
+
 FindContext::FindContext() {
   check(hipGetDevice(&device_), "hipGetDevice");
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-}
-FindContext::~FindContext() {
-  if (stream_) {
-    (void)hipStreamSynchronize(stream_);
-    for (auto& t : timers_) { (void)hipEventDestroy(t.begin); (void)hipEventDestroy(t.end); }
-    (void)hipStreamDestroy(stream_);
+  check(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto& sl : slots_) {
+    sl = std::make_unique<Slot>();
+    check(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming), "hipEventCreate");
   }
 }
+FindContext::~FindContext() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (stream2_) (void)hipStreamSynchronize(stream2_);
+  for (auto& sl : slots_) {
+    if (!sl) continue;
+    for (auto& t : sl->timers) { (void)hipEventDestroy(t.begin); (void)hipEventDestroy(t.end); }
+    if (sl->done) (void)hipEventDestroy(sl->done);
+  }
+  if (stream2_) (void)hipStreamDestroy(stream2_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
 
-void FindContext::time_begin(int stage) {
-  timing_ = profiling_ & (1 << stage);
-  if (!timing_) return;
-  if (timers_used_ == timers_.size()) {
+void FindContext::time_begin(int stage, hipStream_t stream) {
+  Slot& S = *cur_;
+  S.timing = S.profiling & (1 << stage);
+  if (!S.timing) return;
+  if (S.timers_used == S.timers.size()) {
     Timer t;
     check(hipEventCreate(&t.begin), "hipEventCreate");
     check(hipEventCreate(&t.end), "hipEventCreate");
-    timers_.push_back(t);
+    S.timers.push_back(t);
   }
-  timers_[timers_used_].stage = stage;
-  check(hipEventRecord(timers_[timers_used_].begin, stream_), "hipEventRecord");
+  S.timers[S.timers_used].stage = stage;
+  check(hipEventRecord(S.timers[S.timers_used].begin, stream), "hipEventRecord");
 }
-void FindContext::time_end() {
-  if (!timing_) return;
-  check(hipEventRecord(timers_[timers_used_].end, stream_), "hipEventRecord");
-  ++timers_used_;
+void FindContext::time_end(hipStream_t stream) {
+  Slot& S = *cur_;
+  if (!S.timing) return;
+  check(hipEventRecord(S.timers[S.timers_used].end, stream), "hipEventRecord");
+  ++S.timers_used;
 }
+// Every timed event of the slot has completed when this runs (the slot's
+// device part and its refinement rounds were waited for).
 void FindContext::time_collect(FindTrace* trace) {
-  if (profiling_ && trace) {
-    check(hipStreamSynchronize(stream_), "sync");
-    for (size_t i = 0; i < timers_used_; ++i) {
+  Slot& S = *cur_;
+  if (S.profiling && trace) {
+    for (size_t i = 0; i < S.timers_used; ++i) {
       float ms = 0;
-      check(hipEventElapsedTime(&ms, timers_[i].begin, timers_[i].end), "hipEventElapsedTime");
-      trace->kernel_us[timers_[i].stage] += 1000.0 * ms;
+      check(hipEventElapsedTime(&ms, S.timers[i].begin, S.timers[i].end), "hipEventElapsedTime");
+      trace->kernel_us[S.timers[i].stage] += 1000.0 * ms;
     }
   }
-  timers_used_ = 0;
+  S.timers_used = 0;
 }
 
 std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
@@ -324,24 +390,6 @@ std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image,
   return std::move(out.front());
 }
 
-namespace {
-// Host state of one frame of a batch.
-struct Job {
-  FrameInput in;
-  FindTrace* trace = nullptr;
-  int64_t crop_r = 0, crop_c = 0, crop_b = 0, crop_e = 0;
-  struct Lv { int w, h, kind, pitch; size_t off, bytes; };
-  std::vector<Lv> lv;
-  std::vector<int32_t> ranges;
-  size_t off_p1 = 0, off_p2 = 0, off_nz = 0, off_planes = 0, off_ranges = 0;
-  std::vector<Pose> cand, full, refined, results;
-  std::vector<int> verdict;
-  int32_t storage0[4] = {0, 0, 0, 0};
-  size_t pos = 0, batch_size = 0;
-  bool done = false;
-  double coarse_flops = 0, track_flops = 0;
-};
-}  // namespace
 
 std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, const FrameInput* inputs, size_t n_frames,
                                                        const FindOptions& req, FindTrace* traces) {
@@ -353,12 +401,27 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     }
     return all;
   }
+  return complete_batch(submit_batch(model, inputs, n_frames, req, traces));
+}
+
+uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inputs, size_t n_frames,
+                                   const FindOptions& req, FindTrace* traces) {
+  if (n_frames > NCC_MAX_BATCH) raise(kLimitExceeded, "Images", "too many frames for one submitted batch");
+  const uint64_t ticket = next_ticket_;
+  Slot& S = *slots_[ticket % 2];
+  if (S.busy) raise(kInvalidState, "batch", "two batches are pending; complete the oldest first");
+  cur_ = &S;
   const double t0 = now_us();
-  profiling_ = traces ? req.profile : 0;
-  timers_used_ = 0;
+  S.profiling = traces ? req.profile : 0;
+  S.timers_used = 0;
+  S.jobs.clear();
   for (size_t f = 0; f < n_frames && traces; ++f) traces[f] = FindTrace{};
-  std::vector<std::vector<Pose>> results(n_frames);
-  if (!n_frames) return results;
+  auto finish_empty = [&]() {
+    S.model = &model; S.traces = traces; S.n_frames = n_frames; S.empty = true; S.ticket = ticket; S.busy = true;
+    ++next_ticket_;
+    return ticket;
+  };
+  if (!n_frames) return finish_empty();
   // ---- options and levels, shared by the batch (matcher.cpp:345-374)
   FindOptions o = req;
   for (size_t f = 0; f < n_frames; ++f)
@@ -382,7 +445,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   if (!std::isfinite(angle_end)) raise(kInvalidArgument, "angle_extent", "query angle range must be finite");
   const double cs = std::max(o.start, model.params.start);
   const double ce = std::min(angle_end, model.params.start + model.params.extent);
-  if (cs > ce + 1e-12) return results;
+  if (cs > ce + 1e-12) return finish_empty();
   o.start = cs;
   o.extent = std::max(0., ce - cs);
   for (size_t f = 0; f < n_frames; ++f) {
@@ -538,20 +601,20 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   const size_t off_common = off_small + small; small = align_up(small + dense.common_runs.size() * 4);
   const size_t off_frames = off_small + small; small = align_up(small + n_frames * sizeof(NccFrame));
   off = off_small + small;
-  if (work_.bytes < off) {
+  if (S.work.bytes < off) {
     check(hipStreamSynchronize(stream_), "sync");
-    work_ = DeviceBuffer(align_up(off + off / 4, 1 << 20));
+    S.work = DeviceBuffer(align_up(off + off / 4, 1 << 20));
   }
-  uint8_t* dev = work_.as<uint8_t>();
+  uint8_t* dev = S.work.as<uint8_t>();
   // Host-visible (pinned, coherent) staging: the refinement rounds' candidate
   // lists are read by the kernels in place and the full scores and refined
   // poses are written back in place.  A staging area is rewritten only after
   // the synchronisation that ends the launch reading it.
-  stage_.reserve(std::max(small, cap_all * (sizeof(NccMatch) + 4)) + 256);
+  S.stage.reserve(std::max(small, cap_all * (sizeof(NccMatch) + 4)) + 256);
   const size_t rb_full = align_up(cap_all * (sizeof(NccMatch) + 4));
   const size_t rb_counts = rb_full + align_up(cap_all * sizeof(NccMatch));
-  readback_.reserve(rb_counts + n_count_words * 4 + 64);
-  auto* rb = static_cast<uint8_t*>(readback_.ptr);
+  S.readback.reserve(rb_counts + n_count_words * 4 + 64);
+  auto* rb = static_cast<uint8_t*>(S.readback.ptr);
   const NccPatchPool pools[2] = {dmodel->pool.view, view_at(dev + off_qpool, QL)};
   auto* dframes = reinterpret_cast<NccFrame*>(dev + off_frames);
   auto* lists = reinterpret_cast<NccMatch*>(dev + off_lists);
@@ -559,7 +622,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   auto* work = reinterpret_cast<unsigned long long*>(dev + off_counts + align_up((n_track + 1) * n_frames * 4, 8));
   auto* extent = reinterpret_cast<unsigned long long*>(dev + off_extent);
   {
-    uint8_t* st = static_cast<uint8_t*>(stage_.ptr);
+    uint8_t* st = static_cast<uint8_t*>(S.stage.ptr);
     for (size_t l = 0; l < level_count; ++l)
       std::memcpy(st + (off_refs[l] - off_small), refs[l].data(), refs[l].size() * 4);
     for (auto& J : jobs) std::memcpy(st + (J.off_ranges - off_small), J.ranges.data(), J.ranges.size() * 4);
@@ -610,22 +673,22 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
       ncchip_fill_u32(reinterpret_cast<uint32_t*>(dev + off_table), std::max<size_t>(n_track, 1) * n_frames * table_size,
                       0xffffffffu, stream_))
     raise(kInternalError, "device", "fill launch failed");
-  time_begin(kStagePyramid);
+  time_begin(kStagePyramid, stream_);
   for (size_t l = 1; l < level_count; ++l) {
     const auto [w, h] = max_dims(l);
     if (ncchip_pyramid_batch(dframes, F, int(l), jobs[0].lv[l - 1].kind, jobs[0].lv[l].kind, w, h, stream_))
       raise(kInternalError, "device", "pyramid launch failed");
   }
-  time_end();
+  time_end(stream_);
   const auto [Wt_max, Ht_max] = max_dims(top);
-  time_begin(kStagePrefix);
+  time_begin(kStagePrefix, stream_);
   if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, stream_))
     raise(kInternalError, "device", "prefix launch failed");
-  time_end();
+  time_end(stream_);
   const double C = std::ldexp(128.0, int(2 * top));
   const int exact_top = top <= 2;
   const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
-  time_begin(kStageCoarse);
+  time_begin(kStageCoarse, stream_);
   int rc = 0;
   if (use_mfma) {
     NcchipDenseWeights dw{};
@@ -645,15 +708,15 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
                        max_bh, max_bw, o.form, exact_top, model.params.absolute, stream_);
   if (rc == -6) raise(kUnsupported, "form", "correlation form not available in this build");
   if (rc) raise(kInternalError, "device", "coarse launch failed");
-  time_end();
-  time_begin(kStagePeaks);
+  time_end(stream_);
+  time_begin(kStagePeaks, stream_);
   if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
                    lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 1 : 0,
                    stream_))
     raise(kInternalError, "device", "peak launch failed");
   if (!finest && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
     raise(kInternalError, "device", "storage launch failed");
-  time_end();
+  time_end(stream_);
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
   constexpr int kListGrid = 2048;
   for (size_t k = 0; k < n_track; ++k) {
@@ -672,7 +735,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     const double tile_px = (std::ceil(hi_r - lo_r) + 7) * (std::ceil(hi_c - lo_c) + 7);
     const double xmax = 255. * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
-    time_begin(kStageTrack);
+    time_begin(kStageTrack, stream_);
     if (ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                      reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                      lists + k * cap_all, counts + k * n_frames, int(cap), reinterpret_cast<NccMatch*>(dev + off_raw),
@@ -681,7 +744,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
                      o.min_score * .4, tol, step, offset, l - 1 <= 2, model.params.absolute,
                      tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, stream_))
       raise(kInternalError, "device", "tracking launch failed");
-    time_end();
+    time_end(stream_);
   }
   // ---- full score at the finest level (matcher.cpp:813-822)
   NcchipRefineParams rp{};
@@ -697,15 +760,54 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
   rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
   auto* hfull = reinterpret_cast<NccMatch*>(rb + rb_full);
   auto* hcounts = reinterpret_cast<int32_t*>(rb + rb_counts);
-  time_begin(kStageFull);
+  time_begin(kStageFull, stream_);
   if (ncchip_full_score(&rp, pools, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), hfull, 1024,
                         stream_))
     raise(kInternalError, "device", "full score launch failed");
-  time_end();
+  time_end(stream_);
   if (ncchip_copy_u32(reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<uint32_t*>(hcounts), n_count_words,
                       stream_))
     raise(kInternalError, "device", "copy launch failed");
-  check(hipStreamSynchronize(stream_), "find");
+  check(hipEventRecord(S.done, stream_), "hipEventRecord");
+  S.model = &model; S.dmodel = dmodel; S.o = o; S.traces = traces; S.n_frames = n_frames; S.empty = false;
+  S.jobs = std::move(jobs); S.qpool = std::move(qpool);
+  S.cap = cap; S.cap_all = cap_all; S.n_track = n_track; S.top = top; S.finest = finest; S.stride = stride;
+  S.off_key = off_key; S.dev = dev; S.dframes = dframes; S.lists = lists; S.hfull = hfull; S.hcounts = hcounts;
+  S.rb = rb; S.pools[0] = pools[0]; S.pools[1] = pools[1]; S.rp = rp; S.t0 = t0;
+  S.ticket = ticket;
+  S.busy = true;
+  ++next_ticket_;
+  return ticket;
+}
+
+std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
+  Slot& S = *slots_[ticket % 2];
+  if (!S.busy || S.ticket != ticket) raise(kInvalidState, "batch", "no such pending batch");
+  cur_ = &S;
+  struct Release {
+    Slot& s;
+    ~Release() { s.busy = false; }
+  } release{S};
+  const NccModel& model = *S.model;
+  const FindOptions& o = S.o;
+  FindTrace* const traces = S.traces;
+  const size_t n_frames = S.n_frames;
+  std::vector<std::vector<Pose>> results(n_frames);
+  if (S.empty) return results;
+  std::vector<Job>& jobs = S.jobs;
+  const size_t cap = S.cap, cap_all = S.cap_all, n_track = S.n_track, top = S.top, finest = S.finest;
+  const size_t stride = S.stride, off_key = S.off_key;
+  uint8_t* const dev = S.dev;
+  NccFrame* const dframes = S.dframes;
+  NccMatch* const lists = S.lists;
+  NccMatch* const hfull = S.hfull;
+  int32_t* const hcounts = S.hcounts;
+  uint8_t* const rb = S.rb;
+  const NccPatchPool* const pools = S.pools;
+  const NcchipRefineParams& rp = S.rp;
+  const double t0 = S.t0;
+  const LevelGeom& base = model.levels.front();
+  check(hipEventSynchronize(S.done), "find");
   const double t1 = now_us();
   for (size_t f = 0; f < n_frames; ++f)
     if (size_t(hcounts[f]) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
@@ -768,7 +870,7 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
   }
   // refinement rounds: candidate lists and results in host-visible memory
-  auto* dcand = static_cast<NccMatch*>(stage_.ptr);
+  auto* dcand = static_cast<NccMatch*>(S.stage.ptr);
   int32_t* dcand_frame = nullptr;
   auto* dout = reinterpret_cast<NccMatch*>(rb);
   int32_t* dverdict = nullptr;
@@ -822,11 +924,11 @@ std::vector<std::vector<Pose>> FindContext::find_batch(const NccModel& model, co
     }
     if (list.empty()) break;
     upload(list);
-    time_begin(kStageRefine);
-    if (ncchip_refine(&rp, pools, dcand, dcand_frame, int(list.size()), dout, dverdict, stream_))
+    time_begin(kStageRefine, stream2_);
+    if (ncchip_refine(&rp, pools, dcand, dcand_frame, int(list.size()), dout, dverdict, stream2_))
       raise(kInternalError, "device", "refine launch failed");
-    time_end();
-    check(hipStreamSynchronize(stream_), "refine");
+    time_end(stream2_);
+    check(hipStreamSynchronize(stream2_), "refine");
     const int32_t* v = dverdict;
     for (size_t k = 0; k < list.size(); ++k) {
       Job& J = jobs[list[k].first];

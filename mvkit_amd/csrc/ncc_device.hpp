@@ -113,6 +113,16 @@ class FindContext {
   // kernel timings of the batch are reported in every trace.
   std::vector<std::vector<Pose>> find_batch(const NccModel& model, const FrameInput* frames, size_t n_frames,
                                             const FindOptions& options, FindTrace* traces = nullptr);
+  // The two halves of find_batch, for overlapping consecutive batches: submit
+  // enqueues the device-resident part (pyramid to full score, at most
+  // NCC_MAX_BATCH frames) without waiting and returns a ticket; complete
+  // waits for it and runs the refinement rounds (on a second stream, so they
+  // overlap the device work of a batch submitted after this one).  At most
+  // two batches may be pending; the model, the frames and the traces must
+  // stay alive until complete returns.
+  uint64_t submit_batch(const NccModel& model, const FrameInput* frames, size_t n_frames, const FindOptions& options,
+                        FindTrace* traces = nullptr);
+  std::vector<std::vector<Pose>> complete_batch(uint64_t ticket);
   // Same, from host memory (copied to the device first).
   std::vector<Pose> find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
                               size_t pitch, const FindOptions& options, FindTrace* trace = nullptr);
@@ -121,20 +131,17 @@ class FindContext {
   int device() const { return device_; }
 
  private:
-  struct Timer { hipEvent_t begin = nullptr, end = nullptr; int stage = 0; };
-  void time_begin(int stage);
-  void time_end();
+  struct Slot;  // state of a submitted batch
+  void time_begin(int stage, hipStream_t stream);
+  void time_end(hipStream_t stream);
   void time_collect(FindTrace* trace);
-  int profiling_ = 0;   // stage mask of this find
-  bool timing_ = false; // the open stage is timed
-  std::vector<Timer> timers_;
-  size_t timers_used_ = 0;
-  hipStream_t stream_ = nullptr;
+  std::unique_ptr<Slot> slots_[2];
+  Slot* cur_ = nullptr;  // the slot being submitted or completed
+  uint64_t next_ticket_ = 0;
+  hipStream_t stream_ = nullptr;   // device-resident part
+  hipStream_t stream2_ = nullptr;  // refinement rounds
   int device_ = 0;
-  DeviceBuffer work_;
   DeviceBuffer image_;
-  PinnedBuffer stage_;
-  PinnedBuffer readback_;
 };
 
 // Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).

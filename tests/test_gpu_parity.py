@@ -332,3 +332,31 @@ def test_fp8_screen_form_is_unsupported(capi, scene_small):
     with pytest.raises(capi.MvError) as e:
         m.find_trace(scene_small.image, opts)
     assert e.value.status == -6
+
+
+def test_overlapped_batches_match_single_batches(capi):
+    """Two submitted batches in flight return what find_batch returns for
+    each; a third submission while two are pending is refused."""
+    from mvkit_amd import synth
+    say("overlapped batches")
+    scenes = [synth.small_rot(frame=k) for k in range(4)]
+    m = capi.NccModel(scenes[0].template, "auto", -math.radians(20), math.radians(40))
+    opts = capi.NccModel.options(-math.radians(20), math.radians(40), 0.6, 0, 0.5)
+    dev = [torch.from_numpy(s.image).cuda() for s in scenes]
+    h, w = scenes[0].image.shape
+    a_ptrs, b_ptrs = [d.data_ptr() for d in dev[:2]], [d.data_ptr() for d in dev[2:]]
+    want_a, _ = m.find_batch(a_ptrs, w, h, w, opts)
+    want_b, _ = m.find_batch(b_ptrs, w, h, w, opts)
+    ha = m.submit_batch(a_ptrs, w, h, w, opts)
+    hb = m.submit_batch(b_ptrs, w, h, w, opts)
+    with pytest.raises(capi.MvError) as e:
+        m.submit_batch(a_ptrs, w, h, w, opts)
+    assert e.value.status == -7
+    got_a, _ = m.wait_batch(ha)
+    hc = m.submit_batch(a_ptrs, w, h, w, opts)  # a slot is free again
+    got_b, _ = m.wait_batch(hb)
+    got_c, _ = m.wait_batch(hc)
+    for got, want in ((got_a, want_a), (got_b, want_b), (got_c, want_a)):
+        assert len(got) == len(want)
+        for g, x in zip(got, want):
+            assert np.array_equal(g, x)
