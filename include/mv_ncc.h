@@ -123,6 +123,17 @@ MV_API MvStatus mv_get_ncc_model_params(MvTuple ModelID, MvTuple* NumLevels, MvT
 MV_API MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple AngleStart, MvTuple AngleExtent,
                                   MvTuple MinScore, MvTuple NumMatches, MvTuple MaxOverlap, MvTuple SubPixel,
                                   MvTuple NumLevels, MvTuple* Row, MvTuple* Column, MvTuple* Angle, MvTuple* Score);
+/* include/mvkit/c/mv_ncc_matching.h:119-123 (src/c_api/ncc_matching_api.cpp:191-203):
+ * several models in one image.  ModelIDs holds 1..4096 handles; every other
+ * control one value or one per model (NumLevels: highest, a shared pair, one
+ * highest per model or a pair per model).  One-value NumMatches / MaxOverlap
+ * apply to all models together, per-model values to each model alone, with
+ * per-model quotas taken before the cross-model overlap.  Model receives the
+ * index of each match's model in ModelIDs. */
+MV_API MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple AngleStart, MvTuple AngleExtent,
+                                   MvTuple MinScore, MvTuple NumMatches, MvTuple MaxOverlap, MvTuple SubPixel,
+                                   MvTuple NumLevels, MvTuple* Row, MvTuple* Column, MvTuple* Angle, MvTuple* Score,
+                                   MvTuple* Model);
 
 #ifdef __cplusplus
 }

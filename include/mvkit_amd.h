@@ -58,6 +58,19 @@ MV_API MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* d_im
                                             int64_t pitch, const MvAmdFindOptions* options, double* results,
                                             size_t cap, size_t* count, MvAmdFindStats* stats);
 
+/* find_ncc_models on a u8 image given as a HIP device pointer.  options holds
+ * one entry or one per model (n_options); the selection is what
+ * mv_find_ncc_models derives from one-value controls: selection_matches
+ * (-1: per model) and selection_overlap (-1: per model, same-model pairs
+ * only).  Matches go to results as cap x 5 doubles (row, col, angle, score,
+ * model index) in the internal (centroid) convention; their number to count.
+ * Replaces, for device-resident inputs, mv_find_ncc_models
+ * (mvkit include/mvkit/c/mv_ncc_matching.h:119-123) minus the output shift. */
+MV_API MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t* d_image, int64_t width, int64_t height,
+                                             int64_t pitch, const MvAmdFindOptions* options, size_t n_options,
+                                             int64_t selection_matches, double selection_overlap, double* results,
+                                             size_t cap, size_t* count);
+
 /* Finds the model in n u8 images given as HIP device pointers, all with the
  * same options, stage by stage over the whole batch (one launch per stage for
  * the candidates of every frame).  Frame k's matches go to

@@ -158,6 +158,12 @@ typedef struct NcchipRefineParams {
   int32_t n_refs0;
   double step, angle_start, angle_end, min_score;
   int32_t subpixel, rotate, absolute;
+  /* find_models' query-cache rules (matcher.cpp:912-927, 984-1055); all zero
+   * or negative for find: */
+  int32_t recover_first;   /* a candidate below min_score is recovered before its fit */
+  int32_t prefix_recovery; /* recovery windows in the row-prefix form on the whole level 0 */
+  double guard_first;      /* background guard after fit iteration 0 (< 0: off) */
+  double guard;            /* background guard after fit iteration 1 (< 0: off) */
 } NcchipRefineParams;
 
 /* Continuous bilinear score at the finest level of every candidate of a
@@ -171,7 +177,9 @@ int ncchip_full_score(const NcchipRefineParams* p, const NccPatchPool* pools, in
 /* refine_candidate of every given full-scored candidate (Gauss-Newton fit
  * with backtracking, and the level-0 recovery search), one workgroup per
  * candidate; outputs the refined pose and the verdict (score >= MinScore
- * inside the domain).  Replaces matcher.cpp:853-1061. */
+ * inside the domain).  Replaces matcher.cpp:853-1061, with find_models'
+ * recover-first order, prefix-form recovery windows and background guard
+ * when the params ask for them. */
 int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* pools, const NccMatch* cand,
                   const int32_t* cand_frame, int n_cand, NccMatch* out_refined, int32_t* verdict, void* stream);
 

@@ -27,8 +27,8 @@ PUBLIC_SYMBOLS = [
     "mv_get_last_error", "mv_get_last_error_info", "mv_create_tuple", "mv_copy_tuple", "mv_destroy_tuple",
     "mv_length_tuple", "mv_get_type", "mv_get_i", "mv_get_d", "mv_get_s", "mv_get_h", "mv_resize_tuple", "mv_set_i",
     "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
-    "mv_clear_handle", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model",
-    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
+    "mv_clear_handle", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
+    "mvamd_find_ncc_model_device", "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
@@ -112,6 +112,9 @@ def lib() -> C.CDLL:
         L.mv_create_ncc_model.argtypes = [P, P, P, P, P, P, C.POINTER(P)]
         L.mv_get_ncc_model_params.argtypes = [P] + [C.POINTER(P)] * 5
         L.mv_find_ncc_model.argtypes = [P] * 9 + [C.POINTER(P)] * 4
+        L.mv_find_ncc_models.argtypes = [P] * 9 + [C.POINTER(P)] * 5
+        L.mvamd_find_ncc_models_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), S, I64, D, P, S,
+                                                   C.POINTER(S)]
         L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
                                                   C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), P, S, P,
@@ -349,6 +352,53 @@ class NccModel:
         return {"results": res[:n.value].copy(), "coarse": coarse[:min(nc.value, cap)].copy(),
                 "prepared": prep[:min(nprep.value, cap)].copy(), "refined": refd[:min(nprep.value, cap)].copy(),
                 "stats": st}
+
+
+def _reals(v):
+    return [_real(x) for x in v] if isinstance(v, (list, tuple)) else _real(v)
+
+
+def _model_ids(models) -> Tuple:
+    """One tuple holding the handles of several models."""
+    h = C.c_void_p()
+    _check(lib().mv_create_tuple(C.byref(h), len(models)))
+    t = Tuple(handle=h)
+    for i, m in enumerate(models):
+        hv = C.c_void_p()
+        _check(lib().mv_get_h(m.tuple.h, 0, C.byref(hv)))
+        _check(lib().mv_set_h(t.h, hv, i))
+    return t
+
+
+def find_models(models, image, angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5,
+                subpixel="true", num_levels=0):
+    """mv_find_ncc_models (mirrors MvNCCModel::FindNccModels); every control
+    is one value or a list with one per model.  Returns an (n, 5) array of
+    Row, Column, Angle, Score in the C ABI's HALCON convention and the model
+    index."""
+    img = image if isinstance(image, Image) else Image(image)
+    ids = _model_ids(models)
+    outs = [C.c_void_p() for _ in range(5)]
+    args = [_to_tuple(_reals(angle_start)), _to_tuple(_reals(angle_extent)), _to_tuple(_reals(min_score)),
+            _to_tuple(num_matches), _to_tuple(_reals(max_overlap)), _to_tuple(subpixel), _to_tuple(num_levels)]
+    _check(lib().mv_find_ncc_models(img.h, ids.h, *[t.h for t in args], *[C.byref(o) for o in outs]))
+    cols = [Tuple(handle=o).values() for o in outs]
+    return np.array(cols, dtype=np.float64).T.reshape(-1, 5)
+
+
+def find_models_device(models, d_image: int, width: int, height: int, pitch: int, opts, selection_matches=-1,
+                       selection_overlap=-1.0, cap: int = 4096):
+    """mvamd_find_ncc_models_device: opts is one FindOptions or one per model.
+    Returns (n, 5): row, col, angle, score (internal convention), model index."""
+    ids = _model_ids(models)
+    ol = list(opts) if isinstance(opts, (list, tuple)) else [opts]
+    arr = (FindOptions * len(ol))(*ol)
+    res = np.zeros((cap, 5))
+    n = C.c_size_t()
+    _check(lib().mvamd_find_ncc_models_device(ids.h, C.c_void_p(d_image), width, height, pitch, arr, len(ol),
+                                              int(selection_matches), float(selection_overlap), _ptr(res), cap,
+                                              C.byref(n)))
+    return res[:min(n.value, cap)].copy()
 
 
 def halcon_shift(poses: np.ndarray) -> np.ndarray:

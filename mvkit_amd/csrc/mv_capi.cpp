@@ -215,36 +215,81 @@ mvamd::FindContext& context() {
   return *contexts[size_t(dev)];
 }
 
-// NccControls for one model (ncc_matching_api.cpp:126-161).
-mvamd::FindOptions controls(const char* op, MvTuple a, MvTuple e, MvTuple s, MvTuple n, MvTuple o, MvTuple p, MvTuple l) {
+// NccControls (ncc_matching_api.cpp:126-161): each control holds one value
+// or one per model; NumLevels holds the highest level, a shared
+// (highest, lowest) pair, one highest level per model or a pair per model.
+// The selection takes NumMatches / MaxOverlap when they hold one value.
+struct Controls {
+  std::vector<mvamd::FindOptions> options;
+  mvamd::Selection selection;
+};
+Controls controls_n(const char* op, size_t count, MvTuple a, MvTuple e, MvTuple s, MvTuple n, MvTuple o, MvTuple p,
+                    MvTuple l) {
   const Input start(a, op, "AngleStart"), extent(e, op, "AngleExtent"), score(s, op, "MinScore"),
       matches(n, op, "NumMatches"), overlap(o, op, "MaxOverlap"), subpixel(p, op, "SubPixel"), levels(l, op, "NumLevels");
   for (const Input* c : {&start, &extent, &score, &matches, &overlap, &subpixel})
-    if (c->size() != 1) c->bad(mvamd::kErrInvalidCardinality, -1, "one value or one value per model required");
-  if (levels.size() != 1 && levels.size() != 2)
+    if (c->size() != 1 && c->size() != count)
+      c->bad(mvamd::kErrInvalidCardinality, -1, "one value or one value per model required");
+  if (levels.size() != 1 && levels.size() != 2 && levels.size() != count && levels.size() != 2 * count)
     levels.bad(mvamd::kErrInvalidCardinality, -1, "highest level or shared/per-model level pairs required");
-  auto number = [](const Input& v, double lo, double hi) {
-    const double x = v.number(0);
-    if (!std::isfinite(x) || x < lo || x > hi) v.bad(mvamd::kErrInvalidValue, 0, "numeric control outside allowed range");
+  auto number = [](const Input& v, size_t i, double lo, double hi) {
+    const size_t index = v.size() == 1 ? 0 : i;
+    const double x = v.number(index);
+    if (!std::isfinite(x) || x < lo || x > hi) v.bad(mvamd::kErrInvalidValue, int64_t(index), "numeric control outside allowed range");
     return x;
   };
-  mvamd::FindOptions q;
-  q.start = number(start, -std::numeric_limits<double>::max(), std::numeric_limits<double>::max());
-  q.extent = number(extent, 0, 6.28318530717958647692);
-  q.min_score = number(score, 0, 1);
-  q.max_overlap = number(overlap, 0, 1);
-  const int64_t quota = matches.integer(0);
-  if (quota < 0) matches.bad(mvamd::kErrInvalidValue, 0, "match count must be nonnegative");
-  q.num_matches = size_t(quota);
-  const std::string& mode = subpixel.string(0);
-  if (mode != "true" && mode != "false") subpixel.bad(mvamd::kErrUnsupportedValue, 0, "expected true or false");
-  q.subpixel = mode == "true";
-  const int64_t highest = levels.integer(0), lowest = levels.size() == 2 ? levels.integer(1) : 1;
-  if (highest < 0 || highest > std::numeric_limits<int>::max() || lowest < 0 || lowest > std::numeric_limits<int>::max())
-    levels.bad(mvamd::kErrInvalidValue, 0, "invalid highest/lowest level pair");
-  q.highest = int(highest);
-  q.lowest = int(std::max<int64_t>(1, lowest));
-  return q;
+  Controls out;
+  out.options.reserve(count);
+  for (size_t i = 0; i < count; ++i) {
+    mvamd::FindOptions q;
+    q.start = number(start, i, -std::numeric_limits<double>::max(), std::numeric_limits<double>::max());
+    q.extent = number(extent, i, 0, 6.28318530717958647692);
+    q.min_score = number(score, i, 0, 1);
+    q.max_overlap = number(overlap, i, 0, 1);
+    const size_t mi = matches.size() == 1 ? 0 : i;
+    const int64_t quota = matches.integer(mi);
+    if (quota < 0) matches.bad(mvamd::kErrInvalidValue, int64_t(mi), "match count must be nonnegative");
+    q.num_matches = size_t(quota);
+    const size_t si = subpixel.size() == 1 ? 0 : i;
+    const std::string& mode = subpixel.string(si);
+    if (mode != "true" && mode != "false") subpixel.bad(mvamd::kErrUnsupportedValue, int64_t(si), "expected true or false");
+    q.subpixel = mode == "true";
+    size_t high_index = 0, low_index = 0;
+    bool pair = false;
+    if (levels.size() == count && count > 1) high_index = i;
+    else if (levels.size() == 2 * count) { high_index = 2 * i; low_index = high_index + 1; pair = true; }
+    else if (levels.size() == 2) { low_index = 1; pair = true; }
+    const int64_t highest = levels.integer(high_index), lowest = pair ? levels.integer(low_index) : 1;
+    if (highest < 0 || highest > std::numeric_limits<int>::max() || lowest < 0 || lowest > std::numeric_limits<int>::max())
+      levels.bad(mvamd::kErrInvalidValue, int64_t(high_index), "invalid highest/lowest level pair");
+    q.highest = int(highest);
+    q.lowest = int(std::max<int64_t>(1, lowest));
+    out.options.push_back(q);
+  }
+  out.selection.num_matches = matches.size() == 1 ? int64_t(out.options.front().num_matches) : -1;
+  out.selection.max_overlap = overlap.size() == 1 ? out.options.front().max_overlap : -1;
+  return out;
+}
+mvamd::FindOptions controls(const char* op, MvTuple a, MvTuple e, MvTuple s, MvTuple n, MvTuple o, MvTuple p, MvTuple l) {
+  return controls_n(op, 1, a, e, s, n, o, p, l).options.front();
+}
+
+// ModelIDs of find_ncc_models (ncc_matching_api.cpp:111-124): 1..4096 handles.
+std::vector<std::shared_ptr<mvamd::NccModel>> models_of(MvTuple t, const char* op) {
+  const Input view(t, op, "ModelIDs");
+  if (!view.size() || view.size() > 4096)
+    view.bad(mvamd::kErrInvalidCardinality, -1, "one model or a nonempty model array (maximum 4096) required");
+  std::vector<std::shared_ptr<mvamd::NccModel>> out;
+  for (size_t i = 0; i < view.size(); ++i) {
+    if (view.kind(i) != MV_VALUE_HANDLE) view.bad(mvamd::kErrInvalidType, int64_t(i), "NCC model handle required");
+    const Value& v = view.at(i);
+    if (!v.h || !v.h->object) view.bad(mvamd::kErrInvalidType, int64_t(i), "control handle is not an NCC model");
+    std::lock_guard<std::mutex> lock(v.h->object->mu);
+    if (!v.h->object->model)
+      fail(mvamd::kInvalidState, mvamd::kErrInvalidObjectState, op, "ModelIDs", "NCC model handle was cleared", int64_t(i));
+    out.push_back(v.h->object->model);
+  }
+  return out;
 }
 
 mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
@@ -551,6 +596,38 @@ extern "C" MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple A
   });
 }
 
+extern "C" MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple AngleStart, MvTuple AngleExtent,
+                                       MvTuple MinScore, MvTuple NumMatches, MvTuple MaxOverlap, MvTuple SubPixel,
+                                       MvTuple NumLevels, MvTuple* Row, MvTuple* Column, MvTuple* Angle, MvTuple* Score,
+                                       MvTuple* Model) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_models";
+    Outputs<5> out(op, {Row, Column, Angle, Score, Model});
+    required(Image, op, "Image");
+    const auto ms = models_of(ModelIDs, op);
+    const Controls c = controls_n(op, ms.size(), AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel,
+                                  NumLevels);
+    if (Image->empty) { out.commit(); return; }
+    if (Image->pixel_bytes != 1)
+      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Image", "uint2 search images are not supported on the device path");
+    std::vector<const mvamd::NccModel*> ptrs;
+    for (const auto& m : ms) ptrs.push_back(m.get());
+    const auto found = algorithm(op, [&] {
+      return context().find_models_host(ptrs.data(), c.options.data(), ptrs.size(), Image->pixels, Image->width,
+                                        Image->height, Image->width, c.selection, true);
+    });
+    for (const auto& m : found) {
+      const double cs = std::cos(m.pose.angle), sn = std::sin(m.pose.angle);
+      push(out[0], m.pose.row + .5 * (1 - cs + sn));
+      push(out[1], m.pose.col + .5 * (1 - cs - sn));
+      push(out[2], m.pose.angle);
+      push(out[3], m.pose.score);
+      push(out[4], int64_t(m.model));
+    }
+    out.commit();
+  });
+}
+
 // ============================================================ extensions
 namespace {
 void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
@@ -590,6 +667,44 @@ extern "C" MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* 
     if (results) put_poses(found, results, cap);
     *count = found.size();
     fill_stats(trace, stats);
+  });
+}
+
+extern "C" MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t* d_image, int64_t width,
+                                                 int64_t height, int64_t pitch, const MvAmdFindOptions* options,
+                                                 size_t n_options, int64_t selection_matches,
+                                                 double selection_overlap, double* results, size_t cap,
+                                                 size_t* count) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_models";
+    required(d_image, op, "Image");
+    required(options, op, "options");
+    required(count, op, "count");
+    if (width <= 0 || height <= 0 || pitch < width)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
+    const auto ms = models_of(ModelIDs, op);
+    if (n_options != 1 && n_options != ms.size())
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, "options", "one value or one value per model required");
+    std::vector<mvamd::FindOptions> qs;
+    std::vector<const mvamd::NccModel*> ptrs;
+    for (size_t i = 0; i < ms.size(); ++i) {
+      qs.push_back(from_ext(options + (n_options == 1 ? 0 : i)));
+      qs.back().profile = 0;
+      ptrs.push_back(ms[i].get());
+    }
+    mvamd::Selection sel;
+    sel.num_matches = selection_matches;
+    sel.max_overlap = selection_overlap;
+    const mvamd::FrameInput in{d_image, size_t(width), size_t(height), size_t(pitch)};
+    const auto found = algorithm(op, [&] {
+      return context().find_models(ptrs.data(), qs.data(), ptrs.size(), in, sel, true);
+    });
+    for (size_t i = 0; results && i < found.size() && i < cap; ++i) {
+      double* d = results + 5 * i;
+      d[0] = found[i].pose.row; d[1] = found[i].pose.col; d[2] = found[i].pose.angle; d[3] = found[i].pose.score;
+      d[4] = double(found[i].model);
+    }
+    *count = found.size();
   });
 }
 

@@ -317,7 +317,7 @@ struct FindContext::Slot {
   uint8_t* rb = nullptr;
   NccPatchPool pools[2] = {};
   NcchipRefineParams rp{};
-  double t0 = 0;
+  double t0 = 0, t1 = 0, t2 = 0;
 };
 
 FindContext::FindContext() {
@@ -383,6 +383,22 @@ std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* i
   return find(model, image_.as<uint8_t>(), width, height, width, options, trace);
 }
 
+std::vector<ModelMatch> FindContext::find_models_host(const NccModel* const* models, const FindOptions* options,
+                                                      size_t n_models, const uint8_t* image, size_t width,
+                                                      size_t height, size_t pitch, const Selection& selection,
+                                                      bool per_model_quota_before_overlap) {
+  if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
+  const size_t bytes = width * height;
+  check(hipStreamSynchronize(stream2_), "sync");
+  if (image_.bytes < bytes) {
+    check(hipStreamSynchronize(stream_), "sync");
+    image_ = DeviceBuffer(align_up(bytes, 1 << 20));
+  }
+  check(hipMemcpy2DAsync(image_.ptr, width, image, pitch, width, height, hipMemcpyHostToDevice, stream_), "image upload");
+  const FrameInput in{image_.as<uint8_t>(), width, height, width};
+  return find_models(models, options, n_models, in, selection, per_model_quota_before_overlap);
+}
+
 std::vector<Pose> FindContext::find(const NccModel& model, const uint8_t* image, size_t W, size_t H, size_t pitch,
                                     const FindOptions& req, FindTrace* trace) {
   FrameInput in{image, W, H, pitch};
@@ -411,15 +427,23 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
   Slot& S = *slots_[ticket % 2];
   if (S.busy) raise(kInvalidState, "batch", "two batches are pending; complete the oldest first");
   cur_ = &S;
+  enqueue(S, model, inputs, n_frames, req, traces);
+  S.ticket = ticket;
+  S.busy = true;
+  ++next_ticket_;
+  return ticket;
+}
+
+void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inputs, size_t n_frames,
+                          const FindOptions& req, FindTrace* traces) {
   const double t0 = now_us();
   S.profiling = traces ? req.profile : 0;
   S.timers_used = 0;
   S.jobs.clear();
+  S.empty = false;
   for (size_t f = 0; f < n_frames && traces; ++f) traces[f] = FindTrace{};
   auto finish_empty = [&]() {
-    S.model = &model; S.traces = traces; S.n_frames = n_frames; S.empty = true; S.ticket = ticket; S.busy = true;
-    ++next_ticket_;
-    return ticket;
+    S.model = &model; S.o = req; S.traces = traces; S.n_frames = n_frames; S.empty = true;
   };
   if (!n_frames) return finish_empty();
   // ---- options and levels, shared by the batch (matcher.cpp:345-374)
@@ -714,7 +738,8 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
                    lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 1 : 0,
                    stream_))
     raise(kInternalError, "device", "peak launch failed");
-  if (!finest && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
+  // find_models over several models converts the whole level 0 (matcher.cpp:596-611)
+  if (!finest && !o.whole_level0 && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
     raise(kInternalError, "device", "storage launch failed");
   time_end(stream_);
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
@@ -758,6 +783,10 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
   rp.n_refs0 = int32_t(refs[0].size());
   rp.step = model.params.step; rp.angle_start = o.start; rp.angle_end = o.start + o.extent;
   rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
+  rp.recover_first = o.query_cache;
+  rp.prefix_recovery = o.whole_level0;
+  rp.guard_first = -1;  // set by find_models once the candidate count is known
+  rp.guard = -1;
   auto* hfull = reinterpret_cast<NccMatch*>(rb + rb_full);
   auto* hcounts = reinterpret_cast<int32_t*>(rb + rb_counts);
   time_begin(kStageFull, stream_);
@@ -774,41 +803,20 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
   S.cap = cap; S.cap_all = cap_all; S.n_track = n_track; S.top = top; S.finest = finest; S.stride = stride;
   S.off_key = off_key; S.dev = dev; S.dframes = dframes; S.lists = lists; S.hfull = hfull; S.hcounts = hcounts;
   S.rb = rb; S.pools[0] = pools[0]; S.pools[1] = pools[1]; S.rp = rp; S.t0 = t0;
-  S.ticket = ticket;
-  S.busy = true;
-  ++next_ticket_;
-  return ticket;
 }
 
-std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
-  Slot& S = *slots_[ticket % 2];
-  if (!S.busy || S.ticket != ticket) raise(kInvalidState, "batch", "no such pending batch");
+// Waits for a slot's device-resident part and turns its full-scored lists into
+// each frame's candidate order (matcher.cpp:824-827); fills the stage traces.
+void FindContext::collect(Slot& S) {
   cur_ = &S;
-  struct Release {
-    Slot& s;
-    ~Release() { s.busy = false; }
-  } release{S};
-  const NccModel& model = *S.model;
   const FindOptions& o = S.o;
   FindTrace* const traces = S.traces;
   const size_t n_frames = S.n_frames;
-  std::vector<std::vector<Pose>> results(n_frames);
-  if (S.empty) return results;
   std::vector<Job>& jobs = S.jobs;
-  const size_t cap = S.cap, cap_all = S.cap_all, n_track = S.n_track, top = S.top, finest = S.finest;
-  const size_t stride = S.stride, off_key = S.off_key;
-  uint8_t* const dev = S.dev;
-  NccFrame* const dframes = S.dframes;
-  NccMatch* const lists = S.lists;
-  NccMatch* const hfull = S.hfull;
+  const size_t cap = S.cap, cap_all = S.cap_all, n_track = S.n_track;
   int32_t* const hcounts = S.hcounts;
-  uint8_t* const rb = S.rb;
-  const NccPatchPool* const pools = S.pools;
-  const NcchipRefineParams& rp = S.rp;
-  const double t0 = S.t0;
-  const LevelGeom& base = model.levels.front();
   check(hipEventSynchronize(S.done), "find");
-  const double t1 = now_us();
+  S.t1 = now_us();
   for (size_t f = 0; f < n_frames; ++f)
     if (size_t(hcounts[f]) > o.max_candidates) raise(kLimitExceeded, "max_candidates", "coarse candidate budget exceeded");
   auto list_count = [&](size_t k, size_t f) { return std::min<size_t>(size_t(std::max(hcounts[k * n_frames + f], 0)), cap); };
@@ -824,9 +832,9 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
     std::vector<NccMatch> all((n_track + 1) * cap_all);
     std::vector<int32_t> keys(cap_all);
     std::vector<NccFrame> frs(n_frames);
-    check(hipMemcpy(all.data(), lists, all.size() * sizeof(NccMatch), hipMemcpyDeviceToHost), "trace");
-    check(hipMemcpy(keys.data(), dev + off_key, keys.size() * 4, hipMemcpyDeviceToHost), "trace");
-    check(hipMemcpy(frs.data(), dframes, n_frames * sizeof(NccFrame), hipMemcpyDeviceToHost), "trace");
+    check(hipMemcpy(all.data(), S.lists, all.size() * sizeof(NccMatch), hipMemcpyDeviceToHost), "trace");
+    check(hipMemcpy(keys.data(), S.dev + S.off_key, keys.size() * 4, hipMemcpyDeviceToHost), "trace");
+    check(hipMemcpy(frs.data(), S.dframes, n_frames * sizeof(NccFrame), hipMemcpyDeviceToHost), "trace");
     for (size_t f = 0; f < n_frames; ++f) {
       Job& J = jobs[f];
       if (!J.trace) continue;
@@ -853,15 +861,14 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
       }
     }
   }
-  const double t2 = now_us();
-
+  S.t2 = now_us();
   // ---- candidate order (matcher.cpp:824-827)
   for (size_t f = 0; f < n_frames; ++f) {
     Job& J = jobs[f];
     const size_t n = list_count(n_track, f);
     J.full.reserve(n);
     for (size_t i = 0; i < n; ++i) {
-      const NccMatch& m = hfull[f * cap + i];
+      const NccMatch& m = S.hfull[f * cap + i];
       J.full.push_back({m.row, m.col, m.angle, m.score});
     }
     std::sort(J.full.begin(), J.full.end(), score_order);
@@ -869,35 +876,84 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
     J.verdict.assign(J.full.size(), -1);  // -1: not refined
     J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
   }
-  // refinement rounds: candidate lists and results in host-visible memory
-  auto* dcand = static_cast<NccMatch*>(S.stage.ptr);
-  int32_t* dcand_frame = nullptr;
-  auto* dout = reinterpret_cast<NccMatch*>(rb);
-  int32_t* dverdict = nullptr;
-  auto upload = [&](const std::vector<std::pair<uint32_t, Pose>>& list) {
-    dcand_frame = reinterpret_cast<int32_t*>(dcand + list.size());
-    dverdict = reinterpret_cast<int32_t*>(dout + list.size());
-    for (size_t i = 0; i < list.size(); ++i) {
-      const Pose& p = list[i].second;
-      dcand[i] = {p.row, p.col, p.angle, p.score};
-      dcand_frame[i] = int32_t(list[i].first);
-    }
-  };
-  auto result_at = [&](size_t i) {
-    const NccMatch& m = dout[i];
-    return Pose{m.row, m.col, m.angle, m.score};
-  };
+}
 
-  // ---- greedy selection in score order (matcher.cpp:1063-1076).  A
-  // candidate's refinement does not depend on the other candidates, so each
-  // frame's walk asks for the next candidates it may need and every frame's
-  // requests are refined together in one launch.
+// One refinement round: every request is refined by its slot's query in one
+// launch per slot (on the refinement stream, one synchronisation for all).
+// A candidate's refinement does not depend on the other candidates, so the
+// greedy walks ask for the candidates they may need next and take the
+// results in their own order.  Candidate lists and results live in the
+// slots' host-visible staging buffers, free once the slot was collected.
+void FindContext::refine_round(std::vector<RefineRequest>& reqs) {
+  if (reqs.empty()) return;
+  std::vector<size_t> order(reqs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return reqs[a].slot < reqs[b].slot; });
+  struct Launch { Slot* slot; size_t begin, end; };
+  std::vector<Launch> launches;
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i;
+    while (j < order.size() && reqs[order[j]].slot == reqs[order[i]].slot) ++j;
+    launches.push_back({reqs[order[i]].slot, i, j});
+    i = j;
+  }
+  for (const Launch& L : launches) {
+    Slot& S = *L.slot;
+    cur_ = &S;
+    const size_t n = L.end - L.begin;
+    auto* dcand = static_cast<NccMatch*>(S.stage.ptr);
+    auto* dcand_frame = reinterpret_cast<int32_t*>(dcand + n);
+    auto* dout = reinterpret_cast<NccMatch*>(S.rb);
+    auto* dverdict = reinterpret_cast<int32_t*>(dout + n);
+    for (size_t k = 0; k < n; ++k) {
+      const RefineRequest& r = reqs[order[L.begin + k]];
+      dcand[k] = {r.in.row, r.in.col, r.in.angle, r.in.score};
+      dcand_frame[k] = int32_t(r.frame);
+    }
+    time_begin(kStageRefine, stream2_);
+    if (ncchip_refine(&S.rp, S.pools, dcand, dcand_frame, int(n), dout, dverdict, stream2_))
+      raise(kInternalError, "device", "refine launch failed");
+    time_end(stream2_);
+  }
+  check(hipStreamSynchronize(stream2_), "refine");
+  for (const Launch& L : launches) {
+    const size_t n = L.end - L.begin;
+    const auto* dout = reinterpret_cast<const NccMatch*>(L.slot->rb);
+    const auto* dverdict = reinterpret_cast<const int32_t*>(dout + n);
+    for (size_t k = 0; k < n; ++k) {
+      RefineRequest& r = reqs[order[L.begin + k]];
+      const NccMatch& m = dout[k];
+      r.out = {m.row, m.col, m.angle, m.score};
+      r.verdict = dverdict[k];
+    }
+  }
+}
+
+std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
+  Slot& S = *slots_[ticket % 2];
+  if (!S.busy || S.ticket != ticket) raise(kInvalidState, "batch", "no such pending batch");
+  cur_ = &S;
+  struct Release {
+    Slot& s;
+    ~Release() { s.busy = false; }
+  } release{S};
+  const NccModel& model = *S.model;
+  const FindOptions& o = S.o;
+  const size_t n_frames = S.n_frames;
+  std::vector<std::vector<Pose>> results(n_frames);
+  if (S.empty) return results;
+  collect(S);
+  std::vector<Job>& jobs = S.jobs;
+  const LevelGeom& base = model.levels.front();
+
+  // ---- greedy selection in score order (matcher.cpp:1063-1076); every
+  // frame's next requests are refined together in one launch.
   auto suppressed = [&](const Job& J, const Pose& c) {
     for (const auto& p : J.results) if (pose_overlap(c, p, base) > o.max_overlap + 1e-12) return true;
     return false;
   };
   for (;;) {
-    std::vector<std::pair<uint32_t, Pose>> list;
+    std::vector<RefineRequest> reqs;
     std::vector<size_t> index;  // candidate index within its frame
     for (size_t f = 0; f < n_frames; ++f) {
       Job& J = jobs[f];
@@ -916,27 +972,23 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
       size_t taken = 0;
       for (size_t i = J.pos; i < n && taken < J.batch_size; ++i)
         if (std::isfinite(J.full[i].score) && J.verdict[i] < 0 && !suppressed(J, J.full[i])) {
-          list.push_back({uint32_t(f), J.full[i]});
+          reqs.push_back({&S, uint32_t(f), J.full[i], {}, 0});
           index.push_back(i);
           ++taken;
         }
       J.batch_size *= 2;
     }
-    if (list.empty()) break;
-    upload(list);
-    time_begin(kStageRefine, stream2_);
-    if (ncchip_refine(&rp, pools, dcand, dcand_frame, int(list.size()), dout, dverdict, stream2_))
-      raise(kInternalError, "device", "refine launch failed");
-    time_end(stream2_);
-    check(hipStreamSynchronize(stream2_), "refine");
-    const int32_t* v = dverdict;
-    for (size_t k = 0; k < list.size(); ++k) {
-      Job& J = jobs[list[k].first];
-      J.refined[index[k]] = result_at(k);
-      J.verdict[index[k]] = v[k];
+    if (reqs.empty()) break;
+    refine_round(reqs);
+    for (size_t k = 0; k < reqs.size(); ++k) {
+      Job& J = jobs[reqs[k].frame];
+      J.refined[index[k]] = reqs[k].out;
+      J.verdict[index[k]] = reqs[k].verdict;
     }
   }
+  cur_ = &S;
   const double t3 = now_us();
+  FindTrace* const traces = S.traces;
   time_collect(traces);  // event timings of the whole batch, into the first trace
   for (size_t f = 0; f < n_frames; ++f) {
     Job& J = jobs[f];
@@ -944,16 +996,254 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
     results[f] = J.results;
     if (J.trace) {
       FindTrace& tr = *J.trace;
-      tr.top = int(top); tr.finest = int(finest); tr.angle_stride = int(stride);
+      tr.top = int(S.top); tr.finest = int(S.finest); tr.angle_stride = int(S.stride);
       std::copy(J.storage0, J.storage0 + 4, tr.storage);
       tr.prepared = J.full; tr.refined = J.refined; tr.verdict = J.verdict;
       tr.coarse_flops = J.coarse_flops;
       tr.track_flops = J.track_flops;
-      tr.t_coarse_us = t1 - t0; tr.t_track_us = t2 - t1; tr.t_refine_us = t3 - t2; tr.t_total_us = now_us() - t0;
+      tr.t_coarse_us = S.t1 - S.t0; tr.t_track_us = S.t2 - S.t1; tr.t_refine_us = t3 - S.t2; tr.t_total_us = now_us() - S.t0;
       if (f > 0 && traces) std::copy(traces[0].kernel_us, traces[0].kernel_us + kStageCount, tr.kernel_us);
     }
   }
   return results;
+}
+
+// ------------------------------------------------------------- find_models
+namespace {
+struct Box { double x[4], y[4]; };
+struct Bounds { double top, bottom, left, right; };
+
+// Template rectangle [-o-1/2, w-o-1/2] at a pose (multi.hpp:54-60, scale 1).
+Box pose_box(const LevelGeom& lv, const Pose& p) {
+  const double top = -lv.orow - .5, bottom = double(lv.h) - lv.orow - .5;
+  const double left = -lv.ocol - .5, right = double(lv.w) - lv.ocol - .5;
+  const double ca = std::cos(p.angle), sa = std::sin(p.angle);
+  const double xs[4] = {left, right, right, left}, ys[4] = {top, top, bottom, bottom};
+  Box b;
+  for (int i = 0; i < 4; ++i) {
+    b.x[i] = p.col + ca * xs[i] + sa * ys[i];
+    b.y[i] = p.row - sa * xs[i] + ca * ys[i];
+  }
+  return b;
+}
+Bounds enclosure(const Box& b) {  // multi.hpp:19-23
+  Bounds o{b.y[0], b.y[0], b.x[0], b.x[0]};
+  for (int i = 0; i < 4; ++i) {
+    o.top = std::min(o.top, b.y[i]); o.bottom = std::max(o.bottom, b.y[i]);
+    o.left = std::min(o.left, b.x[i]); o.right = std::max(o.right, b.x[i]);
+  }
+  return o;
+}
+bool disjoint(const Bounds& a, const Bounds& b) {  // multi.hpp:24-26
+  return a.right < b.left || b.right < a.left || a.bottom < b.top || b.bottom < a.top;
+}
+double polygon_area(const double* x, const double* y, size_t n) {  // multi.hpp:61-63
+  double a = 0;
+  for (size_t i = 0; i < n; ++i) a += x[i] * y[(i + 1) % n] - x[(i + 1) % n] * y[i];
+  return std::abs(a) * .5;
+}
+}  // namespace
+
+// Intersection over the smaller rectangle, both translated by a's first
+// corner (multi.hpp:64-81).
+double box_overlap(const Pose& pa, const LevelGeom& ga, const Pose& pb, const LevelGeom& gb) {
+  Box a = pose_box(ga, pa), b = pose_box(gb, pb);
+  const double ox = a.x[0], oy = a.y[0];
+  for (int i = 0; i < 4; ++i) { a.x[i] -= ox; a.y[i] -= oy; b.x[i] -= ox; b.y[i] -= oy; }
+  const double den = std::min(polygon_area(a.x, a.y, 4), polygon_area(b.x, b.y, 4));
+  if (den <= 1e-12) return 0;
+  double px[16], py[16], nx[16], ny[16];
+  std::copy(a.x, a.x + 4, px); std::copy(a.y, a.y + 4, py);
+  size_t count = 4;
+  for (int e = 0; e < 4 && count; ++e) {
+    const double ux = b.x[e], uy = b.y[e], vx = b.x[(e + 1) % 4], vy = b.y[(e + 1) % 4];
+    auto cross = [&](double x, double y) { return (vx - ux) * (y - uy) - (vy - uy) * (x - ux); };
+    size_t m = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const size_t j = (i + 1) % count;
+      const double dp = cross(px[i], py[i]), dq = cross(px[j], py[j]);
+      if (dp >= 0) { nx[m] = px[i]; ny[m] = py[i]; ++m; }
+      if ((dp >= 0) != (dq >= 0)) {
+        const double t = dp / (dp - dq);
+        nx[m] = px[i] + t * (px[j] - px[i]); ny[m] = py[i] + t * (py[j] - py[i]); ++m;
+      }
+    }
+    std::copy(nx, nx + 16, px); std::copy(ny, ny + 16, py);
+    count = m;
+  }
+  return std::clamp(polygon_area(px, py, count) / den, 0., 1.);
+}
+
+namespace {
+bool same_request(const FindOptions& a, const FindOptions& b) {  // matcher.cpp:1083-1086
+  return a.start == b.start && a.extent == b.extent && a.min_score == b.min_score && a.max_overlap == b.max_overlap &&
+         a.num_matches == b.num_matches && a.max_candidates == b.max_candidates && a.subpixel == b.subpixel &&
+         a.highest == b.highest && a.lowest == b.lowest;
+}
+bool multi_less(const ModelMatch& a, const ModelMatch& b) {  // multi.hpp:87-93
+  if (a.pose.score != b.pose.score) return a.pose.score > b.pose.score;
+  if (a.model != b.model) return a.model < b.model;
+  return pose_less(a.pose, b.pose);
+}
+}  // namespace
+
+std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, const FindOptions* opts, size_t n_models,
+                                                 const FrameInput& frame, const Selection& sel,
+                                                 bool per_model_quota_before_overlap) {
+  if (!n_models) return {};
+  // every slot of this call is reused: let earlier work on both streams end
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipStreamSynchronize(stream2_), "sync");
+  const bool several = n_models > 1;
+  bool distinct = false;
+  for (size_t i = 0; i < n_models; ++i) distinct |= models[i] != models[0];
+  // ---- one query per distinct (model, options); aliases share its candidates (matcher.cpp:1087-1098)
+  std::vector<size_t> owner(n_models), slot_of(n_models, size_t(-1));
+  size_t n_queries = 0;
+  for (size_t i = 0; i < n_models; ++i) {
+    owner[i] = i;
+    for (size_t j = 0; j < i; ++j)
+      if (models[i] == models[j] && same_request(opts[i], opts[j])) { owner[i] = owner[j]; break; }
+    if (owner[i] == i) slot_of[i] = n_queries++;
+  }
+  while (multi_slots_.size() < n_queries) {
+    multi_slots_.push_back(std::make_unique<Slot>());
+    check(hipEventCreateWithFlags(&multi_slots_.back()->done, hipEventDisableTiming), "hipEventCreate");
+  }
+  // ---- the device-resident parts of every query, back to back on one stream
+  for (size_t i = 0; i < n_models; ++i) {
+    if (owner[i] != i) continue;
+    FindOptions o = opts[i];
+    o.query_cache = true;
+    o.whole_level0 = several;
+    o.profile = 0;
+    o.trace_stages = false;
+    Slot& S = *multi_slots_[slot_of[i]];
+    cur_ = &S;
+    enqueue(S, *models[i], &frame, 1, o, nullptr);
+  }
+  std::vector<ModelMatch> cand;
+  struct CandRef { size_t slot, index; };
+  std::vector<CandRef> where;
+  for (size_t i = 0; i < n_models; ++i) {
+    const size_t k = slot_of[owner[i]];
+    Slot& S = *multi_slots_[k];
+    if (owner[i] == i) {
+      if (S.empty) continue;
+      collect(S);
+      // default background guard (matcher.cpp:839-846), for an AVX2 host
+      // rounding to nearest; the whole level 0 is converted when the query
+      // kept candidates, which it did when there are 32 of them
+      const NccModel& m = *models[i];
+      const FindOptions& o = S.o;  // the angle range clipped to the model's
+      if (!S.finest && distinct && sel.num_matches == 0 && S.jobs[0].full.size() >= 32 && o.subpixel &&
+          !o.num_matches && o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
+          m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several) {
+        S.rp.guard_first = o.min_score >= .8 ? .1 : .2;
+        S.rp.guard = .1;
+      }
+    }
+    if (S.empty) continue;
+    const std::vector<Pose>& full = S.jobs[0].full;
+    for (size_t j = 0; j < full.size(); ++j)
+      if (std::isfinite(full[j].score)) { cand.push_back({full[j], i}); where.push_back({k, j}); }
+  }
+  // ---- select_refined (multi.hpp:83-119) with speculative refinement rounds
+  std::vector<size_t> order(cand.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return multi_less(cand[a], cand[b]); });
+  auto geom = [&](size_t model) -> const LevelGeom& { return models[model]->levels.front(); };
+  // refined pose and verdict per (query, candidate): aliases share them
+  std::vector<std::vector<Pose>> refined(n_queries);
+  std::vector<std::vector<int>> verdict(n_queries);
+  for (size_t i = 0; i < n_models; ++i)
+    if (owner[i] == i && !multi_slots_[slot_of[i]]->empty) {
+      refined[slot_of[i]].assign(multi_slots_[slot_of[i]]->jobs[0].full.size(), Pose{});
+      verdict[slot_of[i]].assign(multi_slots_[slot_of[i]]->jobs[0].full.size(), -1);
+    }
+  auto select = [&](const std::vector<size_t>& ord, const std::vector<ModelMatch>& pool, const Selection& s,
+                    bool with_refine) {
+    std::vector<ModelMatch> result;
+    std::vector<Box> boxes;
+    std::vector<Bounds> encl;
+    std::vector<size_t> counts(n_models, 0);
+    auto quota_full = [&](size_t model) {
+      const int64_t limit = s.num_matches >= 0 ? s.num_matches : int64_t(opts[model].num_matches);
+      return s.num_matches < 0 && limit && counts[model] >= uint64_t(limit);
+    };
+    auto suppressed = [&](const ModelMatch& c) {
+      const double threshold = s.max_overlap >= 0 ? s.max_overlap : opts[c.model].max_overlap;
+      const Box rect = pose_box(geom(c.model), c.pose);
+      const Bounds aabb = enclosure(rect);
+      for (size_t i = 0; i < result.size(); ++i) {
+        if (s.max_overlap < 0 && result[i].model != c.model) continue;
+        if (disjoint(aabb, encl[i])) continue;
+        if (box_overlap(c.pose, geom(c.model), result[i].pose, geom(result[i].model)) > threshold + 1e-12) return true;
+      }
+      return false;
+    };
+    size_t batch = s.num_matches > 0 ? std::max<size_t>(2 * size_t(s.num_matches), 4) : 0;
+    if (!batch) {
+      size_t total = 0;
+      bool bounded = true;
+      for (size_t i = 0; i < n_models; ++i) { if (!opts[i].num_matches) bounded = false; total += opts[i].num_matches; }
+      batch = bounded ? std::max<size_t>(2 * total, 4) : ord.size();
+    }
+    size_t pos = 0;
+    bool stop = false;
+    while (!stop && pos < ord.size()) {
+      // walk while every needed refinement is known
+      while (pos < ord.size()) {
+        const ModelMatch& c = pool[ord[pos]];
+        if (quota_full(c.model) || suppressed(c)) { ++pos; continue; }
+        ModelMatch r = c;
+        bool ok = true;
+        if (with_refine) {
+          const CandRef& w = where[ord[pos]];
+          if (verdict[w.slot][w.index] < 0) break;
+          ok = verdict[w.slot][w.index] != 0;
+          r.pose = refined[w.slot][w.index];
+        }
+        ++pos;
+        if (!ok || suppressed(r)) continue;
+        const Box rect = pose_box(geom(r.model), r.pose);
+        result.push_back(r); boxes.push_back(rect); encl.push_back(enclosure(rect)); ++counts[r.model];
+        if (s.num_matches > 0 && result.size() >= uint64_t(s.num_matches)) { stop = true; break; }
+      }
+      if (stop || pos >= ord.size()) break;
+      // the next candidates the walk may need, as far as the current results tell
+      std::vector<RefineRequest> reqs;
+      std::vector<CandRef> targets;
+      for (size_t i = pos; i < ord.size() && reqs.size() < batch; ++i) {
+        const ModelMatch& c = pool[ord[i]];
+        const CandRef& w = where[ord[i]];
+        if (verdict[w.slot][w.index] != -1 || quota_full(c.model) || suppressed(c)) continue;
+        verdict[w.slot][w.index] = -2;  // requested in this round (aliases share the request)
+        reqs.push_back({multi_slots_[w.slot].get(), 0, c.pose, {}, 0});
+        targets.push_back(w);
+      }
+      refine_round(reqs);
+      for (size_t k = 0; k < reqs.size(); ++k) {
+        refined[targets[k].slot][targets[k].index] = reqs[k].out;
+        verdict[targets[k].slot][targets[k].index] = reqs[k].verdict;
+      }
+      batch *= 2;
+    }
+    std::sort(result.begin(), result.end(), multi_less);
+    return result;
+  };
+  bool any_quota = false;
+  for (size_t i = 0; i < n_models; ++i) any_quota |= opts[i].num_matches > 0;
+  // HALCON's two-stage quota: per-model quotas and overlaps first, then the
+  // cross-model overlap without refilling quotas (matcher.cpp:1108-1112)
+  const bool independent = per_model_quota_before_overlap && sel.num_matches < 0 && sel.max_overlap >= 0 && any_quota;
+  auto result = select(order, cand, independent ? Selection{-1, -1} : sel, true);
+  if (independent) {
+    std::vector<size_t> ord2(result.size());
+    for (size_t i = 0; i < ord2.size(); ++i) ord2[i] = i;
+    result = select(ord2, result, sel, false);
+  }
+  return result;
 }
 
 }  // namespace mvamd

@@ -67,7 +67,19 @@ struct FindOptions {
   double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN)
   int profile = 0;                 // bit s: time stage s (KernelStage) with HIP events on the context's stream
   bool trace_stages = false;       // fill FindTrace's stage lists (coarse, tracked, storage)
+  // find_models' query cache (matcher.cpp:281-301): a candidate below the
+  // minimum score is recovered before its fit
+  bool query_cache = false;
+  // find_models over several models: the whole level 0 is the storage region
+  // and the recovery windows use its row prefix statistics (matcher.cpp:596-611, 991-1003)
+  bool whole_level0 = false;
 };
+
+// Quota and overlap shared by all models of find_models (multi.hpp:14);
+// negative fields mean "per model".
+struct Selection { int64_t num_matches = -1; double max_overlap = -1; };
+// A find_models result: the pose and the index of its model in the call.
+struct ModelMatch { Pose pose; size_t model = 0; };
 
 // Kernel stages timed in profiling mode.
 enum KernelStage { kStagePyramid = 0, kStagePrefix, kStageCoarse, kStagePeaks, kStageTrack, kStageFull, kStageRefine,
@@ -123,19 +135,43 @@ class FindContext {
   uint64_t submit_batch(const NccModel& model, const FrameInput* frames, size_t n_frames, const FindOptions& options,
                         FindTrace* traces = nullptr);
   std::vector<std::vector<Pose>> complete_batch(uint64_t ticket);
-  // Same, from host memory (copied to the device first).
+  // Same as find, from host memory (copied to the device first).
   std::vector<Pose> find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
                               size_t pitch, const FindOptions& options, FindTrace* trace = nullptr);
+  // find_models (matcher.cpp:1077-1124) over one u8 frame in device memory:
+  // one query per distinct (model, options), the candidates of all queries
+  // selected together in (score, model, row, col, angle) order with the
+  // selection's quota/overlap rules, refined on the device.  Each distinct
+  // query keeps a workspace of its own in the context.
+  std::vector<ModelMatch> find_models(const NccModel* const* models, const FindOptions* options, size_t n_models,
+                                      const FrameInput& frame, const Selection& selection,
+                                      bool per_model_quota_before_overlap = true);
+  // Same, from host memory.
+  std::vector<ModelMatch> find_models_host(const NccModel* const* models, const FindOptions* options,
+                                           size_t n_models, const uint8_t* image, size_t width, size_t height,
+                                           size_t pitch, const Selection& selection,
+                                           bool per_model_quota_before_overlap = true);
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
 
  private:
   struct Slot;  // state of a submitted batch
+  struct RefineRequest {
+    Slot* slot;      // the query that refines it
+    uint32_t frame;  // frame of the slot's batch
+    Pose in, out;
+    int verdict;
+  };
+  void enqueue(Slot& S, const NccModel& model, const FrameInput* frames, size_t n_frames, const FindOptions& options,
+               FindTrace* traces);
+  void collect(Slot& S);
+  void refine_round(std::vector<RefineRequest>& reqs);
   void time_begin(int stage, hipStream_t stream);
   void time_end(hipStream_t stream);
   void time_collect(FindTrace* trace);
   std::unique_ptr<Slot> slots_[2];
+  std::vector<std::unique_ptr<Slot>> multi_slots_;  // one per query of find_models
   Slot* cur_ = nullptr;  // the slot being submitted or completed
   uint64_t next_ticket_ = 0;
   hipStream_t stream_ = nullptr;   // device-resident part
@@ -146,5 +182,8 @@ class FindContext {
 
 // Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).
 double pose_overlap(const Pose& a, const Pose& b, const LevelGeom& level);
+// find_models' overlap of two template rectangles: intersection over the
+// smaller area (multi.hpp:64-81).
+double box_overlap(const Pose& a, const LevelGeom& ga, const Pose& b, const LevelGeom& gb);
 
 }  // namespace mvamd

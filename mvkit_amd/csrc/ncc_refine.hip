@@ -471,6 +471,8 @@ struct RefineArgs {
   int n_refs0;
   double step, a_start, a_end, min_score;
   int subpixel, rot, absolute;
+  int recover_first, prefix_recovery;
+  double guard_first, guard;
   const NccMatch* cand;
   int n_cand;
   // list form (full score): frame f's candidates at cand + f * cap
@@ -784,7 +786,10 @@ __device__ __forceinline__ bool refinement_step(const RefineArgs& a, const Patch
 
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
-// fit(): matcher.cpp:903-982 without the multi-model memo or guard.
+// fit(): matcher.cpp:903-982 (the multi-model memo tables only reuse values).
+// With the background guard a fit whose score stays far below min_score after
+// iteration 0 or 1 stops; those iterations then score their proposals
+// without the next derivatives (matcher.cpp:955-978).
 template <class T>
 __device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
   if (!a.subpixel) return;
@@ -806,7 +811,9 @@ __device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccM
       im.angle = clampd(im.angle + st * delta[2], a.a_start, a.a_end);
       if (!domain_contains(a, im.row, im.col)) continue;
       bool next_valid = false;
-      if (bt == 0 && it + 1 < 6) {
+      const bool guard_score_only = (it == 0 && a.guard_first >= 0 && c.score < a.min_score - a.guard_first) ||
+                                    (it == 1 && a.guard >= 0 && c.score < a.min_score - a.guard);
+      if (bt == 0 && it + 1 < 6 && !guard_score_only) {
         im.score = NCC_MISSING;
         const double floor_score = c.score;
         next_valid = refinement_step<T>(a, O, im.row, im.col, im.angle, next_delta, &im.score, &floor_score, sm);
@@ -816,6 +823,8 @@ __device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccM
       }
       if (im.score > c.score) { c = im; accepted = true; cached = next_valid; break; }
     }
+    if (it == 1 && a.guard >= 0 && c.score < a.min_score - a.guard) break;
+    if (it == 0 && a.guard_first >= 0 && c.score < a.min_score - a.guard_first) break;
     if (!accepted || (hypot(delta[0], delta[1]) < 1e-4 && fabs(delta[2]) < 1e-5)) break;
   }
 }
@@ -880,6 +889,44 @@ __device__ __forceinline__ double direct_window_wave(const RefineArgs& a, const 
   return a.absolute ? fabs(sc) : sc;
 }
 
+// Window NCC in the row-prefix form on an exact byte lattice
+// (correlation_four_prefix_avx2 / correlation_one_prefix_avx2,
+// kernels_avx2.cpp:118-171): moments of the dc-relative pixels without a
+// per-window offset (exact integers in FP64, so their order does not
+// matter), var = sq - sum^2/n, -inf when var <= 128 eps sq, numerator
+// D - (sum/n) * weight_sum.  Used by find_models' recovery over the whole level 0.
+template <class T>
+__device__ __forceinline__ double prefix_window_wave(const RefineArgs& a, const PatchRef& p, int row, int col) {
+  const NccPatch& P = p.P;
+  if (row + P.rmin < 0 || col + P.cmin < 0 || row + P.rmax >= a.H0 || col + P.cmax >= a.W0) return NCC_MISSING;
+  if (row + P.rmin < a.sr || col + P.cmin < a.sc || row + P.rmax >= a.sr + a.sh || col + P.cmax >= a.sc + a.sw)
+    return NCC_MISSING;
+  const int lr = row - a.sr, lc = col - a.sc;
+  double s = 0, sq = 0, d = 0;
+  constexpr int U = 8;
+  for (int base = threadIdx.x & 63; base < P.n; base += 64 * U) {
+    int32_t rcv[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + 64 * u;
+      rcv[u] = i < P.n ? p.rc[i] : 0;
+      wv[u] = i < P.n ? p.w[i] : 0.0;
+    }
+    double qv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) qv[u] = base + 64 * u < P.n ? pix<T>(a, lr + sample_row(rcv[u]), lc + sample_col(rcv[u])) : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + 64 * u < P.n) { s += qv[u]; sq += qv[u] * qv[u]; d += qv[u] * wv[u]; }
+  }
+  s = wave_sum(s); sq = wave_sum(sq); d = wave_sum(d);
+  const double n = double(P.n), var = sq - s * s / n;
+  if (var <= 128 * 0x1p-52 * sq) return NCC_MISSING;
+  const double sc = clamp1((d - s / n * P.wsum) / sqrt(var));
+  return a.absolute ? fabs(sc) : sc;
+}
+
 // Recovery search (matcher.cpp:986-1059): the 5x5 level-0 anchors around the
 // seed for every level-0 patch within one angle step.  Windows are spread over
 // the waves; the first strict maximum above `floor_score` in the reference's
@@ -896,7 +943,7 @@ __device__ __forceinline__ bool recovery_search(const RefineArgs& a, const NccMa
     const int r = int(round(seed.row - a.crop_r - p.P.orow)) + (k % 25) / 5 - 2;
     const int c = int(round(seed.col - a.crop_c - p.P.ocol)) + k % 5 - 2;
     if (!domain_contains(a, a.crop_r + r + p.P.orow, a.crop_c + c + p.P.ocol)) continue;
-    const double sc = direct_window_wave<T>(a, p, r, c);
+    const double sc = a.prefix_recovery ? prefix_window_wave<T>(a, p, r, c) : direct_window_wave<T>(a, p, r, c);
     if (sc > wb) { wb = sc; wi = k; }
   }
   if (lane == 0) { sm.win_s[wave] = wb; sm.win_i[wave] = wi; }
@@ -980,7 +1027,9 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
     fit_coarse<T>(a, O, cd, sm);
   } else {
     const NccMatch seed = cd;
-    fit<T>(a, O, cd, sm);
+    // find_models recovers a candidate already below min_score before fitting it (matcher.cpp:984-985)
+    const bool recover_first = a.recover_first && a.subpixel && a.top > 0 && cd.score < a.min_score;
+    if (!recover_first) fit<T>(a, O, cd, sm);
     if (a.subpixel && a.top > 0 && cd.score < a.min_score) {
       NccMatch best = cd;
       if (recovery_search<T>(a, seed, cd.score, best, elig, wave_cnt, sm) &&
@@ -988,6 +1037,8 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
         best.score = continuous<T>(a, O, best.row, best.col, best.angle, sm);
         fit<T>(a, O, best, sm);
         if (best.score > cd.score) cd = best;
+      } else if (recover_first) {
+        fit<T>(a, O, cd, sm);
       }
     }
   }
@@ -1040,6 +1091,8 @@ RefineArgs refine_args(const NcchipRefineParams* p, const NccPatchPool* pools) {
   a.lv_orow = p->level_orow; a.lv_ocol = p->level_ocol; a.refs0 = p->refs0; a.n_refs0 = p->n_refs0;
   a.step = p->step; a.a_start = p->angle_start; a.a_end = p->angle_end; a.min_score = p->min_score;
   a.subpixel = p->subpixel; a.rot = p->rotate; a.absolute = p->absolute;
+  a.recover_first = p->recover_first; a.prefix_recovery = p->prefix_recovery;
+  a.guard_first = p->guard_first; a.guard = p->guard;
   return a;
 }
 }  // namespace

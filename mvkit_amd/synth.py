@@ -222,3 +222,48 @@ def small_rot(width=256, height=192, t=32, count=2, extent_deg=20.0, seed_k=10, 
     e = math.radians(extent_deg)
     return scene(width, height, t, t, count, -e, e, noise, True, SEED_BASE + seed_k + 1000 * frame,
                  min_separation=1.3 * math.hypot(t, t), template_seed=SEED_BASE + seed_k)
+
+
+@dataclass
+class MultiScene:
+    image: np.ndarray                                # u8 search image
+    templates: list = field(default_factory=list)    # u8 templates
+    truth: list = field(default_factory=list)        # [(template index, row, col, angle)]
+    seed: int = 0
+
+
+def multi_scene(width: int, height: int, sizes: list, counts: list, angle_lo: float, angle_hi: float, noise: float,
+                seed: int, gain: bool = False) -> MultiScene:
+    """Several templates (sizes[k] = (h, w), template seed derived from the
+    scene seed and k) pasted counts[k] times each at subpixel centres with
+    rotation in [angle_lo, angle_hi]; placements keep 1.2 x the larger
+    diagonal between centres."""
+    rng = SplitMix64(seed)
+    base = blurred_noise(rng, height, width).astype(np.float64)
+    tmpls = [make_template(SplitMix64((seed + 7919 * (k + 1)) ^ 0x5EED5EED), h, w) for k, (h, w) in enumerate(sizes)]
+    truth = []
+    for k, (th, tw) in enumerate(sizes):
+        diag = math.hypot(th, tw)
+        margin = diag / 2 + 2
+        placed, tries = 0, 0
+        while placed < counts[k] and tries < 10000:
+            tries += 1
+            cy = margin + rng.uniform1() * (height - 1 - 2 * margin)
+            cx = margin + rng.uniform1() * (width - 1 - 2 * margin)
+            ang = angle_lo + rng.uniform1() * (angle_hi - angle_lo)
+            if any(math.hypot(cy - r, cx - c) < 0.6 * (diag + math.hypot(*sizes[j]))
+                   for j, r, c, _ in truth):
+                continue
+            truth.append((k, cy, cx, ang))
+            placed += 1
+    for k, cy, cx, ang in truth:
+        paste(base, tmpls[k], cy, cx, ang)
+    img = _finish(base, rng, noise, gain)
+    return MultiScene(img, tmpls, truth, seed)
+
+
+def small_multi(seed_k: int = 20) -> MultiScene:
+    """Two templates (40x40, 32x48) pasted twice each into 320x256, +-15
+    degrees, N(0,4): oracle-speed find_models cases."""
+    e = math.radians(15)
+    return multi_scene(320, 256, [(40, 40), (32, 48)], [2, 2], -e, e, 4.0, SEED_BASE + seed_k)

@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -392,6 +393,89 @@ static void direct_four(const double* img, size_t w, const Patch& p, int row, in
   }
 }
 
+// correlation_one_prefix_avx2 (kernels_avx2.cpp:118-144) on an exact byte
+// lattice: moments from row prefix sums over the runs, the dot product from
+// the weight runs when 8 * #weight_runs < n, else four lanes per run (lane =
+// position within the run mod 4) plus a scalar tail, combined (l0+l1)+(l2+l3).
+static double one_prefix(const double* img, const Prefix& P, size_t w, const Patch& p, int row, int col, bool absolute) {
+  const size_t pitch = w + 1;
+  double sum = 0, sq = 0;
+  for (const auto& run : p.runs) {
+    const size_t a = size_t(row + run.r) * pitch + size_t(col + run.b), b = size_t(row + run.r) * pitch + size_t(col + run.e + 1);
+    sum += P.sum[b] - P.sum[a]; sq += P.sq[b] - P.sq[a];
+  }
+  const double count = double(p.s.size()), var = sq - sum * sum / count;
+  if (var <= 128 * kEps * sq)
+    return w <= 32000000 && p.s.size() <= 1048576 && !p.runs.empty() ? kMissing
+                                                                     : correlation_gather(img, w, p, row, col, absolute);
+  if (!p.wruns.empty() && p.wruns.size() * 8 < p.s.size()) {
+    double dot = 0;
+    for (const auto& run : p.wruns) {
+      const size_t a = size_t(row + run.r) * pitch + size_t(col + run.b), b = size_t(row + run.r) * pitch + size_t(col + run.e + 1);
+      dot += (P.sum[b] - P.sum[a]) * run.w;
+    }
+    return polar(clamp1((dot - sum / count * p.wsum) / std::sqrt(var)), absolute);
+  }
+  double lane[4] = {0, 0, 0, 0}, tail = 0;
+  size_t index = 0;
+  for (const auto& run : p.runs) {
+    const double* v = img + size_t(row + run.r) * w + size_t(col + run.b);
+    const size_t len = size_t(run.e - run.b + 1);
+    size_t i = 0;
+    for (; i + 4 <= len; i += 4)
+      for (int l = 0; l < 4; ++l) lane[l] += v[i + l] * p.s[index + i + l].w;
+    for (; i < len; ++i) tail += v[i] * p.s[index + i].w;
+    index += len;
+  }
+  const double num = (lane[0] + lane[1]) + (lane[2] + lane[3]) + tail - sum / count * p.wsum;
+  return polar(clamp1(num / std::sqrt(var)), absolute);
+}
+
+// correlation_four_prefix_avx2 (kernels_avx2.cpp:145-171): four neighbouring
+// windows (col .. col+3).  Dot products from the weight runs when
+// 2 * #weight_runs < n, else four accumulators by sample index mod 4 combined
+// (d0+d1)+(d2+d3) before the tail; low-variance windows are -inf on a byte lattice.
+static void four_prefix(const double* img, const Prefix& P, size_t w, const Patch& p, int row, int col, bool absolute,
+                        double* out) {
+  const size_t pitch = w + 1, n = p.s.size();
+  double sum[4] = {}, sq[4] = {}, d0[4] = {}, d1[4] = {}, d2[4] = {}, d3[4] = {};
+  for (const auto& run : p.runs) {
+    const size_t a = size_t(row + run.r) * pitch + size_t(col + run.b), b = size_t(row + run.r) * pitch + size_t(col + run.e + 1);
+    for (int l = 0; l < 4; ++l) { sum[l] += P.sum[b + l] - P.sum[a + l]; sq[l] += P.sq[b + l] - P.sq[a + l]; }
+  }
+  size_t i = 0;
+  if (!p.wruns.empty() && p.wruns.size() * 2 < n) {
+    for (const auto& run : p.wruns) {
+      const size_t a = size_t(row + run.r) * pitch + size_t(col + run.b), b = size_t(row + run.r) * pitch + size_t(col + run.e + 1);
+      for (int l = 0; l < 4; ++l) d0[l] += (P.sum[b + l] - P.sum[a + l]) * run.w;
+    }
+    i = n;
+  }
+  auto px = [&](size_t k, int l) { const auto& x = p.s[k]; return img[size_t(row + x.r) * w + size_t(col + x.c + l)]; };
+  for (; i + 4 <= n; i += 4)
+    for (int l = 0; l < 4; ++l) {
+      d0[l] += px(i, l) * p.s[i].w; d1[l] += px(i + 1, l) * p.s[i + 1].w;
+      d2[l] += px(i + 2, l) * p.s[i + 2].w; d3[l] += px(i + 3, l) * p.s[i + 3].w;
+    }
+  double dot[4], var[4];
+  for (int l = 0; l < 4; ++l) dot[l] = (d0[l] + d1[l]) + (d2[l] + d3[l]);
+  for (; i < n; ++i)
+    for (int l = 0; l < 4; ++l) dot[l] += px(i, l) * p.s[i].w;
+  const double count = double(n);
+  for (int l = 0; l < 4; ++l) {
+    var[l] = sq[l] - sum[l] * sum[l] / count;
+    dot[l] = dot[l] - sum[l] / count * p.wsum;
+  }
+  for (int l = 0; l < 4; ++l) {
+    if (var[l] <= 128 * kEps * sq[l]) {
+      out[l] = w <= 32000000 && n <= 1048576 && !p.runs.empty() ? kMissing
+                                                                 : correlation_gather(img, w, p, row, col + l, absolute);
+      continue;
+    }
+    out[l] = polar(clamp1(dot[l] / std::sqrt(var[l])), absolute);
+  }
+}
+
 // ---------------------------------------------------- continuous (bilinear)
 // strictly_inside: kernels_avx2.cpp:261-277
 static bool strictly_inside(size_t w, size_t h, const Level& lv, const Patch& p, const Match& pose, double ca, double sa) {
@@ -612,6 +696,13 @@ struct Query {
   long crop_r = 0, crop_c = 0;
   size_t img_w = 0, img_h = 0;
   bool empty = false;
+  // find_models' query cache (matcher.cpp:281-301, 1078-1081): `cache` is set
+  // for every query of find_models; `cache_samples` when it holds more than one
+  // model, which converts the whole level 0 to doubles (matcher.cpp:596-611).
+  bool cache = false, cache_samples = false;
+  bool shared_full = false;                 // level 0 is the whole crop (shared_values)
+  double guard_first = -1, guard = -1;      // background guard thresholds (matcher.cpp:912-927)
+  Prefix fine_prefix;                       // row prefix statistics of the whole level 0 (built on first use)
 };
 
 static bool domain_contains(const Query& q, const Match& m) {  // matcher.cpp:249-251 (full domain)
@@ -774,6 +865,17 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
       }
   }
   if (tr) tr->coarse = cand;
+  // find_models over several models: the whole level 0 as doubles (matcher.cpp:596-611)
+  if (!finest && Q.cache_samples && pyr.front().bytes && !cand.empty()) {
+    Gray& f = pyr.front();
+    f.sr = 0; f.sc = 0; f.sh = f.h; f.sw = f.w;
+    f.v.resize(f.sw * f.sh);
+    for (size_t r = 0; r < f.sh; ++r)
+      for (size_t c = 0; c < f.sw; ++c) f.v[r * f.sw + c] = double(f.bytes[r * f.stride + c]) - f.dc;
+    f.bytes = nullptr;
+    Q.shared_full = true;
+    if (tr) { tr->store_r = 0; tr->store_c = 0; tr->store_h = long(f.sh); tr->store_w = long(f.sw); }
+  }
   // level-0 candidate region to doubles: matcher.cpp:612-629
   if (!finest && pyr.front().bytes && !cand.empty()) {
     Gray& f = pyr.front();
@@ -901,7 +1003,15 @@ static bool refine_coarse(Query& Q, Match& cd) {
   return cd.score >= o.min_score && domain_contains(Q, cd);
 }
 
-// refine_candidate: matcher.cpp:878-1061 (single find: no cache, no guard)
+// refine_candidate: matcher.cpp:878-1061.  Without the query cache (find):
+// fit, then the recovery search when the fit stays below min_score.  With it
+// (find_models): a candidate already below min_score is recovered first
+// (matcher.cpp:984-985, 1052-1055), the fit may stop early under the
+// background guard (matcher.cpp:912-927, 958-960, 977-978), and over several
+// models the recovery windows use row prefix statistics of the whole level 0
+// (matcher.cpp:991-1003, 1018-1045).  The memo tables of the cache
+// (fitted poses, score planes, derivatives) only reuse values and are not
+// restated.
 static bool refine(Query& Q, Match& cd) {
   if (Q.finest) return refine_coarse(Q, cd);
   const Model& model = *Q.model; const FindOpts& o = Q.opt;
@@ -928,20 +1038,28 @@ static bool refine(Query& Q, Match& cd) {
           im.angle = std::clamp(im.angle + st * delta[2], o.start, angle_end);
           if (!domain_contains(Q, im)) continue;
           bool next_valid = false;
-          if (bt == 0 && it + 1 < 6) {
+          const bool guard_score_only = (it == 0 && Q.guard_first >= 0 && c.score < o.min_score - Q.guard_first) ||
+                                        (it == 1 && Q.guard >= 0 && c.score < o.min_score - Q.guard);
+          if (bt == 0 && it + 1 < 6 && !guard_score_only) {
             Match nl = im; nl.row -= crop_r; nl.col -= crop_c; im.score = kMissing;
             next_valid = refinement_step(fine, lv0, orig, nl, absolute, o.extent > 0, next_delta, &im.score, &c.score);
             if (!std::isfinite(im.score)) im.score = full_score(im);
           } else im.score = full_score(im);
           if (im.score > c.score) { c = im; accepted = true; cached = next_valid; break; }
         }
+        if (it == 1 && Q.guard >= 0 && c.score < o.min_score - Q.guard) break;
+        if (it == 0 && Q.guard_first >= 0 && c.score < o.min_score - Q.guard_first) break;
         if (!accepted || (std::hypot(delta[0], delta[1]) < 1e-4 && std::abs(delta[2]) < 1e-5)) break;
       }
   };
   const Match seed = cd;
-  fit(cd);
+  const bool recover_first = Q.cache && o.subpixel && top > 0 && cd.score < o.min_score;
+  if (!recover_first) fit(cd);
   if (o.subpixel && top > 0 && cd.score < o.min_score) {
-    // recovery 5x5 level-0 search: matcher.cpp:986-1059 (no prefix statistics)
+    // recovery 5x5 level-0 search: matcher.cpp:986-1059
+    const bool stats = Q.cache && !fine.bytes && Q.shared_full && fine.sw == fine.w && fine.sh == fine.h;
+    if (stats && Q.fine_prefix.sum.empty()) Q.fine_prefix = build_prefix(fine);
+    const Prefix& pf = Q.fine_prefix;
     Match best = cd;
     for (const Patch* pp : Q.qpatches.front()) {
       const Patch& p = *pp;
@@ -952,7 +1070,8 @@ static bool refine(Query& Q, Match& cd) {
         if (r + dr + p.rmin >= int(fine.sr) && r + dr + p.rmax < int(fine.sr + fine.sh) && c - 2 + p.cmin >= int(fine.sc) &&
             c + 1 + p.cmax < int(fine.sc + fine.sw)) {
           double vals[4];
-          direct_four(fine.data(), fine.sw, p, r + dr - int(fine.sr), c - 2 - int(fine.sc), absolute, vals);
+          if (stats) four_prefix(fine.data(), pf, fine.sw, p, r + dr - int(fine.sr), c - 2 - int(fine.sc), absolute, vals);
+          else direct_four(fine.data(), fine.sw, p, r + dr - int(fine.sr), c - 2 - int(fine.sc), absolute, vals);
           for (int ln = 0; ln < 4; ++ln) {
             Match ps{crop_r + r + dr + p.orow, crop_c + c - 2 + ln + p.ocol, p.angle, vals[ln]};
             if (domain_contains(Q, ps) && ps.score > best.score) best = ps;
@@ -962,7 +1081,11 @@ static bool refine(Query& Q, Match& cd) {
         for (; dc <= 2; ++dc) {
           Match ps{crop_r + r + dr + p.orow, crop_c + c + dc + p.ocol, p.angle, 0};
           if (!domain_contains(Q, ps)) continue;
-          ps.score = discrete(fine, p, r + dr, c + dc, absolute, true);
+          if (stats && r + dr + p.rmin >= int(fine.sr) && r + dr + p.rmax < int(fine.sr + fine.sh) &&
+              c + dc + p.cmin >= int(fine.sc) && c + dc + p.cmax < int(fine.sc + fine.sw))
+            ps.score = one_prefix(fine.data(), pf, fine.sw, p, r + dr - int(fine.sr), c + dc - int(fine.sc), absolute);
+          else
+            ps.score = discrete(fine, p, r + dr, c + dc, absolute, true);
           if (ps.score > best.score) best = ps;
         }
       }
@@ -970,7 +1093,7 @@ static bool refine(Query& Q, Match& cd) {
     if (best.row != cd.row || best.col != cd.col || best.angle != cd.angle) {
       best.score = full_score(best); fit(best);
       if (best.score > cd.score) cd = best;
-    }
+    } else if (recover_first) fit(cd);
   }
   return cd.score >= o.min_score && domain_contains(Q, cd);
 }
@@ -1011,6 +1134,158 @@ static bool refine_one(const uint8_t* img, size_t W, size_t H, size_t stride, co
                        Match& cd) {
   Query Q; prepare(Q, img, W, H, stride, model, o, nullptr);
   return refine(Q, cd);
+}
+
+// ------------------------------------------------------------ find_models
+// src/matching/multi.hpp:14-119 and matcher.cpp:1077-1124.
+struct Selection { int64_t num_matches = -1; double max_overlap = -1; };
+struct MultiMatch { Match pose; size_t model = 0; };
+struct MPoint { double x, y; };
+using MBox = std::array<MPoint, 4>;
+struct MBounds { double top, bottom, left, right; };
+
+static MBounds enclosure(const MBox& b) {  // multi.hpp:19-23
+  MBounds out{b[0].y, b[0].y, b[0].x, b[0].x};
+  for (const auto& p : b) {
+    out.top = std::min(out.top, p.y); out.bottom = std::max(out.bottom, p.y);
+    out.left = std::min(out.left, p.x); out.right = std::max(out.right, p.x);
+  }
+  return out;
+}
+static bool disjoint(const MBounds& a, const MBounds& b) {  // multi.hpp:24-26
+  return a.right < b.left || b.right < a.left || a.bottom < b.top || b.bottom < a.top;
+}
+// Template rectangle [-o-1/2, w-o-1/2] at a pose: multi.hpp:54-60 (scale 1).
+static MBox pose_box(const Level& lv, const Match& p) {
+  const MBounds b{-lv.orow - .5, double(lv.h) - lv.orow - .5, -lv.ocol - .5, double(lv.w) - lv.ocol - .5};
+  const double ca = std::cos(p.angle), sa = std::sin(p.angle);
+  MBox q{{{b.left, b.top}, {b.right, b.top}, {b.right, b.bottom}, {b.left, b.bottom}}};
+  for (auto& v : q) { const double x = v.x, y = v.y; v = {p.col + ca * x + sa * y, p.row - sa * x + ca * y}; }
+  return q;
+}
+static double poly_area(const MPoint* p, size_t n) {  // multi.hpp:61-63
+  double a = 0;
+  for (size_t i = 0; i < n; ++i) { const auto& u = p[i]; const auto& v = p[(i + 1) % n]; a += u.x * v.y - v.x * u.y; }
+  return std::abs(a) * .5;
+}
+// Intersection over the smaller area, after translating both boxes by a's
+// first corner: multi.hpp:64-81.
+static double box_overlap(MBox a, MBox b) {
+  const MPoint origin = a[0];
+  for (auto& p : a) { p.x -= origin.x; p.y -= origin.y; }
+  for (auto& p : b) { p.x -= origin.x; p.y -= origin.y; }
+  const double den = std::min(poly_area(a.data(), 4), poly_area(b.data(), 4));
+  if (den <= 1e-12) return 0;
+  std::array<MPoint, 16> poly{}, next{};
+  std::copy(a.begin(), a.end(), poly.begin());
+  size_t count = 4;
+  auto cross = [](MPoint u, MPoint v, MPoint p) { return (v.x - u.x) * (p.y - u.y) - (v.y - u.y) * (p.x - u.x); };
+  for (int e = 0; e < 4 && count; ++e) {
+    const MPoint u = b[size_t(e)], v = b[size_t((e + 1) % 4)];
+    size_t m = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const MPoint p = poly[i], q = poly[(i + 1) % count];
+      const double dp = cross(u, v, p), dq = cross(u, v, q);
+      if (dp >= 0) next[m++] = p;
+      if ((dp >= 0) != (dq >= 0)) { const double t = dp / (dp - dq); next[m++] = {p.x + t * (q.x - p.x), p.y + t * (q.y - p.y)}; }
+    }
+    poly = next; count = m;
+  }
+  return std::clamp(poly_area(poly.data(), count) / den, 0., 1.);
+}
+static bool multi_less(const MultiMatch& a, const MultiMatch& b) {  // multi.hpp:87-93
+  if (a.pose.score != b.pose.score) return a.pose.score > b.pose.score;
+  if (a.model != b.model) return a.model < b.model;
+  if (a.pose.row != b.pose.row) return a.pose.row < b.pose.row;
+  if (a.pose.col != b.pose.col) return a.pose.col < b.pose.col;
+  return a.pose.angle < b.pose.angle;
+}
+// select_refined: multi.hpp:83-119.  Overlap is tested before and after the
+// refinement; a suppressed candidate does not use up its model's quota.
+template <class Refine>
+static std::vector<MultiMatch> select_refined(std::vector<MultiMatch> cand, const std::vector<FindOpts>& opts,
+                                              const std::vector<const Model*>& models, Selection sel, Refine refine) {
+  std::sort(cand.begin(), cand.end(), multi_less);
+  std::vector<MultiMatch> result; std::vector<MBox> boxes; std::vector<MBounds> encl;
+  std::vector<size_t> counts(opts.size());
+  auto suppressed = [&](size_t index, const MBox& rect, const MBounds& aabb, double threshold) {
+    for (size_t i = 0; i < result.size(); ++i) {
+      if (sel.max_overlap < 0 && result[i].model != index) continue;
+      if (disjoint(aabb, encl[i])) continue;
+      if (box_overlap(rect, boxes[i]) > threshold + 1e-12) return true;
+    }
+    return false;
+  };
+  for (auto c : cand) {
+    const size_t index = c.model;
+    const int64_t limit = sel.num_matches >= 0 ? sel.num_matches : int64_t(opts[index].num_matches);
+    if (sel.num_matches < 0 && limit && counts[index] >= uint64_t(limit)) continue;
+    const double threshold = sel.max_overlap >= 0 ? sel.max_overlap : opts[index].max_overlap;
+    const Level& lv = models[index]->levels.front();
+    MBox rect = pose_box(lv, c.pose); MBounds aabb = enclosure(rect);
+    if (suppressed(index, rect, aabb, threshold)) continue;
+    if (!refine(c)) continue;
+    rect = pose_box(lv, c.pose); aabb = enclosure(rect);
+    if (suppressed(index, rect, aabb, threshold)) continue;
+    result.push_back(c); boxes.push_back(rect); encl.push_back(aabb); ++counts[index];
+    if (sel.num_matches > 0 && result.size() >= uint64_t(sel.num_matches)) break;
+  }
+  std::sort(result.begin(), result.end(), multi_less);
+  return result;
+}
+
+static bool same_options(const FindOpts& a, const FindOpts& b) {  // matcher.cpp:1083-1086
+  return a.start == b.start && a.extent == b.extent && a.min_score == b.min_score && a.max_overlap == b.max_overlap &&
+         a.num_matches == b.num_matches && a.max_candidates == b.max_candidates && a.subpixel == b.subpixel &&
+         a.highest == b.highest && a.lowest == b.lowest;
+}
+
+// find_models: matcher.cpp:1077-1124 with the C ABI's two-stage quota
+// (per_model_quota_before_overlap = true, ncc_matching_api.cpp:199).
+static std::vector<MultiMatch> find_models(const uint8_t* img, size_t W, size_t H, size_t stride,
+                                           const std::vector<const Model*>& models, const std::vector<FindOpts>& opts,
+                                           Selection sel, bool per_model_quota_before_overlap,
+                                           std::vector<std::vector<Match>>* prepared = nullptr) {
+  const bool cache_samples = models.size() > 1, unlimited = sel.num_matches == 0;
+  bool cache_derivatives = false;
+  for (const Model* m : models) if (m != models.front()) cache_derivatives = true;
+  std::vector<std::unique_ptr<Query>> queries;
+  std::vector<size_t> owners;
+  std::vector<MultiMatch> cand;
+  if (prepared) prepared->assign(models.size(), {});
+  for (size_t i = 0; i < models.size(); ++i) {
+    size_t owner = i;
+    for (size_t j = 0; j < i; ++j)
+      if (models[i] == models[j] && same_options(opts[i], opts[j])) { owner = owners[j]; break; }
+    owners.push_back(owner);
+    if (owner != i) {
+      for (const auto& p : queries[owner]->cand) if (std::isfinite(p.score)) cand.push_back({p, i});
+      if (prepared) (*prepared)[i] = queries[owner]->cand;
+      queries.push_back(nullptr);
+      continue;
+    }
+    auto Q = std::make_unique<Query>();
+    Q->cache = true; Q->cache_samples = cache_samples;
+    prepare(*Q, img, W, H, stride, *models[i], opts[i], nullptr);
+    // default background guard: matcher.cpp:839-846 (AVX2 host, round to nearest)
+    const Model& m = *models[i]; const FindOpts& o = Q->opt;
+    if (!Q->empty && !Q->finest && cache_derivatives && unlimited && Q->cand.size() >= 32 && o.subpixel && !o.num_matches &&
+        o.min_score >= .75 && o.min_score <= .85 && Q->level_count == 3 && !m.opt.absolute && m.opt.step <= .02 &&
+        o.extent <= .4 && m.original.s.size() <= 32768 && Q->shared_full) {
+      Q->guard_first = o.min_score >= .8 ? .1 : .2;
+      Q->guard = .1;
+    }
+    for (const auto& p : Q->cand) if (std::isfinite(p.score)) cand.push_back({p, i});
+    if (prepared) (*prepared)[i] = Q->cand;
+    queries.push_back(std::move(Q));
+  }
+  auto refine_c = [&](MultiMatch& c) { return refine(*queries[owners[c.model]], c.pose); };
+  bool any_quota = false;
+  for (const auto& o : opts) if (o.num_matches > 0) any_quota = true;
+  const bool independent = per_model_quota_before_overlap && sel.num_matches < 0 && sel.max_overlap >= 0 && any_quota;
+  auto result = select_refined(std::move(cand), opts, models, independent ? Selection{-1, -1} : sel, refine_c);
+  if (independent) result = select_refined(std::move(result), opts, models, sel, [](MultiMatch&) { return true; });
+  return result;
 }
 
 }  // namespace orc
@@ -1157,6 +1432,53 @@ double orc_continuous(void* mp, const uint8_t* img, int w, int h, int stride, do
   const double dc = double(img[0]);
   for (size_t r = 0; r < g.h; ++r) for (size_t c = 0; c < g.w; ++c) g.v[r * g.w + c] = double(img[r * size_t(stride) + c]) - dc;
   return continuous(g, m.levels.front(), m.original, Match{row, col, angle, 0}, m.opt.absolute);
+}
+
+// find_models.  opts: n_models x 8 doubles (start, extent, min_score,
+// max_overlap, num_matches, subpixel, highest, lowest); selection as the C
+// ABI derives it (num_matches -1 / max_overlap -1 mean per model).  results:
+// cap x 5 (row, col, angle, score, model) in the internal convention.
+// prepared (optional): n_models x cap_p x 4, with prepared_n the list sizes.
+int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, int h, int stride, const double* opts,
+                    int64_t sel_num, double sel_overlap, int per_model_quota, double* results, int cap, int* n_result,
+                    double* prepared, int cap_p, int* prepared_n) {
+  return guard([&] {
+    std::vector<const Model*> models;
+    std::vector<FindOpts> o(static_cast<size_t>(n_models));
+    for (int i = 0; i < n_models; ++i) {
+      models.push_back(static_cast<const Model*>(mps[i]));
+      const double* q = opts + 8 * i;
+      FindOpts& f = o[size_t(i)];
+      f.start = q[0]; f.extent = q[1]; f.min_score = q[2]; f.max_overlap = q[3]; f.num_matches = size_t(q[4]);
+      f.subpixel = q[5] != 0; f.highest = int(q[6]); f.lowest = int(q[7]);
+    }
+    std::vector<std::vector<Match>> prep;
+    auto res = find_models(img, size_t(w), size_t(h), size_t(stride), models, o, Selection{sel_num, sel_overlap},
+                           per_model_quota != 0, prepared ? &prep : nullptr);
+    *n_result = int(res.size());
+    for (size_t i = 0; i < res.size() && int(i) < cap; ++i) {
+      double* d = results + 5 * i;
+      d[0] = res[i].pose.row; d[1] = res[i].pose.col; d[2] = res[i].pose.angle; d[3] = res[i].pose.score;
+      d[4] = double(res[i].model);
+    }
+    if (prepared)
+      for (int m = 0; m < n_models; ++m) {
+        prepared_n[m] = int(prep[size_t(m)].size());
+        for (size_t i = 0; i < prep[size_t(m)].size() && int(i) < cap_p; ++i) {
+          double* d = prepared + (size_t(m) * size_t(cap_p) + i) * 4;
+          const Match& x = prep[size_t(m)][i];
+          d[0] = x.row; d[1] = x.col; d[2] = x.angle; d[3] = x.score;
+        }
+      }
+  });
+}
+
+// Overlap of two template rectangles under multi-model rules (multi.hpp:54-81):
+// geometry per pose [w, h, orow, ocol].
+double orc_box_overlap(double ar, double ac, double aa, const double* ga, double br, double bc, double ba, const double* gb) {
+  Level la; la.w = size_t(ga[0]); la.h = size_t(ga[1]); la.orow = ga[2]; la.ocol = ga[3];
+  Level lb; lb.w = size_t(gb[0]); lb.h = size_t(gb[1]); lb.orow = gb[2]; lb.ocol = gb[3];
+  return box_overlap(pose_box(la, Match{ar, ac, aa, 0}), pose_box(lb, Match{br, bc, ba, 0}));
 }
 
 // Refine a single candidate (row, col, angle, score in/out); returns verdict.

@@ -59,6 +59,9 @@ def lib():
         L.orc_continuous.argtypes = [P, P, I, I, I, D, D, D]
         L.orc_continuous.restype = D
         L.orc_refine_one.argtypes = [P, P, I, I, I, D, D, D, I, D, I, P]
+        L.orc_find_models.argtypes = [P, I, P, I, I, I, P, C.c_int64, D, I, P, I, P, P, I, P]
+        L.orc_box_overlap.argtypes = [D, D, D, P, D, D, D, P]
+        L.orc_box_overlap.restype = D
         _lib = L
     return _lib
 
@@ -174,3 +177,41 @@ def zncc(a: np.ndarray, b: np.ndarray, absolute: bool = False) -> float:
 
 def overlap(a, b, w, h, orow, ocol) -> float:
     return lib().orc_overlap(a[0], a[1], a[2], b[0], b[1], b[2], w, h, orow, ocol)
+
+
+def find_models(models: list, image: np.ndarray, opts: list, selection_matches: int = -1,
+                selection_overlap: float = -1.0, per_model_quota: bool = True, prepared: bool = False):
+    """find_models over oracle models.  opts: one dict per model with keys
+    start, extent, min_score, max_overlap, num_matches, subpixel, highest,
+    lowest.  Returns (n, 5): row, col, angle, score (internal convention),
+    model index; with prepared=True also each model's prepared list."""
+    img = np.ascontiguousarray(image, dtype=np.uint8)
+    n = len(models)
+    handles = (C.c_void_p * n)(*[m.h.value for m in models])
+    o = np.zeros((n, 8))
+    for i, q in enumerate(opts):
+        o[i] = [q["start"], q["extent"], q["min_score"], q["max_overlap"], q["num_matches"], int(q.get("subpixel", True)),
+                q.get("highest", 0), q.get("lowest", 1)]
+    cap, cap_p = 4096, 20000
+    res = np.zeros((cap, 5))
+    nres = C.c_int()
+    prep = np.zeros((n, cap_p, 4)) if prepared else None
+    prep_n = np.zeros(n, np.int32) if prepared else None
+    st = lib().orc_find_models(handles, n, _ptr(img), img.shape[1], img.shape[0], img.shape[1], _ptr(o),
+                               int(selection_matches), float(selection_overlap), int(per_model_quota), _ptr(res), cap,
+                               C.byref(nres), _ptr(prep) if prepared else None, cap_p,
+                               _ptr(prep_n) if prepared else None)
+    if st:
+        raise OracleError(st, lib().orc_last_error().decode())
+    out = res[:nres.value].copy()
+    if not prepared:
+        return out
+    return out, [prep[i, :prep_n[i]].copy() for i in range(n)]
+
+
+def box_overlap(a, ga, b, gb) -> float:
+    """Multi-model overlap of poses a, b (row, col, angle) with template
+    geometries ga, gb = (w, h, orow, ocol)."""
+    ga = np.asarray(ga, dtype=np.float64)
+    gb = np.asarray(gb, dtype=np.float64)
+    return lib().orc_box_overlap(a[0], a[1], a[2], _ptr(ga), b[0], b[1], b[2], _ptr(gb))
