@@ -114,6 +114,17 @@ MV_API MvStatus mv_gen_image1_extern(MvObject* Image, MvTuple Type, MvTuple Widt
 
 /* Handles: include/mvkit/c/mv_handle.h:40.  Invalidates every alias. */
 MV_API MvStatus mv_clear_handle(MvTuple Handle);
+/* Serialized items: include/mvkit/c/mv_handle.h:63-97
+ * (src/c_api/serialization_api.cpp:51-141) for NCC model handles.
+ * serialize_handle turns a model into an item holding its MVKIT_NCC 1 bytes;
+ * deserialize_handle rebuilds a model from an item; the _ptr pair copies raw
+ * bytes into an item (Copy must be "true") and exposes an item's bytes, valid
+ * as long as the item's handle is. */
+MV_API MvStatus mv_serialize_handle(MvTuple Handle, MvTuple* SerializedItem);
+MV_API MvStatus mv_deserialize_handle(MvTuple SerializedItem, MvTuple* Handle);
+MV_API MvStatus mv_create_serialized_item_ptr(MvTuple Pointer, MvTuple Size, MvTuple Copy,
+                                              MvTuple* SerializedItemHandle);
+MV_API MvStatus mv_get_serialized_item_ptr(MvTuple SerializedItemHandle, MvTuple* Pointer, MvTuple* Size);
 
 /* NCC operators: include/mvkit/c/mv_ncc_matching.h:30-123 */
 MV_API MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
@@ -123,6 +134,12 @@ MV_API MvStatus mv_get_ncc_model_params(MvTuple ModelID, MvTuple* NumLevels, MvT
 MV_API MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple AngleStart, MvTuple AngleExtent,
                                   MvTuple MinScore, MvTuple NumMatches, MvTuple MaxOverlap, MvTuple SubPixel,
                                   MvTuple NumLevels, MvTuple* Row, MvTuple* Column, MvTuple* Angle, MvTuple* Score);
+/* include/mvkit/c/mv_ncc_matching.h:84, 94 (src/c_api/ncc_matching_api.cpp:97-109,
+ * src/ncc_matching/persist.cpp:14-84): the model as MVKIT_NCC 1 text (the
+ * level-0 template with its mask and centroid, checksummed); reading
+ * validates the file and rebuilds the model exactly as create does. */
+MV_API MvStatus mv_write_ncc_model(MvTuple ModelID, MvTuple FileName);
+MV_API MvStatus mv_read_ncc_model(MvTuple FileName, MvTuple* ModelID);
 /* include/mvkit/c/mv_ncc_matching.h:119-123 (src/c_api/ncc_matching_api.cpp:191-203):
  * several models in one image.  ModelIDs holds 1..4096 handles; every other
  * control one value or one per model (NumLevels: highest, a shared pair, one

@@ -27,7 +27,8 @@ PUBLIC_SYMBOLS = [
     "mv_get_last_error", "mv_get_last_error_info", "mv_create_tuple", "mv_copy_tuple", "mv_destroy_tuple",
     "mv_length_tuple", "mv_get_type", "mv_get_i", "mv_get_d", "mv_get_s", "mv_get_h", "mv_resize_tuple", "mv_set_i",
     "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
-    "mv_clear_handle", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
+    "mv_clear_handle", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
+    "mv_get_serialized_item_ptr", "mv_write_ncc_model", "mv_read_ncc_model", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
     "mvamd_find_ncc_model_device", "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
@@ -113,6 +114,12 @@ def lib() -> C.CDLL:
         L.mv_get_ncc_model_params.argtypes = [P] + [C.POINTER(P)] * 5
         L.mv_find_ncc_model.argtypes = [P] * 9 + [C.POINTER(P)] * 4
         L.mv_find_ncc_models.argtypes = [P] * 9 + [C.POINTER(P)] * 5
+        L.mv_write_ncc_model.argtypes = [P, P]
+        L.mv_read_ncc_model.argtypes = [P, C.POINTER(P)]
+        L.mv_serialize_handle.argtypes = [P, C.POINTER(P)]
+        L.mv_deserialize_handle.argtypes = [P, C.POINTER(P)]
+        L.mv_create_serialized_item_ptr.argtypes = [P, P, P, C.POINTER(P)]
+        L.mv_get_serialized_item_ptr.argtypes = [P, C.POINTER(P), C.POINTER(P)]
         L.mvamd_find_ncc_models_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), S, I64, D, P, S,
                                                    C.POINTER(S)]
         L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
@@ -240,7 +247,51 @@ class NccModel:
         args = [_to_tuple(num_levels), _to_tuple(float(angle_start)), _to_tuple(float(angle_extent)),
                 _to_tuple(angle_step), _to_tuple(metric)]
         _check(lib().mv_create_ncc_model(img.h, *[t.h for t in args], C.byref(out)))
-        self.tuple = Tuple(handle=out)
+        self._bind(Tuple(handle=out))
+
+    @classmethod
+    def _from_tuple(cls, t: "Tuple") -> "NccModel":
+        m = cls.__new__(cls)
+        m._bind(t)
+        return m
+
+    @classmethod
+    def read(cls, path: str) -> "NccModel":
+        """mv_read_ncc_model."""
+        out = C.c_void_p()
+        name = Tuple(str(path))
+        _check(lib().mv_read_ncc_model(name.h, C.byref(out)))
+        return cls._from_tuple(Tuple(handle=out))
+
+    @classmethod
+    def deserialize(cls, data: bytes) -> "NccModel":
+        """mv_create_serialized_item_ptr + mv_deserialize_handle."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        item = C.c_void_p()
+        args = [Tuple(C.addressof(buf)), Tuple(len(data)), Tuple("true")]  # alive across the call
+        _check(lib().mv_create_serialized_item_ptr(*[t.h for t in args], C.byref(item)))
+        it = Tuple(handle=item)
+        out = C.c_void_p()
+        _check(lib().mv_deserialize_handle(it.h, C.byref(out)))
+        return cls._from_tuple(Tuple(handle=out))
+
+    def write(self, path: str) -> None:
+        """mv_write_ncc_model."""
+        name = Tuple(str(path))
+        _check(lib().mv_write_ncc_model(self.tuple.h, name.h))
+
+    def serialize(self) -> bytes:
+        """mv_serialize_handle + mv_get_serialized_item_ptr: the MVKIT_NCC 1 bytes."""
+        item = C.c_void_p()
+        _check(lib().mv_serialize_handle(self.tuple.h, C.byref(item)))
+        it = Tuple(handle=item)
+        ptr, size = C.c_void_p(), C.c_void_p()
+        _check(lib().mv_get_serialized_item_ptr(it.h, C.byref(ptr), C.byref(size)))
+        addr, n = Tuple(handle=ptr).values()[0], Tuple(handle=size).values()[0]
+        return C.string_at(addr, n)
+
+    def _bind(self, t: "Tuple") -> None:
+        self.tuple = t
         info = np.zeros(6)
         _check(lib().mvamd_model_info(self.tuple.h, _ptr(info)))
         self.n_levels, self.n_angles = int(info[0]), int(info[1])

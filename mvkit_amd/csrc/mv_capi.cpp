@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -23,9 +24,13 @@
 using mvamd::Error;
 
 // ------------------------------------------------------------ object model
+// A handle's object: an NCC model, or a serialized item (the bytes of a
+// model, mv_handle.h:63-97).  Both are null once the handle is cleared.
 struct ModelObject {
   std::mutex mu;
-  std::shared_ptr<mvamd::NccModel> model;  // null once cleared
+  std::shared_ptr<mvamd::NccModel> model;
+  std::shared_ptr<const std::string> item;
+  bool is_item = false;
 };
 
 struct MvHandleImpl {
@@ -200,6 +205,7 @@ std::shared_ptr<mvamd::NccModel> model_of(MvTuple t, const char* op) {
   const Value& v = view.at(0);
   if (!v.h || !v.h->object) view.bad(mvamd::kErrInvalidType, 0, "control handle is not an NCC model");
   std::lock_guard<std::mutex> lock(v.h->object->mu);
+  if (v.h->object->is_item) view.bad(mvamd::kErrInvalidType, 0, "control handle is not an NCC model");
   if (!v.h->object->model)
     fail(mvamd::kInvalidState, mvamd::kErrInvalidObjectState, op, "ModelID", "NCC model handle was cleared", 0);
   return v.h->object->model;
@@ -285,6 +291,7 @@ std::vector<std::shared_ptr<mvamd::NccModel>> models_of(MvTuple t, const char* o
     const Value& v = view.at(i);
     if (!v.h || !v.h->object) view.bad(mvamd::kErrInvalidType, int64_t(i), "control handle is not an NCC model");
     std::lock_guard<std::mutex> lock(v.h->object->mu);
+    if (v.h->object->is_item) view.bad(mvamd::kErrInvalidType, int64_t(i), "control handle is not an NCC model");
     if (!v.h->object->model)
       fail(mvamd::kInvalidState, mvamd::kErrInvalidObjectState, op, "ModelIDs", "NCC model handle was cleared", int64_t(i));
     out.push_back(v.h->object->model);
@@ -324,6 +331,55 @@ void put_handle(MvTupleImpl& t, std::shared_ptr<mvamd::NccModel> m) {
   v.h->object = std::make_shared<ModelObject>();
   v.h->object->model = std::move(m);
   t.values.push_back(std::move(v));
+}
+void put_item(MvTupleImpl& t, std::string bytes) {
+  Value v;
+  v.kind = MV_VALUE_HANDLE;
+  v.h = std::make_unique<MvHandleImpl>();
+  v.h->object = std::make_shared<ModelObject>();
+  v.h->object->is_item = true;
+  v.h->object->item = std::make_shared<const std::string>(std::move(bytes));
+  t.values.push_back(std::move(v));
+}
+
+// One live handle of a control tuple (serialization_api.cpp:22-31).
+std::shared_ptr<ModelObject> single_handle(MvTuple t, const char* op, const char* param) {
+  const Input view(t, op, param);
+  if (view.size() != 1) view.bad(mvamd::kErrInvalidCardinality, -1, "exactly one handle required");
+  if (view.kind(0) != MV_VALUE_HANDLE) view.bad(mvamd::kErrInvalidType, 0, "handle required");
+  const Value& v = view.at(0);
+  std::shared_ptr<ModelObject> obj = v.h ? v.h->object : nullptr;
+  bool live = false;
+  if (obj) {
+    std::lock_guard<std::mutex> lock(obj->mu);
+    live = obj->model || obj->item;
+  }
+  if (!live) fail(mvamd::kInvalidState, mvamd::kErrInvalidObjectState, op, param, "handle is empty or cleared", 0);
+  return obj;
+}
+
+// FileName control of read/write (ncc_matching_api.cpp:77-83).
+std::string filename(MvTuple t, const char* op) {
+  const Input view(t, op, "FileName");
+  if (view.size() != 1) view.bad(mvamd::kErrInvalidCardinality, -1, "exactly one file name required");
+  const std::string& name = view.string(0);
+  if (name.empty() || name.find('\0') != std::string::npos)
+    view.bad(mvamd::kErrInvalidValue, 0, "nonempty file name without NUL required");
+  return name;
+}
+
+// Rethrows a codec error under the operation and parameter of a
+// serialization entry point (serialization_api.cpp:14-20).
+template <class F>
+auto codec(const char* op, const char* param, F&& f) -> decltype(f()) {
+  try {
+    return f();
+  } catch (Error& e) {
+    e.operation = op;
+    e.parameter = param;
+    e.index = 0;
+    throw;
+  }
 }
 }  // namespace
 
@@ -511,6 +567,7 @@ extern "C" MvStatus mv_clear_handle(MvTuple Handle) {
       if (v.kind != MV_VALUE_HANDLE || !v.h || !v.h->object) view.bad(mvamd::kErrInvalidType, int64_t(i), "handle required");
       std::lock_guard<std::mutex> lock(v.h->object->mu);
       v.h->object->model.reset();
+      v.h->object->item.reset();
     }
   });
 }
@@ -624,6 +681,123 @@ extern "C" MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple
       push(out[3], m.pose.score);
       push(out[4], int64_t(m.model));
     }
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_write_ncc_model(MvTuple ModelID, MvTuple FileName) {
+  return translate([&] {
+    constexpr auto op = "write_ncc_model";
+    const auto m = model_of(ModelID, op);
+    const std::string name = filename(FileName, op);
+    const std::string bytes = algorithm(op, [&] { return mvamd::serialize_model(*m); });
+    // core/model_io.hpp:14-20
+    std::FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) fail(mvamd::kIoError, mvamd::kErrIoFailure, op, "", "cannot open model for writing");
+    const bool ok = std::fwrite(bytes.data(), 1, bytes.size(), f) == bytes.size() && std::fflush(f) == 0;
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed) fail(mvamd::kIoError, mvamd::kErrIoFailure, op, "", "cannot write model file");
+  });
+}
+
+extern "C" MvStatus mv_read_ncc_model(MvTuple FileName, MvTuple* ModelID) {
+  return translate([&] {
+    constexpr auto op = "read_ncc_model";
+    Outputs<1> out(op, {ModelID});
+    const std::string name = filename(FileName, op);
+    // core/model_io.cpp:13-49: the whole file, bounded
+    constexpr long kLimit = 64L * 1024 * 1024 + 128;
+    std::FILE* f = std::fopen(name.c_str(), "rb");
+    if (!f) fail(mvamd::kIoError, mvamd::kErrIoFailure, op, "", "cannot open model for reading");
+    struct Closer {
+      std::FILE* f;
+      ~Closer() { std::fclose(f); }
+    } closer{f};
+    long size = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) size = std::ftell(f);
+    if (size < 0 || size > kLimit) fail(mvamd::kLimitExceeded, mvamd::kErrInvalidValue, op, "", "model file exceeds size limit");
+    std::string bytes(size_t(size), '\0');
+    if (std::fseek(f, 0, SEEK_SET) != 0 || (size && std::fread(bytes.data(), 1, bytes.size(), f) != bytes.size()))
+      fail(mvamd::kIoError, mvamd::kErrIoFailure, op, "", "cannot read model file");
+    std::shared_ptr<mvamd::NccModel> m = algorithm(op, [&] { return mvamd::deserialize_model(bytes); });
+    put_handle(out[0], std::move(m));
+    out.commit();
+  });
+}
+
+// ======================================================= serialized items
+extern "C" MvStatus mv_serialize_handle(MvTuple Handle, MvTuple* SerializedItem) {
+  return translate([&] {
+    constexpr auto op = "serialize_handle";
+    Outputs<1> out(op, {SerializedItem});
+    const auto obj = single_handle(Handle, op, "Handle");
+    std::shared_ptr<mvamd::NccModel> m;
+    {
+      std::lock_guard<std::mutex> lock(obj->mu);
+      if (!obj->is_item) m = obj->model;
+    }
+    if (!m) fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Handle", "handle type does not support serialization", 0);
+    put_item(out[0], codec(op, "Handle", [&] { return mvamd::serialize_model(*m); }));
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_deserialize_handle(MvTuple SerializedItem, MvTuple* Handle) {
+  return translate([&] {
+    constexpr auto op = "deserialize_handle";
+    Outputs<1> out(op, {Handle});
+    const auto obj = single_handle(SerializedItem, op, "SerializedItem");
+    std::shared_ptr<const std::string> bytes;
+    {
+      std::lock_guard<std::mutex> lock(obj->mu);
+      if (obj->is_item) bytes = obj->item;
+    }
+    if (!bytes) fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, op, "SerializedItem", "serialized item required", 0);
+    if (bytes->compare(0, 10, "MVKIT_NCC ") != 0)
+      fail(mvamd::kDecodeError, mvamd::kErrIoFailure, op, "SerializedItem", "unknown serialized model signature", 0);
+    std::shared_ptr<mvamd::NccModel> m = codec(op, "SerializedItem", [&] { return mvamd::deserialize_model(*bytes); });
+    put_handle(out[0], std::move(m));
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_create_serialized_item_ptr(MvTuple Pointer, MvTuple Size, MvTuple Copy,
+                                                  MvTuple* SerializedItemHandle) {
+  return translate([&] {
+    constexpr auto op = "create_serialized_item_ptr";
+    Outputs<1> out(op, {SerializedItemHandle});
+    const Input pointer(Pointer, op, "Pointer"), size(Size, op, "Size"), copy(Copy, op, "Copy");
+    const auto address = static_cast<uintptr_t>(pointer.scalar_integer());
+    const int64_t length = size.scalar_integer();
+    if (copy.size() != 1) copy.bad(mvamd::kErrInvalidCardinality, -1, "one copy mode required");
+    if (copy.string(0) != "true")
+      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Copy", "only owned copy mode true is supported", 0);
+    if (length <= 0) size.bad(mvamd::kErrInvalidValue, 0, "positive byte count required");
+    constexpr uint64_t kMaxBytes = 256ull * 1024 * 1024 + 128;
+    if (uint64_t(length) > kMaxBytes)
+      fail(mvamd::kLimitExceeded, mvamd::kErrInvalidValue, op, "Size", "serialized item exceeds size limit", 0);
+    if (!address || address > std::numeric_limits<uintptr_t>::max() - size_t(length))
+      pointer.bad(mvamd::kErrInvalidValue, 0, "non-null address without range overflow required");
+    put_item(out[0], std::string(reinterpret_cast<const char*>(address), size_t(length)));
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_get_serialized_item_ptr(MvTuple SerializedItemHandle, MvTuple* Pointer, MvTuple* Size) {
+  return translate([&] {
+    constexpr auto op = "get_serialized_item_ptr";
+    Outputs<2> out(op, {Pointer, Size});
+    const auto obj = single_handle(SerializedItemHandle, op, "SerializedItemHandle");
+    std::shared_ptr<const std::string> bytes;
+    {
+      std::lock_guard<std::mutex> lock(obj->mu);
+      if (obj->is_item) bytes = obj->item;
+    }
+    if (!bytes)
+      fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, op, "SerializedItemHandle", "serialized item required", 0);
+    // the bytes live as long as the item (its handle object)
+    push(out[0], int64_t(reinterpret_cast<intptr_t>(bytes->data())));
+    push(out[1], int64_t(bytes->size()));
     out.commit();
   });
 }

@@ -20,6 +20,8 @@ const char* const kCreateOp = "ncc_model_create";
 [[noreturn]] void raise(Status s, const char* param, const char* msg) { throw Error(s, kCreateOp, param, msg); }
 
 int32_t pack_rc(int32_t r, int32_t c) { return int32_t((uint32_t(r) << 16) | (uint32_t(c) & 0xffffu)); }
+int32_t unpack_row(int32_t rc) { return rc >> 16; }
+int32_t unpack_col(int32_t rc) { return int32_t(int16_t(uint16_t(uint32_t(rc) & 0xffffu))); }
 }  // namespace
 
 ErrorCode default_code(Status s) {
@@ -205,7 +207,6 @@ std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size
     raise(kLimitExceeded, "domain", "template bounding box exceeds one million pixels");
 
   auto m = std::make_unique<NccModel>();
-  m->params = p;
   LevelGeom first;
   first.w = width; first.h = height;
   first.gray.resize(width * height);
@@ -235,9 +236,19 @@ std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size
   first.ocol = double((sum_col - static_cast<long double>(0) * area) / area);
   const double mean = double(total) / double(width * height);
   for (auto& g : first.gray) g -= mean;
+  return rebuild_model(std::move(first), p, m->orow, m->ocol);
+}
 
-  // rebuild(): original patch (division by the norm), pyramid, angle grid
-  m->sample_count = width * height;
+std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& p, double origin_row, double origin_col) {
+  // rasters store sample coordinates as 16-bit pairs
+  if (first.w > 16383 || first.h > 16383)
+    raise(kLimitExceeded, "domain", "template side exceeds 16383 pixels");
+  auto m = std::make_unique<NccModel>();
+  m->params = p;
+  m->orow = origin_row;
+  m->ocol = origin_col;
+  // original patch (division by the norm), pyramid, angle grid
+  m->sample_count = size_t(std::count(first.mask.begin(), first.mask.end(), uint8_t(1)));
   m->levels.push_back(std::move(first));
   // original patch: centred by the incremental mean, divided by the norm
   // (model.cpp:182-204); appended after the angle grid
@@ -300,8 +311,13 @@ std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size
   const LevelGeom& l0 = m->levels.front();
   original.orow = l0.orow;
   original.ocol = l0.ocol;
-  original.rmin = 0; original.rmax = int32_t(l0.h - 1);
-  original.cmin = 0; original.cmax = int32_t(l0.w - 1);
+  // bounds of the masked samples (model.cpp:195-199)
+  original.rmin = original.rmax = unpack_row(orig_rc.front());
+  original.cmin = original.cmax = unpack_col(orig_rc.front());
+  for (const int32_t rc : orig_rc) {
+    original.rmin = std::min(original.rmin, unpack_row(rc)); original.rmax = std::max(original.rmax, unpack_row(rc));
+    original.cmin = std::min(original.cmin, unpack_col(rc)); original.cmax = std::max(original.cmax, unpack_col(rc));
+  }
   original.first = uint32_t(m->pool.w.size());
   original.count = uint32_t(orig_w.size());
   original.run_first = uint32_t(m->pool.runs.size() / 4);

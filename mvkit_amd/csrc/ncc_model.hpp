@@ -106,6 +106,15 @@ struct NccModel {
 std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size_t width, size_t height,
                                        size_t stride_bytes, const CreateParams& params);
 
+// rebuild(): model.cpp:173-242 from a centred level 0 with any mask and the
+// template centroid in image coordinates (persisted models).
+std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& params, double origin_row,
+                                        double origin_col);
+
+// MVKIT_NCC 1 text model (persist.cpp:14-84).
+std::string serialize_model(const NccModel& model);
+std::unique_ptr<NccModel> deserialize_model(const std::string& bytes);
+
 // Builds the raster of one level at `angle` into `pool` (rotate + normalize,
 // model.cpp:39-98).  Returns the raster index, or -1 when degenerate.
 int64_t build_raster(const LevelGeom& level, double angle, RasterPool& pool, const double* prepared_mean = nullptr,

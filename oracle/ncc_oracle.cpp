@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <array>
+#include <charconv>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -38,8 +39,7 @@ constexpr double kEps = std::numeric_limits<double>::epsilon();
 
 // Status codes of include/mvkit/c/mv_status.h:10-24.
 enum Status : int {
-  OK = 0, INVALID_ARGUMENT = -1, INVALID_STATE = -7, LIMIT_EXCEEDED = -103,
-  UNSUPPORTED = -6
+  OK = 0, INVALID_ARGUMENT = -1, INVALID_STATE = -7, LIMIT_EXCEEDED = -103, UNSUPPORTED = -6, DECODE_ERROR = -101
 };
 
 struct Failure : std::runtime_error {
@@ -169,7 +169,9 @@ static bool level_usable(const Level& lv, size_t& count) {
   return energy > 0 && std::isfinite(energy);
 }
 
-// create + rebuild: model.cpp:120-242 (file lines 203-325), full-domain u8 template.
+static Model rebuild(Level first, const CreateOpts& o, double orow, double ocol);
+
+// create: model.cpp:120-171 (file lines 203-254), full-domain u8 template.
 static Model create(const uint8_t* px, size_t w, size_t h, size_t stride, const CreateOpts& o) {
   if (!w || !h) fail(INVALID_ARGUMENT, "one nonempty single-channel image required");
   if (!std::isfinite(o.start) || !std::isfinite(o.extent) || !std::isfinite(o.step) || o.extent < 0 ||
@@ -192,10 +194,14 @@ static Model create(const uint8_t* px, size_t w, size_t h, size_t stride, const 
   first.ocol = double((sum_col - 0.0L * area) / area);
   const double mean = double(isum) / double(w * h);
   for (auto& g : first.gray) g -= mean;
+  return rebuild(std::move(first), o, m0.orow, m0.ocol);
+}
 
-  // rebuild (model.cpp:173-242)
-  Model m; m.opt = o; m.orow = m0.orow; m.ocol = m0.ocol;
-  m.sample_count = w * h;
+// rebuild: model.cpp:173-242 (file lines 256-325), from a centred level 0
+// with any mask (the reference's own templates may have reduced domains).
+static Model rebuild(Level first, const CreateOpts& o, double orow, double ocol) {
+  Model m; m.opt = o; m.orow = orow; m.ocol = ocol;
+  m.sample_count = size_t(std::count(first.mask.begin(), first.mask.end(), uint8_t(1)));
   m.levels.push_back(std::move(first));
   Prepared first_norm{};
   {
@@ -262,6 +268,122 @@ static Model create(const uint8_t* px, size_t w, size_t h, size_t stride, const 
     }
   }
   return m;
+}
+
+// ------------------------------------------------------------- persistence
+// "MVKIT_NCC 1" text model: persist.cpp:14-84, number formatting of
+// core/model_text.hpp:18-62 (classic locale, integers within 2^53 as
+// integers, other reals with 17 significant digits), FNV-1a-64 checksum of
+// the body (persist.cpp:18-21).
+static uint64_t fnv1a64(const std::string& t) {
+  uint64_t h = 14695981039346656037ULL;
+  for (unsigned char c : t) { h ^= c; h *= 1099511628211ULL; }
+  return h;
+}
+static void put_int(std::string& s, long long v) { s += std::to_string(v); }
+static void put_real(std::string& s, double v) {
+  if (v == 0.) { if (std::signbit(v)) s += '-'; s += '0'; return; }
+  if (v == 1.) { s += '1'; return; }
+  if (v >= -9007199254740992. && v <= 9007199254740992.) {
+    const long long i = static_cast<long long>(v);
+    if (static_cast<double>(i) == v) { put_int(s, i); return; }
+  }
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.17g", v);
+  s += buf;
+}
+static std::string serialize(const Model& m) {
+  if (m.levels.empty()) fail(INVALID_STATE, "cannot save empty model");
+  const Level& lv = m.levels.front();
+  std::string t;
+  put_int(t, m.opt.levels); t += ' '; put_int(t, m.opt.absolute ? 1 : 0); t += ' ';
+  put_real(t, m.opt.start); t += ' '; put_real(t, m.opt.extent); t += ' '; put_real(t, m.opt.step); t += '\n';
+  put_real(t, m.orow); t += ' '; put_real(t, m.ocol); t += ' ';
+  put_int(t, (long long)lv.w); t += ' '; put_int(t, (long long)lv.h); t += '\n';
+  put_real(t, lv.orow); t += ' '; put_real(t, lv.ocol); t += '\n';
+  for (size_t i = 0; i < lv.gray.size(); ++i) { put_int(t, lv.mask[i]); t += ' '; put_real(t, lv.gray[i]); t += '\n'; }
+  if (t.size() > 64u * 1024u * 1024u) fail(LIMIT_EXCEEDED, "serialized model exceeds file budget");
+  return "MVKIT_NCC 1\n" + std::to_string(fnv1a64(t)) + "\n" + t;
+}
+
+// Fields of the body: ASCII whitespace separated, an optional leading '+',
+// then the longest number (core/model_text.hpp:113-189).
+struct FieldReader {
+  const char* p; const char* e;
+  void ws() { while (p < e && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == '\v' || *p == '\f')) ++p; }
+  bool sign() {
+    ws();
+    if (p == e) return false;
+    if (*p == '+') { ++p; if (p == e || *p == '+' || *p == '-') return false; }
+    return true;
+  }
+  long long integer() {
+    long long v = 0;
+    if (!sign()) fail(DECODE_ERROR, "invalid or truncated model field");
+    auto r = std::from_chars(p, e, v);
+    if (r.ec != std::errc{}) fail(DECODE_ERROR, "invalid or truncated model field");
+    p = r.ptr;
+    return v;
+  }
+  double real() {
+    double v = 0;
+    if (!sign()) fail(DECODE_ERROR, "invalid or truncated model field");
+    auto r = std::from_chars(p, e, v);
+    if (r.ec != std::errc{}) fail(DECODE_ERROR, "invalid or truncated model field");
+    p = r.ptr;
+    if (!std::isfinite(v)) fail(DECODE_ERROR, "non-finite model field");
+    return v;
+  }
+  size_t count(size_t limit) {
+    const long long v = integer();
+    if (v < 0 || (unsigned long long)v > limit) fail(LIMIT_EXCEEDED, "model count exceeds budget");
+    return size_t(v);
+  }
+};
+
+static Model deserialize(const std::string& bytes) {
+  if (bytes.size() > 64u * 1024u * 1024u + 128) fail(LIMIT_EXCEEDED, "model file exceeds budget");
+  const size_t nl = bytes.find('\n');
+  const std::string magic = bytes.substr(0, nl);
+  if (magic != "MVKIT_NCC 1")
+    fail(magic.rfind("MVKIT_NCC ", 0) == 0 ? UNSUPPORTED : DECODE_ERROR, "invalid model signature or version");
+  size_t pos = nl == std::string::npos ? bytes.size() : nl + 1;
+  while (pos < bytes.size() && std::isspace((unsigned char)bytes[pos])) ++pos;
+  uint64_t expected = 0;
+  auto r = std::from_chars(bytes.data() + pos, bytes.data() + bytes.size(), expected);
+  if (r.ec != std::errc{}) fail(DECODE_ERROR, "invalid or truncated model field");
+  pos = size_t(r.ptr - bytes.data());
+  if (pos >= bytes.size() || bytes[pos] != '\n') fail(DECODE_ERROR, "invalid checksum header");
+  const std::string text = bytes.substr(pos + 1);
+  if (fnv1a64(text) != expected) fail(DECODE_ERROR, "NCC model checksum mismatch");
+  FieldReader in{text.data(), text.data() + text.size()};
+  CreateOpts o;
+  o.levels = int(in.count(16)); o.absolute = in.count(1) != 0;
+  o.start = in.real(); o.extent = in.real(); o.step = in.real();
+  if (o.extent < 0 || o.extent > 6.283185307179586 || o.step <= 0 || !std::isfinite(o.start + o.extent))
+    fail(DECODE_ERROR, "invalid trained angle interval");
+  const double orow = in.real(), ocol = in.real();
+  Level first;
+  first.w = in.count(1048576); first.h = in.count(1048576);
+  if (!first.w || !first.h || first.h > 1048576 / first.w) fail(LIMIT_EXCEEDED, "invalid model dimensions");
+  first.orow = in.real(); first.ocol = in.real();
+  first.gray.resize(first.w * first.h); first.mask.resize(first.gray.size());
+  long double rs = 0, cs = 0; size_t n = 0;
+  for (size_t i = 0; i < first.gray.size(); ++i) {
+    first.mask[i] = uint8_t(in.count(1)); first.gray[i] = in.real();
+    if (first.mask[i]) { rs += i / first.w; cs += i % first.w; ++n; }
+  }
+  if (n < 4 || std::abs(first.orow - double(rs / n)) > 1e-10 || std::abs(first.ocol - double(cs / n)) > 1e-10 ||
+      orow < first.orow || ocol < first.ocol || std::abs((orow - first.orow) - std::round(orow - first.orow)) > 1e-8 ||
+      std::abs((ocol - first.ocol) - std::round(ocol - first.ocol)) > 1e-8)
+    fail(DECODE_ERROR, "invalid template domain centroid");
+  in.ws();
+  if (in.p != in.e) fail(DECODE_ERROR, "trailing model data");
+  try {
+    return rebuild(std::move(first), o, orow, ocol);
+  } catch (const Failure& f) {
+    fail(f.status == LIMIT_EXCEEDED ? LIMIT_EXCEEDED : DECODE_ERROR, "invalid prepared NCC model");
+  }
 }
 
 // ------------------------------------------------------------- search image
@@ -1342,6 +1464,19 @@ void orc_patch_samples(void* mp, int level, int angle, int32_t* rows, int32_t* c
 void orc_level_gray(void* mp, int level, double* gray) {
   const Level& lv = static_cast<Model*>(mp)->levels[size_t(level)];
   std::copy(lv.gray.begin(), lv.gray.end(), gray);
+}
+
+// MVKIT_NCC 1 text of a model (length in *len; bytes copied when cap allows).
+int orc_serialize(void* mp, char* buf, size_t cap, size_t* len) {
+  return guard([&] {
+    const std::string t = serialize(*static_cast<Model*>(mp));
+    *len = t.size();
+    if (buf && cap >= t.size()) std::memcpy(buf, t.data(), t.size());
+  });
+}
+uint64_t orc_checksum(const char* bytes, size_t len) { return fnv1a64(std::string(bytes, len)); }
+int orc_deserialize(const char* bytes, size_t len, void** out) {
+  return guard([&] { *out = new Model(deserialize(std::string(bytes, len))); });
 }
 
 // Search pyramid level values (S_l/4^l - dc, doubles) for a full-domain u8 image.

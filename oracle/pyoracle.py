@@ -62,6 +62,10 @@ def lib():
         L.orc_find_models.argtypes = [P, I, P, I, I, I, P, C.c_int64, D, I, P, I, P, P, I, P]
         L.orc_box_overlap.argtypes = [D, D, D, P, D, D, D, P]
         L.orc_box_overlap.restype = D
+        L.orc_serialize.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.orc_deserialize.argtypes = [P, C.c_size_t, C.POINTER(P)]
+        L.orc_checksum.argtypes = [P, C.c_size_t]
+        L.orc_checksum.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -90,6 +94,29 @@ class Model:
         lib().orc_model_info(self.h, _ptr(info))
         self.n_levels, self.n_angles = int(info[0]), int(info[1])
         self.step, self.origin_row, self.origin_col, self.sample_count = info[2], info[3], info[4], int(info[5])
+
+    @classmethod
+    def deserialize(cls, data: bytes) -> "Model":
+        h = C.c_void_p()
+        st = lib().orc_deserialize(data, len(data), C.byref(h))
+        if st:
+            raise OracleError(st, lib().orc_last_error().decode())
+        m = cls.__new__(cls)
+        m.h = h
+        info = np.zeros(6)
+        lib().orc_model_info(m.h, _ptr(info))
+        m.n_levels, m.n_angles = int(info[0]), int(info[1])
+        m.step, m.origin_row, m.origin_col, m.sample_count = info[2], info[3], info[4], int(info[5])
+        return m
+
+    def serialize(self) -> bytes:
+        n = C.c_size_t()
+        st = lib().orc_serialize(self.h, None, 0, C.byref(n))
+        if st:
+            raise OracleError(st, lib().orc_last_error().decode())
+        buf = C.create_string_buffer(n.value)
+        lib().orc_serialize(self.h, buf, n.value, C.byref(n))
+        return buf.raw[:n.value]
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -215,3 +242,8 @@ def box_overlap(a, ga, b, gb) -> float:
     ga = np.asarray(ga, dtype=np.float64)
     gb = np.asarray(gb, dtype=np.float64)
     return lib().orc_box_overlap(a[0], a[1], a[2], _ptr(ga), b[0], b[1], b[2], _ptr(gb))
+
+
+def checksum(data: bytes) -> int:
+    """FNV-1a-64 of the model body (persist.cpp:18-21)."""
+    return lib().orc_checksum(data, len(data))

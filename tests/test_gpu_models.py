@@ -1,7 +1,8 @@
 """GPU parity of find_ncc_models: the HIP path (through mv_find_ncc_models and
 mvamd_find_ncc_models_device) against the oracle's find_models, for every
 form of the per-model controls, repeated handles, per-model level pairs and
-the background guard."""
+the background guard; and finds with models loaded from MVKIT_NCC 1 files,
+including a template with a masked (elliptical) domain."""
 import math
 import sys
 
@@ -166,3 +167,40 @@ def test_find_models_rot2048(capi, oracle, scene_rot):
     want[:, :4] = capi.halcon_shift(want[:, :4])
     check(got, want, "rot2048")
     assert np.count_nonzero(got[:, 4] == 0) == 5
+
+
+def test_loaded_models_find_like_created(capi, oracle, scene_small, tmp_path):
+    a, x = -math.radians(20), math.radians(40)
+    m = capi.NccModel(scene_small.template, 0, a, x)
+    path = str(tmp_path / "small.ncm")
+    m.write(path)
+    r = capi.NccModel.read(path)
+    say("find with a model read from a file")
+    want = m.find(scene_small.image, a, x, 0.7, 2, 0.5, "true", 0)
+    got = r.find(scene_small.image, a, x, 0.7, 2, 0.5, "true", 0)
+    assert len(want) == 2 and np.array_equal(got, want)
+
+
+def test_masked_template_model_matches_oracle(capi, oracle):
+    """A reduced-domain (elliptical) template as the reference writes it:
+    loaded into both, its pastes are found with the oracle's poses."""
+    from mvkit_amd import synth
+    from test_persist import masked_file
+    data, mask = masked_file(oracle)
+    t = synth.make_template(synth.SplitMix64(99), 26, 30)
+    rng = synth.SplitMix64(4242)
+    base = synth.blurred_noise(rng, 200, 240).astype(np.float64)
+    for cy, cx, ang in [(60.3, 70.8, 0.12), (140.6, 170.2, -0.2)]:
+        synth.paste(base, t, cy, cx, ang)
+    image = np.rint(np.clip(base + 3.0 * rng.normal(200 * 240).reshape(200, 240), 0, 255)).astype(np.uint8)
+    o = oracle.Model.deserialize(data)
+    m = capi.NccModel.deserialize(data)
+    say("find with a masked-template model")
+    want = o.find(image, -0.25, 0.5, 0.6, 2, 0.5)
+    assert len(want) == 2
+    got = m.find(image, -0.25, 0.5, 0.6, 2, 0.5, "true", 0)
+    shifted = capi.halcon_shift(want)
+    assert got.shape == shifted.shape
+    assert np.abs(got[:, :2] - shifted[:, :2]).max() <= POS_TOL
+    assert np.abs(got[:, 2] - shifted[:, 2]).max() <= ANGLE_TOL
+    assert np.abs(got[:, 3] - shifted[:, 3]).max() <= SCORE_TOL
