@@ -28,6 +28,8 @@ typedef struct MvAmdFindOptions {
   double screen;                         /* FP8 screen threshold (form FP8_SCREEN) */
   int32_t profile;                       /* bit s: time stage s (MVAMD_STAGE_*) with HIP events;
                                             MVAMD_PROFILE_ALL times every stage */
+  int32_t vector_tracking;               /* 1: track every level with the FP64 vector kernel
+                                            (0: matrix cores where they apply; same results) */
 } MvAmdFindOptions;
 
 /* Kernel stages reported in MvAmdFindStats.kernel_us. */

@@ -147,6 +147,22 @@ int ncchip_track(const NccFrame* frames, int n_frames, int level, int level_kind
                  double step, double offset, int exact_lattice, int absolute, int tile_pixels, int exact_f64,
                  unsigned long long* work, int grid, void* stream);
 
+/* ncchip_track on matrix cores, for u16 levels 1..4: per candidate, the 5x5
+ * windows of all eligible patches as one split-BF16 GEMM
+ * (v_mfma_f32_32x32x16_bf16) with a rigorous error bound per window; the
+ * windows that can hold the maximum are re-scored exactly in FP64, so the
+ * tracked pose is the one ncchip_track returns.  tile_h / tile_w bound the
+ * rows / columns of any candidate's neighbourhood (the union of its
+ * windows); x_max bounds |S - dc 4^l| at the level.  Candidates with more
+ * than 32 eligible patches or a larger neighbourhood are scored exactly.
+ * Returns -6 when the level, kind or LDS budget does not allow it (then use
+ * ncchip_track).  Same inputs, outputs and reference lines as ncchip_track. */
+int ncchip_track_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, const NccPatchPool* pools,
+                      const int32_t* refs, int n_refs, const NccMatch* in, const int32_t* in_count, int cap,
+                      NccMatch* raw, NccMatch* out, int32_t* out_count, int32_t* table, int table_size, double keep,
+                      double tol, double step, double offset, int exact_lattice, int absolute, int tile_h,
+                      int tile_w, double x_max, unsigned long long* work, int grid, void* stream);
+
 typedef struct NcchipRefineParams {
   const NccFrame* frames;  /* per-frame descriptors (device) */
   int32_t level_kind;      /* pixel kind of the finest level */

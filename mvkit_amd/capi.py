@@ -35,7 +35,7 @@ PUBLIC_SYMBOLS = [
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
                   "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma",
-                  "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_full_score", "ncchip_refine"]
+                  "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine"]
 
 
 class StringView(C.Structure):
@@ -54,7 +54,7 @@ class FindOptions(C.Structure):
     _fields_ = [("angle_start", C.c_double), ("angle_extent", C.c_double), ("min_score", C.c_double),
                 ("max_overlap", C.c_double), ("num_matches", C.c_int64), ("subpixel", C.c_int32),
                 ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
-                ("screen", C.c_double), ("profile", C.c_int32)]
+                ("screen", C.c_double), ("profile", C.c_int32), ("vector_tracking", C.c_int32)]
 
 
 class FindStats(C.Structure):
@@ -329,12 +329,14 @@ class NccModel:
 
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
-                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0) -> FindOptions:
+                highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0,
+                vector_tracking=False) -> FindOptions:
         """profile: True times every stage, an int is a mask of stage bits
-        (bit i = STAGES[i])."""
+        (bit i = STAGES[i]).  vector_tracking: track every level with the FP64
+        vector kernel instead of matrix cores (same results)."""
         mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
-                           lowest, form, screen, mask)
+                           lowest, form, screen, mask, int(vector_tracking))
 
     def find_device(self, d_image: int, width: int, height: int, pitch: int, opts: FindOptions, cap: int = 4096):
         """Find on a device-resident u8 image (integer device address)."""

@@ -34,9 +34,20 @@ typedef struct NccPatch {
   int32_t run_off, n_runs;   // into runs (int4: row, begin, end, 0)
   int32_t wrun_off, n_wruns; // into wruns (int4) / wrun_w (double)
   int32_t use_wruns;         // 2*#weight_runs < #samples (kernels_avx2.cpp:150, 175, 211)
-  int32_t dense_off;         // dense bbox weights for the correlation kernels (-1: none)
-  int32_t dense_h, dense_w;  // bbox rows / padded columns of the dense block
+  // Split-BF16 dense raster for the MFMA tracking kernel (element offset
+  // into dense_hi / dense_lo, -1: none): rows rmin..rmax, columns
+  // cmin-8 .. cmax+8 (zero padded, dense_w a multiple of 8), stored eight
+  // times, copy c shifted left by c elements, so any 8-element run is one
+  // aligned 16-byte load from copy (start mod 8).
+  int32_t dense_off;
+  int32_t dense_h, dense_w;
   int32_t pad;
+  // |sum w x - split-BF16 FP32 dot| <= (4^l - 1) err_lo + max|x| err_x for
+  // x = S - dc 4^l split as in the tracking kernel (hi part multiple of 4^l,
+  // lo part in [0, 4^l)): err_lo = sum|w_lo| + g sum|w_hi|,
+  // err_x = sum|w - w_hi - w_lo| + g sum(|w_hi| + |w_lo|), g = k u / (1 - k u),
+  // k = 3n + 8 FP32 additions, u = 2^-23.
+  double err_lo, err_x;
 } NccPatch;
 
 // Device-resident pool of patches (all pointers are device pointers).
@@ -48,9 +59,8 @@ typedef struct NccPatchPool {
   const int32_t* runs;      // 4 ints per run
   const int32_t* wruns;     // 4 ints per weight run
   const double* wrun_w;
-  const float* dense_f32;   // dense bbox weights, fp32
-  const uint16_t* dense_hi; // dense bbox weights, bf16 high part
-  const uint16_t* dense_lo; // dense bbox weights, bf16 low part (w - hi)
+  const uint16_t* dense_hi; // dense rasters (NccPatch::dense_off), bf16 bits of w
+  const uint16_t* dense_lo; // bf16 bits of w - bf16(w)
 } NccPatchPool;
 
 // A pose in the reference's internal (centroid) convention (model.hpp:26).

@@ -308,6 +308,7 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
   q.subpixel = o->subpixel != 0;
   q.highest = o->highest_level; q.lowest = std::max(1, o->lowest_level);
   q.form = o->form; q.screen = o->screen; q.profile = o->profile;
+  q.track_mfma = o->vector_tracking == 0;
   if (q.form < MVAMD_FORM_FP32 || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
   return q;

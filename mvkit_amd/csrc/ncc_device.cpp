@@ -26,10 +26,35 @@ double now_us() {
   return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Dense split-BF16 rasters are built for rasters whose padded bounding box
+// holds at most this many elements (every tracking level of templates up to
+// several hundred pixels; the level-0 rasters of large templates are left out).
+constexpr size_t kDenseMaxElems = size_t(1) << 16;
+
+struct DenseGeom { int32_t h = 0, w = 0; size_t elems = 0; };
+DenseGeom dense_geom(const Raster& r) {
+  DenseGeom g;
+  g.h = r.rmax - r.rmin + 1;
+  g.w = ((r.cmax - r.cmin + 1) + 16 + 7) & ~7;  // rows of whole 16-byte groups
+  const size_t e = size_t(g.h) * size_t(g.w);
+  if (r.count == 0 || e > kDenseMaxElems) return DenseGeom{};
+  g.elems = 8 * e;  // eight copies
+  return g;
+}
+
 // Byte image of a raster pool: contiguous, 256-byte aligned sections.
 struct PoolLayout {
-  size_t patches = 0, rc = 0, w = 0, wf = 0, runs = 0, wruns = 0, wrun_w = 0, total = 0;
+  size_t patches = 0, rc = 0, w = 0, wf = 0, runs = 0, wruns = 0, wrun_w = 0, dense_hi = 0, dense_lo = 0, total = 0;
+  std::vector<int64_t> dense_off;  // per raster, elements (-1: none)
+  size_t dense_elems = 0;
   explicit PoolLayout(const RasterPool& p) {
+    dense_off.assign(p.rasters.size(), -1);
+    for (size_t i = 0; i < p.rasters.size(); ++i) {
+      const DenseGeom g = dense_geom(p.rasters[i]);
+      if (!g.elems || dense_elems + g.elems > size_t(INT32_MAX)) continue;
+      dense_off[i] = int64_t(dense_elems);
+      dense_elems += g.elems;
+    }
     size_t o = 0;
     patches = o; o = align_up(o + p.rasters.size() * sizeof(NccPatch));
     rc = o; o = align_up(o + p.rc.size() * 4);
@@ -38,9 +63,48 @@ struct PoolLayout {
     runs = o; o = align_up(o + p.runs.size() * 4);
     wruns = o; o = align_up(o + p.wruns.size() * 4);
     wrun_w = o; o = align_up(o + p.wrun_w.size() * 8);
+    dense_hi = o; o = align_up(o + dense_elems * 2);
+    dense_lo = o; o = align_up(o + dense_elems * 2);
     total = std::max<size_t>(o, 256);
   }
 };
+
+uint16_t bf16_bits(double v);
+double bf16_value(uint16_t h);
+
+// Fills raster r's two dense copies and its split-BF16 error coefficients.
+void dense_raster(const RasterPool& p, const Raster& r, uint16_t* hi, uint16_t* lo, NccPatch& P) {
+  const DenseGeom g = dense_geom(r);
+  const size_t plane = size_t(g.h) * size_t(g.w);
+  std::fill(hi, hi + 2 * plane, uint16_t(0));
+  std::fill(lo, lo + 2 * plane, uint16_t(0));
+  double wh1 = 0, wl1 = 0, res1 = 0;
+  for (uint32_t i = 0; i < r.count; ++i) {
+    const int32_t rc = p.rc[r.first + i];
+    const int32_t row = rc >> 16, col = int32_t(int16_t(uint16_t(uint32_t(rc) & 0xffffu)));
+    const double w = p.w[r.first + i];
+    const uint16_t h = bf16_bits(w);
+    const uint16_t l = bf16_bits(w - bf16_value(h));
+    const size_t e = size_t(row - r.rmin) * size_t(g.w) + size_t(col - r.cmin + 8);
+    hi[e] = h;
+    lo[e] = l;
+    wh1 += std::abs(bf16_value(h));
+    wl1 += std::abs(bf16_value(l));
+    res1 += std::abs(w - bf16_value(h) - bf16_value(l));
+  }
+  for (size_t c = 1; c < 8; ++c)  // copy c = copy 0 shifted left by c elements
+    for (size_t e = 0; e < plane; ++e) {
+      hi[c * plane + e] = e + c < plane ? hi[e + c] : uint16_t(0);
+      lo[c * plane + e] = e + c < plane ? lo[e + c] : uint16_t(0);
+    }
+  const double k = 3.0 * r.count + 8, u = std::ldexp(1.0, -23);
+  const double gam = k * u / (1 - k * u);
+  // the sums above round; a relative slack covers them
+  P.err_lo = (wl1 + gam * wh1) * (1 + 1e-9);
+  P.err_x = (res1 + gam * (wh1 + wl1)) * (1 + 1e-9) + 1e-300;
+  P.dense_h = g.h;
+  P.dense_w = g.w;
+}
 
 NccPatch describe(const Raster& r) {
   NccPatch P{};
@@ -56,7 +120,15 @@ NccPatch describe(const Raster& r) {
 
 void serialize(const RasterPool& p, const PoolLayout& L, uint8_t* dst) {
   auto* patches = reinterpret_cast<NccPatch*>(dst + L.patches);
-  for (size_t i = 0; i < p.rasters.size(); ++i) patches[i] = describe(p.rasters[i]);
+  auto* dhi = reinterpret_cast<uint16_t*>(dst + L.dense_hi);
+  auto* dlo = reinterpret_cast<uint16_t*>(dst + L.dense_lo);
+  for (size_t i = 0; i < p.rasters.size(); ++i) {
+    patches[i] = describe(p.rasters[i]);
+    if (L.dense_off[i] >= 0) {
+      patches[i].dense_off = int32_t(L.dense_off[i]);
+      dense_raster(p, p.rasters[i], dhi + L.dense_off[i], dlo + L.dense_off[i], patches[i]);
+    }
+  }
   std::memcpy(dst + L.rc, p.rc.data(), p.rc.size() * 4);
   std::memcpy(dst + L.w, p.w.data(), p.w.size() * 8);
   std::memcpy(dst + L.wf, p.wf.data(), p.wf.size() * 4);
@@ -74,6 +146,8 @@ NccPatchPool view_at(uint8_t* base, const PoolLayout& L) {
   v.runs = reinterpret_cast<const int32_t*>(base + L.runs);
   v.wruns = reinterpret_cast<const int32_t*>(base + L.wruns);
   v.wrun_w = reinterpret_cast<const double*>(base + L.wrun_w);
+  v.dense_hi = reinterpret_cast<const uint16_t*>(base + L.dense_hi);
+  v.dense_lo = reinterpret_cast<const uint16_t*>(base + L.dense_lo);
   return v;
 }
 
@@ -761,7 +835,34 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     const double xmax = 255. * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
     time_begin(kStageTrack, stream_);
-    if (ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
+    // matrix cores where every query patch has a dense raster and at most 32
+    // patches fall in any candidate's angle window
+    bool mfma = o.track_mfma && l - 1 >= 1 && l - 1 <= 4 && jobs[0].lv[l - 1].kind == NCC_U16;
+    if (mfma) {
+      std::vector<double> ang;
+      for (const int32_t r : refs[l - 1]) {
+        mfma &= dense_geom(raster_of(r)).elems > 0;
+        ang.push_back(raster_of(r).angle);
+      }
+      std::sort(ang.begin(), ang.end());
+      for (size_t i = 0, j = 0; mfma && i < ang.size(); ++i) {
+        while (j < ang.size() && ang[j] <= ang[i] + 2 * tol) ++j;
+        mfma &= j - i <= 32;
+      }
+    }
+    int rc_track = -6;
+    if (mfma)
+      rc_track = ncchip_track_mfma(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
+                                   reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
+                                   lists + k * cap_all, counts + k * n_frames, int(cap),
+                                   reinterpret_cast<NccMatch*>(dev + off_raw), lists + (k + 1) * cap_all,
+                                   counts + (k + 1) * n_frames,
+                                   reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size,
+                                   int(table_size), o.min_score * .4, tol, step, offset, l - 1 <= 2,
+                                   model.params.absolute, int(std::ceil(hi_r - lo_r)) + 7,
+                                   int(std::ceil(hi_c - lo_c)) + 7, xmax, work + k, kListGrid, stream_);
+    if (rc_track == -6 &&
+        ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                      reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                      lists + k * cap_all, counts + k * n_frames, int(cap), reinterpret_cast<NccMatch*>(dev + off_raw),
                      lists + (k + 1) * cap_all, counts + (k + 1) * n_frames,
@@ -769,6 +870,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                      o.min_score * .4, tol, step, offset, l - 1 <= 2, model.params.absolute,
                      tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, stream_))
       raise(kInternalError, "device", "tracking launch failed");
+    if (rc_track != -6 && rc_track != 0) raise(kInternalError, "device", "tracking launch failed");
     time_end(stream_);
   }
   // ---- full score at the finest level (matcher.cpp:813-822)

@@ -73,6 +73,8 @@ struct FindOptions {
   // find_models over several models: the whole level 0 is the storage region
   // and the recovery windows use its row prefix statistics (matcher.cpp:596-611, 991-1003)
   bool whole_level0 = false;
+  // tracking of levels 1..4 on matrix cores (exact results; off: FP64 VALU kernel only)
+  bool track_mfma = true;
 };
 
 // Quota and overlap shared by all models of find_models (multi.hpp:14);

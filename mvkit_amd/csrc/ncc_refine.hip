@@ -5,6 +5,8 @@
 // coordinate, bilinear value and window moment is the reference's value.
 // Sums over samples are block tree reductions (the reference sums in 4 lanes),
 // so reduced totals may differ from the reference in the last bits.
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "ncc_common.hip.h"
@@ -53,6 +55,10 @@ struct TrackArgs {
   double tol, step, offset;
   int exact_lattice, absolute;
   int tile_cap;               // LDS tile capacity in pixels
+  // MFMA tracking (k_track_mfma): LDS tile bounds and carve-up
+  int mt_h, mt_w, mt_pitch;   // tile rows / columns, plane pitch (elements)
+  int mt_s, mt_p1, mt_p2, mt_pl;  // byte offsets of S, row prefixes, planes
+  double x_max;               // bound of |S - dc 4^l| at this level
   // the candidate's frame, filled in-kernel
   NccLevel lv;
   double dc, crop_r, crop_c;
@@ -294,6 +300,27 @@ __device__ __forceinline__ int eligible_patches(const Pools& pools, const int32_
   return min(n_e, kMaxEligible);
 }
 
+// eligible_patches over angles cached in LDS (ang[k] = angle of refs[k]).
+__device__ __forceinline__ int eligible_cached(const double* ang, int n_refs, double angle, double tol, int* elig,
+                                               int* wave_cnt) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  int n_e = 0;
+  for (int base = 0; base < n_refs; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    const bool f = k < n_refs && fabs(ang[k] - angle) <= tol;
+    const uint64_t m = __ballot(f);
+    const int pre = __popcll(m & ((uint64_t(1) << lane) - 1));
+    if (lane == 0) wave_cnt[wave] = __popcll(m);
+    __syncthreads();
+    int off = n_e;
+    for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+    if (f && off + pre < kMaxEligible) elig[off + pre] = k;
+    for (int w = 0; w < n_waves; ++w) n_e += wave_cnt[w];
+    __syncthreads();
+  }
+  return min(n_e, kMaxEligible);
+}
+
 // Post-tracking merge and filter of one frame (matcher.cpp:241-248, 758-760):
 // a tracked pose with a finite score >= keep is appended to the frame's
 // output list unless an identical (row, col, angle) is already there.
@@ -455,6 +482,450 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
     keep_distinct(a, f, ci, res);
   }
   __syncthreads();  // the LDS of this candidate is reused by the next
+  }
+}
+
+// ----------------------------------------------------------- MFMA tracking
+// The 25 windows x E eligible patches of one candidate as one GEMM:
+//   D[w][a] = sum over (Y, X) of x(U + Y + dr_w, V + X + dc_w) * W_a(Y, X)
+// where (U, V) is the kernel-space origin shared by all patches and W_a is
+// patch a's dense weight raster placed at its anchor.  M = 32 window slots
+// (25 used), N = 32 patch slots, K = 16 consecutive columns of one row;
+// v_mfma_f32_32x32x16_bf16 with x = x_hi + x_lo exact in BF16 (x_hi =
+// S - (S mod 4^l) - D, x_lo = S mod 4^l) and w = w_hi + w_lo, three products.
+// The four waves split K and their partial tiles are added in a fixed order.
+// Each window's approximate score comes with a bound on its error (the
+// patch's split and FP32 accumulation terms over the window's exact standard
+// deviation); the windows whose upper bound reaches the best lower bound are
+// re-scored exactly in FP64, and the first strict maximum among them in the
+// reference's (patch, dr, dc) order is the candidate's tracked pose, as in
+// k_track.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16_t;
+
+constexpr int kTrkSlots = 32;
+constexpr int kTrkAngCache = 128;
+
+__device__ __forceinline__ uint16_t bf16_exact_bits(int v) { return uint16_t(__float_as_uint(float(v)) >> 16); }
+
+// Exact FP64 score of one window (one wave, samples lane-strided).
+template <class Reader>
+__device__ __forceinline__ double exact_window_wave(const TrackArgs& a, const PatchRef& p, int row, int col,
+                                                    const Reader& rd) {
+  const NccPatch& P = p.P;
+  const int lane = threadIdx.x & 63;
+  int64_t t1 = 0, t2 = 0;
+  double dot = 0;
+  const bool dense_w = !P.use_wruns;
+  constexpr int U = 8;
+  for (int base = lane; base < P.n; base += 64 * U) {
+    int32_t rcv[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + 64 * u;
+      const int j = i < P.n ? i : base;
+      rcv[u] = p.rc[j];
+      wv[u] = i < P.n && dense_w ? p.w[j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t x = base + 64 * u < P.n
+                            ? int64_t(rd.at(row + sample_row(rcv[u]), col + sample_col(rcv[u]))) - a.D
+                            : 0;
+      t1 += x;
+      t2 += x * x;
+      dot = fma(double(x), wv[u], dot);
+    }
+  }
+  if (!dense_w) {
+    for (int j = lane; j < P.n_wruns; j += 64) {
+      const int4 run = reinterpret_cast<const int4*>(p.wruns)[j];
+      int64_t rs = 0;
+      for (int c = run.y; c <= run.z; ++c) rs += rd.at(row + run.x, col + c);
+      dot += double(rs - int64_t(run.z - run.y + 1) * a.D) * p.wrun_w[j];
+    }
+  }
+  t1 = wave_sum_i64(t1);
+  t2 = wave_sum_i64(t2);
+  dot = wave_sum(dot);
+  return window_score(a, P, ldexp(dot, -a.lv.shift), t1, t2);
+}
+
+// Block-wide (score, index) maximum, ties to the lower index; every thread
+// gets the result.
+__device__ __forceinline__ void block_argmax(double& s, int& i, double* ws, int* wi) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    const double o = __shfl_xor(s, m, 64);
+    const int oi = __shfl_xor(i, m, 64);
+    if (o > s || (o == s && oi < i)) { s = o; i = oi; }
+  }
+  if (lane == 0) { ws[wave] = s; wi[wave] = i; }
+  __syncthreads();
+  s = ws[0];
+  i = wi[0];
+  for (int w = 1; w < kTrackThreads / 64; ++w)
+    if (ws[w] > s || (ws[w] == s && wi[w] < i)) { s = ws[w]; i = wi[w]; }
+  __syncthreads();
+}
+
+template <class T, class P2>
+__global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs args) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t dyn[];
+  __shared__ int elig[kMaxEligible];
+  __shared__ int wave_cnt[kTrackThreads / 64];
+  __shared__ int box[4];
+  __shared__ int s_ref[kTrkSlots], s_row[kTrkSlots], s_col[kTrkSlots];
+  __shared__ NccPatch s_pat[kTrkSlots];        // the candidate's eligible patches
+  __shared__ double s_ang[kTrkAngCache];       // angle of every query patch of the level
+  __shared__ int s_dense;
+  __shared__ double red_s[kTrackThreads / 64];
+  __shared__ int red_i[kTrackThreads / 64];
+  __shared__ int off[NCC_MAX_BATCH + 1];
+  uint16_t* sS = reinterpret_cast<uint16_t*>(dyn + args.mt_s);
+  uint32_t* P1 = reinterpret_cast<uint32_t*>(dyn + args.mt_p1);
+  P2* Q2 = reinterpret_cast<P2*>(dyn + args.mt_p2);
+  uint16_t* th0 = reinterpret_cast<uint16_t*>(dyn + args.mt_pl);
+  const int PP = args.mt_pitch;
+  uint16_t* tl0 = th0 + args.mt_h * PP;
+  // after the MFMA loop the planes' space holds two partial tiles [2][32][32],
+  // then the summed tile [32][32] and the windows' upper bounds; the list of
+  // windows to re-score replaces the summed tile once the bounds are known
+  float* part = reinterpret_cast<float*>(dyn + args.mt_pl);
+  float* s_up = part + 1024;
+  int* s_list = reinterpret_cast<int*>(part);
+  list_offsets(args.in_count, args.n_frames, args.cap, off);
+  const bool ang_cached = args.n_refs <= kTrkAngCache;
+  if (ang_cached)
+    for (int k = threadIdx.x; k < args.n_refs; k += kTrackThreads) s_ang[k] = ref_angle(args.pools, args.refs[k]);
+  __syncthreads();
+  const int total = off[args.n_frames];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    TrackArgs a = args;
+    const int f = __builtin_amdgcn_readfirstlane(list_frame(off, a.n_frames, g));
+    const int ci = __builtin_amdgcn_readfirstlane(g - off[f]);
+    bind_frame(a, f);
+    const NccMatch cd = a.in[size_t(f) * a.cap + ci];
+    const int n_e = ang_cached ? eligible_cached(s_ang, a.n_refs, cd.angle, a.tol, elig, wave_cnt)
+                               : eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt);
+    if (threadIdx.x == 0) { box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; s_dense = 1; }
+    __syncthreads();
+    // one round of descriptor loads: each thread takes one eligible patch
+    unsigned long long samples = 0;
+    for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
+      const int32_t ref = a.refs[elig[e]];
+      const NccPatch P = a.pools.patch(ref);
+      samples += 25ull * uint32_t(P.n);
+      int row, col;
+      anchor(a, cd, P, row, col);
+      if (e < kTrkSlots) { s_ref[e] = ref; s_row[e] = row; s_col[e] = col; s_pat[e] = P; }
+      if (P.dense_off < 0) s_dense = 0;
+      atomicMin(&box[0], row - 2 + P.rmin);
+      atomicMax(&box[1], row + 2 + P.rmax);
+      atomicMin(&box[2], col - 2 + P.cmin);
+      atomicMax(&box[3], col + 2 + P.cmax);
+    }
+    if (a.work) {
+      samples = uint64_t(wave_sum_i64(int64_t(samples)));
+      if (lane == 0 && samples) atomicAdd(a.work, samples);
+    }
+    __syncthreads();
+    const int U0 = box[0], V0 = box[2];
+    const int th = box[1] - U0 + 1, tw = box[3] - V0 + 1;
+    const int KH = th - 4, XC = (tw - 4 + 15) / 16;
+    const bool fast = n_e > 0 && n_e <= kTrkSlots && s_dense && th <= a.mt_h && tw <= a.mt_w &&
+                      16 * XC + 5 <= PP && KH > 0;
+    double best = NCC_MISSING;
+    int best_i = 0x7fffffff;
+    bool n_e_skip = false;  // block-uniform
+    const T* lvp = static_cast<const T*>(a.lv.data);
+    if (!fast) {
+      // every window exactly, from the level in global memory
+      const LevelReader<T> rd{lvp, a.lv.pitch};
+      for (int k = wave; k < n_e * 25; k += kTrackThreads / 64) {
+        const int e = k / 25, wi = k % 25;
+        const int32_t ref = __builtin_amdgcn_readfirstlane(a.refs[elig[e]]);
+        const PatchRef p = patch_ref(a.pools, ref);
+        int row, col;
+        anchor(a, cd, p.P, row, col);
+        row += wi / 5 - 2;
+        col += wi % 5 - 2;
+        if (!window_inside(a, p.P, row, col)) continue;
+        const double sc = exact_window_wave(a, p, row, col, rd);
+        if (sc > best) { best = sc; best_i = k; }
+      }
+    } else {
+      // ---- stage S, the split planes (zero past the tile) and the row prefixes
+      const int shift = a.lv.shift, lo_mask = (1 << shift) - 1;
+      const int Di = int(a.D);
+      const int vr0 = max(0, a.sr), vc0 = max(0, a.sc);
+      const int vr1 = min(a.lv.height, a.sr + a.sh), vc1 = min(a.lv.width, a.sc + a.sw);
+      {
+        constexpr int U = 16;  // loads in flight per thread
+        for (int base = threadIdx.x; base < th * PP; base += kTrackThreads * U) {
+          int Sv[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int i = base + kTrackThreads * u;
+            const int tr = i / PP, tc = i - tr * PP;
+            const int r = U0 + tr, c = V0 + tc;
+            Sv[u] = i < th * PP && tc < tw && r >= vr0 && r < vr1 && c >= vc0 && c < vc1
+                        ? int(lvp[size_t(r) * a.lv.pitch + c])
+                        : Di;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int i = base + kTrackThreads * u;
+            if (i >= th * PP) break;
+            const int tr = i / PP, tc = i - tr * PP;
+            const int S = Sv[u];
+            if (tc < tw) sS[tr * tw + tc] = uint16_t(S);
+            const int lo = S & lo_mask;
+            const uint16_t hb = bf16_exact_bits(S - lo - Di), lb = bf16_exact_bits(lo);
+            th0[tr * PP + tc] = hb;
+            tl0[tr * PP + tc] = lb;
+          }
+        }
+      }
+      __syncthreads();
+      // row prefixes of S and S^2: one thread per row, loads in groups of 8
+      for (int tr = threadIdx.x; tr < th; tr += kTrackThreads) {
+        uint32_t s1 = 0;
+        P2 s2 = 0;
+        uint32_t* p1 = P1 + tr * (tw + 1);
+        P2* p2 = Q2 + tr * (tw + 1);
+        const uint16_t* src = sS + tr * tw;
+        p1[0] = 0;
+        p2[0] = 0;
+        for (int c0 = 0; c0 < tw; c0 += 8) {
+          uint32_t v[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[j] = c0 + j < tw ? src[c0 + j] : 0u;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            if (c0 + j >= tw) break;
+            s1 += v[j];
+            s2 += P2(v[j]) * P2(v[j]);
+            p1[c0 + j + 1] = s1;
+            p2[c0 + j + 1] = s2;
+          }
+        }
+      }
+      // ---- MFMA: lane (m, gg) = window slot m / patch slot m, K half gg
+      const int m = lane & 31, gg = lane >> 5;
+      const int wsl = min(m, 24);
+      const int arow = wsl / 5, acol = 8 * gg + wsl % 5;
+      const bool has = m < n_e;
+      const NccPatch* Pm = has ? &s_pat[m] : nullptr;
+      const NccPatchPool& pool_m = a.pools.of(has ? s_ref[m] : 0);
+      const int dh = has ? Pm->dense_h : 0, dw = has ? Pm->dense_w : 0;
+      const int plane_b = dh * dw;
+      const uint16_t* bh_base = has ? pool_m.dense_hi + Pm->dense_off : nullptr;
+      const uint16_t* bl_base = has ? pool_m.dense_lo + Pm->dense_off : nullptr;
+      const int ybase = has ? U0 + 2 - s_row[m] - Pm->rmin : 0;
+      const int xbase = has ? V0 + 2 - s_col[m] - Pm->cmin + 8 + 8 * gg : 0;
+      floatx16_t acc;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+      // this wave's band of K rows; iteration t covers (Y, cx) = (y_lo + t / XC, t % XC)
+      const int y_lo = (KH * wave) / 4, y_hi = (KH * (wave + 1)) / 4;
+      const int n_my = (y_hi - y_lo) * XC;
+      constexpr int kRing = 4;  // weight fragments in flight per lane
+      // Loads are issued unconditionally (clamped addresses, results masked),
+      // so the compiler can count the ones in flight and wait only for the
+      // oldest.  Lanes without a patch read slot 0's raster, unused.
+      const uint16_t* bh_safe = has ? bh_base : a.pools.of(s_ref[0]).dense_hi + s_pat[0].dense_off;
+      const uint16_t* bl_safe = has ? bl_base : a.pools.of(s_ref[0]).dense_lo + s_pat[0].dense_off;
+      auto load_b = [&](bool ok, int Y, int cx, uint4& bh4, uint4& bl4) __attribute__((always_inline)) {
+        const int y = Y + ybase, x0 = 16 * cx + xbase;
+        const bool in = ok && has && y >= 0 && y < dh && x0 >= 0 && x0 + 8 <= dw;
+        const int e = y * dw + x0;
+        const int o = in ? (e & 7) * plane_b + (e & ~7) : 0;
+        const uint4 h = *reinterpret_cast<const uint4*>(bh_safe + o);
+        const uint4 l = *reinterpret_cast<const uint4*>(bl_safe + o);
+        const uint32_t mk = in ? 0xffffffffu : 0u;
+        bh4 = make_uint4(h.x & mk, h.y & mk, h.z & mk, h.w & mk);
+        bl4 = make_uint4(l.x & mk, l.y & mk, l.z & mk, l.w & mk);
+      };
+      // 8 elements from any column: five aligned dwords, shifted by one
+      // element (alignbyte) when the column is odd
+      auto load_a = [&](int Y, int cx, uint4& ah4, uint4& al4) __attribute__((always_inline)) {
+        const int col = 16 * cx + acol;
+        const int sh = (col & 1) * 2;
+        const int row = min(Y, KH - 1) + arow;  // a prefetch past the band reads a valid row, unused
+        const uint32_t* ah = reinterpret_cast<const uint32_t*>(th0 + row * PP + (col & ~1));
+        const uint32_t* al = reinterpret_cast<const uint32_t*>(tl0 + row * PP + (col & ~1));
+        uint32_t h[5], l[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) { h[j] = ah[j]; l[j] = al[j]; }
+        ah4 = make_uint4(__builtin_amdgcn_alignbyte(h[1], h[0], sh), __builtin_amdgcn_alignbyte(h[2], h[1], sh),
+                         __builtin_amdgcn_alignbyte(h[3], h[2], sh), __builtin_amdgcn_alignbyte(h[4], h[3], sh));
+        al4 = make_uint4(__builtin_amdgcn_alignbyte(l[1], l[0], sh), __builtin_amdgcn_alignbyte(l[2], l[1], sh),
+                         __builtin_amdgcn_alignbyte(l[3], l[2], sh), __builtin_amdgcn_alignbyte(l[4], l[3], sh));
+      };
+      auto step = [&](int& Y, int& cx) __attribute__((always_inline)) {
+        if (++cx == XC) { cx = 0; ++Y; }
+      };
+      uint4 rh[kRing], rl[kRing];
+      int pY = y_lo, pcx = 0;  // prefetch cursor
+#pragma unroll
+      for (int j = 0; j < kRing; ++j) {
+        load_b(j < n_my, pY, pcx, rh[j], rl[j]);
+        step(pY, pcx);
+      }
+      int Y = y_lo, cx = 0;
+      uint4 a_h, a_l;
+      load_a(Y, cx, a_h, a_l);
+      for (int t0 = 0; t0 < n_my; t0 += kRing) {
+#pragma unroll
+        for (int j = 0; j < kRing; ++j) {
+          const int t = t0 + j;
+          if (t >= n_my) break;
+          int Yn = Y, cxn = cx;
+          step(Yn, cxn);
+          uint4 n_h, n_l;
+          load_a(Yn, cxn, n_h, n_l);
+          const bf16x8_t A_h = __builtin_bit_cast(bf16x8_t, a_h), A_l = __builtin_bit_cast(bf16x8_t, a_l);
+          const bf16x8_t B_h = __builtin_bit_cast(bf16x8_t, rh[j]), B_l = __builtin_bit_cast(bf16x8_t, rl[j]);
+          load_b(t + kRing < n_my, pY, pcx, rh[j], rl[j]);  // the ring slot's next use
+          step(pY, pcx);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_h, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_l, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_l, B_h, acc, 0, 0, 0);
+          a_h = n_h;
+          a_l = n_l;
+          Y = Yn;
+          cx = cxn;
+        }
+      }
+      __syncthreads();  // the planes are dead: their space takes the partial tiles
+      // C layout: lane (column m, half gg), register v: row 8 (v / 4) + 4 gg + v % 4.
+      // The four partial tiles add as (w0 + w1) + (w2 + w3).
+      auto cidx = [&](int v) __attribute__((always_inline)) { return (8 * (v / 4) + 4 * gg + (v % 4)) * 32 + m; };
+      if (wave & 1) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) part[(wave >> 1) * 1024 + cidx(v)] = acc[v];
+      }
+      __syncthreads();
+      if (!(wave & 1)) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] += part[(wave >> 1) * 1024 + cidx(v)];
+      }
+      __syncthreads();
+      if (wave == 2) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) part[cidx(v)] = acc[v];
+      }
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) part[cidx(v)] = acc[v] + part[cidx(v)];
+      }
+      __syncthreads();
+      // ---- approximate scores with error bounds; the best lower bound
+      const double unit = ldexp(1.0, -a.lv.shift);
+      const double err_scale = unit;
+      double lower_max = NCC_MISSING;
+      for (int k = threadIdx.x; k < n_e * 25; k += kTrackThreads) {
+        const int e = k / 25, wi = k - e * 25;
+        const NccPatch& P = s_pat[e];
+        const int row = s_row[e] + wi / 5 - 2, col = s_col[e] + wi % 5 - 2;
+        float up = -__builtin_inff();
+        if (window_inside(a, P, row, col)) {
+          const int4* runs = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs) + P.run_off;
+          int64_t m1 = 0, m2 = 0;
+#pragma unroll 16
+          for (int j = 0; j < P.n_runs; ++j) {
+            const int4 run = runs[j];
+            const int tr = row + run.x - U0;
+            const int c0 = col + run.y - V0, c1 = col + run.z + 1 - V0;
+            m1 += int64_t(P1[tr * (tw + 1) + c1]) - int64_t(P1[tr * (tw + 1) + c0]);
+            m2 += int64_t(Q2[tr * (tw + 1) + c1]) - int64_t(Q2[tr * (tw + 1) + c0]);
+          }
+          const int64_t n = P.n;
+          const int64_t t1 = m1 - n * a.D;
+          const int64_t t2 = m2 - 2 * a.D * m1 + n * a.D * a.D;
+          const double sum = ldexp(double(t1), -a.lv.shift), sq = ldexp(double(t2), -2 * a.lv.shift);
+          const double var = sq - sum * sum / double(P.n);
+          if (!(var > 128 * __DBL_EPSILON__ * sq)) {
+            // low variance: -inf on an exact lattice, else only the exact form decides
+            up = a.exact_lattice ? -__builtin_inff() : __builtin_inff();
+          } else {
+            const double dot = double(part[wi * 32 + e]) * unit;
+            const double sc = window_score(a, P, dot, t1, t2);
+            const double errd = (double(lo_mask) * P.err_lo + a.x_max * P.err_x) * err_scale;
+            const double beta = errd / sqrt(var) * (1 + 1e-6) + 1e-12;
+            lower_max = fmax(lower_max, sc - beta);
+            up = __double2float_ru(sc + beta);
+          }
+        }
+        s_up[k] = up;
+      }
+      int dummy = 0;
+      block_argmax(lower_max, dummy, red_s, red_i);
+      const double L = lower_max;
+      // a candidate whose windows all stay below the keep threshold is dropped
+      // whichever window is the maximum: no exact scoring needed
+      {
+        double um = NCC_MISSING;
+        for (int k = threadIdx.x; k < n_e * 25; k += kTrackThreads) um = fmax(um, double(s_up[k]));
+        int d2 = 0;
+        block_argmax(um, d2, red_s, red_i);
+        if (um < a.keep) n_e_skip = true;
+      }
+      // ---- the windows that may hold the maximum, in (patch, dr, dc) order
+      int n_list = 0;
+      for (int base = 0; !n_e_skip && base < n_e * 25; base += kTrackThreads) {
+        const int k = base + threadIdx.x;
+        const bool take = k < n_e * 25 && s_up[k] > -__builtin_inff() && double(s_up[k]) >= L;
+        const uint64_t mk = __ballot(take);
+        const int pre = __popcll(mk & ((uint64_t(1) << lane) - 1));
+        if (lane == 0) wave_cnt[wave] = __popcll(mk);
+        __syncthreads();
+        int o = n_list;
+        for (int w = 0; w < wave; ++w) o += wave_cnt[w];
+        if (take) s_list[o + pre] = k;
+        for (int w = 0; w < kTrackThreads / 64; ++w) n_list += wave_cnt[w];
+        __syncthreads();
+      }
+      const TileReader<uint16_t> rd{sS, U0, V0, tw};
+      for (int j = wave; j < n_list; j += kTrackThreads / 64) {
+        const int k = __builtin_amdgcn_readfirstlane(s_list[j]);
+        const int e = k / 25, wi = k - e * 25;
+        const int32_t ref = __builtin_amdgcn_readfirstlane(s_ref[e]);
+        const NccPatchPool& pool = a.pools.of(ref);
+        PatchRef p;
+        p.P = s_pat[e];
+        p.rc = pool.rc + p.P.sample_off;
+        p.w = pool.w + p.P.sample_off;
+        p.wruns = pool.wruns + 4 * p.P.wrun_off;
+        p.wrun_w = pool.wrun_w + p.P.wrun_off;
+        const int row = s_row[e] + wi / 5 - 2, col = s_col[e] + wi % 5 - 2;
+        const double sc = exact_window_wave(a, p, row, col, rd);
+        if (sc > best) { best = sc; best_i = k; }
+      }
+    }
+    block_argmax(best, best_i, red_s, red_i);
+    if (threadIdx.x == 0) {
+      NccMatch res = cd;
+      res.score = NCC_MISSING;
+      if (best > NCC_MISSING) {
+        const int e = best_i / 25;
+        const NccPatch P = e < kTrkSlots ? s_pat[e] : a.pools.patch(a.refs[elig[e]]);
+        int row, col;
+        anchor(a, cd, P, row, col);
+        res.row = a.crop_r + a.step * (row + (best_i % 25) / 5 - 2 + P.orow) + a.offset;
+        res.col = a.crop_c + a.step * (col + best_i % 5 - 2 + P.ocol) + a.offset;
+        res.angle = P.angle;
+        res.score = best;
+      }
+      keep_distinct(a, f, ci, res);
+    }
+    __syncthreads();  // LDS is reused by the next candidate
   }
 }
 
@@ -1080,6 +1551,39 @@ extern "C" int ncchip_track(const NccFrame* frames, int n_frames, int level, int
     if (exact_f64) hipLaunchKernelGGL((k_track<uint32_t, double>), g, b, lds, st, a);
     else hipLaunchKernelGGL((k_track<uint32_t, int64_t>), g, b, lds, st, a);
   }
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_track_mfma(const NccFrame* frames, int n_frames, int level, int level_kind,
+                                 const NccPatchPool* pools, const int32_t* refs, int n_refs, const NccMatch* in,
+                                 const int32_t* in_count, int cap, NccMatch* raw, NccMatch* out, int32_t* out_count,
+                                 int32_t* table, int table_size, double keep, double tol, double step, double offset,
+                                 int exact_lattice, int absolute, int tile_h, int tile_w, double x_max,
+                                 unsigned long long* work, int grid, void* stream) {
+  if (n_frames <= 0 || grid <= 0) return 0;
+  if (n_frames > NCC_MAX_BATCH || table_size < 2 || (table_size & (table_size - 1))) return -1;
+  // the exact BF16 split of S - dc 4^l holds for 16-bit block sums up to level 4
+  if (level_kind != NCC_U16 || level < 1 || level > 4 || tile_h < 5 || tile_w < 5) return -6;
+  TrackArgs a{};
+  a.frames = frames; a.n_frames = n_frames; a.level = level; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
+  a.refs = refs; a.n_refs = n_refs; a.in = in; a.in_count = in_count; a.cap = cap; a.raw = raw; a.out = out;
+  a.out_count = out_count; a.table = table; a.table_size = table_size; a.keep = keep; a.work = work;
+  a.tol = tol; a.step = step; a.offset = offset; a.exact_lattice = exact_lattice; a.absolute = absolute;
+  a.x_max = x_max;
+  const int xc = (tile_w - 4 + 15) / 16;
+  a.mt_h = tile_h; a.mt_w = tile_w; a.mt_pitch = (16 * xc + 6) & ~1;
+  auto al16 = [](size_t v) { return (v + 15) & ~size_t(15); };
+  // sum of S^2 over a tile row in 32 bits when it cannot overflow
+  const bool p2_32 = 255.0 * 255.0 * std::ldexp(1.0, 4 * level) * double(tile_w + 1) < 4294967295.0;
+  size_t o = 0;
+  a.mt_s = int(o); o = al16(o + size_t(tile_h) * tile_w * 2);
+  a.mt_p1 = int(o); o = al16(o + size_t(tile_h) * (tile_w + 1) * 4);
+  a.mt_p2 = int(o); o = al16(o + size_t(tile_h) * (tile_w + 1) * (p2_32 ? 4 : 8));
+  a.mt_pl = int(o); o = al16(o + std::max<size_t>(size_t(2) * tile_h * a.mt_pitch * 2, 2 * 32 * 32 * 4));
+  if (o > 64 * 1024) return -6;  // at least two workgroups per CU
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (p2_32) hipLaunchKernelGGL((k_track_mfma<uint16_t, uint32_t>), dim3(grid), dim3(kTrackThreads), o, st, a);
+  else hipLaunchKernelGGL((k_track_mfma<uint16_t, uint64_t>), dim3(grid), dim3(kTrackThreads), o, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
 

@@ -360,3 +360,22 @@ def test_overlapped_batches_match_single_batches(capi):
         assert len(got) == len(want)
         for g, x in zip(got, want):
             assert np.array_equal(g, x)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_matrix_tracking_equals_vector_tracking(capi, request, case):
+    """Tracking on matrix cores (split-BF16 screen with exact FP64 re-scoring
+    of every window that can be the maximum) gives the FP64 vector kernel's
+    tracked candidates: identical poses, and full scores of those poses."""
+    label, fixture, levels, start, extent, ms, nm = case
+    scene = request.getfixturevalue(fixture)
+    say(f"matrix vs vector tracking {label}")
+    m = capi.NccModel(scene.template, levels if levels else "auto", start, extent)
+    fast = m.find_trace(scene.image, capi.NccModel.options(start, extent, ms, nm, 0.5))
+    slow = m.find_trace(scene.image, capi.NccModel.options(start, extent, ms, nm, 0.5, vector_tracking=True))
+    assert len(fast["prepared"]) == len(slow["prepared"])
+    assert np.array_equal(fast["prepared"][:, :3], slow["prepared"][:, :3])
+    fin = np.isfinite(slow["prepared"][:, 3])
+    assert np.array_equal(np.isfinite(fast["prepared"][:, 3]), fin)
+    assert np.array_equal(fast["prepared"][fin, 3], slow["prepared"][fin, 3])
+    assert np.array_equal(fast["results"], slow["results"])
