@@ -505,6 +505,7 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16_t;
 
 constexpr int kTrkSlots = 32;
 constexpr int kTrkAngCache = 128;
+constexpr int kTrkRunCache = 512;
 
 __device__ __forceinline__ uint16_t bf16_exact_bits(int v) { return uint16_t(__float_as_uint(float(v)) >> 16); }
 
@@ -580,6 +581,10 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
   __shared__ int s_ref[kTrkSlots], s_row[kTrkSlots], s_col[kTrkSlots];
   __shared__ NccPatch s_pat[kTrkSlots];        // the candidate's eligible patches
   __shared__ double s_ang[kTrkAngCache];       // angle of every query patch of the level
+  __shared__ int32_t s_refs[kTrkAngCache];     // and its (pool << 24 | index)
+  __shared__ int2 s_runs[kTrkRunCache];        // runs of the eligible patches, slot after slot:
+                                               // (row, begin | end << 16)
+  __shared__ int s_run0[kTrkSlots + 1];        // first run of each slot in s_runs
   __shared__ int s_dense;
   __shared__ double red_s[kTrackThreads / 64];
   __shared__ int red_i[kTrackThreads / 64];
@@ -599,7 +604,11 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
   list_offsets(args.in_count, args.n_frames, args.cap, off);
   const bool ang_cached = args.n_refs <= kTrkAngCache;
   if (ang_cached)
-    for (int k = threadIdx.x; k < args.n_refs; k += kTrackThreads) s_ang[k] = ref_angle(args.pools, args.refs[k]);
+    for (int k = threadIdx.x; k < args.n_refs; k += kTrackThreads) {
+      const int32_t r = args.refs[k];
+      s_refs[k] = r;
+      s_ang[k] = ref_angle(args.pools, r);
+    }
   __syncthreads();
   const int total = off[args.n_frames];
   const int lane = threadIdx.x & 63;
@@ -617,7 +626,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
     // one round of descriptor loads: each thread takes one eligible patch
     unsigned long long samples = 0;
     for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
-      const int32_t ref = a.refs[elig[e]];
+      const int32_t ref = ang_cached ? s_refs[elig[e]] : a.refs[elig[e]];
       const NccPatch P = a.pools.patch(ref);
       samples += 25ull * uint32_t(P.n);
       int row, col;
@@ -634,6 +643,11 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       if (lane == 0 && samples) atomicAdd(a.work, samples);
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int e = 0; e < min(n_e, kTrkSlots); ++e) { s_run0[e] = t; t += s_pat[e].n_runs; }
+      s_run0[min(n_e, kTrkSlots)] = t;
+    }
     const int U0 = box[0], V0 = box[2];
     const int th = box[1] - U0 + 1, tw = box[3] - V0 + 1;
     const int KH = th - 4, XC = (tw - 4 + 15) / 16;
@@ -664,6 +678,16 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int Di = int(a.D);
       const int vr0 = max(0, a.sr), vc0 = max(0, a.sc);
       const int vr1 = min(a.lv.height, a.sr + a.sh), vc1 = min(a.lv.width, a.sc + a.sw);
+      __syncthreads();  // s_run0
+      const bool runs_cached = s_run0[n_e] <= kTrkRunCache;
+      if (runs_cached)
+        for (int e = 0; e < n_e; ++e) {
+          const int4* src = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs) + s_pat[e].run_off;
+          for (int j = threadIdx.x; j < s_pat[e].n_runs; j += kTrackThreads) {
+            const int4 r = src[j];
+            s_runs[s_run0[e] + j] = make_int2(r.x, (r.y & 0xffff) | (r.z << 16));
+          }
+        }
       {
         constexpr int U = 16;  // loads in flight per thread
         for (int base = threadIdx.x; base < th * PP; base += kTrackThreads * U) {
@@ -837,10 +861,17 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
         float up = -__builtin_inff();
         if (window_inside(a, P, row, col)) {
           const int4* runs = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs) + P.run_off;
+          const int2* cruns = s_runs + s_run0[e];
           int64_t m1 = 0, m2 = 0;
-#pragma unroll 16
+#pragma unroll 8
           for (int j = 0; j < P.n_runs; ++j) {
-            const int4 run = runs[j];
+            int4 run;
+            if (runs_cached) {
+              const int2 c = cruns[j];
+              run = make_int4(c.x, int(int16_t(c.y & 0xffff)), c.y >> 16, 0);
+            } else {
+              run = runs[j];
+            }
             const int tr = row + run.x - U0;
             const int c0 = col + run.y - V0, c1 = col + run.z + 1 - V0;
             m1 += int64_t(P1[tr * (tw + 1) + c1]) - int64_t(P1[tr * (tw + 1) + c0]);
