@@ -42,6 +42,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# the HIP kernel behind each reported stage (one launch per batch)
+HIP_KERNELS = {("track_mfma", f): "ncc::k_track_mfma" for f in range(4)}
+HIP_KERNELS.update({("coarse", 0): "ncc::k_coarse_fp32", ("coarse", 1): "ncc::k_coarse_mfma",
+                    ("coarse", 2): "ncc::k_coarse_fp64"})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +134,8 @@ def main():
 
     def take(st):
         acc["us"] += st.kernel_us[capi.STAGES.index(dominant)]
-        acc["flops"] += st.track_flops if dominant == "track" else st.coarse_flops
+        acc["flops"] += {"track": st.track_flops - st.track_mfma_flops,
+                         "track_mfma": st.track_mfma_flops}.get(dominant, st.coarse_flops)
     t0 = time.perf_counter()
     pipelined(timed, args.steps, take)
     torch.cuda.synchronize()
@@ -139,13 +146,16 @@ def main():
     lat = np.array(latencies) * 1e3
 
     # roofline of the dominant stage: the coarse correlation (FP32 VALU or
-    # BF16/FP8 MFMA) or tracking (FP64 VALU); per frame, device time from the
-    # timed region's events
+    # BF16/FP8 MFMA) or tracking (split-BF16 MFMA on the levels it applies to,
+    # FP64 VALU on the others); per frame, device time from the timed
+    # region's events
     per_frame_us = dom_us / frames_done
     per_frame_flop = flops / frames_done
     achieved = per_frame_flop / (per_frame_us * 1e-6) / 1e12 if per_frame_us > 0 else 0.0
     if dominant == "track":
         bound, peak = "valu-fp64", 78.6
+    elif dominant == "track_mfma":
+        bound, peak = "mfma", 2500.0
     elif dominant == "coarse":
         bound, peak = ("mfma" if form in (1, 3) else "valu"), {0: 157.3, 1: 2500.0, 2: 78.6, 3: 5000.0}[form]
     else:
@@ -203,7 +213,9 @@ def main():
         "dtype": {0: "fp32", 1: "bf16x3", 2: "f64", 3: "fp8+bf16x3"}[form] + " correlation / f64 refinement",
         "form": args.form, "data": "synthetic (deterministic SplitMix64 generator)",
         "config": WORKLOAD,
-        "roofline": {"kernel": dominant, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
+        "roofline": {"kernel": dominant, "hip_kernel": HIP_KERNELS.get((dominant, form), dominant),
+                     "us_per_launch": round(per_frame_us * F, 2),  # one launch of the stage per batch
+                     "bound": bound, "achieved": round(achieved, 3), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 5) if peak else None, "traffic": None,
                      "us_per_frame": round(per_frame_us, 2), "algorithmic_gflop_per_frame": round(per_frame_flop / 1e9, 4)},
         "kernel_us_per_frame": {k: round(v, 2) for k, v in kernel.items()},

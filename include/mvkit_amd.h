@@ -40,8 +40,9 @@ typedef struct MvAmdFindOptions {
 #define MVAMD_STAGE_TRACK 4
 #define MVAMD_STAGE_FULL 5
 #define MVAMD_STAGE_REFINE 6
-#define MVAMD_STAGE_COUNT 7
-#define MVAMD_PROFILE_ALL 0x7f
+#define MVAMD_STAGE_TRACK_MFMA 7   /* tracking levels on matrix cores (MVAMD_STAGE_TRACK: FP64 vector kernel) */
+#define MVAMD_STAGE_COUNT 8
+#define MVAMD_PROFILE_ALL 0xff
 
 typedef struct MvAmdFindStats {
   int32_t n_coarse, n_prepared, top, finest, angle_stride;
@@ -50,6 +51,7 @@ typedef struct MvAmdFindStats {
   double kernel_us[MVAMD_STAGE_COUNT];   /* device time of each profiled stage */
   double coarse_flops;                   /* algorithmic FLOPs of the coarse correlation */
   double track_flops;                    /* algorithmic FLOPs of tracking (when its stage is profiled) */
+  double track_mfma_flops;               /* the part of track_flops tracked on matrix cores */
 } MvAmdFindStats;
 
 /* Find on a u8 image given as a HIP device pointer on the calling thread's

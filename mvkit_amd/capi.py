@@ -61,11 +61,11 @@ class FindStats(C.Structure):
     _fields_ = [("n_coarse", C.c_int32), ("n_prepared", C.c_int32), ("top", C.c_int32), ("finest", C.c_int32),
                 ("angle_stride", C.c_int32), ("storage", C.c_int32 * 4), ("t_coarse_us", C.c_double),
                 ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double),
-                ("kernel_us", C.c_double * 7), ("coarse_flops", C.c_double),
-                ("track_flops", C.c_double)]
+                ("kernel_us", C.c_double * 8), ("coarse_flops", C.c_double),
+                ("track_flops", C.c_double), ("track_mfma_flops", C.c_double)]
 
-PROFILE_ALL = 0x7f
-STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine"]
+PROFILE_ALL = 0xff
+STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine", "track_mfma"]
 
 
 class MvError(RuntimeError):

@@ -816,6 +816,7 @@ void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
   for (int k = 0; k < MVAMD_STAGE_COUNT; ++k) s->kernel_us[k] = t.kernel_us[k];
   s->coarse_flops = t.coarse_flops;
   s->track_flops = t.track_flops;
+  s->track_mfma_flops = t.track_mfma_flops;
 }
 void put_poses(const std::vector<mvamd::Pose>& v, double* dst, size_t cap) {
   for (size_t i = 0; i < v.size() && i < cap; ++i) {
@@ -898,10 +899,11 @@ std::map<uint64_t, PendingBatch>& pending_batches() {
 }
 void fill_batch_stats(const std::vector<mvamd::FindTrace>& traces, MvAmdFindStats* stats) {
   fill_stats(traces[0], stats);
-  double flops = 0, tflops = 0;
-  for (const auto& t : traces) { flops += t.coarse_flops; tflops += t.track_flops; }
+  double flops = 0, tflops = 0, mflops = 0;
+  for (const auto& t : traces) { flops += t.coarse_flops; tflops += t.track_flops; mflops += t.track_mfma_flops; }
   stats->coarse_flops = flops;
   stats->track_flops = tflops;
+  stats->track_mfma_flops = mflops;
 }
 }  // namespace
 

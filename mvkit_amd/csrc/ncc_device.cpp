@@ -356,7 +356,7 @@ struct Job {
   int32_t storage0[4] = {0, 0, 0, 0};
   size_t pos = 0, batch_size = 0;
   bool done = false;
-  double coarse_flops = 0, track_flops = 0;
+  double coarse_flops = 0, track_flops = 0, track_mfma_flops = 0;
 };
 struct Timer { hipEvent_t begin = nullptr, end = nullptr; int stage = 0; };
 }  // namespace
@@ -391,6 +391,7 @@ struct FindContext::Slot {
   uint8_t* rb = nullptr;
   NccPatchPool pools[2] = {};
   NcchipRefineParams rp{};
+  std::vector<bool> track_mfma;  // per tracking level: launched on matrix cores
   double t0 = 0, t1 = 0, t2 = 0;
 };
 
@@ -818,6 +819,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   time_end(stream_);
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
   constexpr int kListGrid = 2048;
+  S.track_mfma.assign(n_track, false);
   for (size_t k = 0; k < n_track; ++k) {
     const size_t l = track_levels[k];
     const double step = std::ldexp(1., int(l - 1)), offset = (step - 1) * .5;
@@ -834,7 +836,6 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     const double tile_px = (std::ceil(hi_r - lo_r) + 7) * (std::ceil(hi_c - lo_c) + 7);
     const double xmax = 255. * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
-    time_begin(kStageTrack, stream_);
     // matrix cores where every query patch has a dense raster and at most 32
     // patches fall in any candidate's angle window
     bool mfma = o.track_mfma && l - 1 >= 1 && l - 1 <= 4 && jobs[0].lv[l - 1].kind == NCC_U16;
@@ -851,7 +852,8 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       }
     }
     int rc_track = -6;
-    if (mfma)
+    if (mfma) {
+      time_begin(kStageTrackMfma, stream_);
       rc_track = ncchip_track_mfma(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                                    reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                                    lists + k * cap_all, counts + k * n_frames, int(cap),
@@ -861,6 +863,10 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                                    int(table_size), o.min_score * .4, tol, step, offset, l - 1 <= 2,
                                    model.params.absolute, int(std::ceil(hi_r - lo_r)) + 7,
                                    int(std::ceil(hi_c - lo_c)) + 7, xmax, work + k, kListGrid, stream_);
+      time_end(stream_);
+      S.track_mfma[k] = rc_track == 0;
+    }
+    if (rc_track == -6) time_begin(kStageTrack, stream_);
     if (rc_track == -6 &&
         ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                      reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
@@ -871,7 +877,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                      tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, stream_))
       raise(kInternalError, "device", "tracking launch failed");
     if (rc_track != -6 && rc_track != 0) raise(kInternalError, "device", "tracking launch failed");
-    time_end(stream_);
+    if (rc_track == -6) time_end(stream_);
   }
   // ---- full score at the finest level (matcher.cpp:813-822)
   NcchipRefineParams rp{};
@@ -927,7 +933,10 @@ void FindContext::collect(Slot& S) {
   if (traces) {
     const auto* hwork = reinterpret_cast<const unsigned long long*>(
         reinterpret_cast<const uint8_t*>(hcounts) + align_up((n_track + 1) * n_frames * 4, 8));
-    for (size_t k = 0; k < n_track; ++k) jobs[0].track_flops += 2.0 * double(hwork[k]);
+    for (size_t k = 0; k < n_track; ++k) {
+      jobs[0].track_flops += 2.0 * double(hwork[k]);
+      if (S.track_mfma[k]) jobs[0].track_mfma_flops += 2.0 * double(hwork[k]);
+    }
   }
   // ---- stage dumps for traces (not on the benchmarked path)
   if (traces && o.trace_stages) {
@@ -1103,6 +1112,7 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
       tr.prepared = J.full; tr.refined = J.refined; tr.verdict = J.verdict;
       tr.coarse_flops = J.coarse_flops;
       tr.track_flops = J.track_flops;
+      tr.track_mfma_flops = J.track_mfma_flops;
       tr.t_coarse_us = S.t1 - S.t0; tr.t_track_us = S.t2 - S.t1; tr.t_refine_us = t3 - S.t2; tr.t_total_us = now_us() - S.t0;
       if (f > 0 && traces) std::copy(traces[0].kernel_us, traces[0].kernel_us + kStageCount, tr.kernel_us);
     }

@@ -84,8 +84,10 @@ struct Selection { int64_t num_matches = -1; double max_overlap = -1; };
 struct ModelMatch { Pose pose; size_t model = 0; };
 
 // Kernel stages timed in profiling mode.
+// kStageTrack: tracking levels on the FP64 vector kernel; kStageTrackMfma:
+// tracking levels on matrix cores.
 enum KernelStage { kStagePyramid = 0, kStagePrefix, kStageCoarse, kStagePeaks, kStageTrack, kStageFull, kStageRefine,
-                   kStageCount };
+                   kStageTrackMfma, kStageCount };
 
 // Intermediate results of one find, for stage-by-stage checks.
 struct FindTrace {
@@ -100,6 +102,7 @@ struct FindTrace {
   double kernel_us[kStageCount] = {};  // device time per stage (profiling mode)
   double coarse_flops = 0;             // 2 * sum over evaluated coarse windows of their sample counts
   double track_flops = 0;              // 2 * samples of the tracked 5x5 windows (counted when tracking is profiled)
+  double track_mfma_flops = 0;         // the part of track_flops tracked on matrix cores
 };
 
 // A u8 search image in device memory (`pitch` bytes per row).
