@@ -30,7 +30,14 @@ typedef struct MvAmdFindOptions {
                                             MVAMD_PROFILE_ALL times every stage */
   int32_t vector_tracking;               /* 1: track every level with the FP64 vector kernel
                                             (0: matrix cores where they apply; same results) */
+  int32_t pixel_type;                    /* MVAMD_PIXEL_*: pixel type of the device images
+                                            (pitches in bytes) */
 } MvAmdFindOptions;
+
+/* Pixel types of device-resident search images ("byte" and "uint2" in the
+ * reference's image objects, c_api/ncc_matching_api.cpp:167-168). */
+#define MVAMD_PIXEL_BYTE 0
+#define MVAMD_PIXEL_UINT2 2
 
 /* Kernel stages reported in MvAmdFindStats.kernel_us. */
 #define MVAMD_STAGE_PYRAMID 0
@@ -54,15 +61,15 @@ typedef struct MvAmdFindStats {
   double track_mfma_flops;               /* the part of track_flops tracked on matrix cores */
 } MvAmdFindStats;
 
-/* Find on a u8 image given as a HIP device pointer on the calling thread's
- * current device.  results: cap x 4 doubles (row, col, angle, score).
+/* Find on a u8 (or, with pixel_type MVAMD_PIXEL_UINT2, u16) image given as
+ * a HIP device pointer on the calling thread's current device.  results: cap x 4 doubles (row, col, angle, score).
  * Replaces, for device-resident inputs, mv_find_ncc_model
  * (mvkit include/mvkit/c/mv_ncc_matching.h:71-74) minus the output shift. */
 MV_API MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* d_image, int64_t width, int64_t height,
                                             int64_t pitch, const MvAmdFindOptions* options, double* results,
                                             size_t cap, size_t* count, MvAmdFindStats* stats);
 
-/* find_ncc_models on a u8 image given as a HIP device pointer.  options holds
+/* find_ncc_models on a u8 or u16 image given as a HIP device pointer.  options holds
  * one entry or one per model (n_options); the selection is what
  * mv_find_ncc_models derives from one-value controls: selection_matches
  * (-1: per model) and selection_overlap (-1: per model, same-model pairs
@@ -75,8 +82,8 @@ MV_API MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t* d_
                                              int64_t selection_matches, double selection_overlap, double* results,
                                              size_t cap, size_t* count);
 
-/* Finds the model in n u8 images given as HIP device pointers, all with the
- * same options, stage by stage over the whole batch (one launch per stage for
+/* Finds the model in n u8 (or u16) images given as HIP device pointers, all
+ * with the same options, stage by stage over the whole batch (one launch per stage for
  * the candidates of every frame).  Frame k's matches go to
  * results + 4*cap*k (row, col, angle, score) and their number to counts[k];
  * each frame's result is the one mvamd_find_ncc_model_device returns for it.

@@ -28,12 +28,12 @@ extern "C" {
 int ncchip_pyramid_level(const void* src, int src_kind, int src_pitch, int src_w, int src_h, void* dst, int dst_kind,
                          int dst_pitch, int dst_w, int dst_h, void* stream);
 
-/* Row prefix sums of S (u32) and S^2 (u64), (w+1) per row with a leading
+/* Row prefix sums of S and S^2 (u64), (w+1) per row with a leading
  * zero, plus the support integral of (S != D) if `support` is non-null
  * ((w+1)*(h+1) u32).  `nz` is (w+1)*h u32 scratch.  Replaces the top-level
  * prefix statistics and support integral (matcher.cpp:487-503). */
 int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src, int shift,
-                      uint32_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream);
+                      uint64_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream);
 
 /* Batched front end.  `frames` (device) holds one NccFrame per frame of a
  * batch (at most NCC_MAX_BATCH); each batched kernel covers every frame in

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libmvkit_amd.so")
 
 MV_VALUE_EMPTY, MV_VALUE_INT64, MV_VALUE_REAL64, MV_VALUE_STRING, MV_VALUE_HANDLE = 0, 1, 2, 4, 16
 FORM_FP32, FORM_BF16X3, FORM_FP64, FORM_FP8_SCREEN = 0, 1, 2, 3
+PIXEL_BYTE, PIXEL_UINT2 = 0, 2  # device image pixel types (MVAMD_PIXEL_*)
 
 STATUS_NAMES = {0: "OK", -1: "INVALID_ARGUMENT", -2: "OUT_OF_RANGE", -3: "TYPE_MISMATCH", -4: "NOT_FOUND",
                 -5: "OUT_OF_MEMORY", -6: "UNSUPPORTED", -7: "INVALID_STATE", -100: "IO_ERROR",
@@ -54,7 +55,8 @@ class FindOptions(C.Structure):
     _fields_ = [("angle_start", C.c_double), ("angle_extent", C.c_double), ("min_score", C.c_double),
                 ("max_overlap", C.c_double), ("num_matches", C.c_int64), ("subpixel", C.c_int32),
                 ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
-                ("screen", C.c_double), ("profile", C.c_int32), ("vector_tracking", C.c_int32)]
+                ("screen", C.c_double), ("profile", C.c_int32), ("vector_tracking", C.c_int32),
+                ("pixel_type", C.c_int32)]
 
 
 class FindStats(C.Structure):
@@ -330,16 +332,18 @@ class NccModel:
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
                 highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0,
-                vector_tracking=False) -> FindOptions:
+                vector_tracking=False, pixel_type=PIXEL_BYTE) -> FindOptions:
         """profile: True times every stage, an int is a mask of stage bits
         (bit i = STAGES[i]).  vector_tracking: track every level with the FP64
-        vector kernel instead of matrix cores (same results)."""
+        vector kernel instead of matrix cores (same results).  pixel_type:
+        PIXEL_BYTE or PIXEL_UINT2 for the device images (pitches in bytes)."""
         mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
-                           lowest, form, screen, mask, int(vector_tracking))
+                           lowest, form, screen, mask, int(vector_tracking), int(pixel_type))
 
     def find_device(self, d_image: int, width: int, height: int, pitch: int, opts: FindOptions, cap: int = 4096):
-        """Find on a device-resident u8 image (integer device address)."""
+        """Find on a device-resident u8 or u16 image (integer device address;
+        opts.pixel_type names the type, pitch is in bytes)."""
         res = np.zeros((cap, 4))
         n = C.c_size_t()
         st = FindStats()

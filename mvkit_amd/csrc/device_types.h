@@ -74,12 +74,13 @@ typedef struct NccMatch { double row, col, angle, score; } NccMatch;
 #define NCC_MAX_LEVELS 16
 #define NCC_MAX_BATCH 256
 typedef struct NccFrame {
-  NccLevel levels[NCC_MAX_LEVELS];  // level 0: the caller's u8 image (crop origin); above: workspace
-  const uint8_t* dc_src;     // reference pixel dc (first crop pixel), device memory
+  NccLevel levels[NCC_MAX_LEVELS];  // level 0: the caller's u8 / u16 image (crop origin); above: workspace
+  const void* dc_src;        // reference pixel dc (first crop pixel), device memory
+  int32_t dc_wide;           // dc_src points to a u16 pixel (else u8)
   int32_t storage0[4];       // level-0 valid rectangle {row, col, height, width}
   double crop_r, crop_c;
   int32_t image_h, image_w;  // search domain (full image)
-  uint32_t* p1;              // top-level row prefix of S, (w+1) per row
+  uint64_t* p1;              // top-level row prefix of S, (w+1) per row
   uint64_t* p2;              // top-level row prefix of S^2
   uint32_t* nz;              // top-level row prefix of [S != D]
   double* planes;            // coarse score planes [angle][h][w]

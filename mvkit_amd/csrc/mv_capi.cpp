@@ -311,7 +311,20 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
   q.track_mfma = o->vector_tracking == 0;
   if (q.form < MVAMD_FORM_FP32 || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
+  if (o->pixel_type < 0 || o->pixel_type > 2)
+    fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, "find_ncc_model", "pixel_type", "unsupported pixel type");
   return q;
+}
+
+// A device-resident search image of the pixel type the options name.
+mvamd::FrameInput device_frame(const char* op, const uint8_t* image, int64_t width, int64_t height, int64_t pitch,
+                               const MvAmdFindOptions* o, int64_t index = -1) {
+  const int bytes = o->pixel_type == MVAMD_PIXEL_UINT2 ? 2 : 1;
+  if (!image || width <= 0 || height <= 0 || pitch < width * bytes || pitch % bytes)
+    fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry", index);
+  mvamd::FrameInput in{image, size_t(width), size_t(height), size_t(pitch)};
+  in.pixel_bytes = bytes;
+  return in;
 }
 
 // Rethrows algorithm errors under the C-ABI operation name (ncc_matching_api.cpp:21-26).
@@ -637,10 +650,9 @@ extern "C" MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple A
     const auto m = model_of(ModelID, op);
     const mvamd::FindOptions q = controls(op, AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel, NumLevels);
     if (Image->empty) { out.commit(); return; }
-    if (Image->pixel_bytes != 1)
-      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Image", "uint2 search images are not supported on the device path");
+    const size_t pitch = Image->width * size_t(Image->pixel_bytes);
     const auto found = algorithm(op, [&] {
-      return context().find_host(*m, Image->pixels, Image->width, Image->height, Image->width, q);
+      return context().find_host(*m, Image->pixels, Image->width, Image->height, pitch, q, nullptr, Image->pixel_bytes);
     });
     // HALCON pixel-edge convention of the translation (ncc_matching_api.cpp:171-176)
     for (const auto& p : found) {
@@ -666,13 +678,12 @@ extern "C" MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple
     const Controls c = controls_n(op, ms.size(), AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel,
                                   NumLevels);
     if (Image->empty) { out.commit(); return; }
-    if (Image->pixel_bytes != 1)
-      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "Image", "uint2 search images are not supported on the device path");
     std::vector<const mvamd::NccModel*> ptrs;
     for (const auto& m : ms) ptrs.push_back(m.get());
+    const size_t pitch = Image->width * size_t(Image->pixel_bytes);
     const auto found = algorithm(op, [&] {
       return context().find_models_host(ptrs.data(), c.options.data(), ptrs.size(), Image->pixels, Image->width,
-                                        Image->height, Image->width, c.selection, true);
+                                        Image->height, pitch, c.selection, true, Image->pixel_bytes);
     });
     for (const auto& m : found) {
       const double cs = std::cos(m.pose.angle), sn = std::sin(m.pose.angle);
@@ -832,13 +843,12 @@ extern "C" MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* 
     constexpr auto op = "find_ncc_model";
     required(d_image, op, "Image");
     required(count, op, "count");
-    if (width <= 0 || height <= 0 || pitch < width)
-      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
     const auto m = model_of(ModelID, op);
     const mvamd::FindOptions q = from_ext(options);
+    const mvamd::FrameInput in = device_frame(op, d_image, width, height, pitch, options);
     mvamd::FindTrace trace;
     const auto found = algorithm(op, [&] {
-      return context().find(*m, d_image, size_t(width), size_t(height), size_t(pitch), q, stats ? &trace : nullptr);
+      return std::move(context().find_batch(*m, &in, 1, q, stats ? &trace : nullptr).front());
     });
     if (results) put_poses(found, results, cap);
     *count = found.size();
@@ -856,8 +866,6 @@ extern "C" MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t
     required(d_image, op, "Image");
     required(options, op, "options");
     required(count, op, "count");
-    if (width <= 0 || height <= 0 || pitch < width)
-      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
     const auto ms = models_of(ModelIDs, op);
     if (n_options != 1 && n_options != ms.size())
       fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, "options", "one value or one value per model required");
@@ -871,7 +879,10 @@ extern "C" MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t
     mvamd::Selection sel;
     sel.num_matches = selection_matches;
     sel.max_overlap = selection_overlap;
-    const mvamd::FrameInput in{d_image, size_t(width), size_t(height), size_t(pitch)};
+    for (size_t i = 1; i < n_options; ++i)
+      if (options[i].pixel_type != options[0].pixel_type)
+        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "options", "one pixel type for the image", int64_t(i));
+    const mvamd::FrameInput in = device_frame(op, d_image, width, height, pitch, options);
     const auto found = algorithm(op, [&] {
       return context().find_models(ptrs.data(), qs.data(), ptrs.size(), in, sel, true);
     });
@@ -918,12 +929,10 @@ extern "C" MvStatus mvamd_find_ncc_model_batch(MvTuple ModelID, const uint8_t* c
     required(heights, op, "heights");
     required(pitches, op, "pitches");
     required(counts, op, "counts");
-    std::vector<mvamd::FrameInput> frames(n);
-    for (size_t k = 0; k < n; ++k) {
-      if (!d_images[k] || widths[k] <= 0 || heights[k] <= 0 || pitches[k] < widths[k])
-        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Images", "invalid image geometry", int64_t(k));
-      frames[k] = {d_images[k], size_t(widths[k]), size_t(heights[k]), size_t(pitches[k])};
-    }
+    required(options, op, "options");
+    std::vector<mvamd::FrameInput> frames;
+    for (size_t k = 0; k < n; ++k)
+      frames.push_back(device_frame(op, d_images[k], widths[k], heights[k], pitches[k], options, int64_t(k)));
     const auto m = model_of(ModelID, op);
     const mvamd::FindOptions q = from_ext(options);
     std::vector<mvamd::FindTrace> traces(stats ? n : 0);
@@ -952,12 +961,10 @@ extern "C" MvStatus mvamd_find_ncc_model_batch_submit(MvTuple ModelID, const uin
     required(ticket, op, "ticket");
     if (n > NCC_MAX_BATCH)
       fail(mvamd::kLimitExceeded, mvamd::kErrInvalidValue, op, "Images", "too many frames for one submitted batch");
-    std::vector<mvamd::FrameInput> frames(n);
-    for (size_t k = 0; k < n; ++k) {
-      if (!d_images[k] || widths[k] <= 0 || heights[k] <= 0 || pitches[k] < widths[k])
-        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Images", "invalid image geometry", int64_t(k));
-      frames[k] = {d_images[k], size_t(widths[k]), size_t(heights[k]), size_t(pitches[k])};
-    }
+    required(options, op, "options");
+    std::vector<mvamd::FrameInput> frames;
+    for (size_t k = 0; k < n; ++k)
+      frames.push_back(device_frame(op, d_images[k], widths[k], heights[k], pitches[k], options, int64_t(k)));
     PendingBatch b;
     b.model = model_of(ModelID, op);
     b.n = n;

@@ -14,10 +14,10 @@
 namespace ncc {
 
 // Exact window moments over a patch's runs from the row prefixes.
-__device__ __forceinline__ void window_moments(const uint32_t* __restrict__ p1, const uint64_t* __restrict__ p2,
+__device__ __forceinline__ void window_moments(const uint64_t* __restrict__ p1, const uint64_t* __restrict__ p2,
                                                int pitch, const int32_t* __restrict__ runs, int n_runs, int r, int c,
                                                int64_t& s1, int64_t& s2) {
-  uint32_t a1 = 0;
+  uint64_t a1 = 0;
   uint64_t a2 = 0;
   for (int k = 0; k < n_runs; ++k) {
     const int4 run = reinterpret_cast<const int4*>(runs)[k];
@@ -43,8 +43,8 @@ struct CoarseArgs {
 // The frame-dependent part of a coarse block's work.
 struct CoarseFrame {
   NccLevel lv;
-  const uint8_t* dc_src;
-  const uint32_t* p1;
+  uint32_t dc;
+  const uint64_t* p1;
   const uint64_t* p2;
   int4 rg;
   double* planes;
@@ -53,7 +53,7 @@ __device__ __forceinline__ CoarseFrame coarse_frame(const CoarseArgs& a, int f, 
   const NccFrame& F = a.frames[f];
   CoarseFrame c;
   c.lv = F.levels[a.level];
-  c.dc_src = F.dc_src;
+  c.dc = frame_dc(F);
   c.p1 = F.p1;
   c.p2 = F.p2;
   c.rg = reinterpret_cast<const int4*>(F.ranges)[ai];
@@ -77,8 +77,8 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs args) {
   const int W = a.lv.width, H = a.lv.height;
   const int r0 = blockIdx.y * kTileR, c0 = blockIdx.x * kTileC;
   if (r0 >= H || c0 >= W) return;
-  const double dc = double(*a.dc_src);
-  const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
+  const double dc = double(a.dc);
+  const int64_t D = int64_t(a.dc) << a.lv.shift;
   const NccPatchPool& pool = args.pools.of(args.patch_ids[ai]);
   const NccPatch P = args.pools.patch(args.patch_ids[ai]);
   const int4 rg = a.rg;
@@ -166,10 +166,10 @@ __global__ void __launch_bounds__(256) k_coarse_fp32(const CoarseArgs args) {
     if (c >= W) break;
     double score = NCC_MISSING;
     if (r >= rg.x && r < rg.y && c >= rg.z && c < rg.w) {
-      // sums of S from sums of T = S - C
-      const int64_t n = P.n;
-      const int64_t sum_s = m1[j] + n * Cl;
-      const int64_t sum_s2 = m2[j] + 2 * Cl * m1[j] + n * Cl * Cl;
+      // sums of S from sums of T = S - C (modulo 2^64, see centred_moments)
+      const uint64_t n = uint64_t(P.n), cu = uint64_t(Cl), t1 = uint64_t(m1[j]);
+      const int64_t sum_s = int64_t(t1 + n * cu);
+      const int64_t sum_s2 = int64_t(uint64_t(m2[j]) + 2 * cu * t1 + n * cu * cu);
       score = normalise(double(acc[j]), args.C, P.wsum_f, P.wsum, P.n, sum_s, sum_s2, D, a.lv.shift, dc,
                         args.exact_lattice, args.absolute);
     }
@@ -186,8 +186,8 @@ __global__ void __launch_bounds__(256) k_coarse_fp64(const CoarseArgs args) {
   const int W = a.lv.width, H = a.lv.height;
   const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
   if (c >= W || r >= H) return;
-  const double dc = double(*a.dc_src);
-  const int64_t D = int64_t(*a.dc_src) << a.lv.shift;
+  const double dc = double(a.dc);
+  const int64_t D = int64_t(a.dc) << a.lv.shift;
   const NccPatchPool& pool = args.pools.of(args.patch_ids[ai]);
   const NccPatch P = args.pools.patch(args.patch_ids[ai]);
   const int4 rg = a.rg;

@@ -100,12 +100,28 @@ __device__ __forceinline__ int list_frame(const int* off, int n_frames, int g) {
 __device__ __forceinline__ int sample_row(int32_t rc) { return rc >> 16; }
 __device__ __forceinline__ int sample_col(int32_t rc) { return int(int16_t(rc & 0xffff)); }
 
+// The frame's reference pixel dc (u8 or u16 search image).
+__device__ __forceinline__ uint32_t frame_dc(const NccFrame& F) {
+  return F.dc_wide ? uint32_t(*static_cast<const uint16_t*>(F.dc_src)) : uint32_t(*static_cast<const uint8_t*>(F.dc_src));
+}
+
+// Centred integer moments t1 = sum(S - D), t2 = sum((S - D)^2) from the
+// plain sums of S and S^2.  Computed modulo 2^64, so they are exact whenever
+// t2 < 2^63, also when sum(S^2) itself wraps (u16 images at high levels).
+__device__ __forceinline__ void centred_moments(int64_t sum_s, int64_t sum_s2, int n, int64_t D, int64_t& t1,
+                                                int64_t& t2) {
+  const uint64_t s = uint64_t(sum_s), d = uint64_t(D), nn = uint64_t(n);
+  t1 = int64_t(s - nn * d);
+  t2 = int64_t(uint64_t(sum_s2) - 2 * d * s + nn * d * d);
+}
+
 // Exact window moments from integer sums of S over n samples: returns the
-// reference's sum(q) and sum(q^2) with q = S*4^-l - dc, both exact in FP64.
+// reference's sum(q) and sum(q^2) with q = S*4^-l - dc (exact in FP64 while
+// |t1|, t2 < 2^53).
 __device__ __forceinline__ void exact_moments(int64_t sum_s, int64_t sum_s2, int n, int64_t D, int shift,
                                               double& sum_q, double& sq_q) {
-  const int64_t t1 = sum_s - int64_t(n) * D;
-  const int64_t t2 = sum_s2 - 2 * D * sum_s + int64_t(n) * D * D;
+  int64_t t1, t2;
+  centred_moments(sum_s, sum_s2, n, D, t1, t2);
   sum_q = ldexp(double(t1), -shift);
   sq_q = ldexp(double(t2), -2 * shift);
 }
@@ -127,8 +143,8 @@ __device__ __forceinline__ double normalise(double dot_c, double C, double wsum_
     // Low-variance rule (kernels_avx2.cpp:128-129, 166-167): -inf on the exact
     // byte lattice; elsewhere the reference recomputes directly, which only
     // differs from -inf when the window is not constant.
-    const int64_t t1 = sum_s - int64_t(n) * D;
-    const int64_t t2 = sum_s2 - 2 * D * sum_s + int64_t(n) * D * D;
+    int64_t t1, t2;
+    centred_moments(sum_s, sum_s2, n, D, t1, t2);
     const bool constant = (__int128)t2 * n == (__int128)t1 * t1;
     if (exact_lattice || constant) return NCC_MISSING;
     const double v = ldexp(double((__int128)t2 * n - (__int128)t1 * t1), -2 * shift) / cnt;

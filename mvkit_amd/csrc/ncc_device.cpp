@@ -450,27 +450,31 @@ void FindContext::time_collect(FindTrace* trace) {
 }
 
 std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
-                                         size_t pitch, const FindOptions& options, FindTrace* trace) {
+                                         size_t pitch, const FindOptions& options, FindTrace* trace, int pixel_bytes) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
-  const size_t bytes = width * height;
+  const size_t row = width * size_t(pixel_bytes), bytes = row * height;
   if (image_.bytes < bytes) image_ = DeviceBuffer(align_up(bytes, 1 << 20));
-  check(hipMemcpy2DAsync(image_.ptr, width, image, pitch, width, height, hipMemcpyHostToDevice, stream_), "image upload");
-  return find(model, image_.as<uint8_t>(), width, height, width, options, trace);
+  check(hipMemcpy2DAsync(image_.ptr, row, image, pitch, row, height, hipMemcpyHostToDevice, stream_), "image upload");
+  FrameInput in{image_.as<uint8_t>(), width, height, row};
+  in.pixel_bytes = pixel_bytes;
+  auto out = find_batch(model, &in, 1, options, trace);
+  return std::move(out.front());
 }
 
 std::vector<ModelMatch> FindContext::find_models_host(const NccModel* const* models, const FindOptions* options,
                                                       size_t n_models, const uint8_t* image, size_t width,
                                                       size_t height, size_t pitch, const Selection& selection,
-                                                      bool per_model_quota_before_overlap) {
+                                                      bool per_model_quota_before_overlap, int pixel_bytes) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
-  const size_t bytes = width * height;
+  const size_t row = width * size_t(pixel_bytes), bytes = row * height;
   check(hipStreamSynchronize(stream2_), "sync");
   if (image_.bytes < bytes) {
     check(hipStreamSynchronize(stream_), "sync");
     image_ = DeviceBuffer(align_up(bytes, 1 << 20));
   }
-  check(hipMemcpy2DAsync(image_.ptr, width, image, pitch, width, height, hipMemcpyHostToDevice, stream_), "image upload");
-  const FrameInput in{image_.as<uint8_t>(), width, height, width};
+  check(hipMemcpy2DAsync(image_.ptr, row, image, pitch, row, height, hipMemcpyHostToDevice, stream_), "image upload");
+  FrameInput in{image_.as<uint8_t>(), width, height, row};
+  in.pixel_bytes = pixel_bytes;
   return find_models(models, options, n_models, in, selection, per_model_quota_before_overlap);
 }
 
@@ -525,9 +529,17 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   FindOptions o = req;
   for (size_t f = 0; f < n_frames; ++f)
     if (!inputs[f].width || !inputs[f].height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
+  // u16 ("uint2") frames: level 0 in u16, block sums in u32 from level 1
+  const bool wide = inputs[0].pixel_bytes == 2;
+  for (size_t f = 0; f < n_frames; ++f)
+    if (inputs[f].pixel_bytes != inputs[0].pixel_bytes || (inputs[f].pixel_bytes != 1 && inputs[f].pixel_bytes != 2) ||
+        (wide && inputs[f].pitch % 2))
+      raise(kInvalidArgument, "image", "frames of one batch share one pixel type (byte or uint2, even pitch)");
   if (model.levels.empty()) raise(kInvalidState, "model", "model is empty");
   if (o.highest < 0 || o.lowest < 1) raise(kInvalidArgument, "NumLevels", "invalid pyramid levels");
   const size_t level_count = o.highest ? std::min(model.levels.size(), size_t(o.highest)) : model.levels.size();
+  if (inputs[0].pixel_bytes == 2 && level_count > 9)
+    raise(kUnsupported, "NumLevels", "uint2 search images support at most 9 pyramid levels");
   o.lowest = int(std::min(size_t(o.lowest), level_count));
   const size_t finest = size_t(o.lowest - 1);
   RasterPool qpool;  // per-find rasters: endpoint angles and the coarse original patch
@@ -606,6 +618,21 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
 
   if (o.form == NCCHIP_FORM_FP8_SCREEN) raise(kUnsupported, "form", "the FP8 screen form is not available in this build");
   if (o.form < NCCHIP_FORM_FP32 || o.form > NCCHIP_FORM_FP8_SCREEN) raise(kInvalidArgument, "form", "unknown correlation form");
+  if (wide && o.form == NCCHIP_FORM_BF16X3)
+    raise(kUnsupported, "form", "the split-BF16 form needs byte search images");
+  if (wide) {
+    // The window moments are exact 64-bit integers: sum((S - D)^2) over a
+    // window must stay below 2^61 at every level searched
+    for (size_t l = finest; l <= top; ++l) {
+      double n_max = 0;
+      for (const int32_t r : refs[l]) n_max = std::max(n_max, double(raster_of(r).count));
+      const double x = 65535. * std::ldexp(1., 2 * int(l));
+      if (n_max * x * x > 0x1p61)
+        raise(kUnsupported, "NumLevels", "template too large for exact uint2 window moments at this pyramid level");
+    }
+    // the FP32 form holds S - C exactly in FP32 up to level 4
+    if (o.form == NCCHIP_FORM_FP32 && top > 4) o.form = NCCHIP_FORM_FP64;
+  }
   const bool use_mfma = o.form == NCCHIP_FORM_BF16X3;
   DenseWeights dense;
   if (use_mfma) {
@@ -632,13 +659,15 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     J.crop_b = std::min<int64_t>(H, ((H - 1 + halo) / lattice + 1) * lattice);
     J.crop_e = std::min<int64_t>(W, ((W - 1 + halo) / lattice + 1) * lattice);
     J.lv.resize(level_count);
-    J.lv[0] = {int(J.crop_e - J.crop_c), int(J.crop_b - J.crop_r), NCC_U8, int(J.in.pitch), 0, 0};
+    J.lv[0] = {int(J.crop_e - J.crop_c), int(J.crop_b - J.crop_r), wide ? NCC_U16 : NCC_U8,
+               int(J.in.pitch / size_t(J.in.pixel_bytes)), 0, 0};
     for (size_t l = 1; l < level_count; ++l) {
+      const bool u16 = !wide && l <= 4;  // block sums of u8 pixels fit 16 bits up to level 4
       J.lv[l].w = std::max(1, J.lv[l - 1].w / 2);
       J.lv[l].h = std::max(1, J.lv[l - 1].h / 2);
-      J.lv[l].kind = l <= 4 ? NCC_U16 : NCC_U32;
+      J.lv[l].kind = u16 ? NCC_U16 : NCC_U32;
       J.lv[l].pitch = J.lv[l].w;
-      J.lv[l].bytes = size_t(J.lv[l].w) * J.lv[l].h * (l <= 4 ? 2 : 4);
+      J.lv[l].bytes = size_t(J.lv[l].w) * J.lv[l].h * (u16 ? 2 : 4);
     }
     J.ranges.resize(4 * A);
     for (size_t i = 0; i < A; ++i) {
@@ -673,7 +702,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   for (auto& J : jobs) {
     const int Wt = J.lv[top].w, Ht = J.lv[top].h;
     for (size_t l = 1; l < level_count; ++l) { J.lv[l].off = off; off = align_up(off + J.lv[l].bytes); }
-    J.off_p1 = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
+    J.off_p1 = off; off = align_up(off + size_t(Wt + 1) * Ht * 8);
     J.off_p2 = off; off = align_up(off + size_t(Wt + 1) * Ht * 8);
     J.off_nz = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
     J.off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
@@ -742,15 +771,17 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       for (size_t l = 0; l < level_count; ++l) {
         NccLevel& L = d.levels[l];
         L.data = l ? static_cast<const void*>(dev + J.lv[l].off)
-                   : static_cast<const void*>(J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c);
+                   : static_cast<const void*>(J.in.image + J.crop_r * int64_t(J.in.pitch) +
+                                              J.crop_c * int64_t(J.in.pixel_bytes));
         L.kind = J.lv[l].kind; L.pitch = J.lv[l].pitch; L.width = J.lv[l].w; L.height = J.lv[l].h;
         L.shift = int32_t(2 * l);
       }
-      d.dc_src = J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c;
+      d.dc_src = J.in.image + J.crop_r * int64_t(J.in.pitch) + J.crop_c * int64_t(J.in.pixel_bytes);
+      d.dc_wide = wide;
       d.storage0[0] = 0; d.storage0[1] = 0; d.storage0[2] = J.lv[0].h; d.storage0[3] = J.lv[0].w;
       d.crop_r = double(J.crop_r); d.crop_c = double(J.crop_c);
       d.image_h = int32_t(J.in.height); d.image_w = int32_t(J.in.width);
-      d.p1 = reinterpret_cast<uint32_t*>(dev + J.off_p1);
+      d.p1 = reinterpret_cast<uint64_t*>(dev + J.off_p1);
       d.p2 = reinterpret_cast<uint64_t*>(dev + J.off_p2);
       d.nz = reinterpret_cast<uint32_t*>(dev + J.off_nz);
       d.planes = reinterpret_cast<double*>(dev + J.off_planes);
@@ -784,8 +815,9 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, stream_))
     raise(kInternalError, "device", "prefix launch failed");
   time_end(stream_);
-  const double C = std::ldexp(128.0, int(2 * top));
-  const int exact_top = top <= 2;
+  // centring of the coarse forms: mid-range of the pixel type at the top level
+  const double C = std::ldexp(wide ? 32768.0 : 128.0, int(2 * top));
+  const int exact_top = !wide && top <= 2;  // exact byte lattice (matcher.cpp:347, 561)
   const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
   time_begin(kStageCoarse, stream_);
   int rc = 0;
@@ -814,7 +846,8 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                    stream_))
     raise(kInternalError, "device", "peak launch failed");
   // find_models over several models converts the whole level 0 (matcher.cpp:596-611)
-  if (!finest && !o.whole_level0 && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
+  // u16 frames keep the whole level 0 (matcher.cpp:612: only byte images are cut down)
+  if (!finest && !o.whole_level0 && !wide && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
     raise(kInternalError, "device", "storage launch failed");
   time_end(stream_);
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
@@ -834,7 +867,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       n_max = std::max(n_max, double(p.count));
     }
     const double tile_px = (std::ceil(hi_r - lo_r) + 7) * (std::ceil(hi_c - lo_c) + 7);
-    const double xmax = 255. * std::ldexp(1., 2 * int(l - 1));
+    const double xmax = (wide ? 65535. : 255.) * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
     // matrix cores where every query patch has a dense raster and at most 32
     // patches fall in any candidate's angle window
@@ -860,7 +893,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                                    reinterpret_cast<NccMatch*>(dev + off_raw), lists + (k + 1) * cap_all,
                                    counts + (k + 1) * n_frames,
                                    reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size,
-                                   int(table_size), o.min_score * .4, tol, step, offset, l - 1 <= 2,
+                                   int(table_size), o.min_score * .4, tol, step, offset, !wide && l - 1 <= 2,
                                    model.params.absolute, int(std::ceil(hi_r - lo_r)) + 7,
                                    int(std::ceil(hi_c - lo_c)) + 7, xmax, work + k, kListGrid, stream_);
       time_end(stream_);
@@ -873,7 +906,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                      lists + k * cap_all, counts + k * n_frames, int(cap), reinterpret_cast<NccMatch*>(dev + off_raw),
                      lists + (k + 1) * cap_all, counts + (k + 1) * n_frames,
                      reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size, int(table_size),
-                     o.min_score * .4, tol, step, offset, l - 1 <= 2, model.params.absolute,
+                     o.min_score * .4, tol, step, offset, !wide && l - 1 <= 2, model.params.absolute,
                      tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, stream_))
       raise(kInternalError, "device", "tracking launch failed");
     if (rc_track != -6 && rc_track != 0) raise(kInternalError, "device", "tracking launch failed");
@@ -892,7 +925,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   rp.step = model.params.step; rp.angle_start = o.start; rp.angle_end = o.start + o.extent;
   rp.min_score = o.min_score; rp.subpixel = o.subpixel; rp.rotate = o.extent > 0; rp.absolute = model.params.absolute;
   rp.recover_first = o.query_cache;
-  rp.prefix_recovery = o.whole_level0;
+  rp.prefix_recovery = o.whole_level0 && !wide;  // byte images only (matcher.cpp:596-611, 991)
   rp.guard_first = -1;  // set by find_models once the candidate count is known
   rp.guard = -1;
   auto* hfull = reinterpret_cast<NccMatch*>(rb + rb_full);
@@ -1244,13 +1277,14 @@ std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, 
       if (S.empty) continue;
       collect(S);
       // default background guard (matcher.cpp:839-846), for an AVX2 host
-      // rounding to nearest; the whole level 0 is converted when the query
-      // kept candidates, which it did when there are 32 of them
+      // rounding to nearest and byte images; the whole level 0 is converted
+      // when the query kept candidates, which it did when there are 32 of them
       const NccModel& m = *models[i];
       const FindOptions& o = S.o;  // the angle range clipped to the model's
       if (!S.finest && distinct && sel.num_matches == 0 && S.jobs[0].full.size() >= 32 && o.subpixel &&
           !o.num_matches && o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
-          m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several) {
+          m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several &&
+          frame.pixel_bytes == 1) {
         S.rp.guard_first = o.min_score >= .8 ? .1 : .2;
         S.rp.guard = .1;
       }

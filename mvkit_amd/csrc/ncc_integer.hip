@@ -63,35 +63,36 @@ __global__ void __launch_bounds__(256) k_pyramid_batch(const NccFrame* __restric
 // Layout: (w+1) entries per row with a leading zero (matcher.cpp:490-494).
 template <class T>
 __device__ __forceinline__ void row_prefix(const T* __restrict__ src, int sp, int w, int h, uint32_t D, int r,
-                                           uint32_t* __restrict__ p1, uint64_t* __restrict__ p2,
+                                           uint64_t* __restrict__ p1, uint64_t* __restrict__ p2,
                                            uint32_t* __restrict__ nz) {
-  __shared__ uint32_t s1[4];
+  __shared__ uint64_t s1[4];
   __shared__ uint64_t s2[4];
   __shared__ uint32_t s3[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const T* row = src + size_t(r) * sp;
-  uint32_t* o1 = p1 + size_t(r) * (w + 1);
+  uint64_t* o1 = p1 + size_t(r) * (w + 1);
   uint64_t* o2 = p2 + size_t(r) * (w + 1);
   uint32_t* o3 = nz + size_t(r) * (w + 1);
   if (threadIdx.x == 0) { o1[0] = 0; o2[0] = 0; o3[0] = 0; }
-  uint32_t carry1 = 0, carry3 = 0;
-  uint64_t carry2 = 0;
+  uint32_t carry3 = 0;
+  uint64_t carry1 = 0, carry2 = 0;
   for (int base = 0; base < w; base += 256) {
     const int c = base + threadIdx.x;
     const uint32_t s = c < w ? uint32_t(row[c]) : 0u;
-    uint32_t v1 = s, v3 = (c < w && s != D) ? 1u : 0u;
+    uint64_t v1 = s;
+    uint32_t v3 = (c < w && s != D) ? 1u : 0u;
     uint64_t v2 = uint64_t(s) * s;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t t1 = __shfl_up(v1, d, 64);
+      const uint64_t t1 = __shfl_up(v1, d, 64);
       const uint64_t t2 = __shfl_up(v2, d, 64);
       const uint32_t t3 = __shfl_up(v3, d, 64);
       if (lane >= d) { v1 += t1; v2 += t2; v3 += t3; }
     }
     if (lane == 63) { s1[wave] = v1; s2[wave] = v2; s3[wave] = v3; }
     __syncthreads();
-    uint32_t add1 = carry1, add3 = carry3;
-    uint64_t add2 = carry2;
+    uint32_t add3 = carry3;
+    uint64_t add1 = carry1, add2 = carry2;
     for (int k = 0; k < wave; ++k) { add1 += s1[k]; add2 += s2[k]; add3 += s3[k]; }
     if (c < w) { o1[c + 1] = v1 + add1; o2[c + 1] = v2 + add2; o3[c + 1] = v3 + add3; }
     carry1 += s1[0] + s1[1] + s1[2] + s1[3];
@@ -104,7 +105,7 @@ __device__ __forceinline__ void row_prefix(const T* __restrict__ src, int sp, in
 template <class T>
 __global__ void __launch_bounds__(256) k_row_prefix(const T* __restrict__ src, int sp, int w, int h,
                                                     const uint8_t* __restrict__ dc_src, int shift,
-                                                    uint32_t* __restrict__ p1, uint64_t* __restrict__ p2,
+                                                    uint64_t* __restrict__ p1, uint64_t* __restrict__ p2,
                                                     uint32_t* __restrict__ nz) {
   if (int(blockIdx.x) >= h) return;
   row_prefix(src, sp, w, h, uint32_t(*dc_src) << shift, blockIdx.x, p1, p2, nz);
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(256) k_row_prefix_batch(const NccFrame* __rest
   const NccFrame& F = frames[blockIdx.y];
   const NccLevel& L = F.levels[level];
   if (int(blockIdx.x) >= L.height) return;
-  row_prefix(static_cast<const T*>(L.data), L.pitch, L.width, L.height, uint32_t(*F.dc_src) << L.shift, blockIdx.x,
+  row_prefix(static_cast<const T*>(L.data), L.pitch, L.width, L.height, frame_dc(F) << L.shift, blockIdx.x,
              F.p1, F.p2, F.nz);
 }
 
@@ -211,7 +212,7 @@ extern "C" int ncchip_copy_u32(const uint32_t* src, uint32_t* dst, size_t n, voi
 }
 
 extern "C" int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src, int shift,
-                                 uint32_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream) {
+                                 uint64_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream) {
   hipStream_t st = as_stream(stream);
   if (kind == NCC_U8) hipLaunchKernelGGL((k_row_prefix<uint8_t>), dim3(h), dim3(256), 0, st, static_cast<const uint8_t*>(src), pitch, w, h, dc_src, shift, p1, p2, nz);
   else if (kind == NCC_U16) hipLaunchKernelGGL((k_row_prefix<uint16_t>), dim3(h), dim3(256), 0, st, static_cast<const uint16_t*>(src), pitch, w, h, dc_src, shift, p1, p2, nz);

@@ -215,8 +215,8 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   __shared__ int s_code[kThreads / 64][64];
   __shared__ float s_dot[kThreads / 64][64];
   int queued = 0;  // wave-uniform
-  const double dc = double(*F.dc_src);
-  const int64_t D = int64_t(*F.dc_src) << shift;
+  const double dc = double(frame_dc(F));
+  const int64_t D = int64_t(frame_dc(F)) << shift;
   const int64_t Cl = Ci;
   auto flush = [&]() __attribute__((always_inline)) {
     __builtin_amdgcn_wave_barrier();

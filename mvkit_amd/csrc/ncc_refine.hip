@@ -70,8 +70,8 @@ struct TrackArgs {
 __device__ __forceinline__ void bind_frame(TrackArgs& a, int f) {
   const NccFrame& F = a.frames[f];
   a.lv = F.levels[a.level];
-  a.dc = double(*F.dc_src);
-  a.D = int64_t(*F.dc_src) << a.lv.shift;
+  a.dc = double(frame_dc(F));
+  a.D = int64_t(frame_dc(F)) << a.lv.shift;
   a.crop_r = F.crop_r;
   a.crop_c = F.crop_c;
   if (a.level == 0) { a.sr = F.storage0[0]; a.sc = F.storage0[1]; a.sh = F.storage0[2]; a.sw = F.storage0[3]; }
@@ -997,7 +997,7 @@ struct RefineArgs {
 __device__ __forceinline__ void bind_frame(RefineArgs& a, int f) {
   const NccFrame& F = a.frames[f];
   a.lv = F.levels[a.finest];
-  a.dc = double(*F.dc_src);
+  a.dc = double(frame_dc(F));
   a.crop_r = F.crop_r;
   a.crop_c = F.crop_c;
   if (a.finest == 0) { a.sr = F.storage0[0]; a.sc = F.storage0[1]; a.sh = F.storage0[2]; a.sw = F.storage0[3]; }

@@ -140,7 +140,7 @@ def paste(image: np.ndarray, tmpl: np.ndarray, cy: float, cx: float, angle: floa
 
 @dataclass
 class Scene:
-    image: np.ndarray                  # u8 search image
+    image: np.ndarray                  # u8 (or u16) search image
     template: np.ndarray               # u8 template
     truth: list = field(default_factory=list)  # [(row, col, angle)] centroid convention
     seed: int = 0
@@ -150,19 +150,24 @@ class Scene:
                 "template": hashlib.sha256(self.template.tobytes()).hexdigest()}
 
 
-def _finish(img: np.ndarray, rng: SplitMix64, sigma: float, gain: bool) -> np.ndarray:
+def _finish(img: np.ndarray, rng: SplitMix64, sigma: float, gain: bool, bits: int = 8) -> np.ndarray:
+    """Gain ramp, N(0, sigma) noise (in 8-bit units) and quantisation to u8,
+    or for bits=16 to u16 with the 8-bit range scaled by 257 (so the noise
+    fills the low byte)."""
     h, w = img.shape
     if gain:
         img = img * (0.8 + 0.4 * np.arange(w, dtype=np.float64) / (w - 1))[None, :]
     img = img + sigma * rng.normal(h * w).reshape(h, w)
+    if bits == 16:
+        return np.rint(np.clip(img * 257.0, 0, 65535)).astype(np.uint16)
     return np.rint(np.clip(img, 0, 255)).astype(np.uint8)
 
 
 def scene(width: int, height: int, tw: int, th: int, count: int, angle_lo: float, angle_hi: float,
           noise: float, gain: bool, seed: int, integer_positions: bool = False,
-          min_separation: float | None = None, template_seed: int | None = None) -> Scene:
+          min_separation: float | None = None, template_seed: int | None = None, bits: int = 8) -> Scene:
     """Generic scene: ``count`` rotated pastes of one template (drawn from
-    ``template_seed``, by default the scene seed)."""
+    ``template_seed``, by default the scene seed); a u16 image for bits=16."""
     rng = SplitMix64(seed)
     base = blurred_noise(rng, height, width).astype(np.float64)
     tseed = seed if template_seed is None else template_seed
@@ -192,7 +197,7 @@ def scene(width: int, height: int, tw: int, th: int, count: int, angle_lo: float
             base[y0:y0 + th, x0:x0 + tw] = tmpl
         else:
             paste(base, tmpl, cy, cx, ang)
-    img = _finish(base, rng, noise, gain)
+    img = _finish(base, rng, noise, gain, bits)
     return Scene(img, tmpl, truth, seed)
 
 
@@ -222,6 +227,25 @@ def small_rot(width=256, height=192, t=32, count=2, extent_deg=20.0, seed_k=10, 
     e = math.radians(extent_deg)
     return scene(width, height, t, t, count, -e, e, noise, True, SEED_BASE + seed_k + 1000 * frame,
                  min_separation=1.3 * math.hypot(t, t), template_seed=SEED_BASE + seed_k)
+
+
+def small_rot16(width=256, height=192, t=32, count=2, extent_deg=20.0, seed_k=11, noise=4.0) -> Scene:
+    """Small rotated scene with a u16 ("uint2") search image."""
+    e = math.radians(extent_deg)
+    return scene(width, height, t, t, count, -e, e, noise, True, SEED_BASE + seed_k,
+                 min_separation=1.3 * math.hypot(t, t), template_seed=SEED_BASE + seed_k, bits=16)
+
+
+def plain16(width=160, height=128, t=32, count=2, seed_k=12, noise=3.0) -> Scene:
+    """u16 scene with pastes at integer positions and angle 0."""
+    return scene(width, height, t, t, count, 0.0, 0.0, noise, False, SEED_BASE + seed_k, integer_positions=True,
+                 bits=16)
+
+
+def workload_rot_2048_u16(seed_k: int = 1) -> Scene:
+    """The 2048x2048 rotated workload with a u16 search image."""
+    return scene(2048, 2048, 128, 128, 5, -math.pi / 6, math.pi / 6, 8.0, True, SEED_BASE + seed_k,
+                 min_separation=1.2 * math.hypot(128, 128), template_seed=SEED_BASE + seed_k, bits=16)
 
 
 @dataclass

@@ -12,8 +12,9 @@
 // where it is cheap to keep; the CPU-only multi-model caches are not restated
 // (they are designed to be result-neutral, matcher.cpp:1077-1124).
 //
-// Scope: single-channel u8 images and templates with a full rectangular
-// domain, single-model find (matcher.cpp:1063-1076).
+// Scope: single-channel u8 ("byte") and u16 ("uint2") search images, u8
+// templates with a full rectangular domain or persisted ones with any mask,
+// find (matcher.cpp:1063-1076) and find_models (matcher.cpp:1077-1124).
 //
 // Build: g++ -O2 -ffp-contract=off (the reference compiles its NCC kernels
 // with -ffp-contract=off, cmake/Targets.cmake:177); see oracle/Makefile.
@@ -822,6 +823,7 @@ struct Query {
   // for every query of find_models; `cache_samples` when it holds more than one
   // model, which converts the whole level 0 to doubles (matcher.cpp:596-611).
   bool cache = false, cache_samples = false;
+  bool u8 = true;                           // byte image: exact_byte_statistics (matcher.cpp:347)
   bool shared_full = false;                 // level 0 is the whole crop (shared_values)
   double guard_first = -1, guard = -1;      // background guard thresholds (matcher.cpp:912-927)
   Prefix fine_prefix;                       // row prefix statistics of the whole level 0 (built on first use)
@@ -832,9 +834,19 @@ static bool domain_contains(const Query& q, const Match& m) {  // matcher.cpp:24
   return r >= 0 && c >= 0 && r < long(q.img_h) && c < long(q.img_w);
 }
 
-static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& req, Trace* tr) {
+// Pixel (r, c) of a u8 or u16 image with `stride` bytes per row.
+static double pixel(const uint8_t* img, size_t stride, int bytes, size_t r, size_t c) {
+  if (bytes == 1) return double(img[r * stride + c]);
+  uint16_t v;
+  std::memcpy(&v, img + r * stride + 2 * c, 2);
+  return double(v);
+}
+
+static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& req, Trace* tr,
+                    int pixel_bytes = 1) {
   FindOpts o = req;
   Q.model = &model; Q.opt = o; Q.img_w = W; Q.img_h = H;
+  Q.u8 = pixel_bytes == 1;
   if (!W || !H) fail(INVALID_ARGUMENT, "one nonempty single-channel image required");
   if (model.levels.empty()) fail(INVALID_STATE, "model is empty");
   if (o.highest < 0 || o.lowest < 1) fail(INVALID_ARGUMENT, "invalid pyramid levels");
@@ -877,8 +889,15 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
   auto& pyr = Q.pyr;
   {
     Gray g0; g0.w = size_t(crop_rt - crop_c); g0.h = size_t(crop_b - crop_r);
-    g0.bytes = img + crop_r * stride + crop_c; g0.stride = stride; g0.dc = double(img[crop_r * stride + crop_c]);
+    g0.dc = pixel(img, stride, pixel_bytes, size_t(crop_r), size_t(crop_c));
     g0.sw = g0.w; g0.sh = g0.h;
+    if (Q.u8) { g0.bytes = img + crop_r * stride + crop_c; g0.stride = stride; }
+    else {  // u16: the whole crop as doubles, pixel - dc (matcher.cpp:415-433)
+      g0.v.resize(g0.w * g0.h);
+      for (size_t r = 0; r < g0.h; ++r)
+        for (size_t c = 0; c < g0.w; ++c)
+          g0.v[r * g0.w + c] = pixel(img, stride, 2, size_t(crop_r) + r, size_t(crop_c) + c) - g0.dc;
+    }
     pyr.push_back(std::move(g0));
   }
   for (size_t l = 1; l < level_count; ++l) {
@@ -919,7 +938,8 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
   std::sort(ca.begin(), ca.end(), [](const Patch* a, const Patch* b) { return a->angle < b->angle; });
   { size_t kept = 0; for (size_t i = 0; i < ca.size(); ++i) if (i % stride_a == 0 || i + 1 == ca.size()) ca[kept++] = ca[i]; ca.resize(kept); }
   if (tr) { tr->top = int(top); tr->finest = int(finest); tr->angle_stride = int(stride_a); tr->crop_r = crop_r; tr->crop_c = crop_c; }
-  Prefix top_prefix; const bool have_top_prefix = !coarse.bytes;
+  // row prefixes of the top level: byte images only (matcher.cpp:489)
+  Prefix top_prefix; const bool have_top_prefix = Q.u8 && !coarse.bytes;
   if (have_top_prefix) top_prefix = build_prefix(coarse);
   // support integral: matcher.cpp:498-503
   const size_t pitch = coarse.w + 1;
@@ -934,7 +954,7 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
   };
   const double scale = std::ldexp(1., int(top)), phase = (scale - 1) * .5;
   const bool absolute = model.opt.absolute;
-  const bool exact_top = top <= 2;  // u8 image: exact_byte_statistics && top<=2
+  const bool exact_top = Q.u8 && top <= 2;  // exact_byte_statistics && top<=2
   auto& cand = Q.cand;
   std::vector<double> scores(coarse.w * coarse.h);
   // coarse exhaustive NCC: matcher.cpp:542-594
@@ -960,6 +980,13 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
           if (!has_support(p, r, c, 3)) continue;
           for (int l = 0; l < 4; ++l)
             scores[r * coarse.w + c + l] = prefix_window(coarse.data(), top_prefix, coarse.w, p, r, c + l, absolute, exact_top, DotOrder::Four);
+          if (tr) { tr->coarse_windows += 4; tr->coarse_samples += 4.0 * p.s.size(); }
+        }
+      } else if (!coarse.bytes) {
+        // no row prefixes (u16 image): direct four-window form (matcher.cpp:565-571)
+        for (; c + 3 < ce2; c += 4) {
+          if (!has_support(p, r, c, 3)) continue;
+          direct_four(coarse.data(), coarse.w, p, r, c, absolute, scores.data() + r * coarse.w + c);
           if (tr) { tr->coarse_windows += 4; tr->coarse_samples += 4.0 * p.s.size(); }
         }
       }
@@ -1026,8 +1053,8 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
     }
     const Gray& tg = pyr[l - 1];
     Prefix pf; bool have_pf = false;
-    if (!tg.bytes && tg.sw == tg.w && tg.sh == tg.h && cand.size() >= 4) { pf = build_prefix(tg); have_pf = true; }
-    const bool exact_l = l <= 3;
+    if (Q.u8 && !tg.bytes && tg.sw == tg.w && tg.sh == tg.h && cand.size() >= 4) { pf = build_prefix(tg); have_pf = true; }
+    const bool exact_l = Q.u8 && l <= 3;
     const double step = std::ldexp(1., int(l - 1)), offset = (step - 1) * .5;
     for (auto& cd : cand) {
       Match best = cd; best.score = kMissing;
@@ -1221,8 +1248,9 @@ static bool refine(Query& Q, Match& cd) {
 }
 
 // find: matcher.cpp:1063-1076
-static std::vector<Match> find(const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& o, Trace* tr) {
-  Query Q; prepare(Q, img, W, H, stride, model, o, tr);
+static std::vector<Match> find(const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& o, Trace* tr,
+                               int pixel_bytes = 1) {
+  Query Q; prepare(Q, img, W, H, stride, model, o, tr, pixel_bytes);
   std::vector<Match> result;
   if (tr) { tr->refined.assign(Q.cand.size(), Match{}); tr->accepted_flag.assign(Q.cand.size(), -1); }
   const Level& lv0 = model.levels.front();
@@ -1253,8 +1281,8 @@ static std::vector<Match> find(const uint8_t* img, size_t W, size_t H, size_t st
 // Refines one given candidate exactly as find() would (used to compare the
 // device's speculative refinement with the oracle, candidate by candidate).
 static bool refine_one(const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& o,
-                       Match& cd) {
-  Query Q; prepare(Q, img, W, H, stride, model, o, nullptr);
+                       Match& cd, int pixel_bytes = 1) {
+  Query Q; prepare(Q, img, W, H, stride, model, o, nullptr, pixel_bytes);
   return refine(Q, cd);
 }
 
@@ -1367,7 +1395,7 @@ static bool same_options(const FindOpts& a, const FindOpts& b) {  // matcher.cpp
 static std::vector<MultiMatch> find_models(const uint8_t* img, size_t W, size_t H, size_t stride,
                                            const std::vector<const Model*>& models, const std::vector<FindOpts>& opts,
                                            Selection sel, bool per_model_quota_before_overlap,
-                                           std::vector<std::vector<Match>>* prepared = nullptr) {
+                                           std::vector<std::vector<Match>>* prepared = nullptr, int pixel_bytes = 1) {
   const bool cache_samples = models.size() > 1, unlimited = sel.num_matches == 0;
   bool cache_derivatives = false;
   for (const Model* m : models) if (m != models.front()) cache_derivatives = true;
@@ -1388,7 +1416,7 @@ static std::vector<MultiMatch> find_models(const uint8_t* img, size_t W, size_t 
     }
     auto Q = std::make_unique<Query>();
     Q->cache = true; Q->cache_samples = cache_samples;
-    prepare(*Q, img, W, H, stride, *models[i], opts[i], nullptr);
+    prepare(*Q, img, W, H, stride, *models[i], opts[i], nullptr, pixel_bytes);
     // default background guard: matcher.cpp:839-846 (AVX2 host, round to nearest)
     const Model& m = *models[i]; const FindOpts& o = Q->opt;
     if (!Q->empty && !Q->finest && cache_derivatives && unlimited && Q->cand.size() >= 32 && o.subpixel && !o.num_matches &&
@@ -1513,18 +1541,21 @@ struct OrcFindOut {
   long crop_r, crop_c, store_r, store_c, store_h, store_w;
 };
 
-// Runs find. results: cap x 4 (row, col, angle, score) in the reference's
+// Runs find on a u8 or u16 image (pixel_bytes 1 or 2, stride in bytes).
+// results: cap x 4 (row, col, angle, score) in the reference's
 // internal convention (centroid coordinates, no output shift).  Optional
 // traces: coarse (cap_c x 4), prepared (cap_p x 4), refined (cap_p x 5 with
 // the verdict in column 4).
-int orc_find(void* mp, const uint8_t* img, int w, int h, int stride, double start, double extent, double min_score,
-             int num_matches, double max_overlap, int subpixel, int highest, int lowest, double* results, int cap,
-             double* coarse, int cap_c, double* prepared, int cap_p, double* refined, OrcFindOut* info) {
+int orc_find(void* mp, const uint8_t* img, int w, int h, int stride, int pixel_bytes, double start, double extent,
+             double min_score, int num_matches, double max_overlap, int subpixel, int highest, int lowest,
+             double* results, int cap, double* coarse, int cap_c, double* prepared, int cap_p, double* refined,
+             OrcFindOut* info) {
   return guard([&] {
     FindOpts o; o.start = start; o.extent = extent; o.min_score = min_score; o.num_matches = size_t(num_matches);
     o.max_overlap = max_overlap; o.subpixel = subpixel != 0; o.highest = highest; o.lowest = lowest;
     Trace tr;
-    auto res = find(img, size_t(w), size_t(h), size_t(stride), *static_cast<Model*>(mp), o, &tr);
+    if (pixel_bytes != 1 && pixel_bytes != 2) fail(INVALID_ARGUMENT, "pixel type must be byte or uint2");
+    auto res = find(img, size_t(w), size_t(h), size_t(stride), *static_cast<Model*>(mp), o, &tr, pixel_bytes);
     auto put = [](const Match& m, double* d) { d[0] = m.row; d[1] = m.col; d[2] = m.angle; d[3] = m.score; };
     for (size_t i = 0; i < res.size() && int(i) < cap; ++i) put(res[i], results + 4 * i);
     if (coarse) for (size_t i = 0; i < tr.coarse.size() && int(i) < cap_c; ++i) put(tr.coarse[i], coarse + 4 * i);
@@ -1574,7 +1605,8 @@ double orc_continuous(void* mp, const uint8_t* img, int w, int h, int stride, do
 // ABI derives it (num_matches -1 / max_overlap -1 mean per model).  results:
 // cap x 5 (row, col, angle, score, model) in the internal convention.
 // prepared (optional): n_models x cap_p x 4, with prepared_n the list sizes.
-int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, int h, int stride, const double* opts,
+int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, int h, int stride, int pixel_bytes,
+                    const double* opts,
                     int64_t sel_num, double sel_overlap, int per_model_quota, double* results, int cap, int* n_result,
                     double* prepared, int cap_p, int* prepared_n) {
   return guard([&] {
@@ -1589,7 +1621,7 @@ int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, i
     }
     std::vector<std::vector<Match>> prep;
     auto res = find_models(img, size_t(w), size_t(h), size_t(stride), models, o, Selection{sel_num, sel_overlap},
-                           per_model_quota != 0, prepared ? &prep : nullptr);
+                           per_model_quota != 0, prepared ? &prep : nullptr, pixel_bytes);
     *n_result = int(res.size());
     for (size_t i = 0; i < res.size() && int(i) < cap; ++i) {
       double* d = results + 5 * i;

@@ -51,7 +51,7 @@ def lib():
         L.orc_level_gray.argtypes = [P, I, P]
         L.orc_pyramid.argtypes = [P, I, I, I, I, P]
         L.orc_row_prefix.argtypes = [P, I, I, P, P]
-        L.orc_find.argtypes = [P, P, I, I, I, D, D, D, I, D, I, I, I, P, I, P, I, P, I, P, C.POINTER(FindOut)]
+        L.orc_find.argtypes = [P, P, I, I, I, I, D, D, D, I, D, I, I, I, P, I, P, I, P, I, P, C.POINTER(FindOut)]
         L.orc_zncc.argtypes = [P, P, I, I]
         L.orc_zncc.restype = D
         L.orc_overlap.argtypes = [D, D, D, D, D, D, I, I, D, D]
@@ -59,7 +59,7 @@ def lib():
         L.orc_continuous.argtypes = [P, P, I, I, I, D, D, D]
         L.orc_continuous.restype = D
         L.orc_refine_one.argtypes = [P, P, I, I, I, D, D, D, I, D, I, P]
-        L.orc_find_models.argtypes = [P, I, P, I, I, I, P, C.c_int64, D, I, P, I, P, P, I, P]
+        L.orc_find_models.argtypes = [P, I, P, I, I, I, I, P, C.c_int64, D, I, P, I, P, P, I, P]
         L.orc_box_overlap.argtypes = [D, D, D, P, D, D, D, P]
         L.orc_box_overlap.restype = D
         L.orc_serialize.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -145,14 +145,16 @@ class Model:
 
     def find(self, image: np.ndarray, start: float, extent: float, min_score: float, num_matches: int,
              max_overlap: float, subpixel: bool = True, highest: int = 0, lowest: int = 1, trace: bool = False):
-        img = np.ascontiguousarray(image, dtype=np.uint8)
+        """find on a u8 or u16 image (the dtype selects the pixel type)."""
+        img = search_image(image)
         cap = 4096
         res = np.zeros((cap, 4))
         coarse = np.zeros((20000, 4)) if trace else None
         prep = np.zeros((20000, 4)) if trace else None
         refd = np.zeros((20000, 5)) if trace else None
         info = FindOut()
-        st = lib().orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], start, extent, min_score,
+        st = lib().orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize, start,
+                            extent, min_score,
                             num_matches, max_overlap, int(subpixel), highest, lowest, _ptr(res), cap,
                             _ptr(coarse) if trace else None, 20000, _ptr(prep) if trace else None, 20000,
                             _ptr(refd) if trace else None, C.byref(info))
@@ -206,13 +208,19 @@ def overlap(a, b, w, h, orow, ocol) -> float:
     return lib().orc_overlap(a[0], a[1], a[2], b[0], b[1], b[2], w, h, orow, ocol)
 
 
+def search_image(image: np.ndarray) -> np.ndarray:
+    """A contiguous u16 array for u16 input, else u8."""
+    return np.ascontiguousarray(image, dtype=np.uint16 if image.dtype == np.uint16 else np.uint8)
+
+
 def find_models(models: list, image: np.ndarray, opts: list, selection_matches: int = -1,
                 selection_overlap: float = -1.0, per_model_quota: bool = True, prepared: bool = False):
     """find_models over oracle models.  opts: one dict per model with keys
     start, extent, min_score, max_overlap, num_matches, subpixel, highest,
     lowest.  Returns (n, 5): row, col, angle, score (internal convention),
-    model index; with prepared=True also each model's prepared list."""
-    img = np.ascontiguousarray(image, dtype=np.uint8)
+    model index; with prepared=True also each model's prepared list.  The
+    image is u8 or u16 (by dtype)."""
+    img = search_image(image)
     n = len(models)
     handles = (C.c_void_p * n)(*[m.h.value for m in models])
     o = np.zeros((n, 8))
@@ -224,7 +232,7 @@ def find_models(models: list, image: np.ndarray, opts: list, selection_matches: 
     nres = C.c_int()
     prep = np.zeros((n, cap_p, 4)) if prepared else None
     prep_n = np.zeros(n, np.int32) if prepared else None
-    st = lib().orc_find_models(handles, n, _ptr(img), img.shape[1], img.shape[0], img.shape[1], _ptr(o),
+    st = lib().orc_find_models(handles, n, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize, _ptr(o),
                                int(selection_matches), float(selection_overlap), int(per_model_quota), _ptr(res), cap,
                                C.byref(nres), _ptr(prep) if prepared else None, cap_p,
                                _ptr(prep_n) if prepared else None)

@@ -97,7 +97,7 @@ def test_row_prefix_and_support_bit_exact(capi, oracle):
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     h, w = img.shape
     src = torch.from_numpy(img).cuda()
-    p1 = torch.zeros((h, w + 1), dtype=torch.int32, device="cuda")
+    p1 = torch.zeros((h, w + 1), dtype=torch.int64, device="cuda")
     p2 = torch.zeros((h, w + 1), dtype=torch.int64, device="cuda")
     nz = torch.zeros((h, w + 1), dtype=torch.int32, device="cuda")
     sup = torch.zeros((h + 1, w + 1), dtype=torch.int32, device="cuda")

@@ -1,0 +1,88 @@
+"""u16 ("uint2") search images in the CPU oracle (matcher.cpp:347, 401-433,
+489, 565-577, 647, 736).
+
+The reference holds no u16 tests or fixtures.  Its u16 path differs from the
+byte path only in how the same FP64 values are produced and scored: level 0
+becomes pixel - dc doubles over the whole crop, the top level has no row
+prefixes (direct four-window and scalar forms), tracking has no prefixes and
+the exact-byte-lattice rule is off.  So the u16 oracle is pinned by
+ * intensity-affine invariance of NCC: a u8 image I and the u16 images
+   257 I and 200 I + 5000 give the byte path's result (pinned by the u8
+   checks) up to rounding, for find and find_models;
+ * a closed form on genuinely 16-bit data: at integer positions and angle 0
+   without subpixel fitting, each match is the pasted centroid and its
+   score is the textbook zero-mean normalised correlation of the template
+   with the image window, computed here in numpy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+E = math.radians(20)
+
+
+def affine_images(img):
+    i = img.astype(np.uint16)
+    return [i * 257, i * 200 + 5000]
+
+
+def test_affine_u16_images_give_the_byte_result(oracle, scene_small):
+    m = oracle.Model(scene_small.template, levels=0, start=-E, extent=2 * E)
+    want = m.find(scene_small.image, -E, 2 * E, 0.7, 0, 0.5)
+    assert len(want) == 2
+    for img in affine_images(scene_small.image):
+        got = m.find(img, -E, 2 * E, 0.7, 0, 0.5)
+        assert got.shape == want.shape
+        assert np.abs(got - want).max() < 1e-9
+
+
+def test_affine_u16_images_find_models(oracle):
+    from mvkit_amd import synth
+    s = synth.small_multi()
+    e = math.radians(16)
+    models = [oracle.Model(t, levels=3, start=-e, extent=2 * e) for t in s.templates]
+    opts = [dict(start=-e, extent=2 * e, min_score=0.7, max_overlap=0.5, num_matches=0)] * 2
+    want = oracle.find_models(models, s.image, opts, 0, 0.5)
+    assert len(want) == 4
+    for img in affine_images(s.image):
+        got = oracle.find_models(models, img, opts, 0, 0.5)
+        assert got.shape == want.shape
+        assert np.array_equal(got[:, 4], want[:, 4])
+        assert np.abs(got[:, :4] - want[:, :4]).max() < 1e-9
+
+
+def textbook_zncc(t, w):
+    a = t.astype(np.float64) - t.mean()
+    b = w.astype(np.float64) - w.mean()
+    return float((a * b).sum() / math.sqrt((a * a).sum() * (b * b).sum()))
+
+
+def test_integer_positions_score_the_textbook_zncc():
+    """16-bit content at integer positions: the discrete result is the paste
+    centroid and the textbook ZNCC of template and window."""
+    from oracle import pyoracle as oracle
+    from mvkit_amd import synth
+    s = synth.plain16()
+    assert s.image.dtype == np.uint16 and s.image.max() > 40000 and np.any(s.image % 257)
+    t = s.template
+    th, tw = t.shape
+    m = oracle.Model(t, levels=2, start=0.0, extent=0.0)
+    got = m.find(s.image, 0.0, 0.0, 0.7, 0, 0.5, subpixel=False)
+    assert len(got) == len(s.truth)
+    for cy, cx, _ in s.truth:
+        hit = [g for g in got if abs(g[0] - cy) < 1e-9 and abs(g[1] - cx) < 1e-9]
+        assert len(hit) == 1 and hit[0][2] == 0.0
+        y0, x0 = int(round(cy - (th - 1) / 2)), int(round(cx - (tw - 1) / 2))
+        want = textbook_zncc(t, s.image[y0:y0 + th, x0:x0 + tw])
+        assert hit[0][3] == pytest.approx(want, abs=1e-12)
+
+
+def test_oracle_rejects_other_pixel_sizes(oracle, scene_small):
+    m = oracle.Model(scene_small.template, levels=0, start=-E, extent=2 * E)
+    img = np.ascontiguousarray(scene_small.image)
+    res = np.zeros((16, 4))
+    info = oracle.FindOut()
+    st = oracle.lib().orc_find(m.h, oracle._ptr(img), img.shape[1], img.shape[0], img.shape[1], 4, -E, 2 * E, 0.7, 0,
+                               0.5, 1, 0, 1, oracle._ptr(res), 16, None, 0, None, 0, None, oracle.C.byref(info))
+    assert st == -1
