@@ -112,6 +112,25 @@ MV_API MvStatus mv_gen_image1(MvObject* Image, MvTuple Type, MvTuple Width, MvTu
 MV_API MvStatus mv_gen_image1_extern(MvObject* Image, MvTuple Type, MvTuple Width, MvTuple Height,
                                      MvTuple PixelPointer, MvTuple ClearProc);
 
+/* Regions and image domains, for reduced-domain (ROI) templates and searches.
+ * mv_gen_region_runs: include/mvkit/c/mv_region.h:40 (src/c_api/region_api.cpp:31-78);
+ *   one region from runs (row, first column, last column; integers, or reals
+ *   rounded by +0.5 and truncation), sorted and merged.
+ * mv_get_region_runs: mv_region.h:54; the canonical runs of one region.
+ * mv_gen_rectangle1: mv_region.h:71 (region_control_api.cpp:102-113); one
+ *   rectangle per control position, corners rounded to the nearest integer.
+ * mv_reduce_domain: mv_image_processing.h:166 (image_processing_api.cpp:239-252,
+ *   core/image.cpp:267-320); the image with its domain intersected with the
+ *   first region clipped to the image.
+ * mv_full_domain: mv_image_processing.h:176; the image with its full domain.
+ * mv_get_domain: mv_image_processing.h:155; the image's domain as a region. */
+MV_API MvStatus mv_gen_region_runs(MvObject* Region, MvTuple Row, MvTuple ColumnBegin, MvTuple ColumnEnd);
+MV_API MvStatus mv_get_region_runs(MvObject Region, MvTuple* Row, MvTuple* ColumnBegin, MvTuple* ColumnEnd);
+MV_API MvStatus mv_gen_rectangle1(MvObject* Rectangle, MvTuple Row1, MvTuple Column1, MvTuple Row2, MvTuple Column2);
+MV_API MvStatus mv_reduce_domain(MvObject Image, MvObject Region, MvObject* ImageReduced);
+MV_API MvStatus mv_full_domain(MvObject Image, MvObject* ImageFull);
+MV_API MvStatus mv_get_domain(MvObject Image, MvObject* Domain);
+
 /* Handles: include/mvkit/c/mv_handle.h:40.  Invalidates every alias. */
 MV_API MvStatus mv_clear_handle(MvTuple Handle);
 /* Serialized items: include/mvkit/c/mv_handle.h:63-97

@@ -69,6 +69,15 @@ MV_API MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* d_im
                                             int64_t pitch, const MvAmdFindOptions* options, double* results,
                                             size_t cap, size_t* count, MvAmdFindStats* stats);
 
+/* Find on an image object (host pixels, u8 or u16, with its reduced domain
+ * if it has one) with the extension options (correlation form, profiling;
+ * pixel_type is taken from the image).  results: cap x 4 doubles (row, col,
+ * angle, score) in the internal (centroid) convention.  Replaces
+ * mv_find_ncc_model (mvkit include/mvkit/c/mv_ncc_matching.h:71-74) minus
+ * the output shift, for callers that choose the correlation form. */
+MV_API MvStatus mvamd_find_ncc_model_image(MvObject Image, MvTuple ModelID, const MvAmdFindOptions* options,
+                                           double* results, size_t cap, size_t* count, MvAmdFindStats* stats);
+
 /* find_ncc_models on a u8 or u16 image given as a HIP device pointer.  options holds
  * one entry or one per model (n_options); the selection is what
  * mv_find_ncc_models derives from one-value controls: selection_matches

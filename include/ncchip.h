@@ -118,6 +118,24 @@ int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int max_w, int
 int ncchip_storage0(NccFrame* frames, int n_frames, const int32_t* count, const unsigned long long* extent,
                     double margin, void* stream);
 
+/* Reduced search domains (matcher.cpp:376-393, 505-534, 631-645).  Frames
+ * whose dom_sat is null are skipped by all three.
+ * ncchip_domain_sat: builds each frame's summed-area table of its domain mask
+ * over the crop from dom_runs / dom_row (grid: max crop height, max crop
+ * width).
+ * ncchip_domain_planes: sets the coarse score of every anchor whose pose may
+ * not reach the domain (may_contain at spacing `scale`) to missing; between
+ * the coarse correlation and ncchip_peaks, same plane order flag `hwa`.
+ * ncchip_domain_snap: moves the candidates of a list (frame f's at
+ * cand + f * cap, count[f] of them) that are not on a domain pixel to the
+ * nearest domain pixel within 4 rows and columns, else scores them missing. */
+int ncchip_domain_sat(const NccFrame* frames, int n_frames, int max_h, int max_w, void* stream);
+int ncchip_domain_planes(const NccFrame* frames, int n_frames, int level, int max_w, int max_h,
+                         const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, double scale, double phase,
+                         int hwa, void* stream);
+int ncchip_domain_snap(const NccFrame* frames, int n_frames, NccMatch* cand, const int32_t* count, int cap,
+                       void* stream);
+
 /* Every kernel taking `dc_src` reads the reference pixel dc (the first
  * pixel of the search crop, matcher.cpp:410-413) from that device address;
  * integer levels use D = dc << shift. */

@@ -28,15 +28,18 @@ PUBLIC_SYMBOLS = [
     "mv_get_last_error", "mv_get_last_error_info", "mv_create_tuple", "mv_copy_tuple", "mv_destroy_tuple",
     "mv_length_tuple", "mv_get_type", "mv_get_i", "mv_get_d", "mv_get_s", "mv_get_h", "mv_resize_tuple", "mv_set_i",
     "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
+    "mv_gen_region_runs", "mv_get_region_runs", "mv_gen_rectangle1", "mv_reduce_domain", "mv_full_domain",
+    "mv_get_domain",
     "mv_clear_handle", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
     "mv_get_serialized_item_ptr", "mv_write_ncc_model", "mv_read_ncc_model", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
-    "mvamd_find_ncc_model_device", "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
+    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_image", "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
                   "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma",
-                  "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine"]
+                  "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine",
+                  "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap"]
 
 
 class StringView(C.Structure):
@@ -111,6 +114,12 @@ def lib() -> C.CDLL:
         L.mv_clear_obj.argtypes = [P]
         L.mv_gen_image1.argtypes = [C.POINTER(P), P, P, P, P]
         L.mv_gen_image1_extern.argtypes = [C.POINTER(P), P, P, P, P, P]
+        L.mv_gen_region_runs.argtypes = [C.POINTER(P), P, P, P]
+        L.mv_get_region_runs.argtypes = [P] + [C.POINTER(P)] * 3
+        L.mv_gen_rectangle1.argtypes = [C.POINTER(P), P, P, P, P]
+        L.mv_reduce_domain.argtypes = [P, P, C.POINTER(P)]
+        L.mv_full_domain.argtypes = [P, C.POINTER(P)]
+        L.mv_get_domain.argtypes = [P, C.POINTER(P)]
         L.mv_clear_handle.argtypes = [P]
         L.mv_create_ncc_model.argtypes = [P, P, P, P, P, P, C.POINTER(P)]
         L.mv_get_ncc_model_params.argtypes = [P] + [C.POINTER(P)] * 5
@@ -124,6 +133,8 @@ def lib() -> C.CDLL:
         L.mv_get_serialized_item_ptr.argtypes = [P, C.POINTER(P), C.POINTER(P)]
         L.mvamd_find_ncc_models_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), S, I64, D, P, S,
                                                    C.POINTER(S)]
+        L.mvamd_find_ncc_model_image.argtypes = [P, P, C.POINTER(FindOptions), P, S, C.POINTER(S),
+                                                 C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
                                                   C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), P, S, P,
@@ -207,12 +218,51 @@ class Tuple:
             self.h = None
 
 
-class Image:
-    """Image object holding a copy of a u8 (or u16) plane."""
+class Region:
+    """Region object: from runs (n x 3: row, first column, last column) via
+    mv_gen_region_runs, or rectangles via mv_gen_rectangle1."""
 
-    def __init__(self, pixels: np.ndarray | None):
+    def __init__(self, runs=None, handle=None):
+        if handle is not None:
+            self.h = handle
+            return
+        r = np.asarray(runs if runs is not None else np.zeros((0, 3)), dtype=np.int64).reshape(-1, 3)
         h = C.c_void_p()
-        if pixels is None:
+        t = [Tuple([int(x) for x in r[:, k]]) for k in range(3)]
+        _check(lib().mv_gen_region_runs(C.byref(h), *[x.h for x in t]))
+        self.h = h
+
+    @classmethod
+    def rectangle1(cls, row1, col1, row2, col2) -> "Region":
+        h = C.c_void_p()
+        t = [Tuple(v) for v in (row1, col1, row2, col2)]
+        _check(lib().mv_gen_rectangle1(C.byref(h), *[x.h for x in t]))
+        return cls(handle=h)
+
+    def runs(self) -> np.ndarray:
+        outs = [C.c_void_p() for _ in range(3)]
+        _check(lib().mv_get_region_runs(self.h, *[C.byref(o) for o in outs]))
+        cols = [Tuple(handle=o).values() for o in outs]
+        return np.array(cols, dtype=np.int64).T.reshape(-1, 3)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.mv_clear_obj(self.h)
+            except Exception:  # interpreter shutdown
+                pass
+            self.h = None
+
+
+class Image:
+    """Image object holding a copy of a u8 (or u16) plane, optionally with a
+    reduced domain (runs n x 3: row, first column, last column, or a Region)."""
+
+    def __init__(self, pixels: np.ndarray | None, domain=None, handle=None):
+        h = C.c_void_p()
+        if handle is not None:
+            h = handle
+        elif pixels is None:
             _check(lib().mv_gen_empty_obj(C.byref(h)))
         else:
             a = np.ascontiguousarray(pixels)
@@ -220,6 +270,26 @@ class Image:
             args = [Tuple(kind), Tuple(int(a.shape[1])), Tuple(int(a.shape[0])), Tuple(int(a.ctypes.data))]
             _check(lib().mv_gen_image1(C.byref(h), *[t.h for t in args]))
         self.h = h
+        if domain is not None:
+            reduced = self.reduce_domain(domain)
+            self.h, reduced.h = reduced.h, self.h
+
+    def reduce_domain(self, region) -> "Image":
+        """mv_reduce_domain with a Region or runs."""
+        g = region if isinstance(region, Region) else Region(region)
+        out = C.c_void_p()
+        _check(lib().mv_reduce_domain(self.h, g.h, C.byref(out)))
+        return Image(None, handle=out)
+
+    def full_domain(self) -> "Image":
+        out = C.c_void_p()
+        _check(lib().mv_full_domain(self.h, C.byref(out)))
+        return Image(None, handle=out)
+
+    def domain(self) -> "Region":
+        out = C.c_void_p()
+        _check(lib().mv_get_domain(self.h, C.byref(out)))
+        return Region(None, handle=out)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -242,9 +312,10 @@ def _real(v):
 class NccModel:
     """NCC model handle (mirrors mvkit's MvNCCModel, mv_ncc_matching.hpp)."""
 
-    def __init__(self, template: np.ndarray, num_levels="auto", angle_start=0.0, angle_extent=0.0,
+    def __init__(self, template, num_levels="auto", angle_start=0.0, angle_extent=0.0,
                  angle_step="auto", metric="use_polarity"):
-        img = Image(template)
+        """template: an array or an Image (which may have a reduced domain)."""
+        img = template if isinstance(template, Image) else Image(template)
         out = C.c_void_p()
         args = [_to_tuple(num_levels), _to_tuple(float(angle_start)), _to_tuple(float(angle_extent)),
                 _to_tuple(angle_step), _to_tuple(metric)]
@@ -340,6 +411,17 @@ class NccModel:
         mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
                            lowest, form, screen, mask, int(vector_tracking), int(pixel_type))
+
+    def find_image(self, image, opts: FindOptions, cap: int = 4096):
+        """mvamd_find_ncc_model_image: find on an Image (or array) with the
+        extension options; poses in the internal convention."""
+        img = image if isinstance(image, Image) else Image(image)
+        res = np.zeros((cap, 4))
+        n = C.c_size_t()
+        st = FindStats()
+        _check(lib().mvamd_find_ncc_model_image(img.h, self.tuple.h, C.byref(opts), _ptr(res), cap, C.byref(n),
+                                                C.byref(st)))
+        return res[:min(n.value, cap)].copy(), st
 
     def find_device(self, d_image: int, width: int, height: int, pitch: int, opts: FindOptions, cap: int = 4096):
         """Find on a device-resident u8 or u16 image (integer device address;

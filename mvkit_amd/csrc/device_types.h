@@ -85,6 +85,18 @@ typedef struct NccFrame {
   uint32_t* nz;              // top-level row prefix of [S != D]
   double* planes;            // coarse score planes [angle][h][w]
   const int32_t* ranges;     // per coarse angle: valid anchor range {rb, re, cb, ce}
+  // Reduced search domain (image.hpp:108-113); dom_sat null: the full image.
+  // dom_sat: summed-area table of the domain mask over the crop, (level-0
+  // height + 1) x (width + 1) u32, entry (r, c) = domain pixels in crop rows
+  // < r and columns < c.  dom_runs: the domain's runs as int4 {row, first
+  // column, last column, 0} in image coordinates, dom_row[r] the index of
+  // crop row r's first run (level-0 height + 1 entries).
+  uint32_t* dom_sat;
+  const int32_t* dom_runs;
+  const int32_t* dom_row;
+  int32_t dom_box[4];        // bounding box {row_min, col_min, row_max, col_max}
+  int32_t dom_sparse;        // area < 0.7 x bounding box area (matcher.cpp:505)
+  int32_t dom_pad;
 } NccFrame;
 
 #ifdef __cplusplus

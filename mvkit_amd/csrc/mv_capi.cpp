@@ -60,16 +60,29 @@ struct MvTupleImpl {
   std::vector<Value> values;
 };
 
-struct MvObjectImpl {
-  bool empty = true;
-  size_t width = 0, height = 0;
-  int pixel_bytes = 1;  // 1: byte, 2: uint2
+// Pixel storage shared by an image and the images reduced from it.
+struct PixelStore {
   std::vector<uint8_t> owned;
   const uint8_t* pixels = nullptr;
-  void (*clear)(void*) = nullptr;
-  ~MvObjectImpl() {
+  void (*clear)(void*) = nullptr;  // caller's release procedure (gen_image1_extern)
+  ~PixelStore() {
     if (clear && pixels) clear(const_cast<uint8_t*>(pixels));
   }
+};
+
+// An iconic object: an empty object array, one image (with an optional
+// reduced domain, image.hpp:108-113) or an array of regions.
+struct MvObjectImpl {
+  enum Kind { kNone, kImage, kRegion } kind = kNone;
+  bool empty = true;  // no object
+  // image
+  size_t width = 0, height = 0;
+  int pixel_bytes = 1;  // 1: byte, 2: uint2
+  std::shared_ptr<PixelStore> store;
+  const uint8_t* pixels = nullptr;
+  std::shared_ptr<const mvamd::Region> domain;  // null: the full image
+  // regions
+  std::vector<std::shared_ptr<const mvamd::Region>> regions;
 };
 
 // ------------------------------------------------------------- error state
@@ -192,8 +205,16 @@ void push(MvTupleImpl& t, double v) { Value x; x.kind = MV_VALUE_REAL64; x.d = v
 void push(MvTupleImpl& t, int64_t v) { Value x; x.kind = MV_VALUE_INT64; x.i = v; t.values.push_back(std::move(x)); }
 void push(MvTupleImpl& t, const char* v) { Value x; x.kind = MV_VALUE_STRING; x.s = v; t.values.push_back(std::move(x)); }
 
+// IconicTuple::require_kind (iconic_tuple.cpp:198-210): every object of the
+// array is of the kind (an empty array is of any kind).
+void require_kind(MvObject o, MvObjectImpl::Kind kind, const char* op) {
+  if (!o->empty && o->kind != kind)
+    fail(mvamd::kTypeMismatch, mvamd::kErrInvalidType, op, "Objects", "iconic tuple contains an incompatible object kind", 0);
+}
+
 const MvObjectImpl& single_image(MvObject o, const char* op, const char* param) {
   required(o, op, param);
+  require_kind(o, MvObjectImpl::kImage, op);
   if (o->empty) fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, param, "one image required");
   return *o;
 }
@@ -549,9 +570,13 @@ extern "C" MvStatus mv_gen_image1(MvObject* Image, MvTuple Type, MvTuple Width, 
     int bytes; size_t w, h; uintptr_t addr;
     image_controls(op, Type, Width, Height, PixelPointer, bytes, w, h, addr);
     auto img = std::make_unique<MvObjectImpl>();
+    img->kind = MvObjectImpl::kImage;
     img->empty = false; img->width = w; img->height = h; img->pixel_bytes = bytes;
-    img->owned.assign(reinterpret_cast<const uint8_t*>(addr), reinterpret_cast<const uint8_t*>(addr) + w * h * bytes);
-    img->pixels = img->owned.data();
+    img->store = std::make_shared<PixelStore>();
+    img->store->owned.assign(reinterpret_cast<const uint8_t*>(addr),
+                             reinterpret_cast<const uint8_t*>(addr) + w * h * bytes);
+    img->store->pixels = img->store->owned.data();
+    img->pixels = img->store->pixels;
     *Image = img.release();
   });
 }
@@ -565,10 +590,152 @@ extern "C" MvStatus mv_gen_image1_extern(MvObject* Image, MvTuple Type, MvTuple 
     const Input cp(ClearProc, op, "ClearProc");
     const int64_t proc = cp.size() ? cp.scalar_integer() : 0;
     auto img = std::make_unique<MvObjectImpl>();
+    img->kind = MvObjectImpl::kImage;
     img->empty = false; img->width = w; img->height = h; img->pixel_bytes = bytes;
-    img->pixels = reinterpret_cast<const uint8_t*>(addr);
-    img->clear = reinterpret_cast<void (*)(void*)>(uintptr_t(proc));
+    img->store = std::make_shared<PixelStore>();
+    img->store->pixels = reinterpret_cast<const uint8_t*>(addr);
+    img->store->clear = reinterpret_cast<void (*)(void*)>(uintptr_t(proc));
+    img->pixels = img->store->pixels;
     *Image = img.release();
+  });
+}
+
+// ================================================================ regions
+namespace {
+std::unique_ptr<MvObjectImpl> region_object(std::vector<mvamd::Region> regions) {
+  auto o = std::make_unique<MvObjectImpl>();
+  o->kind = MvObjectImpl::kRegion;
+  o->empty = regions.empty();
+  for (auto& g : regions) o->regions.push_back(std::make_shared<const mvamd::Region>(std::move(g)));
+  return o;
+}
+// Run coordinate of a control value: integers as they are, reals rounded by
+// +0.5 and truncation (region_api.cpp:39-46).
+int32_t run_coordinate(const Input& in, size_t i) {
+  double v;
+  if (in.kind(i) == MV_VALUE_INT64) v = double(in.integer(i));
+  else v = std::trunc(in.number(i) + .5);
+  if (!std::isfinite(v) || v < -2147483648.0 || v > 2147483647.0)
+    in.bad(mvamd::kErrIndexOutOfRange, int64_t(i), "run coordinate exceeds int32 range");
+  return int32_t(v);
+}
+}  // namespace
+
+extern "C" MvStatus mv_gen_region_runs(MvObject* Region, MvTuple Row, MvTuple ColumnBegin, MvTuple ColumnEnd) {
+  return translate([&] {
+    constexpr auto op = "gen_region_runs";
+    required(Region, op, "Region");
+    const Input rows(Row, op, "Row"), begins(ColumnBegin, op, "ColumnBegin"), ends(ColumnEnd, op, "ColumnEnd");
+    const size_t n = rows.size();
+    if (begins.size() != n) begins.bad(mvamd::kErrInvalidCardinality, -1, "equal run coordinate lengths required");
+    if (ends.size() != n) ends.bad(mvamd::kErrInvalidCardinality, -1, "equal run coordinate lengths required");
+    std::vector<mvamd::RegionRun> runs;
+    runs.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+      const int32_t b = run_coordinate(begins, i), e = run_coordinate(ends, i);
+      if (b > e) ends.bad(mvamd::kErrInvalidValue, int64_t(i), "run end must not precede its beginning");
+      runs.push_back({run_coordinate(rows, i), b, e});
+    }
+    std::vector<mvamd::Region> one;
+    one.push_back(algorithm(op, [&] { return mvamd::Region::from_runs(std::move(runs)); }));
+    *Region = region_object(std::move(one)).release();
+  });
+}
+
+extern "C" MvStatus mv_get_region_runs(MvObject Region, MvTuple* Row, MvTuple* ColumnBegin, MvTuple* ColumnEnd) {
+  return translate([&] {
+    constexpr auto op = "get_region_runs";
+    Outputs<3> out(op, {Row, ColumnBegin, ColumnEnd});
+    required(Region, op, "Region");
+    require_kind(Region, MvObjectImpl::kRegion, op);
+    if (Region->regions.size() > 1)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, "Region", "one region or a zero-object array required");
+    if (!Region->regions.empty())
+      for (const auto& r : Region->regions[0]->runs) {
+        push(out[0], int64_t(r.row));
+        push(out[1], int64_t(r.cb));
+        push(out[2], int64_t(r.ce));
+      }
+    out.commit();
+  });
+}
+
+extern "C" MvStatus mv_gen_rectangle1(MvObject* Rectangle, MvTuple Row1, MvTuple Column1, MvTuple Row2,
+                                      MvTuple Column2) {
+  return translate([&] {
+    constexpr auto op = "mv_gen_rectangle1";
+    required(Rectangle, op, "Rectangle");
+    const Input in[4] = {Input(Row1, op, "Row1"), Input(Column1, op, "Column1"), Input(Row2, op, "Row2"),
+                         Input(Column2, op, "Column2")};
+    // generation_count (region_control_api.cpp:20-27)
+    size_t count = 0;
+    bool none = false;
+    for (const auto& x : in) { count = std::max(count, x.size()); none = none || x.size() == 0; }
+    if (none) count = 0;
+    for (const auto& x : in)
+      if (x.size() != 1 && x.size() != count)
+        x.bad(mvamd::kErrInvalidCardinality, -1, "generation controls must have matching lengths or be scalar");
+    std::vector<mvamd::Region> out;
+    for (size_t i = 0; i < count; ++i) {
+      int64_t v[4];
+      for (int k = 0; k < 4; ++k) {
+        const size_t j = in[k].size() == 1 ? 0 : i;
+        const double x = in[k].number(j);
+        if (!std::isfinite(x)) in[k].bad(mvamd::kErrInvalidValue, int64_t(j), "finite control value required");
+        if (x < -2147483648.0 || x > 2147483647.0) in[k].bad(mvamd::kErrIndexOutOfRange, int64_t(j), "coordinate exceeds int32 range");
+        v[k] = int64_t(std::round(x));
+      }
+      out.push_back(mvamd::Region::rectangle(v[0], v[1], v[2], v[3]));
+    }
+    *Rectangle = region_object(std::move(out)).release();
+  });
+}
+
+// ================================================================ domains
+extern "C" MvStatus mv_reduce_domain(MvObject Image, MvObject Region, MvObject* ImageReduced) {
+  return translate([&] {
+    constexpr auto op = "reduce_domain";
+    required(Image, op, "Image");
+    required(Region, op, "Region");
+    required(ImageReduced, op, "ImageReduced");
+    require_kind(Region, MvObjectImpl::kRegion, op);
+    require_kind(Image, MvObjectImpl::kImage, op);
+    auto o = std::make_unique<MvObjectImpl>();
+    if (!Image->empty && !Region->empty) {
+      // the first region (image_processing_api.cpp:247-248), clipped to the
+      // image and intersected with the current domain (image.cpp:267-320)
+      *o = *Image;
+      auto d = Region->regions[0]->clipped(0, 0, int64_t(Image->height) - 1, int64_t(Image->width) - 1);
+      if (Image->domain) d = Image->domain->intersect(d);
+      o->domain = std::make_shared<const mvamd::Region>(std::move(d));
+    }
+    *ImageReduced = o.release();
+  });
+}
+
+extern "C" MvStatus mv_full_domain(MvObject Image, MvObject* ImageFull) {
+  return translate([&] {
+    constexpr auto op = "full_domain";
+    required(Image, op, "Image");
+    required(ImageFull, op, "ImageFull");
+    require_kind(Image, MvObjectImpl::kImage, op);
+    auto o = std::make_unique<MvObjectImpl>(*Image);
+    o->domain.reset();
+    *ImageFull = o.release();
+  });
+}
+
+extern "C" MvStatus mv_get_domain(MvObject Image, MvObject* Domain) {
+  return translate([&] {
+    constexpr auto op = "get_domain";
+    required(Image, op, "Image");
+    required(Domain, op, "Domain");
+    require_kind(Image, MvObjectImpl::kImage, op);
+    std::vector<mvamd::Region> out;
+    if (!Image->empty)
+      out.push_back(Image->domain ? *Image->domain
+                                  : mvamd::Region::rectangle(0, 0, int64_t(Image->height) - 1, int64_t(Image->width) - 1));
+    *Domain = region_object(std::move(out)).release();
   });
 }
 
@@ -618,7 +785,8 @@ extern "C" MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, Mv
     if (name == "ignore_global_polarity") p.absolute = true;
     else if (name != "use_polarity") metric.bad(mvamd::kErrUnsupportedValue, 0, "unsupported NCC metric");
     std::shared_ptr<mvamd::NccModel> m = algorithm(op, [&] {
-      return mvamd::create_model(src.pixels, src.pixel_bytes, src.width, src.height, src.width * src.pixel_bytes, p);
+      return mvamd::create_model(src.pixels, src.pixel_bytes, src.width, src.height, src.width * src.pixel_bytes, p,
+                                 src.domain.get());
     });
     put_handle(out[0], std::move(m));
     out.commit();
@@ -647,12 +815,14 @@ extern "C" MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple A
     constexpr auto op = "find_ncc_model";
     Outputs<4> out(op, {Row, Column, Angle, Score});
     required(Image, op, "Image");
+    require_kind(Image, MvObjectImpl::kImage, op);
     const auto m = model_of(ModelID, op);
     const mvamd::FindOptions q = controls(op, AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel, NumLevels);
     if (Image->empty) { out.commit(); return; }
     const size_t pitch = Image->width * size_t(Image->pixel_bytes);
     const auto found = algorithm(op, [&] {
-      return context().find_host(*m, Image->pixels, Image->width, Image->height, pitch, q, nullptr, Image->pixel_bytes);
+      return context().find_host(*m, Image->pixels, Image->width, Image->height, pitch, q, nullptr, Image->pixel_bytes,
+                                 Image->domain.get());
     });
     // HALCON pixel-edge convention of the translation (ncc_matching_api.cpp:171-176)
     for (const auto& p : found) {
@@ -674,6 +844,7 @@ extern "C" MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple
     constexpr auto op = "find_ncc_models";
     Outputs<5> out(op, {Row, Column, Angle, Score, Model});
     required(Image, op, "Image");
+    require_kind(Image, MvObjectImpl::kImage, op);
     const auto ms = models_of(ModelIDs, op);
     const Controls c = controls_n(op, ms.size(), AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel,
                                   NumLevels);
@@ -683,7 +854,8 @@ extern "C" MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple
     const size_t pitch = Image->width * size_t(Image->pixel_bytes);
     const auto found = algorithm(op, [&] {
       return context().find_models_host(ptrs.data(), c.options.data(), ptrs.size(), Image->pixels, Image->width,
-                                        Image->height, pitch, c.selection, true, Image->pixel_bytes);
+                                        Image->height, pitch, c.selection, true, Image->pixel_bytes,
+                                        Image->domain.get());
     });
     for (const auto& m : found) {
       const double cs = std::cos(m.pose.angle), sn = std::sin(m.pose.angle);
@@ -850,6 +1022,30 @@ extern "C" MvStatus mvamd_find_ncc_model_device(MvTuple ModelID, const uint8_t* 
     const auto found = algorithm(op, [&] {
       return std::move(context().find_batch(*m, &in, 1, q, stats ? &trace : nullptr).front());
     });
+    if (results) put_poses(found, results, cap);
+    *count = found.size();
+    fill_stats(trace, stats);
+  });
+}
+
+extern "C" MvStatus mvamd_find_ncc_model_image(MvObject Image, MvTuple ModelID, const MvAmdFindOptions* options,
+                                               double* results, size_t cap, size_t* count, MvAmdFindStats* stats) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_model";
+    required(Image, op, "Image");
+    require_kind(Image, MvObjectImpl::kImage, op);
+    required(count, op, "count");
+    const auto m = model_of(ModelID, op);
+    const mvamd::FindOptions q = from_ext(options);
+    mvamd::FindTrace trace;
+    std::vector<mvamd::Pose> found;
+    if (!Image->empty) {
+      const size_t pitch = Image->width * size_t(Image->pixel_bytes);
+      found = algorithm(op, [&] {
+        return context().find_host(*m, Image->pixels, Image->width, Image->height, pitch, q, stats ? &trace : nullptr,
+                                   Image->pixel_bytes, Image->domain.get());
+      });
+    }
     if (results) put_poses(found, results, cap);
     *count = found.size();
     fill_stats(trace, stats);

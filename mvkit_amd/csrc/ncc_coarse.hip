@@ -330,9 +330,57 @@ __global__ void k_storage0(NccFrame* __restrict__ frames, int n_frames, const in
   for (int i = 0; i < 4; ++i) F.storage0[i] = s[i];
 }
 
+// Reduced search domain on the coarse planes (matcher.cpp:535-579): an anchor
+// whose pose may not reach the domain (may_contain at the top-level spacing)
+// scores missing, as the reference leaves it.  Runs between the coarse
+// correlation and the peaks, for frames with a reduced domain only.
+struct DomainPlaneArgs {
+  const NccFrame* frames;
+  int level, n_angles;
+  Pools pools;
+  const int32_t* patch_ids;
+  double scale, phase;
+  int hwa;  // planes in [row][col][angle] order (else [angle][row][col])
+};
+
+constexpr int kDomainPlaneRows = 8;
+
+__global__ void __launch_bounds__(256) k_domain_planes(const DomainPlaneArgs a) {
+  const int f = blockIdx.z / a.n_angles, ai = blockIdx.z % a.n_angles;
+  const NccFrame& F = a.frames[f];
+  if (!F.dom_sat) return;
+  const int H = F.levels[a.level].height, W = F.levels[a.level].width;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= W) return;
+  const NccPatch& P = a.pools.patch(a.patch_ids[ai]);
+  const int4 rg = reinterpret_cast<const int4*>(F.ranges)[ai];
+  const int r_end = min(H, int(blockIdx.y + 1) * kDomainPlaneRows);
+  for (int r = blockIdx.y * kDomainPlaneRows; r < r_end; ++r) {
+    if (r < rg.x || r >= rg.y || c < rg.z || c >= rg.w) continue;  // scored missing already
+    const double row = F.crop_r + a.scale * (r + P.orow) + a.phase;
+    const double col = F.crop_c + a.scale * (c + P.ocol) + a.phase;
+    if (may_contain(F, row, col, a.scale)) continue;
+    const size_t idx = a.hwa ? (size_t(r) * W + c) * a.n_angles + ai : (size_t(ai) * H + r) * W + c;
+    F.planes[idx] = NCC_MISSING;
+  }
+}
+
 }  // namespace ncc
 
 using namespace ncc;
+
+extern "C" int ncchip_domain_planes(const NccFrame* frames, int n_frames, int level, int max_w, int max_h,
+                                    const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, double scale,
+                                    double phase, int hwa, void* stream) {
+  if (n_frames <= 0 || n_angles <= 0) return 0;
+  if (size_t(n_frames) * n_angles > 65535) return -1;
+  DomainPlaneArgs a;
+  a.frames = frames; a.level = level; a.n_angles = n_angles; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
+  a.patch_ids = patch_ids; a.scale = scale; a.phase = phase; a.hwa = hwa;
+  const dim3 grid((max_w + 255) / 256, (max_h + kDomainPlaneRows - 1) / kDomainPlaneRows, unsigned(n_frames * n_angles));
+  hipLaunchKernelGGL(k_domain_planes, grid, dim3(256), 0, static_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
 
 extern "C" int ncchip_coarse(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
                              double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles,

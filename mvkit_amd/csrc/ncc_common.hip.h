@@ -105,6 +105,44 @@ __device__ __forceinline__ uint32_t frame_dc(const NccFrame& F) {
   return F.dc_wide ? uint32_t(*static_cast<const uint16_t*>(F.dc_src)) : uint32_t(*static_cast<const uint8_t*>(F.dc_src));
 }
 
+// Domain pixels of a frame in image rows r0..r1 x columns c0..c1 (inclusive),
+// from its summed-area table over the crop.
+__device__ __forceinline__ uint32_t dom_count(const NccFrame& F, int64_t r0, int64_t c0, int64_t r1, int64_t c1) {
+  const int64_t h = F.levels[0].height, w = F.levels[0].width;
+  const int64_t cr = int64_t(F.crop_r), cc = int64_t(F.crop_c);
+  r0 = max(r0 - cr, int64_t(0)); r1 = min(r1 - cr, h - 1);
+  c0 = max(c0 - cc, int64_t(0)); c1 = min(c1 - cc, w - 1);
+  if (r0 > r1 || c0 > c1) return 0;
+  const uint32_t* S = F.dom_sat;
+  const size_t p = size_t(w) + 1;
+  return S[size_t(r1 + 1) * p + size_t(c1 + 1)] - S[size_t(r0) * p + size_t(c1 + 1)] -
+         S[size_t(r1 + 1) * p + size_t(c0)] + S[size_t(r0) * p + size_t(c0)];
+}
+
+// domain_contains (matcher.cpp:249-251): the pose's nearest pixel is a domain
+// pixel.
+__device__ __forceinline__ bool in_domain(const NccFrame& F, double row, double col) {
+  const double r = floor(row + .5), c = floor(col + .5);
+  if (!F.dom_sat) return r >= 0 && c >= 0 && r < double(F.image_h) && c < double(F.image_w);
+  if (r < -1e9 || r > 1e9 || c < -1e9 || c > 1e9) return false;
+  return dom_count(F, int64_t(r), int64_t(c), int64_t(r), int64_t(c)) != 0;
+}
+
+// may_contain (matcher.cpp:506-534) for a reduced domain: at spacing 1 the
+// pose's pixel; otherwise the pose within 2*spacing of the bounding box and,
+// for a sparse domain, a domain pixel in the rows and columns within
+// 2*spacing of the pose (a run overlapping [col - m, col + m] holds such a
+// pixel because the run ends are integers and 2m >= 1).
+__device__ __forceinline__ bool may_contain(const NccFrame& F, double row, double col, double spacing) {
+  if (spacing == 1) return in_domain(F, row, col);
+  const double m = 2 * spacing;
+  if (row < F.dom_box[0] - m || row > F.dom_box[2] + m || col < F.dom_box[1] - m || col > F.dom_box[3] + m)
+    return false;
+  if (!F.dom_sparse) return true;
+  return dom_count(F, int64_t(ceil(row - m)), int64_t(ceil(col - m)), int64_t(floor(row + m)),
+                   int64_t(floor(col + m))) != 0;
+}
+
 // Centred integer moments t1 = sum(S - D), t2 = sum((S - D)^2) from the
 // plain sums of S and S^2.  Computed modulo 2^64, so they are exact whenever
 // t2 < 2^63, also when sum(S^2) itself wraps (u16 images at high levels).

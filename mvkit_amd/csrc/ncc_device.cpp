@@ -351,6 +351,10 @@ struct Job {
   std::vector<Lv> lv;
   std::vector<int32_t> ranges;
   size_t off_p1 = 0, off_p2 = 0, off_nz = 0, off_planes = 0, off_ranges = 0;
+  int64_t box[4] = {0, 0, -1, -1};  // search box {row_min, col_min, row_max, col_max} (matcher.cpp:376-383)
+  // reduced domain: summed-area table, runs (int4) and per-crop-row run index
+  size_t off_sat = 0, off_runs = 0, off_row = 0;
+  std::vector<int32_t> dom_runs, dom_row;
   std::vector<Pose> cand, full, refined, results;
   std::vector<int> verdict;
   int32_t storage0[4] = {0, 0, 0, 0};
@@ -450,13 +454,15 @@ void FindContext::time_collect(FindTrace* trace) {
 }
 
 std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
-                                         size_t pitch, const FindOptions& options, FindTrace* trace, int pixel_bytes) {
+                                         size_t pitch, const FindOptions& options, FindTrace* trace, int pixel_bytes,
+                                         const Region* domain) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
   const size_t row = width * size_t(pixel_bytes), bytes = row * height;
   if (image_.bytes < bytes) image_ = DeviceBuffer(align_up(bytes, 1 << 20));
   check(hipMemcpy2DAsync(image_.ptr, row, image, pitch, row, height, hipMemcpyHostToDevice, stream_), "image upload");
   FrameInput in{image_.as<uint8_t>(), width, height, row};
   in.pixel_bytes = pixel_bytes;
+  in.domain = domain;
   auto out = find_batch(model, &in, 1, options, trace);
   return std::move(out.front());
 }
@@ -464,7 +470,8 @@ std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* i
 std::vector<ModelMatch> FindContext::find_models_host(const NccModel* const* models, const FindOptions* options,
                                                       size_t n_models, const uint8_t* image, size_t width,
                                                       size_t height, size_t pitch, const Selection& selection,
-                                                      bool per_model_quota_before_overlap, int pixel_bytes) {
+                                                      bool per_model_quota_before_overlap, int pixel_bytes,
+                                                      const Region* domain) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
   const size_t row = width * size_t(pixel_bytes), bytes = row * height;
   check(hipStreamSynchronize(stream2_), "sync");
@@ -475,6 +482,7 @@ std::vector<ModelMatch> FindContext::find_models_host(const NccModel* const* mod
   check(hipMemcpy2DAsync(image_.ptr, row, image, pitch, row, height, hipMemcpyHostToDevice, stream_), "image upload");
   FrameInput in{image_.as<uint8_t>(), width, height, row};
   in.pixel_bytes = pixel_bytes;
+  in.domain = domain;
   return find_models(models, options, n_models, in, selection, per_model_quota_before_overlap);
 }
 
@@ -564,6 +572,12 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     if (W > size_t(std::numeric_limits<int>::max()) || H > size_t(std::numeric_limits<int>::max()) || H > 32000000 / W)
       raise(kLimitExceeded, "image", "search image exceeds pixel budget");
   }
+  // reduced domains: one frame per find; an empty domain finds nothing (matcher.cpp:381)
+  const bool reduced = inputs[0].domain != nullptr;
+  for (size_t f = 0; f < n_frames; ++f)
+    if ((inputs[f].domain != nullptr) != reduced || (reduced && n_frames != 1))
+      raise(kInvalidArgument, "image", "reduced search domains are supported for single-frame finds");
+  if (reduced && inputs[0].domain->empty()) return finish_empty();
   auto dmodel = device_model(model, stream_);
 
   // ---- query patches per level (matcher.cpp:463-482), shared by the batch
@@ -654,10 +668,15 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     J.in = inputs[f];
     J.trace = traces ? &traces[f] : nullptr;
     const int64_t W = int64_t(J.in.width), H = int64_t(J.in.height);
-    J.crop_r = std::max<int64_t>(0, 0 - halo) / lattice * lattice;
-    J.crop_c = std::max<int64_t>(0, 0 - halo) / lattice * lattice;
-    J.crop_b = std::min<int64_t>(H, ((H - 1 + halo) / lattice + 1) * lattice);
-    J.crop_e = std::min<int64_t>(W, ((W - 1 + halo) / lattice + 1) * lattice);
+    if (const Region* D = J.in.domain) {
+      J.box[0] = D->row_min; J.box[1] = D->col_min; J.box[2] = D->row_max; J.box[3] = D->col_max;
+    } else {
+      J.box[0] = 0; J.box[1] = 0; J.box[2] = H - 1; J.box[3] = W - 1;
+    }
+    J.crop_r = std::max<int64_t>(0, J.box[0] - halo) / lattice * lattice;
+    J.crop_c = std::max<int64_t>(0, J.box[1] - halo) / lattice * lattice;
+    J.crop_b = std::min<int64_t>(H, ((J.box[2] + halo) / lattice + 1) * lattice);
+    J.crop_e = std::min<int64_t>(W, ((J.box[3] + halo) / lattice + 1) * lattice);
     J.lv.resize(level_count);
     J.lv[0] = {int(J.crop_e - J.crop_c), int(J.crop_b - J.crop_r), wide ? NCC_U16 : NCC_U8,
                int(J.in.pitch / size_t(J.in.pixel_bytes)), 0, 0};
@@ -674,11 +693,24 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       const Raster& p = raster_of(refs[top][i]);
       const double margin = scale == 1 ? .5 : 2 * scale;
       int32_t* rg = &J.ranges[4 * i];
-      rg[0] = std::max({0, -p.rmin, int(std::ceil((0 - margin - J.crop_r - phase) / scale - p.orow))});
-      rg[1] = std::min(J.lv[top].h - p.rmax, int(std::floor((H - 1 + margin - J.crop_r - phase) / scale - p.orow)) + 1);
-      rg[2] = std::max({0, -p.cmin, int(std::ceil((0 - margin - J.crop_c - phase) / scale - p.ocol))});
-      rg[3] = std::min(J.lv[top].w - p.cmax, int(std::floor((W - 1 + margin - J.crop_c - phase) / scale - p.ocol)) + 1);
+      rg[0] = std::max({0, -p.rmin, int(std::ceil((J.box[0] - margin - J.crop_r - phase) / scale - p.orow))});
+      rg[1] = std::min(J.lv[top].h - p.rmax, int(std::floor((J.box[2] + margin - J.crop_r - phase) / scale - p.orow)) + 1);
+      rg[2] = std::max({0, -p.cmin, int(std::ceil((J.box[1] - margin - J.crop_c - phase) / scale - p.ocol))});
+      rg[3] = std::min(J.lv[top].w - p.cmax, int(std::floor((J.box[3] + margin - J.crop_c - phase) / scale - p.ocol)) + 1);
       J.coarse_flops += 2.0 * double(p.count) * std::max(0, rg[1] - rg[0]) * std::max(0, rg[3] - rg[2]);
+    }
+    if (const Region* D = J.in.domain) {
+      // the domain's runs (all inside the crop: the crop covers the box) and
+      // the index of each crop row's first run
+      J.dom_row.assign(size_t(J.lv[0].h) + 1, 0);
+      size_t k = 0;
+      for (int r = 0; r <= J.lv[0].h; ++r) {
+        while (k < D->runs.size() && D->runs[k].row < J.crop_r + r) ++k;
+        J.dom_row[size_t(r)] = int32_t(k);
+      }
+      for (const RegionRun& run : D->runs) {
+        J.dom_runs.push_back(run.row); J.dom_runs.push_back(run.cb); J.dom_runs.push_back(run.ce); J.dom_runs.push_back(0);
+      }
     }
   }
 
@@ -706,6 +738,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     J.off_p2 = off; off = align_up(off + size_t(Wt + 1) * Ht * 8);
     J.off_nz = off; off = align_up(off + size_t(Wt + 1) * Ht * 4);
     J.off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
+    if (J.in.domain) { J.off_sat = off; off = align_up(off + (size_t(J.lv[0].h) + 1) * (size_t(J.lv[0].w) + 1) * 4); }
   }
   const size_t off_lists = off; off = align_up(off + (n_track + 1) * cap_all * sizeof(NccMatch));
   const size_t off_raw = off; off = align_up(off + cap_all * sizeof(NccMatch));
@@ -721,6 +754,11 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   size_t small = 0;
   for (size_t l = 0; l < level_count; ++l) { off_refs[l] = off_small + small; small = align_up(small + refs[l].size() * 4 + 4); }
   for (auto& J : jobs) { J.off_ranges = off_small + small; small = align_up(small + J.ranges.size() * 4); }
+  for (auto& J : jobs) {
+    if (!J.in.domain) continue;
+    J.off_runs = off_small + small; small = align_up(small + J.dom_runs.size() * 4);
+    J.off_row = off_small + small; small = align_up(small + J.dom_row.size() * 4);
+  }
   const size_t off_qpool = off_small + small; small = align_up(small + QL.total);
   const size_t off_extent = off_small + small; small = align_up(small + n_frames * 32);
   const size_t off_wh = off_small + small; small = align_up(small + dense.wh.size() * 2);
@@ -754,6 +792,11 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     for (size_t l = 0; l < level_count; ++l)
       std::memcpy(st + (off_refs[l] - off_small), refs[l].data(), refs[l].size() * 4);
     for (auto& J : jobs) std::memcpy(st + (J.off_ranges - off_small), J.ranges.data(), J.ranges.size() * 4);
+    for (auto& J : jobs) {
+      if (!J.in.domain) continue;
+      std::memcpy(st + (J.off_runs - off_small), J.dom_runs.data(), J.dom_runs.size() * 4);
+      std::memcpy(st + (J.off_row - off_small), J.dom_row.data(), J.dom_row.size() * 4);
+    }
     serialize(qpool, QL, st + (off_qpool - off_small));
     if (use_mfma) {
       std::memcpy(st + (off_wh - off_small), dense.wh.data(), dense.wh.size() * 2);
@@ -786,6 +829,13 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       d.nz = reinterpret_cast<uint32_t*>(dev + J.off_nz);
       d.planes = reinterpret_cast<double*>(dev + J.off_planes);
       d.ranges = reinterpret_cast<const int32_t*>(dev + J.off_ranges);
+      if (const Region* D = J.in.domain) {
+        d.dom_sat = reinterpret_cast<uint32_t*>(dev + J.off_sat);
+        d.dom_runs = reinterpret_cast<const int32_t*>(dev + J.off_runs);
+        d.dom_row = reinterpret_cast<const int32_t*>(dev + J.off_row);
+        for (int i = 0; i < 4; ++i) d.dom_box[i] = int32_t(J.box[i]);
+        d.dom_sparse = double(D->area) < .7 * double(D->row_max - D->row_min + 1) * double(D->col_max - D->col_min + 1);
+      }
       fr[f] = d;
     }
     check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, stream_), "upload");
@@ -803,6 +853,8 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       ncchip_fill_u32(reinterpret_cast<uint32_t*>(dev + off_table), std::max<size_t>(n_track, 1) * n_frames * table_size,
                       0xffffffffu, stream_))
     raise(kInternalError, "device", "fill launch failed");
+  if (reduced && ncchip_domain_sat(dframes, F, max_dims(0).second, max_dims(0).first, stream_))
+    raise(kInternalError, "device", "domain launch failed");
   time_begin(kStagePyramid, stream_);
   for (size_t l = 1; l < level_count; ++l) {
     const auto [w, h] = max_dims(l);
@@ -841,6 +893,10 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   if (rc) raise(kInternalError, "device", "coarse launch failed");
   time_end(stream_);
   time_begin(kStagePeaks, stream_);
+  // reduced domain: anchors whose pose may not reach it score missing (matcher.cpp:556-578)
+  if (reduced && ncchip_domain_planes(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), scale, phase,
+                                      use_mfma ? 1 : 0, stream_))
+    raise(kInternalError, "device", "domain launch failed");
   if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
                    lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 1 : 0,
                    stream_))
@@ -930,6 +986,12 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   rp.guard = -1;
   auto* hfull = reinterpret_cast<NccMatch*>(rb + rb_full);
   auto* hcounts = reinterpret_cast<int32_t*>(rb + rb_counts);
+  // reduced domain: the level-1 hand-over to the fit snaps the candidates to
+  // the domain (matcher.cpp:631-645); duplicates it makes are merged after the
+  // full score, which only depends on the pose
+  if (reduced && o.subpixel && finest == 0 && top >= 1 &&
+      ncchip_domain_snap(dframes, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), stream_))
+    raise(kInternalError, "device", "domain launch failed");
   time_begin(kStageFull, stream_);
   if (ncchip_full_score(&rp, pools, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), hfull, 1024,
                         stream_))
@@ -1016,6 +1078,13 @@ void FindContext::collect(Slot& S) {
       J.full.push_back({m.row, m.col, m.angle, m.score});
     }
     std::sort(J.full.begin(), J.full.end(), score_order);
+    if (J.in.domain) {
+      // merge_identical after the snap (matcher.cpp:644): equal poses have equal
+      // full scores, so they are adjacent here
+      J.full.erase(std::unique(J.full.begin(), J.full.end(), [](const Pose& a, const Pose& b) {
+        return a.row == b.row && a.col == b.col && a.angle == b.angle;
+      }), J.full.end());
+    }
     J.refined.assign(J.full.size(), Pose{});
     J.verdict.assign(J.full.size(), -1);  // -1: not refined
     J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
@@ -1284,7 +1353,7 @@ std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, 
       if (!S.finest && distinct && sel.num_matches == 0 && S.jobs[0].full.size() >= 32 && o.subpixel &&
           !o.num_matches && o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
           m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several &&
-          frame.pixel_bytes == 1) {
+          frame.pixel_bytes == 1 && !frame.domain) {
         S.rp.guard_first = o.min_score >= .8 ? .1 : .2;
         S.rp.guard = .1;
       }

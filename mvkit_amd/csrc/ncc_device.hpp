@@ -10,6 +10,7 @@
 
 #include "../../include/ncchip.h"
 #include "ncc_model.hpp"
+#include "ncc_region.hpp"
 
 namespace mvamd {
 
@@ -105,11 +106,14 @@ struct FindTrace {
   double track_mfma_flops = 0;         // the part of track_flops tracked on matrix cores
 };
 
-// A u8 or u16 search image in device memory (`pitch` bytes per row).
+// A u8 or u16 search image in device memory (`pitch` bytes per row), with
+// its domain: null for the full image, else a region inside the image (the
+// reference's reduced domain, image.hpp:108-113; it may be empty).
 struct FrameInput {
   const uint8_t* image;
   size_t width, height, pitch;
   int pixel_bytes = 1;  // 1: u8 ("byte"), 2: u16 ("uint2")
+  const Region* domain = nullptr;
 };
 
 // One find pipeline: owns a stream and grow-only device/pinned workspaces.
@@ -144,7 +148,7 @@ class FindContext {
   // Same as find, from host memory (copied to the device first).
   std::vector<Pose> find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
                               size_t pitch, const FindOptions& options, FindTrace* trace = nullptr,
-                              int pixel_bytes = 1);
+                              int pixel_bytes = 1, const Region* domain = nullptr);
   // find_models (matcher.cpp:1077-1124) over one u8 frame in device memory:
   // one query per distinct (model, options), the candidates of all queries
   // selected together in (score, model, row, col, angle) order with the
@@ -157,7 +161,8 @@ class FindContext {
   std::vector<ModelMatch> find_models_host(const NccModel* const* models, const FindOptions* options,
                                            size_t n_models, const uint8_t* image, size_t width, size_t height,
                                            size_t pitch, const Selection& selection,
-                                           bool per_model_quota_before_overlap = true, int pixel_bytes = 1);
+                                           bool per_model_quota_before_overlap = true, int pixel_bytes = 1,
+                                           const Region* domain = nullptr);
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }

@@ -147,9 +147,82 @@ __global__ void __launch_bounds__(256) k_support_cols(const uint32_t* __restrict
   }
 }
 
+// Row pass of a frame's domain summed-area table: entry (r + 1, c + 1) = the
+// domain pixels of crop row r in columns 0..c (one block per crop row,
+// blockIdx.y = frame; frames without a reduced domain return at once).
+__global__ void __launch_bounds__(256) k_domain_rows(const NccFrame* __restrict__ frames) {
+  const NccFrame& F = frames[blockIdx.y];
+  if (!F.dom_sat) return;
+  const int h = F.levels[0].height, w = F.levels[0].width;
+  const int r = blockIdx.x;
+  if (r >= h) return;
+  __shared__ uint32_t part[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row = int64_t(F.crop_r) + r, c0 = int64_t(F.crop_c);
+  const int first = F.dom_row[r], last = F.dom_row[r + 1];
+  uint32_t* out = F.dom_sat + size_t(r + 1) * (size_t(w) + 1);
+  if (threadIdx.x == 0) out[0] = 0;
+  uint32_t carry = 0;
+  for (int base = 0; base < w; base += 256) {
+    const int c = base + threadIdx.x;
+    uint32_t v = 0;
+    if (c < w && first < last) {
+      // the row's last run starting at or before the column (runs of a row
+      // are sorted by first column)
+      const int4* runs = reinterpret_cast<const int4*>(F.dom_runs);
+      const int64_t x = c0 + c;
+      int lo = first, hi = last - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (runs[mid].y <= x) lo = mid; else hi = mid - 1;
+      }
+      const int4 run = runs[lo];
+      v = (run.x == row && run.y <= x && x <= run.z) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(v, d, 64);
+      if (lane >= d) v += t;
+    }
+    if (lane == 63) part[wave] = v;
+    __syncthreads();
+    uint32_t add = carry;
+    for (int k = 0; k < wave; ++k) add += part[k];
+    if (c < w) out[c + 1] = v + add;
+    carry += part[0] + part[1] + part[2] + part[3];
+    __syncthreads();
+  }
+}
+
+// Column pass: accumulates the row prefixes down each column in place and
+// writes the zero first row (one thread per column, blockIdx.y = frame).
+__global__ void __launch_bounds__(256) k_domain_cols(const NccFrame* __restrict__ frames) {
+  const NccFrame& F = frames[blockIdx.y];
+  if (!F.dom_sat) return;
+  const int h = F.levels[0].height, w = F.levels[0].width;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c > w) return;
+  const size_t p = size_t(w) + 1;
+  uint32_t* S = F.dom_sat;
+  S[c] = 0;
+  uint32_t acc = 0;
+  for (int r = 1; r <= h; ++r) {
+    acc += S[size_t(r) * p + c];
+    S[size_t(r) * p + c] = acc;
+  }
+}
+
 }  // namespace ncc
 
 using namespace ncc;
+
+extern "C" int ncchip_domain_sat(const NccFrame* frames, int n_frames, int max_h, int max_w, void* stream) {
+  if (n_frames <= 0 || n_frames > 65535 || max_h <= 0 || max_w <= 0) return n_frames ? -1 : 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_domain_rows, dim3(max_h, n_frames), dim3(256), 0, st, frames);
+  hipLaunchKernelGGL(k_domain_cols, dim3((max_w + 1 + 255) / 256, n_frames), dim3(256), 0, st, frames);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
 
 static inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 

@@ -197,45 +197,54 @@ LevelGeom halve(const LevelGeom& pv) {
 NccModel::~NccModel() = default;
 
 std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size_t width, size_t height,
-                                       size_t stride_bytes, const CreateParams& p) {
+                                       size_t stride_bytes, const CreateParams& p, const Region* domain) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
   if (!std::isfinite(p.start) || !std::isfinite(p.extent) || !std::isfinite(p.step) || p.extent < 0 || p.extent > kTau ||
       !std::isfinite(p.start + p.extent) || p.step < 0 || p.levels < 0 || p.levels > 16)
     raise(kInvalidArgument, "options", "invalid levels or angle range");
-  if (width * height < 4) raise(kInvalidArgument, "domain", "template domain requires at least four pixels");
-  if (height > kMaxPixels / width)
-    raise(kLimitExceeded, "domain", "template bounding box exceeds one million pixels");
+  // the domain's runs and bounding box (the whole image without a reduced domain)
+  const Region full = domain ? Region{} : Region::rectangle(0, 0, int64_t(height) - 1, int64_t(width) - 1);
+  const Region& D = domain ? *domain : full;
+  if (D.empty() || D.area < 4) raise(kInvalidArgument, "domain", "template domain requires at least four pixels");
+  if (D.row_min < 0 || D.col_min < 0 || D.row_max >= int64_t(height) || D.col_max >= int64_t(width))
+    raise(kInvalidArgument, "domain", "template domain must be inside the image");
+  const size_t bw = size_t(D.col_max - D.col_min + 1), bh = size_t(D.row_max - D.row_min + 1);
+  if (bh > kMaxPixels / bw) raise(kLimitExceeded, "domain", "template bounding box exceeds one million pixels");
 
   auto m = std::make_unique<NccModel>();
   LevelGeom first;
-  first.w = width; first.h = height;
-  first.gray.resize(width * height);
-  first.mask.assign(width * height, 1);
-  // centroid of the (rectangular) domain, accumulated per row run in long
-  // double as model.cpp:142-146, 158-161
+  first.w = bw; first.h = bh;
+  first.gray.assign(bw * bh, 0.0);
+  first.mask.assign(bw * bh, 0);
+  // centroid of the domain, accumulated per row run in long double
+  // (model.cpp:138-161); integer samples summed exactly in int64
   long double sum_row = 0, sum_col = 0;
   int64_t total = 0;
   const auto* base = static_cast<const uint8_t*>(pixels);
-  for (size_t r = 0; r < height; ++r) {
-    const int64_t count = int64_t(width);
-    sum_row += static_cast<long double>(r) * count;
-    sum_col += (static_cast<long double>(0) + int64_t(width - 1)) * count / 2;
-    const uint8_t* row = base + r * stride_bytes;
-    for (size_t c = 0; c < width; ++c) {
+  for (const RegionRun& run : D.runs) {
+    const int64_t count = int64_t(run.ce) - run.cb + 1;
+    sum_row += static_cast<long double>(run.row) * count;
+    sum_col += (static_cast<long double>(run.cb) + run.ce) * count / 2;
+    const uint8_t* row = base + size_t(run.row) * stride_bytes;
+    const size_t off = size_t(run.row - D.row_min) * bw;
+    for (int64_t c = run.cb; c <= run.ce; ++c) {
       uint32_t v;
       if (pixel_bytes == 1) v = row[c];
       else { uint16_t t; std::memcpy(&t, row + 2 * c, 2); v = t; }
-      first.gray[r * width + c] = double(v);
+      const size_t i = off + size_t(c - D.col_min);
+      first.gray[i] = double(v);
+      first.mask[i] = 1;
       total += v;
     }
   }
-  const long double area = static_cast<long double>(width * height);
+  const long double area = static_cast<long double>(D.area);
   m->orow = double(sum_row / area);
   m->ocol = double(sum_col / area);
-  first.orow = double((sum_row - static_cast<long double>(0) * area) / area);
-  first.ocol = double((sum_col - static_cast<long double>(0) * area) / area);
-  const double mean = double(total) / double(width * height);
-  for (auto& g : first.gray) g -= mean;
+  first.orow = double((sum_row - static_cast<long double>(D.row_min) * area) / area);
+  first.ocol = double((sum_col - static_cast<long double>(D.col_min) * area) / area);
+  const double mean = double(total) / double(D.area);
+  for (size_t i = 0; i < first.gray.size(); ++i)
+    if (first.mask[i]) first.gray[i] -= mean;
   return rebuild_model(std::move(first), p, m->orow, m->ocol);
 }
 

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "device_types.h"
+#include "ncc_region.hpp"
 
 namespace mvamd {
 
@@ -101,10 +102,12 @@ struct NccModel {
   ~NccModel();
 };
 
-// create(): model.cpp:120-171 + rebuild (model.cpp:173-242) for a full-domain
-// single-channel template of u8 or u16 samples.
+// create(): model.cpp:120-171 + rebuild (model.cpp:173-242) for a
+// single-channel template of u8 or u16 samples over its domain (null: the
+// whole image; else a region inside the image).
 std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size_t width, size_t height,
-                                       size_t stride_bytes, const CreateParams& params);
+                                       size_t stride_bytes, const CreateParams& params,
+                                       const Region* domain = nullptr);
 
 // rebuild(): model.cpp:173-242 from a centred level 0 with any mask and the
 // template centroid in image coordinates (persisted models).

@@ -64,6 +64,7 @@ struct TrackArgs {
   double dc, crop_r, crop_c;
   int64_t D;
   int sr, sc, sh, sw;
+  const NccFrame* dom;        // the frame when it has a reduced domain, else null
 };
 
 // Loads frame f into the per-block copy of the arguments.
@@ -76,6 +77,7 @@ __device__ __forceinline__ void bind_frame(TrackArgs& a, int f) {
   a.crop_c = F.crop_c;
   if (a.level == 0) { a.sr = F.storage0[0]; a.sc = F.storage0[1]; a.sh = F.storage0[2]; a.sw = F.storage0[3]; }
   else { a.sr = 0; a.sc = 0; a.sh = a.lv.height; a.sw = a.lv.width; }
+  a.dom = F.dom_sat ? &F : nullptr;
 }
 
 // Pixel source of a tracking window: an LDS tile of the candidate's
@@ -114,6 +116,16 @@ __device__ __forceinline__ double window_score(const TrackArgs& a, const NccPatc
 __device__ __forceinline__ bool window_inside(const TrackArgs& a, const NccPatch& P, int row, int col) {
   return row + P.rmin >= 0 && col + P.cmin >= 0 && row + P.rmax < a.lv.height && col + P.cmax < a.lv.width &&
          row + P.rmin >= a.sr && col + P.cmin >= a.sc && row + P.rmax < a.sr + a.sh && col + P.cmax < a.sc + a.sw;
+}
+
+// A tracking window that is scored: inside the level's valid region and, for
+// a reduced domain, at a pose that may reach the domain (matcher.cpp:673-751,
+// may_contain at this level's spacing).
+__device__ __forceinline__ bool window_ok(const TrackArgs& a, const NccPatch& P, int row, int col) {
+  if (!window_inside(a, P, row, col)) return false;
+  if (!a.dom) return true;
+  return may_contain(*a.dom, a.crop_r + a.step * (row + P.orow) + a.offset,
+                     a.crop_c + a.step * (col + P.ocol) + a.offset, a.step);
 }
 
 // Exchanges of doubles for the transposed butterfly, as pairs of dwords:
@@ -272,7 +284,7 @@ __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p,
       if (k == q) { d = dk; T1 = s1; T2 = s2; }
     }
   }
-  if (q < 5 && window_inside(a, P, row, col - 2 + q)) score = window_score(a, P, d * unit, T1, T2);
+  if (q < 5 && window_ok(a, P, row, col - 2 + q)) score = window_score(a, P, d * unit, T1, T2);
 }
 
 constexpr int kMaxEligible = 1024;
@@ -668,7 +680,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
         anchor(a, cd, p.P, row, col);
         row += wi / 5 - 2;
         col += wi % 5 - 2;
-        if (!window_inside(a, p.P, row, col)) continue;
+        if (!window_ok(a, p.P, row, col)) continue;
         const double sc = exact_window_wave(a, p, row, col, rd);
         if (sc > best) { best = sc; best_i = k; }
       }
@@ -859,7 +871,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
         const NccPatch& P = s_pat[e];
         const int row = s_row[e] + wi / 5 - 2, col = s_col[e] + wi % 5 - 2;
         float up = -__builtin_inff();
-        if (window_inside(a, P, row, col)) {
+        if (window_ok(a, P, row, col)) {
           const int4* runs = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs) + P.run_off;
           const int2* cruns = s_runs + s_run0[e];
           int64_t m1 = 0, m2 = 0;
@@ -987,7 +999,8 @@ struct RefineArgs {
   NccLevel lv;            // finest search level
   double dc, crop_r, crop_c;
   int sr, sc, sh, sw;     // storage rectangle at the finest level
-  int img_h, img_w;       // search domain (full image)
+  int img_h, img_w;       // full image
+  const NccFrame* dom;    // the frame when it has a reduced domain, else null
   int H0, W0;             // level-0 dims (crop)
   // LDS copy of the candidate's neighbourhood (storage-local rectangle)
   const void* tile;
@@ -1004,6 +1017,7 @@ __device__ __forceinline__ void bind_frame(RefineArgs& a, int f) {
   else { a.sr = 0; a.sc = 0; a.sh = a.lv.height; a.sw = a.lv.width; }
   a.img_h = F.image_h;
   a.img_w = F.image_w;
+  a.dom = F.dom_sat ? &F : nullptr;
   a.H0 = F.levels[0].height;
   a.W0 = F.levels[0].width;
 }
@@ -1020,6 +1034,7 @@ struct Shared {
 };
 
 __device__ __forceinline__ bool domain_contains(const RefineArgs& a, double row, double col) {
+  if (a.dom) return in_domain(*a.dom, row, col);
   const double r = floor(row + .5), c = floor(col + .5);
   return r >= 0 && c >= 0 && r < double(a.img_h) && c < double(a.img_w);
 }
@@ -1550,6 +1565,41 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
   }
 }
 
+// Level-1 hand-over to the fit with a reduced domain (matcher.cpp:631-645): a
+// candidate whose pose is not on a domain pixel moves to the nearest domain
+// pixel of the runs within 4 rows (column clamped to the run and at most 4
+// from the pose; the first run in (row, column) order wins ties), else its
+// score becomes missing.  One thread per candidate of frame blockIdx.y's list.
+__global__ void __launch_bounds__(256) k_domain_snap(const NccFrame* __restrict__ frames, NccMatch* cand,
+                                                     const int32_t* __restrict__ count, int cap) {
+  const int f = blockIdx.y;
+  const NccFrame& F = frames[f];
+  if (!F.dom_sat) return;
+  const int n = min(max(count[f], 0), cap);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  NccMatch p = cand[size_t(f) * cap + i];
+  if (in_domain(F, p.row, p.col)) return;
+  const int64_t h = F.levels[0].height, cr = int64_t(F.crop_r);
+  const int64_t r0 = max(int64_t(ceil(p.row - 4)) - cr, int64_t(0));
+  const int64_t r1 = min(int64_t(floor(p.row + 4)) - cr, h - 1);
+  const int4* runs = reinterpret_cast<const int4*>(F.dom_runs);
+  double best = __builtin_inf(), nr = p.row, nc = p.col;
+  for (int64_t r = r0; r <= r1; ++r)
+    for (int k = F.dom_row[r]; k < F.dom_row[r + 1]; ++k) {
+      const int4 run = runs[k];
+      double c = round(p.col);
+      if (c < double(run.y)) c = double(run.y);
+      else if (double(run.z) < c) c = double(run.z);
+      if (fabs(c - p.col) > 4) continue;
+      const double d = hypot(double(run.x) - p.row, c - p.col);
+      if (d < best) { best = d; nr = double(run.x); nc = c; }
+    }
+  if (isfinite(best)) { p.row = nr; p.col = nc; }
+  else p.score = NCC_MISSING;
+  cand[size_t(f) * cap + i] = p;
+}
+
 }  // namespace ncc
 
 using namespace ncc;
@@ -1658,5 +1708,14 @@ extern "C" int ncchip_refine(const NcchipRefineParams* p, const NccPatchPool* po
   if (p->level_kind == NCC_U8) hipLaunchKernelGGL(k_refine<uint8_t>, g, b, kRefineTileBytes, st, a);
   else if (p->level_kind == NCC_U16) hipLaunchKernelGGL(k_refine<uint16_t>, g, b, kRefineTileBytes, st, a);
   else hipLaunchKernelGGL(k_refine<uint32_t>, g, b, kRefineTileBytes, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" int ncchip_domain_snap(const NccFrame* frames, int n_frames, NccMatch* cand, const int32_t* count, int cap,
+                                  void* stream) {
+  if (n_frames <= 0 || cap <= 0) return 0;
+  if (n_frames > 65535) return -1;
+  hipLaunchKernelGGL(k_domain_snap, dim3((cap + 255) / 256, n_frames), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     frames, cand, count, cap);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

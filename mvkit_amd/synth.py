@@ -291,3 +291,36 @@ def small_multi(seed_k: int = 20) -> MultiScene:
     degrees, N(0,4): oracle-speed find_models cases."""
     e = math.radians(15)
     return multi_scene(320, 256, [(40, 40), (32, 48)], [2, 2], -e, e, 4.0, SEED_BASE + seed_k)
+
+
+# ------------------------------------------------------------- search domains
+def mask_runs(mask: np.ndarray, row0: int = 0, col0: int = 0) -> np.ndarray:
+    """Canonical runs (n x 3: row, first column, last column) of a boolean
+    mask placed at (row0, col0)."""
+    out = []
+    for r in range(mask.shape[0]):
+        m = np.concatenate([[False], mask[r].astype(bool), [False]])
+        d = np.diff(m.astype(np.int8))
+        starts, ends = np.nonzero(d == 1)[0], np.nonzero(d == -1)[0] - 1
+        out += [(row0 + r, col0 + a, col0 + b) for a, b in zip(starts, ends)]
+    return np.array(out, dtype=np.int64).reshape(-1, 3)
+
+
+def rect_runs(r0: int, c0: int, r1: int, c1: int) -> np.ndarray:
+    return np.array([(r, c0, c1) for r in range(r0, r1 + 1)], dtype=np.int64).reshape(-1, 3)
+
+
+def disk_runs(cy: float, cx: float, radius: float) -> np.ndarray:
+    """Pixels within `radius` of (cy, cx)."""
+    r0, r1 = int(math.floor(cy - radius)), int(math.ceil(cy + radius))
+    c0, c1 = int(math.floor(cx - radius)), int(math.ceil(cx + radius))
+    yy, xx = np.mgrid[r0:r1 + 1, c0:c1 + 1]
+    return mask_runs((yy - cy) ** 2 + (xx - cx) ** 2 <= radius * radius, r0, c0)
+
+
+def random_domain(rng: SplitMix64, h: int, w: int, n: int, max_len: int = 40) -> np.ndarray:
+    """n random runs (unsorted, possibly overlapping) inside an h x w image."""
+    rows = (rng.uniform(n) * h).astype(np.int64)
+    starts = (rng.uniform(n) * w).astype(np.int64)
+    lens = (rng.uniform(n) * max_len).astype(np.int64)
+    return np.stack([rows, starts, np.minimum(starts + lens, w - 1)], 1)

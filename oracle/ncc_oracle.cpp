@@ -172,29 +172,42 @@ static bool level_usable(const Level& lv, size_t& count) {
 
 static Model rebuild(Level first, const CreateOpts& o, double orow, double ocol);
 
-// create: model.cpp:120-171 (file lines 203-254), full-domain u8 template.
-static Model create(const uint8_t* px, size_t w, size_t h, size_t stride, const CreateOpts& o) {
+// create: model.cpp:120-171 (file lines 203-254), u8 template over its
+// domain given as canonical runs (row, first column, last column) inside the
+// image; an empty list means the whole image.
+static Model create(const uint8_t* px, size_t w, size_t h, size_t stride, const CreateOpts& o,
+                    std::vector<std::array<int64_t, 3>> runs = {}) {
   if (!w || !h) fail(INVALID_ARGUMENT, "one nonempty single-channel image required");
   if (!std::isfinite(o.start) || !std::isfinite(o.extent) || !std::isfinite(o.step) || o.extent < 0 ||
       o.extent > kTau || !std::isfinite(o.start + o.extent) || o.step < 0 || o.levels < 0 || o.levels > 16)
     fail(INVALID_ARGUMENT, "invalid levels or angle range");
-  if (w * h < 4) fail(INVALID_ARGUMENT, "template domain requires at least four pixels");
-  if (h > 1048576 / w) fail(LIMIT_EXCEEDED, "template bounding box exceeds one million pixels");
-  Level first; first.w = w; first.h = h;
-  first.gray.resize(w * h); first.mask.assign(w * h, 1);
+  if (runs.empty())
+    for (size_t r = 0; r < h; ++r) runs.push_back({int64_t(r), 0, int64_t(w) - 1});
+  int64_t area_i = 0, r0 = runs.front()[0], r1 = runs.back()[0], c0 = runs.front()[1], c1 = runs.front()[2];
+  for (const auto& x : runs) { area_i += x[2] - x[1] + 1; c0 = std::min(c0, x[1]); c1 = std::max(c1, x[2]); }
+  if (area_i < 4) fail(INVALID_ARGUMENT, "template domain requires at least four pixels");
+  if (r0 < 0 || c0 < 0 || r1 >= int64_t(h) || c1 >= int64_t(w)) fail(INVALID_ARGUMENT, "template domain must be inside the image");
+  const size_t bw = size_t(c1 - c0 + 1), bh = size_t(r1 - r0 + 1);
+  if (bh > 1048576 / bw) fail(LIMIT_EXCEEDED, "template bounding box exceeds one million pixels");
+  Level first; first.w = bw; first.h = bh;
+  first.gray.assign(bw * bh, 0.0); first.mask.assign(bw * bh, 0);
   long double sum_row = 0, sum_col = 0; int64_t isum = 0;
-  for (size_t r = 0; r < h; ++r) {
-    const int64_t count = int64_t(w);
-    sum_row += static_cast<long double>(r) * count;
-    sum_col += (static_cast<long double>(0) + int64_t(w - 1)) * count / 2;
-    for (size_t c = 0; c < w; ++c) { first.gray[r * w + c] = double(px[r * stride + c]); isum += px[r * stride + c]; }
+  for (const auto& x : runs) {
+    const int64_t count = x[2] - x[1] + 1;
+    sum_row += static_cast<long double>(x[0]) * count;
+    sum_col += (static_cast<long double>(x[1]) + x[2]) * count / 2;
+    for (int64_t c = x[1]; c <= x[2]; ++c) {
+      const size_t i = size_t(x[0] - r0) * bw + size_t(c - c0);
+      first.gray[i] = double(px[size_t(x[0]) * stride + size_t(c)]); first.mask[i] = 1;
+      isum += px[size_t(x[0]) * stride + size_t(c)];
+    }
   }
-  const long double area = (long double)(w * h);
+  const long double area = (long double)area_i;
   Model m0; m0.orow = double(sum_row / area); m0.ocol = double(sum_col / area);
-  first.orow = double((sum_row - 0.0L * area) / area);
-  first.ocol = double((sum_col - 0.0L * area) / area);
-  const double mean = double(isum) / double(w * h);
-  for (auto& g : first.gray) g -= mean;
+  first.orow = double((sum_row - static_cast<long double>(r0) * area) / area);
+  first.ocol = double((sum_col - static_cast<long double>(c0) * area) / area);
+  const double mean = double(isum) / double(area_i);
+  for (size_t i = 0; i < first.gray.size(); ++i) if (first.mask[i]) first.gray[i] -= mean;
   return rebuild(std::move(first), o, m0.orow, m0.ocol);
 }
 
@@ -807,6 +820,57 @@ struct Trace {
   long crop_r = 0, crop_c = 0, store_r = 0, store_c = 0, store_h = 0, store_w = 0;
 };
 
+// ------------------------------------------------------------------ domain
+// Region: canonical run-length encoding (region.hpp:32-117, region.cpp:64-117):
+// runs sorted by (row, col_begin), disjoint and non-adjacent within a row.
+struct DRun { int64_t row, cb, ce; };
+struct Domain {
+  std::vector<DRun> runs;
+  int64_t area = 0, rmin = 0, cmin = 0, rmax = -1, cmax = -1;
+  bool empty() const { return runs.empty(); }
+  // Region::contains (region.cpp:937-945): lexicographic (row, col_end) search
+  bool contains(int64_t r, int64_t c) const {
+    auto it = std::lower_bound(runs.begin(), runs.end(), std::make_pair(r, c), [](const DRun& x, std::pair<int64_t, int64_t> t) {
+      return x.row < t.first || (x.row == t.first && x.ce < t.second);
+    });
+    return it != runs.end() && it->row == r && it->cb <= c;
+  }
+};
+
+// Region::from_runs (region.cpp:64-117): sort, merge overlapping or adjacent
+// runs of a row, area and bounding box.
+static Domain make_domain(std::vector<DRun> in) {
+  for (const auto& x : in) if (x.cb > x.ce) fail(INVALID_ARGUMENT, "run begin column must not exceed end column");
+  std::sort(in.begin(), in.end(), [](const DRun& a, const DRun& b) {
+    if (a.row != b.row) return a.row < b.row;
+    if (a.cb != b.cb) return a.cb < b.cb;
+    return a.ce < b.ce;
+  });
+  Domain d;
+  if (in.empty()) return d;
+  DRun cur = in.front();
+  auto emit = [&](const DRun& x) { d.runs.push_back(x); d.area += x.ce - x.cb + 1; };
+  for (size_t i = 1; i < in.size(); ++i) {
+    const DRun& x = in[i];
+    if (cur.row == x.row && (x.cb <= cur.ce || x.cb == cur.ce + 1)) cur.ce = std::max(cur.ce, x.ce);
+    else { emit(cur); cur = x; }
+  }
+  emit(cur);
+  d.rmin = d.runs.front().row; d.rmax = d.runs.back().row;
+  d.cmin = d.runs.front().cb; d.cmax = d.runs.front().ce;
+  for (const auto& x : d.runs) { d.cmin = std::min(d.cmin, x.cb); d.cmax = std::max(d.cmax, x.ce); }
+  return d;
+}
+
+// Image::reduce_domain on a full-domain image (image.cpp:267-320): the
+// region clipped to the image rectangle.
+static Domain clip_domain(const Domain& d, int64_t h, int64_t w) {
+  std::vector<DRun> out;
+  for (const auto& x : d.runs)
+    if (x.row >= 0 && x.row < h && x.ce >= 0 && x.cb < w) out.push_back({x.row, std::max<int64_t>(x.cb, 0), std::min<int64_t>(x.ce, w - 1)});
+  return make_domain(std::move(out));
+}
+
 struct Query {
   const Model* model = nullptr;
   FindOpts opt;
@@ -819,6 +883,10 @@ struct Query {
   long crop_r = 0, crop_c = 0;
   size_t img_w = 0, img_h = 0;
   bool empty = false;
+  // the image's domain (image.hpp:108-113): null for the full image
+  const Domain* dom = nullptr;
+  bool sparse = false;                      // area < 0.7 bbox area (matcher.cpp:505)
+  int64_t box_rmin = 0, box_cmin = 0, box_rmax = -1, box_cmax = -1;  // search box (domain bounding box)
   // find_models' query cache (matcher.cpp:281-301, 1078-1081): `cache` is set
   // for every query of find_models; `cache_samples` when it holds more than one
   // model, which converts the whole level 0 to doubles (matcher.cpp:596-611).
@@ -829,9 +897,26 @@ struct Query {
   Prefix fine_prefix;                       // row prefix statistics of the whole level 0 (built on first use)
 };
 
-static bool domain_contains(const Query& q, const Match& m) {  // matcher.cpp:249-251 (full domain)
+static bool domain_contains(const Query& q, const Match& m) {  // matcher.cpp:249-251
   const long r = long(std::floor(m.row + .5)), c = long(std::floor(m.col + .5));
+  if (q.dom) return q.dom->contains(r, c);
   return r >= 0 && c >= 0 && r < long(q.img_h) && c < long(q.img_w);
+}
+
+// may_contain: matcher.cpp:506-534 (the run scan; the cached interval lists
+// of a query cache hold the same runs)
+static bool may_contain(const Query& q, const Match& pose, double spacing, double span = 0) {
+  if (spacing == 1 && span == 0) return domain_contains(q, pose);
+  const double margin = spacing == 1 ? .5 : 2 * spacing;
+  if (pose.row < q.box_rmin - margin || pose.row > q.box_rmax + margin || pose.col + span < q.box_cmin - margin ||
+      pose.col > q.box_cmax + margin)
+    return false;
+  if (!q.sparse) return true;
+  const auto& runs = q.dom->runs;
+  auto it = std::lower_bound(runs.begin(), runs.end(), pose.row - margin, [](const DRun& x, double row) { return double(x.row) < row; });
+  for (; it != runs.end() && double(it->row) <= pose.row + margin; ++it)
+    if (double(it->cb) <= pose.col + span + margin && double(it->ce) >= pose.col - margin) return true;
+  return false;
 }
 
 // Pixel (r, c) of a u8 or u16 image with `stride` bytes per row.
@@ -843,10 +928,11 @@ static double pixel(const uint8_t* img, size_t stride, int bytes, size_t r, size
 }
 
 static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& req, Trace* tr,
-                    int pixel_bytes = 1) {
+                    int pixel_bytes = 1, const Domain* dom = nullptr) {
   FindOpts o = req;
   Q.model = &model; Q.opt = o; Q.img_w = W; Q.img_h = H;
   Q.u8 = pixel_bytes == 1;
+  Q.dom = dom;
   if (!W || !H) fail(INVALID_ARGUMENT, "one nonempty single-channel image required");
   if (model.levels.empty()) fail(INVALID_STATE, "model is empty");
   if (o.highest < 0 || o.lowest < 1) fail(INVALID_ARGUMENT, "invalid pyramid levels");
@@ -874,16 +960,24 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
   Q.opt = o;
   if (W > size_t(std::numeric_limits<int>::max()) || H > size_t(std::numeric_limits<int>::max()) || H > 32000000 / W)
     fail(LIMIT_EXCEEDED, "search image exceeds pixel budget");
-  // crop + halo: matcher.cpp:383-393 (full domain)
+  // search box: the domain's bounding box (matcher.cpp:376-383)
+  if (dom) {
+    if (dom->empty()) { Q.empty = true; return; }
+    Q.box_rmin = dom->rmin; Q.box_cmin = dom->cmin; Q.box_rmax = dom->rmax; Q.box_cmax = dom->cmax;
+    Q.sparse = double(dom->area) < .7 * double(dom->rmax - dom->rmin + 1) * double(dom->cmax - dom->cmin + 1);
+  } else {
+    Q.box_rmin = 0; Q.box_cmin = 0; Q.box_rmax = int64_t(H) - 1; Q.box_cmax = int64_t(W) - 1;
+  }
+  // crop + halo: matcher.cpp:383-393
   const Level& base = model.levels.front();
   const double radius = std::hypot(std::max(base.orow, double(base.h - 1) - base.orow),
                                    std::max(base.ocol, double(base.w - 1) - base.ocol));
   const long lattice = long(size_t(1) << (level_count - 1));
   const long halo = long(std::ceil(radius)) + 4 * lattice;
-  const long crop_r = std::max<long>(0, 0 - halo) / lattice * lattice;
-  const long crop_c = std::max<long>(0, 0 - halo) / lattice * lattice;
-  const long crop_b = std::min<long>(long(H), ((long(H) - 1 + halo) / lattice + 1) * lattice);
-  const long crop_rt = std::min<long>(long(W), ((long(W) - 1 + halo) / lattice + 1) * lattice);
+  const long crop_r = std::max<long>(0, long(Q.box_rmin) - halo) / lattice * lattice;
+  const long crop_c = std::max<long>(0, long(Q.box_cmin) - halo) / lattice * lattice;
+  const long crop_b = std::min<long>(long(H), ((long(Q.box_rmax) + halo) / lattice + 1) * lattice);
+  const long crop_rt = std::min<long>(long(W), ((long(Q.box_cmax) + halo) / lattice + 1) * lattice);
   Q.crop_r = crop_r; Q.crop_c = crop_c;
   // search pyramid: matcher.cpp:401-455
   auto& pyr = Q.pyr;
@@ -962,7 +1056,11 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
     const Patch& p = *pp;
     std::fill(scores.begin(), scores.end(), kMissing);
     const double margin = scale == 1 ? .5 : 2 * scale;
-    const long row_min_box = 0, row_max_box = long(H) - 1, col_min_box = 0, col_max_box = long(W) - 1;
+    const long row_min_box = long(Q.box_rmin), row_max_box = long(Q.box_rmax), col_min_box = long(Q.box_cmin),
+               col_max_box = long(Q.box_cmax);
+    auto pose_of = [&](int r, int c) {
+      return Match{crop_r + scale * (r + p.orow) + phase, crop_c + scale * (c + p.ocol) + phase, p.angle, 0};
+    };
     const int rb = std::max({0, -p.rmin, int(std::ceil((row_min_box - margin - crop_r - phase) / scale - p.orow))});
     const int re = std::min(int(coarse.h) - p.rmax, int(std::floor((row_max_box + margin - crop_r - phase) / scale - p.orow)) + 1);
     const int cb = std::max({0, -p.cmin, int(std::ceil((col_min_box - margin - crop_c - phase) / scale - p.ocol))});
@@ -971,27 +1069,28 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
       int c = cb;
       if (have_top_prefix) {
         for (; c + 15 < ce2; c += 16) {
-          if (!has_support(p, r, c, 15)) continue;
+          if (!may_contain(Q, pose_of(r, c), scale, 15 * scale) || !has_support(p, r, c, 15)) continue;
           for (int l = 0; l < 16; ++l)
             scores[r * coarse.w + c + l] = prefix_window(coarse.data(), top_prefix, coarse.w, p, r, c + l, absolute, exact_top, DotOrder::Single);
+          for (int l = 0; l < 16; ++l) if (!may_contain(Q, pose_of(r, c + l), scale)) scores[r * coarse.w + c + l] = kMissing;
           if (tr) { tr->coarse_windows += 16; tr->coarse_samples += 16.0 * p.s.size(); }
         }
+      }
+      if (have_top_prefix || !coarse.bytes) {
+        // four windows: prefix form, or without row prefixes (u16 image) the
+        // direct form (matcher.cpp:565-571)
         for (; c + 3 < ce2; c += 4) {
-          if (!has_support(p, r, c, 3)) continue;
-          for (int l = 0; l < 4; ++l)
-            scores[r * coarse.w + c + l] = prefix_window(coarse.data(), top_prefix, coarse.w, p, r, c + l, absolute, exact_top, DotOrder::Four);
-          if (tr) { tr->coarse_windows += 4; tr->coarse_samples += 4.0 * p.s.size(); }
-        }
-      } else if (!coarse.bytes) {
-        // no row prefixes (u16 image): direct four-window form (matcher.cpp:565-571)
-        for (; c + 3 < ce2; c += 4) {
-          if (!has_support(p, r, c, 3)) continue;
-          direct_four(coarse.data(), coarse.w, p, r, c, absolute, scores.data() + r * coarse.w + c);
+          if (!may_contain(Q, pose_of(r, c), scale, 3 * scale) || !has_support(p, r, c, 3)) continue;
+          if (have_top_prefix)
+            for (int l = 0; l < 4; ++l)
+              scores[r * coarse.w + c + l] = prefix_window(coarse.data(), top_prefix, coarse.w, p, r, c + l, absolute, exact_top, DotOrder::Four);
+          else direct_four(coarse.data(), coarse.w, p, r, c, absolute, scores.data() + r * coarse.w + c);
+          for (int l = 0; l < 4; ++l) if (!may_contain(Q, pose_of(r, c + l), scale)) scores[r * coarse.w + c + l] = kMissing;
           if (tr) { tr->coarse_windows += 4; tr->coarse_samples += 4.0 * p.s.size(); }
         }
       }
       for (; c < ce2; ++c)
-        if (has_support(p, r, c, 0)) {
+        if (may_contain(Q, pose_of(r, c), scale) && has_support(p, r, c, 0)) {
           scores[r * coarse.w + c] = discrete(coarse, p, r, c, absolute, false);
           if (tr) { tr->coarse_windows += 1; tr->coarse_samples += double(p.s.size()); }
         }
@@ -1046,8 +1145,23 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
   // coarse-to-fine tracking: matcher.cpp:630-761 (no query cache)
   for (size_t l = top; l > finest; --l) {
     if (l == 1 && o.subpixel) {
-      // full domain: every candidate rounds into the domain, so no snapping (matcher.cpp:633-643)
-      for (auto& p : cand) if (!domain_contains(Q, p)) p.score = kMissing;
+      // candidates outside the domain snap to the nearest domain pixel within
+      // 4 rows and 4 columns (first run wins ties), else drop (matcher.cpp:631-645)
+      for (auto& p : cand) {
+        if (domain_contains(Q, p)) continue;
+        if (!Q.dom) { p.score = kMissing; continue; }
+        double best = std::numeric_limits<double>::infinity();
+        Match nearest = p;
+        const auto& runs = Q.dom->runs;
+        auto it = std::lower_bound(runs.begin(), runs.end(), p.row - 4, [](const DRun& x, double row) { return double(x.row) < row; });
+        for (; it != runs.end() && double(it->row) <= p.row + 4; ++it) {
+          const double c = std::clamp(std::round(p.col), double(it->cb), double(it->ce));
+          if (std::abs(c - p.col) > 4) continue;
+          const double d = std::hypot(double(it->row) - p.row, c - p.col);
+          if (d < best) { best = d; nearest.row = double(it->row); nearest.col = c; }
+        }
+        if (std::isfinite(best)) p = nearest; else p.score = kMissing;
+      }
       merge_identical(cand);
       break;
     }
@@ -1073,7 +1187,10 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
             double vals[5];
             for (int ln = 0; ln < 5; ++ln)
               vals[ln] = prefix_window(tg.data(), pf, tg.w, p, row + dr, col - 2 + ln, absolute, exact_l, DotOrder::Single);
-            for (int ln = 0; ln < 5; ++ln) { Match ps = pose_at(row + dr, col - 2 + ln, vals[ln]); if (ps.score > best.score) best = ps; }
+            for (int ln = 0; ln < 5; ++ln) {
+              Match ps = pose_at(row + dr, col - 2 + ln, vals[ln]);
+              if (may_contain(Q, ps, step) && ps.score > best.score) best = ps;
+            }
             if (tr) { tr->track_windows += 5; tr->track_samples += 5.0 * p.s.size(); }
             dc = 3;
           } else if (!tg.bytes && row + dr + p.rmin >= int(tg.sr) && row + dr + p.rmax < int(tg.sr + tg.sh) &&
@@ -1084,12 +1201,16 @@ static void prepare(Query& Q, const uint8_t* img, size_t W, size_t H, size_t str
                 vals[ln] = prefix_window(tg.data(), pf, tg.w, p, row + dr, col - 2 + ln, absolute, exact_l, DotOrder::Four);
             else
               direct_four(tg.data(), tg.sw, p, row + dr - int(tg.sr), col - 2 - int(tg.sc), absolute, vals);
-            for (int ln = 0; ln < 4; ++ln) { Match ps = pose_at(row + dr, col - 2 + ln, vals[ln]); if (ps.score > best.score) best = ps; }
+            for (int ln = 0; ln < 4; ++ln) {
+              Match ps = pose_at(row + dr, col - 2 + ln, vals[ln]);
+              if (may_contain(Q, ps, step) && ps.score > best.score) best = ps;
+            }
             if (tr) { tr->track_windows += 4; tr->track_samples += 4.0 * p.s.size(); }
             dc = 2;
           }
           for (; dc <= 2; ++dc) {
             Match ps = pose_at(row + dr, col + dc, 0);
+            if (!may_contain(Q, ps, step)) continue;
             ps.score = discrete(tg, p, row + dr, col + dc, absolute, true);
             if (tr) { tr->track_windows += 1; tr->track_samples += double(p.s.size()); }
             if (ps.score > best.score) best = ps;
@@ -1249,8 +1370,8 @@ static bool refine(Query& Q, Match& cd) {
 
 // find: matcher.cpp:1063-1076
 static std::vector<Match> find(const uint8_t* img, size_t W, size_t H, size_t stride, const Model& model, const FindOpts& o, Trace* tr,
-                               int pixel_bytes = 1) {
-  Query Q; prepare(Q, img, W, H, stride, model, o, tr, pixel_bytes);
+                               int pixel_bytes = 1, const Domain* dom = nullptr) {
+  Query Q; prepare(Q, img, W, H, stride, model, o, tr, pixel_bytes, dom);
   std::vector<Match> result;
   if (tr) { tr->refined.assign(Q.cand.size(), Match{}); tr->accepted_flag.assign(Q.cand.size(), -1); }
   const Level& lv0 = model.levels.front();
@@ -1395,7 +1516,8 @@ static bool same_options(const FindOpts& a, const FindOpts& b) {  // matcher.cpp
 static std::vector<MultiMatch> find_models(const uint8_t* img, size_t W, size_t H, size_t stride,
                                            const std::vector<const Model*>& models, const std::vector<FindOpts>& opts,
                                            Selection sel, bool per_model_quota_before_overlap,
-                                           std::vector<std::vector<Match>>* prepared = nullptr, int pixel_bytes = 1) {
+                                           std::vector<std::vector<Match>>* prepared = nullptr, int pixel_bytes = 1,
+                                           const Domain* dom = nullptr) {
   const bool cache_samples = models.size() > 1, unlimited = sel.num_matches == 0;
   bool cache_derivatives = false;
   for (const Model* m : models) if (m != models.front()) cache_derivatives = true;
@@ -1416,12 +1538,13 @@ static std::vector<MultiMatch> find_models(const uint8_t* img, size_t W, size_t 
     }
     auto Q = std::make_unique<Query>();
     Q->cache = true; Q->cache_samples = cache_samples;
-    prepare(*Q, img, W, H, stride, *models[i], opts[i], nullptr, pixel_bytes);
-    // default background guard: matcher.cpp:839-846 (AVX2 host, round to nearest)
+    prepare(*Q, img, W, H, stride, *models[i], opts[i], nullptr, pixel_bytes, dom);
+    // default background guard: matcher.cpp:839-846 (AVX2 host, round to
+    // nearest, byte image, no reduced domain)
     const Model& m = *models[i]; const FindOpts& o = Q->opt;
     if (!Q->empty && !Q->finest && cache_derivatives && unlimited && Q->cand.size() >= 32 && o.subpixel && !o.num_matches &&
         o.min_score >= .75 && o.min_score <= .85 && Q->level_count == 3 && !m.opt.absolute && m.opt.step <= .02 &&
-        o.extent <= .4 && m.original.s.size() <= 32768 && Q->shared_full) {
+        o.extent <= .4 && m.original.s.size() <= 32768 && Q->shared_full && !dom) {
       Q->guard_first = o.min_score >= .8 ? .1 : .2;
       Q->guard = .1;
     }
@@ -1456,11 +1579,16 @@ extern "C" {
 
 const char* orc_last_error() { return g_last_error.c_str(); }
 
+// Creates a model from a u8 template; dom_runs (n_dom_runs x 3, canonical)
+// is its reduced domain when reduced != 0.
 int orc_create(const uint8_t* px, int w, int h, int stride, int levels, double start, double extent, double step,
-               int absolute, void** out) {
+               int absolute, const int64_t* dom_runs, int n_dom_runs, int reduced, void** out) {
   return guard([&] {
     CreateOpts o; o.levels = levels; o.absolute = absolute != 0; o.start = start; o.extent = extent; o.step = step;
-    *out = new Model(create(px, size_t(w), size_t(h), size_t(stride), o));
+    std::vector<std::array<int64_t, 3>> runs;
+    for (int i = 0; i < n_dom_runs; ++i) runs.push_back({dom_runs[3 * i], dom_runs[3 * i + 1], dom_runs[3 * i + 2]});
+    if (reduced && runs.empty()) fail(INVALID_ARGUMENT, "template domain requires at least four pixels");
+    *out = new Model(create(px, size_t(w), size_t(h), size_t(stride), o, std::move(runs)));
   });
 }
 void orc_free(void* m) { delete static_cast<Model*>(m); }
@@ -1541,21 +1669,45 @@ struct OrcFindOut {
   long crop_r, crop_c, store_r, store_c, store_h, store_w;
 };
 
-// Runs find on a u8 or u16 image (pixel_bytes 1 or 2, stride in bytes).
+// The domain of an image reduced to the given runs (n x 3: row, first
+// column, last column; any order, overlaps allowed), clipped to the image.
+static Domain image_domain(const int64_t* runs, int n, int h, int w) {
+  std::vector<DRun> v;
+  for (int i = 0; i < n; ++i) v.push_back({runs[3 * i], runs[3 * i + 1], runs[3 * i + 2]});
+  return clip_domain(make_domain(std::move(v)), h, w);
+}
+
+// The canonical runs of image_domain (out: cap x 3); returns their number.
+int orc_domain_runs(const int64_t* runs, int n, int h, int w, int64_t* out, int cap) {
+  int count = -1;
+  const int st = guard([&] {
+    const Domain d = image_domain(runs, n, h, w);
+    for (size_t i = 0; i < d.runs.size() && int(i) < cap; ++i) {
+      out[3 * i] = d.runs[i].row; out[3 * i + 1] = d.runs[i].cb; out[3 * i + 2] = d.runs[i].ce;
+    }
+    count = int(d.runs.size());
+  });
+  return st ? st : count;
+}
+
+// Runs find on a u8 or u16 image (pixel_bytes 1 or 2, stride in bytes),
+// optionally with its domain reduced to dom_runs (reduced != 0).
 // results: cap x 4 (row, col, angle, score) in the reference's
 // internal convention (centroid coordinates, no output shift).  Optional
 // traces: coarse (cap_c x 4), prepared (cap_p x 4), refined (cap_p x 5 with
 // the verdict in column 4).
-int orc_find(void* mp, const uint8_t* img, int w, int h, int stride, int pixel_bytes, double start, double extent,
-             double min_score, int num_matches, double max_overlap, int subpixel, int highest, int lowest,
-             double* results, int cap, double* coarse, int cap_c, double* prepared, int cap_p, double* refined,
-             OrcFindOut* info) {
+int orc_find(void* mp, const uint8_t* img, int w, int h, int stride, int pixel_bytes, const int64_t* dom_runs,
+             int n_dom_runs, int reduced, double start, double extent, double min_score, int num_matches,
+             double max_overlap, int subpixel, int highest, int lowest, double* results, int cap, double* coarse,
+             int cap_c, double* prepared, int cap_p, double* refined, OrcFindOut* info) {
   return guard([&] {
     FindOpts o; o.start = start; o.extent = extent; o.min_score = min_score; o.num_matches = size_t(num_matches);
     o.max_overlap = max_overlap; o.subpixel = subpixel != 0; o.highest = highest; o.lowest = lowest;
     Trace tr;
     if (pixel_bytes != 1 && pixel_bytes != 2) fail(INVALID_ARGUMENT, "pixel type must be byte or uint2");
-    auto res = find(img, size_t(w), size_t(h), size_t(stride), *static_cast<Model*>(mp), o, &tr, pixel_bytes);
+    const Domain dom = reduced ? image_domain(dom_runs, n_dom_runs, h, w) : Domain{};
+    auto res = find(img, size_t(w), size_t(h), size_t(stride), *static_cast<Model*>(mp), o, &tr, pixel_bytes,
+                    reduced ? &dom : nullptr);
     auto put = [](const Match& m, double* d) { d[0] = m.row; d[1] = m.col; d[2] = m.angle; d[3] = m.score; };
     for (size_t i = 0; i < res.size() && int(i) < cap; ++i) put(res[i], results + 4 * i);
     if (coarse) for (size_t i = 0; i < tr.coarse.size() && int(i) < cap_c; ++i) put(tr.coarse[i], coarse + 4 * i);
@@ -1600,13 +1752,13 @@ double orc_continuous(void* mp, const uint8_t* img, int w, int h, int stride, do
   return continuous(g, m.levels.front(), m.original, Match{row, col, angle, 0}, m.opt.absolute);
 }
 
-// find_models.  opts: n_models x 8 doubles (start, extent, min_score,
+// find_models (domain as for orc_find).  opts: n_models x 8 doubles (start, extent, min_score,
 // max_overlap, num_matches, subpixel, highest, lowest); selection as the C
 // ABI derives it (num_matches -1 / max_overlap -1 mean per model).  results:
 // cap x 5 (row, col, angle, score, model) in the internal convention.
 // prepared (optional): n_models x cap_p x 4, with prepared_n the list sizes.
 int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, int h, int stride, int pixel_bytes,
-                    const double* opts,
+                    const int64_t* dom_runs, int n_dom_runs, int reduced, const double* opts,
                     int64_t sel_num, double sel_overlap, int per_model_quota, double* results, int cap, int* n_result,
                     double* prepared, int cap_p, int* prepared_n) {
   return guard([&] {
@@ -1620,8 +1772,9 @@ int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, i
       f.subpixel = q[5] != 0; f.highest = int(q[6]); f.lowest = int(q[7]);
     }
     std::vector<std::vector<Match>> prep;
+    const Domain dom = reduced ? image_domain(dom_runs, n_dom_runs, h, w) : Domain{};
     auto res = find_models(img, size_t(w), size_t(h), size_t(stride), models, o, Selection{sel_num, sel_overlap},
-                           per_model_quota != 0, prepared ? &prep : nullptr, pixel_bytes);
+                           per_model_quota != 0, prepared ? &prep : nullptr, pixel_bytes, reduced ? &dom : nullptr);
     *n_result = int(res.size());
     for (size_t i = 0; i < res.size() && int(i) < cap; ++i) {
       double* d = results + 5 * i;

@@ -42,7 +42,7 @@ def lib():
         D = C.c_double
         I = C.c_int
         L.orc_last_error.restype = C.c_char_p
-        L.orc_create.argtypes = [P, I, I, I, I, D, D, D, I, C.POINTER(P)]
+        L.orc_create.argtypes = [P, I, I, I, I, D, D, D, I, P, I, I, C.POINTER(P)]
         L.orc_free.argtypes = [P]
         L.orc_model_info.argtypes = [P, P]
         L.orc_level_info.argtypes = [P, I, P]
@@ -51,7 +51,7 @@ def lib():
         L.orc_level_gray.argtypes = [P, I, P]
         L.orc_pyramid.argtypes = [P, I, I, I, I, P]
         L.orc_row_prefix.argtypes = [P, I, I, P, P]
-        L.orc_find.argtypes = [P, P, I, I, I, I, D, D, D, I, D, I, I, I, P, I, P, I, P, I, P, C.POINTER(FindOut)]
+        L.orc_find.argtypes = [P, P, I, I, I, I, P, I, I, D, D, D, I, D, I, I, I, P, I, P, I, P, I, P, C.POINTER(FindOut)]
         L.orc_zncc.argtypes = [P, P, I, I]
         L.orc_zncc.restype = D
         L.orc_overlap.argtypes = [D, D, D, D, D, D, I, I, D, D]
@@ -59,11 +59,12 @@ def lib():
         L.orc_continuous.argtypes = [P, P, I, I, I, D, D, D]
         L.orc_continuous.restype = D
         L.orc_refine_one.argtypes = [P, P, I, I, I, D, D, D, I, D, I, P]
-        L.orc_find_models.argtypes = [P, I, P, I, I, I, I, P, C.c_int64, D, I, P, I, P, P, I, P]
+        L.orc_find_models.argtypes = [P, I, P, I, I, I, I, P, I, I, P, C.c_int64, D, I, P, I, P, P, I, P]
         L.orc_box_overlap.argtypes = [D, D, D, P, D, D, D, P]
         L.orc_box_overlap.restype = D
         L.orc_serialize.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_size_t)]
         L.orc_deserialize.argtypes = [P, C.c_size_t, C.POINTER(P)]
+        L.orc_domain_runs.argtypes = [P, I, I, I, P, I]
         L.orc_checksum.argtypes = [P, C.c_size_t]
         L.orc_checksum.restype = C.c_uint64
         _lib = L
@@ -82,11 +83,14 @@ class OracleError(RuntimeError):
 
 class Model:
     def __init__(self, template: np.ndarray, levels: int = 0, start: float = 0.0, extent: float = 0.0,
-                 step: float = 0.0, absolute: bool = False):
+                 step: float = 0.0, absolute: bool = False, domain=None):
+        """domain: None (the whole template) or canonical runs (n x 3: row,
+        first column, last column) of the template's reduced domain."""
         t = np.ascontiguousarray(template, dtype=np.uint8)
+        runs, n_runs, reduced = _domain(domain)
         h = C.c_void_p()
         st = lib().orc_create(_ptr(t), t.shape[1], t.shape[0], t.shape[1], levels, start, extent, step,
-                              int(absolute), C.byref(h))
+                              int(absolute), _ptr(runs), n_runs, reduced, C.byref(h))
         if st:
             raise OracleError(st, lib().orc_last_error().decode())
         self.h = h
@@ -144,17 +148,21 @@ class Model:
                 "n_wruns": int(hdr[10]), "rows": rows, "cols": cols, "w": w}
 
     def find(self, image: np.ndarray, start: float, extent: float, min_score: float, num_matches: int,
-             max_overlap: float, subpixel: bool = True, highest: int = 0, lowest: int = 1, trace: bool = False):
-        """find on a u8 or u16 image (the dtype selects the pixel type)."""
+             max_overlap: float, subpixel: bool = True, highest: int = 0, lowest: int = 1, trace: bool = False,
+             domain=None):
+        """find on a u8 or u16 image (the dtype selects the pixel type);
+        domain: None (full image) or runs (n x 3: row, first column, last
+        column) the image's domain is reduced to."""
         img = search_image(image)
+        runs, n_runs, reduced = _domain(domain)
         cap = 4096
         res = np.zeros((cap, 4))
         coarse = np.zeros((20000, 4)) if trace else None
         prep = np.zeros((20000, 4)) if trace else None
         refd = np.zeros((20000, 5)) if trace else None
         info = FindOut()
-        st = lib().orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize, start,
-                            extent, min_score,
+        st = lib().orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize,
+                            _ptr(runs), n_runs, reduced, start, extent, min_score,
                             num_matches, max_overlap, int(subpixel), highest, lowest, _ptr(res), cap,
                             _ptr(coarse) if trace else None, 20000, _ptr(prep) if trace else None, 20000,
                             _ptr(refd) if trace else None, C.byref(info))
@@ -208,19 +216,39 @@ def overlap(a, b, w, h, orow, ocol) -> float:
     return lib().orc_overlap(a[0], a[1], a[2], b[0], b[1], b[2], w, h, orow, ocol)
 
 
+def _domain(domain):
+    if domain is None:
+        return np.zeros((1, 3), np.int64), 0, 0
+    runs = np.ascontiguousarray(np.asarray(domain, dtype=np.int64).reshape(-1, 3))
+    return (runs if len(runs) else np.zeros((1, 3), np.int64)), len(runs), 1
+
+
+def domain_runs(runs, h: int, w: int) -> np.ndarray:
+    """The oracle's canonical form of a domain given by runs, clipped to an
+    h x w image (n x 3: row, first column, last column)."""
+    r, n, _ = _domain(runs)
+    cap = max(1, 2 * n + 16)
+    out = np.zeros((cap, 3), np.int64)
+    k = lib().orc_domain_runs(_ptr(r), n, h, w, _ptr(out), cap)
+    if k < 0:
+        raise OracleError(k, lib().orc_last_error().decode())
+    return out[:k].copy()
+
+
 def search_image(image: np.ndarray) -> np.ndarray:
     """A contiguous u16 array for u16 input, else u8."""
     return np.ascontiguousarray(image, dtype=np.uint16 if image.dtype == np.uint16 else np.uint8)
 
 
 def find_models(models: list, image: np.ndarray, opts: list, selection_matches: int = -1,
-                selection_overlap: float = -1.0, per_model_quota: bool = True, prepared: bool = False):
+                selection_overlap: float = -1.0, per_model_quota: bool = True, prepared: bool = False, domain=None):
     """find_models over oracle models.  opts: one dict per model with keys
     start, extent, min_score, max_overlap, num_matches, subpixel, highest,
     lowest.  Returns (n, 5): row, col, angle, score (internal convention),
     model index; with prepared=True also each model's prepared list.  The
     image is u8 or u16 (by dtype)."""
     img = search_image(image)
+    runs, n_runs, reduced = _domain(domain)
     n = len(models)
     handles = (C.c_void_p * n)(*[m.h.value for m in models])
     o = np.zeros((n, 8))
@@ -232,7 +260,8 @@ def find_models(models: list, image: np.ndarray, opts: list, selection_matches: 
     nres = C.c_int()
     prep = np.zeros((n, cap_p, 4)) if prepared else None
     prep_n = np.zeros(n, np.int32) if prepared else None
-    st = lib().orc_find_models(handles, n, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize, _ptr(o),
+    st = lib().orc_find_models(handles, n, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize,
+                               _ptr(runs), n_runs, reduced, _ptr(o),
                                int(selection_matches), float(selection_overlap), int(per_model_quota), _ptr(res), cap,
                                C.byref(nres), _ptr(prep) if prepared else None, cap_p,
                                _ptr(prep_n) if prepared else None)

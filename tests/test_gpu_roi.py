@@ -1,0 +1,161 @@
+"""GPU parity of reduced-domain (ROI) searches and templates
+(matcher.cpp:376-393, 505-534, 631-645, 1060; model.cpp:120-171): the HIP
+path through mv_find_ncc_model / mv_find_ncc_models on image objects whose
+domain was reduced with mv_reduce_domain, and mvamd_find_ncc_model_image for
+the split-BF16 form, against the oracle's ROI find on the same domains."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ANGLE_TOL, POS_TOL, SCORE_TOL
+from mvkit_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+E = math.radians(20)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    torch.cuda.set_device(0)
+    return torch.device("cuda:0")
+
+
+def check(got, want, label=""):
+    assert got.shape == want.shape, f"{label}: count {len(got)} != {len(want)}\n{got}\n{want}"
+    if len(want) == 0:
+        return
+    assert np.abs(got[:, 0] - want[:, 0]).max() <= POS_TOL, label
+    assert np.abs(got[:, 1] - want[:, 1]).max() <= POS_TOL, label
+    assert np.abs(got[:, 2] - want[:, 2]).max() <= ANGLE_TOL, label
+    assert np.abs(got[:, 3] - want[:, 3]).max() <= SCORE_TOL, label
+
+
+@pytest.fixture(scope="module")
+def small(capi, oracle, scene_small):
+    m = capi.NccModel(scene_small.template, "auto", -E, 2 * E)
+    o = oracle.Model(scene_small.template, levels=0, start=-E, extent=2 * E)
+    return m, o
+
+
+def run(capi, oracle, scene, m, o, dom, min_score=0.7, nm=0, subpixel=True, label=""):
+    h, w = scene.image.shape
+    dom = oracle.domain_runs(dom, h, w)
+    say(f"ROI find {label}: {len(dom)} runs, min_score={min_score}, subpixel={subpixel}")
+    want = o.find(scene.image, -E, 2 * E, min_score, nm, 0.5, subpixel, domain=dom)
+    img = capi.Image(scene.image, domain=dom)
+    got = m.find(img, -E, 2 * E, min_score, nm, 0.5, "true" if subpixel else "false", 0)
+    check(got, capi.halcon_shift(want) if len(want) else want, label)
+    return got, want
+
+
+def domains(scene, truth):
+    h, w = scene.image.shape
+    out = [("full_rect", synth.rect_runs(0, 0, h - 1, w - 1)), ("empty", np.zeros((0, 3), np.int64))]
+    for k, (cy, cx, _) in enumerate(truth):
+        out.append((f"rect{k}", synth.rect_runs(int(cy) - 25, int(cx) - 25, int(cy) + 25, int(cx) + 25)))
+        out.append((f"disk{k}", synth.disk_runs(round(cy), round(cx), 3.0)))
+        c0 = int(round(cx)) + 2
+        out.append((f"band{k}", synth.rect_runs(int(cy) - 15, c0, int(cy) + 15, c0 + 2)))
+    rng = synth.SplitMix64(4)
+    for k in range(3):
+        out.append((f"random{k}", synth.random_domain(rng, h, w, 60 + 60 * k, max_len=25)))
+    return out
+
+
+def test_roi_find_matches_oracle(capi, oracle, scene_small, small):
+    m, o = small
+    for label, dom in domains(scene_small, scene_small.truth):
+        min_score = 0.3 if label.startswith(("band", "random")) else 0.7
+        run(capi, oracle, scene_small, m, o, dom, min_score, label=label)
+
+
+def test_roi_find_without_subpixel(capi, oracle, scene_small, small):
+    """Tracking down to level 0 tests each position's own pixel."""
+    m, o = small
+    for label, dom in domains(scene_small, scene_small.truth)[2:8]:
+        run(capi, oracle, scene_small, m, o, dom, 0.5, subpixel=False, label=label + "_nosub")
+
+
+def test_roi_split_bf16_form(capi, oracle, scene_small, small):
+    """The matrix-core coarse form writes [row][col][angle] planes; the
+    domain masks them the same way."""
+    m, o = small
+    h, w = scene_small.image.shape
+    for label, dom in domains(scene_small, scene_small.truth)[2:6]:
+        dom = oracle.domain_runs(dom, h, w)
+        say(f"ROI bf16x3 {label}")
+        want = o.find(scene_small.image, -E, 2 * E, 0.7, 0, 0.5, domain=dom)
+        opts = capi.NccModel.options(-E, 2 * E, 0.7, 0, 0.5, form=capi.FORM_BF16X3)
+        got, _ = m.find_image(capi.Image(scene_small.image, domain=dom), opts)
+        check(got, want, label)
+
+
+def test_roi_rot2048_sparse(capi, oracle, scene_rot):
+    """Two small disks around two of the five instances: a sparse domain over
+    the 2048x2048 workload."""
+    a, x = -math.pi / 6, math.pi / 3
+    m = capi.NccModel(scene_rot.template, 4, a, x)
+    o = oracle.Model(scene_rot.template, levels=4, start=a, extent=x)
+    t0, t1 = scene_rot.truth[0], scene_rot.truth[3]
+    dom = np.concatenate([synth.disk_runs(round(t0[0]), round(t0[1]), 20.0),
+                          synth.disk_runs(round(t1[0]), round(t1[1]), 20.0)])
+    h, w = scene_rot.image.shape
+    dom = oracle.domain_runs(dom, h, w)
+    say("ROI rot2048")
+    want = o.find(scene_rot.image, a, x, 0.7, 0, 0.5, domain=dom)
+    got = m.find(capi.Image(scene_rot.image, domain=dom), a, x, 0.7, 0, 0.5, "true", 0)
+    check(got, capi.halcon_shift(want), "rot2048 roi")
+    assert len(got) == 2
+
+
+def test_roi_find_models(capi, oracle):
+    s = synth.small_multi()
+    e = math.radians(16)
+    c = [capi.NccModel(t, 3, -e, 2 * e) for t in s.templates]
+    o = [oracle.Model(t, levels=3, start=-e, extent=2 * e) for t in s.templates]
+    h, w = s.image.shape
+    parts = [synth.disk_runs(round(r), round(col), 6.0) for _, r, col, _ in s.truth[:3]]
+    dom = oracle.domain_runs(np.concatenate(parts), h, w)
+    say("ROI find_models")
+    got = capi.find_models(c, capi.Image(s.image, domain=dom), -e, 2 * e, 0.7, 0, 0.5, "true", 0)
+    opts = [dict(start=-e, extent=2 * e, min_score=0.7, max_overlap=0.5, num_matches=0)] * 2
+    want = oracle.find_models(o, s.image, opts, 0, 0.5, domain=dom)
+    want[:, :4] = capi.halcon_shift(want[:, :4])
+    assert got.shape == want.shape and len(got) == 3
+    assert np.array_equal(got[:, 4], want[:, 4])
+    check(got[:, :4], want[:, :4], "find_models roi")
+
+
+def test_roi_u16(capi, oracle):
+    s = synth.small_rot16()
+    m = capi.NccModel(s.template, "auto", -E, 2 * E)
+    o = oracle.Model(s.template, levels=0, start=-E, extent=2 * E)
+    for label, dom in domains(s, s.truth)[2:6]:
+        run(capi, oracle, s, m, o, dom, 0.7, label=label + "_u16")
+
+
+def test_domain_template_model(capi, oracle, scene_small):
+    """A model created over an elliptical template domain finds the pastes
+    as the oracle's model of the same domain does."""
+    from test_roi import ellipse
+    t = scene_small.template
+    runs = synth.mask_runs(ellipse(*t.shape))
+    m = capi.NccModel(capi.Image(t, domain=runs), "auto", -E, 2 * E)
+    o = oracle.Model(t, levels=0, start=-E, extent=2 * E, domain=runs)
+    assert m.serialize() == o.serialize()
+    say("find with a domain-template model")
+    want = o.find(scene_small.image, -E, 2 * E, 0.6, 0, 0.5)
+    got = m.find(scene_small.image, -E, 2 * E, 0.6, 0, 0.5, "true", 0)
+    check(got, capi.halcon_shift(want), "domain template")
+    assert len(got) == 2

@@ -83,6 +83,8 @@ def test_oracle_rejects_other_pixel_sizes(oracle, scene_small):
     img = np.ascontiguousarray(scene_small.image)
     res = np.zeros((16, 4))
     info = oracle.FindOut()
-    st = oracle.lib().orc_find(m.h, oracle._ptr(img), img.shape[1], img.shape[0], img.shape[1], 4, -E, 2 * E, 0.7, 0,
-                               0.5, 1, 0, 1, oracle._ptr(res), 16, None, 0, None, 0, None, oracle.C.byref(info))
+    runs = np.zeros((1, 3), np.int64)
+    st = oracle.lib().orc_find(m.h, oracle._ptr(img), img.shape[1], img.shape[0], img.shape[1], 4, oracle._ptr(runs), 0,
+                               0, -E, 2 * E, 0.7, 0, 0.5, 1, 0, 1, oracle._ptr(res), 16, None, 0, None, 0, None,
+                               oracle.C.byref(info))
     assert st == -1
