@@ -127,6 +127,17 @@ MV_API MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* image
                                            size_t* n_coarse, double* prepared, double* refined, size_t cap_prepared,
                                            size_t* n_prepared, MvAmdFindStats* stats);
 
+/* mv_create_ncc_model (mvkit include/mvkit/c/mv_ncc_matching.h:30-33) with a
+ * choice of where the rotated rasters are built: MVAMD_CREATE_HOST (host
+ * FP64), MVAMD_CREATE_DEVICE (the calling thread's current HIP device; error
+ * without one) or MVAMD_CREATE_AUTO (the device when one is present, what
+ * mv_create_ncc_model does).  Both builders give bit-identical models. */
+#define MVAMD_CREATE_AUTO 0
+#define MVAMD_CREATE_HOST 1
+#define MVAMD_CREATE_DEVICE 2
+MV_API MvStatus mvamd_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
+                                       MvTuple AngleStep, MvTuple Metric, int32_t builder, MvTuple* ModelID);
+
 /* Model introspection.  info: levels, angles per level, angle step,
  * origin row, origin column, sample count. */
 MV_API MvStatus mvamd_model_info(MvTuple ModelID, double* info);

@@ -136,6 +136,34 @@ int ncchip_domain_planes(const NccFrame* frames, int n_frames, int level, int ma
 int ncchip_domain_snap(const NccFrame* frames, int n_frames, NccMatch* cand, const int32_t* count, int cap,
                        void* stream);
 
+/* Model construction on the device (model.cpp:39-98, 173-242), one
+ * workgroup per raster job.  Stages, in order:
+ *   NCCHIP_CREATE_COUNT: stats[j].count = samples of job j;
+ *   NCCHIP_CREATE_SAMPLES: with jobs[j].sample_off set, the samples (rc, v)
+ *     in grid order, then the weights (w, wf) and the rest of stats[j]
+ *     (status, bounds, weight sums, run counts);
+ *   NCCHIP_CREATE_RUNS: with jobs[j].run_off / wrun_off set, the row runs
+ *     (4 ints: row, first column, last column, 0), the weight runs and their
+ *     weights. */
+#define NCCHIP_CREATE_COUNT 0
+#define NCCHIP_CREATE_SAMPLES 1
+#define NCCHIP_CREATE_RUNS 2
+typedef struct NcchipCreateBuffers {
+  const NccTemplateLevel* levels;
+  const NccRasterJob* jobs;
+  NccRasterStats* stats;
+  int32_t* rc;
+  double* v;
+  double* w;
+  float* wf;
+  int32_t* runs;
+  int32_t* wruns;
+  double* wrun_w;
+  int32_t* run_start;   /* scratch: one int per row run */
+  int32_t* wrun_start;  /* scratch: one int per weight run */
+} NcchipCreateBuffers;
+int ncchip_create_rasters(int stage, const NcchipCreateBuffers* buffers, int n_jobs, void* stream);
+
 /* Every kernel taking `dc_src` reads the reference pixel dc (the first
  * pixel of the search crop, matcher.cpp:410-413) from that device address;
  * integer levels use D = dc << shift. */

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libmvkit_amd.so")
 MV_VALUE_EMPTY, MV_VALUE_INT64, MV_VALUE_REAL64, MV_VALUE_STRING, MV_VALUE_HANDLE = 0, 1, 2, 4, 16
 FORM_FP32, FORM_BF16X3, FORM_FP64, FORM_FP8_SCREEN = 0, 1, 2, 3
 PIXEL_BYTE, PIXEL_UINT2 = 0, 2  # device image pixel types (MVAMD_PIXEL_*)
+CREATE_AUTO, CREATE_HOST, CREATE_DEVICE = 0, 1, 2  # raster builders (MVAMD_CREATE_*)
 
 STATUS_NAMES = {0: "OK", -1: "INVALID_ARGUMENT", -2: "OUT_OF_RANGE", -3: "TYPE_MISMATCH", -4: "NOT_FOUND",
                 -5: "OUT_OF_MEMORY", -6: "UNSUPPORTED", -7: "INVALID_STATE", -100: "IO_ERROR",
@@ -32,14 +33,15 @@ PUBLIC_SYMBOLS = [
     "mv_get_domain",
     "mv_clear_handle", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
     "mv_get_serialized_item_ptr", "mv_write_ncc_model", "mv_read_ncc_model", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
-    "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_image", "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
+    "mvamd_create_ncc_model", "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_image",
+    "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
                   "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma",
                   "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine",
-                  "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap"]
+                  "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap", "ncchip_create_rasters"]
 
 
 class StringView(C.Structure):
@@ -122,6 +124,7 @@ def lib() -> C.CDLL:
         L.mv_get_domain.argtypes = [P, C.POINTER(P)]
         L.mv_clear_handle.argtypes = [P]
         L.mv_create_ncc_model.argtypes = [P, P, P, P, P, P, C.POINTER(P)]
+        L.mvamd_create_ncc_model.argtypes = [P, P, P, P, P, P, C.c_int32, C.POINTER(P)]
         L.mv_get_ncc_model_params.argtypes = [P] + [C.POINTER(P)] * 5
         L.mv_find_ncc_model.argtypes = [P] * 9 + [C.POINTER(P)] * 4
         L.mv_find_ncc_models.argtypes = [P] * 9 + [C.POINTER(P)] * 5
@@ -313,13 +316,18 @@ class NccModel:
     """NCC model handle (mirrors mvkit's MvNCCModel, mv_ncc_matching.hpp)."""
 
     def __init__(self, template, num_levels="auto", angle_start=0.0, angle_extent=0.0,
-                 angle_step="auto", metric="use_polarity"):
-        """template: an array or an Image (which may have a reduced domain)."""
+                 angle_step="auto", metric="use_polarity", builder=None):
+        """template: an array or an Image (which may have a reduced domain).
+        builder: None (mv_create_ncc_model) or CREATE_HOST / CREATE_DEVICE /
+        CREATE_AUTO (mvamd_create_ncc_model)."""
         img = template if isinstance(template, Image) else Image(template)
         out = C.c_void_p()
         args = [_to_tuple(num_levels), _to_tuple(float(angle_start)), _to_tuple(float(angle_extent)),
                 _to_tuple(angle_step), _to_tuple(metric)]
-        _check(lib().mv_create_ncc_model(img.h, *[t.h for t in args], C.byref(out)))
+        if builder is None:
+            _check(lib().mv_create_ncc_model(img.h, *[t.h for t in args], C.byref(out)))
+        else:
+            _check(lib().mvamd_create_ncc_model(img.h, *[t.h for t in args], int(builder), C.byref(out)))
         self._bind(Tuple(handle=out))
 
     @classmethod

@@ -99,6 +99,38 @@ typedef struct NccFrame {
   int32_t dom_pad;
 } NccFrame;
 
+// Model construction on the device (create / rebuild, model.cpp:39-98,
+// 173-242).  One template pyramid level: centred values and mask, h x w.
+typedef struct NccTemplateLevel {
+  const double* gray;
+  const uint8_t* mask;
+  int32_t w, h;
+  double orow, ocol;         // level centroid
+} NccTemplateLevel;
+
+// One raster to build: level, sampling grid (the level grid when unrotated,
+// else the rotated bounding box of offsets from the centroid) and the
+// angle's cosine and sine from the host's libm; prepared: the level-0
+// unrotated raster, centred by the template's mean and energy.
+typedef struct NccRasterJob {
+  double ca, sa;
+  double mean, energy;
+  int32_t level, unrotated, prepared, pad;
+  int32_t r_lo, r_hi, c_lo, c_hi;
+  int64_t sample_off;        // first sample in the output arrays
+  int64_t run_off, wrun_off; // first row run / weight run
+} NccRasterJob;
+
+// What the host needs back of one built raster.
+typedef struct NccRasterStats {
+  int64_t count;             // samples
+  int32_t status;            // 0: usable, 1: degenerate (fewer than 4 samples or no energy)
+  int32_t rmin, rmax, cmin, cmax;
+  int32_t n_runs, n_wruns;
+  int32_t pad;
+  double wsum, wsum_f;
+} NccRasterStats;
+
 #ifdef __cplusplus
 }
 #endif

@@ -754,11 +754,26 @@ extern "C" MvStatus mv_clear_handle(MvTuple Handle) {
 }
 
 // ============================================================ NCC operators
-extern "C" MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
-                                        MvTuple AngleStep, MvTuple Metric, MvTuple* ModelID) {
+namespace {
+bool have_device() {
+  static const bool present = [] {
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+  }();
+  return present;
+}
+
+// mv_create_ncc_model with a choice of raster builder (MVAMD_CREATE_*).
+MvStatus create_model_handle(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
+                             MvTuple AngleStep, MvTuple Metric, int32_t builder, MvTuple* ModelID) {
   return translate([&] {
     constexpr auto op = "create_ncc_model";
     Outputs<1> out(op, {ModelID});
+    if (builder < MVAMD_CREATE_AUTO || builder > MVAMD_CREATE_DEVICE)
+      fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, op, "builder", "unknown raster builder");
+    if (builder == MVAMD_CREATE_DEVICE && !have_device())
+      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "builder", "no HIP device for the device builder");
+    const bool on_device = builder == MVAMD_CREATE_DEVICE || (builder == MVAMD_CREATE_AUTO && have_device());
     const MvObjectImpl& src = single_image(Template, op, "Template");
     const Input levels(NumLevels, op, "NumLevels"), start(AngleStart, op, "AngleStart"),
         extent(AngleExtent, op, "AngleExtent"), step(AngleStep, op, "AngleStep"), metric(Metric, op, "Metric");
@@ -784,13 +799,30 @@ extern "C" MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, Mv
     const std::string& name = metric.string(0);
     if (name == "ignore_global_polarity") p.absolute = true;
     else if (name != "use_polarity") metric.bad(mvamd::kErrUnsupportedValue, 0, "unsupported NCC metric");
-    std::shared_ptr<mvamd::NccModel> m = algorithm(op, [&] {
-      return mvamd::create_model(src.pixels, src.pixel_bytes, src.width, src.height, src.width * src.pixel_bytes, p,
-                                 src.domain.get());
+    std::shared_ptr<mvamd::NccModel> m = algorithm(op, [&]() -> std::unique_ptr<mvamd::NccModel> {
+      if (!on_device)
+        return mvamd::create_model(src.pixels, src.pixel_bytes, src.width, src.height, src.width * src.pixel_bytes,
+                                   p, src.domain.get());
+      mvamd::FirstLevel f = mvamd::create_first_level(src.pixels, src.pixel_bytes, src.width, src.height,
+                                                      src.width * src.pixel_bytes, p, src.domain.get());
+      return context().rebuild(std::move(f.level), p, f.orow, f.ocol);
     });
     put_handle(out[0], std::move(m));
     out.commit();
   });
+}
+}  // namespace
+
+extern "C" MvStatus mv_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
+                                        MvTuple AngleStep, MvTuple Metric, MvTuple* ModelID) {
+  return create_model_handle(Template, NumLevels, AngleStart, AngleExtent, AngleStep, Metric, MVAMD_CREATE_AUTO,
+                             ModelID);
+}
+
+extern "C" MvStatus mvamd_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart,
+                                           MvTuple AngleExtent, MvTuple AngleStep, MvTuple Metric, int32_t builder,
+                                           MvTuple* ModelID) {
+  return create_model_handle(Template, NumLevels, AngleStart, AngleExtent, AngleStep, Metric, builder, ModelID);
 }
 
 extern "C" MvStatus mv_get_ncc_model_params(MvTuple ModelID, MvTuple* NumLevels, MvTuple* AngleStart,

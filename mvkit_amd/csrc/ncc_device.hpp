@@ -167,6 +167,11 @@ class FindContext {
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
 
+  // rebuild (model.cpp:173-242) with the rotated rasters built on this
+  // context's device; the model is bit-identical to rebuild_model's.
+  std::unique_ptr<NccModel> rebuild(LevelGeom first, const CreateParams& params, double origin_row,
+                                    double origin_col);
+
  private:
   struct Slot;  // state of a submitted batch
   struct RefineRequest {
@@ -190,6 +195,7 @@ class FindContext {
   hipStream_t stream2_ = nullptr;  // refinement rounds
   int device_ = 0;
   DeviceBuffer image_;
+  DeviceBuffer create_;  // model construction workspace (grow-only)
 };
 
 // Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).

@@ -111,35 +111,48 @@ bool finish(const Staging& s, double angle, double orow, double ocol, RasterPool
 }
 }  // namespace
 
+RasterGrid raster_grid(const LevelGeom& lv, double angle) {
+  RasterGrid g;
+  g.unrotated = std::abs(std::remainder(angle, kTau)) < 1e-12;
+  if (g.unrotated) {
+    g.r_lo = 0; g.r_hi = int(lv.h) - 1; g.c_lo = 0; g.c_hi = int(lv.w) - 1;
+    return g;
+  }
+  // inverse bilinear rotation about the centroid; raster coordinates are
+  // offsets from the centroid (model.cpp:77-96)
+  g.ca = std::cos(angle); g.sa = std::sin(angle);
+  const double ca = g.ca, sa = g.sa;
+  double lo_r = 0, hi_r = 0, lo_c = 0, hi_c = 0;
+  const double corner_r[2] = {0., double(lv.h - 1)}, corner_c[2] = {0., double(lv.w - 1)};
+  for (double cr : corner_r)
+    for (double cc : corner_c) {
+      const double dr = cr - lv.orow, dc = cc - lv.ocol;
+      const double y = ca * dr - sa * dc, x = sa * dr + ca * dc;
+      lo_r = std::min(lo_r, y); hi_r = std::max(hi_r, y);
+      lo_c = std::min(lo_c, x); hi_c = std::max(hi_c, x);
+    }
+  const double rows = std::ceil(hi_r) - std::floor(lo_r) + 1, cols = std::ceil(hi_c) - std::floor(lo_c) + 1;
+  if (rows > 4 * kMaxPixels / cols) raise(kLimitExceeded, "domain", "rotated bounding box exceeds budget");
+  g.r_lo = int(std::floor(lo_r)); g.r_hi = int(std::ceil(hi_r));
+  g.c_lo = int(std::floor(lo_c)); g.c_hi = int(std::ceil(hi_c));
+  return g;
+}
+
 int64_t build_raster(const LevelGeom& lv, double angle, RasterPool& pool, const double* pmean, const double* penergy) {
   Staging s;
   double orow = 0, ocol = 0;
-  if (std::abs(std::remainder(angle, kTau)) < 1e-12) {
+  const RasterGrid g = raster_grid(lv, angle);
+  if (g.unrotated) {
     // unrotated raster on the level grid, origin at the level centroid
     orow = lv.orow; ocol = lv.ocol;
     for (size_t r = 0; r < lv.h; ++r)
       for (size_t c = 0; c < lv.w; ++c)
         if (lv.mask[r * lv.w + c]) { s.r.push_back(int32_t(r)); s.c.push_back(int32_t(c)); s.v.push_back(lv.gray[r * lv.w + c]); }
   } else {
-    // inverse bilinear rotation about the centroid; raster coordinates are
-    // offsets from the centroid (model.cpp:77-96)
-    const double ca = std::cos(angle), sa = std::sin(angle);
-    double lo_r = 0, hi_r = 0, lo_c = 0, hi_c = 0;
-    const double corner_r[2] = {0., double(lv.h - 1)}, corner_c[2] = {0., double(lv.w - 1)};
-    for (double cr : corner_r)
-      for (double cc : corner_c) {
-        const double dr = cr - lv.orow, dc = cc - lv.ocol;
-        const double y = ca * dr - sa * dc, x = sa * dr + ca * dc;
-        lo_r = std::min(lo_r, y); hi_r = std::max(hi_r, y);
-        lo_c = std::min(lo_c, x); hi_c = std::max(hi_c, x);
-      }
-    const double rows = std::ceil(hi_r) - std::floor(lo_r) + 1, cols = std::ceil(hi_c) - std::floor(lo_c) + 1;
-    if (rows > 4 * kMaxPixels / cols) raise(kLimitExceeded, "domain", "rotated bounding box exceeds budget");
-    const int r_lo = int(std::floor(lo_r)), r_hi = int(std::ceil(hi_r));
-    const int c_lo = int(std::floor(lo_c)), c_hi = int(std::ceil(hi_c));
+    const double ca = g.ca, sa = g.sa;
     const double ymax = double(lv.h - 1), xmax = double(lv.w - 1);
-    for (int r = r_lo; r <= r_hi; ++r)
-      for (int c = c_lo; c <= c_hi; ++c) {
+    for (int r = g.r_lo; r <= g.r_hi; ++r)
+      for (int c = g.c_lo; c <= g.c_hi; ++c) {
         const double ty = ca * r + sa * c + lv.orow, tx = -sa * r + ca * c + lv.ocol;
         if (ty < 0 || tx < 0 || ty > ymax || tx > xmax) continue;
         const size_t y0 = size_t(ty), x0 = size_t(tx);
@@ -154,7 +167,7 @@ int64_t build_raster(const LevelGeom& lv, double angle, RasterPool& pool, const 
   }
   // the shared first-level mean/energy only replaces the reduction of the
   // unrotated raster (model.cpp:154-158); rotated rasters normalise themselves
-  const bool unrotated = std::abs(std::remainder(angle, kTau)) < 1e-12;
+  const bool unrotated = g.unrotated;
   Raster out;
   if (!finish(s, angle, orow, ocol, pool, unrotated ? pmean : nullptr, unrotated ? penergy : nullptr, out)) return -1;
   pool.rasters.push_back(out);
@@ -198,6 +211,12 @@ NccModel::~NccModel() = default;
 
 std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size_t width, size_t height,
                                        size_t stride_bytes, const CreateParams& p, const Region* domain) {
+  FirstLevel f = create_first_level(pixels, pixel_bytes, width, height, stride_bytes, p, domain);
+  return rebuild_model(std::move(f.level), p, f.orow, f.ocol);
+}
+
+FirstLevel create_first_level(const void* pixels, int pixel_bytes, size_t width, size_t height, size_t stride_bytes,
+                              const CreateParams& p, const Region* domain) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
   if (!std::isfinite(p.start) || !std::isfinite(p.extent) || !std::isfinite(p.step) || p.extent < 0 || p.extent > kTau ||
       !std::isfinite(p.start + p.extent) || p.step < 0 || p.levels < 0 || p.levels > 16)
@@ -211,8 +230,8 @@ std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size
   const size_t bw = size_t(D.col_max - D.col_min + 1), bh = size_t(D.row_max - D.row_min + 1);
   if (bh > kMaxPixels / bw) raise(kLimitExceeded, "domain", "template bounding box exceeds one million pixels");
 
-  auto m = std::make_unique<NccModel>();
-  LevelGeom first;
+  FirstLevel out;
+  LevelGeom& first = out.level;
   first.w = bw; first.h = bh;
   first.gray.assign(bw * bh, 0.0);
   first.mask.assign(bw * bh, 0);
@@ -238,21 +257,23 @@ std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size
     }
   }
   const long double area = static_cast<long double>(D.area);
-  m->orow = double(sum_row / area);
-  m->ocol = double(sum_col / area);
+  out.orow = double(sum_row / area);
+  out.ocol = double(sum_col / area);
   first.orow = double((sum_row - static_cast<long double>(D.row_min) * area) / area);
   first.ocol = double((sum_col - static_cast<long double>(D.col_min) * area) / area);
   const double mean = double(total) / double(D.area);
   for (size_t i = 0; i < first.gray.size(); ++i)
     if (first.mask[i]) first.gray[i] -= mean;
-  return rebuild_model(std::move(first), p, m->orow, m->ocol);
+  return out;
 }
 
-std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& p, double origin_row, double origin_col) {
+ModelPlan plan_model(LevelGeom first, const CreateParams& p, double origin_row, double origin_col) {
   // rasters store sample coordinates as 16-bit pairs
   if (first.w > 16383 || first.h > 16383)
     raise(kLimitExceeded, "domain", "template side exceeds 16383 pixels");
-  auto m = std::make_unique<NccModel>();
+  ModelPlan plan;
+  plan.model = std::make_unique<NccModel>();
+  NccModel* m = plan.model.get();
   m->params = p;
   m->orow = origin_row;
   m->ocol = origin_col;
@@ -261,9 +282,10 @@ std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& p, 
   m->levels.push_back(std::move(first));
   // original patch: centred by the incremental mean, divided by the norm
   // (model.cpp:182-204); appended after the angle grid
-  double first_mean = 0, first_energy = 0;
-  std::vector<int32_t> orig_rc;
-  std::vector<double> orig_w;
+  double& first_mean = plan.first_mean;
+  double& first_energy = plan.first_energy;
+  std::vector<int32_t>& orig_rc = plan.orig_rc;
+  std::vector<double>& orig_w = plan.orig_w;
   {
     const LevelGeom& lv = m->levels.front();
     size_t cnt = 0;
@@ -302,20 +324,38 @@ std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& p, 
   if (!std::isfinite(intervals) || intervals >= double(kMaxAngles))
     raise(kLimitExceeded, "angle_step", "angle grid exceeds limit");
   m->n_angles = size_t(intervals) + 1;
+  return plan;
+}
+
+double raster_angle(const NccModel& m, size_t i) { return m.params.start + std::min(m.params.extent, i * m.params.step); }
+void raise_degenerate_raster() { raise(kInvalidArgument, "domain", "rotation produces a degenerate template"); }
+void raise_sample_budget() { raise(kLimitExceeded, "angle_extent", "model raster budget exceeded"); }
+size_t max_model_samples() { return kMaxSamples; }
+
+std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& p, double origin_row, double origin_col) {
+  ModelPlan plan = plan_model(std::move(first), p, origin_row, origin_col);
+  NccModel* m = plan.model.get();
   size_t total_samples = 0;
   for (size_t li = 0; li < m->levels.size(); ++li) {
     LevelGeom& lv = m->levels[li];
     lv.first_raster = uint32_t(m->pool.rasters.size());
     for (size_t i = 0; i < m->n_angles; ++i) {
-      const double angle = p.start + std::min(p.extent, i * m->params.step);
-      const int64_t id = li == 0 ? build_raster(lv, angle, m->pool, &first_mean, &first_energy)
+      const double angle = raster_angle(*m, i);
+      const int64_t id = li == 0 ? build_raster(lv, angle, m->pool, &plan.first_mean, &plan.first_energy)
                                  : build_raster(lv, angle, m->pool);
-      if (id < 0) raise(kInvalidArgument, "domain", "rotation produces a degenerate template");
+      if (id < 0) raise_degenerate_raster();
       const size_t n = m->pool.rasters[size_t(id)].count;
-      if (n > kMaxSamples - total_samples) raise(kLimitExceeded, "angle_extent", "model raster budget exceeded");
+      if (n > kMaxSamples - total_samples) raise_sample_budget();
       total_samples += n;
     }
   }
+  return finish_model(std::move(plan));
+}
+
+std::unique_ptr<NccModel> finish_model(ModelPlan plan) {
+  NccModel* m = plan.model.get();
+  const std::vector<int32_t>& orig_rc = plan.orig_rc;
+  const std::vector<double>& orig_w = plan.orig_w;
   Raster original;
   const LevelGeom& l0 = m->levels.front();
   original.orow = l0.orow;
@@ -340,7 +380,7 @@ std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& p, 
   // continuous score centres the image samples instead (matcher.cpp:146-156)
   m->pool.rasters.push_back(original);
   m->original = uint32_t(m->pool.rasters.size() - 1);
-  return m;
+  return std::move(plan.model);
 }
 
 }  // namespace mvamd

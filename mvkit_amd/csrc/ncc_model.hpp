@@ -109,10 +109,49 @@ std::unique_ptr<NccModel> create_model(const void* pixels, int pixel_bytes, size
                                        size_t stride_bytes, const CreateParams& params,
                                        const Region* domain = nullptr);
 
+// The part of create() before rebuild (model.cpp:120-166): the centred level 0
+// over the domain and the template centroid in image coordinates.
+struct FirstLevel {
+  LevelGeom level;
+  double orow = 0, ocol = 0;
+};
+FirstLevel create_first_level(const void* pixels, int pixel_bytes, size_t width, size_t height, size_t stride_bytes,
+                              const CreateParams& params, const Region* domain = nullptr);
+
 // rebuild(): model.cpp:173-242 from a centred level 0 with any mask and the
 // template centroid in image coordinates (persisted models).
 std::unique_ptr<NccModel> rebuild_model(LevelGeom first, const CreateParams& params, double origin_row,
                                         double origin_col);
+
+// rebuild() in parts, for builders of the rotated rasters elsewhere (the
+// device): plan_model does everything up to the rasters (pyramid, angle grid,
+// the original patch's samples), the caller appends level-major, angle-minor
+// rasters to model->pool (level.first_raster set) exactly as build_raster
+// would, and finish_model appends the original patch.
+struct ModelPlan {
+  std::unique_ptr<NccModel> model;
+  double first_mean = 0, first_energy = 0;  // level 0's (unrotated raster's) mean and energy
+  std::vector<int32_t> orig_rc;
+  std::vector<double> orig_w;
+};
+ModelPlan plan_model(LevelGeom first, const CreateParams& params, double origin_row, double origin_col);
+std::unique_ptr<NccModel> finish_model(ModelPlan plan);
+// Angle i of the model's grid (model.cpp:232-238).
+double raster_angle(const NccModel& m, size_t i);
+// The rebuild's raster errors, and its sample budget.
+[[noreturn]] void raise_degenerate_raster();
+[[noreturn]] void raise_sample_budget();
+size_t max_model_samples();
+
+// Sampling grid of one raster (model.cpp:66-98): the level grid for an
+// unrotated angle, else the rotated bounding box (raises LIMIT_EXCEEDED past
+// the rotated-box budget) with the angle's cosine and sine.
+struct RasterGrid {
+  bool unrotated = false;
+  double ca = 1, sa = 0;
+  int r_lo = 0, r_hi = -1, c_lo = 0, c_hi = -1;
+};
+RasterGrid raster_grid(const LevelGeom& level, double angle);
 
 // MVKIT_NCC 1 text model (persist.cpp:14-84).
 std::string serialize_model(const NccModel& model);
