@@ -188,6 +188,16 @@ def main():
         score_err[name] = {"max": float(max(errs)), "rms": float(np.sqrt(np.mean(np.square(errs)))),
                            "peaks": len(errs), "missing": len(ref) - len(errs)}
 
+    # model construction, host and device builders (identical models)
+    create_ms = {}
+    capi.NccModel(scenes[0].template, 4, START, EXTENT, builder=capi.CREATE_DEVICE)
+    for name, b in (("host", capi.CREATE_HOST), ("device", capi.CREATE_DEVICE)):
+        t = time.perf_counter()
+        for _ in range(3):
+            capi.NccModel(scenes[0].template, 4, START, EXTENT, builder=b)
+        create_ms[name] = round((time.perf_counter() - t) / 3 * 1e3, 2)
+    log(f"model create ms: {create_ms}")
+
     cpu = None
     if args.cpu_finds > 0:
         from oracle import pyoracle
@@ -223,6 +233,7 @@ def main():
         "fp32_form": fp32,
         "speedup_vs_fp32": round(mpx / fp32["value"], 3) if fp32 else None,
         "coarse_score_error_vs_fp64": score_err,
+        "model_create_ms": create_ms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
