@@ -371,6 +371,7 @@ struct FindContext::Slot {
   DeviceBuffer work;
   PinnedBuffer stage, readback;
   hipEvent_t done = nullptr;  // the device-resident part has finished
+  hipStream_t stream = nullptr;  // the device-resident part's stream (batches of both slots overlap)
   std::vector<Timer> timers;
   size_t timers_used = 0;
   int profiling = 0;          // stage mask
@@ -403,19 +404,28 @@ FindContext::FindContext() {
   check(hipGetDevice(&device_), "hipGetDevice");
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   check(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking), "hipStreamCreate");
-  for (auto& sl : slots_) {
+  check(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming), "hipEventCreate");
+  for (size_t i = 0; i < 2; ++i) {
+    auto& sl = slots_[i];
     sl = std::make_unique<Slot>();
+    check(hipStreamCreateWithFlags(&slot_streams_[i], hipStreamNonBlocking), "hipStreamCreate");
+    sl->stream = slot_streams_[i];
     check(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming), "hipEventCreate");
   }
 }
 FindContext::~FindContext() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (stream2_) (void)hipStreamSynchronize(stream2_);
+  for (hipStream_t q : slot_streams_)
+    if (q) (void)hipStreamSynchronize(q);
   for (auto& sl : slots_) {
     if (!sl) continue;
     for (auto& t : sl->timers) { (void)hipEventDestroy(t.begin); (void)hipEventDestroy(t.end); }
     if (sl->done) (void)hipEventDestroy(sl->done);
   }
+  if (uploaded_) (void)hipEventDestroy(uploaded_);
+  for (hipStream_t q : slot_streams_)
+    if (q) (void)hipStreamDestroy(q);
   if (stream2_) (void)hipStreamDestroy(stream2_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -524,6 +534,13 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
 void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inputs, size_t n_frames,
                           const FindOptions& req, FindTrace* traces) {
   const double t0 = now_us();
+  // the slot's stream; work queued on the context stream before (image
+  // uploads) completes first
+  hipStream_t qs = S.stream ? S.stream : stream_;
+  if (qs != stream_) {
+    check(hipEventRecord(uploaded_, stream_), "hipEventRecord");
+    check(hipStreamWaitEvent(qs, uploaded_, 0), "hipStreamWaitEvent");
+  }
   S.profiling = traces ? req.profile : 0;
   S.timers_used = 0;
   S.jobs.clear();
@@ -578,7 +595,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     if ((inputs[f].domain != nullptr) != reduced || (reduced && n_frames != 1))
       raise(kInvalidArgument, "image", "reduced search domains are supported for single-frame finds");
   if (reduced && inputs[0].domain->empty()) return finish_empty();
-  auto dmodel = device_model(model, stream_);
+  auto dmodel = device_model(model, qs);
 
   // ---- query patches per level (matcher.cpp:463-482), shared by the batch
   std::vector<std::vector<int32_t>> refs(level_count);
@@ -768,7 +785,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   const size_t off_frames = off_small + small; small = align_up(small + n_frames * sizeof(NccFrame));
   off = off_small + small;
   if (S.work.bytes < off) {
-    check(hipStreamSynchronize(stream_), "sync");
+    check(hipStreamSynchronize(qs), "sync");
     S.work = DeviceBuffer(align_up(off + off / 4, 1 << 20));
   }
   uint8_t* dev = S.work.as<uint8_t>();
@@ -838,7 +855,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       }
       fr[f] = d;
     }
-    check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, stream_), "upload");
+    check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, qs), "upload");
   }
   auto max_dims = [&](size_t l) {
     int w = 0, h = 0;
@@ -849,29 +866,29 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
 
   // ---- the device-resident part: pyramid, top-level statistics, coarse
   // search, peaks, tracking, full score; one synchronisation at the end
-  if (ncchip_fill_u32(reinterpret_cast<uint32_t*>(counts), n_count_words, 0u, stream_) ||
+  if (ncchip_fill_u32(reinterpret_cast<uint32_t*>(counts), n_count_words, 0u, qs) ||
       ncchip_fill_u32(reinterpret_cast<uint32_t*>(dev + off_table), std::max<size_t>(n_track, 1) * n_frames * table_size,
-                      0xffffffffu, stream_))
+                      0xffffffffu, qs))
     raise(kInternalError, "device", "fill launch failed");
-  if (reduced && ncchip_domain_sat(dframes, F, max_dims(0).second, max_dims(0).first, stream_))
+  if (reduced && ncchip_domain_sat(dframes, F, max_dims(0).second, max_dims(0).first, qs))
     raise(kInternalError, "device", "domain launch failed");
-  time_begin(kStagePyramid, stream_);
+  time_begin(kStagePyramid, qs);
   for (size_t l = 1; l < level_count; ++l) {
     const auto [w, h] = max_dims(l);
-    if (ncchip_pyramid_batch(dframes, F, int(l), jobs[0].lv[l - 1].kind, jobs[0].lv[l].kind, w, h, stream_))
+    if (ncchip_pyramid_batch(dframes, F, int(l), jobs[0].lv[l - 1].kind, jobs[0].lv[l].kind, w, h, qs))
       raise(kInternalError, "device", "pyramid launch failed");
   }
-  time_end(stream_);
+  time_end(qs);
   const auto [Wt_max, Ht_max] = max_dims(top);
-  time_begin(kStagePrefix, stream_);
-  if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, stream_))
+  time_begin(kStagePrefix, qs);
+  if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, qs))
     raise(kInternalError, "device", "prefix launch failed");
-  time_end(stream_);
+  time_end(qs);
   // centring of the coarse forms: mid-range of the pixel type at the top level
   const double C = std::ldexp(wide ? 32768.0 : 128.0, int(2 * top));
   const int exact_top = !wide && top <= 2;  // exact byte lattice (matcher.cpp:347, 561)
   const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
-  time_begin(kStageCoarse, stream_);
+  time_begin(kStageCoarse, qs);
   int rc = 0;
   if (use_mfma) {
     NcchipDenseWeights dw{};
@@ -884,28 +901,28 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     dw.prune_below = 0.9 * (o.min_score * .4);
     rc = ncchip_coarse_mfma(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
                             &dw, exact_top, model.params.absolute,
-                            o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, stream_);
+                            o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, qs);
   }
   else
     rc = ncchip_coarse(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
-                       max_bh, max_bw, o.form, exact_top, model.params.absolute, stream_);
+                       max_bh, max_bw, o.form, exact_top, model.params.absolute, qs);
   if (rc == -6) raise(kUnsupported, "form", "correlation form not available in this build");
   if (rc) raise(kInternalError, "device", "coarse launch failed");
-  time_end(stream_);
-  time_begin(kStagePeaks, stream_);
+  time_end(qs);
+  time_begin(kStagePeaks, qs);
   // reduced domain: anchors whose pose may not reach it score missing (matcher.cpp:556-578)
   if (reduced && ncchip_domain_planes(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), scale, phase,
-                                      use_mfma ? 1 : 0, stream_))
+                                      use_mfma ? 1 : 0, qs))
     raise(kInternalError, "device", "domain launch failed");
   if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
                    lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 1 : 0,
-                   stream_))
+                   qs))
     raise(kInternalError, "device", "peak launch failed");
   // find_models over several models converts the whole level 0 (matcher.cpp:596-611)
   // u16 frames keep the whole level 0 (matcher.cpp:612: only byte images are cut down)
-  if (!finest && !o.whole_level0 && !wide && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, stream_))
+  if (!finest && !o.whole_level0 && !wide && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, qs))
     raise(kInternalError, "device", "storage launch failed");
-  time_end(stream_);
+  time_end(qs);
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
   constexpr int kListGrid = 2048;
   S.track_mfma.assign(n_track, false);
@@ -923,6 +940,23 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       n_max = std::max(n_max, double(p.count));
     }
     const double tile_px = (std::ceil(hi_r - lo_r) + 7) * (std::ceil(hi_c - lo_c) + 7);
+    // the matrix-core kernel's tile: a candidate's eligible patches lie within
+    // tol of its angle, hence within 2 tol of each other; bound the union of
+    // every such group (candidates beyond the tile take the kernel's exact
+    // global-memory path)
+    int mt_h = 0, mt_w = 0;
+    for (const int32_t ri : refs[l - 1]) {
+      const Raster& pi = raster_of(ri);
+      double gr0 = 1e300, gr1 = -1e300, gc0 = 1e300, gc1 = -1e300;
+      for (const int32_t rj : refs[l - 1]) {
+        const Raster& pj = raster_of(rj);
+        if (std::abs(pj.angle - pi.angle) > 2 * tol) continue;
+        gr0 = std::min(gr0, pj.rmin - pj.orow); gr1 = std::max(gr1, pj.rmax - pj.orow);
+        gc0 = std::min(gc0, pj.cmin - pj.ocol); gc1 = std::max(gc1, pj.cmax - pj.ocol);
+      }
+      mt_h = std::max(mt_h, int(std::ceil(gr1 - gr0)) + 7);
+      mt_w = std::max(mt_w, int(std::ceil(gc1 - gc0)) + 7);
+    }
     const double xmax = (wide ? 65535. : 255.) * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
     // matrix cores where every query patch has a dense raster and at most 32
@@ -942,7 +976,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     }
     int rc_track = -6;
     if (mfma) {
-      time_begin(kStageTrackMfma, stream_);
+      time_begin(kStageTrackMfma, qs);
       rc_track = ncchip_track_mfma(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                                    reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                                    lists + k * cap_all, counts + k * n_frames, int(cap),
@@ -950,12 +984,11 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                                    counts + (k + 1) * n_frames,
                                    reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size,
                                    int(table_size), o.min_score * .4, tol, step, offset, !wide && l - 1 <= 2,
-                                   model.params.absolute, int(std::ceil(hi_r - lo_r)) + 7,
-                                   int(std::ceil(hi_c - lo_c)) + 7, xmax, work + k, kListGrid, stream_);
-      time_end(stream_);
+                                   model.params.absolute, mt_h, mt_w, xmax, work + k, kListGrid, qs);
+      time_end(qs);
       S.track_mfma[k] = rc_track == 0;
     }
-    if (rc_track == -6) time_begin(kStageTrack, stream_);
+    if (rc_track == -6) time_begin(kStageTrack, qs);
     if (rc_track == -6 &&
         ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                      reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
@@ -963,10 +996,10 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                      lists + (k + 1) * cap_all, counts + (k + 1) * n_frames,
                      reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size, int(table_size),
                      o.min_score * .4, tol, step, offset, !wide && l - 1 <= 2, model.params.absolute,
-                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, stream_))
+                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, qs))
       raise(kInternalError, "device", "tracking launch failed");
     if (rc_track != -6 && rc_track != 0) raise(kInternalError, "device", "tracking launch failed");
-    if (rc_track == -6) time_end(stream_);
+    if (rc_track == -6) time_end(qs);
   }
   // ---- full score at the finest level (matcher.cpp:813-822)
   NcchipRefineParams rp{};
@@ -990,17 +1023,17 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   // the domain (matcher.cpp:631-645); duplicates it makes are merged after the
   // full score, which only depends on the pose
   if (reduced && o.subpixel && finest == 0 && top >= 1 &&
-      ncchip_domain_snap(dframes, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), stream_))
+      ncchip_domain_snap(dframes, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), qs))
     raise(kInternalError, "device", "domain launch failed");
-  time_begin(kStageFull, stream_);
+  time_begin(kStageFull, qs);
   if (ncchip_full_score(&rp, pools, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), hfull, 1024,
-                        stream_))
+                        qs))
     raise(kInternalError, "device", "full score launch failed");
-  time_end(stream_);
+  time_end(qs);
   if (ncchip_copy_u32(reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<uint32_t*>(hcounts), n_count_words,
-                      stream_))
+                      qs))
     raise(kInternalError, "device", "copy launch failed");
-  check(hipEventRecord(S.done, stream_), "hipEventRecord");
+  check(hipEventRecord(S.done, qs), "hipEventRecord");
   S.model = &model; S.dmodel = dmodel; S.o = o; S.traces = traces; S.n_frames = n_frames; S.empty = false;
   S.jobs = std::move(jobs); S.qpool = std::move(qpool);
   S.cap = cap; S.cap_all = cap_all; S.n_track = n_track; S.top = top; S.finest = finest; S.stride = stride;
@@ -1322,6 +1355,7 @@ std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, 
   }
   while (multi_slots_.size() < n_queries) {
     multi_slots_.push_back(std::make_unique<Slot>());
+    multi_slots_.back()->stream = stream_;
     check(hipEventCreateWithFlags(&multi_slots_.back()->done, hipEventDisableTiming), "hipEventCreate");
   }
   // ---- the device-resident parts of every query, back to back on one stream

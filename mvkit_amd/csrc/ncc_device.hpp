@@ -193,6 +193,8 @@ class FindContext {
   uint64_t next_ticket_ = 0;
   hipStream_t stream_ = nullptr;   // device-resident part
   hipStream_t stream2_ = nullptr;  // refinement rounds
+  hipStream_t slot_streams_[2] = {nullptr, nullptr};  // device-resident part of each batch slot
+  hipEvent_t uploaded_ = nullptr;  // orders the context stream's uploads before a slot's stream
   int device_ = 0;
   DeviceBuffer image_;
   DeviceBuffer create_;  // model construction workspace (grow-only)

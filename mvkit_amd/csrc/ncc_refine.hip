@@ -1661,7 +1661,9 @@ extern "C" int ncchip_track_mfma(const NccFrame* frames, int n_frames, int level
   a.mt_p1 = int(o); o = al16(o + size_t(tile_h) * (tile_w + 1) * 4);
   a.mt_p2 = int(o); o = al16(o + size_t(tile_h) * (tile_w + 1) * (p2_32 ? 4 : 8));
   a.mt_pl = int(o); o = al16(o + std::max<size_t>(size_t(2) * tile_h * a.mt_pitch * 2, 2 * 32 * 32 * 4));
-  if (o > 64 * 1024) return -6;  // at least two workgroups per CU
+  // at least two workgroups per CU: a larger neighbourhood (one workgroup
+  // per CU) tracks slower than the FP64 vector kernel
+  if (o > 64 * 1024) return -6;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (p2_32) hipLaunchKernelGGL((k_track_mfma<uint16_t, uint32_t>), dim3(grid), dim3(kTrackThreads), o, st, a);
   else hipLaunchKernelGGL((k_track_mfma<uint16_t, uint64_t>), dim3(grid), dim3(kTrackThreads), o, st, a);
