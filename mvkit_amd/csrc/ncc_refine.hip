@@ -206,7 +206,7 @@ __device__ __forceinline__ void track_row(const TrackArgs& a, const PatchRef& p,
   while (!in[kmin]) ++kmin;
   while (!in[kmax]) --kmax;
   const int32_t Di = int32_t(a.D);
-  constexpr int U = 4;
+  constexpr int U = 8;
   // Branch-free over the U samples of a chunk: samples past the end re-read
   // the chunk's first sample with x forced to 0, so all 5U loads issue before
   // the first wait.  Weights are 0 when the dot product uses weight runs.
