@@ -85,6 +85,10 @@ typedef struct NcchipDenseWeights {
   const int32_t* common_runs;
   int32_t n_common_runs, n_common;
   double prune_below;
+  /* > -inf: sparse planes -- only scores >= peak_thr are written, each with
+   * its bit in the frame's peak_bits (zeroed by the caller); the peaks
+   * kernel then runs with hwa = 2.  -inf: every plane entry is written. */
+  double peak_thr;
 } NcchipDenseWeights;
 
 /* Split-BF16 MFMA coarse search (the planes of ncchip_coarse, stored in
@@ -108,6 +112,7 @@ int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int leve
  * bit patterns; the caller initialises them to h, 0, w, 0) feeds
  * ncchip_storage0.  hwa != 0: the planes are in [row][col][angle] order
  * (as ncchip_coarse_mfma writes them), else [angle][row][col]. */
+/* hwa = 2: the sparse planes of the split-BF16 form (peak_bits set). */
 int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int max_w, int max_h, const NccPatchPool* pools,
                  const int32_t* patch_ids, int n_angles, double threshold, double scale, double phase, NccMatch* cand,
                  int32_t* key, int cap, int32_t* count, unsigned long long* extent, int hwa, void* stream);

@@ -83,7 +83,11 @@ typedef struct NccFrame {
   uint64_t* p1;              // top-level row prefix of S, (w+1) per row
   uint64_t* p2;              // top-level row prefix of S^2
   uint32_t* nz;              // top-level row prefix of [S != D]
-  double* planes;            // coarse score planes [angle][h][w]
+  double* planes;            // coarse score planes [angle][h][w] (split-BF16 form: [h][w][angle])
+  // split-BF16 form: per top-level position, ceil(angles / 32) words with a
+  // bit per angle scoring at least the peak threshold; only those scores are
+  // written to the planes (null: every plane entry is written)
+  uint32_t* peak_bits;
   const int32_t* ranges;     // per coarse angle: valid anchor range {rb, re, cb, ce}
   // Reduced search domain (image.hpp:108-113); dom_sat null: the full image.
   // dom_sat: summed-area table of the domain mask over the crop, (level-0

@@ -309,6 +309,113 @@ __global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
   }
 }
 
+// Peaks of the sparse planes of the split-BF16 form: only scores at or above
+// the threshold are written, each with its bit in the frame's peak_bits
+// (per position, one word per 32 angles).  Windows below the threshold can
+// neither be peaks nor suppress one, so the 3x3 rule only needs the flagged
+// neighbours.  One thread per column, the rows of a block in turn; a lane
+// may emit several angles of its position.  Most positions hold no flag, so
+// a block takes one row: many short blocks finish soonest.
+constexpr int kPeakBitRows = 1;
+__global__ void __launch_bounds__(256) k_peaks_bits(const PeakArgs a) {
+  const int f = blockIdx.z;
+  const NccFrame& F = a.frames[f];
+  const int H = F.levels[a.level].height, W = F.levels[a.level].width;
+  if (int(blockIdx.x) * 256 >= W) return;  // uniform per block
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int nw = (a.n_angles + 31) / 32;
+  const size_t A = size_t(a.n_angles);
+  const int r0 = blockIdx.y * kPeakBitRows;
+  const int r_end = min(H, r0 + kPeakBitRows);
+  for (int w = 0; w < nw; ++w) {
+    uint32_t words[kPeakBitRows];
+#pragma unroll
+    for (int i = 0; i < kPeakBitRows; ++i)
+      words[i] = (c < W && r0 + i < r_end) ? F.peak_bits[(size_t(r0 + i) * W + c) * nw + w] : 0u;
+#pragma unroll
+    for (int i = 0; i < kPeakBitRows; ++i) {
+      const int r = r0 + i;
+      uint32_t peaks = 0;
+      const size_t pos = size_t(r) * W + c;
+      {
+        const uint32_t bits = words[i];
+        if (bits) {
+          uint32_t nb[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const int rr = r + k / 3 - 1, cc = c + k % 3 - 1;
+            nb[k] = (k != 4 && rr >= 0 && cc >= 0 && rr < H && cc < W) ? F.peak_bits[(size_t(rr) * W + cc) * nw + w] : 0u;
+          }
+          uint32_t pend = bits;
+          while (pend) {
+            const int s = __builtin_ctz(pend);
+            pend &= pend - 1;
+            const double v = F.planes[pos * A + size_t(w) * 32 + s];
+            bool peak = true;
+            for (int k = 0; k < 9 && peak; ++k) {
+              if (!((nb[k] >> s) & 1u)) continue;
+              const int dr = k / 3 - 1, dc = k % 3 - 1;
+              const double o = F.planes[(size_t(r + dr) * W + (c + dc)) * A + size_t(w) * 32 + s];
+              if (o > v || (o == v && (dr < 0 || (!dr && dc < 0)))) peak = false;
+            }
+            if (peak) peaks |= 1u << s;
+          }
+        }
+      }
+      // emission: a wave-wide exclusive scan of the peak counts
+      const int cnt = __popc(peaks);
+      int incl = cnt;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+      }
+      const int total = __shfl(incl, 63, 64);
+      if (!total) continue;  // wave-uniform
+      int base = 0;
+      if (lane == 63) base = atomicAdd(a.count + f, total);
+      base = __shfl(base, 63, 64);
+      int k = base + incl - cnt;
+      double orow = __builtin_inf(), ocol = __builtin_inf(), xrow = 0, xcol = 0;
+      uint32_t pm = peaks;
+      while (pm) {
+        const int s = __builtin_ctz(pm);
+        pm &= pm - 1;
+        const int ai = w * 32 + s;
+        const NccPatch& P = a.pools.patch(a.patch_ids[ai]);
+        NccMatch q;
+        const double dr = a.scale * (r + P.orow) + a.phase, dcol = a.scale * (c + P.ocol) + a.phase;
+        q.row = F.crop_r + dr;
+        q.col = F.crop_c + dcol;
+        q.angle = P.angle;
+        q.score = F.planes[pos * A + size_t(ai)];
+        if (k < a.cap) {
+          a.cand[size_t(f) * a.cap + k] = q;
+          a.key[size_t(f) * a.cap + k] = ai;
+          orow = fmin(orow, q.row - F.crop_r); xrow = fmax(xrow, q.row - F.crop_r);
+          ocol = fmin(ocol, q.col - F.crop_c); xcol = fmax(xcol, q.col - F.crop_c);
+        }
+        ++k;
+      }
+#pragma unroll
+      for (int s = 32; s > 0; s >>= 1) {
+        orow = fmin(orow, __shfl_xor(orow, s, 64));
+        ocol = fmin(ocol, __shfl_xor(ocol, s, 64));
+        xrow = fmax(xrow, __shfl_xor(xrow, s, 64));
+        xcol = fmax(xcol, __shfl_xor(xcol, s, 64));
+      }
+      if (lane == 0 && isfinite(orow)) {
+        unsigned long long* e = a.extent + 4 * size_t(f);
+        atomicMin(e + 0, __double_as_longlong(orow));
+        atomicMax(e + 1, __double_as_longlong(xrow));
+        atomicMin(e + 2, __double_as_longlong(ocol));
+        atomicMax(e + 3, __double_as_longlong(xcol));
+      }
+    }
+  }
+}
+
 // Level-0 storage rectangle of every frame from its candidate extent
 // (matcher.cpp:612-629), written into the frame descriptors.  Frames without
 // candidates keep the full level.
@@ -360,6 +467,13 @@ __global__ void __launch_bounds__(256) k_domain_planes(const DomainPlaneArgs a) 
     const double row = F.crop_r + a.scale * (r + P.orow) + a.phase;
     const double col = F.crop_c + a.scale * (c + P.ocol) + a.phase;
     if (may_contain(F, row, col, a.scale)) continue;
+    if (a.hwa == 2) {
+      // sparse planes: drop the angle's peak bit
+      uint32_t* word = F.peak_bits + (size_t(r) * W + c) * size_t((a.n_angles + 31) / 32) + ai / 32;
+      const uint32_t bit = 1u << (ai % 32);
+      if (*word & bit) atomicAnd(word, ~bit);
+      continue;
+    }
     const size_t idx = a.hwa ? (size_t(r) * W + c) * a.n_angles + ai : (size_t(ai) * H + r) * W + c;
     F.planes[idx] = NCC_MISSING;
   }
@@ -421,6 +535,11 @@ extern "C" int ncchip_peaks(const NccFrame* frames, int n_frames, int level, int
   a.frames = frames; a.level = level; a.n_angles = n_angles; a.pools.p0 = pools[0]; a.pools.p1 = pools[1];
   a.patch_ids = patch_ids; a.thr = threshold; a.scale = scale; a.phase = phase; a.cand = cand; a.key = key;
   a.cap = cap; a.count = count; a.extent = extent; a.hwa = hwa;
+  if (hwa == 2) {
+    const dim3 grid((max_w + 255) / 256, (max_h + kPeakBitRows - 1) / kPeakBitRows, unsigned(n_frames));
+    hipLaunchKernelGGL(k_peaks_bits, grid, dim3(256), 0, static_cast<hipStream_t>(stream), a);
+    return hipGetLastError() == hipSuccess ? 0 : -1000;
+  }
   const dim3 grid(hwa ? (max_w * n_angles + 255) / 256 : (max_w + 255) / 256, (max_h + kPeakRows - 1) / kPeakRows,
                   unsigned(hwa ? n_frames : n_frames * n_angles));
   hipLaunchKernelGGL(k_peaks, grid, dim3(256), 0, static_cast<hipStream_t>(stream), a);

@@ -354,6 +354,7 @@ struct Job {
   int64_t box[4] = {0, 0, -1, -1};  // search box {row_min, col_min, row_max, col_max} (matcher.cpp:376-383)
   // reduced domain: summed-area table, runs (int4) and per-crop-row run index
   size_t off_sat = 0, off_runs = 0, off_row = 0;
+  size_t off_bits = 0;  // split-BF16 form: per top-level position, a bit per coarse angle at or above the threshold
   std::vector<int32_t> dom_runs, dom_row;
   std::vector<Pose> cand, full, refined, results;
   std::vector<int> verdict;
@@ -757,6 +758,15 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     J.off_planes = off; off = align_up(off + A * size_t(Ht) * Wt * 8);
     if (J.in.domain) { J.off_sat = off; off = align_up(off + (size_t(J.lv[0].h) + 1) * (size_t(J.lv[0].w) + 1) * 4); }
   }
+  // the peak bits of every frame in one region (one fill clears them)
+  const size_t bit_words = (A + 31) / 32;
+  const size_t off_bits = off;
+  for (auto& J : jobs) {
+    J.off_bits = off;
+    if (use_mfma) off += size_t(J.lv[top].h) * size_t(J.lv[top].w) * bit_words * 4;
+  }
+  const size_t n_bit_words = (off - off_bits) / 4;
+  off = align_up(off);
   const size_t off_lists = off; off = align_up(off + (n_track + 1) * cap_all * sizeof(NccMatch));
   const size_t off_raw = off; off = align_up(off + cap_all * sizeof(NccMatch));
   const size_t off_key = off; off = align_up(off + cap_all * 4);
@@ -845,6 +855,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
       d.p2 = reinterpret_cast<uint64_t*>(dev + J.off_p2);
       d.nz = reinterpret_cast<uint32_t*>(dev + J.off_nz);
       d.planes = reinterpret_cast<double*>(dev + J.off_planes);
+      d.peak_bits = use_mfma ? reinterpret_cast<uint32_t*>(dev + J.off_bits) : nullptr;
       d.ranges = reinterpret_cast<const int32_t*>(dev + J.off_ranges);
       if (const Region* D = J.in.domain) {
         d.dom_sat = reinterpret_cast<uint32_t*>(dev + J.off_sat);
@@ -867,6 +878,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   // ---- the device-resident part: pyramid, top-level statistics, coarse
   // search, peaks, tracking, full score; one synchronisation at the end
   if (ncchip_fill_u32(reinterpret_cast<uint32_t*>(counts), n_count_words, 0u, qs) ||
+      ncchip_fill_u32(reinterpret_cast<uint32_t*>(dev + off_bits), n_bit_words, 0u, qs) ||
       ncchip_fill_u32(reinterpret_cast<uint32_t*>(dev + off_table), std::max<size_t>(n_track, 1) * n_frames * table_size,
                       0xffffffffu, qs))
     raise(kInternalError, "device", "fill launch failed");
@@ -899,6 +911,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     dw.n_common_runs = int32_t(dense.common_runs.size() / 4);
     dw.n_common = dense.n_common;
     dw.prune_below = 0.9 * (o.min_score * .4);
+    dw.peak_thr = o.min_score * .4;  // sparse planes: the peaks kernel reads the flagged scores only
     rc = ncchip_coarse_mfma(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
                             &dw, exact_top, model.params.absolute,
                             o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, qs);
@@ -912,10 +925,10 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   time_begin(kStagePeaks, qs);
   // reduced domain: anchors whose pose may not reach it score missing (matcher.cpp:556-578)
   if (reduced && ncchip_domain_planes(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), scale, phase,
-                                      use_mfma ? 1 : 0, qs))
+                                      use_mfma ? 2 : 0, qs))
     raise(kInternalError, "device", "domain launch failed");
   if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
-                   lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 1 : 0,
+                   lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 2 : 0,
                    qs))
     raise(kInternalError, "device", "peak launch failed");
   // find_models over several models converts the whole level 0 (matcher.cpp:596-611)

@@ -51,6 +51,7 @@ struct MfmaArgs {
   const int4* common_runs;   // support shared by every coarse patch
   int n_common_runs, n_common;
   double prune_below;        // score bound under which windows are written -inf
+  double peak_thr;           // sparse planes: scores below are not written (-inf: dense planes)
 };
 
 // Calls fn(integral_constant<int, i>) for i = 0 .. N-1 (compile-time indices).
@@ -215,6 +216,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   __shared__ int s_code[kThreads / 64][64];
   __shared__ float s_dot[kThreads / 64][64];
   int queued = 0;  // wave-uniform
+  const bool sparse = args.peak_thr > -__builtin_inf();
   const double dc = double(frame_dc(F));
   const int64_t D = int64_t(frame_dc(F)) << shift;
   const int64_t Cl = Ci;
@@ -247,7 +249,13 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
       const int64_t sum_s2 = m2 + 2 * Cl * m1 + nq * Cl * Cl;
       const double score = normalise(double(dv), args.C, args.wsum_eff[aa], Pq.wsum, Pq.n, sum_s, sum_s2, D, shift,
                                      dc, args.exact_lattice, args.absolute);
-      F.planes[(size_t(r0 + r) * W + c0 + wc * kWaveCols + col) * args.n_angles + aa] = score;
+      const size_t pos = size_t(r0 + r) * W + c0 + wc * kWaveCols + col;
+      if (!sparse) {
+        F.planes[pos * args.n_angles + aa] = score;
+      } else if (score >= args.peak_thr) {
+        F.planes[pos * args.n_angles + aa] = score;
+        atomicOr(F.peak_bits + pos * args.n_tiles + tile, 1u << slot);
+      }
     }
     __builtin_amdgcn_wave_barrier();
     queued = 0;
@@ -283,7 +291,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
         const double isd = s_inv_sd[wave][r][cw + j];
         keep = isd < 0 || (fabs(double(dot[j])) + num_slack) * isd >= args.prune_below;
       }
-      if (exists && !keep) plane[(size_t(row) * W + c) * args.n_angles] = NCC_MISSING;
+      if (exists && !keep && !sparse) plane[(size_t(row) * W + c) * args.n_angles] = NCC_MISSING;
       const uint64_t mask = __ballot(keep);
       const int cnt = __popcll(mask);
       if (cnt) {
@@ -318,7 +326,7 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   a.wsum_eff = dw->wsum_eff; a.ymin = dw->ymin; a.xmin = dw->xmin; a.yspan = dw->yspan; a.xc = dw->xchunks;
   a.exact_lattice = exact_lattice; a.absolute = absolute;
   a.common_runs = reinterpret_cast<const int4*>(dw->common_runs); a.n_common_runs = dw->n_common_runs;
-  a.n_common = dw->n_common; a.prune_below = dw->prune_below;
+  a.n_common = dw->n_common; a.prune_below = dw->prune_below; a.peak_thr = dw->peak_thr;
   if (size_t(n_frames) * a.n_tiles > 65535) return -1;
   const int cols = kBlockCols + 16 * a.xc + 1;
   a.pitch = (cols + 1) & ~1;
