@@ -45,6 +45,10 @@ int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const 
 int ncchip_pyramid_batch(const NccFrame* frames, int n_frames, int level, int src_kind, int dst_kind, int max_w,
                          int max_h, void* stream);
 
+/* Levels 1..n_levels (n_levels <= 3) of every frame from a u8 level 0 in
+ * one launch (u16 block sums, as ncchip_pyramid_batch level by level). */
+int ncchip_pyramid3_u8(const NccFrame* frames, int n_frames, int n_levels, int max_w, int max_h, void* stream);
+
 /* Row prefixes of `level` (p1, p2, nz of each descriptor).
  * Replaces matcher.cpp:487-495. */
 int ncchip_row_prefix_batch(const NccFrame* frames, int n_frames, int level, int kind, int max_h, void* stream);

@@ -41,7 +41,8 @@ PUBLIC_SYMBOLS = [
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
                   "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma",
                   "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine",
-                  "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap", "ncchip_create_rasters"]
+                  "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap", "ncchip_create_rasters",
+                  "ncchip_pyramid3_u8"]
 
 
 class StringView(C.Structure):

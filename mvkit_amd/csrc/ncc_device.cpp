@@ -885,7 +885,14 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   if (reduced && ncchip_domain_sat(dframes, F, max_dims(0).second, max_dims(0).first, qs))
     raise(kInternalError, "device", "domain launch failed");
   time_begin(kStagePyramid, qs);
-  for (size_t l = 1; l < level_count; ++l) {
+  // levels 1-3 of byte frames in one pass, the others level by level
+  size_t fused = 0;
+  if (!wide && level_count > 1) {
+    fused = std::min<size_t>(3, level_count - 1);
+    const auto [w0, h0] = max_dims(0);
+    if (ncchip_pyramid3_u8(dframes, F, int(fused), w0, h0, qs)) raise(kInternalError, "device", "pyramid launch failed");
+  }
+  for (size_t l = 1 + fused; l < level_count; ++l) {
     const auto [w, h] = max_dims(l);
     if (ncchip_pyramid_batch(dframes, F, int(l), jobs[0].lv[l - 1].kind, jobs[0].lv[l].kind, w, h, qs))
       raise(kInternalError, "device", "pyramid launch failed");

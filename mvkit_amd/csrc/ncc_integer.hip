@@ -147,6 +147,72 @@ __global__ void __launch_bounds__(256) k_support_cols(const uint32_t* __restrict
   }
 }
 
+// Levels 1..n (n <= 3) of a u8 level 0 in one pass: a block stages 8 rows x
+// 1024 columns of level 0 in LDS and derives the 4 x 512, 2 x 256 and
+// 1 x 128 block sums below it with the per-level edge clamp (a clamped source
+// row or column stays inside the block's tile, as the tiles are aligned to
+// the 2x2 lattice of every level), writing each level once.
+constexpr int kFuseRows = 8, kFuseCols = 1024;
+__global__ void __launch_bounds__(256) k_pyramid3_u8(const NccFrame* __restrict__ frames, int n_levels) {
+  __shared__ uint16_t t0[kFuseRows][kFuseCols];       // level 0 (u8 values)
+  __shared__ uint16_t t1[kFuseRows / 2][kFuseCols / 2];
+  __shared__ uint16_t t2[kFuseRows / 4][kFuseCols / 4];
+  const NccFrame& F = frames[blockIdx.z];
+  const NccLevel& L0 = F.levels[0];
+  const int R0 = blockIdx.y * kFuseRows, C0 = blockIdx.x * kFuseCols;
+  if (R0 >= L0.height || C0 >= L0.width) return;  // uniform per block
+  const uint8_t* src = static_cast<const uint8_t*>(L0.data);
+  // level 0: 4 consecutive bytes per thread and row, one dword when the
+  // rows are dword-aligned (all rows' loads issue before the LDS writes)
+  const int c = 4 * threadIdx.x, gc = C0 + c;
+  const bool words = ((reinterpret_cast<uintptr_t>(src) | uintptr_t(L0.pitch)) & 3) == 0 && gc + 3 < L0.width;
+  uint32_t v[kFuseRows];
+#pragma unroll
+  for (int r = 0; r < kFuseRows; ++r) {
+    const int gr = R0 + r;
+    v[r] = 0;
+    if (gr < L0.height) {
+      const uint8_t* p = src + size_t(gr) * L0.pitch + gc;
+      if (words) v[r] = *reinterpret_cast<const uint32_t*>(p);
+      else
+        for (int j = 0; j < 4; ++j) v[r] |= uint32_t(gc + j < L0.width ? p[j] : 0) << (8 * j);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kFuseRows; ++r)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t0[r][c + j] = uint16_t((v[r] >> (8 * j)) & 0xff);
+  __syncthreads();
+  // level l + 1 from level l: out (r, c) = sum of rows 2r, min(2r + 1, h - 1)
+  // and columns 2c, min(2c + 1, w - 1) of level l (gray_pyramid's clamp);
+  // the result also goes to the LDS tile of the next level when there is one
+  auto step = [&](const uint16_t* in, int in_pitch, int in_r0, int in_c0, int in_h, int in_w, uint16_t* out,
+                  int out_pitch, const NccLevel& D, int rows, int cols) {
+    uint16_t* dst = static_cast<uint16_t*>(const_cast<void*>(D.data));
+    for (int i = threadIdx.x; i < rows * cols; i += 256) {
+      const int r = i / cols, c = i % cols;
+      const int gr = in_r0 / 2 + r, gc = in_c0 / 2 + c;
+      if (gr >= D.height || gc >= D.width) continue;
+      const int y0 = 2 * gr - in_r0, y1 = min(2 * gr + 1, in_h - 1) - in_r0;
+      const int x0 = 2 * gc - in_c0, x1 = min(2 * gc + 1, in_w - 1) - in_c0;
+      const uint32_t v = uint32_t(in[y0 * in_pitch + x0]) + in[y0 * in_pitch + x1] + in[y1 * in_pitch + x0] +
+                         in[y1 * in_pitch + x1];
+      if (out) out[r * out_pitch + c] = uint16_t(v);
+      dst[size_t(gr) * D.pitch + gc] = uint16_t(v);
+    }
+  };
+  step(&t0[0][0], kFuseCols, R0, C0, L0.height, L0.width, n_levels > 1 ? &t1[0][0] : nullptr, kFuseCols / 2,
+       F.levels[1], kFuseRows / 2, kFuseCols / 2);
+  if (n_levels < 2) return;
+  __syncthreads();
+  step(&t1[0][0], kFuseCols / 2, R0 / 2, C0 / 2, F.levels[1].height, F.levels[1].width,
+       n_levels > 2 ? &t2[0][0] : nullptr, kFuseCols / 4, F.levels[2], kFuseRows / 4, kFuseCols / 4);
+  if (n_levels < 3) return;
+  __syncthreads();
+  step(&t2[0][0], kFuseCols / 4, R0 / 4, C0 / 4, F.levels[2].height, F.levels[2].width, nullptr, 0, F.levels[3],
+       kFuseRows / 8, kFuseCols / 8);
+}
+
 // Row pass of a frame's domain summed-area table: entry (r + 1, c + 1) = the
 // domain pixels of crop row r in columns 0..c (one block per crop row,
 // blockIdx.y = frame; frames without a reduced domain return at once).
@@ -215,6 +281,15 @@ __global__ void __launch_bounds__(256) k_domain_cols(const NccFrame* __restrict_
 }  // namespace ncc
 
 using namespace ncc;
+
+extern "C" int ncchip_pyramid3_u8(const NccFrame* frames, int n_frames, int n_levels, int max_w, int max_h,
+                                  void* stream) {
+  if (n_frames <= 0 || n_levels <= 0) return 0;
+  if (n_frames > 65535 || n_levels > 3) return -1;
+  const dim3 grid((max_w + kFuseCols - 1) / kFuseCols, (max_h + kFuseRows - 1) / kFuseRows, unsigned(n_frames));
+  hipLaunchKernelGGL(k_pyramid3_u8, grid, dim3(256), 0, static_cast<hipStream_t>(stream), frames, n_levels);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
 
 extern "C" int ncchip_domain_sat(const NccFrame* frames, int n_frames, int max_h, int max_w, void* stream) {
   if (n_frames <= 0 || n_frames > 65535 || max_h <= 0 || max_w <= 0) return n_frames ? -1 : 0;

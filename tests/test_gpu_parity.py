@@ -379,3 +379,21 @@ def test_matrix_tracking_equals_vector_tracking(capi, request, case):
     assert np.array_equal(np.isfinite(fast["prepared"][:, 3]), fin)
     assert np.array_equal(fast["prepared"][fin, 3], slow["prepared"][fin, 3])
     assert np.array_equal(fast["results"], slow["results"])
+
+
+@pytest.mark.parametrize("shape", [(187, 251), (129, 255), (191, 131)])
+def test_odd_sized_images_match_oracle(capi, oracle, scene_small, shape):
+    """Odd image sizes: every pyramid level drops its last odd row and column."""
+    h, w = shape
+    img = np.ascontiguousarray(scene_small.image[:h, :w])
+    start, extent = -math.radians(20), math.radians(40)
+    say(f"odd-sized find {shape}")
+    m = capi.NccModel(scene_small.template, 0, start, extent)
+    o = oracle.Model(scene_small.template, levels=0, start=start, extent=extent)
+    want = capi.halcon_shift(o.find(img, start, extent, 0.5, 0, 0.5))
+    got = m.find(img, start, extent, 0.5, 0, 0.5, "true", 0)
+    check_matches(got, want, f"odd {shape}")
+    opts = capi.NccModel.options(start, extent, 0.5, 0, 0.5, form=capi.FORM_BF16X3)
+    d = torch.from_numpy(img).cuda()
+    got2, _ = m.find_device(d.data_ptr(), w, h, w, opts)
+    check_matches(capi.halcon_shift(got2) if len(got2) else got2, want, f"odd bf16 {shape}")
