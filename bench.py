@@ -46,6 +46,22 @@ def log(msg):
 HIP_KERNELS = {("track_mfma", f): "ncc::k_track_mfma" for f in range(4)}
 HIP_KERNELS.update({("coarse", 0): "ncc::k_coarse_fp32", ("coarse", 1): "ncc::k_coarse_mfma",
                     ("coarse", 2): "ncc::k_coarse_fp64"})
+HIP_KERNELS.update({("track", f): "ncc::k_track" for f in range(4)})
+HIP_KERNELS.update({("refine", f): "ncc::k_refine" for f in range(4)})
+TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       "ncc_find_rot2048_pm30_b16_hbm_traffic.md")
+
+
+def hbm_bytes_per_frame(hip_kernel):
+    """Calibrated FETCH_SIZE + WRITE_SIZE bytes per frame of a kernel on this
+    workload, from the committed rocprofv3 counter summary (None if absent)."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    for line in open(TRAFFIC):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].strip("`").split("<")[0] == hip_kernel:
+            return float(cells[4])
+    return None
 
 
 def main():
@@ -58,6 +74,9 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=20,
                     help="timed steps of the plain FP32 form for the comparison (0: skip)")
     ap.add_argument("--cpu-finds", type=int, default=24, help="finds in the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 64 MiB device-to-device copies (the reference kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the NCC find path shards by frame, and only the single-GPU "
@@ -69,6 +88,16 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = 64 << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda").random_(0, 255)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {n} bytes three times")
+        return
     F = max(1, args.inflight)
     log(f"generating {F} frames")
     scenes = [synth.workload_rot_2048(frame=k) for k in range(F)]
@@ -161,6 +190,9 @@ def main():
     else:
         bound, peak = "latency", None
     total_kernel = sum(kernel.values())
+    # HBM bytes of one launch of the dominant kernel (the batch's frames)
+    per_frame_bytes = hbm_bytes_per_frame(HIP_KERNELS.get((dominant, form), ""))
+    traffic = round(per_frame_bytes * F) if per_frame_bytes is not None else None
 
     # the plain FP32 form on the same frames, for the split-BF16 comparison
     fp32 = None
@@ -226,7 +258,8 @@ def main():
         "roofline": {"kernel": dominant, "hip_kernel": HIP_KERNELS.get((dominant, form), dominant),
                      "us_per_launch": round(per_frame_us * F, 2),  # one launch of the stage per batch
                      "bound": bound, "achieved": round(achieved, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 5) if peak else None, "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 5) if peak else None,
+                     "traffic": traffic,
                      "us_per_frame": round(per_frame_us, 2), "algorithmic_gflop_per_frame": round(per_frame_flop / 1e9, 4)},
         "kernel_us_per_frame": {k: round(v, 2) for k, v in kernel.items()},
         "dominant_kernel": {"stage": dominant, "share": round(kernel[dominant] / total_kernel, 3) if total_kernel else 0},
