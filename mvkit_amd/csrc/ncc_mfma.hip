@@ -94,33 +94,36 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   const int shift = lv.shift;
   const int lo_mask = (1 << shift) - 1;
   // ---- stage the tile: image rows r0 + ymin .., columns c0 + xmin ..
-  // (U loads in flight per thread before the first use)
+  // Wave w stages rows w, w + 4, ..; its lanes stride the columns, kU
+  // loads in flight per lane before the first store.
   {
-    constexpr int U = 8;
+    constexpr int kU = 4;
     const int tr0 = r0 + args.ymin, tc0 = c0 + args.xmin;
     const Tpix* img = static_cast<const Tpix*>(lv.data);
-    const int total = rows * cols;
-    for (int base = threadIdx.x; base < total; base += kThreads * U) {
-      int Sv[U];
+    const int lane = threadIdx.x & 63;
+    for (int tr = threadIdx.x >> 6; tr < rows; tr += kThreads / 64) {
+      const int gr = tr0 + tr;
+      const bool row_ok = gr >= 0 && gr < H;
+      const Tpix* src = img + size_t(row_ok ? gr : 0) * lv.pitch;
+      for (int cb = lane; cb < cols; cb += 64 * kU) {
+        int Sv[kU];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + kThreads * u;
-        const int tr = i / cols, tc = i % cols;
-        const int gr = tr0 + tr, gc = tc0 + tc;
-        Sv[u] = (i < total && gr >= 0 && gr < H && gc >= 0 && gc < W) ? int(img[size_t(gr) * lv.pitch + gc]) : Ci;
-      }
+        for (int u = 0; u < kU; ++u) {
+          const int tc = cb + 64 * u, gc = tc0 + tc;
+          Sv[u] = (row_ok && tc < cols && gc >= 0 && gc < W) ? int(src[gc]) : Ci;
+        }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = base + kThreads * u;
-        if (i >= total) break;
-        const int tr = i / cols, tc = i % cols;
-        const int S = Sv[u];
-        const int lo = S & lo_mask;
-        const uint16_t hb = bf16_bits_exact(S - Ci - lo), lb = bf16_bits_exact(lo);
-        th0[tr * P + tc] = hb;
-        tl0[tr * P + tc] = lb;
-        if (tc > 0) { th1[tr * P + tc - 1] = hb; tl1[tr * P + tc - 1] = lb; }
-        su[tr * P + tc] = uint16_t(S);
+        for (int u = 0; u < kU; ++u) {
+          const int tc = cb + 64 * u;
+          if (tc >= cols) break;
+          const int S = Sv[u];
+          const int lo = S & lo_mask;
+          const uint16_t hb = bf16_bits_exact(S - Ci - lo), lb = bf16_bits_exact(lo);
+          th0[tr * P + tc] = hb;
+          tl0[tr * P + tc] = lb;
+          if (tc > 0) { th1[tr * P + tc - 1] = hb; tl1[tr * P + tc - 1] = lb; }
+          su[tr * P + tc] = uint16_t(S);
+        }
       }
     }
   }
@@ -174,7 +177,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   // its own mean), so by Cauchy-Schwarz (unit-energy weights)
   //   |score| <= (|dot| + 128 * 4^l * |wsum|) * inv_sd_common.
   // Lane l: wave row l >> 4, anchors 2 (l & 15) .. + 1, slid along the shared runs.
-  __shared__ double s_inv_sd[kThreads / 64][kWaveRows][kWaveCols];
+  __shared__ float s_inv_sd_f[kThreads / 64][kWaveRows][kWaveCols];
   const bool prune = args.prune_below > 0 && args.n_common > 0;
   if (prune) {
     const int rr = lane >> 4, ca = wc * kWaveCols + 2 * (lane & 15);
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
     for (int j = 0; j < 2; ++j) {
       // n * sum(t^2) - sum(t)^2 = n^2 * variance, exact in 128 bits
       const __int128 vn = (__int128)args.n_common * m2[j] - (__int128)m1[j] * m1[j];
-      s_inv_sd[wave][rr][2 * (lane & 15) + j] = vn > 0 ? sqrt(double(args.n_common) / double(vn)) : -1.0;
+      s_inv_sd_f[wave][rr][2 * (lane & 15) + j] = vn > 0 ? float(sqrt(double(args.n_common) / double(vn))) : -1.f;
     }
   }
   __syncthreads();
@@ -269,42 +272,69 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   double* plane = F.planes + (active ? a : 0);
   const int cw = 16 * g;  // first of this lane's 16 anchors, relative to the wave's 32
   const uint64_t below = (uint64_t(1) << lane) - 1;
+  // The bound test runs in FP32 (it only selects the windows scored exactly;
+  // the bound sits a tenth under the peak threshold, far beyond FP32's
+  // rounding), one bit per window (wave row r, anchor j: bit 16 r + j); the
+  // kept windows are then queued one per lane per pass, so the scoring code
+  // appears once.
+  const float prune_f = float(args.prune_below), slack_f = float(num_slack);
+  float dot[kWaveRows][16];
+  uint64_t keep = 0;
+  // this lane's 16 anchors c_j = cb + j: bits of those inside the frame
+  // (in_width) and inside the angle's valid column range (in_range)
+  const int cb = c0 + wc * kWaveCols + cw;
+  auto bits_below = [](int k) { return k <= 0 ? 0u : k >= 16 ? 0xffffu : (1u << k) - 1u; };
+  const uint32_t in_width = bits_below(W - cb);
+  const uint32_t in_range = bits_below(min(W, rg.w) - cb) & ~bits_below(rg.z - cb);
   static_for<kWaveRows>([&](auto rc) __attribute__((always_inline)) {
     constexpr int r = decltype(rc)::value;  // wave row; block row wr0 + r
-    float dot[16];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[r][i]), __float_as_uint(acc[r][8 + i]),
                                                        false, false);
       // first result: anchors 16 g + 8 (i / 4) + (i % 4); second: the same + 4
-      dot[8 * (i / 4) + (i % 4)] = __uint_as_float(sw[0]);
-      dot[8 * (i / 4) + 4 + (i % 4)] = __uint_as_float(sw[1]);
+      dot[r][8 * (i / 4) + (i % 4)] = __uint_as_float(sw[0]);
+      dot[r][8 * (i / 4) + 4 + (i % 4)] = __uint_as_float(sw[1]);
     }
     const int row = r0 + wr0 + r;
     const bool row_in = active && row < H && row >= rg.x && row < rg.y;
+    uint32_t pass = 0xffffu;
+    if (prune) {
+      pass = 0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int c = c0 + wc * kWaveCols + cw + j;
-      const bool exists = active && row < H && c < W;
-      bool keep = row_in && c < W && c >= rg.z && c < rg.w;
-      if (keep && prune) {
-        const double isd = s_inv_sd[wave][r][cw + j];
-        keep = isd < 0 || (fabs(double(dot[j])) + num_slack) * isd >= args.prune_below;
+      for (int j = 0; j < 16; ++j) {
+        const float isd = s_inv_sd_f[wave][r][cw + j];
+        pass |= uint32_t((isd < 0.f) | ((fabsf(dot[r][j]) + slack_f) * isd >= prune_f)) << j;
       }
-      if (exists && !keep && !sparse) plane[(size_t(row) * W + c) * args.n_angles] = NCC_MISSING;
-      const uint64_t mask = __ballot(keep);
-      const int cnt = __popcll(mask);
-      if (cnt) {
-        if (queued + cnt > 64) flush();
-        if (keep) {
-          const int pos = queued + __popcll(mask & below);
-          s_code[wave][pos] = ((wr0 + r) << 16) | ((cw + j) << 8) | m;
-          s_dot[wave][pos] = dot[j];
-        }
-        queued += cnt;
+    }
+    const uint32_t kr = (row_in ? in_range : 0u) & pass;
+    keep |= uint64_t(kr) << (16 * r);
+    if (!sparse) {
+      uint32_t miss = (active && row < H ? in_width : 0u) & ~kr;
+      while (miss) {
+        const int j = __builtin_ctz(miss);
+        plane[(size_t(row) * W + c0 + wc * kWaveCols + cw + j) * args.n_angles] = NCC_MISSING;
+        miss &= miss - 1;
       }
     }
   });
+  while (__ballot(keep != 0)) {
+    const bool has = keep != 0;
+    const int b = has ? __builtin_ctzll(keep) : 0;
+    const uint64_t mask = __ballot(has);
+    const int cnt = __popcll(mask);
+    if (queued + cnt > 64) flush();
+    if (has) {
+      float dv = dot[0][0];
+#pragma unroll
+      for (int q = 1; q < 16 * kWaveRows; ++q) dv = b == q ? dot[q / 16][q % 16] : dv;
+      const int pos = queued + __popcll(mask & below);
+      s_code[wave][pos] = ((wr0 + (b >> 4)) << 16) | ((cw + (b & 15)) << 8) | m;
+      s_dot[wave][pos] = dv;
+      keep &= keep - 1;
+    }
+    queued += cnt;
+  }
   if (queued) flush();
 }
 
@@ -333,7 +363,7 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   const int rows = kMRows + a.yspan - 1;
   const size_t lds = size_t(5) * rows * a.pitch * sizeof(uint16_t);
   // beside the static tables (bounds, survivor queue)
-  constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(double) + 64 * 8);
+  constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(float) + 64 * 8);
   if (lds > kLdsMax) return -6;
   const dim3 grid((max_w + kBlockCols - 1) / kBlockCols, (max_h + kMRows - 1) / kMRows, unsigned(n_frames * a.n_tiles));
   hipStream_t st = static_cast<hipStream_t>(stream);

@@ -1307,8 +1307,12 @@ __device__ __forceinline__ double clampd(double v, double lo, double hi) { retur
 // With the background guard a fit whose score stays far below min_score after
 // iteration 0 or 1 stops; those iterations then score their proposals
 // without the next derivatives (matcher.cpp:955-978).
+// coarse: refine_coarse_candidate() (matcher.cpp:853-877, the lowest level
+// above 0) -- no look-ahead derivatives, no guard, no small-step stop.
+// Each kind of evaluation has one call site, so the sampling pass is inlined
+// twice (with and without derivatives) rather than once per use.
 template <class T>
-__device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
+__device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccMatch& c, bool coarse, Shared& sm) {
   if (!a.subpixel) return;
   bool cached = false;
   double next_delta[3] = {0, 0, 0};
@@ -1319,6 +1323,8 @@ __device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccM
     cached = false;
     const double factor = fmin(fmin(1., .75 / fmax(1e-20, fabs(delta[0]))),
                                fmin(.75 / fmax(1e-20, fabs(delta[1])), a.rot ? a.step / fmax(1e-20, fabs(delta[2])) : 1.));
+    const bool guard0 = !coarse && it == 0 && a.guard_first >= 0 && c.score < a.min_score - a.guard_first;
+    const bool guard1 = !coarse && it == 1 && a.guard >= 0 && c.score < a.min_score - a.guard;
     bool accepted = false;
     for (int bt = 0; bt < 4; ++bt) {
       const double st = ldexp(factor, -bt);
@@ -1327,46 +1333,20 @@ __device__ __forceinline__ void fit(const RefineArgs& a, const PatchRef& O, NccM
       im.col += a.scale * st * delta[1];
       im.angle = clampd(im.angle + st * delta[2], a.a_start, a.a_end);
       if (!domain_contains(a, im.row, im.col)) continue;
-      bool next_valid = false;
-      const bool guard_score_only = (it == 0 && a.guard_first >= 0 && c.score < a.min_score - a.guard_first) ||
-                                    (it == 1 && a.guard >= 0 && c.score < a.min_score - a.guard);
-      if (bt == 0 && it + 1 < 6 && !guard_score_only) {
+      bool next_valid = false, need_score = true;
+      if (!coarse && bt == 0 && it + 1 < 6 && !guard0 && !guard1) {
         im.score = NCC_MISSING;
         const double floor_score = c.score;
         next_valid = refinement_step<T>(a, O, im.row, im.col, im.angle, next_delta, &im.score, &floor_score, sm);
-        if (!isfinite(im.score)) im.score = continuous<T>(a, O, im.row, im.col, im.angle, sm);
-      } else {
-        im.score = continuous<T>(a, O, im.row, im.col, im.angle, sm);
+        need_score = !isfinite(im.score);
       }
+      if (need_score) im.score = continuous<T>(a, O, im.row, im.col, im.angle, sm);
       if (im.score > c.score) { c = im; accepted = true; cached = next_valid; break; }
     }
-    if (it == 1 && a.guard >= 0 && c.score < a.min_score - a.guard) break;
-    if (it == 0 && a.guard_first >= 0 && c.score < a.min_score - a.guard_first) break;
-    if (!accepted || (hypot(delta[0], delta[1]) < 1e-4 && fabs(delta[2]) < 1e-5)) break;
-  }
-}
-
-// refine_coarse_candidate(): matcher.cpp:853-877 (lowest level above 0).
-template <class T>
-__device__ __forceinline__ void fit_coarse(const RefineArgs& a, const PatchRef& O, NccMatch& c, Shared& sm) {
-  if (!a.subpixel) return;
-  for (int it = 0; it < 6; ++it) {
-    double delta[3] = {0, 0, 0};
-    if (!refinement_step<T>(a, O, c.row, c.col, c.angle, delta, nullptr, nullptr, sm)) break;
-    const double factor = fmin(fmin(1., .75 / fmax(1e-20, fabs(delta[0]))),
-                               fmin(.75 / fmax(1e-20, fabs(delta[1])), a.rot ? a.step / fmax(1e-20, fabs(delta[2])) : 1.));
-    bool accepted = false;
-    for (int bt = 0; bt < 4; ++bt) {
-      const double st = ldexp(factor, -bt);
-      NccMatch nx = c;
-      nx.row += a.scale * st * delta[0];
-      nx.col += a.scale * st * delta[1];
-      nx.angle = clampd(nx.angle + st * delta[2], a.a_start, a.a_end);
-      if (!domain_contains(a, nx.row, nx.col)) continue;
-      nx.score = continuous<T>(a, O, nx.row, nx.col, nx.angle, sm);
-      if (nx.score > c.score) { c = nx; accepted = true; break; }
-    }
-    if (!accepted) break;
+    // the guard's stop tests the score after this iteration's step
+    if (!coarse && it == 1 && a.guard >= 0 && c.score < a.min_score - a.guard) break;
+    if (!coarse && it == 0 && a.guard_first >= 0 && c.score < a.min_score - a.guard_first) break;
+    if (!accepted || (!coarse && hypot(delta[0], delta[1]) < 1e-4 && fabs(delta[2]) < 1e-5)) break;
   }
 }
 
@@ -1540,22 +1520,44 @@ __global__ void __launch_bounds__(kRefineThreads) k_refine(const RefineArgs args
     to_local(a, cd.row, cd.col, pr, pc);
     stage_tile<T>(a, lds_tile, pr, pc, template_reach(a, O.P) + 12);
   }
-  if (a.finest) {
-    fit_coarse<T>(a, O, cd, sm);
-  } else {
+  // The sequence below -- fit the pose; at level 0, if it stays under
+  // min_score, the recovery search, then fit the recovered pose and keep it
+  // if it scores higher -- runs as a loop with one fit call (at most two
+  // fits), so the fit's code is inlined once.  find_models recovers a
+  // candidate already below min_score before fitting it (matcher.cpp:984-985);
+  // when that recovery finds nothing, the candidate itself is fitted.
+  {
+    const bool coarse = a.finest != 0;
+    const bool may_recover = !coarse && a.subpixel && a.top > 0;
     const NccMatch seed = cd;
-    // find_models recovers a candidate already below min_score before fitting it (matcher.cpp:984-985)
-    const bool recover_first = a.recover_first && a.subpixel && a.top > 0 && cd.score < a.min_score;
-    if (!recover_first) fit<T>(a, O, cd, sm);
-    if (a.subpixel && a.top > 0 && cd.score < a.min_score) {
+    const bool recover_first = a.recover_first && may_recover && cd.score < a.min_score;
+    NccMatch work = cd;
+    bool do_fit = !recover_first, fitting_best = false, recovered = false;
+    for (int step = 0; step < 3; ++step) {
+      if (do_fit) {
+        fit<T>(a, O, work, coarse, sm);
+        do_fit = false;
+        if (fitting_best) {
+          if (work.score > cd.score) cd = work;
+          break;
+        }
+        cd = work;
+        if (recovered) break;
+      }
+      if (recovered || !(may_recover && cd.score < a.min_score)) break;
+      recovered = true;
       NccMatch best = cd;
       if (recovery_search<T>(a, seed, cd.score, best, elig, wave_cnt, sm) &&
           (best.row != cd.row || best.col != cd.col || best.angle != cd.angle)) {
         best.score = continuous<T>(a, O, best.row, best.col, best.angle, sm);
-        fit<T>(a, O, best, sm);
-        if (best.score > cd.score) cd = best;
+        work = best;
+        fitting_best = true;
+        do_fit = true;
       } else if (recover_first) {
-        fit<T>(a, O, cd, sm);
+        work = cd;
+        do_fit = true;
+      } else {
+        break;
       }
     }
   }
