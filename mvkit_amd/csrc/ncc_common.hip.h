@@ -81,9 +81,12 @@ struct Pools {
 // the per-frame counts in LDS so that a flat index g in [0, off[n]) maps to
 // (frame, entry).  Every thread of the block calls it.
 __device__ __forceinline__ void list_offsets(const int32_t* counts, int n_frames, int cap, int* off) {
+  // the counts in one round of loads, then a serial scan in LDS
+  for (int f = threadIdx.x; f < n_frames; f += blockDim.x) off[f + 1] = min(max(counts[f], 0), cap);
+  __syncthreads();
   if (threadIdx.x == 0) {
     int t = 0;
-    for (int f = 0; f < n_frames; ++f) { off[f] = t; t += min(max(counts[f], 0), cap); }
+    for (int f = 0; f < n_frames; ++f) { const int c = off[f + 1]; off[f] = t; t += c; }
     off[n_frames] = t;
   }
   __syncthreads();

@@ -94,36 +94,37 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   const int shift = lv.shift;
   const int lo_mask = (1 << shift) - 1;
   // ---- stage the tile: image rows r0 + ymin .., columns c0 + xmin ..
-  // Wave w stages rows w, w + 4, ..; its lanes stride the columns, kU
-  // loads in flight per lane before the first store.
+  // (U loads in flight per thread before the first use).  Element i is
+  // (i / cols, i % cols); the quotient comes from a float reciprocal, exact
+  // here (i < 2^16, cols < 2^10: (i + 0.5) / cols stays more than 2^-11 away
+  // from an integer).
   {
-    constexpr int kU = 4;
+    constexpr int U = 8;
     const int tr0 = r0 + args.ymin, tc0 = c0 + args.xmin;
     const Tpix* img = static_cast<const Tpix*>(lv.data);
-    const int lane = threadIdx.x & 63;
-    for (int tr = threadIdx.x >> 6; tr < rows; tr += kThreads / 64) {
-      const int gr = tr0 + tr;
-      const bool row_ok = gr >= 0 && gr < H;
-      const Tpix* src = img + size_t(row_ok ? gr : 0) * lv.pitch;
-      for (int cb = lane; cb < cols; cb += 64 * kU) {
-        int Sv[kU];
+    const int total = rows * cols;
+    const float inv_cols = 1.f / float(cols);
+    for (int base = threadIdx.x; base < total; base += kThreads * U) {
+      int Sv[U];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int tc = cb + 64 * u, gc = tc0 + tc;
-          Sv[u] = (row_ok && tc < cols && gc >= 0 && gc < W) ? int(src[gc]) : Ci;
-        }
+      for (int u = 0; u < U; ++u) {
+        const int i = base + kThreads * u;
+        const int tr = int((float(i) + .5f) * inv_cols), tc = i - tr * cols;
+        const int gr = tr0 + tr, gc = tc0 + tc;
+        Sv[u] = (i < total && gr >= 0 && gr < H && gc >= 0 && gc < W) ? int(img[size_t(gr) * lv.pitch + gc]) : Ci;
+      }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int tc = cb + 64 * u;
-          if (tc >= cols) break;
-          const int S = Sv[u];
-          const int lo = S & lo_mask;
-          const uint16_t hb = bf16_bits_exact(S - Ci - lo), lb = bf16_bits_exact(lo);
-          th0[tr * P + tc] = hb;
-          tl0[tr * P + tc] = lb;
-          if (tc > 0) { th1[tr * P + tc - 1] = hb; tl1[tr * P + tc - 1] = lb; }
-          su[tr * P + tc] = uint16_t(S);
-        }
+      for (int u = 0; u < U; ++u) {
+        const int i = base + kThreads * u;
+        if (i >= total) break;
+        const int tr = int((float(i) + .5f) * inv_cols), tc = i - tr * cols;
+        const int S = Sv[u];
+        const int lo = S & lo_mask;
+        const uint16_t hb = bf16_bits_exact(S - Ci - lo), lb = bf16_bits_exact(lo);
+        th0[tr * P + tc] = hb;
+        tl0[tr * P + tc] = lb;
+        if (tc > 0) { th1[tr * P + tc - 1] = hb; tl1[tr * P + tc - 1] = lb; }
+        su[tr * P + tc] = uint16_t(S);
       }
     }
   }

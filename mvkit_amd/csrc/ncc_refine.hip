@@ -407,24 +407,25 @@ __global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
   bind_frame(a, f);
   const NccMatch cd = a.in[size_t(f) * a.cap + ci];
   const int n_e = eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt);
-  if (threadIdx.x == 0 && a.work) {
-    unsigned long long samples = 0;
-    for (int e = 0; e < n_e; ++e) samples += 25ull * uint32_t(a.pools.patch(a.refs[elig[e]]).n);
-    atomicAdd(a.work, samples);
-  }
   // union of the windows' sample rectangles, clipped to the valid region
   const int lo_r = max(0, a.sr), lo_c = max(0, a.sc);
   const int hi_r = min(a.lv.height, a.sr + a.sh) - 1, hi_c = min(a.lv.width, a.sc + a.sw) - 1;
   if (threadIdx.x == 0) { box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; }
   __syncthreads();
+  unsigned long long samples = 0;  // the work counter, in the same round of descriptor loads
   for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
     const NccPatch& P = a.pools.patch(a.refs[elig[e]]);
+    samples += 25ull * uint32_t(P.n);
     int row, col;
     anchor(a, cd, P, row, col);
     atomicMin(&box[0], max(lo_r, row - 2 + P.rmin));
     atomicMax(&box[1], min(hi_r, row + 2 + P.rmax));
     atomicMin(&box[2], max(lo_c, col - 2 + P.cmin));
     atomicMax(&box[3], min(hi_c, col + 2 + P.cmax));
+  }
+  if (a.work) {
+    samples = uint64_t(wave_sum_i64(int64_t(samples)));
+    if ((threadIdx.x & 63) == 0 && samples) atomicAdd(a.work, samples);
   }
   __syncthreads();
   const int r0 = box[0], c0 = box[2];
@@ -692,22 +693,25 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int vr1 = min(a.lv.height, a.sr + a.sh), vc1 = min(a.lv.width, a.sc + a.sw);
       __syncthreads();  // s_run0
       const bool runs_cached = s_run0[n_e] <= kTrkRunCache;
-      if (runs_cached)
-        for (int e = 0; e < n_e; ++e) {
-          const int4* src = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs) + s_pat[e].run_off;
-          for (int j = threadIdx.x; j < s_pat[e].n_runs; j += kTrackThreads) {
-            const int4 r = src[j];
-            s_runs[s_run0[e] + j] = make_int2(r.x, (r.y & 0xffff) | (r.z << 16));
-          }
+      if (runs_cached)  // every eligible patch's runs in one round of loads
+        for (int j = threadIdx.x; j < s_run0[n_e]; j += kTrackThreads) {
+          int e = 0;
+          while (s_run0[e + 1] <= j) ++e;
+          const int4 r = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs)[s_pat[e].run_off + j - s_run0[e]];
+          s_runs[j] = make_int2(r.x, (r.y & 0xffff) | (r.z << 16));
         }
       {
+        // element i of the tile is (i / PP, i % PP); the quotient comes from
+        // a float reciprocal (exact here: i < 2^16 and PP < 2^10, so
+        // (i + 0.5) / PP stays more than 2^-11 away from an integer)
         constexpr int U = 16;  // loads in flight per thread
+        const float inv_pp = 1.f / float(PP);
         for (int base = threadIdx.x; base < th * PP; base += kTrackThreads * U) {
           int Sv[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const int i = base + kTrackThreads * u;
-            const int tr = i / PP, tc = i - tr * PP;
+            const int tr = int((float(i) + .5f) * inv_pp), tc = i - tr * PP;
             const int r = U0 + tr, c = V0 + tc;
             Sv[u] = i < th * PP && tc < tw && r >= vr0 && r < vr1 && c >= vc0 && c < vc1
                         ? int(lvp[size_t(r) * a.lv.pitch + c])
@@ -717,13 +721,12 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
           for (int u = 0; u < U; ++u) {
             const int i = base + kTrackThreads * u;
             if (i >= th * PP) break;
-            const int tr = i / PP, tc = i - tr * PP;
+            const int tr = int((float(i) + .5f) * inv_pp), tc = i - tr * PP;
             const int S = Sv[u];
             if (tc < tw) sS[tr * tw + tc] = uint16_t(S);
             const int lo = S & lo_mask;
-            const uint16_t hb = bf16_exact_bits(S - lo - Di), lb = bf16_exact_bits(lo);
-            th0[tr * PP + tc] = hb;
-            tl0[tr * PP + tc] = lb;
+            th0[tr * PP + tc] = bf16_exact_bits(S - lo - Di);
+            tl0[tr * PP + tc] = bf16_exact_bits(lo);
           }
         }
       }
