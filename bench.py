@@ -207,6 +207,31 @@ def main():
         fp32 = {"value": round(2048 * 2048 * args.compare_steps * F / e32 / 1e6, 2), "unit": "Mpx/s",
                 "ms_per_frame": round(e32 / (args.compare_steps * F) * 1e3, 4), "steps": args.compare_steps}
 
+    # the FP8-screened form on the same frames: throughput, and its matches
+    # against the split-BF16 form's (same instances; poses and scores within
+    # the parity tolerances)
+    fp8 = None
+    if args.compare_steps > 0 and form == FORMS["bf16x3"]:
+        o8 = capi.NccModel.options(START, EXTENT, 0.7, 5, 0.5, True, form=FORMS["fp8"])
+        got8, _ = model.find_batch(ptrs, 2048, 2048, 2048, o8)
+        want16, _ = step(opts)
+        same = sum(int(np.array_equal(a, b)) for a, b in zip(got8, want16))
+        within = sum(int(a.shape == b.shape and (len(a) == 0 or (np.abs(a[:, :2] - b[:, :2]).max() <= 0.02 and
+                                                              np.abs(a[:, 2] - b[:, 2]).max() <= math.radians(0.01) and
+                                                              np.abs(a[:, 3] - b[:, 3]).max() <= 1e-4)))
+                     for a, b in zip(got8, want16))
+        pipelined(o8, 2)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pipelined(o8, args.compare_steps)
+        torch.cuda.synchronize()
+        e8 = time.perf_counter() - t
+        v8 = 2048 * 2048 * args.compare_steps * F / e8 / 1e6
+        fp8 = {"value": round(v8, 2), "unit": "Mpx/s", "ms_per_frame": round(e8 / (args.compare_steps * F) * 1e3, 4),
+               "steps": args.compare_steps, "speedup_vs_bf16x3": round(v8 / mpx, 3), "screen": 0.55,
+               "frames_identical_to_bf16x3": same, "frames_within_tolerance": within, "frames": F}
+        log(f"fp8-screened form: {fp8}")
+
     # coarse score error of each form against this build's FP64 form (the
     # reference's sample order in FP64) on frame 0, matched by (row, col, angle)
     def coarse(f):
@@ -214,7 +239,7 @@ def main():
         return {(r, c, a): sc for r, c, a, sc in model.find_trace(scenes[0].image, o)["coarse"]}
     ref = coarse(FORMS["fp64"])
     score_err = {}
-    for name in ("fp32", "bf16x3"):
+    for name in ("fp32", "bf16x3", "fp8"):
         got = coarse(FORMS[name])
         errs = [abs(got[k] - v) for k, v in ref.items() if k in got]
         score_err[name] = {"max": float(max(errs)), "rms": float(np.sqrt(np.mean(np.square(errs)))),
@@ -265,6 +290,7 @@ def main():
         "dominant_kernel": {"stage": dominant, "share": round(kernel[dominant] / total_kernel, 3) if total_kernel else 0},
         "fp32_form": fp32,
         "speedup_vs_fp32": round(mpx / fp32["value"], 3) if fp32 else None,
+        "fp8_screened_form": fp8,
         "coarse_score_error_vs_fp64": score_err,
         "model_create_ms": create_ms,
         "cpu_baseline": cpu,

@@ -25,7 +25,7 @@ typedef struct MvAmdFindOptions {
   int32_t subpixel;
   int32_t highest_level, lowest_level;  /* 1-based; highest 0 = all */
   int32_t form;                          /* MVAMD_FORM_* */
-  double screen;                         /* FP8 screen threshold (form FP8_SCREEN) */
+  double screen;                         /* FP8 screen threshold (form FP8_SCREEN; -inf: 0.55) */
   int32_t profile;                       /* bit s: time stage s (MVAMD_STAGE_*) with HIP events;
                                             MVAMD_PROFILE_ALL times every stage */
   int32_t vector_tracking;               /* 1: track every level with the FP64 vector kernel

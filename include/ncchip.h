@@ -93,6 +93,9 @@ typedef struct NcchipDenseWeights {
    * its bit in the frame's peak_bits (zeroed by the caller); the peaks
    * kernel then runs with hwa = 2.  -inf: every plane entry is written. */
   double peak_thr;
+  /* FP8 screen: power of two with max |w_hi| * w8_scale <= 128 (E4M3 range
+   * for either encoding of the format). */
+  double w8_scale;
 } NcchipDenseWeights;
 
 /* Split-BF16 MFMA coarse search (the planes of ncchip_coarse, stored in
@@ -100,8 +103,11 @@ typedef struct NcchipDenseWeights {
  * products on v_mfma_f32_32x32x16_bf16 (image split T_hi + T_lo, weights
  * w_hi + w_lo, three products in FP32), exact integer window moments and the
  * shared FP64 normalisation.  Top levels of kind NCC_U8 / NCC_U16 up to level 4.
- * `screen` > -inf asks for the FP8-E4M3 screen, which this build does not
- * provide (returns -6 = MV_STATUS_UNSUPPORTED). */
+ * `screen` > -inf adds the FP8-E4M3 screen: an E4M3 MFMA pass over each
+ * wave's 32 anchors x 4 rows x 32 angles, after which a wave none of whose
+ * windows reaches (|dot8| + C |wsum|) / sd_common >= screen skips its
+ * split-BF16 products (needs weights->w8_scale; the surviving waves' scores
+ * are the split-BF16 form's). */
 int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
                        double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles,
                        const NcchipDenseWeights* weights, int exact_lattice, int absolute, double screen,

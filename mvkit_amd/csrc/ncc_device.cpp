@@ -17,6 +17,10 @@ namespace mvamd {
 
 namespace {
 constexpr double kMissing = -std::numeric_limits<double>::infinity();
+// FP8-screened form: the bound threshold a wave of coarse windows must reach
+// to be scored, when the options give none (instances score well above it at
+// the coarse level; peaks below it may be dropped)
+constexpr double kDefaultScreen = 0.55;
 const char* const kFindOp = "ncc_model_find";
 [[noreturn]] void raise(Status s, const char* param, const char* msg) { throw Error(s, kFindOp, param, msg); }
 
@@ -176,6 +180,7 @@ struct DenseWeights {
   std::vector<int32_t> common_runs;  // 4 ints per run of the common support
   int n_common = 0;
   int ymin = 0, xmin = 0, yspan = 0, xchunks = 0, max_run = 0;
+  double w8_scale = 1;  // power of two with max |w_hi| * w8_scale <= 128
 };
 DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std::vector<const RasterPool*>& pools) {
   DenseWeights d;
@@ -191,6 +196,7 @@ DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std:
   d.wh.assign(tiles * size_t(d.yspan) * d.xchunks * 64 * 8, 0);
   d.wl.assign(d.wh.size(), 0);
   d.wsum_eff.assign(patches.size(), 0.0);
+  double wmax = 0;
   for (size_t a = 0; a < patches.size(); ++a) {
     const Raster& p = *patches[a];
     const RasterPool& pool = *pools[a];
@@ -210,11 +216,13 @@ DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std:
       d.wl[e] = l;
     }
     d.wsum_eff[a] = ws;
+    for (uint32_t i = 0; i < p.count; ++i) wmax = std::max(wmax, std::abs(bf16_value(bf16_bits(pool.w[p.first + i]))));
     for (uint32_t k = 0; k < p.run_count; ++k) {
       const int32_t* run = &pool.runs[4 * size_t(p.run_first + k)];
       d.max_run = std::max(d.max_run, run[2] - run[1] + 1);
     }
   }
+  if (wmax > 0) d.w8_scale = std::ldexp(1.0, int(std::floor(std::log2(128.0 / wmax))));
   // support common to every patch, as row runs
   const int xspan = xmax - d.xmin + 1;
   std::vector<int> hits(size_t(d.yspan) * xspan, 0);
@@ -648,10 +656,9 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     max_bw = std::max(max_bw, p.cmax - p.cmin);
   }
 
-  if (o.form == NCCHIP_FORM_FP8_SCREEN) raise(kUnsupported, "form", "the FP8 screen form is not available in this build");
   if (o.form < NCCHIP_FORM_FP32 || o.form > NCCHIP_FORM_FP8_SCREEN) raise(kInvalidArgument, "form", "unknown correlation form");
-  if (wide && o.form == NCCHIP_FORM_BF16X3)
-    raise(kUnsupported, "form", "the split-BF16 form needs byte search images");
+  if (wide && (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN))
+    raise(kUnsupported, "form", "the split-BF16 and FP8-screened forms need byte search images");
   if (wide) {
     // The window moments are exact 64-bit integers: sum((S - D)^2) over a
     // window must stay below 2^61 at every level searched
@@ -665,7 +672,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     // the FP32 form holds S - C exactly in FP32 up to level 4
     if (o.form == NCCHIP_FORM_FP32 && top > 4) o.form = NCCHIP_FORM_FP64;
   }
-  const bool use_mfma = o.form == NCCHIP_FORM_BF16X3;
+  const bool use_mfma = o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN;
   DenseWeights dense;
   if (use_mfma) {
     std::vector<const Raster*> ps;
@@ -919,9 +926,11 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     dw.n_common = dense.n_common;
     dw.prune_below = 0.9 * (o.min_score * .4);
     dw.peak_thr = o.min_score * .4;  // sparse planes: the peaks kernel reads the flagged scores only
+    dw.w8_scale = dense.w8_scale;
     rc = ncchip_coarse_mfma(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
                             &dw, exact_top, model.params.absolute,
-                            o.form == NCCHIP_FORM_FP8_SCREEN ? o.screen : kMissing, qs);
+                            o.form != NCCHIP_FORM_FP8_SCREEN ? kMissing : o.screen > kMissing ? o.screen : kDefaultScreen,
+                            qs);
   }
   else
     rc = ncchip_coarse(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),

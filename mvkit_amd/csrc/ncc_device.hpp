@@ -65,7 +65,7 @@ struct FindOptions {
   bool subpixel = true;
   int highest = 0, lowest = 1;
   int form = NCCHIP_FORM_FP32;     // coarse correlation form
-  double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN)
+  double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN; -inf: 0.55)
   int profile = 0;                 // bit s: time stage s (KernelStage) with HIP events on the context's stream
   bool trace_stages = false;       // fill FindTrace's stage lists (coarse, tracked, storage)
   // find_models' query cache (matcher.cpp:281-301): a candidate below the

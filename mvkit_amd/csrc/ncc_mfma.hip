@@ -13,11 +13,21 @@
 // T_hi*w_hi, T_hi*w_lo and T_lo*w_hi are exact in FP32 and accumulate in FP32
 // (T_lo*w_lo, below 2^-8 of the smallest kept term, is dropped).
 //
+// FP8 screen (kScreen): before the split-BF16 products, one pass of
+// v_mfma_f32_32x32x16_fp8_fp8 over the same tile (image T scaled by 4^-l,
+// weights w_hi scaled by a power of two, both converted by the hardware to
+// E4M3) gives approximate dot products; a wave whose windows all have
+// (|dot8| + C |wsum|) / sd_common below the screen threshold skips its
+// split-BF16 products and scoring (its windows are written nowhere in
+// sparse mode, -inf in dense mode).  The survivors' scores are the
+// split-BF16 form's, bit for bit; the screen only decides which waves run.
+//
 // The window moments sum(S) and sum(S^2) over each patch's exact support are
 // exact integers, slid along each row run in 32-bit (64-bit where a run could
 // overflow) arithmetic on the vector ALUs, and the FP64 normalisation is the
 // one every correlation form shares, so only the dot product carries the
 // split-BF16 error.
+#include <cmath>
 #include <type_traits>
 
 #include "ncc_common.hip.h"
@@ -52,7 +62,20 @@ struct MfmaArgs {
   int n_common_runs, n_common;
   double prune_below;        // score bound under which windows are written -inf
   double peak_thr;           // sparse planes: scores below are not written (-inf: dense planes)
+  // FP8 screen
+  double screen;             // bound threshold under which a wave skips its split-BF16 work
+  float img8_scale;          // 4^-l: T * img8_scale is at most 128 in magnitude
+  float w8_scale;            // power of two: w_hi * w8_scale is at most 128 in magnitude
+  double dot8_unscale;       // 1 / (img8_scale * w8_scale)
+  int pitch8;                // LDS row pitch of the FP8 image plane (bytes, multiple of 4)
 };
+
+// Two E4M3 bytes of a and b, by the hardware's conversion, into bytes
+// (2 kHi, 2 kHi + 1) of old.
+template <bool kHi>
+__device__ __forceinline__ int fp8_pack2(float a, float b, int old) {
+  return __builtin_amdgcn_cvt_pk_fp8_f32(a, b, old, kHi);
+}
 
 // Calls fn(integral_constant<int, i>) for i = 0 .. N-1 (compile-time indices).
 template <int N, class Fn>
@@ -68,7 +91,7 @@ __device__ __forceinline__ uint16_t bf16_bits_exact(int v) {
   return uint16_t(__float_as_uint(float(v)) >> 16);
 }
 
-template <class Tpix, class RunAcc>
+template <class Tpix, class RunAcc, bool kScreen>
 __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args) {
   extern __shared__ uint16_t lds[];
   const int f = blockIdx.z / args.n_tiles, tile = blockIdx.z % args.n_tiles;
@@ -85,6 +108,7 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   uint16_t* tl0 = th1 + rows * P;     // T_lo bits
   uint16_t* tl1 = tl0 + rows * P;
   uint16_t* su = tl1 + rows * P;      // S
+  uint8_t* x8 = reinterpret_cast<uint8_t*>(su + rows * P);  // E4M3 of T * img8_scale (screen only)
   // the angle tile's split weights (MFMA B fragments), read from L2 one
   // iteration ahead of their use
   const int nb = args.yspan * args.xc * 64;  // B fragments (uint4) per half
@@ -125,23 +149,120 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
         tl0[tr * P + tc] = lb;
         if (tc > 0) { th1[tr * P + tc - 1] = hb; tl1[tr * P + tc - 1] = lb; }
         su[tr * P + tc] = uint16_t(S);
+        if constexpr (kScreen) x8[tr * args.pitch8 + tc] = uint8_t(fp8_pack2<false>(float(S - Ci) * args.img8_scale, 0.f, 0));
       }
     }
   }
   __syncthreads();
-  // ---- MFMA: this wave's 32 anchors x 32 angle slots x kMRows rows
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wc = wave & 3, wr0 = (wave >> 2) * kWaveRows;  // column group, first block row of this wave
   const int m = lane & 31, g = lane >> 5;
+  // ---- epilogue, part 1: for each of the wave's 8 x 32 windows, the inverse
+  // standard deviation of the support shared by every patch.  Its variance is
+  // at most any patch's window variance (a subset's squared deviations about
+  // its own mean), so by Cauchy-Schwarz (unit-energy weights)
+  //   |score| <= (|dot| + 128 * 4^l * |wsum|) * inv_sd_common.
+  // Lane l: wave row l >> 4, anchors 2 (l & 15) .. + 1, slid along the shared runs.
+  __shared__ float s_inv_sd_f[kThreads / 64][kWaveRows][kWaveCols];
+  const bool prune = args.prune_below > 0 && args.n_common > 0;
+  auto common_sd = [&]() __attribute__((always_inline)) {
+    if (prune) {
+      const int rr = lane >> 4, ca = wc * kWaveCols + 2 * (lane & 15);
+      int64_t m1[2] = {0, 0}, m2[2] = {0, 0};
+      for (int k = 0; k < args.n_common_runs; ++k) {
+        const int4 run = args.common_runs[k];
+        const uint16_t* src = su + (wr0 + rr + run.x - args.ymin) * P + (ca - args.xmin);
+        int32_t s1 = 0;
+        RunAcc s2 = 0;
+        for (int x = run.y; x <= run.z; ++x) {
+          const int32_t t = int32_t(src[x]) - Ci;
+          s1 += t;
+          s2 += RunAcc(uint32_t(t * t));
+        }
+        m1[0] += s1;
+        m2[0] += int64_t(s2);
+        {
+          const int32_t out = int32_t(src[run.y]) - Ci, in = int32_t(src[run.z + 1]) - Ci;
+          s1 += in - out;
+          s2 += RunAcc(uint32_t(in * in)) - RunAcc(uint32_t(out * out));
+          m1[1] += s1;
+          m2[1] += int64_t(s2);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        // n * sum(t^2) - sum(t)^2 = n^2 * variance, exact in 128 bits
+        const __int128 vn = (__int128)args.n_common * m2[j] - (__int128)m1[j] * m1[j];
+        s_inv_sd_f[wave][rr][2 * (lane & 15) + j] = vn > 0 ? float(sqrt(double(args.n_common) / double(vn))) : -1.f;
+      }
+    }
+  };
+  // ---- MFMA: this wave's 32 anchors x 32 angle slots x kMRows rows
+  const int col_base = wc * kWaveCols + m + 8 * g;  // tile column of this lane's first A element for chunk 0
+  bool run_bf16 = true;  // wave-uniform
+  if constexpr (kScreen) {
+    // the screen needs the common-support deviations first
+    common_sd();
+    __syncthreads();
+    floatx16 acc8[kWaveRows];
+#pragma unroll
+    for (int r = 0; r < kWaveRows; ++r)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc8[r][v] = 0.f;
+    const int n_it8 = args.yspan * args.xc;
+    for (int it = 0; it < n_it8; ++it) {
+      const int yi = it / args.xc, ci = it - yi * args.xc;
+      const uint4 bq = gwh[it * 64 + lane];
+      int b0 = fp8_pack2<false>(__uint_as_float(bq.x << 16) * args.w8_scale, __uint_as_float(bq.x & 0xffff0000u) * args.w8_scale, 0);
+      b0 = fp8_pack2<true>(__uint_as_float(bq.y << 16) * args.w8_scale, __uint_as_float(bq.y & 0xffff0000u) * args.w8_scale, b0);
+      int b1 = fp8_pack2<false>(__uint_as_float(bq.z << 16) * args.w8_scale, __uint_as_float(bq.z & 0xffff0000u) * args.w8_scale, 0);
+      b1 = fp8_pack2<true>(__uint_as_float(bq.w << 16) * args.w8_scale, __uint_as_float(bq.w & 0xffff0000u) * args.w8_scale, b1);
+      const long B8 = long(uint32_t(b0)) | (long(uint32_t(b1)) << 32);
+      const int cs = col_base + 16 * ci;
+      const int sh = cs & 3;
+#pragma unroll
+      for (int r = 0; r < kWaveRows; ++r) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(x8 + (wr0 + r + yi) * args.pitch8 + (cs & ~3));
+        const uint32_t d0 = w[0], d1 = w[1], d2 = w[2];
+        const long A8 = long(__builtin_amdgcn_alignbyte(d1, d0, sh)) | (long(__builtin_amdgcn_alignbyte(d2, d1, sh)) << 32);
+        acc8[r] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(A8, B8, acc8[r], 0, 0, 0);
+      }
+    }
+    // does any of the wave's windows pass the screen?
+    const int a8 = tile * 32 + m;
+    const bool act8 = a8 < args.n_angles;
+    const int4 rg8 = reinterpret_cast<const int4*>(F.ranges)[act8 ? a8 : 0];
+    const double slack8 = double(Ci) * fabs(args.wsum_eff[act8 ? a8 : 0]);
+    const int cw8 = 16 * g, cb8 = c0 + wc * kWaveCols + cw8;
+    bool pass = false;
+#pragma unroll
+    for (int r = 0; r < kWaveRows; ++r) {
+      const int row = r0 + wr0 + r;
+      const bool row_in = act8 && row < H && row >= rg8.x && row < rg8.y;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc8[r][i]), __float_as_uint(acc8[r][8 + i]),
+                                                         false, false);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int j = 8 * (i / 4) + 4 * h + (i % 4);
+          const int c = cb8 + j;
+          const float isd = s_inv_sd_f[wave][r][cw8 + j];
+          const double bnd = (fabs(double(__uint_as_float(sw[h])) * args.dot8_unscale) + slack8) * double(isd);
+          pass |= row_in && c < W && c >= rg8.z && c < rg8.w && (!prune || isd < 0.f || bnd >= args.screen);
+        }
+      }
+    }
+    run_bf16 = __ballot(pass) != 0;
+  }
   floatx16 acc[kWaveRows];
 #pragma unroll
   for (int r = 0; r < kWaveRows; ++r)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[r][v] = 0.f;
-  const int col_base = wc * kWaveCols + m + 8 * g;  // tile column of this lane's first A element for chunk 0
   const int n_iter = args.yspan * args.xc;
   uint4 bh_next = gwh[lane], bl_next = gwl[lane];
-  for (int it = 0; it < n_iter; ++it) {
+  for (int it = 0; run_bf16 && it < n_iter; ++it) {
     const int yi = it / args.xc, ci = it - yi * args.xc;
     {
       const uint4 bh4 = bh_next, bl4 = bl_next;
@@ -172,45 +293,10 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
       for (int r = 0; r < kWaveRows; ++r) acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[r], bh, acc[r], 0, 0, 0);
     }
   }
-  // ---- epilogue, part 1: for each of the wave's 8 x 32 windows, the inverse
-  // standard deviation of the support shared by every patch.  Its variance is
-  // at most any patch's window variance (a subset's squared deviations about
-  // its own mean), so by Cauchy-Schwarz (unit-energy weights)
-  //   |score| <= (|dot| + 128 * 4^l * |wsum|) * inv_sd_common.
-  // Lane l: wave row l >> 4, anchors 2 (l & 15) .. + 1, slid along the shared runs.
-  __shared__ float s_inv_sd_f[kThreads / 64][kWaveRows][kWaveCols];
-  const bool prune = args.prune_below > 0 && args.n_common > 0;
-  if (prune) {
-    const int rr = lane >> 4, ca = wc * kWaveCols + 2 * (lane & 15);
-    int64_t m1[2] = {0, 0}, m2[2] = {0, 0};
-    for (int k = 0; k < args.n_common_runs; ++k) {
-      const int4 run = args.common_runs[k];
-      const uint16_t* src = su + (wr0 + rr + run.x - args.ymin) * P + (ca - args.xmin);
-      int32_t s1 = 0;
-      RunAcc s2 = 0;
-      for (int x = run.y; x <= run.z; ++x) {
-        const int32_t t = int32_t(src[x]) - Ci;
-        s1 += t;
-        s2 += RunAcc(uint32_t(t * t));
-      }
-      m1[0] += s1;
-      m2[0] += int64_t(s2);
-      {
-        const int32_t out = int32_t(src[run.y]) - Ci, in = int32_t(src[run.z + 1]) - Ci;
-        s1 += in - out;
-        s2 += RunAcc(uint32_t(in * in)) - RunAcc(uint32_t(out * out));
-        m1[1] += s1;
-        m2[1] += int64_t(s2);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      // n * sum(t^2) - sum(t)^2 = n^2 * variance, exact in 128 bits
-      const __int128 vn = (__int128)args.n_common * m2[j] - (__int128)m1[j] * m1[j];
-      s_inv_sd_f[wave][rr][2 * (lane & 15) + j] = vn > 0 ? float(sqrt(double(args.n_common) / double(vn))) : -1.f;
-    }
+  if constexpr (!kScreen) {
+    common_sd();
+    __syncthreads();
   }
-  __syncthreads();
   // part 2.  Lane (slot m = lane & 31, half g = lane >> 5) holds the anchors
   // 8 * (v / 4) + 4 * g + (v % 4) of its wave's 32; one exchange with the
   // partner half gives it the 16 consecutive anchors 16 g .. 16 g + 15.
@@ -264,6 +350,18 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
     __builtin_amdgcn_wave_barrier();
     queued = 0;
   };
+  if (!run_bf16) {  // screened out: nothing to score (no block barrier follows)
+    const int aa = tile * 32 + m;
+    if (!sparse && aa < args.n_angles)
+      for (int r = 0; r < kWaveRows; ++r) {
+        const int row = r0 + wr0 + r;
+        for (int j = 0; j < 16; ++j) {
+          const int c = c0 + wc * kWaveCols + 16 * g + j;
+          if (row < H && c < W) F.planes[(size_t(row) * W + c) * args.n_angles + aa] = NCC_MISSING;
+        }
+      }
+    return;
+  }
   const int a = tile * 32 + m;
   const bool active = a < args.n_angles;
   const int4 rg = reinterpret_cast<const int4*>(F.ranges)[active ? a : 0];
@@ -348,8 +446,9 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
                                   int n_angles, const NcchipDenseWeights* dw, int exact_lattice, int absolute,
                                   double screen, void* stream) {
   if (n_frames <= 0 || n_angles <= 0) return 0;
-  if (screen > -__builtin_inf()) return -6;            // FP8 screen: not in this build
   if (level_kind == NCC_U32 || level > 4) return -6;   // T_lo exact in BF16 only up to level 4
+  const bool screened = screen > -__builtin_inf();
+  if (screened && !(dw->w8_scale > 0)) return -1;
   MfmaArgs a{};
   a.frames = frames; a.level = level; a.n_angles = n_angles; a.n_tiles = (n_angles + 31) / 32; a.C = C;
   a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.patch_ids = patch_ids;
@@ -362,7 +461,15 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   const int cols = kBlockCols + 16 * a.xc + 1;
   a.pitch = (cols + 1) & ~1;
   const int rows = kMRows + a.yspan - 1;
-  const size_t lds = size_t(5) * rows * a.pitch * sizeof(uint16_t);
+  a.pitch8 = (cols + 11) & ~3;  // an 8-byte A fragment read as three dwords from any column
+  size_t lds = size_t(5) * rows * a.pitch * sizeof(uint16_t);
+  if (screened) {
+    lds += size_t(rows) * a.pitch8;
+    a.screen = screen;
+    a.img8_scale = float(std::ldexp(1.0, -2 * level));
+    a.w8_scale = float(dw->w8_scale);
+    a.dot8_unscale = 1.0 / (double(a.img8_scale) * double(a.w8_scale));
+  }
   // beside the static tables (bounds, survivor queue)
   constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(float) + 64 * 8);
   if (lds > kLdsMax) return -6;
@@ -371,7 +478,11 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   // per-run sums of t^2 fit 32 bits when (longest run) * (128 * 4^level)^2 < 2^32
   const double tmax = 128.0 * double(1 << (2 * level));
   const bool narrow = double(dw->max_run) * tmax * tmax < 4294967296.0;
-#define NCC_MFMA(TP, RA) hipLaunchKernelGGL((k_coarse_mfma<TP, RA>), grid, dim3(kThreads), lds, st, a)
+#define NCC_MFMA(TP, RA)                                                                    \
+  do {                                                                                      \
+    if (screened) hipLaunchKernelGGL((k_coarse_mfma<TP, RA, true>), grid, dim3(kThreads), lds, st, a); \
+    else hipLaunchKernelGGL((k_coarse_mfma<TP, RA, false>), grid, dim3(kThreads), lds, st, a);         \
+  } while (0)
   if (level_kind == NCC_U8) {
     if (narrow) NCC_MFMA(uint8_t, uint32_t); else NCC_MFMA(uint8_t, uint64_t);
   } else {
