@@ -387,7 +387,7 @@ __device__ __forceinline__ void anchor(const TrackArgs& a, const NccMatch& cd, c
 // reference's (patch, dr, dc) order (matcher.cpp:670-756).  Windows are
 // spread over the waves; the candidate's neighbourhood is staged in LDS.
 template <class T, class Acc>
-__global__ void __launch_bounds__(kTrackThreads) k_track(const TrackArgs args) {
+__global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args) {
   extern __shared__ uint32_t tile_words[];
   T* tile = reinterpret_cast<T*>(tile_words);
   __shared__ int elig[kMaxEligible];
