@@ -159,3 +159,20 @@ def test_domain_template_model(capi, oracle, scene_small):
     got = m.find(scene_small.image, -E, 2 * E, 0.6, 0, 0.5, "true", 0)
     check(got, capi.halcon_shift(want), "domain template")
     assert len(got) == 2
+
+
+def test_roi_fp8_screen_form(capi, oracle, scene_small, small):
+    """The FP8-screened form masks the domain as the split-BF16 form does:
+    rectangles and disks around the instances give the split-BF16 form's
+    matches (the instances score far above the screen)."""
+    m, _ = small
+    h, w = scene_small.image.shape
+    for label, dom in domains(scene_small, scene_small.truth):
+        if not label.startswith(("rect", "disk")):
+            continue
+        dom = oracle.domain_runs(dom, h, w)
+        say(f"ROI fp8 screen {label}")
+        img = capi.Image(scene_small.image, domain=dom)
+        want, _ = m.find_image(img, capi.NccModel.options(-E, 2 * E, 0.7, 0, 0.5, form=capi.FORM_BF16X3))
+        got, _ = m.find_image(img, capi.NccModel.options(-E, 2 * E, 0.7, 0, 0.5, form=capi.FORM_FP8_SCREEN))
+        assert len(want) == 1 and np.array_equal(got, want), label
