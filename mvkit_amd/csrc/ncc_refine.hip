@@ -1102,6 +1102,28 @@ __device__ __forceinline__ bool strictly_inside(const RefineArgs& a, const NccPa
   return ymin > margin && xmin > margin && ymax < double(a.sh - 1) - margin && xmax < double(a.sw - 1) - margin;
 }
 
+// Block-wide sums of the 18 refinement moments: the first 16 with one
+// transposed butterfly per wave (17 exchanges instead of 96), the last two
+// with plain wave sums, then across the waves through LDS in wave order.
+__device__ __forceinline__ void block_sum18(double (&v)[18], double* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  double head[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) head[i] = v[i];
+  const double t = wave_transpose_sum16(head);  // lane L: total of value L >> 2
+  const double t16 = wave_sum(v[16]), t17 = wave_sum(v[17]);
+  if ((lane & 3) == 0) scratch[wave * 18 + (lane >> 2)] = t;
+  if (lane == 0) { scratch[wave * 18 + 16] = t16; scratch[wave * 18 + 17] = t17; }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 18; ++i) {
+    double a = 0;
+    for (int w = 0; w < n_waves; ++w) a += scratch[w * 18 + i];
+    v[i] = a;
+  }
+  __syncthreads();
+}
+
 // Bilinear sample (and gradients) at storage-local (y, x) known to be inside.
 template <class T>
 __device__ __forceinline__ void bilinear_at(const RefineArgs& a, double y, double x, bool interior, double& q,
@@ -1177,7 +1199,7 @@ __device__ __forceinline__ bool bilinear_pass(const RefineArgs& a, const PatchRe
     }
   }
   if (block_any(bad, &sm.flag)) return false;
-  if (deriv) block_sum<18>(v, sm.red);
+  if (deriv) block_sum18(v, sm.red);
   else {
     double t[3] = {v[0], v[1], v[2]};
     block_sum<3>(t, sm.red);
