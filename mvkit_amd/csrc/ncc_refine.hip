@@ -1030,6 +1030,7 @@ constexpr int kRefineTileBytes = 64 * 1024;
 
 struct Shared {
   double red[(kRefineThreads / 64) * 18];
+  double tot[18];
   int flag;
   double bcast[8];
   double win_s[kRefineThreads / 64];
@@ -1102,10 +1103,12 @@ __device__ __forceinline__ bool strictly_inside(const RefineArgs& a, const NccPa
   return ymin > margin && xmin > margin && ymax < double(a.sh - 1) - margin && xmax < double(a.sw - 1) - margin;
 }
 
-// Block-wide sums of the 18 refinement moments: the first 16 with one
-// transposed butterfly per wave (17 exchanges instead of 96), the last two
-// with plain wave sums, then across the waves through LDS in wave order.
-__device__ __forceinline__ void block_sum18(double (&v)[18], double* scratch) {
+// Block-wide sums of the 18 refinement moments, for thread 0 only (the
+// Gauss-Newton solve runs there): the first 16 with one transposed butterfly
+// per wave (17 exchanges instead of 96), the last two with plain wave sums;
+// then thread t < 18 adds value t over the waves (in wave order) and thread
+// 0 reads the 18 totals.  Other threads' v is left unspecified.
+__device__ __forceinline__ void block_sum18(double (&v)[18], double* scratch, double* tot) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
   double head[16];
 #pragma unroll
@@ -1115,13 +1118,15 @@ __device__ __forceinline__ void block_sum18(double (&v)[18], double* scratch) {
   if ((lane & 3) == 0) scratch[wave * 18 + (lane >> 2)] = t;
   if (lane == 0) { scratch[wave * 18 + 16] = t16; scratch[wave * 18 + 17] = t17; }
   __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 18; ++i) {
+  if (threadIdx.x < 18) {
     double a = 0;
-    for (int w = 0; w < n_waves; ++w) a += scratch[w * 18 + i];
-    v[i] = a;
+    for (int w = 0; w < n_waves; ++w) a += scratch[w * 18 + threadIdx.x];
+    tot[threadIdx.x] = a;
   }
   __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < 18; ++i) v[i] = tot[i];
 }
 
 // Bilinear sample (and gradients) at storage-local (y, x) known to be inside.
@@ -1145,7 +1150,8 @@ __device__ __forceinline__ bool inside(const RefineArgs& a, double y, double x, 
 
 // Block-wide bilinear moments of the unrotated patch at a storage-local pose.
 // out: [0] sum q, [1] sum q^2, [2] sum w q, then gs[3], qg[3], wg[3], and
-// gg00, gg10, gg11, gg20, gg21, gg22.  Returns false if any sample is outside.
+// gg00, gg10, gg11, gg20, gg21, gg22 (with deriv, on thread 0 only).  Returns
+// false if any sample is outside.
 template <class T>
 __device__ __forceinline__ bool bilinear_pass(const RefineArgs& a, const PatchRef& O, double pr, double pc, double angle, bool deriv,
                               double* out, Shared& sm) {
@@ -1199,7 +1205,7 @@ __device__ __forceinline__ bool bilinear_pass(const RefineArgs& a, const PatchRe
     }
   }
   if (block_any(bad, &sm.flag)) return false;
-  if (deriv) block_sum18(v, sm.red);
+  if (deriv) block_sum18(v, sm.red, sm.tot);  // the derivative pass feeds thread 0's solve only
   else {
     double t[3] = {v[0], v[1], v[2]};
     block_sum<3>(t, sm.red);
