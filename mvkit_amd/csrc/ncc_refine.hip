@@ -382,6 +382,25 @@ __device__ __forceinline__ void anchor(const TrackArgs& a, const NccMatch& cd, c
   col = int(round((cd.col - a.crop_c - a.offset) / a.step - P.ocol));
 }
 
+// Block-wide (score, index) maximum, ties to the lower index; every thread
+// gets the result.
+__device__ __forceinline__ void block_argmax(double& s, int& i, double* ws, int* wi) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    const double o = __shfl_xor(s, m, 64);
+    const int oi = __shfl_xor(i, m, 64);
+    if (o > s || (o == s && oi < i)) { s = o; i = oi; }
+  }
+  if (lane == 0) { ws[wave] = s; wi[wave] = i; }
+  __syncthreads();
+  s = ws[0];
+  i = wi[0];
+  for (int w = 1; w < kTrackThreads / 64; ++w)
+    if (ws[w] > s || (ws[w] == s && wi[w] < i)) { s = ws[w]; i = wi[w]; }
+  __syncthreads();
+}
+
 // One block per candidate: score the 5x5 anchors around the candidate for every
 // query patch within the angle window and keep the first strict maximum in the
 // reference's (patch, dr, dc) order (matcher.cpp:670-756).  Windows are
@@ -393,8 +412,8 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args
   __shared__ int elig[kMaxEligible];
   __shared__ int wave_cnt[kTrackThreads / 64];
   __shared__ int box[4];
-  __shared__ double red_s[kTrackThreads];
-  __shared__ int red_i[kTrackThreads];
+  __shared__ double red_s[kTrackThreads / 64];
+  __shared__ int red_i[kTrackThreads / 64];
   __shared__ int off[NCC_MAX_BATCH + 1];
   list_offsets(args.in_count, args.n_frames, args.cap, off);
   const int total = off[args.n_frames];
@@ -465,32 +484,19 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args
     else track_row<Acc>(a, p, row + dr, col, LevelReader<T>{lvp, a.lv.pitch}, sc, k);
     if (k >= 0 && sc > best) { best = sc; best_i = (task / 5) * 25 + (dr + 2) * 5 + k; }
   }
-  red_s[threadIdx.x] = best;
-  red_i[threadIdx.x] = best_i;
-  __syncthreads();
-  for (int h = kTrackThreads / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
-      const double o = red_s[threadIdx.x + h];
-      const int oi = red_i[threadIdx.x + h];
-      if (o > red_s[threadIdx.x] || (o == red_s[threadIdx.x] && oi < red_i[threadIdx.x])) {
-        red_s[threadIdx.x] = o;
-        red_i[threadIdx.x] = oi;
-      }
-    }
-    __syncthreads();
-  }
+  block_argmax(best, best_i, red_s, red_i);  // first strict maximum in (patch, dr, dc) order
   if (threadIdx.x == 0) {
     NccMatch res = cd;
     res.score = NCC_MISSING;
-    if (red_s[0] > NCC_MISSING) {
-      const int wi = red_i[0];
+    if (best > NCC_MISSING) {
+      const int wi = best_i;
       const NccPatch& P = a.pools.patch(a.refs[elig[wi / 25]]);
       int row, col;
       anchor(a, cd, P, row, col);
       res.row = a.crop_r + a.step * (row + (wi % 25) / 5 - 2 + P.orow) + a.offset;
       res.col = a.crop_c + a.step * (col + wi % 5 - 2 + P.ocol) + a.offset;
       res.angle = P.angle;
-      res.score = red_s[0];
+      res.score = best;
     }
     keep_distinct(a, f, ci, res);
   }
@@ -564,25 +570,6 @@ __device__ __forceinline__ double exact_window_wave(const TrackArgs& a, const Pa
   t2 = wave_sum_i64(t2);
   dot = wave_sum(dot);
   return window_score(a, P, ldexp(dot, -a.lv.shift), t1, t2);
-}
-
-// Block-wide (score, index) maximum, ties to the lower index; every thread
-// gets the result.
-__device__ __forceinline__ void block_argmax(double& s, int& i, double* ws, int* wi) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-    const double o = __shfl_xor(s, m, 64);
-    const int oi = __shfl_xor(i, m, 64);
-    if (o > s || (o == s && oi < i)) { s = o; i = oi; }
-  }
-  if (lane == 0) { ws[wave] = s; wi[wave] = i; }
-  __syncthreads();
-  s = ws[0];
-  i = wi[0];
-  for (int w = 1; w < kTrackThreads / 64; ++w)
-    if (ws[w] > s || (ws[w] == s && wi[w] < i)) { s = ws[w]; i = wi[w]; }
-  __syncthreads();
 }
 
 template <class T, class P2>
