@@ -587,6 +587,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
   __shared__ int s_run0[kTrkSlots + 1];        // first run of each slot in s_runs
   __shared__ int s_dense;
   __shared__ double red_s[kTrackThreads / 64];
+  __shared__ double red_u[kTrackThreads / 64];
   __shared__ int red_i[kTrackThreads / 64];
   __shared__ int off[NCC_MAX_BATCH + 1];
   uint16_t* sS = reinterpret_cast<uint16_t*>(dyn + args.mt_s);
@@ -898,18 +899,25 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
         }
         s_up[k] = up;
       }
-      int dummy = 0;
-      block_argmax(lower_max, dummy, red_s, red_i);
-      const double L = lower_max;
-      // a candidate whose windows all stay below the keep threshold is dropped
-      // whichever window is the maximum: no exact scoring needed
-      {
-        double um = NCC_MISSING;
-        for (int k = threadIdx.x; k < n_e * 25; k += kTrackThreads) um = fmax(um, double(s_up[k]));
-        int d2 = 0;
-        block_argmax(um, d2, red_s, red_i);
-        if (um < a.keep) n_e_skip = true;
+      // one block reduction for both maxima: the best lower bound L and the
+      // best upper bound (a candidate whose windows all stay below the keep
+      // threshold is dropped whichever window is the maximum: no exact
+      // scoring needed)
+      double um = NCC_MISSING;
+      for (int k = threadIdx.x; k < n_e * 25; k += kTrackThreads) um = fmax(um, double(s_up[k]));
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) {
+        lower_max = fmax(lower_max, __shfl_xor(lower_max, d, 64));
+        um = fmax(um, __shfl_xor(um, d, 64));
       }
+      if (lane == 0) { red_s[wave] = lower_max; red_u[wave] = um; }
+      __syncthreads();
+      lower_max = red_s[0];
+      um = red_u[0];
+      for (int w = 1; w < kTrackThreads / 64; ++w) { lower_max = fmax(lower_max, red_s[w]); um = fmax(um, red_u[w]); }
+      __syncthreads();
+      const double L = lower_max;
+      if (um < a.keep) n_e_skip = true;
       // ---- the windows that may hold the maximum, in (patch, dr, dc) order
       int n_list = 0;
       for (int base = 0; !n_e_skip && base < n_e * 25; base += kTrackThreads) {
