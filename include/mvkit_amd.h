@@ -13,7 +13,12 @@
 extern "C" {
 #endif
 
-/* Coarse correlation forms (same values as NCCHIP_FORM_*). */
+/* Coarse correlation forms (same values as NCCHIP_FORM_*).  AUTO: split-BF16
+ * on byte images, FP32 on uint2 images (what mv_find_ncc_model runs).  A
+ * split-BF16 or FP8-screened search whose top level or coarse bounding box
+ * does not fit the matrix-core kernel runs the FP32 form instead
+ * (MvAmdFindStats.coarse_form reports the form that ran). */
+#define MVAMD_FORM_AUTO (-1)
 #define MVAMD_FORM_FP32 0
 #define MVAMD_FORM_BF16X3 1
 #define MVAMD_FORM_FP64 2
@@ -59,6 +64,8 @@ typedef struct MvAmdFindStats {
   double coarse_flops;                   /* algorithmic FLOPs of the coarse correlation */
   double track_flops;                    /* algorithmic FLOPs of tracking (when its stage is profiled) */
   double track_mfma_flops;               /* the part of track_flops tracked on matrix cores */
+  int32_t coarse_form;                   /* MVAMD_FORM_* the coarse search ran (-1: none ran) */
+  int32_t reserved;
 } MvAmdFindStats;
 
 /* Find on a u8 (or, with pixel_type MVAMD_PIXEL_UINT2, u16) image given as
@@ -118,7 +125,8 @@ MV_API MvStatus mvamd_find_ncc_model_batch_submit(MvTuple ModelID, const uint8_t
 MV_API MvStatus mvamd_find_ncc_model_batch_wait(uint64_t ticket, double* results, size_t cap, size_t* counts,
                                                 MvAmdFindStats* stats);
 
-/* Find on a host u8 image returning the stages: coarse peaks (cap_coarse x 4),
+/* Find on a host image (u8, or u16 with options->pixel_type MVAMD_PIXEL_UINT2;
+ * pitch in bytes) returning the stages: coarse peaks (cap_coarse x 4),
  * full-scored candidates in score order (cap_prepared x 4) and, per such
  * candidate, the refined pose and verdict (cap_prepared x 5). */
 MV_API MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* image, int64_t width, int64_t height,

@@ -108,6 +108,12 @@ typedef struct NcchipDenseWeights {
  * windows reaches (|dot8| + C |wsum|) / sd_common >= screen skips its
  * split-BF16 products (needs weights->w8_scale; the surviving waves' scores
  * are the split-BF16 form's). */
+/* 1 when ncchip_coarse_mfma can run the top level `level` of pixel kind
+ * level_kind over a union bounding box of yspan rows and xchunks 16-column
+ * chunks (its image tile fits in LDS; levels up to 4 of u8/u16 block sums),
+ * else 0.  The host driver then runs the FP32 form for that search. */
+int ncchip_coarse_mfma_fits(int level, int level_kind, int yspan, int xchunks, int screened);
+
 int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
                        double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles,
                        const NcchipDenseWeights* weights, int exact_lattice, int absolute, double screen,

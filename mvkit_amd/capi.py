@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmvkit_amd.so")
 
 MV_VALUE_EMPTY, MV_VALUE_INT64, MV_VALUE_REAL64, MV_VALUE_STRING, MV_VALUE_HANDLE = 0, 1, 2, 4, 16
-FORM_FP32, FORM_BF16X3, FORM_FP64, FORM_FP8_SCREEN = 0, 1, 2, 3
+FORM_AUTO, FORM_FP32, FORM_BF16X3, FORM_FP64, FORM_FP8_SCREEN = -1, 0, 1, 2, 3
 PIXEL_BYTE, PIXEL_UINT2 = 0, 2  # device image pixel types (MVAMD_PIXEL_*)
 CREATE_AUTO, CREATE_HOST, CREATE_DEVICE = 0, 1, 2  # raster builders (MVAMD_CREATE_*)
 
@@ -39,7 +39,7 @@ PUBLIC_SYMBOLS = [
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
-                  "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma",
+                  "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma", "ncchip_coarse_mfma_fits",
                   "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine",
                   "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap", "ncchip_create_rasters",
                   "ncchip_pyramid3_u8"]
@@ -70,7 +70,8 @@ class FindStats(C.Structure):
                 ("angle_stride", C.c_int32), ("storage", C.c_int32 * 4), ("t_coarse_us", C.c_double),
                 ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double),
                 ("kernel_us", C.c_double * 8), ("coarse_flops", C.c_double),
-                ("track_flops", C.c_double), ("track_mfma_flops", C.c_double)]
+                ("track_flops", C.c_double), ("track_mfma_flops", C.c_double), ("coarse_form", C.c_int32),
+                ("reserved", C.c_int32)]
 
 PROFILE_ALL = 0xff
 STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine", "track_mfma"]

@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <optional>
 #include <string>
 #include <variant>
 #include <vector>
@@ -232,73 +233,155 @@ std::shared_ptr<mvamd::NccModel> model_of(MvTuple t, const char* op) {
   return v.h->object->model;
 }
 
-// Per-thread find pipelines, one per device.
+// Find pipelines.  A context owns HIP streams, events and device memory, so
+// it must never be destroyed from thread-local storage: at thread or process
+// exit the HIP runtime and profilers may already have torn down their own
+// per-thread state.  Contexts belong to a process-wide pool instead; a thread
+// borrows one per device on its first call and hands it back, without any
+// HIP call, when it ends.  The next thread to need a context on that device
+// takes the idle one and drains it first.  The pool is never destroyed: its
+// contexts' device memory goes back to the driver with the process.
+class ContextPool {
+ public:
+  mvamd::FindContext* take(int device) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto& idle = idle_[device];
+      if (!idle.empty()) {
+        mvamd::FindContext* c = idle.back();
+        idle.pop_back();
+        c->reset();
+        return c;
+      }
+    }
+    return new mvamd::FindContext();  // on the calling thread's current device
+  }
+  void give_back(int device, mvamd::FindContext* c) {
+    std::lock_guard<std::mutex> lock(mu_);
+    idle_[device].push_back(c);
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<int, std::vector<mvamd::FindContext*>> idle_;
+};
+ContextPool& context_pool() {
+  static ContextPool* pool = new ContextPool;  // deliberately never destroyed
+  return *pool;
+}
+struct BorrowedContexts {
+  std::map<int, mvamd::FindContext*> by_device;
+  ~BorrowedContexts() {
+    for (auto& [device, c] : by_device) context_pool().give_back(device, c);
+  }
+};
 mvamd::FindContext& context() {
-  thread_local std::vector<std::unique_ptr<mvamd::FindContext>> contexts;
+  thread_local BorrowedContexts mine;
   int dev = 0;
   mvamd::check(hipGetDevice(&dev), "hipGetDevice");
-  if (contexts.size() <= size_t(dev)) contexts.resize(size_t(dev) + 1);
-  if (!contexts[size_t(dev)]) contexts[size_t(dev)] = std::make_unique<mvamd::FindContext>();
-  return *contexts[size_t(dev)];
+  mvamd::FindContext*& c = mine.by_device[dev];
+  if (!c) c = context_pool().take(dev);
+  return *c;
 }
 
-// NccControls (ncc_matching_api.cpp:126-161): each control holds one value
-// or one per model; NumLevels holds the highest level, a shared
-// (highest, lowest) pair, one highest level per model or a pair per model.
-// The selection takes NumMatches / MaxOverlap when they hold one value.
+// ---------------------------------------------------- find control schema
+// The find operators (ncc_matching_api.cpp:126-161 gives the rules) take six
+// value controls and NumLevels.  Each value control holds one value shared
+// by every model of the call or one value per model; NumLevels holds the
+// highest level or a (highest, lowest) pair, shared or per model.  The table
+// below says how one value becomes a FindOptions field.  Validation runs in
+// two passes so that the first fault reported is the same as mvkit's: the
+// cardinality of every control in argument order, then per model the values
+// by form (ranged reals, then the count, then the keyword), then the levels.
+enum class ValueForm { kReal, kCount, kKeyword };
+struct FindControl {
+  const char* param;
+  ValueForm form;
+  double lo, hi;  // kReal: closed range of accepted values
+  void (*store)(mvamd::FindOptions&, double);
+};
+constexpr double kLargest = std::numeric_limits<double>::max();
+constexpr double kFullTurn = 2 * 3.14159265358979323846;
+constexpr FindControl kFindControls[] = {
+    {"AngleStart", ValueForm::kReal, -kLargest, kLargest, [](mvamd::FindOptions& q, double v) { q.start = v; }},
+    {"AngleExtent", ValueForm::kReal, 0, kFullTurn, [](mvamd::FindOptions& q, double v) { q.extent = v; }},
+    {"MinScore", ValueForm::kReal, 0, 1, [](mvamd::FindOptions& q, double v) { q.min_score = v; }},
+    {"NumMatches", ValueForm::kCount, 0, 0, [](mvamd::FindOptions& q, double v) { q.num_matches = size_t(v); }},
+    {"MaxOverlap", ValueForm::kReal, 0, 1, [](mvamd::FindOptions& q, double v) { q.max_overlap = v; }},
+    {"SubPixel", ValueForm::kKeyword, 0, 0, [](mvamd::FindOptions& q, double v) { q.subpixel = v != 0; }},
+};
+constexpr size_t kValueControls = sizeof(kFindControls) / sizeof(kFindControls[0]);
+
+// Index of model i's value in a control holding one shared value or one per model.
+size_t broadcast(const Input& in, size_t i) { return in.size() == 1 ? 0 : i; }
+
+double control_value(const Input& in, const FindControl& c, size_t i) {
+  const size_t k = broadcast(in, i);
+  switch (c.form) {
+    case ValueForm::kReal: {
+      const double x = in.number(k);
+      if (!(x >= c.lo && x <= c.hi)) in.bad(mvamd::kErrInvalidValue, int64_t(k), "value lies outside the control's range");
+      return x;
+    }
+    case ValueForm::kCount: {
+      const int64_t x = in.integer(k);
+      if (x < 0) in.bad(mvamd::kErrInvalidValue, int64_t(k), "a count cannot be negative");
+      return double(x);
+    }
+    case ValueForm::kKeyword: {
+      const std::string& word = in.string(k);
+      if (word == "true") return 1;
+      if (word != "false") in.bad(mvamd::kErrUnsupportedValue, int64_t(k), "keyword must be 'true' or 'false'");
+      return 0;
+    }
+  }
+  return 0;
+}
+
+// Where model i's highest and (optional) lowest level sit in NumLevels.
+struct LevelSlots { size_t highest = 0, lowest = 0; bool paired = false; };
+LevelSlots level_slots(size_t n_values, size_t models, size_t i) {
+  if (n_values == models && models > 1) return {i, 0, false};     // one highest level per model
+  if (n_values == 2 * models) return {2 * i, 2 * i + 1, true};    // one pair per model
+  if (n_values == 2) return {0, 1, true};                         // one shared pair
+  return {0, 0, false};                                           // one shared highest level
+}
+
 struct Controls {
   std::vector<mvamd::FindOptions> options;
   mvamd::Selection selection;
 };
-Controls controls_n(const char* op, size_t count, MvTuple a, MvTuple e, MvTuple s, MvTuple n, MvTuple o, MvTuple p,
-                    MvTuple l) {
-  const Input start(a, op, "AngleStart"), extent(e, op, "AngleExtent"), score(s, op, "MinScore"),
-      matches(n, op, "NumMatches"), overlap(o, op, "MaxOverlap"), subpixel(p, op, "SubPixel"), levels(l, op, "NumLevels");
-  for (const Input* c : {&start, &extent, &score, &matches, &overlap, &subpixel})
-    if (c->size() != 1 && c->size() != count)
-      c->bad(mvamd::kErrInvalidCardinality, -1, "one value or one value per model required");
-  if (levels.size() != 1 && levels.size() != 2 && levels.size() != count && levels.size() != 2 * count)
-    levels.bad(mvamd::kErrInvalidCardinality, -1, "highest level or shared/per-model level pairs required");
-  auto number = [](const Input& v, size_t i, double lo, double hi) {
-    const size_t index = v.size() == 1 ? 0 : i;
-    const double x = v.number(index);
-    if (!std::isfinite(x) || x < lo || x > hi) v.bad(mvamd::kErrInvalidValue, int64_t(index), "numeric control outside allowed range");
-    return x;
-  };
+Controls find_controls(const char* op, size_t models, const MvTuple (&tuples)[kValueControls], MvTuple num_levels) {
+  std::vector<Input> in;
+  in.reserve(kValueControls);
+  for (size_t c = 0; c < kValueControls; ++c) in.emplace_back(tuples[c], op, kFindControls[c].param);
+  const Input levels(num_levels, op, "NumLevels");
+  for (const Input& x : in)
+    if (x.size() != 1 && x.size() != models)
+      x.bad(mvamd::kErrInvalidCardinality, -1, "control needs one value or one value per model");
+  const size_t nl = levels.size();
+  if (nl != 1 && nl != 2 && nl != models && nl != 2 * models)
+    levels.bad(mvamd::kErrInvalidCardinality, -1, "levels need one highest level or a pair, shared or per model");
   Controls out;
-  out.options.reserve(count);
-  for (size_t i = 0; i < count; ++i) {
-    mvamd::FindOptions q;
-    q.start = number(start, i, -std::numeric_limits<double>::max(), std::numeric_limits<double>::max());
-    q.extent = number(extent, i, 0, 6.28318530717958647692);
-    q.min_score = number(score, i, 0, 1);
-    q.max_overlap = number(overlap, i, 0, 1);
-    const size_t mi = matches.size() == 1 ? 0 : i;
-    const int64_t quota = matches.integer(mi);
-    if (quota < 0) matches.bad(mvamd::kErrInvalidValue, int64_t(mi), "match count must be nonnegative");
-    q.num_matches = size_t(quota);
-    const size_t si = subpixel.size() == 1 ? 0 : i;
-    const std::string& mode = subpixel.string(si);
-    if (mode != "true" && mode != "false") subpixel.bad(mvamd::kErrUnsupportedValue, int64_t(si), "expected true or false");
-    q.subpixel = mode == "true";
-    size_t high_index = 0, low_index = 0;
-    bool pair = false;
-    if (levels.size() == count && count > 1) high_index = i;
-    else if (levels.size() == 2 * count) { high_index = 2 * i; low_index = high_index + 1; pair = true; }
-    else if (levels.size() == 2) { low_index = 1; pair = true; }
-    const int64_t highest = levels.integer(high_index), lowest = pair ? levels.integer(low_index) : 1;
-    if (highest < 0 || highest > std::numeric_limits<int>::max() || lowest < 0 || lowest > std::numeric_limits<int>::max())
-      levels.bad(mvamd::kErrInvalidValue, int64_t(high_index), "invalid highest/lowest level pair");
+  out.options.resize(models);
+  for (size_t i = 0; i < models; ++i) {
+    mvamd::FindOptions& q = out.options[i];
+    for (const ValueForm form : {ValueForm::kReal, ValueForm::kCount, ValueForm::kKeyword})
+      for (size_t c = 0; c < kValueControls; ++c)
+        if (kFindControls[c].form == form) kFindControls[c].store(q, control_value(in[c], kFindControls[c], i));
+    const LevelSlots at = level_slots(nl, models, i);
+    const int64_t highest = levels.integer(at.highest);
+    const int64_t lowest = at.paired ? levels.integer(at.lowest) : 1;
+    constexpr int64_t kIntMax = std::numeric_limits<int>::max();
+    if (highest < 0 || highest > kIntMax || lowest < 0 || lowest > kIntMax)
+      levels.bad(mvamd::kErrInvalidValue, int64_t(at.highest), "pyramid levels out of range");
     q.highest = int(highest);
     q.lowest = int(std::max<int64_t>(1, lowest));
-    out.options.push_back(q);
   }
-  out.selection.num_matches = matches.size() == 1 ? int64_t(out.options.front().num_matches) : -1;
-  out.selection.max_overlap = overlap.size() == 1 ? out.options.front().max_overlap : -1;
+  // one-value NumMatches / MaxOverlap apply to the whole selection (mv_ncc_matching.h:96-100)
+  out.selection.num_matches = in[3].size() == 1 ? int64_t(out.options.front().num_matches) : -1;
+  out.selection.max_overlap = in[4].size() == 1 ? out.options.front().max_overlap : -1;
   return out;
-}
-mvamd::FindOptions controls(const char* op, MvTuple a, MvTuple e, MvTuple s, MvTuple n, MvTuple o, MvTuple p, MvTuple l) {
-  return controls_n(op, 1, a, e, s, n, o, p, l).options.front();
 }
 
 // ModelIDs of find_ncc_models (ncc_matching_api.cpp:111-124): 1..4096 handles.
@@ -330,9 +413,9 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
   q.highest = o->highest_level; q.lowest = std::max(1, o->lowest_level);
   q.form = o->form; q.screen = o->screen; q.profile = o->profile;
   q.track_mfma = o->vector_tracking == 0;
-  if (q.form < MVAMD_FORM_FP32 || q.form > MVAMD_FORM_FP8_SCREEN)
+  if (q.form < MVAMD_FORM_AUTO || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
-  if (o->pixel_type < 0 || o->pixel_type > 2)
+  if (o->pixel_type != MVAMD_PIXEL_BYTE && o->pixel_type != MVAMD_PIXEL_UINT2)
     fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, "find_ncc_model", "pixel_type", "unsupported pixel type");
   return q;
 }
@@ -763,51 +846,83 @@ bool have_device() {
   return present;
 }
 
-// mv_create_ncc_model with a choice of raster builder (MVAMD_CREATE_*).
+// create_ncc_model's controls (the rules of ncc_matching_api.cpp:29-66).
+// Every control holds exactly one value; NumLevels and AngleStep may hold the
+// keyword "auto" instead of a number, Metric one of the metric names.  A
+// control is read in table order, so the first fault reported is mvkit's.
+struct CreateControl {
+  const char* param;
+  bool may_be_auto;
+  bool integral;       // the value must be an integer element
+  double lo, hi;       // closed range of accepted values
+};
+constexpr CreateControl kLevelsControl{"NumLevels", true, true, 0, 16};
+constexpr CreateControl kStartControl{"AngleStart", false, false, -kLargest, kLargest};
+constexpr CreateControl kExtentControl{"AngleExtent", false, false, -kLargest, kLargest};
+constexpr CreateControl kStepControl{"AngleStep", true, false, 0, 3.14159265358979323846 / 16};
+// Metric names in the order of CreateParams::absolute (false, true).
+constexpr const char* kMetrics[] = {"use_polarity", "ignore_global_polarity"};
+
+// The control's number, or nothing for "auto".
+std::optional<double> create_value(MvTuple t, const char* op, const CreateControl& c) {
+  const Input in(t, op, c.param);
+  if (in.size() != 1) in.bad(mvamd::kErrInvalidCardinality, -1, "control takes a single value");
+  if (c.may_be_auto && in.kind(0) == MV_VALUE_STRING) {
+    if (in.string(0) != "auto") in.bad(mvamd::kErrUnsupportedValue, 0, "the only keyword accepted here is 'auto'");
+    return std::nullopt;
+  }
+  const double x = c.integral ? double(in.integer(0)) : in.number(0);
+  if (!std::isfinite(x)) in.bad(mvamd::kErrInvalidValue, 0, "value is not a finite number");
+  if (x < c.lo || x > c.hi) in.bad(mvamd::kErrInvalidValue, 0, "value lies outside the control's range");
+  return x;
+}
+
+mvamd::CreateParams create_controls(const char* op, MvTuple levels, MvTuple start, MvTuple extent, MvTuple step,
+                                    MvTuple metric) {
+  mvamd::CreateParams p;
+  if (const auto n = create_value(levels, op, kLevelsControl)) p.levels = int(*n);
+  p.start = *create_value(start, op, kStartControl);
+  p.extent = *create_value(extent, op, kExtentControl);
+  if (const auto s = create_value(step, op, kStepControl)) p.step = *s;
+  const Input m(metric, op, "Metric");
+  if (m.size() != 1) m.bad(mvamd::kErrInvalidCardinality, -1, "control takes a single value");
+  const std::string& name = m.string(0);
+  const auto* hit = std::find_if(std::begin(kMetrics), std::end(kMetrics), [&](const char* k) { return name == k; });
+  if (hit == std::end(kMetrics)) m.bad(mvamd::kErrUnsupportedValue, 0, "not an NCC metric name");
+  p.absolute = hit != std::begin(kMetrics);
+  return p;
+}
+
+// Where the rotated rasters of a new model are built (MVAMD_CREATE_*).
+bool build_on_device(int32_t builder, const char* op) {
+  switch (builder) {
+    case MVAMD_CREATE_HOST: return false;
+    case MVAMD_CREATE_AUTO: return have_device();
+    case MVAMD_CREATE_DEVICE:
+      if (!have_device()) fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "builder", "no HIP device present");
+      return true;
+    default: fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, op, "builder", "unknown raster builder");
+  }
+}
+
 MvStatus create_model_handle(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
                              MvTuple AngleStep, MvTuple Metric, int32_t builder, MvTuple* ModelID) {
   return translate([&] {
     constexpr auto op = "create_ncc_model";
     Outputs<1> out(op, {ModelID});
-    if (builder < MVAMD_CREATE_AUTO || builder > MVAMD_CREATE_DEVICE)
-      fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, op, "builder", "unknown raster builder");
-    if (builder == MVAMD_CREATE_DEVICE && !have_device())
-      fail(mvamd::kUnsupported, mvamd::kErrUnsupportedValue, op, "builder", "no HIP device for the device builder");
-    const bool on_device = builder == MVAMD_CREATE_DEVICE || (builder == MVAMD_CREATE_AUTO && have_device());
-    const MvObjectImpl& src = single_image(Template, op, "Template");
-    const Input levels(NumLevels, op, "NumLevels"), start(AngleStart, op, "AngleStart"),
-        extent(AngleExtent, op, "AngleExtent"), step(AngleStep, op, "AngleStep"), metric(Metric, op, "Metric");
-    auto automatic = [](const Input& in) {
-      if (in.size() != 1) in.bad(mvamd::kErrInvalidCardinality, -1, "exactly one value required");
-      if (in.kind(0) != MV_VALUE_STRING) return false;
-      if (in.string(0) != "auto") in.bad(mvamd::kErrUnsupportedValue, 0, "expected auto");
-      return true;
-    };
-    mvamd::CreateParams p;
-    if (!automatic(levels)) {
-      const int64_t n = levels.scalar_integer();
-      if (n < 0 || n > 16) levels.bad(mvamd::kErrInvalidValue, 0, "levels must be 0..16 or auto");
-      p.levels = int(n);
-    }
-    p.start = start.scalar_number();
-    p.extent = extent.scalar_number();
-    if (!automatic(step)) {
-      p.step = step.scalar_number();
-      if (p.step < 0 || p.step > 3.14159265358979323846 / 16) step.bad(mvamd::kErrInvalidValue, 0, "angle step must be in [0,pi/16] or auto");
-    }
-    if (metric.size() != 1) metric.bad(mvamd::kErrInvalidCardinality, -1, "exactly one metric required");
-    const std::string& name = metric.string(0);
-    if (name == "ignore_global_polarity") p.absolute = true;
-    else if (name != "use_polarity") metric.bad(mvamd::kErrUnsupportedValue, 0, "unsupported NCC metric");
-    std::shared_ptr<mvamd::NccModel> m = algorithm(op, [&]() -> std::unique_ptr<mvamd::NccModel> {
+    const bool on_device = build_on_device(builder, op);
+    const MvObjectImpl& tmpl = single_image(Template, op, "Template");
+    const mvamd::CreateParams p = create_controls(op, NumLevels, AngleStart, AngleExtent, AngleStep, Metric);
+    const size_t pitch = tmpl.width * size_t(tmpl.pixel_bytes);
+    std::shared_ptr<mvamd::NccModel> model = algorithm(op, [&]() -> std::unique_ptr<mvamd::NccModel> {
       if (!on_device)
-        return mvamd::create_model(src.pixels, src.pixel_bytes, src.width, src.height, src.width * src.pixel_bytes,
-                                   p, src.domain.get());
-      mvamd::FirstLevel f = mvamd::create_first_level(src.pixels, src.pixel_bytes, src.width, src.height,
-                                                      src.width * src.pixel_bytes, p, src.domain.get());
-      return context().rebuild(std::move(f.level), p, f.orow, f.ocol);
+        return mvamd::create_model(tmpl.pixels, tmpl.pixel_bytes, tmpl.width, tmpl.height, pitch, p, tmpl.domain.get());
+      // the O(area) first level on the host, the rotated rasters on the device
+      mvamd::FirstLevel first = mvamd::create_first_level(tmpl.pixels, tmpl.pixel_bytes, tmpl.width, tmpl.height,
+                                                          pitch, p, tmpl.domain.get());
+      return context().rebuild(std::move(first.level), p, first.orow, first.ocol);
     });
-    put_handle(out[0], std::move(m));
+    put_handle(out[0], std::move(model));
     out.commit();
   });
 }
@@ -849,7 +964,9 @@ extern "C" MvStatus mv_find_ncc_model(MvObject Image, MvTuple ModelID, MvTuple A
     required(Image, op, "Image");
     require_kind(Image, MvObjectImpl::kImage, op);
     const auto m = model_of(ModelID, op);
-    const mvamd::FindOptions q = controls(op, AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel, NumLevels);
+    const mvamd::FindOptions q =
+        find_controls(op, 1, {AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel}, NumLevels)
+            .options.front();
     if (Image->empty) { out.commit(); return; }
     const size_t pitch = Image->width * size_t(Image->pixel_bytes);
     const auto found = algorithm(op, [&] {
@@ -878,8 +995,8 @@ extern "C" MvStatus mv_find_ncc_models(MvObject Image, MvTuple ModelIDs, MvTuple
     required(Image, op, "Image");
     require_kind(Image, MvObjectImpl::kImage, op);
     const auto ms = models_of(ModelIDs, op);
-    const Controls c = controls_n(op, ms.size(), AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel,
-                                  NumLevels);
+    const Controls c =
+        find_controls(op, ms.size(), {AngleStart, AngleExtent, MinScore, NumMatches, MaxOverlap, SubPixel}, NumLevels);
     if (Image->empty) { out.commit(); return; }
     std::vector<const mvamd::NccModel*> ptrs;
     for (const auto& m : ms) ptrs.push_back(m.get());
@@ -1032,6 +1149,7 @@ void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
   s->coarse_flops = t.coarse_flops;
   s->track_flops = t.track_flops;
   s->track_mfma_flops = t.track_mfma_flops;
+  s->coarse_form = t.coarse_form;
 }
 void put_poses(const std::vector<mvamd::Pose>& v, double* dst, size_t cap) {
   for (size_t i = 0; i < v.size() && i < cap; ++i) {
@@ -1246,14 +1364,15 @@ extern "C" MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* i
     constexpr auto op = "find_ncc_model";
     required(image, op, "Image");
     required(count, op, "count");
-    if (width <= 0 || height <= 0 || pitch < width)
-      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
     const auto m = model_of(ModelID, op);
     mvamd::FindOptions q = from_ext(options);
     q.trace_stages = true;
+    const int bytes = options->pixel_type == MVAMD_PIXEL_UINT2 ? 2 : 1;
+    if (width <= 0 || height <= 0 || pitch < width * bytes || pitch % bytes)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
     mvamd::FindTrace trace;
     const auto found = algorithm(op, [&] {
-      return context().find_host(*m, image, size_t(width), size_t(height), size_t(pitch), q, &trace);
+      return context().find_host(*m, image, size_t(width), size_t(height), size_t(pitch), q, &trace, bytes);
     });
     if (results) put_poses(found, results, cap);
     *count = found.size();

@@ -439,6 +439,18 @@ FindContext::~FindContext() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+void FindContext::reset() {
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipStreamSynchronize(stream2_), "sync");
+  for (hipStream_t q : slot_streams_) check(hipStreamSynchronize(q), "sync");
+  for (auto& sl : slots_) {
+    sl->busy = false;
+    sl->traces = nullptr;
+    for (auto& J : sl->jobs) J.trace = nullptr;
+  }
+  cur_ = nullptr;
+}
+
 void FindContext::time_begin(int stage, hipStream_t stream) {
   Slot& S = *cur_;
   S.timing = S.profiling & (1 << stage);
@@ -656,6 +668,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     max_bw = std::max(max_bw, p.cmax - p.cmin);
   }
 
+  if (o.form == kFormAuto) o.form = wide ? NCCHIP_FORM_FP32 : NCCHIP_FORM_BF16X3;
   if (o.form < NCCHIP_FORM_FP32 || o.form > NCCHIP_FORM_FP8_SCREEN) raise(kInvalidArgument, "form", "unknown correlation form");
   if (wide && (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN))
     raise(kUnsupported, "form", "the split-BF16 and FP8-screened forms need byte search images");
@@ -672,8 +685,27 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     // the FP32 form holds S - C exactly in FP32 up to level 4
     if (o.form == NCCHIP_FORM_FP32 && top > 4) o.form = NCCHIP_FORM_FP64;
   }
-  const bool use_mfma = o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN;
+  // the matrix-core tracking kernel goes with the split-BF16 and FP8-screened
+  // forms; the plain FP32 and FP64 forms keep the whole search off matrix cores
+  const bool track_on_mfma = o.track_mfma && (o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN);
+  bool use_mfma = o.form == NCCHIP_FORM_BF16X3 || o.form == NCCHIP_FORM_FP8_SCREEN;
   DenseWeights dense;
+  if (use_mfma) {
+    // the union bounding box of the coarse patches must fit the kernel's LDS
+    // tile at this level; otherwise the coarse search runs the FP32 form
+    int ymin = 1 << 30, ymax = -(1 << 30), xmin = 1 << 30, xmax = -(1 << 30);
+    for (size_t i = 0; i < A; ++i) {
+      const Raster& p = raster_of(refs[top][i]);
+      ymin = std::min(ymin, p.rmin); ymax = std::max(ymax, p.rmax);
+      xmin = std::min(xmin, p.cmin); xmax = std::max(xmax, p.cmax);
+    }
+    const int top_kind = wide || top > 4 ? NCC_U32 : (top ? NCC_U16 : NCC_U8);
+    if (!ncchip_coarse_mfma_fits(int(top), top_kind, ymax - ymin + 1, (xmax - xmin + 16) / 16,
+                                 o.form == NCCHIP_FORM_FP8_SCREEN)) {
+      o.form = NCCHIP_FORM_FP32;
+      use_mfma = false;
+    }
+  }
   if (use_mfma) {
     std::vector<const Raster*> ps;
     std::vector<const RasterPool*> pp;
@@ -990,7 +1022,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
     // matrix cores where every query patch has a dense raster and at most 32
     // patches fall in any candidate's angle window
-    bool mfma = o.track_mfma && l - 1 >= 1 && l - 1 <= 4 && jobs[0].lv[l - 1].kind == NCC_U16;
+    bool mfma = track_on_mfma && l - 1 >= 1 && l - 1 <= 4 && jobs[0].lv[l - 1].kind == NCC_U16;
     if (mfma) {
       std::vector<double> ang;
       for (const int32_t r : refs[l - 1]) {
@@ -1208,10 +1240,17 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
   Slot& S = *slots_[ticket % 2];
   if (!S.busy || S.ticket != ticket) raise(kInvalidState, "batch", "no such pending batch");
   cur_ = &S;
+  // On every exit, including an error between refinement launches, the
+  // slot's staging buffers are free again only once the refinement stream
+  // has drained (a launch queued before the error may still read them).
   struct Release {
     Slot& s;
-    ~Release() { s.busy = false; }
-  } release{S};
+    hipStream_t refine;
+    ~Release() {
+      (void)hipStreamSynchronize(refine);
+      s.busy = false;
+    }
+  } release{S, stream2_};
   const NccModel& model = *S.model;
   const FindOptions& o = S.o;
   const size_t n_frames = S.n_frames;
@@ -1277,6 +1316,7 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
       tr.coarse_flops = J.coarse_flops;
       tr.track_flops = J.track_flops;
       tr.track_mfma_flops = J.track_mfma_flops;
+      tr.coarse_form = o.form;
       tr.t_coarse_us = S.t1 - S.t0; tr.t_track_us = S.t2 - S.t1; tr.t_refine_us = t3 - S.t2; tr.t_total_us = now_us() - S.t0;
       if (f > 0 && traces) std::copy(traces[0].kernel_us, traces[0].kernel_us + kStageCount, tr.kernel_us);
     }

@@ -59,12 +59,18 @@ class DeviceModel {
 // Returns the model's copy on the current device, uploading it on first use.
 std::shared_ptr<DeviceModel> device_model(const NccModel& m, hipStream_t stream);
 
+constexpr int kFormAuto = -1;
+
 struct FindOptions {
   double start = 0, extent = 0, min_score = .8, max_overlap = .5;
   size_t num_matches = 1, max_candidates = 10000;
   bool subpixel = true;
   int highest = 0, lowest = 1;
-  int form = NCCHIP_FORM_FP32;     // coarse correlation form
+  // coarse correlation form (NCCHIP_FORM_*); kFormAuto: split-BF16 on byte
+  // images, FP32 on u16 images.  A split-BF16 or FP8-screened search whose
+  // top level or union bounding box does not fit the matrix-core kernel runs
+  // the FP32 form (FindTrace::coarse_form says which form ran).
+  int form = kFormAuto;
   double screen = -1.0 / 0.0;      // FP8 screen threshold (form NCCHIP_FORM_FP8_SCREEN; -inf: 0.55)
   int profile = 0;                 // bit s: time stage s (KernelStage) with HIP events on the context's stream
   bool trace_stages = false;       // fill FindTrace's stage lists (coarse, tracked, storage)
@@ -74,7 +80,9 @@ struct FindOptions {
   // find_models over several models: the whole level 0 is the storage region
   // and the recovery windows use its row prefix statistics (matcher.cpp:596-611, 991-1003)
   bool whole_level0 = false;
-  // tracking of levels 1..4 on matrix cores (exact results; off: FP64 VALU kernel only)
+  // tracking of levels 1..4 on matrix cores with the split-BF16 and
+  // FP8-screened forms (exact results; off: FP64 VALU kernel only).  The FP32
+  // and FP64 forms track on the FP64 VALU kernel.
   bool track_mfma = true;
 };
 
@@ -104,6 +112,7 @@ struct FindTrace {
   double coarse_flops = 0;             // 2 * sum over evaluated coarse windows of their sample counts
   double track_flops = 0;              // 2 * samples of the tracked 5x5 windows (counted when tracking is profiled)
   double track_mfma_flops = 0;         // the part of track_flops tracked on matrix cores
+  int coarse_form = -1;                // correlation form the coarse search ran (NCCHIP_FORM_*)
 };
 
 // A u8 or u16 search image in device memory (`pitch` bytes per row), with
@@ -163,6 +172,11 @@ class FindContext {
                                            size_t pitch, const Selection& selection,
                                            bool per_model_quota_before_overlap = true, int pixel_bytes = 1,
                                            const Region* domain = nullptr);
+
+  // Waits for all work queued by this context and drops its pending batches
+  // (their results are lost): the state of a context handed from a thread
+  // that ended to another thread.
+  void reset();
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }

@@ -441,12 +441,32 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
 
 using namespace ncc;
 
+namespace {
+// LDS bytes of one k_coarse_mfma block, beside its static tables; 0 when the
+// level cannot run in the split-BF16 form.
+size_t coarse_mfma_lds(int level, int level_kind, int yspan, int xchunks, bool screened, int* pitch, int* pitch8) {
+  if (level_kind == NCC_U32 || level > 4) return 0;  // T_lo exact in BF16 only up to level 4
+  const int cols = kBlockCols + 16 * xchunks + 1;
+  const int rows = kMRows + yspan - 1;
+  *pitch = (cols + 1) & ~1;
+  *pitch8 = (cols + 11) & ~3;  // an 8-byte A fragment read as three dwords from any column
+  size_t lds = size_t(5) * rows * *pitch * sizeof(uint16_t);
+  if (screened) lds += size_t(rows) * *pitch8;
+  constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(float) + 64 * 8);
+  return lds > kLdsMax ? 0 : lds;
+}
+}  // namespace
+
+extern "C" int ncchip_coarse_mfma_fits(int level, int level_kind, int yspan, int xchunks, int screened) {
+  int pitch = 0, pitch8 = 0;
+  return coarse_mfma_lds(level, level_kind, yspan, xchunks, screened != 0, &pitch, &pitch8) > 0;
+}
+
 extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w,
                                   int max_h, double C, const NccPatchPool* pools, const int32_t* patch_ids,
                                   int n_angles, const NcchipDenseWeights* dw, int exact_lattice, int absolute,
                                   double screen, void* stream) {
   if (n_frames <= 0 || n_angles <= 0) return 0;
-  if (level_kind == NCC_U32 || level > 4) return -6;   // T_lo exact in BF16 only up to level 4
   const bool screened = screen > -__builtin_inf();
   if (screened && !(dw->w8_scale > 0)) return -1;
   MfmaArgs a{};
@@ -458,21 +478,14 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   a.common_runs = reinterpret_cast<const int4*>(dw->common_runs); a.n_common_runs = dw->n_common_runs;
   a.n_common = dw->n_common; a.prune_below = dw->prune_below; a.peak_thr = dw->peak_thr;
   if (size_t(n_frames) * a.n_tiles > 65535) return -1;
-  const int cols = kBlockCols + 16 * a.xc + 1;
-  a.pitch = (cols + 1) & ~1;
-  const int rows = kMRows + a.yspan - 1;
-  a.pitch8 = (cols + 11) & ~3;  // an 8-byte A fragment read as three dwords from any column
-  size_t lds = size_t(5) * rows * a.pitch * sizeof(uint16_t);
+  const size_t lds = coarse_mfma_lds(level, level_kind, a.yspan, a.xc, screened, &a.pitch, &a.pitch8);
+  if (!lds) return -6;
   if (screened) {
-    lds += size_t(rows) * a.pitch8;
     a.screen = screen;
     a.img8_scale = float(std::ldexp(1.0, -2 * level));
     a.w8_scale = float(dw->w8_scale);
     a.dot8_unscale = 1.0 / (double(a.img8_scale) * double(a.w8_scale));
   }
-  // beside the static tables (bounds, survivor queue)
-  constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(float) + 64 * 8);
-  if (lds > kLdsMax) return -6;
   const dim3 grid((max_w + kBlockCols - 1) / kBlockCols, (max_h + kMRows - 1) / kMRows, unsigned(n_frames * a.n_tiles));
   hipStream_t st = static_cast<hipStream_t>(stream);
   // per-run sums of t^2 fit 32 bits when (longest run) * (128 * 4^level)^2 < 2^32

@@ -159,3 +159,71 @@ def test_gen_image_rejects_bad_type(capi):
     args = [capi.Tuple("real"), capi.Tuple(4), capi.Tuple(4), capi.Tuple(int(a.ctypes.data))]
     h = C.c_void_p()
     assert capi.lib().mv_gen_image1(C.byref(h), *[x.h for x in args]) == -6
+
+
+def test_find_control_fault_order(capi, scene_plain):
+    """When several controls are wrong the first fault reported is mvkit's:
+    cardinalities in argument order, then per model the ranged reals, the
+    count, the keyword and the levels (ncc_matching_api.cpp:126-161)."""
+    m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
+    img = capi.Image(scene_plain.image)
+    # NumMatches precedes MaxOverlap among the cardinalities ...
+    expect_error(capi, lambda: m.find(img, 0.0, 0.0, 0.8, [1, 2], [0.5, 0.5]), -1, code=2, parameter="NumMatches")
+    # ... while MaxOverlap's value is checked before NumMatches' value
+    expect_error(capi, lambda: m.find(img, 0.0, 0.0, 0.8, -1, 2.0), -1, code=4, parameter="MaxOverlap")
+    # the keyword after the count, the levels last
+    expect_error(capi, lambda: m.find(img, 0.0, 0.0, 0.8, -1, 0.5, "maybe"), -1, code=4, parameter="NumMatches")
+    expect_error(capi, lambda: m.find(img, 0.0, 0.0, 0.8, 1, 0.5, "maybe", -1), -1, code=7, parameter="SubPixel")
+    e = expect_error(capi, lambda: m.find(img, 0.0, 0.0, 0.8, 1, 0.5, "true", [2, -1]), -1, code=4,
+                     parameter="NumLevels")
+    assert e.parameter_index == 0
+    # an integer control given as a real is a type mismatch at its index
+    e = expect_error(capi, lambda: m.find(img, 0.0, 0.0, 0.8, 1.0), -3, code=3, parameter="NumMatches")
+    assert e.parameter_index == 0
+
+
+def test_find_models_control_indices(capi, scene_plain):
+    m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
+    img = capi.Image(scene_plain.image)
+    e = expect_error(capi, lambda: capi.find_models([m, m], img, 0.0, [0.0, 7.0], 0.8, 1), -1, code=4,
+                     parameter="AngleExtent")
+    assert e.parameter_index == 1
+    e = expect_error(capi, lambda: capi.find_models([m, m], img, 0.0, 0.0, 0.8, 1, 0.5, "true", [1, 2, 3, -4]),
+                     -1, code=4, parameter="NumLevels")
+    assert e.parameter_index == 2
+    expect_error(capi, lambda: capi.find_models([m, m], img, 0.0, 0.0, 0.8, 1, 0.5, "true", [1, 2, 3]), -1, code=2,
+                 parameter="NumLevels")
+
+
+def test_create_control_fault_order(capi, scene_plain):
+    t = scene_plain.template
+    # NumLevels is read before the angles, AngleStep before Metric
+    expect_error(capi, lambda: capi.NccModel(t, 17, math.inf), -1, code=4, parameter="NumLevels")
+    expect_error(capi, lambda: capi.NccModel(t, 3, math.inf), -1, code=4, parameter="AngleStart")
+    expect_error(capi, lambda: capi.NccModel(t, 3, 0.0, 0.0, "sometimes", "sad"), -1, code=7, parameter="AngleStep")
+    expect_error(capi, lambda: capi.NccModel(t, 3.0), -3, code=3, parameter="NumLevels")
+    expect_error(capi, lambda: capi.NccModel(t, [3, 4]), -1, code=2, parameter="NumLevels")
+    expect_error(capi, lambda: capi.NccModel(t, 3, 0.0, 0.0, "auto", 5), -3, code=3, parameter="Metric")
+    expect_error(capi, lambda: capi.NccModel(t, 3, 0.0, 0.0, "auto", "sad", builder=7), -1, code=7,
+                 parameter="builder")
+
+
+def test_extension_pixel_type_checks(capi, scene_plain):
+    """Only MVAMD_PIXEL_BYTE and MVAMD_PIXEL_UINT2 are pixel types; the trace
+    entry checks the pitch against the pixel size (both before any device
+    work)."""
+    m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
+    opts = capi.NccModel.options(0.0, 0.0, 0.8, 1, pixel_type=1)
+    expect_error(capi, lambda: m.find_device(0x1000, 64, 64, 64, opts), -6, code=7, parameter="pixel_type")
+    opts = capi.NccModel.options(0.0, 0.0, 0.8, 1, form=-2)
+    expect_error(capi, lambda: m.find_device(0x1000, 64, 64, 64, opts), -1, parameter="form")
+    img = np.zeros((16, 33), np.uint8)
+    res = np.zeros((4, 4)); n = C.c_size_t()
+    st = capi.FindStats()
+    o16 = capi.NccModel.options(0.0, 0.0, 0.8, 1, pixel_type=capi.PIXEL_UINT2)
+    lib = capi.lib()
+    for w, pitch in ((16, 31), (17, 33)):  # odd pitch, pitch short of a u16 row
+        status = lib.mvamd_find_ncc_model_trace(m.tuple.h, img.ctypes.data_as(C.c_void_p), w, 16, pitch,
+                                                C.byref(o16), res.ctypes.data_as(C.c_void_p), 4, C.byref(n), None,
+                                                0, None, None, None, 0, None, C.byref(st))
+        assert status == -1, (w, pitch, status)
