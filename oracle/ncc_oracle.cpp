@@ -1725,6 +1725,42 @@ int orc_find(void* mp, const uint8_t* img, int w, int h, int stride, int pixel_b
   });
 }
 
+// The tracked candidate lists of a find on a u8 image (full domain), one per
+// tracking level in tracking order, each in merge_identical's order
+// (row, col, angle), concatenated into out (cap x 4); counts[k] is the size
+// of list k (at most max_lists lists; *n_lists receives their number).
+int orc_find_tracked(void* mp, const uint8_t* img, int w, int h, int stride, double start, double extent,
+                     double min_score, int num_matches, double max_overlap, int subpixel, int highest, int lowest,
+                     double* out, int cap, int* counts, int max_lists, int* n_lists) {
+  return guard([&] {
+    FindOpts o; o.start = start; o.extent = extent; o.min_score = min_score; o.num_matches = size_t(num_matches);
+    o.max_overlap = max_overlap; o.subpixel = subpixel != 0; o.highest = highest; o.lowest = lowest;
+    Trace tr;
+    find(img, size_t(w), size_t(h), size_t(stride), *static_cast<Model*>(mp), o, &tr, 1, nullptr);
+    int k = 0;
+    for (size_t l = 0; l < tr.tracked.size() && int(l) < max_lists; ++l) {
+      counts[l] = int(tr.tracked[l].size());
+      for (const Match& m : tr.tracked[l]) {
+        if (k < cap) { out[4 * k] = m.row; out[4 * k + 1] = m.col; out[4 * k + 2] = m.angle; out[4 * k + 3] = m.score; }
+        ++k;
+      }
+    }
+    *n_lists = int(tr.tracked.size());
+  });
+}
+
+// The candidate list after merge_identical (matcher.cpp:241-248): in (n x 4),
+// out receives the merged list; returns its length.
+int orc_merge_identical(const double* in, int n, double* out) {
+  std::vector<Match> c(size_t(std::max(n, 0)));
+  for (int i = 0; i < n; ++i) c[size_t(i)] = Match{in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3]};
+  merge_identical(c);
+  for (size_t i = 0; i < c.size(); ++i) {
+    out[4 * i] = c[i].row; out[4 * i + 1] = c[i].col; out[4 * i + 2] = c[i].angle; out[4 * i + 3] = c[i].score;
+  }
+  return int(c.size());
+}
+
 // Textbook ZNCC on two sample arrays (normalized_correlation, model.cpp:105-118).
 double orc_zncc(const double* a, const double* b, int n, int absolute) {
   double ma = 0, mb = 0;

@@ -56,6 +56,8 @@ def lib():
         L.orc_zncc.restype = D
         L.orc_overlap.argtypes = [D, D, D, D, D, D, I, I, D, D]
         L.orc_overlap.restype = D
+        L.orc_find_tracked.argtypes = [P, P, I, I, I, D, D, D, I, D, I, I, I, P, I, P, I, C.POINTER(I)]
+        L.orc_merge_identical.argtypes = [P, I, P]
         L.orc_continuous.argtypes = [P, P, I, I, I, D, D, D]
         L.orc_continuous.restype = D
         L.orc_refine_one.argtypes = [P, P, I, I, I, D, D, D, I, D, I, P]
@@ -174,6 +176,26 @@ class Model:
         return out, {"coarse": coarse[:info.n_coarse].copy(), "prepared": prep[:info.n_prepared].copy(),
                      "refined": refd[:info.n_prepared].copy(), "info": info}
 
+    def tracked(self, image: np.ndarray, start: float, extent: float, min_score: float, num_matches: int,
+                max_overlap: float, subpixel: bool = True, highest: int = 0, lowest: int = 1) -> list:
+        """The candidate list after each tracking level of a find on a u8
+        image (each in (row, col, angle) order)."""
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        cap, lists = 100000, 16
+        out = np.zeros((cap, 4))
+        counts = np.zeros(lists, np.int32)
+        n = C.c_int()
+        st = lib().orc_find_tracked(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], start, extent,
+                                    min_score, num_matches, max_overlap, int(subpixel), highest, lowest, _ptr(out),
+                                    cap, _ptr(counts), lists, C.byref(n))
+        if st:
+            raise OracleError(st, lib().orc_last_error().decode())
+        res, k = [], 0
+        for i in range(n.value):
+            res.append(out[k:k + counts[i]].copy())
+            k += counts[i]
+        return res
+
     def continuous(self, image: np.ndarray, row: float, col: float, angle: float) -> float:
         img = np.ascontiguousarray(image, dtype=np.uint8)
         return lib().orc_continuous(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], row, col, angle)
@@ -271,6 +293,14 @@ def find_models(models: list, image: np.ndarray, opts: list, selection_matches: 
     if not prepared:
         return out
     return out, [prep[i, :prep_n[i]].copy() for i in range(n)]
+
+
+def merge_identical(cand: np.ndarray) -> np.ndarray:
+    """The oracle's merge_identical of an (n, 4) candidate list."""
+    c = np.ascontiguousarray(cand, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros((max(len(c), 1), 4))
+    k = lib().orc_merge_identical(_ptr(c), len(c), _ptr(out))
+    return out[:k].copy()
 
 
 def box_overlap(a, ga, b, gb) -> float:
