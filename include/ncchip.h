@@ -35,6 +35,18 @@ int ncchip_pyramid_level(const void* src, int src_kind, int src_pitch, int src_w
 int ncchip_row_prefix(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src, int shift,
                       uint64_t* p1, uint64_t* p2, uint32_t* nz, uint32_t* support, void* stream);
 
+/* ncchip_row_prefix without the `nz` workspace argument and without any
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The support indicator's row prefixes go
+ * to a static device array of 2^22 u32 words (16 MiB) per GPU, held until
+ * the process ends, so (w + 1) * h must not exceed 4194304 (else -103,
+ * LIMIT_EXCEEDED).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other host threads) race.  Outputs are
+ * those of ncchip_row_prefix. */
+int ncchip_row_prefix_noalloc(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src, int shift,
+                              uint64_t* p1, uint64_t* p2, uint32_t* support, void* stream);
+
 /* Batched front end.  `frames` (device) holds one NccFrame per frame of a
  * batch (at most NCC_MAX_BATCH); each batched kernel covers every frame in
  * one launch, sized by the largest frame (max_w x max_h of the level

@@ -42,7 +42,7 @@ NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_b
                   "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma", "ncchip_coarse_mfma_fits",
                   "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine",
                   "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap", "ncchip_create_rasters",
-                  "ncchip_pyramid3_u8"]
+                  "ncchip_pyramid3_u8", "ncchip_row_prefix_noalloc"]
 
 
 class StringView(C.Structure):

@@ -3,6 +3,7 @@
 // All results are bit-exact images of the reference's FP64 values
 // (matcher.cpp:401-455, 487-495, 498-503).
 #include <algorithm>
+#include <mutex>
 
 #include "ncc_common.hip.h"
 #include "../../include/ncchip.h"
@@ -367,4 +368,45 @@ extern "C" int ncchip_row_prefix(const void* src, int kind, int pitch, int w, in
   else hipLaunchKernelGGL((k_row_prefix<uint32_t>), dim3(h), dim3(256), 0, st, static_cast<const uint32_t*>(src), pitch, w, h, dc_src, shift, p1, p2, nz);
   if (support) hipLaunchKernelGGL(k_support_cols, dim3((w + 1 + 255) / 256), dim3(256), 0, st, nz, w, h, support);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+// ------------------------------------------------------------------------
+// Row prefixes without allocation.  The support indicator's row prefixes
+// (the `nz` workspace of ncchip_row_prefix) live in a file-scope device
+// array: one copy per GPU, held until the process ends.  Host code never
+// uses the array's name as an address; the address of each device's copy
+// comes from hipGetSymbolAddress, taken once per device with that device
+// current.
+namespace {
+constexpr size_t kNoAllocWords = size_t(1) << 22;  // 16 MiB: (w + 1) * h <= 4194304
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (null when the device cannot be
+// queried or the symbol has no address on it).
+uint32_t* device_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" int ncchip_row_prefix_noalloc(const void* src, int kind, int pitch, int w, int h, const uint8_t* dc_src,
+                                         int shift, uint64_t* p1, uint64_t* p2, uint32_t* support, void* stream) {
+  if (w <= 0 || h <= 0 || !src || !dc_src || !p1 || !p2) return -1;
+  if (size_t(w + 1) * size_t(h) > kNoAllocWords) return -103;  // beyond the static workspace
+  uint32_t* nz = device_scratch();
+  if (!nz) return -1000;
+  // every (w + 1) * h word of nz is written by the row pass before the
+  // column pass reads it, so nothing of an earlier call needs clearing
+  return ncchip_row_prefix(src, kind, pitch, w, h, dc_src, shift, p1, p2, nz, support, stream);
 }
