@@ -37,6 +37,9 @@ typedef struct MvAmdFindOptions {
                                             (0: matrix cores where they apply; same results) */
   int32_t pixel_type;                    /* MVAMD_PIXEL_*: pixel type of the device images
                                             (pitches in bytes) */
+  int64_t max_candidates;                /* coarse candidate budget: the reference's internal
+                                            FindOptions::max_candidates (model.hpp:21), which its
+                                            C ABI leaves at 10 000; 0 selects 10 000 */
 } MvAmdFindOptions;
 
 /* Pixel types of device-resident search images ("byte" and "uint2" in the
@@ -97,6 +100,23 @@ MV_API MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t* d_
                                              int64_t pitch, const MvAmdFindOptions* options, size_t n_options,
                                              int64_t selection_matches, double selection_overlap, double* results,
                                              size_t cap, size_t* count);
+
+/* find_ncc_models over n_images u8 (or u16) images given as HIP device
+ * pointers, with the models, options and selection of
+ * mvamd_find_ncc_models_device; image k's matches go to results + 5*cap*k
+ * (row, col, angle, score, model index) and their number to counts[k], each
+ * image's result the one mvamd_find_ncc_models_device returns for it.  One
+ * query per distinct (model, options) covers all images (one launch per
+ * stage); coarse_queue != 0 runs the split-BF16 coarse searches of every
+ * query in one persistent launch fed by an (image, angle tile, block)
+ * work queue (0: one launch per query).  At most 256 images.
+ * Replaces, for many device-resident images, mv_find_ncc_models
+ * (mvkit include/mvkit/c/mv_ncc_matching.h:119-123) per image. */
+MV_API MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t* const* d_images, const int64_t* widths,
+                                            const int64_t* heights, const int64_t* pitches, size_t n_images,
+                                            const MvAmdFindOptions* options, size_t n_options,
+                                            int64_t selection_matches, double selection_overlap, int32_t coarse_queue,
+                                            double* results, size_t cap, size_t* counts);
 
 /* Finds the model in n u8 (or u16) images given as HIP device pointers, all
  * with the same options, stage by stage over the whole batch (one launch per stage for

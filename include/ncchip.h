@@ -131,6 +131,35 @@ int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int leve
                        const NcchipDenseWeights* weights, int exact_lattice, int absolute, double screen,
                        void* stream);
 
+/* The split-BF16 coarse searches of several queries (models of uneven
+ * template size and angle count, over frames of uneven size) in one
+ * persistent launch per kernel instantiation, fed by an (image, angle tile,
+ * block row, block column) work queue: each job holds the arguments of one
+ * ncchip_coarse_mfma call, and the planes written are the ones that call
+ * writes.  frame_dims (host, optional) gives each frame's width and height at
+ * the searched level, so that the queue holds the frame's own blocks only.
+ * d_work (device) and h_stage (pinned host, read by an asynchronous copy
+ * that must complete before h_stage is reused) hold
+ * ncchip_coarse_mfma_queue_bytes(n_jobs, total frames) bytes.  Returns -6
+ * when a job does not fit the kernel (see ncchip_coarse_mfma_fits), -103
+ * when the buffers are too small.  Replaces, for a batch of queries, the
+ * coarse loop of prepare() (matcher.cpp:542-579). */
+typedef struct NcchipCoarseJob {
+  const NccFrame* frames;     /* device */
+  int32_t n_frames, level, level_kind, max_w, max_h;
+  const int32_t* frame_dims;  /* host: (w, h) per frame, or null */
+  double C;
+  NccPatchPool pools[2];
+  const int32_t* patch_ids;   /* device */
+  int32_t n_angles;
+  NcchipDenseWeights weights;
+  int32_t exact_lattice, absolute;
+  double screen;
+} NcchipCoarseJob;
+size_t ncchip_coarse_mfma_queue_bytes(int n_jobs, int n_frames_total);
+int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs, void* d_work, size_t work_bytes, void* h_stage,
+                             size_t stage_bytes, void* stream);
+
 /* Per-angle 3x3 peaks of every frame's planes with the reference's tie rule
  * (matcher.cpp:580-594), emitted as candidate poses (matcher.cpp:590-592:
  * row = crop_r + scale * (r + origin_r) + phase) into frame f's list at

@@ -34,12 +34,13 @@ PUBLIC_SYMBOLS = [
     "mv_clear_handle", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
     "mv_get_serialized_item_ptr", "mv_write_ncc_model", "mv_read_ncc_model", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
     "mvamd_create_ncc_model", "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_image",
-    "mvamd_find_ncc_models_device", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
+    "mvamd_find_ncc_models_device", "mvamd_find_ncc_models_batch", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
                   "ncchip_fill_u32", "ncchip_copy_u32", "ncchip_coarse", "ncchip_coarse_mfma", "ncchip_coarse_mfma_fits",
+                  "ncchip_coarse_mfma_queue", "ncchip_coarse_mfma_queue_bytes",
                   "ncchip_peaks", "ncchip_storage0", "ncchip_track", "ncchip_track_mfma", "ncchip_full_score", "ncchip_refine",
                   "ncchip_domain_sat", "ncchip_domain_planes", "ncchip_domain_snap", "ncchip_create_rasters",
                   "ncchip_pyramid3_u8", "ncchip_row_prefix_noalloc"]
@@ -62,7 +63,7 @@ class FindOptions(C.Structure):
                 ("max_overlap", C.c_double), ("num_matches", C.c_int64), ("subpixel", C.c_int32),
                 ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
                 ("screen", C.c_double), ("profile", C.c_int32), ("vector_tracking", C.c_int32),
-                ("pixel_type", C.c_int32)]
+                ("pixel_type", C.c_int32), ("max_candidates", C.c_int64)]
 
 
 class FindStats(C.Structure):
@@ -138,6 +139,8 @@ def lib() -> C.CDLL:
         L.mv_get_serialized_item_ptr.argtypes = [P, C.POINTER(P), C.POINTER(P)]
         L.mvamd_find_ncc_models_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), S, I64, D, P, S,
                                                    C.POINTER(S)]
+        L.mvamd_find_ncc_models_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), S, I64, D, I32, P, S, P]
+        L.ncchip_coarse_mfma_queue_bytes.restype = S
         L.mvamd_find_ncc_model_image.argtypes = [P, P, C.POINTER(FindOptions), P, S, C.POINTER(S),
                                                  C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
@@ -413,14 +416,14 @@ class NccModel:
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
                 highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0,
-                vector_tracking=False, pixel_type=PIXEL_BYTE) -> FindOptions:
+                vector_tracking=False, pixel_type=PIXEL_BYTE, max_candidates=0) -> FindOptions:
         """profile: True times every stage, an int is a mask of stage bits
         (bit i = STAGES[i]).  vector_tracking: track every level with the FP64
         vector kernel instead of matrix cores (same results).  pixel_type:
         PIXEL_BYTE or PIXEL_UINT2 for the device images (pitches in bytes)."""
         mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
-                           lowest, form, screen, mask, int(vector_tracking), int(pixel_type))
+                           lowest, form, screen, mask, int(vector_tracking), int(pixel_type), int(max_candidates))
 
     def find_image(self, image, opts: FindOptions, cap: int = 4096):
         """mvamd_find_ncc_model_image: find on an Image (or array) with the
@@ -548,6 +551,31 @@ def find_models_device(models, d_image: int, width: int, height: int, pitch: int
                                               int(selection_matches), float(selection_overlap), _ptr(res), cap,
                                               C.byref(n)))
     return res[:min(n.value, cap)].copy()
+
+
+def find_models_batch(models, d_images: list, widths, heights, pitches, opts, selection_matches=-1,
+                      selection_overlap=-1.0, coarse_queue=True, cap: int = 1024):
+    """mvamd_find_ncc_models_batch: find_models over several device-resident
+    images (widths/heights/pitches: one value or one per image).  Returns one
+    (n, 5) array per image: row, col, angle, score (internal convention),
+    model index."""
+    n = len(d_images)
+
+    def per_image(v):
+        return list(v) if isinstance(v, (list, tuple)) else [v] * n
+    ids = _model_ids(models)
+    ol = list(opts) if isinstance(opts, (list, tuple)) else [opts]
+    arr = (FindOptions * len(ol))(*ol)
+    ptrs = (C.c_void_p * n)(*d_images)
+    ws = (C.c_int64 * n)(*per_image(widths))
+    hs = (C.c_int64 * n)(*per_image(heights))
+    ps = (C.c_int64 * n)(*per_image(pitches))
+    res = np.zeros((max(n, 1), cap, 5))
+    counts = np.zeros(max(n, 1), np.uint64)
+    _check(lib().mvamd_find_ncc_models_batch(ids.h, ptrs, ws, hs, ps, n, arr, len(ol), int(selection_matches),
+                                             float(selection_overlap), int(bool(coarse_queue)), _ptr(res), cap,
+                                             _ptr(counts)))
+    return [res[k, :min(int(counts[k]), cap)].copy() for k in range(n)]
 
 
 def halcon_shift(poses: np.ndarray) -> np.ndarray:

@@ -413,6 +413,10 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
   q.highest = o->highest_level; q.lowest = std::max(1, o->lowest_level);
   q.form = o->form; q.screen = o->screen; q.profile = o->profile;
   q.track_mfma = o->vector_tracking == 0;
+  if (o->max_candidates < 0 || o->max_candidates > (int64_t(1) << 24))
+    fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, "find_ncc_model", "max_candidates",
+         "candidate budget must be 0 (the default) or 1..2^24");
+  if (o->max_candidates > 0) q.max_candidates = size_t(o->max_candidates);
   if (q.form < MVAMD_FORM_AUTO || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
   if (o->pixel_type != MVAMD_PIXEL_BYTE && o->pixel_type != MVAMD_PIXEL_UINT2)
@@ -1238,6 +1242,55 @@ extern "C" MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t
       d[4] = double(found[i].model);
     }
     *count = found.size();
+  });
+}
+
+extern "C" MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t* const* d_images,
+                                                const int64_t* widths, const int64_t* heights, const int64_t* pitches,
+                                                size_t n_images, const MvAmdFindOptions* options, size_t n_options,
+                                                int64_t selection_matches, double selection_overlap,
+                                                int32_t coarse_queue, double* results, size_t cap, size_t* counts) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_models";
+    required(d_images, op, "Images");
+    required(widths, op, "widths");
+    required(heights, op, "heights");
+    required(pitches, op, "pitches");
+    required(options, op, "options");
+    required(counts, op, "counts");
+    if (n_images > NCC_MAX_BATCH)
+      fail(mvamd::kLimitExceeded, mvamd::kErrInvalidValue, op, "Images", "too many images for one batch");
+    const auto ms = models_of(ModelIDs, op);
+    if (n_options != 1 && n_options != ms.size())
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, "options", "one value or one value per model required");
+    std::vector<mvamd::FindOptions> qs;
+    std::vector<const mvamd::NccModel*> ptrs;
+    for (size_t i = 0; i < ms.size(); ++i) {
+      qs.push_back(from_ext(options + (n_options == 1 ? 0 : i)));
+      qs.back().profile = 0;
+      ptrs.push_back(ms[i].get());
+    }
+    for (size_t i = 1; i < n_options; ++i)
+      if (options[i].pixel_type != options[0].pixel_type)
+        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "options", "one pixel type for the images", int64_t(i));
+    std::vector<mvamd::FrameInput> frames;
+    for (size_t k = 0; k < n_images; ++k)
+      frames.push_back(device_frame(op, d_images[k], widths[k], heights[k], pitches[k], options, int64_t(k)));
+    mvamd::Selection sel;
+    sel.num_matches = selection_matches;
+    sel.max_overlap = selection_overlap;
+    const auto found = algorithm(op, [&] {
+      return context().find_models_frames(ptrs.data(), qs.data(), ptrs.size(), frames.data(), n_images, sel, true,
+                                          coarse_queue != 0);
+    });
+    for (size_t k = 0; k < n_images; ++k) {
+      for (size_t i = 0; results && i < found[k].size() && i < cap; ++i) {
+        double* d = results + 5 * (cap * k + i);
+        const auto& m = found[k][i];
+        d[0] = m.pose.row; d[1] = m.pose.col; d[2] = m.pose.angle; d[3] = m.pose.score; d[4] = double(m.model);
+      }
+      counts[k] = found[k].size();
+    }
   });
 }
 

@@ -370,9 +370,44 @@ struct Job {
   size_t pos = 0, batch_size = 0;
   bool done = false;
   double coarse_flops = 0, track_flops = 0, track_mfma_flops = 0;
+  // find_models' background guard for this frame (matcher.cpp:839-846, 912-927; < 0: off)
+  double guard_first = -1, guard = -1;
 };
 struct Timer { hipEvent_t begin = nullptr, end = nullptr; int stage = 0; };
 }  // namespace
+
+// What the back half of a batch needs from its front half.
+struct FindContext::Plan {
+  hipStream_t qs = nullptr;
+  const NccModel* model = nullptr;
+  std::shared_ptr<DeviceModel> dmodel;
+  FindOptions o;
+  FindTrace* traces = nullptr;
+  size_t n_frames = 0, level_count = 0, top = 0, finest = 0, stride = 1, n_track = 0, A = 0;
+  size_t cap = 0, cap_all = 0, table_size = 0, n_count_words = 0;
+  std::vector<size_t> track_levels;
+  std::vector<std::vector<int32_t>> refs;
+  RasterPool qpool;
+  int64_t coarse_original = -1;
+  std::vector<Job> jobs;
+  DenseWeights dense;
+  bool use_mfma = false, track_on_mfma = false, wide = false, reduced = false;
+  bool coarse_deferred = false;  // the coarse search runs in a shared persistent launch
+  int max_bh = 0, max_bw = 0, F = 0, Wt_max = 0, Ht_max = 0;
+  double scale = 1, phase = 0, radius = 0, t0 = 0;
+  uint8_t* dev = nullptr;
+  NccFrame* dframes = nullptr;
+  NccMatch* lists = nullptr;
+  int32_t* counts = nullptr;
+  unsigned long long* extent = nullptr;
+  unsigned long long* work = nullptr;
+  NccPatchPool pools[2] = {};
+  size_t off_key = 0, off_raw = 0, off_table = 0, off_wh = 0, off_wl = 0, off_wsum = 0, off_common = 0;
+  size_t rb_full = 0, rb_counts = 0;
+  std::vector<size_t> off_refs;
+  uint8_t* rb = nullptr;
+  std::vector<int32_t> frame_dims;  // top-level (w, h) per frame, for the coarse queue
+};
 
 // A submitted batch: its buffers (reused by later batches in the same slot)
 // and what the completion needs.
@@ -407,6 +442,7 @@ struct FindContext::Slot {
   NcchipRefineParams rp{};
   std::vector<bool> track_mfma;  // per tracking level: launched on matrix cores
   double t0 = 0, t1 = 0, t2 = 0;
+  std::unique_ptr<Plan> plan;  // between the two halves of enqueue
 };
 
 FindContext::FindContext() {
@@ -554,6 +590,15 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
 
 void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inputs, size_t n_frames,
                           const FindOptions& req, FindTrace* traces) {
+  if (enqueue_front(S, model, inputs, n_frames, req, traces)) enqueue_back(S);
+}
+
+// The front half of a batch: host planning, the small uploads, the search
+// pyramid and the top-level statistics.  Returns false when there is nothing
+// to search (the slot is then complete); else the plan for the back half is
+// left in S.plan.
+bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput* inputs, size_t n_frames,
+                                const FindOptions& req, FindTrace* traces) {
   const double t0 = now_us();
   // the slot's stream; work queued on the context stream before (image
   // uploads) completes first
@@ -567,10 +612,11 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   S.jobs.clear();
   S.empty = false;
   for (size_t f = 0; f < n_frames && traces; ++f) traces[f] = FindTrace{};
+  S.plan.reset();
   auto finish_empty = [&]() {
     S.model = &model; S.o = req; S.traces = traces; S.n_frames = n_frames; S.empty = true;
   };
-  if (!n_frames) return finish_empty();
+  if (!n_frames) return finish_empty(), false;
   // ---- options and levels, shared by the batch (matcher.cpp:345-374)
   FindOptions o = req;
   for (size_t f = 0; f < n_frames; ++f)
@@ -602,7 +648,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   if (!std::isfinite(angle_end)) raise(kInvalidArgument, "angle_extent", "query angle range must be finite");
   const double cs = std::max(o.start, model.params.start);
   const double ce = std::min(angle_end, model.params.start + model.params.extent);
-  if (cs > ce + 1e-12) return finish_empty();
+  if (cs > ce + 1e-12) return finish_empty(), false;
   o.start = cs;
   o.extent = std::max(0., ce - cs);
   for (size_t f = 0; f < n_frames; ++f) {
@@ -615,7 +661,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   for (size_t f = 0; f < n_frames; ++f)
     if ((inputs[f].domain != nullptr) != reduced || (reduced && n_frames != 1))
       raise(kInvalidArgument, "image", "reduced search domains are supported for single-frame finds");
-  if (reduced && inputs[0].domain->empty()) return finish_empty();
+  if (reduced && inputs[0].domain->empty()) return finish_empty(), false;
   auto dmodel = device_model(model, qs);
 
   // ---- query patches per level (matcher.cpp:463-482), shared by the batch
@@ -942,31 +988,107 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, qs))
     raise(kInternalError, "device", "prefix launch failed");
   time_end(qs);
+  // ---- the plan of the back half
+  S.plan = std::make_unique<Plan>();
+  Plan& P = *S.plan;
+  P.qs = qs; P.model = &model; P.dmodel = dmodel; P.o = o; P.traces = traces; P.n_frames = n_frames;
+  P.level_count = level_count; P.top = top; P.finest = finest; P.stride = stride; P.n_track = n_track; P.A = A;
+  P.cap = cap; P.cap_all = cap_all; P.table_size = table_size; P.n_count_words = n_count_words;
+  P.track_levels = std::move(track_levels); P.refs = std::move(refs); P.qpool = std::move(qpool);
+  P.coarse_original = coarse_original; P.jobs = std::move(jobs); P.dense = std::move(dense);
+  P.use_mfma = use_mfma; P.track_on_mfma = track_on_mfma; P.wide = wide; P.reduced = reduced;
+  P.max_bh = max_bh; P.max_bw = max_bw; P.F = F; P.Wt_max = Wt_max; P.Ht_max = Ht_max;
+  P.scale = scale; P.phase = phase; P.radius = radius; P.t0 = t0;
+  P.dev = dev; P.dframes = dframes; P.lists = lists; P.counts = counts; P.extent = extent; P.work = work;
+  P.pools[0] = pools[0]; P.pools[1] = pools[1];
+  P.off_key = off_key; P.off_raw = off_raw; P.off_table = off_table; P.off_wh = off_wh; P.off_wl = off_wl;
+  P.off_wsum = off_wsum; P.off_common = off_common; P.rb_full = rb_full; P.rb_counts = rb_counts;
+  P.off_refs = std::move(off_refs); P.rb = rb;
+  return true;
+}
+
+// The split-BF16 coarse search of a planned batch as one ncchip_coarse_mfma
+// job (the arguments the back half passes to it).
+NcchipCoarseJob FindContext::coarse_job(Slot& S) {
+  Plan& P = *S.plan;
+  const NccModel& model = *P.model;
+  const FindOptions& o = P.o;
+  const DenseWeights& dense = P.dense;
+  uint8_t* const dev = P.dev;
+  NcchipCoarseJob j{};
+  j.frames = P.dframes; j.n_frames = P.F; j.level = int(P.top); j.level_kind = P.jobs[0].lv[P.top].kind;
+  j.max_w = P.Wt_max; j.max_h = P.Ht_max;
+  P.frame_dims.clear();
+  for (const Job& J : P.jobs) { P.frame_dims.push_back(J.lv[P.top].w); P.frame_dims.push_back(J.lv[P.top].h); }
+  j.frame_dims = P.frame_dims.data();
+  j.C = std::ldexp(P.wide ? 32768.0 : 128.0, int(2 * P.top));
+  j.pools[0] = P.pools[0]; j.pools[1] = P.pools[1];
+  j.patch_ids = reinterpret_cast<const int32_t*>(dev + P.off_refs[P.top]);
+  j.n_angles = int(P.A);
+  NcchipDenseWeights& dw = j.weights;
+  dw.wh = dev + P.off_wh; dw.wl = dev + P.off_wl; dw.wsum_eff = reinterpret_cast<const double*>(dev + P.off_wsum);
+  dw.ymin = dense.ymin; dw.xmin = dense.xmin; dw.yspan = dense.yspan; dw.xchunks = dense.xchunks;
+  dw.max_run = dense.max_run;
+  dw.common_runs = reinterpret_cast<const int32_t*>(dev + P.off_common);
+  dw.n_common_runs = int32_t(dense.common_runs.size() / 4);
+  dw.n_common = dense.n_common;
+  dw.prune_below = 0.9 * (o.min_score * .4);
+  dw.peak_thr = o.min_score * .4;  // sparse planes: the peaks kernel reads the flagged scores only
+  dw.w8_scale = dense.w8_scale;
+  j.exact_lattice = !P.wide && P.top <= 2;  // exact byte lattice (matcher.cpp:347, 561)
+  j.absolute = model.params.absolute;
+  j.screen = o.form != NCCHIP_FORM_FP8_SCREEN ? kMissing : o.screen > kMissing ? o.screen : kDefaultScreen;
+  return j;
+}
+
+// The back half: coarse search (unless the caller ran it in a shared
+// persistent launch), peaks, tracking, full score and the read-back copy.
+void FindContext::enqueue_back(Slot& S) {
+  Plan& P = *S.plan;
+  cur_ = &S;
+  hipStream_t qs = P.qs;
+  const NccModel& model = *P.model;
+  const std::shared_ptr<DeviceModel> dmodel = P.dmodel;
+  const FindOptions o = P.o;
+  FindTrace* const traces = P.traces;
+  const size_t n_frames = P.n_frames, top = P.top, finest = P.finest, stride = P.stride, n_track = P.n_track;
+  const size_t A = P.A, cap = P.cap, cap_all = P.cap_all, table_size = P.table_size, n_count_words = P.n_count_words;
+  const std::vector<size_t>& track_levels = P.track_levels;
+  const std::vector<std::vector<int32_t>>& refs = P.refs;
+  RasterPool& qpool = P.qpool;
+  const int64_t coarse_original = P.coarse_original;
+  std::vector<Job>& jobs = P.jobs;
+  const bool use_mfma = P.use_mfma, track_on_mfma = P.track_on_mfma, wide = P.wide, reduced = P.reduced;
+  const int max_bh = P.max_bh, max_bw = P.max_bw, F = P.F, Wt_max = P.Wt_max, Ht_max = P.Ht_max;
+  const double scale = P.scale, phase = P.phase, radius = P.radius, t0 = P.t0;
+  uint8_t* const dev = P.dev;
+  NccFrame* const dframes = P.dframes;
+  NccMatch* const lists = P.lists;
+  int32_t* const counts = P.counts;
+  unsigned long long* const extent = P.extent;
+  unsigned long long* const work = P.work;
+  const NccPatchPool pools[2] = {P.pools[0], P.pools[1]};
+  const size_t off_key = P.off_key, off_raw = P.off_raw, off_table = P.off_table;
+  const std::vector<size_t>& off_refs = P.off_refs;
+  uint8_t* const rb = P.rb;
+  const size_t rb_full = P.rb_full, rb_counts = P.rb_counts;
+  auto raster_of = [&](int32_t r) -> const Raster& {
+    return (r >> 24) ? qpool.rasters[size_t(r & 0xffffff)] : model.pool.rasters[size_t(r & 0xffffff)];
+  };
   // centring of the coarse forms: mid-range of the pixel type at the top level
   const double C = std::ldexp(wide ? 32768.0 : 128.0, int(2 * top));
   const int exact_top = !wide && top <= 2;  // exact byte lattice (matcher.cpp:347, 561)
   const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
   time_begin(kStageCoarse, qs);
   int rc = 0;
-  if (use_mfma) {
-    NcchipDenseWeights dw{};
-    dw.wh = dev + off_wh; dw.wl = dev + off_wl; dw.wsum_eff = reinterpret_cast<const double*>(dev + off_wsum);
-    dw.ymin = dense.ymin; dw.xmin = dense.xmin; dw.yspan = dense.yspan; dw.xchunks = dense.xchunks;
-    dw.max_run = dense.max_run;
-    dw.common_runs = reinterpret_cast<const int32_t*>(dev + off_common);
-    dw.n_common_runs = int32_t(dense.common_runs.size() / 4);
-    dw.n_common = dense.n_common;
-    dw.prune_below = 0.9 * (o.min_score * .4);
-    dw.peak_thr = o.min_score * .4;  // sparse planes: the peaks kernel reads the flagged scores only
-    dw.w8_scale = dense.w8_scale;
-    rc = ncchip_coarse_mfma(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
-                            &dw, exact_top, model.params.absolute,
-                            o.form != NCCHIP_FORM_FP8_SCREEN ? kMissing : o.screen > kMissing ? o.screen : kDefaultScreen,
-                            qs);
-  }
-  else
+  if (use_mfma && !P.coarse_deferred) {
+    const NcchipCoarseJob j = coarse_job(S);
+    rc = ncchip_coarse_mfma(j.frames, j.n_frames, j.level, j.level_kind, j.max_w, j.max_h, j.C, j.pools, j.patch_ids,
+                            j.n_angles, &j.weights, j.exact_lattice, j.absolute, j.screen, qs);
+  } else if (!use_mfma) {
     rc = ncchip_coarse(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
                        max_bh, max_bw, o.form, exact_top, model.params.absolute, qs);
+  }
   if (rc == -6) raise(kUnsupported, "form", "correlation form not available in this build");
   if (rc) raise(kInternalError, "device", "coarse launch failed");
   time_end(qs);
@@ -1100,6 +1222,7 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
   S.cap = cap; S.cap_all = cap_all; S.n_track = n_track; S.top = top; S.finest = finest; S.stride = stride;
   S.off_key = off_key; S.dev = dev; S.dframes = dframes; S.lists = lists; S.hfull = hfull; S.hcounts = hcounts;
   S.rb = rb; S.pools[0] = pools[0]; S.pools[1] = pools[1]; S.rp = rp; S.t0 = t0;
+  S.plan.reset();
 }
 
 // Waits for a slot's device-resident part and turns its full-scored lists into
@@ -1193,40 +1316,64 @@ void FindContext::collect(Slot& S) {
 // slots' host-visible staging buffers, free once the slot was collected.
 void FindContext::refine_round(std::vector<RefineRequest>& reqs) {
   if (reqs.empty()) return;
+  // the background guard of find_models is per frame: a frame with the guard
+  // on and one without are refined by separate launches of the same slot
+  auto guard_of = [](const RefineRequest& r) { return r.slot->jobs[r.frame].guard_first >= 0; };
   std::vector<size_t> order(reqs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return reqs[a].slot < reqs[b].slot; });
-  struct Launch { Slot* slot; size_t begin, end; };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    if (reqs[a].slot != reqs[b].slot) return reqs[a].slot < reqs[b].slot;
+    return guard_of(reqs[a]) < guard_of(reqs[b]);
+  });
+  struct Launch { Slot* slot; size_t begin, end, offset; };
   std::vector<Launch> launches;
   for (size_t i = 0; i < order.size();) {
     size_t j = i;
-    while (j < order.size() && reqs[order[j]].slot == reqs[order[i]].slot) ++j;
-    launches.push_back({reqs[order[i]].slot, i, j});
+    while (j < order.size() && reqs[order[j]].slot == reqs[order[i]].slot &&
+           guard_of(reqs[order[j]]) == guard_of(reqs[order[i]]))
+      ++j;
+    // the launches of one slot use consecutive parts of its staging areas
+    size_t offset = 0;
+    if (!launches.empty() && launches.back().slot == reqs[order[i]].slot)
+      offset = launches.back().offset + (launches.back().end - launches.back().begin);
+    launches.push_back({reqs[order[i]].slot, i, j, offset});
     i = j;
   }
-  for (const Launch& L : launches) {
+  std::vector<size_t> slot_total(launches.size());
+  for (size_t k = 0; k < launches.size(); ++k) {
+    size_t n = 0;
+    for (const Launch& L : launches) if (L.slot == launches[k].slot) n += L.end - L.begin;
+    slot_total[k] = n;
+  }
+  for (size_t li = 0; li < launches.size(); ++li) {
+    const Launch& L = launches[li];
     Slot& S = *L.slot;
     cur_ = &S;
-    const size_t n = L.end - L.begin;
-    auto* dcand = static_cast<NccMatch*>(S.stage.ptr);
-    auto* dcand_frame = reinterpret_cast<int32_t*>(dcand + n);
-    auto* dout = reinterpret_cast<NccMatch*>(S.rb);
-    auto* dverdict = reinterpret_cast<int32_t*>(dout + n);
+    const size_t n = L.end - L.begin, total = slot_total[li];
+    auto* dcand = static_cast<NccMatch*>(S.stage.ptr) + L.offset;
+    auto* dcand_frame = reinterpret_cast<int32_t*>(static_cast<NccMatch*>(S.stage.ptr) + total) + L.offset;
+    auto* dout = reinterpret_cast<NccMatch*>(S.rb) + L.offset;
+    auto* dverdict = reinterpret_cast<int32_t*>(reinterpret_cast<NccMatch*>(S.rb) + total) + L.offset;
     for (size_t k = 0; k < n; ++k) {
       const RefineRequest& r = reqs[order[L.begin + k]];
       dcand[k] = {r.in.row, r.in.col, r.in.angle, r.in.score};
       dcand_frame[k] = int32_t(r.frame);
     }
+    NcchipRefineParams rp = S.rp;
+    const Job& J = S.jobs[reqs[order[L.begin]].frame];
+    rp.guard_first = J.guard_first;
+    rp.guard = J.guard;
     time_begin(kStageRefine, stream2_);
-    if (ncchip_refine(&S.rp, S.pools, dcand, dcand_frame, int(n), dout, dverdict, stream2_))
+    if (ncchip_refine(&rp, S.pools, dcand, dcand_frame, int(n), dout, dverdict, stream2_))
       raise(kInternalError, "device", "refine launch failed");
     time_end(stream2_);
   }
   check(hipStreamSynchronize(stream2_), "refine");
-  for (const Launch& L : launches) {
-    const size_t n = L.end - L.begin;
-    const auto* dout = reinterpret_cast<const NccMatch*>(L.slot->rb);
-    const auto* dverdict = reinterpret_cast<const int32_t*>(dout + n);
+  for (size_t li = 0; li < launches.size(); ++li) {
+    const Launch& L = launches[li];
+    const size_t n = L.end - L.begin, total = slot_total[li];
+    const auto* dout = reinterpret_cast<const NccMatch*>(L.slot->rb) + L.offset;
+    const auto* dverdict = reinterpret_cast<const int32_t*>(reinterpret_cast<const NccMatch*>(L.slot->rb) + total) + L.offset;
     for (size_t k = 0; k < n; ++k) {
       RefineRequest& r = reqs[order[L.begin + k]];
       const NccMatch& m = dout[k];
@@ -1406,14 +1553,116 @@ bool multi_less(const ModelMatch& a, const ModelMatch& b) {  // multi.hpp:87-93
 std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, const FindOptions* opts, size_t n_models,
                                                  const FrameInput& frame, const Selection& sel,
                                                  bool per_model_quota_before_overlap) {
-  if (!n_models) return {};
+  return std::move(find_models_frames(models, opts, n_models, &frame, 1, sel, per_model_quota_before_overlap).front());
+}
+
+namespace {
+// One frame's greedy selection (select_refined, multi.hpp:83-119) as a walk
+// that stops where it needs refinements it does not have yet, so that the
+// walks of all frames of a batch share their refinement rounds.
+struct SelectionWalk {
+  struct Ref { size_t slot, index; };
+  const NccModel* const* models;
+  const FindOptions* opts;
+  size_t n_models;
+  const std::vector<ModelMatch>* pool;
+  const std::vector<Ref>* where;       // per pool entry (with_refine)
+  std::vector<size_t> ord;
+  Selection s;
+  bool with_refine;
+  std::vector<ModelMatch> result;
+  std::vector<Box> boxes;
+  std::vector<Bounds> encl;
+  std::vector<size_t> counts;
+  size_t batch = 0, pos = 0;
+  bool stop = false;
+
+  void init() {
+    counts.assign(n_models, 0);
+    batch = s.num_matches > 0 ? std::max<size_t>(2 * size_t(s.num_matches), 4) : 0;
+    if (!batch) {
+      size_t total = 0;
+      bool bounded = true;
+      for (size_t i = 0; i < n_models; ++i) { if (!opts[i].num_matches) bounded = false; total += opts[i].num_matches; }
+      batch = bounded ? std::max<size_t>(2 * total, 4) : ord.size();
+    }
+  }
+  const LevelGeom& geom(size_t model) const { return models[model]->levels.front(); }
+  bool quota_full(size_t model) const {
+    const int64_t limit = s.num_matches >= 0 ? s.num_matches : int64_t(opts[model].num_matches);
+    return s.num_matches < 0 && limit && counts[model] >= uint64_t(limit);
+  }
+  bool suppressed(const ModelMatch& c) const {
+    const double threshold = s.max_overlap >= 0 ? s.max_overlap : opts[c.model].max_overlap;
+    const Bounds aabb = enclosure(pose_box(geom(c.model), c.pose));
+    for (size_t i = 0; i < result.size(); ++i) {
+      if (s.max_overlap < 0 && result[i].model != c.model) continue;
+      if (disjoint(aabb, encl[i])) continue;
+      if (box_overlap(c.pose, geom(c.model), result[i].pose, geom(result[i].model)) > threshold + 1e-12) return true;
+    }
+    return false;
+  }
+  bool done() const { return stop || pos >= ord.size(); }
+  // Walks while every needed refinement is known (verdict: -1 unknown, -2
+  // requested, else the verdict).
+  template <class Verdict, class Refined>
+  void advance(Verdict verdict, Refined refined) {
+    while (pos < ord.size()) {
+      const ModelMatch& c = (*pool)[ord[pos]];
+      if (quota_full(c.model) || suppressed(c)) { ++pos; continue; }
+      ModelMatch r = c;
+      bool ok = true;
+      if (with_refine) {
+        const Ref& w = (*where)[ord[pos]];
+        if (verdict(w) < 0) return;
+        ok = verdict(w) != 0;
+        r.pose = refined(w);
+      }
+      ++pos;
+      if (!ok || suppressed(r)) continue;
+      const Box rect = pose_box(geom(r.model), r.pose);
+      result.push_back(r); boxes.push_back(rect); encl.push_back(enclosure(rect)); ++counts[r.model];
+      if (s.num_matches > 0 && result.size() >= uint64_t(s.num_matches)) { stop = true; return; }
+    }
+  }
+  // The next candidates the walk may need, as far as the current results tell.
+  template <class Verdict, class Mark>
+  void requests(std::vector<Ref>& targets, Verdict verdict, Mark mark) {
+    size_t taken = 0;
+    for (size_t i = pos; i < ord.size() && taken < batch; ++i) {
+      const ModelMatch& c = (*pool)[ord[i]];
+      const Ref& w = (*where)[ord[i]];
+      if (verdict(w) != -1 || quota_full(c.model) || suppressed(c)) continue;
+      mark(w);  // requested in this round (aliases share the request)
+      targets.push_back(w);
+      ++taken;
+    }
+    batch *= 2;
+  }
+  std::vector<ModelMatch> finish() {
+    std::sort(result.begin(), result.end(), multi_less);
+    return result;
+  }
+};
+}  // namespace
+
+std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccModel* const* models,
+                                                                     const FindOptions* opts, size_t n_models,
+                                                                     const FrameInput* frames, size_t n_frames,
+                                                                     const Selection& sel,
+                                                                     bool per_model_quota_before_overlap,
+                                                                     bool coarse_queue) {
+  std::vector<std::vector<ModelMatch>> out(n_frames);
+  if (!n_models || !n_frames) return out;
+  if (n_frames > NCC_MAX_BATCH) raise(kLimitExceeded, "Images", "too many images for one batch");
   // every slot of this call is reused: let earlier work on both streams end
   check(hipStreamSynchronize(stream_), "sync");
   check(hipStreamSynchronize(stream2_), "sync");
   const bool several = n_models > 1;
   bool distinct = false;
   for (size_t i = 0; i < n_models; ++i) distinct |= models[i] != models[0];
-  // ---- one query per distinct (model, options); aliases share its candidates (matcher.cpp:1087-1098)
+  // ---- one query per distinct (model, options) over all frames; aliases
+  // share its candidates (matcher.cpp:1087-1098)
   std::vector<size_t> owner(n_models), slot_of(n_models, size_t(-1));
   size_t n_queries = 0;
   for (size_t i = 0; i < n_models; ++i) {
@@ -1427,7 +1676,10 @@ std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, 
     multi_slots_.back()->stream = stream_;
     check(hipEventCreateWithFlags(&multi_slots_.back()->done, hipEventDisableTiming), "hipEventCreate");
   }
-  // ---- the device-resident parts of every query, back to back on one stream
+  // ---- the device-resident parts of every query, on one stream: all front
+  // halves, the split-BF16 coarse searches of all queries in one persistent
+  // launch (per kernel instantiation), then all back halves
+  std::vector<Slot*> planned;
   for (size_t i = 0; i < n_models; ++i) {
     if (owner[i] != i) continue;
     FindOptions o = opts[i];
@@ -1437,11 +1689,32 @@ std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, 
     o.trace_stages = false;
     Slot& S = *multi_slots_[slot_of[i]];
     cur_ = &S;
-    enqueue(S, *models[i], &frame, 1, o, nullptr);
+    if (enqueue_front(S, *models[i], frames, n_frames, o, nullptr)) planned.push_back(&S);
   }
-  std::vector<ModelMatch> cand;
-  struct CandRef { size_t slot, index; };
-  std::vector<CandRef> where;
+  if (coarse_queue) {
+    std::vector<NcchipCoarseJob> jobs;
+    std::vector<Slot*> deferred;
+    size_t n_units = 0;
+    for (Slot* S : planned)
+      if (S->plan->use_mfma) {
+        jobs.push_back(coarse_job(*S));
+        deferred.push_back(S);
+        n_units += S->plan->n_frames;
+      }
+    if (!jobs.empty()) {
+      const size_t bytes = ncchip_coarse_mfma_queue_bytes(int(jobs.size()), int(n_units));
+      if (queue_dev_.bytes < bytes) queue_dev_ = DeviceBuffer(align_up(bytes, 1 << 16));
+      queue_stage_.reserve(bytes);
+      const int rc = ncchip_coarse_mfma_queue(jobs.data(), int(jobs.size()), queue_dev_.ptr, queue_dev_.bytes,
+                                              queue_stage_.ptr, queue_stage_.bytes, stream_);
+      if (rc) raise(kInternalError, "device", "coarse queue launch failed");
+      for (Slot* S : deferred) S->plan->coarse_deferred = true;
+    }
+  }
+  for (Slot* S : planned) enqueue_back(*S);
+  // ---- candidates per frame in (score, model, row, col, angle) order
+  std::vector<std::vector<ModelMatch>> cand(n_frames);
+  std::vector<std::vector<SelectionWalk::Ref>> where(n_frames);
   for (size_t i = 0; i < n_models; ++i) {
     const size_t k = slot_of[owner[i]];
     Slot& S = *multi_slots_[k];
@@ -1453,115 +1726,90 @@ std::vector<ModelMatch> FindContext::find_models(const NccModel* const* models, 
       // when the query kept candidates, which it did when there are 32 of them
       const NccModel& m = *models[i];
       const FindOptions& o = S.o;  // the angle range clipped to the model's
-      if (!S.finest && distinct && sel.num_matches == 0 && S.jobs[0].full.size() >= 32 && o.subpixel &&
-          !o.num_matches && o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
-          m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several &&
-          frame.pixel_bytes == 1 && !frame.domain) {
-        S.rp.guard_first = o.min_score >= .8 ? .1 : .2;
-        S.rp.guard = .1;
+      for (size_t f = 0; f < n_frames; ++f) {
+        Job& J = S.jobs[f];
+        if (!S.finest && distinct && sel.num_matches == 0 && J.full.size() >= 32 && o.subpixel && !o.num_matches &&
+            o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
+            m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several &&
+            frames[f].pixel_bytes == 1 && !frames[f].domain) {
+          J.guard_first = o.min_score >= .8 ? .1 : .2;
+          J.guard = .1;
+        }
       }
     }
     if (S.empty) continue;
-    const std::vector<Pose>& full = S.jobs[0].full;
-    for (size_t j = 0; j < full.size(); ++j)
-      if (std::isfinite(full[j].score)) { cand.push_back({full[j], i}); where.push_back({k, j}); }
+    for (size_t f = 0; f < n_frames; ++f) {
+      const std::vector<Pose>& full = S.jobs[f].full;
+      for (size_t j = 0; j < full.size(); ++j)
+        if (std::isfinite(full[j].score)) { cand[f].push_back({full[j], i}); where[f].push_back({k, j}); }
+    }
   }
-  // ---- select_refined (multi.hpp:83-119) with speculative refinement rounds
-  std::vector<size_t> order(cand.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return multi_less(cand[a], cand[b]); });
-  auto geom = [&](size_t model) -> const LevelGeom& { return models[model]->levels.front(); };
-  // refined pose and verdict per (query, candidate): aliases share them
-  std::vector<std::vector<Pose>> refined(n_queries);
-  std::vector<std::vector<int>> verdict(n_queries);
+  // refined pose and verdict per (query, frame, candidate): aliases share them
+  std::vector<std::vector<std::vector<Pose>>> refined(n_queries, std::vector<std::vector<Pose>>(n_frames));
+  std::vector<std::vector<std::vector<int>>> verdict(n_queries, std::vector<std::vector<int>>(n_frames));
   for (size_t i = 0; i < n_models; ++i)
-    if (owner[i] == i && !multi_slots_[slot_of[i]]->empty) {
-      refined[slot_of[i]].assign(multi_slots_[slot_of[i]]->jobs[0].full.size(), Pose{});
-      verdict[slot_of[i]].assign(multi_slots_[slot_of[i]]->jobs[0].full.size(), -1);
-    }
-  auto select = [&](const std::vector<size_t>& ord, const std::vector<ModelMatch>& pool, const Selection& s,
-                    bool with_refine) {
-    std::vector<ModelMatch> result;
-    std::vector<Box> boxes;
-    std::vector<Bounds> encl;
-    std::vector<size_t> counts(n_models, 0);
-    auto quota_full = [&](size_t model) {
-      const int64_t limit = s.num_matches >= 0 ? s.num_matches : int64_t(opts[model].num_matches);
-      return s.num_matches < 0 && limit && counts[model] >= uint64_t(limit);
-    };
-    auto suppressed = [&](const ModelMatch& c) {
-      const double threshold = s.max_overlap >= 0 ? s.max_overlap : opts[c.model].max_overlap;
-      const Box rect = pose_box(geom(c.model), c.pose);
-      const Bounds aabb = enclosure(rect);
-      for (size_t i = 0; i < result.size(); ++i) {
-        if (s.max_overlap < 0 && result[i].model != c.model) continue;
-        if (disjoint(aabb, encl[i])) continue;
-        if (box_overlap(c.pose, geom(c.model), result[i].pose, geom(result[i].model)) > threshold + 1e-12) return true;
+    if (owner[i] == i && !multi_slots_[slot_of[i]]->empty)
+      for (size_t f = 0; f < n_frames; ++f) {
+        refined[slot_of[i]][f].assign(multi_slots_[slot_of[i]]->jobs[f].full.size(), Pose{});
+        verdict[slot_of[i]][f].assign(multi_slots_[slot_of[i]]->jobs[f].full.size(), -1);
       }
-      return false;
-    };
-    size_t batch = s.num_matches > 0 ? std::max<size_t>(2 * size_t(s.num_matches), 4) : 0;
-    if (!batch) {
-      size_t total = 0;
-      bool bounded = true;
-      for (size_t i = 0; i < n_models; ++i) { if (!opts[i].num_matches) bounded = false; total += opts[i].num_matches; }
-      batch = bounded ? std::max<size_t>(2 * total, 4) : ord.size();
-    }
-    size_t pos = 0;
-    bool stop = false;
-    while (!stop && pos < ord.size()) {
-      // walk while every needed refinement is known
-      while (pos < ord.size()) {
-        const ModelMatch& c = pool[ord[pos]];
-        if (quota_full(c.model) || suppressed(c)) { ++pos; continue; }
-        ModelMatch r = c;
-        bool ok = true;
-        if (with_refine) {
-          const CandRef& w = where[ord[pos]];
-          if (verdict[w.slot][w.index] < 0) break;
-          ok = verdict[w.slot][w.index] != 0;
-          r.pose = refined[w.slot][w.index];
-        }
-        ++pos;
-        if (!ok || suppressed(r)) continue;
-        const Box rect = pose_box(geom(r.model), r.pose);
-        result.push_back(r); boxes.push_back(rect); encl.push_back(enclosure(rect)); ++counts[r.model];
-        if (s.num_matches > 0 && result.size() >= uint64_t(s.num_matches)) { stop = true; break; }
-      }
-      if (stop || pos >= ord.size()) break;
-      // the next candidates the walk may need, as far as the current results tell
-      std::vector<RefineRequest> reqs;
-      std::vector<CandRef> targets;
-      for (size_t i = pos; i < ord.size() && reqs.size() < batch; ++i) {
-        const ModelMatch& c = pool[ord[i]];
-        const CandRef& w = where[ord[i]];
-        if (verdict[w.slot][w.index] != -1 || quota_full(c.model) || suppressed(c)) continue;
-        verdict[w.slot][w.index] = -2;  // requested in this round (aliases share the request)
-        reqs.push_back({multi_slots_[w.slot].get(), 0, c.pose, {}, 0});
-        targets.push_back(w);
-      }
-      refine_round(reqs);
-      for (size_t k = 0; k < reqs.size(); ++k) {
-        refined[targets[k].slot][targets[k].index] = reqs[k].out;
-        verdict[targets[k].slot][targets[k].index] = reqs[k].verdict;
-      }
-      batch *= 2;
-    }
-    std::sort(result.begin(), result.end(), multi_less);
-    return result;
-  };
   bool any_quota = false;
   for (size_t i = 0; i < n_models; ++i) any_quota |= opts[i].num_matches > 0;
   // HALCON's two-stage quota: per-model quotas and overlaps first, then the
   // cross-model overlap without refilling quotas (matcher.cpp:1108-1112)
   const bool independent = per_model_quota_before_overlap && sel.num_matches < 0 && sel.max_overlap >= 0 && any_quota;
-  auto result = select(order, cand, independent ? Selection{-1, -1} : sel, true);
-  if (independent) {
-    std::vector<size_t> ord2(result.size());
-    for (size_t i = 0; i < ord2.size(); ++i) ord2[i] = i;
-    result = select(ord2, result, sel, false);
+  std::vector<SelectionWalk> walks(n_frames);
+  for (size_t f = 0; f < n_frames; ++f) {
+    SelectionWalk& w = walks[f];
+    w.models = models; w.opts = opts; w.n_models = n_models; w.pool = &cand[f]; w.where = &where[f];
+    w.s = independent ? Selection{-1, -1} : sel;
+    w.with_refine = true;
+    w.ord.resize(cand[f].size());
+    for (size_t i = 0; i < w.ord.size(); ++i) w.ord[i] = i;
+    std::sort(w.ord.begin(), w.ord.end(), [&](size_t a, size_t b) { return multi_less(cand[f][a], cand[f][b]); });
+    w.init();
   }
-  return result;
+  // ---- the walks of all frames, with shared speculative refinement rounds
+  for (;;) {
+    std::vector<RefineRequest> reqs;
+    std::vector<std::pair<size_t, SelectionWalk::Ref>> targets;  // (frame, candidate)
+    for (size_t f = 0; f < n_frames; ++f) {
+      SelectionWalk& w = walks[f];
+      auto v = [&](const SelectionWalk::Ref& r) { return verdict[r.slot][f][r.index]; };
+      w.advance(v, [&](const SelectionWalk::Ref& r) { return refined[r.slot][f][r.index]; });
+      if (w.done()) continue;
+      std::vector<SelectionWalk::Ref> t;
+      w.requests(t, v, [&](const SelectionWalk::Ref& r) { verdict[r.slot][f][r.index] = -2; });
+      for (const auto& r : t) {
+        reqs.push_back({multi_slots_[r.slot].get(), uint32_t(f), multi_slots_[r.slot]->jobs[f].full[r.index], {}, 0});
+        targets.push_back({f, r});
+      }
+    }
+    if (reqs.empty()) break;
+    refine_round(reqs);
+    for (size_t k = 0; k < reqs.size(); ++k) {
+      const auto& [f, r] = targets[k];
+      refined[r.slot][f][r.index] = reqs[k].out;
+      verdict[r.slot][f][r.index] = reqs[k].verdict;
+    }
+  }
+  for (size_t f = 0; f < n_frames; ++f) {
+    std::vector<ModelMatch> result = walks[f].finish();
+    if (independent) {
+      SelectionWalk w2;
+      w2.models = models; w2.opts = opts; w2.n_models = n_models; w2.pool = &result; w2.where = nullptr;
+      w2.s = sel;
+      w2.with_refine = false;
+      w2.ord.resize(result.size());
+      for (size_t i = 0; i < w2.ord.size(); ++i) w2.ord[i] = i;
+      w2.init();
+      auto never = [](const SelectionWalk::Ref&) { return 0; };
+      w2.advance(never, [](const SelectionWalk::Ref&) { return Pose{}; });
+      result = w2.finish();
+    }
+    out[f] = std::move(result);
+  }
+  return out;
 }
 
 }  // namespace mvamd

@@ -166,6 +166,17 @@ class FindContext {
   std::vector<ModelMatch> find_models(const NccModel* const* models, const FindOptions* options, size_t n_models,
                                       const FrameInput& frame, const Selection& selection,
                                       bool per_model_quota_before_overlap = true);
+  // find_models over several frames at once, each frame's result the one
+  // find_models returns for it: one query per distinct (model, options) over
+  // all frames (one launch per stage for the frames of a query), the
+  // split-BF16 coarse searches of all queries in one persistent launch when
+  // coarse_queue is set, and the frames' greedy selections sharing their
+  // refinement rounds.
+  std::vector<std::vector<ModelMatch>> find_models_frames(const NccModel* const* models, const FindOptions* options,
+                                                          size_t n_models, const FrameInput* frames, size_t n_frames,
+                                                          const Selection& selection,
+                                                          bool per_model_quota_before_overlap = true,
+                                                          bool coarse_queue = true);
   // Same, from host memory.
   std::vector<ModelMatch> find_models_host(const NccModel* const* models, const FindOptions* options,
                                            size_t n_models, const uint8_t* image, size_t width, size_t height,
@@ -188,6 +199,7 @@ class FindContext {
 
  private:
   struct Slot;  // state of a submitted batch
+  struct Plan;  // what the back half of a batch's enqueue needs
   struct RefineRequest {
     Slot* slot;      // the query that refines it
     uint32_t frame;  // frame of the slot's batch
@@ -196,6 +208,10 @@ class FindContext {
   };
   void enqueue(Slot& S, const NccModel& model, const FrameInput* frames, size_t n_frames, const FindOptions& options,
                FindTrace* traces);
+  bool enqueue_front(Slot& S, const NccModel& model, const FrameInput* frames, size_t n_frames,
+                     const FindOptions& options, FindTrace* traces);
+  void enqueue_back(Slot& S);
+  NcchipCoarseJob coarse_job(Slot& S);
   void collect(Slot& S);
   void refine_round(std::vector<RefineRequest>& reqs);
   void time_begin(int stage, hipStream_t stream);
@@ -212,6 +228,8 @@ class FindContext {
   int device_ = 0;
   DeviceBuffer image_;
   DeviceBuffer create_;  // model construction workspace (grow-only)
+  DeviceBuffer queue_dev_;     // the persistent coarse search's work queue
+  PinnedBuffer queue_stage_;   // its host-side staging
 };
 
 // Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).

@@ -27,8 +27,11 @@
 // overflow) arithmetic on the vector ALUs, and the FP64 normalisation is the
 // one every correlation form shares, so only the dot product carries the
 // split-BF16 error.
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "ncc_common.hip.h"
 #include "../../include/ncchip.h"
@@ -91,14 +94,18 @@ __device__ __forceinline__ uint16_t bf16_bits_exact(int v) {
   return uint16_t(__float_as_uint(float(v)) >> 16);
 }
 
+// One block's work: output rows 4 by .. 4 by + 3, anchors 128 bx .. + 127 of
+// frame f, angle tile `tile` (32 coarse angles).  Returns uniformly when the
+// block lies past the frame; after the screen decision a wave may return on
+// its own (no block barrier follows it).
 template <class Tpix, class RunAcc, bool kScreen>
-__global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args) {
+__device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, const int tile, const int bx,
+                                            const int by) {
   extern __shared__ uint16_t lds[];
-  const int f = blockIdx.z / args.n_tiles, tile = blockIdx.z % args.n_tiles;
   const NccFrame& F = args.frames[f];
   const NccLevel lv = F.levels[args.level];
   const int H = lv.height, W = lv.width;
-  const int r0 = blockIdx.y * kMRows, c0 = blockIdx.x * kBlockCols;
+  const int r0 = by * kMRows, c0 = bx * kBlockCols;
   if (r0 >= H || c0 >= W) return;  // uniform per block
   const int P = args.pitch;
   const int rows = kMRows + args.yspan - 1;
@@ -437,6 +444,50 @@ __global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args
   if (queued) flush();
 }
 
+template <class Tpix, class RunAcc, bool kScreen>
+__global__ void __launch_bounds__(kThreads, 3) k_coarse_mfma(const MfmaArgs args) {
+  coarse_tile<Tpix, RunAcc, kScreen>(args, int(blockIdx.z) / args.n_tiles, int(blockIdx.z) % args.n_tiles,
+                                     int(blockIdx.x), int(blockIdx.y));
+}
+
+// The coarse work queue: every (job, frame) unit contributes gx x gy blocks
+// per angle tile; its items are numbered from first[unit], angle tile
+// outermost, then block rows, then block columns (neighbouring items share
+// image rows in L2).
+struct QueueUnit { int job, frame, gx, gy; };
+struct QueueArgs {
+  const MfmaArgs* args;      // per job (device)
+  const QueueUnit* units;    // this launch's units (device)
+  const int* first;          // first item of each unit, n_units + 1 entries (device)
+  int n_units, n_items;
+  unsigned* counter;         // next item to take; zero when the launch starts
+};
+
+// Persistent coarse search: each block takes items from the queue until it
+// is drained (every block exits once the counter passes n_items).
+template <class Tpix, class RunAcc, bool kScreen>
+__global__ void __launch_bounds__(kThreads, 3) k_coarse_queue(const QueueArgs q) {
+  __shared__ int s_item;
+  for (;;) {
+    __syncthreads();  // the previous item is done with LDS and s_item
+    if (threadIdx.x == 0) s_item = int(atomicAdd(q.counter, 1u));
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= q.n_items) return;
+    int lo = 0, hi = q.n_units - 1;  // last unit whose first item is <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (q.first[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const QueueUnit u = q.units[lo];
+    const int local = item - q.first[lo];
+    const int per_tile = u.gx * u.gy;
+    const int tile = local / per_tile, rem = local - tile * per_tile;
+    coarse_tile<Tpix, RunAcc, kScreen>(q.args[u.job], u.frame, tile, rem % u.gx, rem / u.gx);
+  }
+}
+
 }  // namespace ncc
 
 using namespace ncc;
@@ -462,14 +513,15 @@ extern "C" int ncchip_coarse_mfma_fits(int level, int level_kind, int yspan, int
   return coarse_mfma_lds(level, level_kind, yspan, xchunks, screened != 0, &pitch, &pitch8) > 0;
 }
 
-extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w,
-                                  int max_h, double C, const NccPatchPool* pools, const int32_t* patch_ids,
-                                  int n_angles, const NcchipDenseWeights* dw, int exact_lattice, int absolute,
-                                  double screen, void* stream) {
-  if (n_frames <= 0 || n_angles <= 0) return 0;
+namespace {
+// Kernel arguments of one split-BF16 coarse search; 0 on success, -6 when the
+// level or the tile does not fit the kernel, -1 on bad arguments.
+int make_args(const NccFrame* frames, int n_frames, int level, int level_kind, double C, const NccPatchPool* pools,
+              const int32_t* patch_ids, int n_angles, const NcchipDenseWeights* dw, int exact_lattice, int absolute,
+              double screen, MfmaArgs& a, size_t& lds, bool& narrow) {
   const bool screened = screen > -__builtin_inf();
   if (screened && !(dw->w8_scale > 0)) return -1;
-  MfmaArgs a{};
+  a = MfmaArgs{};
   a.frames = frames; a.level = level; a.n_angles = n_angles; a.n_tiles = (n_angles + 31) / 32; a.C = C;
   a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.patch_ids = patch_ids;
   a.wh = reinterpret_cast<const uint4*>(dw->wh); a.wl = reinterpret_cast<const uint4*>(dw->wl);
@@ -478,7 +530,7 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
   a.common_runs = reinterpret_cast<const int4*>(dw->common_runs); a.n_common_runs = dw->n_common_runs;
   a.n_common = dw->n_common; a.prune_below = dw->prune_below; a.peak_thr = dw->peak_thr;
   if (size_t(n_frames) * a.n_tiles > 65535) return -1;
-  const size_t lds = coarse_mfma_lds(level, level_kind, a.yspan, a.xc, screened, &a.pitch, &a.pitch8);
+  lds = coarse_mfma_lds(level, level_kind, a.yspan, a.xc, screened, &a.pitch, &a.pitch8);
   if (!lds) return -6;
   if (screened) {
     a.screen = screen;
@@ -486,21 +538,151 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
     a.w8_scale = float(dw->w8_scale);
     a.dot8_unscale = 1.0 / (double(a.img8_scale) * double(a.w8_scale));
   }
-  const dim3 grid((max_w + kBlockCols - 1) / kBlockCols, (max_h + kMRows - 1) / kMRows, unsigned(n_frames * a.n_tiles));
-  hipStream_t st = static_cast<hipStream_t>(stream);
   // per-run sums of t^2 fit 32 bits when (longest run) * (128 * 4^level)^2 < 2^32
   const double tmax = 128.0 * double(1 << (2 * level));
-  const bool narrow = double(dw->max_run) * tmax * tmax < 4294967296.0;
-#define NCC_MFMA(TP, RA)                                                                    \
-  do {                                                                                      \
-    if (screened) hipLaunchKernelGGL((k_coarse_mfma<TP, RA, true>), grid, dim3(kThreads), lds, st, a); \
-    else hipLaunchKernelGGL((k_coarse_mfma<TP, RA, false>), grid, dim3(kThreads), lds, st, a);         \
-  } while (0)
-  if (level_kind == NCC_U8) {
-    if (narrow) NCC_MFMA(uint8_t, uint32_t); else NCC_MFMA(uint8_t, uint64_t);
-  } else {
-    if (narrow) NCC_MFMA(uint16_t, uint32_t); else NCC_MFMA(uint16_t, uint64_t);
+  narrow = double(dw->max_run) * tmax * tmax < 4294967296.0;
+  return 0;
+}
+
+// Instantiation of a coarse launch: pixel type, run accumulator width, screen.
+int variant(int level_kind, bool narrow, bool screened) {
+  return (level_kind == NCC_U8 ? 0 : 4) | (narrow ? 0 : 2) | (screened ? 1 : 0);
+}
+
+template <class TP, class RA, bool kScreen>
+void launch_grid(const dim3& grid, size_t lds, hipStream_t st, const MfmaArgs& a) {
+  hipLaunchKernelGGL((k_coarse_mfma<TP, RA, kScreen>), grid, dim3(kThreads), lds, st, a);
+}
+void launch_grid_variant(int v, const dim3& grid, size_t lds, hipStream_t st, const MfmaArgs& a) {
+  switch (v) {
+    case 0: launch_grid<uint8_t, uint32_t, false>(grid, lds, st, a); break;
+    case 1: launch_grid<uint8_t, uint32_t, true>(grid, lds, st, a); break;
+    case 2: launch_grid<uint8_t, uint64_t, false>(grid, lds, st, a); break;
+    case 3: launch_grid<uint8_t, uint64_t, true>(grid, lds, st, a); break;
+    case 4: launch_grid<uint16_t, uint32_t, false>(grid, lds, st, a); break;
+    case 5: launch_grid<uint16_t, uint32_t, true>(grid, lds, st, a); break;
+    case 6: launch_grid<uint16_t, uint64_t, false>(grid, lds, st, a); break;
+    default: launch_grid<uint16_t, uint64_t, true>(grid, lds, st, a); break;
   }
-#undef NCC_MFMA
+}
+
+template <class TP, class RA, bool kScreen>
+int launch_queue(size_t lds, hipStream_t st, const QueueArgs& q, int n_cus) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_coarse_queue<TP, RA, kScreen>),
+                                                   kThreads, lds) != hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  const int blocks = std::max(1, std::min(q.n_items, n_cus * per_cu));
+  hipLaunchKernelGGL((k_coarse_queue<TP, RA, kScreen>), dim3(blocks), dim3(kThreads), lds, st, q);
+  return 0;
+}
+int launch_queue_variant(int v, size_t lds, hipStream_t st, const QueueArgs& q, int n_cus) {
+  switch (v) {
+    case 0: return launch_queue<uint8_t, uint32_t, false>(lds, st, q, n_cus);
+    case 1: return launch_queue<uint8_t, uint32_t, true>(lds, st, q, n_cus);
+    case 2: return launch_queue<uint8_t, uint64_t, false>(lds, st, q, n_cus);
+    case 3: return launch_queue<uint8_t, uint64_t, true>(lds, st, q, n_cus);
+    case 4: return launch_queue<uint16_t, uint32_t, false>(lds, st, q, n_cus);
+    case 5: return launch_queue<uint16_t, uint32_t, true>(lds, st, q, n_cus);
+    case 6: return launch_queue<uint16_t, uint64_t, false>(lds, st, q, n_cus);
+    default: return launch_queue<uint16_t, uint64_t, true>(lds, st, q, n_cus);
+  }
+}
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+}  // namespace
+
+extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w,
+                                  int max_h, double C, const NccPatchPool* pools, const int32_t* patch_ids,
+                                  int n_angles, const NcchipDenseWeights* dw, int exact_lattice, int absolute,
+                                  double screen, void* stream) {
+  if (n_frames <= 0 || n_angles <= 0) return 0;
+  MfmaArgs a;
+  size_t lds = 0;
+  bool narrow = false;
+  const int rc = make_args(frames, n_frames, level, level_kind, C, pools, patch_ids, n_angles, dw, exact_lattice,
+                           absolute, screen, a, lds, narrow);
+  if (rc) return rc;
+  const dim3 grid((max_w + kBlockCols - 1) / kBlockCols, (max_h + kMRows - 1) / kMRows, unsigned(n_frames * a.n_tiles));
+  launch_grid_variant(variant(level_kind, narrow, screen > -__builtin_inf()), grid, lds, static_cast<hipStream_t>(stream), a);
+  return hipGetLastError() == hipSuccess ? 0 : -1000;
+}
+
+extern "C" size_t ncchip_coarse_mfma_queue_bytes(int n_jobs, int n_units) {
+  return align256(size_t(n_jobs) * sizeof(MfmaArgs)) + align256(size_t(n_units) * sizeof(QueueUnit)) +
+         align256((size_t(n_units) + 8) * sizeof(int)) + 256;
+}
+
+extern "C" int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs, void* d_work, size_t work_bytes,
+                                        void* h_stage, size_t stage_bytes, void* stream) {
+  if (n_jobs <= 0) return 0;
+  if (!jobs || !d_work || !h_stage) return -1;
+  std::vector<MfmaArgs> args(static_cast<size_t>(n_jobs));
+  std::vector<size_t> lds(static_cast<size_t>(n_jobs));
+  std::vector<int> var(static_cast<size_t>(n_jobs));
+  int n_units = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    const NcchipCoarseJob& J = jobs[j];
+    if (J.n_frames <= 0 || J.n_angles <= 0) continue;
+    bool narrow = false;
+    const int rc = make_args(J.frames, J.n_frames, J.level, J.level_kind, J.C, J.pools, J.patch_ids, J.n_angles,
+                             &J.weights, J.exact_lattice, J.absolute, J.screen, args[size_t(j)], lds[size_t(j)], narrow);
+    if (rc) return rc;
+    var[size_t(j)] = variant(J.level_kind, narrow, J.screen > -__builtin_inf());
+    n_units += J.n_frames;
+  }
+  const size_t need = ncchip_coarse_mfma_queue_bytes(n_jobs, n_units);
+  if (work_bytes < need || stage_bytes < need) return -103;
+  auto* base = static_cast<uint8_t*>(h_stage);
+  const size_t off_units = align256(size_t(n_jobs) * sizeof(MfmaArgs));
+  const size_t off_first = off_units + align256(size_t(n_units) * sizeof(QueueUnit));
+  const size_t off_counters = off_first + align256((size_t(n_units) + 8) * sizeof(int));
+  std::memcpy(base, args.data(), size_t(n_jobs) * sizeof(MfmaArgs));
+  auto* units = reinterpret_cast<QueueUnit*>(base + off_units);
+  auto* first = reinterpret_cast<int*>(base + off_first);
+  std::memset(base + off_counters, 0, 256);
+  // units grouped by instantiation; each group's first[] ends with its item count
+  struct Group { int variant, unit0, n_units, first0; size_t lds; int64_t items; };
+  std::vector<Group> groups;
+  int u = 0, fpos = 0;
+  for (int v = 0; v < 8; ++v) {
+    Group g{v, u, 0, fpos, 0, 0};
+    for (int j = 0; j < n_jobs; ++j) {
+      const NcchipCoarseJob& J = jobs[j];
+      if (J.n_frames <= 0 || J.n_angles <= 0 || var[size_t(j)] != v) continue;
+      g.lds = std::max(g.lds, lds[size_t(j)]);
+      for (int f = 0; f < J.n_frames; ++f) {
+        const int w = J.frame_dims ? J.frame_dims[2 * f] : J.max_w, h = J.frame_dims ? J.frame_dims[2 * f + 1] : J.max_h;
+        const int gx = (w + kBlockCols - 1) / kBlockCols, gy = (h + kMRows - 1) / kMRows;
+        units[u] = QueueUnit{j, f, std::max(gx, 1), std::max(gy, 1)};
+        first[fpos++] = int(g.items);
+        g.items += int64_t(std::max(gx, 1)) * std::max(gy, 1) * args[size_t(j)].n_tiles;
+        ++u;
+        ++g.n_units;
+      }
+    }
+    if (!g.n_units) continue;
+    if (g.items >= (int64_t(1) << 31)) return -103;
+    first[fpos++] = int(g.items);
+    groups.push_back(g);
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(d_work, h_stage, need, hipMemcpyHostToDevice, st) != hipSuccess) return -1000;
+  int dev = 0, n_cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&n_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cus <= 0)
+    n_cus = 256;
+  auto* dbase = static_cast<uint8_t*>(d_work);
+  for (size_t k = 0; k < groups.size(); ++k) {
+    const Group& g = groups[k];
+    QueueArgs q{};
+    q.args = reinterpret_cast<const MfmaArgs*>(dbase);
+    q.units = reinterpret_cast<const QueueUnit*>(dbase + off_units) + g.unit0;
+    q.first = reinterpret_cast<const int*>(dbase + off_first) + g.first0;
+    q.n_units = g.n_units;
+    q.n_items = int(g.items);
+    q.counter = reinterpret_cast<unsigned*>(dbase + off_counters) + k;
+    launch_queue_variant(g.variant, g.lds, st, q, n_cus);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }

@@ -293,6 +293,96 @@ def small_multi(seed_k: int = 20) -> MultiScene:
     return multi_scene(320, 256, [(40, 40), (32, 48)], [2, 2], -e, e, 4.0, SEED_BASE + seed_k)
 
 
+@dataclass
+class BatchModel:
+    template: np.ndarray
+    start: float
+    extent: float
+    step: float       # 0: the automatic step
+    levels: int = 0   # 0: automatic
+
+
+@dataclass
+class BatchWorkload:
+    models: list                                     # BatchModel
+    images: list = field(default_factory=list)       # u8 search images of uneven size
+    truth: list = field(default_factory=list)        # per image: [(model, row, col, angle)]
+
+
+RANGES = ((-math.radians(10), math.radians(20)), (-math.radians(45), math.radians(90)), (0.0, 2 * math.pi))
+RASTER_BUDGET = 16_000_000  # samples over all levels and angles (the reference's create budget)
+
+
+def budget_step(h: int, w: int, extent: float) -> float:
+    """0 (the automatic step) when the model fits the raster budget with it,
+    else the smallest step of 1/1000 rad granularity whose angle grid keeps
+    the rasters (about 4/3 w h samples per angle over the pyramid) within
+    0.9 of the budget."""
+    auto = max(.01, min(.15, 1 / max(1.0, math.hypot(w, h) / 2)))
+    per_angle = 4.0 / 3.0 * w * h
+    if (math.ceil(extent / auto) + 1) * per_angle <= 0.9 * RASTER_BUDGET:
+        return 0.0
+    n = int(0.9 * RASTER_BUDGET / per_angle) - 1
+    return math.ceil(extent / max(n, 1) * 1000) / 1000
+
+
+def batch_levels(h: int, w: int, min_top: int = 256) -> int:
+    """The most pyramid levels that keep at least min_top template pixels at
+    the top level (a coarser top level makes the coarse search of small
+    templates exceed the reference's 10 000-candidate budget)."""
+    levels = 1
+    while (h >> levels) * (w >> levels) >= min_top and min(h >> levels, w >> levels) >= 8:
+        levels += 1
+    return levels
+
+
+def batch_workload(n_models: int = 32, n_images: int = 16, side: tuple = (32, 256),
+                   image_sizes: tuple = ((640, 480), (4096, 3072)), max_instances: int = 6, noise: float = 6.0,
+                   seed_k: int = 4) -> BatchWorkload:
+    """Heterogeneous batch: n_models templates with sides uniform in `side`
+    (width and height drawn apart) and angle ranges drawn from +-10 deg,
+    +-45 deg and the full turn (an explicit step where the automatic one
+    would exceed the raster budget); n_images blurred-noise images with
+    width and height uniform between image_sizes[0] and image_sizes[1], each
+    holding 0..max_instances rotated pastes of models drawn at random, with
+    N(0, noise).  Each model's pyramid levels: batch_levels."""
+    rng = SplitMix64(SEED_BASE + seed_k)
+    models = []
+    for k in range(n_models):
+        h = side[0] + int(rng.uniform1() * (side[1] - side[0] + 1))
+        w = side[0] + int(rng.uniform1() * (side[1] - side[0] + 1))
+        start, extent = RANGES[min(2, int(rng.uniform1() * 3))]
+        tmpl = make_template(SplitMix64((SEED_BASE + seed_k + 104729 * (k + 1)) ^ 0x5EED5EED), h, w)
+        models.append(BatchModel(tmpl, start, extent, budget_step(h, w, extent), batch_levels(h, w)))
+    work = BatchWorkload(models)
+    (w0, h0), (w1, h1) = image_sizes
+    for j in range(n_images):
+        W = w0 + int(rng.uniform1() * (w1 - w0 + 1))
+        H = h0 + int(rng.uniform1() * (h1 - h0 + 1))
+        base = blurred_noise(rng, H, W).astype(np.float64)
+        count = int(rng.uniform1() * (max_instances + 1))
+        truth, tries = [], 0
+        while len(truth) < count and tries < 1000:
+            tries += 1
+            k = min(n_models - 1, int(rng.uniform1() * n_models))
+            th, tw = models[k].template.shape
+            diag = math.hypot(th, tw)
+            if diag + 4 >= min(H, W):
+                continue
+            cy = diag / 2 + 2 + rng.uniform1() * (H - 1 - diag - 4)
+            cx = diag / 2 + 2 + rng.uniform1() * (W - 1 - diag - 4)
+            ang = models[k].start + rng.uniform1() * models[k].extent
+            if any(math.hypot(cy - r, cx - c) < 0.6 * (diag + math.hypot(*models[m].template.shape))
+                   for m, r, c, _ in truth):
+                continue
+            truth.append((k, cy, cx, ang))
+        for k, cy, cx, ang in truth:
+            paste(base, models[k].template, cy, cx, ang)
+        work.images.append(_finish(base, rng, noise, False))
+        work.truth.append(truth)
+    return work
+
+
 # ------------------------------------------------------------- search domains
 def mask_runs(mask: np.ndarray, row0: int = 0, col0: int = 0) -> np.ndarray:
     """Canonical runs (n x 3: row, first column, last column) of a boolean

@@ -1788,8 +1788,8 @@ double orc_continuous(void* mp, const uint8_t* img, int w, int h, int stride, do
   return continuous(g, m.levels.front(), m.original, Match{row, col, angle, 0}, m.opt.absolute);
 }
 
-// find_models (domain as for orc_find).  opts: n_models x 8 doubles (start, extent, min_score,
-// max_overlap, num_matches, subpixel, highest, lowest); selection as the C
+// find_models (domain as for orc_find).  opts: n_models x 9 doubles (start, extent, min_score,
+// max_overlap, num_matches, subpixel, highest, lowest, max_candidates with 0 the default 10 000); selection as the C
 // ABI derives it (num_matches -1 / max_overlap -1 mean per model).  results:
 // cap x 5 (row, col, angle, score, model) in the internal convention.
 // prepared (optional): n_models x cap_p x 4, with prepared_n the list sizes.
@@ -1802,10 +1802,11 @@ int orc_find_models(void* const* mps, int n_models, const uint8_t* img, int w, i
     std::vector<FindOpts> o(static_cast<size_t>(n_models));
     for (int i = 0; i < n_models; ++i) {
       models.push_back(static_cast<const Model*>(mps[i]));
-      const double* q = opts + 8 * i;
+      const double* q = opts + 9 * i;
       FindOpts& f = o[size_t(i)];
       f.start = q[0]; f.extent = q[1]; f.min_score = q[2]; f.max_overlap = q[3]; f.num_matches = size_t(q[4]);
       f.subpixel = q[5] != 0; f.highest = int(q[6]); f.lowest = int(q[7]);
+      if (q[8] > 0) f.max_candidates = size_t(q[8]);  // the reference's FindOptions::max_candidates (model.hpp:21)
     }
     std::vector<std::vector<Match>> prep;
     const Domain dom = reduced ? image_domain(dom_runs, n_dom_runs, h, w) : Domain{};

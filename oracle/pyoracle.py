@@ -266,17 +266,18 @@ def find_models(models: list, image: np.ndarray, opts: list, selection_matches: 
                 selection_overlap: float = -1.0, per_model_quota: bool = True, prepared: bool = False, domain=None):
     """find_models over oracle models.  opts: one dict per model with keys
     start, extent, min_score, max_overlap, num_matches, subpixel, highest,
-    lowest.  Returns (n, 5): row, col, angle, score (internal convention),
+    lowest and optionally max_candidates (the coarse candidate budget, 10 000
+    by default).  Returns (n, 5): row, col, angle, score (internal convention),
     model index; with prepared=True also each model's prepared list.  The
     image is u8 or u16 (by dtype)."""
     img = search_image(image)
     runs, n_runs, reduced = _domain(domain)
     n = len(models)
     handles = (C.c_void_p * n)(*[m.h.value for m in models])
-    o = np.zeros((n, 8))
+    o = np.zeros((n, 9))
     for i, q in enumerate(opts):
         o[i] = [q["start"], q["extent"], q["min_score"], q["max_overlap"], q["num_matches"], int(q.get("subpixel", True)),
-                q.get("highest", 0), q.get("lowest", 1)]
+                q.get("highest", 0), q.get("lowest", 1), q.get("max_candidates", 0)]
     cap, cap_p = 4096, 20000
     res = np.zeros((cap, 5))
     nres = C.c_int()

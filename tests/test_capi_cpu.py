@@ -16,7 +16,7 @@ def declared_symbols():
     names = set()
     for h in ("mv_ncc.h", "mvkit_amd.h", "ncchip.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
-        names |= set(re.findall(r"^(?:MV_API\s+)?(?:MvStatus|int)\s+((?:mv|mvamd|ncchip)_\w+)\s*\(", text, re.M))
+        names |= set(re.findall(r"^(?:MV_API\s+)?(?:MvStatus|int|size_t)\s+((?:mv|mvamd|ncchip)_\w+)\s*\(", text, re.M))
     return sorted(names)
 
 
