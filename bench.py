@@ -64,6 +64,150 @@ def hbm_bytes_per_frame(hip_kernel):
     return None
 
 
+def physical_cores() -> int:
+    """Physical cores of the host available to this process."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":")[1].strip()
+            elif line.startswith("core id"):
+                core = line.split(":")[1].strip()
+            elif not line.strip() and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    return max(1, min(avail, len(cores) if cores else avail))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, args):
+    """The CPU restatement (AVX2 build where the host has AVX2, one find per
+    thread) on frame 0 of the workload: the median latency of one core, and
+    the throughput of one find per physical core (at most 16, the GPU box's
+    share of the host) running concurrently."""
+    import threading
+    from oracle import pyoracle
+    avx2 = pyoracle.host_has_avx2()
+    o = pyoracle.Model(scene.template, levels=4, start=START, extent=EXTENT, avx2=avx2)
+    o.find(scene.image, START, EXTENT, 0.7, 5, 0.5)
+    lat = []
+    for _ in range(args.cpu_finds):
+        t = time.perf_counter()
+        o.find(scene.image, START, EXTENT, 0.7, 5, 0.5)
+        lat.append(time.perf_counter() - t)
+    one = float(np.median(lat))
+    n = min(16, physical_cores())
+    log(f"CPU baseline: 1-core median {one * 1e3:.1f} ms; {n} threads x {args.cpu_finds} finds")
+    models = [pyoracle.Model(scene.template, levels=4, start=START, extent=EXTENT, avx2=avx2) for _ in range(n)]
+
+    def worker(m):
+        for _ in range(args.cpu_finds):
+            m.find(scene.image, START, EXTENT, 0.7, 5, 0.5)
+    threads = [threading.Thread(target=worker, args=(m,)) for m in models]
+    t = time.perf_counter()
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    wall = time.perf_counter() - t
+    mpx = 2048 * 2048 * n * args.cpu_finds / wall / 1e6
+    return {"value": round(mpx, 3), "unit": "Mpx/s", "cores": n, "kind": "port",
+            "sample": f"{n} threads x {args.cpu_finds} finds of frame 0, one find per physical core at a time "
+                      f"(FP64 restatement, {'AVX2' if avx2 else 'scalar'} build)",
+            "latency_ms_1core_median": round(one * 1e3, 2), "mpx_s_1core": round(2048 * 2048 / one / 1e6, 3),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(), "avx2_build": avx2}
+
+
+def batched_models(capi, torch, args):
+    """The batched heterogeneous workload: 32 models (templates 32-256 px per
+    side, +-10 deg / +-45 deg / full turn) x 16 images (640x480 .. 4096x3072,
+    0-6 instances each, N(0,6)) in one mvamd_find_ncc_models_batch call,
+    split-BF16, MinScore 0.8, the coarse candidate budget raised to 100 000
+    (small templates exceed the reference's 10 000 on the large images); the
+    persistent coarse launch timed by HIP events, and against one coarse
+    launch per query."""
+    from mvkit_amd import synth
+    log("batched models: generating 32 models x 16 images")
+    w = synth.batch_workload()
+    t = time.perf_counter()
+    models = [capi.NccModel(m.template, m.levels, m.start, m.extent, m.step if m.step else "auto") for m in w.models]
+    create_s = time.perf_counter() - t
+    dev = [torch.from_numpy(im).cuda() for im in w.images]
+    shapes = [im.shape for im in w.images]
+    mpx = sum(a * b for a, b in shapes) / 1e6
+    opts = [capi.NccModel.options(m.start, m.extent, 0.8, 0, 0.5, form=capi.FORM_BF16X3, max_candidates=100000)
+            for m in w.models]
+    ptrs, ws, hs = [d.data_ptr() for d in dev], [s[1] for s in shapes], [s[0] for s in shapes]
+
+    def call(queue):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, st = capi.find_models_batch(models, ptrs, ws, hs, ws, opts, 0, 0.5, coarse_queue=queue, stats=True)
+        return res, st, time.perf_counter() - t0
+    call(True)
+    runs = [call(True) for _ in range(3)]
+    res, st, _ = runs[-1]
+    dt = float(np.median([r[2] for r in runs]))
+    per_query = float(np.median([call(False)[2] for _ in range(2)]))
+    hit = sum(any(int(x[4]) == k and math.hypot(x[0] - cy, x[1] - cx) < 1.0 for x in r)
+              for r, tr in zip(res, w.truth) for k, cy, cx, _ in tr)
+    tflops = st.coarse_flops / st.coarse_queue_us / 1e6 if st.coarse_queue_us else 0.0
+    out = {"workload": "32 models x 16 images (batch_workload, seed 0x6D766B6974+4)", "images": len(dev),
+           "models": len(models), "mpx": round(mpx, 1), "ms_per_batch": round(dt * 1e3, 1),
+           "images_per_s": round(len(dev) / dt, 2), "mpx_s": round(mpx / dt, 1),
+           "per_query_launches_ms": round(per_query * 1e3, 1), "speedup_persistent": round(per_query / dt, 3),
+           "instances": sum(len(tr) for tr in w.truth), "instances_found": hit,
+           "matches": sum(len(r) for r in res), "candidates": st.candidates, "model_create_s": round(create_s, 2),
+           "coarse_queue": {"kernel": "ncc::k_coarse_queue", "us": round(st.coarse_queue_us, 1),
+                            "algorithmic_gflop": round(st.coarse_flops / 1e9, 1), "achieved": round(tflops, 2),
+                            "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 5)},
+           "min_score": 0.8, "max_candidates": 100000}
+    log(f"batched models: {out}")
+    return out
+
+
+def precision_sweep(capi, args):
+    """Each correlation form's coarse score error against the FP64 oracle on
+    the full-turn workload (2048^2, 256^2 template, 5 levels, AngleStep 0.035),
+    and whether its matches are the oracle's within the parity tolerances."""
+    from mvkit_amd import synth
+    from oracle import pyoracle
+    log("precision sweep: full-turn workload")
+    s = synth.workload_full_2048()
+    o = pyoracle.Model(s.template, levels=5, start=0.0, extent=2 * math.pi, step=0.035)
+    want, tr = o.find(s.image, 0.0, 2 * math.pi, 0.7, 5, 0.5, True, trace=True)
+    ref = {(r, c, a): sc for r, c, a, sc in tr["coarse"]}
+    m = capi.NccModel(s.template, 5, 0.0, 2 * math.pi, 0.035)
+    out = {"workload": "full2048: 2048^2, 256^2 template x5, 0..2pi, 5 levels, AngleStep 0.035", "forms": {}}
+    for name, form in (("fp32", 0), ("bf16x3", 1), ("fp8_screen", 3)):
+        g = m.find_trace(s.image, capi.NccModel.options(0.0, 2 * math.pi, 0.7, 5, 0.5, form=form))
+        errs = np.array([abs(sc - ref[(r, c, a)]) for r, c, a, sc in g["coarse"] if (r, c, a) in ref])
+        res = g["results"]
+        same = res.shape == want.shape and (len(want) == 0 or (
+            np.abs(res[:, :2] - want[:, :2]).max() <= 0.02 and np.abs(res[:, 2] - want[:, 2]).max() <= math.radians(0.01)
+            and np.abs(res[:, 3] - want[:, 3]).max() <= 1e-4))
+        out["forms"][name] = {"coarse_err_max": float(errs.max()), "coarse_err_rms": float(np.sqrt((errs ** 2).mean())),
+                              "peaks": int(len(errs)), "oracle_peaks": len(ref), "matches_oracle": bool(same)}
+    log(f"precision sweep: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +217,15 @@ def main():
     ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--compare-steps", type=int, default=20,
                     help="timed steps of the plain FP32 form for the comparison (0: skip)")
-    ap.add_argument("--cpu-finds", type=int, default=24, help="finds in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-finds", type=int, default=6,
+                    help="finds per thread in the CPU baseline sample: the 1-core latency sample and each core's "
+                         "share of the all-core sample (0: skip)")
+    ap.add_argument("--latency-calls", type=int, default=200,
+                    help="single-frame mv_find_ncc_model calls timed for p50/p99 (0: skip)")
+    ap.add_argument("--batch-models", type=int, default=1,
+                    help="time the batched heterogeneous workload (32 models x 16 images, 0: skip)")
+    ap.add_argument("--precision-sweep", type=int, default=1,
+                    help="report each correlation form's coarse score error on the full-turn workload (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 64 MiB device-to-device copies (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
@@ -255,19 +407,30 @@ def main():
         create_ms[name] = round((time.perf_counter() - t) / 3 * 1e3, 2)
     log(f"model create ms: {create_ms}")
 
-    cpu = None
-    if args.cpu_finds > 0:
-        from oracle import pyoracle
-        log(f"CPU baseline: {args.cpu_finds} oracle finds on one core")
-        o = pyoracle.Model(scenes[0].template, levels=4, start=START, extent=EXTENT)
-        o.find(scenes[0].image, START, EXTENT, 0.7, 5, 0.5)
-        t = time.perf_counter()
-        for _ in range(args.cpu_finds):
-            o.find(scenes[0].image, START, EXTENT, 0.7, 5, 0.5)
-        cpu_s = (time.perf_counter() - t) / args.cpu_finds
-        cpu = {"value": round(2048 * 2048 / cpu_s / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_finds} finds of frame 0 (scalar FP64 restatement, one thread)",
-               "ms_per_frame": round(cpu_s * 1e3, 2)}
+    # single-frame latency of the drop-in entry: mv_find_ncc_model on a host
+    # image object (upload, find, refinement, tuples with the HALCON shift)
+    latency = None
+    if args.latency_calls > 0:
+        log(f"single-frame latency: {args.latency_calls} mv_find_ncc_model calls")
+        mc = capi.NccModel(scenes[0].template, 4, START, EXTENT)
+        img = capi.Image(scenes[0].image)
+        for _ in range(10):
+            mc.find(img, START, EXTENT, 0.7, 5, 0.5, "true", 0)
+        lat1 = []
+        for _ in range(args.latency_calls):
+            t = time.perf_counter()
+            got = mc.find(img, START, EXTENT, 0.7, 5, 0.5, "true", 0)
+            lat1.append(time.perf_counter() - t)
+        assert len(got) == 5
+        lat1 = np.array(lat1) * 1e3
+        latency = {"entry": "mv_find_ncc_model (host u8 image object, C ABI)", "calls": args.latency_calls,
+                   "p50_ms": round(float(np.percentile(lat1, 50)), 4), "p99_ms": round(float(np.percentile(lat1, 99)), 4),
+                   "mpx_s_at_p50": round(2048 * 2048 / (np.percentile(lat1, 50) * 1e-3) / 1e6, 1)}
+        log(f"single-frame latency: {latency}")
+
+    batched = batched_models(capi, torch, args) if args.batch_models else None
+    sweep = precision_sweep(capi, args) if args.precision_sweep else None
+    cpu = cpu_baseline(scenes[0], args) if args.cpu_finds > 0 else None
 
     line = {
         "metric": "search Mpx/s & ms/frame, 2048\u00b2 \u00b130\u00b0: split-BF16 vs FP32 at \u22641e-4 score err",
@@ -293,6 +456,9 @@ def main():
         "fp8_screened_form": fp8,
         "coarse_score_error_vs_fp64": score_err,
         "model_create_ms": create_ms,
+        "single_frame_latency": latency,
+        "batched_models": batched,
+        "precision_sweep": sweep,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -40,6 +40,10 @@ typedef struct MvAmdFindOptions {
   int64_t max_candidates;                /* coarse candidate budget: the reference's internal
                                             FindOptions::max_candidates (model.hpp:21), which its
                                             C ABI leaves at 10 000; 0 selects 10 000 */
+  int32_t no_graph;                      /* 0: the device part of a find whose model, options, frame
+                                            geometry and workspace repeat is replayed as a captured
+                                            HIP graph; 1: always launched kernel by kernel */
+  int32_t reserved;
 } MvAmdFindOptions;
 
 /* Pixel types of device-resident search images ("byte" and "uint2" in the
@@ -109,14 +113,24 @@ MV_API MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t* d_
  * query per distinct (model, options) covers all images (one launch per
  * stage); coarse_queue != 0 runs the split-BF16 coarse searches of every
  * query in one persistent launch fed by an (image, angle tile, block)
- * work queue (0: one launch per query).  At most 256 images.
+ * work queue (0: one launch per query).  At most 256 images.  stats
+ * (optional) receives the call's totals.
  * Replaces, for many device-resident images, mv_find_ncc_models
  * (mvkit include/mvkit/c/mv_ncc_matching.h:119-123) per image. */
+typedef struct MvAmdBatchStats {
+  double coarse_queue_us;   /* device time of the persistent coarse launches (HIP events) */
+  double coarse_flops;      /* algorithmic FLOPs of every query's coarse search */
+  double total_us;          /* host time of the call */
+  int32_t queries;          /* distinct (model, options) queries */
+  int32_t queue_queries;    /* those searched in the persistent launch */
+  int32_t candidates;       /* full-scored candidates over all queries and images */
+  int32_t reserved;
+} MvAmdBatchStats;
 MV_API MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t* const* d_images, const int64_t* widths,
                                             const int64_t* heights, const int64_t* pitches, size_t n_images,
                                             const MvAmdFindOptions* options, size_t n_options,
                                             int64_t selection_matches, double selection_overlap, int32_t coarse_queue,
-                                            double* results, size_t cap, size_t* counts);
+                                            double* results, size_t cap, size_t* counts, MvAmdBatchStats* stats);
 
 /* Finds the model in n u8 (or u16) images given as HIP device pointers, all
  * with the same options, stage by stage over the whole batch (one launch per stage for

@@ -63,7 +63,8 @@ class FindOptions(C.Structure):
                 ("max_overlap", C.c_double), ("num_matches", C.c_int64), ("subpixel", C.c_int32),
                 ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
                 ("screen", C.c_double), ("profile", C.c_int32), ("vector_tracking", C.c_int32),
-                ("pixel_type", C.c_int32), ("max_candidates", C.c_int64)]
+                ("pixel_type", C.c_int32), ("max_candidates", C.c_int64), ("no_graph", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class FindStats(C.Structure):
@@ -73,6 +74,12 @@ class FindStats(C.Structure):
                 ("kernel_us", C.c_double * 8), ("coarse_flops", C.c_double),
                 ("track_flops", C.c_double), ("track_mfma_flops", C.c_double), ("coarse_form", C.c_int32),
                 ("reserved", C.c_int32)]
+
+class BatchStats(C.Structure):
+    _fields_ = [("coarse_queue_us", C.c_double), ("coarse_flops", C.c_double), ("total_us", C.c_double),
+                ("queries", C.c_int32), ("queue_queries", C.c_int32), ("candidates", C.c_int32),
+                ("reserved", C.c_int32)]
+
 
 PROFILE_ALL = 0xff
 STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine", "track_mfma"]
@@ -139,7 +146,8 @@ def lib() -> C.CDLL:
         L.mv_get_serialized_item_ptr.argtypes = [P, C.POINTER(P), C.POINTER(P)]
         L.mvamd_find_ncc_models_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), S, I64, D, P, S,
                                                    C.POINTER(S)]
-        L.mvamd_find_ncc_models_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), S, I64, D, I32, P, S, P]
+        L.mvamd_find_ncc_models_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), S, I64, D, I32, P, S, P,
+                                                  C.POINTER(BatchStats)]
         L.ncchip_coarse_mfma_queue_bytes.restype = S
         L.mvamd_find_ncc_model_image.argtypes = [P, P, C.POINTER(FindOptions), P, S, C.POINTER(S),
                                                  C.POINTER(FindStats)]
@@ -416,14 +424,16 @@ class NccModel:
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
                 highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0,
-                vector_tracking=False, pixel_type=PIXEL_BYTE, max_candidates=0) -> FindOptions:
+                vector_tracking=False, pixel_type=PIXEL_BYTE, max_candidates=0, graph=True) -> FindOptions:
         """profile: True times every stage, an int is a mask of stage bits
         (bit i = STAGES[i]).  vector_tracking: track every level with the FP64
         vector kernel instead of matrix cores (same results).  pixel_type:
-        PIXEL_BYTE or PIXEL_UINT2 for the device images (pitches in bytes)."""
+        PIXEL_BYTE or PIXEL_UINT2 for the device images (pitches in bytes).
+        graph: replay the device part as a captured HIP graph when it repeats."""
         mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
-                           lowest, form, screen, mask, int(vector_tracking), int(pixel_type), int(max_candidates))
+                           lowest, form, screen, mask, int(vector_tracking), int(pixel_type), int(max_candidates),
+                           int(not graph), 0)
 
     def find_image(self, image, opts: FindOptions, cap: int = 4096):
         """mvamd_find_ncc_model_image: find on an Image (or array) with the
@@ -554,11 +564,11 @@ def find_models_device(models, d_image: int, width: int, height: int, pitch: int
 
 
 def find_models_batch(models, d_images: list, widths, heights, pitches, opts, selection_matches=-1,
-                      selection_overlap=-1.0, coarse_queue=True, cap: int = 1024):
+                      selection_overlap=-1.0, coarse_queue=True, cap: int = 1024, stats: bool = False):
     """mvamd_find_ncc_models_batch: find_models over several device-resident
     images (widths/heights/pitches: one value or one per image).  Returns one
     (n, 5) array per image: row, col, angle, score (internal convention),
-    model index."""
+    model index; with stats=True also the call's BatchStats."""
     n = len(d_images)
 
     def per_image(v):
@@ -572,10 +582,12 @@ def find_models_batch(models, d_images: list, widths, heights, pitches, opts, se
     ps = (C.c_int64 * n)(*per_image(pitches))
     res = np.zeros((max(n, 1), cap, 5))
     counts = np.zeros(max(n, 1), np.uint64)
+    st = BatchStats()
     _check(lib().mvamd_find_ncc_models_batch(ids.h, ptrs, ws, hs, ps, n, arr, len(ol), int(selection_matches),
                                              float(selection_overlap), int(bool(coarse_queue)), _ptr(res), cap,
-                                             _ptr(counts)))
-    return [res[k, :min(int(counts[k]), cap)].copy() for k in range(n)]
+                                             _ptr(counts), C.byref(st) if stats else None))
+    out = [res[k, :min(int(counts[k]), cap)].copy() for k in range(n)]
+    return (out, st) if stats else out
 
 
 def halcon_shift(poses: np.ndarray) -> np.ndarray:

@@ -417,6 +417,7 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
     fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, "find_ncc_model", "max_candidates",
          "candidate budget must be 0 (the default) or 1..2^24");
   if (o->max_candidates > 0) q.max_candidates = size_t(o->max_candidates);
+  q.graph = o->no_graph == 0;
   if (q.form < MVAMD_FORM_AUTO || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
   if (o->pixel_type != MVAMD_PIXEL_BYTE && o->pixel_type != MVAMD_PIXEL_UINT2)
@@ -1249,7 +1250,8 @@ extern "C" MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t*
                                                 const int64_t* widths, const int64_t* heights, const int64_t* pitches,
                                                 size_t n_images, const MvAmdFindOptions* options, size_t n_options,
                                                 int64_t selection_matches, double selection_overlap,
-                                                int32_t coarse_queue, double* results, size_t cap, size_t* counts) {
+                                                int32_t coarse_queue, double* results, size_t cap, size_t* counts,
+                                                MvAmdBatchStats* stats) {
   return translate([&] {
     constexpr auto op = "find_ncc_models";
     required(d_images, op, "Images");
@@ -1279,10 +1281,20 @@ extern "C" MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t*
     mvamd::Selection sel;
     sel.num_matches = selection_matches;
     sel.max_overlap = selection_overlap;
+    mvamd::BatchStats bs;
     const auto found = algorithm(op, [&] {
       return context().find_models_frames(ptrs.data(), qs.data(), ptrs.size(), frames.data(), n_images, sel, true,
-                                          coarse_queue != 0);
+                                          coarse_queue != 0, stats ? &bs : nullptr);
     });
+    if (stats) {
+      stats->coarse_queue_us = bs.coarse_queue_us;
+      stats->coarse_flops = bs.coarse_flops;
+      stats->total_us = bs.total_us;
+      stats->queries = bs.queries;
+      stats->queue_queries = bs.queue_queries;
+      stats->candidates = bs.candidates;
+      stats->reserved = 0;
+    }
     for (size_t k = 0; k < n_images; ++k) {
       for (size_t i = 0; results && i < found[k].size() && i < cap; ++i) {
         double* d = results + 5 * (cap * k + i);

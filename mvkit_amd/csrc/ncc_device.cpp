@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <utility>
 
 namespace mvamd {
 
@@ -407,6 +408,16 @@ struct FindContext::Plan {
   std::vector<size_t> off_refs;
   uint8_t* rb = nullptr;
   std::vector<int32_t> frame_dims;  // top-level (w, h) per frame, for the coarse queue
+  size_t off_small = 0, small = 0, off_bits = 0, n_bit_words = 0;
+  std::vector<int> track_rc;        // per tracking level: ncchip_track_mfma's return (-6: vector kernel)
+};
+
+// A captured device part of a batch, replayed while its key repeats.
+struct SlotGraph {
+  std::string key;
+  hipGraphExec_t exec = nullptr;
+  std::shared_ptr<DeviceModel> dmodel;  // its device raster pool stays at the captured address
+  std::vector<int> track_rc;
 };
 
 // A submitted batch: its buffers (reused by later batches in the same slot)
@@ -443,6 +454,8 @@ struct FindContext::Slot {
   std::vector<bool> track_mfma;  // per tracking level: launched on matrix cores
   double t0 = 0, t1 = 0, t2 = 0;
   std::unique_ptr<Plan> plan;  // between the two halves of enqueue
+  std::vector<int> track_rc;   // the last batch's tracking kernel choices
+  std::vector<SlotGraph> graphs;  // captured device parts, most recent first
 };
 
 FindContext::FindContext() {
@@ -520,13 +533,30 @@ void FindContext::time_collect(FindTrace* trace) {
   S.timers_used = 0;
 }
 
+// Copies a host image (rows of `row` bytes, `pitch` apart) to image_ on the
+// context stream as one transfer: directly when the rows are contiguous,
+// else gathered into pinned staging first (a strided copy from pageable
+// memory moves row by row).
+void FindContext::upload_image(const uint8_t* image, size_t row, size_t height, size_t pitch) {
+  const size_t bytes = row * height;
+  if (pitch == row) {
+    check(hipMemcpyAsync(image_.ptr, image, bytes, hipMemcpyHostToDevice, stream_), "image upload");
+    return;
+  }
+  check(hipStreamSynchronize(stream_), "sync");  // the staging of an earlier upload is free
+  upload_stage_.reserve(bytes);
+  auto* dst = static_cast<uint8_t*>(upload_stage_.ptr);
+  for (size_t r = 0; r < height; ++r) std::memcpy(dst + r * row, image + r * pitch, row);
+  check(hipMemcpyAsync(image_.ptr, dst, bytes, hipMemcpyHostToDevice, stream_), "image upload");
+}
+
 std::vector<Pose> FindContext::find_host(const NccModel& model, const uint8_t* image, size_t width, size_t height,
                                          size_t pitch, const FindOptions& options, FindTrace* trace, int pixel_bytes,
                                          const Region* domain) {
   if (!width || !height) raise(kInvalidArgument, "image", "one nonempty single-channel image required");
   const size_t row = width * size_t(pixel_bytes), bytes = row * height;
   if (image_.bytes < bytes) image_ = DeviceBuffer(align_up(bytes, 1 << 20));
-  check(hipMemcpy2DAsync(image_.ptr, row, image, pitch, row, height, hipMemcpyHostToDevice, stream_), "image upload");
+  upload_image(image, row, height, pitch);
   FrameInput in{image_.as<uint8_t>(), width, height, row};
   in.pixel_bytes = pixel_bytes;
   in.domain = domain;
@@ -546,7 +576,7 @@ std::vector<ModelMatch> FindContext::find_models_host(const NccModel* const* mod
     check(hipStreamSynchronize(stream_), "sync");
     image_ = DeviceBuffer(align_up(bytes, 1 << 20));
   }
-  check(hipMemcpy2DAsync(image_.ptr, row, image, pitch, row, height, hipMemcpyHostToDevice, stream_), "image upload");
+  upload_image(image, row, height, pitch);
   FrameInput in{image_.as<uint8_t>(), width, height, row};
   in.pixel_bytes = pixel_bytes;
   in.domain = domain;
@@ -590,7 +620,81 @@ uint64_t FindContext::submit_batch(const NccModel& model, const FrameInput* inpu
 
 void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inputs, size_t n_frames,
                           const FindOptions& req, FindTrace* traces) {
-  if (enqueue_front(S, model, inputs, n_frames, req, traces)) enqueue_back(S);
+  if (!enqueue_front(S, model, inputs, n_frames, req, traces)) return;
+  hipStream_t qs = S.plan->qs;
+  if (graphs_enabled_ && S.plan->o.graph && !S.profiling && !S.plan->reduced) {
+    launch_graph(S);
+  } else {
+    front_launches(S);
+    enqueue_back(S);
+  }
+  check(hipEventRecord(S.done, qs), "hipEventRecord");
+}
+
+// The key of a planned batch's device work: everything its kernel arguments
+// depend on (the frames' pixels and addresses reach the kernels through the
+// uploaded descriptors, not through arguments).
+std::string FindContext::graph_key(Slot& S) {
+  const Plan& P = *S.plan;
+  std::string k;
+  auto put = [&](const auto& v) { k.append(reinterpret_cast<const char*>(&v), sizeof(v)); };
+  put(P.dmodel.get()); put(S.work.ptr); put(S.stage.ptr); put(S.readback.ptr); put(P.qs);
+  const FindOptions& o = P.o;
+  put(o.start); put(o.extent); put(o.min_score); put(o.max_candidates); put(o.subpixel); put(o.highest); put(o.lowest);
+  put(o.form); put(o.screen); put(o.track_mfma); put(o.query_cache); put(o.whole_level0);
+  put(P.n_frames);
+  for (const Job& J : P.jobs) { put(J.in.width); put(J.in.height); put(J.in.pitch); put(J.in.pixel_bytes); }
+  return k;
+}
+
+// The device work of a planned batch as a HIP graph: captured the first time
+// a key is seen (launching it kernel by kernel inside the capture), replayed
+// afterwards with the host bookkeeping of the back half run without launches.
+void FindContext::launch_graph(Slot& S) {
+  Plan& P = *S.plan;
+  hipStream_t qs = P.qs;
+  const std::string key = graph_key(S);
+  for (size_t i = 0; i < S.graphs.size(); ++i) {
+    if (S.graphs[i].key != key) continue;
+    std::rotate(S.graphs.begin(), S.graphs.begin() + long(i), S.graphs.begin() + long(i) + 1);  // most recent first
+    SlotGraph& g = S.graphs.front();
+    P.track_rc = g.track_rc;
+    dry_ = true;
+    try {
+      enqueue_back(S);
+    } catch (...) {
+      dry_ = false;
+      throw;
+    }
+    dry_ = false;
+    check(hipGraphLaunch(g.exec, qs), "hipGraphLaunch");
+    return;
+  }
+  SlotGraph entry;
+  entry.key = key;
+  entry.dmodel = P.dmodel;
+  check(hipStreamBeginCapture(qs, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+  hipGraph_t graph = nullptr;
+  try {
+    front_launches(S);
+    enqueue_back(S);
+  } catch (...) {
+    (void)hipStreamEndCapture(qs, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  check(hipStreamEndCapture(qs, &graph), "hipStreamEndCapture");
+  const hipError_t e = hipGraphInstantiate(&entry.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  check(e, "hipGraphInstantiate");
+  entry.track_rc = S.track_rc;
+  check(hipGraphLaunch(entry.exec, qs), "hipGraphLaunch");
+  constexpr size_t kGraphsPerSlot = 8;
+  if (S.graphs.size() == kGraphsPerSlot) {  // the least recent is idle: its slot's last batch completed
+    (void)hipGraphExecDestroy(S.graphs.back().exec);
+    S.graphs.pop_back();
+  }
+  S.graphs.insert(S.graphs.begin(), std::move(entry));
 }
 
 // The front half of a batch: host planning, the small uploads, the search
@@ -951,7 +1055,6 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
       }
       fr[f] = d;
     }
-    check(hipMemcpyAsync(dev + off_small, st, small, hipMemcpyHostToDevice, qs), "upload");
   }
   auto max_dims = [&](size_t l) {
     int w = 0, h = 0;
@@ -960,6 +1063,49 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
   };
   const int F = int(n_frames);
 
+  const auto [Wt_max, Ht_max] = max_dims(top);
+  // ---- the plan of the back half
+  S.plan = std::make_unique<Plan>();
+  Plan& P = *S.plan;
+  P.qs = qs; P.model = &model; P.dmodel = dmodel; P.o = o; P.traces = traces; P.n_frames = n_frames;
+  P.level_count = level_count; P.top = top; P.finest = finest; P.stride = stride; P.n_track = n_track; P.A = A;
+  P.cap = cap; P.cap_all = cap_all; P.table_size = table_size; P.n_count_words = n_count_words;
+  P.track_levels = std::move(track_levels); P.refs = std::move(refs); P.qpool = std::move(qpool);
+  P.coarse_original = coarse_original; P.jobs = std::move(jobs); P.dense = std::move(dense);
+  P.use_mfma = use_mfma; P.track_on_mfma = track_on_mfma; P.wide = wide; P.reduced = reduced;
+  P.max_bh = max_bh; P.max_bw = max_bw; P.F = F; P.Wt_max = Wt_max; P.Ht_max = Ht_max;
+  P.scale = scale; P.phase = phase; P.radius = radius; P.t0 = t0;
+  P.dev = dev; P.dframes = dframes; P.lists = lists; P.counts = counts; P.extent = extent; P.work = work;
+  P.pools[0] = pools[0]; P.pools[1] = pools[1];
+  P.off_key = off_key; P.off_raw = off_raw; P.off_table = off_table; P.off_wh = off_wh; P.off_wl = off_wl;
+  P.off_wsum = off_wsum; P.off_common = off_common; P.rb_full = rb_full; P.rb_counts = rb_counts;
+  P.off_refs = std::move(off_refs); P.rb = rb;
+  P.off_small = off_small; P.small = small; P.off_bits = off_bits; P.n_bit_words = n_bit_words;
+  return true;
+}
+
+// The front half's device work: the small uploads, the clearing of counters,
+// hash tables and peak bits, the reduced-domain table, the search pyramid
+// and the top-level row prefixes.
+void FindContext::front_launches(Slot& S) {
+  Plan& P = *S.plan;
+  cur_ = &S;
+  hipStream_t qs = P.qs;
+  uint8_t* const dev = P.dev;
+  const size_t n_frames = P.n_frames, n_track = P.n_track, table_size = P.table_size, level_count = P.level_count;
+  const size_t top = P.top, n_count_words = P.n_count_words, n_bit_words = P.n_bit_words, off_bits = P.off_bits;
+  const size_t off_table = P.off_table;
+  const std::vector<Job>& jobs = P.jobs;
+  const int F = P.F, Ht_max = P.Ht_max;
+  const bool wide = P.wide, reduced = P.reduced;
+  NccFrame* const dframes = P.dframes;
+  int32_t* const counts = P.counts;
+  auto max_dims = [&](size_t l) {
+    int w = 0, h = 0;
+    for (const auto& J : jobs) { w = std::max(w, J.lv[l].w); h = std::max(h, J.lv[l].h); }
+    return std::pair<int, int>(w, h);
+  };
+  check(hipMemcpyAsync(dev + P.off_small, S.stage.ptr, P.small, hipMemcpyHostToDevice, qs), "upload");
   // ---- the device-resident part: pyramid, top-level statistics, coarse
   // search, peaks, tracking, full score; one synchronisation at the end
   if (ncchip_fill_u32(reinterpret_cast<uint32_t*>(counts), n_count_words, 0u, qs) ||
@@ -983,28 +1129,10 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
       raise(kInternalError, "device", "pyramid launch failed");
   }
   time_end(qs);
-  const auto [Wt_max, Ht_max] = max_dims(top);
   time_begin(kStagePrefix, qs);
   if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, qs))
     raise(kInternalError, "device", "prefix launch failed");
   time_end(qs);
-  // ---- the plan of the back half
-  S.plan = std::make_unique<Plan>();
-  Plan& P = *S.plan;
-  P.qs = qs; P.model = &model; P.dmodel = dmodel; P.o = o; P.traces = traces; P.n_frames = n_frames;
-  P.level_count = level_count; P.top = top; P.finest = finest; P.stride = stride; P.n_track = n_track; P.A = A;
-  P.cap = cap; P.cap_all = cap_all; P.table_size = table_size; P.n_count_words = n_count_words;
-  P.track_levels = std::move(track_levels); P.refs = std::move(refs); P.qpool = std::move(qpool);
-  P.coarse_original = coarse_original; P.jobs = std::move(jobs); P.dense = std::move(dense);
-  P.use_mfma = use_mfma; P.track_on_mfma = track_on_mfma; P.wide = wide; P.reduced = reduced;
-  P.max_bh = max_bh; P.max_bw = max_bw; P.F = F; P.Wt_max = Wt_max; P.Ht_max = Ht_max;
-  P.scale = scale; P.phase = phase; P.radius = radius; P.t0 = t0;
-  P.dev = dev; P.dframes = dframes; P.lists = lists; P.counts = counts; P.extent = extent; P.work = work;
-  P.pools[0] = pools[0]; P.pools[1] = pools[1];
-  P.off_key = off_key; P.off_raw = off_raw; P.off_table = off_table; P.off_wh = off_wh; P.off_wl = off_wl;
-  P.off_wsum = off_wsum; P.off_common = off_common; P.rb_full = rb_full; P.rb_counts = rb_counts;
-  P.off_refs = std::move(off_refs); P.rb = rb;
-  return true;
 }
 
 // The split-BF16 coarse search of a planned batch as one ncchip_coarse_mfma
@@ -1081,7 +1209,8 @@ void FindContext::enqueue_back(Slot& S) {
   const auto* coarse_refs = reinterpret_cast<const int32_t*>(dev + off_refs[top]);
   time_begin(kStageCoarse, qs);
   int rc = 0;
-  if (use_mfma && !P.coarse_deferred) {
+  if (dry_) {
+  } else if (use_mfma && !P.coarse_deferred) {
     const NcchipCoarseJob j = coarse_job(S);
     rc = ncchip_coarse_mfma(j.frames, j.n_frames, j.level, j.level_kind, j.max_w, j.max_h, j.C, j.pools, j.patch_ids,
                             j.n_angles, &j.weights, j.exact_lattice, j.absolute, j.screen, qs);
@@ -1094,21 +1223,22 @@ void FindContext::enqueue_back(Slot& S) {
   time_end(qs);
   time_begin(kStagePeaks, qs);
   // reduced domain: anchors whose pose may not reach it score missing (matcher.cpp:556-578)
-  if (reduced && ncchip_domain_planes(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), scale, phase,
+  if (reduced && !dry_ && ncchip_domain_planes(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), scale, phase,
                                       use_mfma ? 2 : 0, qs))
     raise(kInternalError, "device", "domain launch failed");
-  if (ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
+  if (!dry_ && ncchip_peaks(dframes, F, int(top), Wt_max, Ht_max, pools, coarse_refs, int(A), o.min_score * .4, scale, phase,
                    lists, reinterpret_cast<int32_t*>(dev + off_key), int(cap), counts, extent, use_mfma ? 2 : 0,
                    qs))
     raise(kInternalError, "device", "peak launch failed");
   // find_models over several models converts the whole level 0 (matcher.cpp:596-611)
   // u16 frames keep the whole level 0 (matcher.cpp:612: only byte images are cut down)
-  if (!finest && !o.whole_level0 && !wide && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, qs))
+  if (!finest && !o.whole_level0 && !wide && !dry_ && ncchip_storage0(dframes, F, counts, extent, radius + 4 * scale + 8, qs))
     raise(kInternalError, "device", "storage launch failed");
   time_end(qs);
   // ---- coarse-to-fine tracking (matcher.cpp:630-761), all frames per launch
   constexpr int kListGrid = 2048;
   S.track_mfma.assign(n_track, false);
+  if (!dry_) P.track_rc.assign(n_track, -6);
   for (size_t k = 0; k < n_track; ++k) {
     const size_t l = track_levels[k];
     const double step = std::ldexp(1., int(l - 1)), offset = (step - 1) * .5;
@@ -1160,7 +1290,7 @@ void FindContext::enqueue_back(Slot& S) {
     int rc_track = -6;
     if (mfma) {
       time_begin(kStageTrackMfma, qs);
-      rc_track = ncchip_track_mfma(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
+      rc_track = dry_ ? P.track_rc[k] : ncchip_track_mfma(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                                    reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                                    lists + k * cap_all, counts + k * n_frames, int(cap),
                                    reinterpret_cast<NccMatch*>(dev + off_raw), lists + (k + 1) * cap_all,
@@ -1170,9 +1300,10 @@ void FindContext::enqueue_back(Slot& S) {
                                    model.params.absolute, mt_h, mt_w, xmax, work + k, kListGrid, qs);
       time_end(qs);
       S.track_mfma[k] = rc_track == 0;
+      P.track_rc[k] = rc_track;
     }
     if (rc_track == -6) time_begin(kStageTrack, qs);
-    if (rc_track == -6 &&
+    if (rc_track == -6 && !dry_ &&
         ncchip_track(dframes, F, int(l - 1), jobs[0].lv[l - 1].kind, pools,
                      reinterpret_cast<const int32_t*>(dev + off_refs[l - 1]), int(refs[l - 1].size()),
                      lists + k * cap_all, counts + k * n_frames, int(cap), reinterpret_cast<NccMatch*>(dev + off_raw),
@@ -1206,22 +1337,22 @@ void FindContext::enqueue_back(Slot& S) {
   // the domain (matcher.cpp:631-645); duplicates it makes are merged after the
   // full score, which only depends on the pose
   if (reduced && o.subpixel && finest == 0 && top >= 1 &&
-      ncchip_domain_snap(dframes, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), qs))
+      !dry_ && ncchip_domain_snap(dframes, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), qs))
     raise(kInternalError, "device", "domain launch failed");
   time_begin(kStageFull, qs);
-  if (ncchip_full_score(&rp, pools, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), hfull, 1024,
+  if (!dry_ && ncchip_full_score(&rp, pools, F, lists + n_track * cap_all, counts + n_track * n_frames, int(cap), hfull, 1024,
                         qs))
     raise(kInternalError, "device", "full score launch failed");
   time_end(qs);
-  if (ncchip_copy_u32(reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<uint32_t*>(hcounts), n_count_words,
+  if (!dry_ && ncchip_copy_u32(reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<uint32_t*>(hcounts), n_count_words,
                       qs))
     raise(kInternalError, "device", "copy launch failed");
-  check(hipEventRecord(S.done, qs), "hipEventRecord");
   S.model = &model; S.dmodel = dmodel; S.o = o; S.traces = traces; S.n_frames = n_frames; S.empty = false;
   S.jobs = std::move(jobs); S.qpool = std::move(qpool);
   S.cap = cap; S.cap_all = cap_all; S.n_track = n_track; S.top = top; S.finest = finest; S.stride = stride;
   S.off_key = off_key; S.dev = dev; S.dframes = dframes; S.lists = lists; S.hfull = hfull; S.hcounts = hcounts;
   S.rb = rb; S.pools[0] = pools[0]; S.pools[1] = pools[1]; S.rp = rp; S.t0 = t0;
+  S.track_rc = P.track_rc;
   S.plan.reset();
 }
 
@@ -1651,7 +1782,8 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
                                                                      const FrameInput* frames, size_t n_frames,
                                                                      const Selection& sel,
                                                                      bool per_model_quota_before_overlap,
-                                                                     bool coarse_queue) {
+                                                                     bool coarse_queue, BatchStats* stats) {
+  const double t_call = now_us();
   std::vector<std::vector<ModelMatch>> out(n_frames);
   if (!n_models || !n_frames) return out;
   if (n_frames > NCC_MAX_BATCH) raise(kLimitExceeded, "Images", "too many images for one batch");
@@ -1689,7 +1821,10 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
     o.trace_stages = false;
     Slot& S = *multi_slots_[slot_of[i]];
     cur_ = &S;
-    if (enqueue_front(S, *models[i], frames, n_frames, o, nullptr)) planned.push_back(&S);
+    if (enqueue_front(S, *models[i], frames, n_frames, o, nullptr)) {
+      front_launches(S);
+      planned.push_back(&S);
+    }
   }
   if (coarse_queue) {
     std::vector<NcchipCoarseJob> jobs;
@@ -1705,13 +1840,23 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
       const size_t bytes = ncchip_coarse_mfma_queue_bytes(int(jobs.size()), int(n_units));
       if (queue_dev_.bytes < bytes) queue_dev_ = DeviceBuffer(align_up(bytes, 1 << 16));
       queue_stage_.reserve(bytes);
+      if (stats && !queue_events_[0]) {
+        check(hipEventCreate(&queue_events_[0]), "hipEventCreate");
+        check(hipEventCreate(&queue_events_[1]), "hipEventCreate");
+      }
+      if (stats) check(hipEventRecord(queue_events_[0], stream_), "hipEventRecord");
       const int rc = ncchip_coarse_mfma_queue(jobs.data(), int(jobs.size()), queue_dev_.ptr, queue_dev_.bytes,
                                               queue_stage_.ptr, queue_stage_.bytes, stream_);
       if (rc) raise(kInternalError, "device", "coarse queue launch failed");
+      if (stats) check(hipEventRecord(queue_events_[1], stream_), "hipEventRecord");
       for (Slot* S : deferred) S->plan->coarse_deferred = true;
+      if (stats) stats->queue_queries = int(deferred.size());
     }
   }
-  for (Slot* S : planned) enqueue_back(*S);
+  for (Slot* S : planned) {
+    enqueue_back(*S);
+    check(hipEventRecord(S->done, stream_), "hipEventRecord");
+  }
   // ---- candidates per frame in (score, model, row, col, angle) order
   std::vector<std::vector<ModelMatch>> cand(n_frames);
   std::vector<std::vector<SelectionWalk::Ref>> where(n_frames);
@@ -1808,6 +1953,20 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
       result = w2.finish();
     }
     out[f] = std::move(result);
+  }
+  if (stats) {
+    stats->queries = int(n_queries);
+    for (size_t q = 0; q < n_queries; ++q) {
+      const Slot& S = *multi_slots_[q];
+      if (S.empty) continue;
+      for (const Job& J : S.jobs) { stats->coarse_flops += J.coarse_flops; stats->candidates += int(J.full.size()); }
+    }
+    if (stats->queue_queries) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, queue_events_[0], queue_events_[1]), "hipEventElapsedTime");
+      stats->coarse_queue_us = 1000.0 * ms;
+    }
+    stats->total_us = now_us() - t_call;
   }
   return out;
 }

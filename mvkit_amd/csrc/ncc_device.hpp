@@ -6,6 +6,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../../include/ncchip.h"
@@ -84,6 +85,9 @@ struct FindOptions {
   // FP8-screened forms (exact results; off: FP64 VALU kernel only).  The FP32
   // and FP64 forms track on the FP64 VALU kernel.
   bool track_mfma = true;
+  // replay the device part as a captured HIP graph when the batch's key
+  // (model, options, frame geometry, workspace) repeats
+  bool graph = true;
 };
 
 // Quota and overlap shared by all models of find_models (multi.hpp:14);
@@ -91,6 +95,14 @@ struct FindOptions {
 struct Selection { int64_t num_matches = -1; double max_overlap = -1; };
 // A find_models result: the pose and the index of its model in the call.
 struct ModelMatch { Pose pose; size_t model = 0; };
+
+// Totals of a find_models_frames call.
+struct BatchStats {
+  double coarse_queue_us = 0;   // device time of the persistent coarse launches (HIP events on their stream)
+  double coarse_flops = 0;      // algorithmic FLOPs of every query's coarse search
+  double total_us = 0;          // host time of the call
+  int queries = 0, queue_queries = 0, candidates = 0;
+};
 
 // Kernel stages timed in profiling mode.
 // kStageTrack: tracking levels on the FP64 vector kernel; kStageTrackMfma:
@@ -176,7 +188,7 @@ class FindContext {
                                                           size_t n_models, const FrameInput* frames, size_t n_frames,
                                                           const Selection& selection,
                                                           bool per_model_quota_before_overlap = true,
-                                                          bool coarse_queue = true);
+                                                          bool coarse_queue = true, BatchStats* stats = nullptr);
   // Same, from host memory.
   std::vector<ModelMatch> find_models_host(const NccModel* const* models, const FindOptions* options,
                                            size_t n_models, const uint8_t* image, size_t width, size_t height,
@@ -210,10 +222,14 @@ class FindContext {
                FindTrace* traces);
   bool enqueue_front(Slot& S, const NccModel& model, const FrameInput* frames, size_t n_frames,
                      const FindOptions& options, FindTrace* traces);
+  void front_launches(Slot& S);
+  std::string graph_key(Slot& S);
+  void launch_graph(Slot& S);
   void enqueue_back(Slot& S);
   NcchipCoarseJob coarse_job(Slot& S);
   void collect(Slot& S);
   void refine_round(std::vector<RefineRequest>& reqs);
+  void upload_image(const uint8_t* image, size_t row, size_t height, size_t pitch);
   void time_begin(int stage, hipStream_t stream);
   void time_end(hipStream_t stream);
   void time_collect(FindTrace* trace);
@@ -227,9 +243,13 @@ class FindContext {
   hipEvent_t uploaded_ = nullptr;  // orders the context stream's uploads before a slot's stream
   int device_ = 0;
   DeviceBuffer image_;
+  PinnedBuffer upload_stage_;  // strided host images gathered for one transfer
+  bool dry_ = false;            // the back half runs its host bookkeeping only (graph replay)
+  bool graphs_enabled_ = true;
   DeviceBuffer create_;  // model construction workspace (grow-only)
   DeviceBuffer queue_dev_;     // the persistent coarse search's work queue
   PinnedBuffer queue_stage_;   // its host-side staging
+  hipEvent_t queue_events_[2] = {nullptr, nullptr};  // around the persistent coarse launches
 };
 
 // Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).

@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libncc_oracle.so")
+LIB_AVX2 = os.path.join(HERE, "libncc_oracle_avx2.so")  # the same source built for AVX2 hosts (CPU baseline)
 
 
 def build(force: bool = False) -> str:
@@ -29,15 +30,24 @@ class FindOut(C.Structure):
                 ("store_h", C.c_long), ("store_w", C.c_long)]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def host_has_avx2() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return any(line.startswith("flags") and " avx2" in line for line in f)
+    except OSError:
+        return False
+
+
+def lib(avx2: bool = False):
+    key = "avx2" if avx2 else "scalar"
+    if key not in _libs:
+        path = LIB_AVX2 if avx2 else LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         P = C.c_void_p
         D = C.c_double
         I = C.c_int
@@ -69,8 +79,8 @@ def lib():
         L.orc_domain_runs.argtypes = [P, I, I, I, P, I]
         L.orc_checksum.argtypes = [P, C.c_size_t]
         L.orc_checksum.restype = C.c_uint64
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _ptr(a: np.ndarray):
@@ -85,19 +95,21 @@ class OracleError(RuntimeError):
 
 class Model:
     def __init__(self, template: np.ndarray, levels: int = 0, start: float = 0.0, extent: float = 0.0,
-                 step: float = 0.0, absolute: bool = False, domain=None):
+                 step: float = 0.0, absolute: bool = False, domain=None, avx2: bool = False):
         """domain: None (the whole template) or canonical runs (n x 3: row,
-        first column, last column) of the template's reduced domain."""
+        first column, last column) of the template's reduced domain.  avx2:
+        use the AVX2 build of the oracle (same results)."""
+        self._L = lib(avx2)
         t = np.ascontiguousarray(template, dtype=np.uint8)
         runs, n_runs, reduced = _domain(domain)
         h = C.c_void_p()
-        st = lib().orc_create(_ptr(t), t.shape[1], t.shape[0], t.shape[1], levels, start, extent, step,
+        st = self._L.orc_create(_ptr(t), t.shape[1], t.shape[0], t.shape[1], levels, start, extent, step,
                               int(absolute), _ptr(runs), n_runs, reduced, C.byref(h))
         if st:
-            raise OracleError(st, lib().orc_last_error().decode())
+            raise OracleError(st, self._L.orc_last_error().decode())
         self.h = h
         info = np.zeros(6)
-        lib().orc_model_info(self.h, _ptr(info))
+        self._L.orc_model_info(self.h, _ptr(info))
         self.n_levels, self.n_angles = int(info[0]), int(info[1])
         self.step, self.origin_row, self.origin_col, self.sample_count = info[2], info[3], info[4], int(info[5])
 
@@ -108,43 +120,44 @@ class Model:
         if st:
             raise OracleError(st, lib().orc_last_error().decode())
         m = cls.__new__(cls)
+        m._L = lib()
         m.h = h
         info = np.zeros(6)
-        lib().orc_model_info(m.h, _ptr(info))
+        m._L.orc_model_info(m.h, _ptr(info))
         m.n_levels, m.n_angles = int(info[0]), int(info[1])
         m.step, m.origin_row, m.origin_col, m.sample_count = info[2], info[3], info[4], int(info[5])
         return m
 
     def serialize(self) -> bytes:
         n = C.c_size_t()
-        st = lib().orc_serialize(self.h, None, 0, C.byref(n))
+        st = self._L.orc_serialize(self.h, None, 0, C.byref(n))
         if st:
-            raise OracleError(st, lib().orc_last_error().decode())
+            raise OracleError(st, self._L.orc_last_error().decode())
         buf = C.create_string_buffer(n.value)
-        lib().orc_serialize(self.h, buf, n.value, C.byref(n))
+        self._L.orc_serialize(self.h, buf, n.value, C.byref(n))
         return buf.raw[:n.value]
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_free(self.h)
+            self._L.orc_free(self.h)
             self.h = None
 
     def level(self, l: int) -> dict:
         out = np.zeros(4)
-        lib().orc_level_info(self.h, l, _ptr(out))
+        self._L.orc_level_info(self.h, l, _ptr(out))
         w, h = int(out[0]), int(out[1])
         g = np.zeros(w * h)
-        lib().orc_level_gray(self.h, l, _ptr(g))
+        self._L.orc_level_gray(self.h, l, _ptr(g))
         return {"w": w, "h": h, "orow": out[2], "ocol": out[3], "gray": g.reshape(h, w)}
 
     def patch(self, level: int, angle: int) -> dict:
         hdr = np.zeros(11)
-        lib().orc_patch_info(self.h, level, angle, _ptr(hdr))
+        self._L.orc_patch_info(self.h, level, angle, _ptr(hdr))
         n = int(hdr[0])
         rows = np.zeros(n, np.int32)
         cols = np.zeros(n, np.int32)
         w = np.zeros(n)
-        lib().orc_patch_samples(self.h, level, angle, _ptr(rows), _ptr(cols), _ptr(w))
+        self._L.orc_patch_samples(self.h, level, angle, _ptr(rows), _ptr(cols), _ptr(w))
         return {"n": n, "angle": hdr[1], "orow": hdr[2], "ocol": hdr[3], "rmin": int(hdr[4]), "rmax": int(hdr[5]),
                 "cmin": int(hdr[6]), "cmax": int(hdr[7]), "wsum": hdr[8], "n_runs": int(hdr[9]),
                 "n_wruns": int(hdr[10]), "rows": rows, "cols": cols, "w": w}
@@ -163,13 +176,13 @@ class Model:
         prep = np.zeros((20000, 4)) if trace else None
         refd = np.zeros((20000, 5)) if trace else None
         info = FindOut()
-        st = lib().orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize,
+        st = self._L.orc_find(self.h, _ptr(img), img.shape[1], img.shape[0], img.strides[0], img.itemsize,
                             _ptr(runs), n_runs, reduced, start, extent, min_score,
                             num_matches, max_overlap, int(subpixel), highest, lowest, _ptr(res), cap,
                             _ptr(coarse) if trace else None, 20000, _ptr(prep) if trace else None, 20000,
                             _ptr(refd) if trace else None, C.byref(info))
         if st:
-            raise OracleError(st, lib().orc_last_error().decode())
+            raise OracleError(st, self._L.orc_last_error().decode())
         out = res[:info.n_result].copy()
         if not trace:
             return out
@@ -185,11 +198,11 @@ class Model:
         out = np.zeros((cap, 4))
         counts = np.zeros(lists, np.int32)
         n = C.c_int()
-        st = lib().orc_find_tracked(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], start, extent,
+        st = self._L.orc_find_tracked(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], start, extent,
                                     min_score, num_matches, max_overlap, int(subpixel), highest, lowest, _ptr(out),
                                     cap, _ptr(counts), lists, C.byref(n))
         if st:
-            raise OracleError(st, lib().orc_last_error().decode())
+            raise OracleError(st, self._L.orc_last_error().decode())
         res, k = [], 0
         for i in range(n.value):
             res.append(out[k:k + counts[i]].copy())
@@ -198,7 +211,7 @@ class Model:
 
     def continuous(self, image: np.ndarray, row: float, col: float, angle: float) -> float:
         img = np.ascontiguousarray(image, dtype=np.uint8)
-        return lib().orc_continuous(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], row, col, angle)
+        return self._L.orc_continuous(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], row, col, angle)
 
 
 def pyramid(image: np.ndarray, levels: int) -> list:
