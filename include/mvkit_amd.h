@@ -132,6 +132,40 @@ MV_API MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t* con
                                             int64_t selection_matches, double selection_overlap, int32_t coarse_queue,
                                             double* results, size_t cap, size_t* counts, MvAmdBatchStats* stats);
 
+/* A request loop on the calling thread's current device: requests (one host
+ * image, 1..n models with find_ncc_models' options and selection) queue up,
+ * are admitted while the estimated device working sets of the requests in
+ * flight stay within hbm_budget bytes (the oldest request first; always when
+ * none is in flight), and run in micro-batches of up to max_batch requests
+ * by `workers` threads whose device work overlaps on the GPU; models stay
+ * resident on the device.  Each request's result (row, col, angle, score,
+ * model index; internal convention) is the one mvamd_find_ncc_models_device
+ * returns for it alone.  Zero fields select the defaults (2 workers, 8
+ * requests per batch, 16 GiB, 4096 waiting requests).  Submit and wait may
+ * be called from any threads; destroy waits for the requests in flight and
+ * must not race with submit or wait.  Replaces, for a serving loop,
+ * concurrent mv_find_ncc_models calls on shared models
+ * (docs/runtime.md:31). */
+typedef struct MvAmdServerConfig {
+  int32_t workers, max_batch;
+  int64_t hbm_budget;
+  int64_t queue_limit;
+  int32_t no_coarse_queue;  /* 1: one coarse launch per query instead of a persistent launch per batch */
+  int32_t reserved;
+} MvAmdServerConfig;
+typedef struct MvAmdRequestTimes {
+  double admitted_us, started_us, finished_us;  /* after submission */
+  double bytes;                                 /* estimated working set it was admitted with */
+  int32_t batch_size, worker;
+} MvAmdRequestTimes;
+MV_API MvStatus mvamd_server_create(const MvAmdServerConfig* config, void** server);
+MV_API MvStatus mvamd_server_destroy(void* server);
+MV_API MvStatus mvamd_server_submit(void* server, const uint8_t* image, int64_t width, int64_t height, int64_t pitch,
+                                    MvTuple ModelIDs, const MvAmdFindOptions* options, size_t n_options,
+                                    int64_t selection_matches, double selection_overlap, uint64_t* request);
+MV_API MvStatus mvamd_server_wait(void* server, uint64_t request, double* results, size_t cap, size_t* count,
+                                  MvAmdRequestTimes* times);
+
 /* Finds the model in n u8 (or u16) images given as HIP device pointers, all
  * with the same options, stage by stage over the whole batch (one launch per stage for
  * the candidates of every frame).  Frame k's matches go to

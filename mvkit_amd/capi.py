@@ -34,7 +34,8 @@ PUBLIC_SYMBOLS = [
     "mv_clear_handle", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
     "mv_get_serialized_item_ptr", "mv_write_ncc_model", "mv_read_ncc_model", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
     "mvamd_create_ncc_model", "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_image",
-    "mvamd_find_ncc_models_device", "mvamd_find_ncc_models_batch", "mvamd_find_ncc_model_batch", "mvamd_find_ncc_model_batch_submit",
+    "mvamd_find_ncc_models_device", "mvamd_find_ncc_models_batch", "mvamd_find_ncc_model_batch",
+    "mvamd_server_create", "mvamd_server_destroy", "mvamd_server_submit", "mvamd_server_wait", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
 ]
 # Symbols declared by include/ncchip.h.
@@ -74,6 +75,16 @@ class FindStats(C.Structure):
                 ("kernel_us", C.c_double * 8), ("coarse_flops", C.c_double),
                 ("track_flops", C.c_double), ("track_mfma_flops", C.c_double), ("coarse_form", C.c_int32),
                 ("reserved", C.c_int32)]
+
+class ServerConfig(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("max_batch", C.c_int32), ("hbm_budget", C.c_int64),
+                ("queue_limit", C.c_int64), ("no_coarse_queue", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RequestTimes(C.Structure):
+    _fields_ = [("admitted_us", C.c_double), ("started_us", C.c_double), ("finished_us", C.c_double),
+                ("bytes", C.c_double), ("batch_size", C.c_int32), ("worker", C.c_int32)]
+
 
 class BatchStats(C.Structure):
     _fields_ = [("coarse_queue_us", C.c_double), ("coarse_flops", C.c_double), ("total_us", C.c_double),
@@ -149,6 +160,11 @@ def lib() -> C.CDLL:
         L.mvamd_find_ncc_models_batch.argtypes = [P, P, P, P, P, S, C.POINTER(FindOptions), S, I64, D, I32, P, S, P,
                                                   C.POINTER(BatchStats)]
         L.ncchip_coarse_mfma_queue_bytes.restype = S
+        L.mvamd_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(P)]
+        L.mvamd_server_destroy.argtypes = [P]
+        L.mvamd_server_submit.argtypes = [P, P, I64, I64, I64, P, C.POINTER(FindOptions), S, I64, D,
+                                          C.POINTER(C.c_uint64)]
+        L.mvamd_server_wait.argtypes = [P, C.c_uint64, P, S, C.POINTER(S), C.POINTER(RequestTimes)]
         L.mvamd_find_ncc_model_image.argtypes = [P, P, C.POINTER(FindOptions), P, S, C.POINTER(S),
                                                  C.POINTER(FindStats)]
         L.mvamd_find_ncc_model_device.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S),
@@ -588,6 +604,42 @@ def find_models_batch(models, d_images: list, widths, heights, pitches, opts, se
                                              _ptr(counts), C.byref(st) if stats else None))
     out = [res[k, :min(int(counts[k]), cap)].copy() for k in range(n)]
     return (out, st) if stats else out
+
+
+class Server:
+    """The request loop (mvamd_server_*): submit() queues a host image and
+    its models, wait() returns the matches ((n, 5): row, col, angle, score,
+    model index) and the request's times."""
+
+    def __init__(self, workers=0, max_batch=0, hbm_budget=0, queue_limit=0, coarse_queue=True):
+        cfg = ServerConfig(workers, max_batch, hbm_budget, queue_limit, int(not coarse_queue), 0)
+        h = C.c_void_p()
+        _check(lib().mvamd_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def submit(self, image: np.ndarray, models, opts, selection_matches=-1, selection_overlap=-1.0) -> int:
+        img = np.ascontiguousarray(image)
+        ids = _model_ids(models)
+        ol = list(opts) if isinstance(opts, (list, tuple)) else [opts]
+        arr = (FindOptions * len(ol))(*ol)
+        rid = C.c_uint64()
+        _check(lib().mvamd_server_submit(self.h, _ptr(img), img.shape[1], img.shape[0], img.strides[0], ids.h, arr,
+                                         len(ol), int(selection_matches), float(selection_overlap), C.byref(rid)))
+        return rid.value
+
+    def wait(self, request: int, cap: int = 1024):
+        res = np.zeros((cap, 5))
+        n = C.c_size_t()
+        t = RequestTimes()
+        _check(lib().mvamd_server_wait(self.h, request, _ptr(res), cap, C.byref(n), C.byref(t)))
+        return res[:min(n.value, cap)].copy(), {"admitted_us": t.admitted_us, "started_us": t.started_us,
+                                                "finished_us": t.finished_us, "bytes": t.bytes,
+                                                "batch_size": t.batch_size, "worker": t.worker}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().mvamd_server_destroy(self.h))
+            self.h = None
 
 
 def halcon_shift(poses: np.ndarray) -> np.ndarray:

@@ -21,6 +21,7 @@
 #include "../../include/mvkit_amd.h"
 #include "ncc_device.hpp"
 #include "ncc_model.hpp"
+#include "ncc_serve.hpp"
 
 using mvamd::Error;
 
@@ -1302,6 +1303,82 @@ extern "C" MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t*
         d[0] = m.pose.row; d[1] = m.pose.col; d[2] = m.pose.angle; d[3] = m.pose.score; d[4] = double(m.model);
       }
       counts[k] = found[k].size();
+    }
+  });
+}
+
+// ============================================================ request loop
+extern "C" MvStatus mvamd_server_create(const MvAmdServerConfig* config, void** server) {
+  return translate([&] {
+    constexpr auto op = "server_create";
+    required(config, op, "config");
+    required(server, op, "server");
+    mvamd::ServeConfig c;
+    if (config->workers) c.workers = config->workers;
+    if (config->max_batch) c.max_batch = size_t(std::max(config->max_batch, 0));
+    if (config->hbm_budget > 0) c.hbm_budget = size_t(config->hbm_budget);
+    if (config->queue_limit > 0) c.queue_limit = size_t(config->queue_limit);
+    c.coarse_queue = config->no_coarse_queue == 0;
+    if (config->workers < 0 || config->max_batch < 0)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "config", "negative worker or batch count");
+    *server = algorithm(op, [&] { return new mvamd::NccServer(c); });
+  });
+}
+
+extern "C" MvStatus mvamd_server_destroy(void* server) {
+  return translate([&] { delete static_cast<mvamd::NccServer*>(server); });
+}
+
+extern "C" MvStatus mvamd_server_submit(void* server, const uint8_t* image, int64_t width, int64_t height,
+                                        int64_t pitch, MvTuple ModelIDs, const MvAmdFindOptions* options,
+                                        size_t n_options, int64_t selection_matches, double selection_overlap,
+                                        uint64_t* request) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_models";
+    required(server, op, "server");
+    required(image, op, "Image");
+    required(options, op, "options");
+    required(request, op, "request");
+    const auto ms = models_of(ModelIDs, op);
+    if (n_options != 1 && n_options != ms.size())
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidCardinality, op, "options", "one value or one value per model required");
+    std::vector<mvamd::FindOptions> qs;
+    for (size_t i = 0; i < n_options; ++i) {
+      qs.push_back(from_ext(options + i));
+      qs.back().profile = 0;
+      if (options[i].pixel_type != options[0].pixel_type)
+        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "options", "one pixel type for the image", int64_t(i));
+    }
+    const int bytes = options->pixel_type == MVAMD_PIXEL_UINT2 ? 2 : 1;
+    if (width <= 0 || height <= 0 || pitch < width * bytes)
+      fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "Image", "invalid image geometry");
+    mvamd::Selection sel;
+    sel.num_matches = selection_matches;
+    sel.max_overlap = selection_overlap;
+    *request = algorithm(op, [&] {
+      return static_cast<mvamd::NccServer*>(server)->submit(image, size_t(width), size_t(height), size_t(pitch), bytes,
+                                                            ms, std::move(qs), sel);
+    });
+  });
+}
+
+extern "C" MvStatus mvamd_server_wait(void* server, uint64_t request, double* results, size_t cap, size_t* count,
+                                      MvAmdRequestTimes* times) {
+  return translate([&] {
+    constexpr auto op = "find_ncc_models";
+    required(server, op, "server");
+    required(count, op, "count");
+    mvamd::ServeTimes t;
+    const auto found = algorithm(op, [&] { return static_cast<mvamd::NccServer*>(server)->wait(request, &t); });
+    for (size_t i = 0; results && i < found.size() && i < cap; ++i) {
+      double* d = results + 5 * i;
+      d[0] = found[i].pose.row; d[1] = found[i].pose.col; d[2] = found[i].pose.angle; d[3] = found[i].pose.score;
+      d[4] = double(found[i].model);
+    }
+    *count = found.size();
+    if (times) {
+      times->admitted_us = t.admitted_us; times->started_us = t.started_us; times->finished_us = t.finished_us;
+      times->bytes = t.bytes; times->batch_size = t.batch_size; times->worker = t.worker;
     }
   });
 }

@@ -1692,7 +1692,7 @@ namespace {
 // that stops where it needs refinements it does not have yet, so that the
 // walks of all frames of a batch share their refinement rounds.
 struct SelectionWalk {
-  struct Ref { size_t slot, index; };
+  struct Ref { size_t slot, frame, index; };  // query, frame within the query's batch, candidate
   const NccModel* const* models;
   const FindOptions* opts;
   size_t n_models;
@@ -1783,27 +1783,57 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
                                                                      const Selection& sel,
                                                                      bool per_model_quota_before_overlap,
                                                                      bool coarse_queue, BatchStats* stats) {
+  std::vector<FrameRequest> reqs(n_frames);
+  for (size_t f = 0; f < n_frames; ++f) {
+    FrameRequest& r = reqs[f];
+    r.frame = frames[f];
+    r.models.assign(models, models + n_models);
+    r.options.assign(opts, opts + n_models);
+    r.selection = sel;
+    r.per_model_quota_before_overlap = per_model_quota_before_overlap;
+  }
+  return find_requests(reqs.data(), n_frames, coarse_queue, stats);
+}
+
+std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameRequest* requests, size_t n_requests,
+                                                                bool coarse_queue, BatchStats* stats) {
   const double t_call = now_us();
-  std::vector<std::vector<ModelMatch>> out(n_frames);
-  if (!n_models || !n_frames) return out;
-  if (n_frames > NCC_MAX_BATCH) raise(kLimitExceeded, "Images", "too many images for one batch");
+  std::vector<std::vector<ModelMatch>> out(n_requests);
+  if (!n_requests) return out;
   // every slot of this call is reused: let earlier work on both streams end
   check(hipStreamSynchronize(stream_), "sync");
   check(hipStreamSynchronize(stream2_), "sync");
-  const bool several = n_models > 1;
-  bool distinct = false;
-  for (size_t i = 0; i < n_models; ++i) distinct |= models[i] != models[0];
-  // ---- one query per distinct (model, options) over all frames; aliases
-  // share its candidates (matcher.cpp:1087-1098)
-  std::vector<size_t> owner(n_models), slot_of(n_models, size_t(-1));
-  size_t n_queries = 0;
-  for (size_t i = 0; i < n_models; ++i) {
-    owner[i] = i;
-    for (size_t j = 0; j < i; ++j)
-      if (models[i] == models[j] && same_request(opts[i], opts[j])) { owner[i] = owner[j]; break; }
-    if (owner[i] == i) slot_of[i] = n_queries++;
+  // ---- queries: one per distinct (model, options, several models in the
+  // request), each over the frames of the requests that ask for it; within
+  // a request, aliases (same model, same options) share a query's
+  // candidates (matcher.cpp:1087-1098)
+  struct Query {
+    const NccModel* model;
+    FindOptions o;
+    bool several;
+    std::vector<size_t> requests;  // frame j of the query's batch is request requests[j]
+  };
+  std::vector<Query> queries;
+  // per request and model position: (query, frame within the query's batch)
+  std::vector<std::vector<std::pair<size_t, size_t>>> at(n_requests);
+  for (size_t r = 0; r < n_requests; ++r) {
+    const FrameRequest& R = requests[r];
+    const bool several = R.models.size() > 1;
+    at[r].resize(R.models.size());
+    for (size_t i = 0; i < R.models.size(); ++i) {
+      size_t q = 0;
+      while (q < queries.size() && !(queries[q].model == R.models[i] && queries[q].several == several &&
+                                     same_request(queries[q].o, R.options[i])))
+        ++q;
+      if (q == queries.size()) queries.push_back({R.models[i], R.options[i], several, {}});
+      std::vector<size_t>& rs = queries[q].requests;
+      if (rs.empty() || rs.back() != r) rs.push_back(r);
+      at[r][i] = {q, rs.size() - 1};
+    }
   }
-  while (multi_slots_.size() < n_queries) {
+  for (const Query& q : queries)
+    if (q.requests.size() > NCC_MAX_BATCH) raise(kLimitExceeded, "Images", "too many images for one batch");
+  while (multi_slots_.size() < queries.size()) {
     multi_slots_.push_back(std::make_unique<Slot>());
     multi_slots_.back()->stream = stream_;
     check(hipEventCreateWithFlags(&multi_slots_.back()->done, hipEventDisableTiming), "hipEventCreate");
@@ -1812,16 +1842,17 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
   // halves, the split-BF16 coarse searches of all queries in one persistent
   // launch (per kernel instantiation), then all back halves
   std::vector<Slot*> planned;
-  for (size_t i = 0; i < n_models; ++i) {
-    if (owner[i] != i) continue;
-    FindOptions o = opts[i];
+  for (size_t q = 0; q < queries.size(); ++q) {
+    FindOptions o = queries[q].o;
     o.query_cache = true;
-    o.whole_level0 = several;
+    o.whole_level0 = queries[q].several;
     o.profile = 0;
     o.trace_stages = false;
-    Slot& S = *multi_slots_[slot_of[i]];
+    std::vector<FrameInput> fr;
+    for (size_t r : queries[q].requests) fr.push_back(requests[r].frame);
+    Slot& S = *multi_slots_[q];
     cur_ = &S;
-    if (enqueue_front(S, *models[i], frames, n_frames, o, nullptr)) {
+    if (enqueue_front(S, *queries[q].model, fr.data(), fr.size(), o, nullptr)) {
       front_launches(S);
       planned.push_back(&S);
     }
@@ -1857,93 +1888,111 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
     enqueue_back(*S);
     check(hipEventRecord(S->done, stream_), "hipEventRecord");
   }
-  // ---- candidates per frame in (score, model, row, col, angle) order
-  std::vector<std::vector<ModelMatch>> cand(n_frames);
-  std::vector<std::vector<SelectionWalk::Ref>> where(n_frames);
-  for (size_t i = 0; i < n_models; ++i) {
-    const size_t k = slot_of[owner[i]];
-    Slot& S = *multi_slots_[k];
-    if (owner[i] == i) {
+  for (size_t q = 0; q < queries.size(); ++q)
+    if (!multi_slots_[q]->empty) collect(*multi_slots_[q]);
+  // ---- per request: the background guard of each of its queries' frames
+  // (matcher.cpp:839-846, for an AVX2 host rounding to nearest and byte
+  // images; the whole level 0 is converted when the query kept candidates,
+  // which it did when there are 32 of them), and its candidates in
+  // (score, model, row, col, angle) order
+  std::vector<std::vector<ModelMatch>> cand(n_requests);
+  std::vector<std::vector<SelectionWalk::Ref>> where(n_requests);
+  for (size_t r = 0; r < n_requests; ++r) {
+    const FrameRequest& R = requests[r];
+    const bool several = R.models.size() > 1;
+    bool distinct = false;
+    for (const NccModel* m : R.models) distinct |= m != R.models[0];
+    std::vector<bool> seen(R.models.size(), false);
+    for (size_t i = 0; i < R.models.size(); ++i) {
+      const auto [q, j] = at[r][i];
+      Slot& S = *multi_slots_[q];
       if (S.empty) continue;
-      collect(S);
-      // default background guard (matcher.cpp:839-846), for an AVX2 host
-      // rounding to nearest and byte images; the whole level 0 is converted
-      // when the query kept candidates, which it did when there are 32 of them
-      const NccModel& m = *models[i];
-      const FindOptions& o = S.o;  // the angle range clipped to the model's
-      for (size_t f = 0; f < n_frames; ++f) {
-        Job& J = S.jobs[f];
-        if (!S.finest && distinct && sel.num_matches == 0 && J.full.size() >= 32 && o.subpixel && !o.num_matches &&
-            o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
+      bool owner = true;  // the first model position of this request on query q
+      for (size_t k = 0; k < i; ++k) owner &= at[r][k].first != q;
+      if (owner) {
+        const NccModel& m = *R.models[i];
+        const FindOptions& o = S.o;  // the angle range clipped to the model's
+        Job& J = S.jobs[j];
+        if (!S.finest && distinct && R.selection.num_matches == 0 && J.full.size() >= 32 && o.subpixel &&
+            !o.num_matches && o.min_score >= .75 && o.min_score <= .85 && S.top + 1 == 3 && !m.params.absolute &&
             m.params.step <= .02 && o.extent <= .4 && m.pool.rasters[m.original].count <= 32768 && several &&
-            frames[f].pixel_bytes == 1 && !frames[f].domain) {
+            R.frame.pixel_bytes == 1 && !R.frame.domain) {
           J.guard_first = o.min_score >= .8 ? .1 : .2;
           J.guard = .1;
         }
       }
-    }
-    if (S.empty) continue;
-    for (size_t f = 0; f < n_frames; ++f) {
-      const std::vector<Pose>& full = S.jobs[f].full;
-      for (size_t j = 0; j < full.size(); ++j)
-        if (std::isfinite(full[j].score)) { cand[f].push_back({full[j], i}); where[f].push_back({k, j}); }
+      const std::vector<Pose>& full = S.jobs[j].full;
+      for (size_t k = 0; k < full.size(); ++k)
+        if (std::isfinite(full[k].score)) { cand[r].push_back({full[k], i}); where[r].push_back({q, j, k}); }
     }
   }
   // refined pose and verdict per (query, frame, candidate): aliases share them
-  std::vector<std::vector<std::vector<Pose>>> refined(n_queries, std::vector<std::vector<Pose>>(n_frames));
-  std::vector<std::vector<std::vector<int>>> verdict(n_queries, std::vector<std::vector<int>>(n_frames));
-  for (size_t i = 0; i < n_models; ++i)
-    if (owner[i] == i && !multi_slots_[slot_of[i]]->empty)
-      for (size_t f = 0; f < n_frames; ++f) {
-        refined[slot_of[i]][f].assign(multi_slots_[slot_of[i]]->jobs[f].full.size(), Pose{});
-        verdict[slot_of[i]][f].assign(multi_slots_[slot_of[i]]->jobs[f].full.size(), -1);
-      }
-  bool any_quota = false;
-  for (size_t i = 0; i < n_models; ++i) any_quota |= opts[i].num_matches > 0;
-  // HALCON's two-stage quota: per-model quotas and overlaps first, then the
-  // cross-model overlap without refilling quotas (matcher.cpp:1108-1112)
-  const bool independent = per_model_quota_before_overlap && sel.num_matches < 0 && sel.max_overlap >= 0 && any_quota;
-  std::vector<SelectionWalk> walks(n_frames);
-  for (size_t f = 0; f < n_frames; ++f) {
-    SelectionWalk& w = walks[f];
-    w.models = models; w.opts = opts; w.n_models = n_models; w.pool = &cand[f]; w.where = &where[f];
-    w.s = independent ? Selection{-1, -1} : sel;
+  std::vector<std::vector<std::vector<Pose>>> refined(queries.size());
+  std::vector<std::vector<std::vector<int>>> verdict(queries.size());
+  for (size_t q = 0; q < queries.size(); ++q) {
+    const Slot& S = *multi_slots_[q];
+    refined[q].resize(queries[q].requests.size());
+    verdict[q].resize(queries[q].requests.size());
+    if (S.empty) continue;
+    for (size_t j = 0; j < queries[q].requests.size(); ++j) {
+      refined[q][j].assign(S.jobs[j].full.size(), Pose{});
+      verdict[q][j].assign(S.jobs[j].full.size(), -1);
+    }
+  }
+  std::vector<SelectionWalk> walks(n_requests);
+  std::vector<bool> independent(n_requests);
+  for (size_t r = 0; r < n_requests; ++r) {
+    const FrameRequest& R = requests[r];
+    bool any_quota = false;
+    for (const FindOptions& o : R.options) any_quota |= o.num_matches > 0;
+    // HALCON's two-stage quota: per-model quotas and overlaps first, then the
+    // cross-model overlap without refilling quotas (matcher.cpp:1108-1112)
+    independent[r] = R.per_model_quota_before_overlap && R.selection.num_matches < 0 && R.selection.max_overlap >= 0 &&
+                     any_quota;
+    SelectionWalk& w = walks[r];
+    w.models = R.models.data(); w.opts = R.options.data(); w.n_models = R.models.size();
+    w.pool = &cand[r]; w.where = &where[r];
+    w.s = independent[r] ? Selection{-1, -1} : R.selection;
     w.with_refine = true;
-    w.ord.resize(cand[f].size());
+    w.ord.resize(cand[r].size());
     for (size_t i = 0; i < w.ord.size(); ++i) w.ord[i] = i;
-    std::sort(w.ord.begin(), w.ord.end(), [&](size_t a, size_t b) { return multi_less(cand[f][a], cand[f][b]); });
+    std::sort(w.ord.begin(), w.ord.end(), [&](size_t a, size_t b) { return multi_less(cand[r][a], cand[r][b]); });
     w.init();
   }
-  // ---- the walks of all frames, with shared speculative refinement rounds
+  // ---- the walks of all requests, with shared speculative refinement rounds
+  auto v = [&](const SelectionWalk::Ref& x) { return verdict[x.slot][x.frame][x.index]; };
+  auto pose = [&](const SelectionWalk::Ref& x) { return refined[x.slot][x.frame][x.index]; };
   for (;;) {
     std::vector<RefineRequest> reqs;
-    std::vector<std::pair<size_t, SelectionWalk::Ref>> targets;  // (frame, candidate)
-    for (size_t f = 0; f < n_frames; ++f) {
-      SelectionWalk& w = walks[f];
-      auto v = [&](const SelectionWalk::Ref& r) { return verdict[r.slot][f][r.index]; };
-      w.advance(v, [&](const SelectionWalk::Ref& r) { return refined[r.slot][f][r.index]; });
+    std::vector<SelectionWalk::Ref> targets;
+    for (size_t r = 0; r < n_requests; ++r) {
+      SelectionWalk& w = walks[r];
+      w.advance(v, pose);
       if (w.done()) continue;
       std::vector<SelectionWalk::Ref> t;
-      w.requests(t, v, [&](const SelectionWalk::Ref& r) { verdict[r.slot][f][r.index] = -2; });
-      for (const auto& r : t) {
-        reqs.push_back({multi_slots_[r.slot].get(), uint32_t(f), multi_slots_[r.slot]->jobs[f].full[r.index], {}, 0});
-        targets.push_back({f, r});
+      w.requests(t, v, [&](const SelectionWalk::Ref& x) { verdict[x.slot][x.frame][x.index] = -2; });
+      for (const auto& x : t) {
+        reqs.push_back({multi_slots_[x.slot].get(), uint32_t(x.frame), multi_slots_[x.slot]->jobs[x.frame].full[x.index],
+                        {}, 0});
+        targets.push_back(x);
       }
     }
     if (reqs.empty()) break;
     refine_round(reqs);
     for (size_t k = 0; k < reqs.size(); ++k) {
-      const auto& [f, r] = targets[k];
-      refined[r.slot][f][r.index] = reqs[k].out;
-      verdict[r.slot][f][r.index] = reqs[k].verdict;
+      const SelectionWalk::Ref& x = targets[k];
+      refined[x.slot][x.frame][x.index] = reqs[k].out;
+      verdict[x.slot][x.frame][x.index] = reqs[k].verdict;
     }
   }
-  for (size_t f = 0; f < n_frames; ++f) {
-    std::vector<ModelMatch> result = walks[f].finish();
-    if (independent) {
+  for (size_t r = 0; r < n_requests; ++r) {
+    const FrameRequest& R = requests[r];
+    std::vector<ModelMatch> result = walks[r].finish();
+    if (independent[r]) {
       SelectionWalk w2;
-      w2.models = models; w2.opts = opts; w2.n_models = n_models; w2.pool = &result; w2.where = nullptr;
-      w2.s = sel;
+      w2.models = R.models.data(); w2.opts = R.options.data(); w2.n_models = R.models.size();
+      w2.pool = &result; w2.where = nullptr;
+      w2.s = R.selection;
       w2.with_refine = false;
       w2.ord.resize(result.size());
       for (size_t i = 0; i < w2.ord.size(); ++i) w2.ord[i] = i;
@@ -1952,11 +2001,11 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
       w2.advance(never, [](const SelectionWalk::Ref&) { return Pose{}; });
       result = w2.finish();
     }
-    out[f] = std::move(result);
+    out[r] = std::move(result);
   }
   if (stats) {
-    stats->queries = int(n_queries);
-    for (size_t q = 0; q < n_queries; ++q) {
+    stats->queries = int(queries.size());
+    for (size_t q = 0; q < queries.size(); ++q) {
       const Slot& S = *multi_slots_[q];
       if (S.empty) continue;
       for (const Job& J : S.jobs) { stats->coarse_flops += J.coarse_flops; stats->candidates += int(J.full.size()); }

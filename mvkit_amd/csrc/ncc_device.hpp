@@ -96,7 +96,7 @@ struct Selection { int64_t num_matches = -1; double max_overlap = -1; };
 // A find_models result: the pose and the index of its model in the call.
 struct ModelMatch { Pose pose; size_t model = 0; };
 
-// Totals of a find_models_frames call.
+// Totals of a find_models_frames / find_requests call.
 struct BatchStats {
   double coarse_queue_us = 0;   // device time of the persistent coarse launches (HIP events on their stream)
   double coarse_flops = 0;      // algorithmic FLOPs of every query's coarse search
@@ -135,6 +135,16 @@ struct FrameInput {
   size_t width, height, pitch;
   int pixel_bytes = 1;  // 1: u8 ("byte"), 2: u16 ("uint2")
   const Region* domain = nullptr;
+};
+
+// One request of find_requests: a frame and the models to find in it, with
+// find_models' per-model options and selection.
+struct FrameRequest {
+  FrameInput frame{};
+  std::vector<const NccModel*> models;
+  std::vector<FindOptions> options;  // one per model
+  Selection selection;
+  bool per_model_quota_before_overlap = true;
 };
 
 // One find pipeline: owns a stream and grow-only device/pinned workspaces.
@@ -189,6 +199,13 @@ class FindContext {
                                                           const Selection& selection,
                                                           bool per_model_quota_before_overlap = true,
                                                           bool coarse_queue = true, BatchStats* stats = nullptr);
+  // find_models for several requests at once, each with its own frame,
+  // models, options and selection; each request's result is the one
+  // find_models returns for it alone.  One query per distinct (model,
+  // options, several models in the request) covers the frames of the
+  // requests that ask for it.
+  std::vector<std::vector<ModelMatch>> find_requests(const FrameRequest* requests, size_t n_requests,
+                                                     bool coarse_queue = true, BatchStats* stats = nullptr);
   // Same, from host memory.
   std::vector<ModelMatch> find_models_host(const NccModel* const* models, const FindOptions* options,
                                            size_t n_models, const uint8_t* image, size_t width, size_t height,
