@@ -182,6 +182,98 @@ def batched_models(capi, torch, args):
     return out
 
 
+def serving(capi, torch, args):
+    """The serving loop under open-loop Poisson arrivals: each request is one
+    frame (1280x1024, 2448x2048 or 5472x3648) with 1-4 models drawn from a
+    64-model pool (templates 48^2-320^2, +-15 deg to the full turn), submitted
+    to the request loop at its arrival time; latency is from arrival to the
+    end of the request's device and host work.  Split-BF16 with micro-batches
+    and two workers, against plain FP32 per-frame launches (one worker, one
+    request at a time, no persistent coarse launch)."""
+    import threading
+    from mvkit_amd import synth
+    log("serving: generating the 64-model pool and frames")
+    w = synth.serving_workload()
+    t = time.perf_counter()
+    models = [capi.NccModel(m.template, m.levels, m.start, m.extent, m.step if m.step else "auto") for m in w.models]
+    create_s = time.perf_counter() - t
+    rng = np.random.default_rng(0x6D766B6974 + 5)
+    n_max = int(max(args.serve_rates) * args.serve_seconds) + 1
+    trace = [(int(rng.integers(len(w.images))), list(rng.integers(len(models), size=int(rng.integers(1, 5)))))
+             for _ in range(n_max)]
+
+    def opts_for(subset, form):
+        return [capi.NccModel.options(w.models[i].start, w.models[i].extent, 0.8, 0, 0.5, form=form,
+                                      max_candidates=300000, vector_tracking=form == capi.FORM_FP32) for i in subset]
+
+    def run(rate, form, **server_kw):
+        n = max(1, int(rate * args.serve_seconds))
+        gaps = rng.exponential(1.0 / rate, size=n)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        prepared = [(w.images[f], [models[i] for i in sub], opts_for(sub, form)) for f, sub in trace[:n]]
+        server = capi.Server(**server_kw)
+        subs = [None] * n
+        ready = threading.Semaphore(0)
+
+        def submitter():
+            t0 = time.perf_counter()
+            for k in range(n):
+                delay = t0 + arrivals[k] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                img, ms, os_ = prepared[k]
+                rid = server.submit(img, ms, os_, 0, 0.5)
+                subs[k] = (rid, t0 + arrivals[k], time.perf_counter())
+                ready.release()
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, ends, batches, failed = [], [], [], 0
+        for k in range(n):
+            ready.acquire()
+            rid, arrival, submitted = subs[k]
+            try:
+                _, tm = server.wait(rid)
+            except capi.MvError:  # e.g. a request over the candidate budget (the reference raises the same)
+                failed += 1
+                continue
+            end = submitted + tm["finished_us"] * 1e-6
+            lat.append(end - arrival)
+            ends.append(end)
+            batches.append(tm["batch_size"])
+        th.join()
+        server.close()
+        first = subs[0][1]
+        lat = np.array(lat) * 1e3
+        return {"offered_req_s": rate, "requests": n, "failed": failed,
+                "achieved_req_s": round((n - failed) / (max(ends) - first), 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "max_ms": round(float(lat.max()), 2), "mean_batch": round(float(np.mean(batches)), 2)}
+    # warm: every frame and model once on a server of each kind
+    for form, kw in ((capi.FORM_BF16X3, {"workers": 2, "max_batch": 8}),
+                     (capi.FORM_FP32, {"workers": 1, "max_batch": 1, "coarse_queue": False})):
+        srv = capi.Server(**kw)
+        ids = [srv.submit(w.images[i % len(w.images)], [models[i]], opts_for([i], form), 0, 0.5)
+               for i in range(len(models))]
+        for i in ids:
+            try:
+                srv.wait(i)
+            except capi.MvError:
+                pass
+        srv.close()
+    out = {"workload": "serving_workload (seed 0x6D766B6974+5): 64 models, 2 frames of each size, 1-4 models per "
+                       "request, MinScore 0.8, candidate budget 300 000", "model_create_s": round(create_s, 2),
+           "seconds_per_rate": args.serve_seconds, "split_bf16_batched": [], "fp32_per_frame": []}
+    for rate in args.serve_rates:
+        r = run(rate, capi.FORM_BF16X3, workers=2, max_batch=8)
+        log(f"serving split-BF16, micro-batches: {r}")
+        out["split_bf16_batched"].append(r)
+    for rate in args.serve_rates[:2]:
+        r = run(rate, capi.FORM_FP32, workers=1, max_batch=1, coarse_queue=False)
+        log(f"serving FP32 per frame: {r}")
+        out["fp32_per_frame"].append(r)
+    return out
+
+
 def precision_sweep(capi, args):
     """Each correlation form's coarse score error against the FP64 oracle on
     the full-turn workload (2048^2, 256^2 template, 5 levels, AngleStep 0.035),
@@ -224,6 +316,10 @@ def main():
                     help="single-frame mv_find_ncc_model calls timed for p50/p99 (0: skip)")
     ap.add_argument("--batch-models", type=int, default=1,
                     help="time the batched heterogeneous workload (32 models x 16 images, 0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=1.5,
+                    help="seconds of Poisson arrivals per rate in the serving measurement (0: skip)")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 100.0, 200.0, 400.0],
+                    help="offered request rates (requests/s) of the serving measurement")
     ap.add_argument("--precision-sweep", type=int, default=1,
                     help="report each correlation form's coarse score error on the full-turn workload (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,6 +526,7 @@ def main():
 
     batched = batched_models(capi, torch, args) if args.batch_models else None
     sweep = precision_sweep(capi, args) if args.precision_sweep else None
+    serve = serving(capi, torch, args) if args.serve_seconds > 0 else None
     cpu = cpu_baseline(scenes[0], args) if args.cpu_finds > 0 else None
 
     line = {
@@ -459,6 +556,7 @@ def main():
         "single_frame_latency": latency,
         "batched_models": batched,
         "precision_sweep": sweep,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -383,6 +383,49 @@ def batch_workload(n_models: int = 32, n_images: int = 16, side: tuple = (32, 25
     return work
 
 
+SERVE_RANGES = (math.radians(15), math.radians(30), math.radians(60), math.radians(90), math.pi)
+SERVE_FRAMES = ((1280, 1024), (2448, 2048), (5472, 3648))
+
+
+def serving_workload(n_models: int = 64, side: tuple = (48, 320), frames_per_size: int = 2,
+                     instances: int = 4, noise: float = 6.0, seed_k: int = 5) -> BatchWorkload:
+    """The serving pool: n_models square templates with sides uniform in
+    `side` and half-ranges drawn from +-15, +-30, +-60, +-90 and +-180 deg
+    (the full turn; an explicit step where the automatic one would exceed
+    the raster budget, levels as batch_levels), and frames_per_size frames
+    of each of 1280x1024, 2448x2048 and 5472x3648, each holding `instances`
+    rotated pastes of pool models, N(0, noise)."""
+    rng = SplitMix64(SEED_BASE + seed_k)
+    models = []
+    for k in range(n_models):
+        t = side[0] + int(rng.uniform1() * (side[1] - side[0] + 1))
+        half = SERVE_RANGES[min(len(SERVE_RANGES) - 1, int(rng.uniform1() * len(SERVE_RANGES)))]
+        start, extent = (0.0, 2 * math.pi) if half == math.pi else (-half, 2 * half)
+        tmpl = make_template(SplitMix64((SEED_BASE + seed_k + 7907 * (k + 1)) ^ 0x5EED5EED), t, t)
+        models.append(BatchModel(tmpl, start, extent, budget_step(t, t, extent), batch_levels(t, t)))
+    work = BatchWorkload(models)
+    for (W, H) in SERVE_FRAMES:
+        for _ in range(frames_per_size):
+            base = blurred_noise(rng, H, W).astype(np.float64)
+            truth, tries = [], 0
+            while len(truth) < instances and tries < 1000:
+                tries += 1
+                k = min(n_models - 1, int(rng.uniform1() * n_models))
+                diag = math.hypot(*models[k].template.shape)
+                cy = diag / 2 + 2 + rng.uniform1() * (H - 1 - diag - 4)
+                cx = diag / 2 + 2 + rng.uniform1() * (W - 1 - diag - 4)
+                ang = models[k].start + rng.uniform1() * models[k].extent
+                if any(math.hypot(cy - r, cx - c) < 0.6 * (diag + math.hypot(*models[m].template.shape))
+                       for m, r, c, _ in truth):
+                    continue
+                truth.append((k, cy, cx, ang))
+            for k, cy, cx, ang in truth:
+                paste(base, models[k].template, cy, cx, ang)
+            work.images.append(_finish(base, rng, noise, False))
+            work.truth.append(truth)
+    return work
+
+
 # ------------------------------------------------------------- search domains
 def mask_runs(mask: np.ndarray, row0: int = 0, col0: int = 0) -> np.ndarray:
     """Canonical runs (n x 3: row, first column, last column) of a boolean
