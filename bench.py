@@ -245,7 +245,7 @@ def serving(capi, torch, args):
         first = subs[0][1]
         lat = np.array(lat) * 1e3
         return {"offered_req_s": rate, "requests": n, "failed": failed,
-                "achieved_req_s": round((n - failed) / (max(ends) - first), 1),
+                "achieved_req_s": round(float((n - failed) / (max(ends) - first)), 1),
                 "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
                 "max_ms": round(float(lat.max()), 2), "mean_batch": round(float(np.mean(batches)), 2)}
     # warm: every frame and model once on a server of each kind
@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 64 MiB device-to-device copies (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
+    ap.add_argument("--part", default="all", choices=["all", "batched", "serving"],
+                    help="batched / serving: run only that measurement (for profiling it) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the NCC find path shards by frame, and only the single-GPU "
@@ -345,6 +347,12 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"copied {n} bytes three times")
+        return
+    if args.part == "batched":
+        print(json.dumps({"batched_models": batched_models(capi, torch, args)}), flush=True)
+        return
+    if args.part == "serving":
+        print(json.dumps({"serving": serving(capi, torch, args)}), flush=True)
         return
     F = max(1, args.inflight)
     log(f"generating {F} frames")
@@ -521,7 +529,7 @@ def main():
         lat1 = np.array(lat1) * 1e3
         latency = {"entry": "mv_find_ncc_model (host u8 image object, C ABI)", "calls": args.latency_calls,
                    "p50_ms": round(float(np.percentile(lat1, 50)), 4), "p99_ms": round(float(np.percentile(lat1, 99)), 4),
-                   "mpx_s_at_p50": round(2048 * 2048 / (np.percentile(lat1, 50) * 1e-3) / 1e6, 1)}
+                   "mpx_s_at_p50": round(float(2048 * 2048 / (np.percentile(lat1, 50) * 1e-3) / 1e6), 1)}
         log(f"single-frame latency: {latency}")
 
     batched = batched_models(capi, torch, args) if args.batch_models else None
