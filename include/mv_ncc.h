@@ -133,6 +133,11 @@ MV_API MvStatus mv_get_domain(MvObject Image, MvObject* Domain);
 
 /* Handles: include/mvkit/c/mv_handle.h:40.  Invalidates every alias. */
 MV_API MvStatus mv_clear_handle(MvTuple Handle);
+/* include/mvkit/c/mv_handle.h:52 (src/c_api/handle_api.cpp:67-100): the
+ * semantic type of a control tuple as a one-string tuple: "integer", "real",
+ * "string", "ncc_model" or "serialized_item" (live handles of one kind),
+ * "handle" (cleared or mixed handles), "any" (empty or mixed). */
+MV_API MvStatus mv_tuple_sem_type(MvTuple T, MvTuple* SemType);
 /* Serialized items: include/mvkit/c/mv_handle.h:63-97
  * (src/c_api/serialization_api.cpp:51-141) for NCC model handles.
  * serialize_handle turns a model into an item holding its MVKIT_NCC 1 bytes;

@@ -59,6 +59,22 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_program(source: str, output: str) -> str:
+    """Compiles a C++ program that uses the library through include/mv_ncc.hpp
+    (host code only: g++, linked against libmvkit_amd.so with an rpath to
+    the package directory, so the program runs from where it was built)."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    rpath = os.path.relpath(HERE, os.path.dirname(os.path.abspath(output)))
+    tmp = output + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-pedantic", "-Werror", f"-I{include}", source,
+           "-o", tmp, f"-L{HERE}", "-lmvkit_amd", f"-Wl,-rpath,$ORIGIN/{rpath}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, output)
+    return output
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

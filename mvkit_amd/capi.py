@@ -31,7 +31,7 @@ PUBLIC_SYMBOLS = [
     "mv_set_d", "mv_set_s", "mv_set_h", "mv_gen_empty_obj", "mv_clear_obj", "mv_gen_image1", "mv_gen_image1_extern",
     "mv_gen_region_runs", "mv_get_region_runs", "mv_gen_rectangle1", "mv_reduce_domain", "mv_full_domain",
     "mv_get_domain",
-    "mv_clear_handle", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
+    "mv_clear_handle", "mv_tuple_sem_type", "mv_serialize_handle", "mv_deserialize_handle", "mv_create_serialized_item_ptr",
     "mv_get_serialized_item_ptr", "mv_write_ncc_model", "mv_read_ncc_model", "mv_create_ncc_model", "mv_get_ncc_model_params", "mv_find_ncc_model", "mv_find_ncc_models",
     "mvamd_create_ncc_model", "mvamd_find_ncc_model_device", "mvamd_find_ncc_model_image",
     "mvamd_find_ncc_models_device", "mvamd_find_ncc_models_batch", "mvamd_find_ncc_model_batch",
@@ -144,6 +144,7 @@ def lib() -> C.CDLL:
         L.mv_full_domain.argtypes = [P, C.POINTER(P)]
         L.mv_get_domain.argtypes = [P, C.POINTER(P)]
         L.mv_clear_handle.argtypes = [P]
+        L.mv_tuple_sem_type.argtypes = [P, C.POINTER(P)]
         L.mv_create_ncc_model.argtypes = [P, P, P, P, P, P, C.POINTER(P)]
         L.mvamd_create_ncc_model.argtypes = [P, P, P, P, P, P, C.c_int32, C.POINTER(P)]
         L.mv_get_ncc_model_params.argtypes = [P] + [C.POINTER(P)] * 5

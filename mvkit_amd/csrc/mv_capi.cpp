@@ -841,6 +841,41 @@ extern "C" MvStatus mv_clear_handle(MvTuple Handle) {
     }
   });
 }
+// The semantic type of a control tuple (handle_api.cpp:67-100): integer, real
+// or string when every element has that kind; a tuple of live handles of one
+// kind names that kind (ncc_model, serialized_item); handles that are cleared
+// or of mixed kinds are "handle"; an empty or mixed tuple is "any".
+extern "C" MvStatus mv_tuple_sem_type(MvTuple T, MvTuple* SemType) {
+  return translate([&] {
+    constexpr auto op = "tuple_sem_type";
+    Outputs<1> out(op, {SemType});
+    const Input view(T, op, "T");
+    const char* name = "any";
+    if (view.size()) {
+      const MvValueKind kind = view.at(0).kind;
+      bool same = true;
+      for (size_t i = 1; i < view.size() && same; ++i) same = view.at(i).kind == kind;
+      if (same && kind == MV_VALUE_INT64) name = "integer";
+      else if (same && kind == MV_VALUE_REAL64) name = "real";
+      else if (same && kind == MV_VALUE_STRING) name = "string";
+      else if (same && kind == MV_VALUE_HANDLE) {
+        int type = -1;  // 0: model, 1: serialized item, 2: cleared or mixed
+        for (size_t i = 0; i < view.size() && type != 2; ++i) {
+          const MvHandleImpl* h = view.at(i).h.get();
+          int t = 2;
+          if (h && h->object) {
+            std::lock_guard<std::mutex> lock(h->object->mu);
+            if (h->object->is_item ? bool(h->object->item) : bool(h->object->model)) t = h->object->is_item ? 1 : 0;
+          }
+          type = (type < 0 || type == t) ? t : 2;
+        }
+        name = type == 0 ? "ncc_model" : type == 1 ? "serialized_item" : "handle";
+      }
+    }
+    push(out[0], name);
+    out.commit();
+  });
+}
 
 // ============================================================ NCC operators
 namespace {
