@@ -244,7 +244,12 @@ int ncchip_create_rasters(int stage, const NcchipCreateBuffers* buffers, int n_j
  * candidate's neighbourhood (the union of its windows), staged in LDS when
  * it fits in 64 KiB (0: read the level from HBM); exact_f64 != 0 asserts
  * n * (max |S - D|)^2 < 2^53 for every query patch, which lets the moments
- * accumulate exactly in FP64.  `work` (optional, device) accumulates the
+ * accumulate exactly in FP64.  max_eligible bounds the eligible patches of
+ * any candidate (0: unknown); when it is at most 32, the neighbourhood fits
+ * in LDS and the level is u8 or u16, each sample is loaded once for all 25
+ * windows of its patch and the moments come from sliding integer sums along
+ * the support runs (same results); x_max bounds |S - D| at the level (below
+ * 2^15 the neighbourhood is staged as x = S - D in 16 bits).  `work` (optional, device) accumulates the
  * samples of every evaluated window (algorithmic work = 2 FLOP per sample).
  * `grid` workgroups loop over the candidates.
  * Replaces matcher.cpp:670-760 (kernels_avx2.cpp:118-238) and
@@ -253,7 +258,7 @@ int ncchip_track(const NccFrame* frames, int n_frames, int level, int level_kind
                  const int32_t* refs, int n_refs, const NccMatch* in, const int32_t* in_count, int cap, NccMatch* raw,
                  NccMatch* out, int32_t* out_count, int32_t* table, int table_size, double keep, double tol,
                  double step, double offset, int exact_lattice, int absolute, int tile_pixels, int exact_f64,
-                 unsigned long long* work, int grid, void* stream);
+                 int max_eligible, double x_max, unsigned long long* work, int grid, void* stream);
 
 /* ncchip_track on matrix cores, for u16 levels 1..4: per candidate, the 5x5
  * windows of all eligible patches as one split-BF16 GEMM

@@ -1272,21 +1272,24 @@ void FindContext::enqueue_back(Slot& S) {
     }
     const double xmax = (wide ? 65535. : 255.) * std::ldexp(1., 2 * int(l - 1));
     const int exact_f64 = n_max * xmax * xmax < 0x1p53;
-    // matrix cores where every query patch has a dense raster and at most 32
-    // patches fall in any candidate's angle window
-    bool mfma = track_on_mfma && l - 1 >= 1 && l - 1 <= 4 && jobs[0].lv[l - 1].kind == NCC_U16;
-    if (mfma) {
+    // the most query patches within any candidate's angle window: within tol
+    // of the candidate's angle, so within 2 tol of the smallest of them
+    // (widened by a relative 1e-9 against rounding of the eligibility test)
+    int max_eligible = 0;
+    {
       std::vector<double> ang;
-      for (const int32_t r : refs[l - 1]) {
-        mfma &= dense_geom(raster_of(r)).elems > 0;
-        ang.push_back(raster_of(r).angle);
-      }
+      for (const int32_t r : refs[l - 1]) ang.push_back(raster_of(r).angle);
       std::sort(ang.begin(), ang.end());
-      for (size_t i = 0, j = 0; mfma && i < ang.size(); ++i) {
-        while (j < ang.size() && ang[j] <= ang[i] + 2 * tol) ++j;
-        mfma &= j - i <= 32;
+      for (size_t i = 0, j = 0; i < ang.size(); ++i) {
+        while (j < ang.size() && ang[j] <= ang[i] + 2 * tol * (1 + 1e-9)) ++j;
+        max_eligible = std::max(max_eligible, int(j - i));
       }
     }
+    // matrix cores where every query patch has a dense raster and at most 32
+    // patches fall in any candidate's angle window
+    bool mfma = track_on_mfma && l - 1 >= 1 && l - 1 <= 4 && jobs[0].lv[l - 1].kind == NCC_U16 && max_eligible <= 32;
+    if (mfma)
+      for (const int32_t r : refs[l - 1]) mfma &= dense_geom(raster_of(r)).elems > 0;
     int rc_track = -6;
     if (mfma) {
       time_begin(kStageTrackMfma, qs);
@@ -1310,7 +1313,7 @@ void FindContext::enqueue_back(Slot& S) {
                      lists + (k + 1) * cap_all, counts + (k + 1) * n_frames,
                      reinterpret_cast<int32_t*>(dev + off_table) + k * n_frames * table_size, int(table_size),
                      o.min_score * .4, tol, step, offset, !wide && l - 1 <= 2, model.params.absolute,
-                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, work + k, kListGrid, qs))
+                     tile_px < 1e9 ? int(tile_px) : 0, exact_f64, max_eligible, xmax, work + k, kListGrid, qs))
       raise(kInternalError, "device", "tracking launch failed");
     if (rc_track != -6 && rc_track != 0) raise(kInternalError, "device", "tracking launch failed");
     if (rc_track == -6) time_end(qs);

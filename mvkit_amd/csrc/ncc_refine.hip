@@ -164,8 +164,10 @@ __device__ __forceinline__ double wave_transpose_sum16(double (&v)[16]) {
     const bool hi = (lane & m) != 0;
 #pragma unroll
     for (int j = 0; j < half; ++j) {
-      const double send = hi ? v[j] : v[j + half];
-      const double keep = hi ? v[j + half] : v[j];
+      // both values loaded, then selected (a select of addresses would keep v in memory)
+      const double lo_v = v[j], hi_v = v[j + half];
+      const double send = hi ? lo_v : hi_v;
+      const double keep = hi ? hi_v : lo_v;
       v[j] = keep + __shfl_xor(send, m, 64);
     }
   }
@@ -501,6 +503,361 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args
     keep_distinct(a, f, ci, res);
   }
   __syncthreads();  // the LDS of this candidate is reused by the next
+  }
+}
+
+// ------------------------------------------------- tracking, window-major
+// k_track with every sample loaded once for all 25 windows of its patch: a
+// wave takes a slice of one eligible patch's samples and accumulates the 25
+// dot products sum(x * w) (x = S - D from the LDS tile, FP64), which one
+// transposed butterfly reduces over the wave; the slices of a patch are added
+// in a fixed order.  The exact moments sum(S) and sum(S^2) of the 25 windows
+// come from sliding sums along the patch's support runs (lanes over runs,
+// integers), and a patch whose dot product uses weight runs slides along those
+// the same way.  Per sample this reads the pixel of each window once (as
+// k_track does) but the sample's position and weight once instead of five
+// times.  Windows outside the valid region read the nearest inside window's
+// pixels and are discarded.  The tracked pose is the first strict maximum in
+// the reference's (patch, dr, dc) order, as in k_track.
+constexpr int kWinPatches = 32;  // eligible patches per candidate (the host guarantees the bound)
+constexpr int kWinSlices = 4;    // sample slices per patch when at most 8 patches are eligible
+constexpr int kWinRunCache = 512;  // support runs of a candidate's eligible patches cached in LDS
+
+// Sums 32 values over the wave with one transposed butterfly (32 exchanges
+// instead of 192); lane L ends with the total of value (L >> 1) & 31.
+__device__ __forceinline__ double wave_transpose_sum32(double (&v)[32]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) v[j] = sum_swap32(v[j], v[j + 16]);  // lanes 32-63 keep j + 16
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = sum_swap16(v[j], v[j + 8]);    // odd 16-lane rows keep j + 8
+#pragma unroll
+  for (int half = 4, m = 8; half >= 1; half >>= 1, m >>= 1) {
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      // both values loaded, then selected (a select of addresses would keep v in memory)
+      const double lo_v = v[j], hi_v = v[j + half];
+      const double send = hi ? lo_v : hi_v;
+      const double keep = hi ? hi_v : lo_v;
+      v[j] = keep + __shfl_xor(send, m, 64);
+    }
+  }
+  return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// kX16: the tile holds x = S - D as int16 (every |S - D| of the level is
+// below 2^15), so the dot loop reads x directly; else it holds S.
+template <class T, bool kX16>
+__global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs args) {
+  using Tile = typename std::conditional<kX16, int16_t, T>::type;
+  extern __shared__ uint32_t tile_words[];
+  Tile* tile = reinterpret_cast<Tile*>(tile_words);
+  __shared__ int elig[kMaxEligible];
+  __shared__ int wave_cnt[kTrackThreads / 64];
+  __shared__ int box[4];
+  __shared__ double red_s[kTrackThreads / 64];
+  __shared__ int red_i[kTrackThreads / 64];
+  __shared__ int off[NCC_MAX_BATCH + 1];
+  __shared__ NccPatch s_pat[kWinPatches];
+  __shared__ int32_t s_ref[kWinPatches];
+  __shared__ int s_row[kWinPatches], s_col[kWinPatches];
+  __shared__ int s_lim[kWinPatches][4];          // inside windows: dr_lo, dr_hi, dc_lo, dc_hi (dr_lo > dr_hi: none)
+  __shared__ double s_dot[kWinPatches][25];      // per (patch, slice) task
+  __shared__ long long s_m1[kWinPatches][25], s_m2[kWinPatches][25];
+  __shared__ int2 s_runs[kWinRunCache];          // support runs of the eligible patches: (row, begin | end << 16)
+  __shared__ int s_run0[kWinPatches + 1];        // first run of each patch in s_runs
+  list_offsets(args.in_count, args.n_frames, args.cap, off);
+  const int total = off[args.n_frames];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    TrackArgs a = args;
+    const int f = __builtin_amdgcn_readfirstlane(list_frame(off, a.n_frames, g));
+    const int ci = __builtin_amdgcn_readfirstlane(g - off[f]);
+    bind_frame(a, f);
+    const NccMatch cd = a.in[size_t(f) * a.cap + ci];
+    const int n_e = min(eligible_patches(a.pools, a.refs, a.n_refs, cd.angle, a.tol, elig, wave_cnt), kWinPatches);
+    const int lo_r = max(0, a.sr), lo_c = max(0, a.sc);
+    const int hi_r = min(a.lv.height, a.sr + a.sh) - 1, hi_c = min(a.lv.width, a.sc + a.sw) - 1;
+    if (threadIdx.x == 0) { box[0] = INT_MAX; box[1] = INT_MIN; box[2] = INT_MAX; box[3] = INT_MIN; }
+    __syncthreads();
+    unsigned long long samples = 0;
+    for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
+      const int32_t ref = a.refs[elig[e]];
+      const NccPatch P = a.pools.patch(ref);
+      samples += 25ull * uint32_t(P.n);
+      int row, col;
+      anchor(a, cd, P, row, col);
+      s_pat[e] = P; s_ref[e] = ref; s_row[e] = row; s_col[e] = col;
+      // the windows (row + dr, col + dc) inside the valid region
+      const int dr_lo = max(-2, lo_r - row - P.rmin), dr_hi = min(2, hi_r - row - P.rmax);
+      const int dc_lo = max(-2, lo_c - col - P.cmin), dc_hi = min(2, hi_c - col - P.cmax);
+      const bool any = dr_lo <= dr_hi && dc_lo <= dc_hi;
+      s_lim[e][0] = any ? dr_lo : 1; s_lim[e][1] = any ? dr_hi : 0; s_lim[e][2] = dc_lo; s_lim[e][3] = dc_hi;
+      if (any) {
+        atomicMin(&box[0], row + dr_lo + P.rmin);
+        atomicMax(&box[1], row + dr_hi + P.rmax);
+        atomicMin(&box[2], col + dc_lo + P.cmin);
+        atomicMax(&box[3], col + dc_hi + P.cmax);
+      }
+    }
+    if (a.work) {
+      samples = uint64_t(wave_sum_i64(int64_t(samples)));
+      if (lane == 0 && samples) atomicAdd(a.work, samples);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int e = 0; e < n_e; ++e) { s_run0[e] = t; t += s_pat[e].n_runs; }
+      s_run0[n_e] = t;
+    }
+    // no eligible patch or no inside window leaves the box empty (INT_MAX,
+    // INT_MIN): compare the bounds before forming sizes.  th * tw <= tile_cap
+    // holds by the host's bound on the neighbourhood; it is checked anyway.
+    const bool any_window = box[0] <= box[1] && box[2] <= box[3] &&
+                            int64_t(box[1] - box[0] + 1) * (box[3] - box[2] + 1) <= a.tile_cap;  // block-uniform
+    const int r0 = any_window ? box[0] : 0, c0 = any_window ? box[2] : 0;
+    const int th = any_window ? box[1] - box[0] + 1 : 0, tw = any_window ? box[3] - box[2] + 1 : 0;
+    const T* lvp = static_cast<const T*>(a.lv.data);
+    const int32_t Di = int32_t(a.D);
+    __syncthreads();  // s_run0
+    const int n_runs_all = s_run0[n_e];
+    const bool runs_cached = n_runs_all <= kWinRunCache;
+    if (any_window) {
+      // the runs (at most two per thread) and the tile in one round of loads
+      int4 myrun[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int j = threadIdx.x + kTrackThreads * u;
+        if (runs_cached && j < n_runs_all) {
+          int e = 0;
+          while (s_run0[e + 1] <= j) ++e;
+          myrun[u] = reinterpret_cast<const int4*>(a.pools.of(s_ref[e]).runs)[s_pat[e].run_off + j - s_run0[e]];
+        }
+      }
+      constexpr int U = 16;
+      for (int base = threadIdx.x; base < th * tw; base += kTrackThreads * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = base + kTrackThreads * u;
+          v[u] = i < th * tw ? lvp[size_t(r0 + i / tw) * a.lv.pitch + c0 + i % tw] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (base + kTrackThreads * u < th * tw) {
+            if constexpr (kX16) tile[base + kTrackThreads * u] = int16_t(int32_t(v[u]) - Di);
+            else tile[base + kTrackThreads * u] = v[u];
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int j = threadIdx.x + kTrackThreads * u;
+        if (runs_cached && j < n_runs_all)
+          s_runs[j] = make_int2(myrun[u].x, (myrun[u].y & 0xffff) | (myrun[u].z << 16));
+      }
+    }
+    __syncthreads();
+    const int Q = n_e <= 8 ? kWinSlices : 1;
+    // ---- dot products: task (patch e, slice q)
+    for (int task = wave; any_window && task < n_e * Q; task += kTrackThreads / 64) {
+      const int e = task / Q, q = task - e * Q;
+      const NccPatch& P = s_pat[e];
+      if (P.use_wruns) continue;  // wave-uniform: its dot product comes from the weight runs below
+      const int dr_lo = s_lim[e][0], dr_hi = s_lim[e][1], dc_lo = s_lim[e][2], dc_hi = s_lim[e][3];
+      double dot[32];
+#pragma unroll
+      for (int k = 0; k < 32; ++k) dot[k] = 0;
+      if (dr_lo <= dr_hi) {
+        const NccPatchPool& pool = a.pools.of(s_ref[e]);
+        const int32_t* rc = pool.rc + P.sample_off;
+        const double* w = pool.w + P.sample_off;
+        // tile offsets of window row dr relative to the sample's pixel in
+        // window (0, 0); columns dc - 2 = 0..4 as immediate offsets when no
+        // column is clamped, else through coff
+        int roff[5], coff[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          roff[k] = min(max(k - 2, dr_lo), dr_hi) * tw;
+          coff[k] = min(max(k - 2, dc_lo), dc_hi) + 2;
+        }
+        const bool full_cols = dc_lo == -2 && dc_hi == 2;  // wave-uniform
+        const int base0 = (s_row[e] - r0) * tw + (s_col[e] - c0) - 2;
+        // slice [j0, j1) (j0 <= P.n - 1); lane takes j0 + lane + 64 t.  Samples
+        // are loaded kRing steps ahead of their use; past the slice a lane
+        // re-reads sample j0 with weight 0.
+        const int j0 = (P.n * q) / Q, j1 = (P.n * (q + 1)) / Q;
+        const int n_it = (j1 - j0 + 63) / 64;  // wave-uniform
+        constexpr int kRing = 4;
+        int32_t ring_rc[kRing];
+        double ring_w[kRing];
+        auto fetch = [&](int t, int32_t& r, double& wt) __attribute__((always_inline)) {
+          const int j = j0 + lane + 64 * t;
+          const bool ok = j < j1;
+          r = rc[ok ? j : j0];
+          const double v = w[ok ? j : j0];
+          wt = ok ? v : 0.0;
+        };
+        auto accumulate = [&](const int32_t (&pix)[25], double wt) __attribute__((always_inline)) {
+#pragma unroll
+          for (int k = 0; k < 25; ++k) dot[k] = fma(double(kX16 ? pix[k] : pix[k] - Di), wt, dot[k]);
+        };
+#pragma unroll
+        for (int u = 0; u < kRing; ++u) fetch(u, ring_rc[u], ring_w[u]);
+        for (int t0 = 0; t0 < n_it; t0 += kRing) {
+#pragma unroll
+          for (int u = 0; u < kRing; ++u) {
+            if (t0 + u >= n_it) break;
+            const int32_t r = ring_rc[u];
+            const double wt = ring_w[u];
+            fetch(t0 + u + kRing, ring_rc[u], ring_w[u]);
+            const int p0 = base0 + sample_row(r) * tw + sample_col(r);
+            // the 25 pixels first (all reads in flight), then the products
+            int32_t pix[25];
+            if (full_cols) {
+#pragma unroll
+              for (int dr = 0; dr < 5; ++dr) {
+                const Tile* rowp = tile + (p0 + roff[dr]);
+#pragma unroll
+                for (int dc = 0; dc < 5; ++dc) pix[5 * dr + dc] = int32_t(rowp[dc]);
+              }
+            } else {
+#pragma unroll
+              for (int dr = 0; dr < 5; ++dr)
+#pragma unroll
+                for (int dc = 0; dc < 5; ++dc) pix[5 * dr + dc] = int32_t(tile[p0 + roff[dr] + coff[dc]]);
+            }
+            accumulate(pix, wt);
+          }
+        }
+      }
+      const double tot = wave_transpose_sum32(dot);
+      const int k = (lane >> 1) & 31;
+      if (!(lane & 1) && k < 25) s_dot[task][k] = tot;
+    }
+    // ---- exact moments (and weight-run dot products): task (patch e, window
+    // row dr).  Sums of the tile value v (x or S) and v^2 over each support
+    // run for the first inside column, then slid one column at a time; the 15
+    // totals (5 sums, 5 sums of squares, 5 weight-run dots) are reduced in one
+    // transposed butterfly (the integer sums stay below 2^53, exact in FP64).
+    for (int task = wave; any_window && task < n_e * 5; task += kTrackThreads / 64) {
+      const int e = task / 5, dri = task - e * 5;
+      const NccPatch& P = s_pat[e];
+      const int dr_lo = s_lim[e][0], dr_hi = s_lim[e][1], dc_lo = s_lim[e][2], dc_hi = s_lim[e][3];
+      const int dr = min(max(dri - 2, dr_lo), dr_hi);
+      double v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = 0;
+      if (dr_lo <= dr_hi) {
+        const NccPatchPool& pool = a.pools.of(s_ref[e]);
+        const int base0 = (s_row[e] + dr - r0) * tw + (s_col[e] - c0);
+        const int4* runs = reinterpret_cast<const int4*>(pool.runs) + P.run_off;
+        int64_t m1[5] = {0, 0, 0, 0, 0}, m2[5] = {0, 0, 0, 0, 0};
+        for (int j = lane; j < P.n_runs; j += 64) {
+          int4 run;
+          if (runs_cached) {
+            const int2 c = s_runs[s_run0[e] + j];
+            run = make_int4(c.x, int(int16_t(c.y & 0xffff)), c.y >> 16, 0);
+          } else {
+            run = runs[j];
+          }
+          const Tile* src = tile + base0 + run.x * tw;
+          int64_t s1 = 0, s2 = 0;
+          int c = run.y + dc_lo;
+          const int ce = run.z + dc_lo;
+          for (; c + 7 <= ce; c += 8) {  // eight reads in flight
+            int64_t t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t[u] = int64_t(src[c + u]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              s1 += t[u];
+              s2 += t[u] * t[u];
+            }
+          }
+          for (; c <= ce; ++c) {
+            const int64_t t = int64_t(src[c]);
+            s1 += t;
+            s2 += t * t;
+          }
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            const int dc = k - 2;
+            if (dc > dc_lo && dc <= dc_hi) {  // slide from dc - 1 to dc
+              const int64_t in = int64_t(src[run.z + dc]), out = int64_t(src[run.y + dc - 1]);
+              s1 += in - out;
+              s2 += in * in - out * out;
+            }
+            m1[k] += s1;
+            m2[k] += s2;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { v[k] = double(m1[k]); v[5 + k] = double(m2[k]); }
+        if (P.use_wruns) {
+          const int4* wr = reinterpret_cast<const int4*>(pool.wruns) + P.wrun_off;
+          const double* ww = pool.wrun_w + P.wrun_off;
+          for (int j = lane; j < P.n_wruns; j += 64) {
+            const int4 run = wr[j];
+            const Tile* src = tile + base0 + run.x * tw;
+            int64_t rs = 0;  // sum of x over the run
+            for (int c = run.y + dc_lo; c <= run.z + dc_lo; ++c) rs += int64_t(src[c]) - (kX16 ? 0 : a.D);
+            const double wj = ww[j];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+              const int dc = k - 2;
+              if (dc > dc_lo && dc <= dc_hi) rs += int64_t(src[run.z + dc]) - int64_t(src[run.y + dc - 1]);
+              v[10 + k] += double(rs) * wj;
+            }
+          }
+        }
+      }
+      const double tot = wave_transpose_sum16(v);
+      const int k = (lane >> 2) & 15;
+      if ((lane & 3) == 0) {
+        if (k < 5) s_m1[e][5 * dri + k] = (long long)tot;
+        else if (k < 10) s_m2[e][5 * dri + k - 5] = (long long)tot;
+        else if (k < 15 && P.use_wruns) s_dot[e * Q][5 * dri + k - 10] = tot;  // slice 0 only (the dot pass skips such patches)
+      }
+    }
+    __syncthreads();
+    // ---- scores, and the first strict maximum in (patch, dr, dc) order
+    double best = NCC_MISSING;
+    int best_i = 0x7fffffff;
+    const double unit = ldexp(1.0, -a.lv.shift);
+    for (int k = threadIdx.x; any_window && k < n_e * 25; k += kTrackThreads) {
+      const int e = k / 25, wi = k - e * 25;
+      const int dr = wi / 5 - 2, dc = wi % 5 - 2;
+      if (dr < s_lim[e][0] || dr > s_lim[e][1] || dc < s_lim[e][2] || dc > s_lim[e][3]) continue;
+      const NccPatch& P = s_pat[e];
+      const int row = s_row[e] + dr, col = s_col[e] + dc;
+      if (!window_ok(a, P, row, col)) continue;
+      double d = s_dot[e * Q][wi];
+      if (!P.use_wruns)
+        for (int q = 1; q < Q; ++q) d += s_dot[e * Q + q][wi];
+      const int64_t n = P.n, m1 = s_m1[e][wi], m2 = s_m2[e][wi];
+      // centred moments t1 = sum(S - D), t2 = sum((S - D)^2)
+      const int64_t t1 = kX16 ? m1 : m1 - n * a.D;
+      const int64_t t2 = kX16 ? m2 : m2 - 2 * a.D * m1 + n * a.D * a.D;
+      const double sc = window_score(a, P, d * unit, t1, t2);
+      if (sc > best) { best = sc; best_i = k; }
+    }
+    block_argmax(best, best_i, red_s, red_i);
+    if (threadIdx.x == 0) {
+      NccMatch res = cd;
+      res.score = NCC_MISSING;
+      if (best > NCC_MISSING) {
+        const int e = best_i / 25, wi = best_i % 25;
+        const NccPatch& P = s_pat[e];
+        res.row = a.crop_r + a.step * (s_row[e] + wi / 5 - 2 + P.orow) + a.offset;
+        res.col = a.crop_c + a.step * (s_col[e] + wi % 5 - 2 + P.ocol) + a.offset;
+        res.angle = P.angle;
+        res.score = best;
+      }
+      keep_distinct(a, f, ci, res);
+    }
+    __syncthreads();  // the LDS of this candidate is reused by the next
   }
 }
 
@@ -1636,7 +1993,8 @@ extern "C" int ncchip_track(const NccFrame* frames, int n_frames, int level, int
                             const int32_t* refs, int n_refs, const NccMatch* in, const int32_t* in_count, int cap,
                             NccMatch* raw, NccMatch* out, int32_t* out_count, int32_t* table, int table_size,
                             double keep, double tol, double step, double offset, int exact_lattice, int absolute,
-                            int tile_pixels, int exact_f64, unsigned long long* work, int grid, void* stream) {
+                            int tile_pixels, int exact_f64, int max_eligible, double x_max,
+                            unsigned long long* work, int grid, void* stream) {
   if (n_frames <= 0 || grid <= 0) return 0;
   if (n_frames > NCC_MAX_BATCH || table_size < 2 || (table_size & (table_size - 1))) return -1;
   TrackArgs a{};
@@ -1650,6 +2008,21 @@ extern "C" int ncchip_track(const NccFrame* frames, int n_frames, int level, int
   const size_t lds = (size_t(a.tile_cap) * px + 15) & ~size_t(15);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 g(grid), b(kTrackThreads);
+  if (level_kind != NCC_U32 && a.tile_cap > 0 && size_t(a.tile_cap) * 2 <= kMaxTileBytes && max_eligible > 0 &&
+      max_eligible <= kWinPatches) {
+    // every candidate's neighbourhood in LDS and at most 32 eligible patches:
+    // the window-major kernel (moments exact in integers, whatever exact_f64)
+    const bool x16 = x_max < 32768;  // x = S - D as int16 (the tile has the pixel size of 16-bit levels)
+    const size_t lds16 = (size_t(a.tile_cap) * 2 + 15) & ~size_t(15);
+    if (level_kind == NCC_U8) {
+      if (x16) hipLaunchKernelGGL((k_track_win<uint8_t, true>), g, b, lds16, st, a);
+      else hipLaunchKernelGGL((k_track_win<uint8_t, false>), g, b, lds, st, a);
+    } else {
+      if (x16) hipLaunchKernelGGL((k_track_win<uint16_t, true>), g, b, lds16, st, a);
+      else hipLaunchKernelGGL((k_track_win<uint16_t, false>), g, b, lds, st, a);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1000;
+  }
   if (level_kind == NCC_U8) {
     if (exact_f64) hipLaunchKernelGGL((k_track<uint8_t, double>), g, b, lds, st, a);
     else hipLaunchKernelGGL((k_track<uint8_t, int64_t>), g, b, lds, st, a);
