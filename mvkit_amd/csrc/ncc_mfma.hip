@@ -46,6 +46,12 @@ constexpr int kWaveRows = 4;               // output rows per wave
 constexpr int kMRows = kWaveRows;          // output rows per block
 constexpr int kWaveCols = 32;              // anchors per wave (MFMA M)
 constexpr int kBlockCols = 4 * kWaveCols;
+constexpr int kRunCache = 512;             // support runs of a tile's 32 patches kept in LDS
+constexpr int kCommonCache = 64;           // runs of the support common to every patch kept in LDS
+// static LDS of coarse_tile beside the image planes: slot descriptors, run
+// caches, per-wave deviations and score queues
+constexpr size_t kStaticLds = 32 * (4 + 4 + 4 + 8 + 8) + 33 * 4 + kRunCache * 8 + kCommonCache * 16 +
+                              (kThreads / 64) * (kWaveRows * kWaveCols * 4 + 64 * 8) + 64;
 
 struct MfmaArgs {
   const NccFrame* frames;
@@ -124,6 +130,31 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
   const int Ci = int(args.C);
   const int shift = lv.shift;
   const int lo_mask = (1 << shift) - 1;
+  // The kept windows' exact moments need each slot's patch descriptor and
+  // support runs, the common-support deviations the common runs: they are
+  // loaded with the tile (threads 0-31: the slots' descriptors, 64..: the
+  // common runs; then every thread at most two runs) and kept in LDS.
+  __shared__ int s_slot_n[32], s_slot_roff[32], s_run0[33];
+  __shared__ int32_t s_slot_ref[32];
+  __shared__ double s_slot_wsum[32], s_slot_weff[32];
+  __shared__ int2 s_runs[kRunCache];      // (row, begin | end << 16)
+  __shared__ int4 s_common[kCommonCache];
+  int slot_n = 0, slot_nruns = 0, slot_roff = 0;
+  int32_t slot_ref = 0;
+  double slot_wsum = 0, slot_weff = 0;
+  if (threadIdx.x < 32) {
+    const int aa = tile * 32 + int(threadIdx.x);
+    if (aa < args.n_angles) {
+      slot_ref = args.patch_ids[aa];
+      const NccPatch& Pq = args.pools.patch(slot_ref);
+      slot_n = Pq.n; slot_nruns = Pq.n_runs; slot_roff = Pq.run_off; slot_wsum = Pq.wsum;
+      slot_weff = args.wsum_eff[aa];
+    }
+  }
+  const bool common_cached = args.n_common_runs <= kCommonCache;
+  const int cj = int(threadIdx.x) - 64;
+  int4 crun = make_int4(0, 0, 0, 0);
+  if (common_cached && cj >= 0 && cj < args.n_common_runs) crun = args.common_runs[cj];
   // ---- stage the tile: image rows r0 + ymin .., columns c0 + xmin ..
   // (U loads in flight per thread before the first use).  Element i is
   // (i / cols, i % cols); the quotient comes from a float reciprocal, exact
@@ -160,7 +191,48 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
       }
     }
   }
+  if (threadIdx.x < 32) {  // first run of each slot: exclusive scan over lanes 0-31 of wave 0
+    int incl = slot_nruns;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const int o = __shfl_up(incl, d, 64);
+      if (int(threadIdx.x) >= d) incl += o;
+    }
+    s_run0[threadIdx.x] = incl - slot_nruns;
+    if (threadIdx.x == 31) s_run0[32] = incl;
+    s_slot_n[threadIdx.x] = slot_n;
+    s_slot_roff[threadIdx.x] = slot_roff;
+    s_slot_ref[threadIdx.x] = slot_ref;
+    s_slot_wsum[threadIdx.x] = slot_wsum;
+    s_slot_weff[threadIdx.x] = slot_weff;
+  }
+  if (common_cached && cj >= 0 && cj < args.n_common_runs) s_common[cj] = crun;
   __syncthreads();
+  // the slots' runs: loads issued here, stored before the epilogue's barrier
+  const int n_runs_all = s_run0[32];
+  const bool runs_cached = n_runs_all <= kRunCache;
+  int4 myrun[2] = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int j = int(threadIdx.x) + kThreads * u;
+    if (runs_cached && j < n_runs_all) {
+      int lo = 0, hi = 31;  // last slot whose first run is <= j
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_run0[mid] <= j) lo = mid;
+        else hi = mid - 1;
+      }
+      myrun[u] = reinterpret_cast<const int4*>(args.pools.of(s_slot_ref[lo]).runs)[s_slot_roff[lo] + j - s_run0[lo]];
+    }
+  }
+  auto store_runs = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = int(threadIdx.x) + kThreads * u;
+      if (runs_cached && j < n_runs_all)
+        s_runs[j] = make_int2(myrun[u].x, (myrun[u].y & 0xffff) | (myrun[u].z << 16));
+    }
+  };
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wc = wave & 3, wr0 = (wave >> 2) * kWaveRows;  // column group, first block row of this wave
   const int m = lane & 31, g = lane >> 5;
@@ -177,11 +249,22 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
       const int rr = lane >> 4, ca = wc * kWaveCols + 2 * (lane & 15);
       int64_t m1[2] = {0, 0}, m2[2] = {0, 0};
       for (int k = 0; k < args.n_common_runs; ++k) {
-        const int4 run = args.common_runs[k];
+        const int4 run = common_cached ? s_common[k] : args.common_runs[k];
         const uint16_t* src = su + (wr0 + rr + run.x - args.ymin) * P + (ca - args.xmin);
         int32_t s1 = 0;
         RunAcc s2 = 0;
-        for (int x = run.y; x <= run.z; ++x) {
+        int x = run.y;
+        for (; x + 7 <= run.z; x += 8) {  // eight reads in flight
+          int32_t t[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) t[u] = int32_t(src[x + u]) - Ci;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            s1 += t[u];
+            s2 += RunAcc(uint32_t(t[u] * t[u]));
+          }
+        }
+        for (; x <= run.z; ++x) {
           const int32_t t = int32_t(src[x]) - Ci;
           s1 += t;
           s2 += RunAcc(uint32_t(t * t));
@@ -209,6 +292,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
   bool run_bf16 = true;  // wave-uniform
   if constexpr (kScreen) {
     // the screen needs the common-support deviations first
+    store_runs();
     common_sd();
     __syncthreads();
     floatx16 acc8[kWaveRows];
@@ -301,6 +385,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     }
   }
   if constexpr (!kScreen) {
+    store_runs();
     common_sd();
     __syncthreads();
   }
@@ -324,16 +409,32 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
       const float dv = s_dot[wave][lane];
       const int r = code >> 16, col = (code >> 8) & 0xff, slot = code & 0xff;
       const int aa = tile * 32 + slot;
-      const int32_t rf = args.patch_ids[aa];
-      const NccPatch Pq = args.pools.patch(rf);
-      const int4* rq = reinterpret_cast<const int4*>(args.pools.of(rf).runs) + Pq.run_off;
+      const int run0 = s_run0[slot], n_runs = s_run0[slot + 1] - run0;
+      const int4* rq = reinterpret_cast<const int4*>(args.pools.of(s_slot_ref[slot]).runs) + s_slot_roff[slot];
       int64_t m1 = 0, m2 = 0;
-      for (int k = 0; k < Pq.n_runs; ++k) {
-        const int4 run = rq[k];
+      for (int k = 0; k < n_runs; ++k) {
+        int4 run;
+        if (runs_cached) {
+          const int2 c = s_runs[run0 + k];
+          run = make_int4(c.x, int(int16_t(c.y & 0xffff)), c.y >> 16, 0);
+        } else {
+          run = rq[k];
+        }
         const uint16_t* src = su + (r + run.x - args.ymin) * P + (wc * kWaveCols + col - args.xmin);
         int32_t s1 = 0;
         RunAcc s2 = 0;
-        for (int x = run.y; x <= run.z; ++x) {
+        int x = run.y;
+        for (; x + 7 <= run.z; x += 8) {  // eight reads in flight
+          int32_t t[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) t[u] = int32_t(src[x + u]) - Ci;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            s1 += t[u];
+            s2 += RunAcc(uint32_t(t[u] * t[u]));
+          }
+        }
+        for (; x <= run.z; ++x) {
           const int32_t t = int32_t(src[x]) - Ci;
           s1 += t;
           s2 += RunAcc(uint32_t(t * t));
@@ -341,10 +442,10 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
         m1 += s1;
         m2 += int64_t(s2);
       }
-      const int64_t nq = Pq.n;
+      const int64_t nq = s_slot_n[slot];
       const int64_t sum_s = m1 + nq * Cl;
       const int64_t sum_s2 = m2 + 2 * Cl * m1 + nq * Cl * Cl;
-      const double score = normalise(double(dv), args.C, args.wsum_eff[aa], Pq.wsum, Pq.n, sum_s, sum_s2, D, shift,
+      const double score = normalise(double(dv), args.C, s_slot_weff[slot], s_slot_wsum[slot], int(nq), sum_s, sum_s2, D, shift,
                                      dc, args.exact_lattice, args.absolute);
       const size_t pos = size_t(r0 + r) * W + c0 + wc * kWaveCols + col;
       if (!sparse) {
@@ -503,7 +604,7 @@ size_t coarse_mfma_lds(int level, int level_kind, int yspan, int xchunks, bool s
   *pitch8 = (cols + 11) & ~3;  // an 8-byte A fragment read as three dwords from any column
   size_t lds = size_t(5) * rows * *pitch * sizeof(uint16_t);
   if (screened) lds += size_t(rows) * *pitch8;
-  constexpr size_t kLdsMax = 160 * 1024 - (kThreads / 64) * (kWaveRows * kWaveCols * sizeof(float) + 64 * 8);
+  constexpr size_t kLdsMax = 160 * 1024 - kStaticLds;
   return lds > kLdsMax ? 0 : lds;
 }
 }  // namespace
