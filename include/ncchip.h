@@ -81,15 +81,21 @@ int ncchip_coarse(const NccFrame* frames, int n_frames, int level, int level_kin
                   const NccPatchPool* pools, const int32_t* patch_ids, int n_angles, int max_bbox_h, int max_bbox_w,
                   int form, int exact_lattice, int absolute, void* stream);
 
-/* Dense split-BF16 weights of the coarse patches over their union bounding
- * box (rows ymin .. ymin + yspan - 1, columns xmin .. xmin + 16 * xchunks - 1),
- * laid out as MFMA B fragments: for angle tile t (32 angles), template row
- * index yi and 16-column chunk ci, 64 lanes x 8 bf16 (lane L: angle
- * 32 t + L % 32, columns 16 ci + 8 (L / 32) .. + 7).  wsum_eff[a] is the FP64
+/* Dense split-BF16 weights of the coarse patches within their union bounding
+ * box (rows ymin .. ymin + yspan - 1, columns xmin .. xmin + 16 * xchunks - 1,
+ * which sizes the kernel's image tile), laid out as MFMA B fragments by K
+ * step: the steps of angle tile t (32 angles) are step0[t] .. step0[t+1] - 1;
+ * step s covers two 8-column segments, segs[2 s] (half-wave 0) and
+ * segs[2 s + 1] (half-wave 1), each packed as (y << 16 | x) relative to
+ * (ymin, xmin); a tile's segments cover, row by row, the union of its
+ * patches' supports.  wh / wl hold per step 64 lanes x 8 bf16 (lane L: angle
+ * 32 t + L % 32, the 8 columns of segment L / 32).  wsum_eff[a] is the FP64
  * sum of w_hi + w_lo over angle a's samples; max_run the longest row run. */
 typedef struct NcchipDenseWeights {
   const void* wh;          /* device, 16-byte aligned */
   const void* wl;
+  const int32_t* step0;    /* device, tiles + 1 entries */
+  const int32_t* segs;     /* device, 2 per step */
   const double* wsum_eff;  /* device */
   int32_t ymin, xmin, yspan, xchunks, max_run;
   /* Runs (4 ints: row, first col, last col, 0) of the support common to

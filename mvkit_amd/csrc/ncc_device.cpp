@@ -174,10 +174,13 @@ double bf16_value(uint16_t h) {
 }
 
 // Split-BF16 dense weights of the coarse patches, laid out as MFMA B
-// fragments (see NcchipDenseWeights).
+// fragments by K step (see NcchipDenseWeights): per angle tile, the rows of
+// the union of its patches' supports cut into 8-column segments, two per step.
 struct DenseWeights {
-  std::vector<uint16_t> wh, wl;  // [tile][yspan][xchunks][64 lanes][8]
-  std::vector<double> wsum_eff;  // per angle
+  std::vector<uint16_t> wh, wl;      // [K step][64 lanes][8]
+  std::vector<int32_t> step0;        // first step of each tile, tiles + 1 entries
+  std::vector<int32_t> segs;         // 2 per step: (y << 16 | x) of half-wave 0 and 1
+  std::vector<double> wsum_eff;      // per angle
   std::vector<int32_t> common_runs;  // 4 ints per run of the common support
   int n_common = 0;
   int ymin = 0, xmin = 0, yspan = 0, xchunks = 0, max_run = 0;
@@ -194,7 +197,33 @@ DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std:
   d.yspan = ymax - d.ymin + 1;
   d.xchunks = (xmax - d.xmin + 16) / 16;
   const size_t tiles = (patches.size() + 31) / 32;
-  d.wh.assign(tiles * size_t(d.yspan) * d.xchunks * 64 * 8, 0);
+  // segments of every tile: row by row over the union of its supports
+  std::vector<std::vector<int>> first_seg(tiles, std::vector<int>(size_t(d.yspan), -1));
+  std::vector<std::vector<int>> row_lo(tiles, std::vector<int>(size_t(d.yspan), 1 << 30));
+  d.step0.assign(tiles + 1, 0);
+  for (size_t t = 0; t < tiles; ++t) {
+    std::vector<int> hi(size_t(d.yspan), -(1 << 30));
+    for (size_t a = 32 * t; a < std::min(patches.size(), 32 * t + 32); ++a) {
+      const Raster& p = *patches[a];
+      const RasterPool& pool = *pools[a];
+      for (uint32_t i = 0; i < p.count; ++i) {
+        const int32_t rc = pool.rc[p.first + i];
+        const int yi = (rc >> 16) - d.ymin, xi = int(int16_t(rc & 0xffff)) - d.xmin;
+        row_lo[t][size_t(yi)] = std::min(row_lo[t][size_t(yi)], xi);
+        hi[size_t(yi)] = std::max(hi[size_t(yi)], xi);
+      }
+    }
+    std::vector<int32_t> seg;
+    for (int y = 0; y < d.yspan; ++y) {
+      if (row_lo[t][size_t(y)] > hi[size_t(y)]) continue;
+      first_seg[t][size_t(y)] = int(seg.size());
+      for (int x = row_lo[t][size_t(y)]; x <= hi[size_t(y)]; x += 8) seg.push_back((y << 16) | x);
+    }
+    if (seg.size() & 1) seg.push_back(0);  // a step's second half over zero weights
+    d.step0[t + 1] = d.step0[t] + int(seg.size() / 2);
+    d.segs.insert(d.segs.end(), seg.begin(), seg.end());
+  }
+  d.wh.assign(size_t(d.step0[tiles]) * 64 * 8, 0);
   d.wl.assign(d.wh.size(), 0);
   d.wsum_eff.assign(patches.size(), 0.0);
   double wmax = 0;
@@ -210,9 +239,10 @@ DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std:
       const uint16_t h = bf16_bits(w), l = bf16_bits(w - bf16_value(h));
       ws += bf16_value(h) + bf16_value(l);
       const int yi = y - d.ymin, xi = x - d.xmin;
-      const int ci = xi / 16, g = (xi % 16) / 8, j = xi % 8;
-      const size_t lane = size_t(g) * 32 + slot;
-      const size_t e = ((((t * d.yspan + size_t(yi)) * d.xchunks + size_t(ci)) * 64 + lane) * 8) + size_t(j);
+      const int k = first_seg[t][size_t(yi)] + (xi - row_lo[t][size_t(yi)]) / 8;
+      const int j = (xi - row_lo[t][size_t(yi)]) % 8;
+      const size_t step = size_t(d.step0[t]) + size_t(k / 2), lane = size_t(k % 2) * 32 + slot;
+      const size_t e = (step * 64 + lane) * 8 + size_t(j);
       d.wh[e] = h;
       d.wl[e] = l;
     }
@@ -404,6 +434,7 @@ struct FindContext::Plan {
   unsigned long long* work = nullptr;
   NccPatchPool pools[2] = {};
   size_t off_key = 0, off_raw = 0, off_table = 0, off_wh = 0, off_wl = 0, off_wsum = 0, off_common = 0;
+  size_t off_step0 = 0, off_segs = 0;
   size_t rb_full = 0, rb_counts = 0;
   std::vector<size_t> off_refs;
   uint8_t* rb = nullptr;
@@ -979,6 +1010,8 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
   const size_t off_extent = off_small + small; small = align_up(small + n_frames * 32);
   const size_t off_wh = off_small + small; small = align_up(small + dense.wh.size() * 2);
   const size_t off_wl = off_small + small; small = align_up(small + dense.wl.size() * 2);
+  const size_t off_step0 = off_small + small; small = align_up(small + dense.step0.size() * 4);
+  const size_t off_segs = off_small + small; small = align_up(small + dense.segs.size() * 4);
   const size_t off_wsum = off_small + small; small = align_up(small + dense.wsum_eff.size() * 8);
   const size_t off_common = off_small + small; small = align_up(small + dense.common_runs.size() * 4);
   const size_t off_frames = off_small + small; small = align_up(small + n_frames * sizeof(NccFrame));
@@ -1017,6 +1050,8 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
     if (use_mfma) {
       std::memcpy(st + (off_wh - off_small), dense.wh.data(), dense.wh.size() * 2);
       std::memcpy(st + (off_wl - off_small), dense.wl.data(), dense.wl.size() * 2);
+      std::memcpy(st + (off_step0 - off_small), dense.step0.data(), dense.step0.size() * 4);
+      std::memcpy(st + (off_segs - off_small), dense.segs.data(), dense.segs.size() * 4);
       std::memcpy(st + (off_wsum - off_small), dense.wsum_eff.data(), dense.wsum_eff.size() * 8);
       std::memcpy(st + (off_common - off_small), dense.common_runs.data(), dense.common_runs.size() * 4);
     }
@@ -1078,7 +1113,7 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
   P.dev = dev; P.dframes = dframes; P.lists = lists; P.counts = counts; P.extent = extent; P.work = work;
   P.pools[0] = pools[0]; P.pools[1] = pools[1];
   P.off_key = off_key; P.off_raw = off_raw; P.off_table = off_table; P.off_wh = off_wh; P.off_wl = off_wl;
-  P.off_wsum = off_wsum; P.off_common = off_common; P.rb_full = rb_full; P.rb_counts = rb_counts;
+  P.off_wsum = off_wsum; P.off_common = off_common; P.off_step0 = off_step0; P.off_segs = off_segs; P.rb_full = rb_full; P.rb_counts = rb_counts;
   P.off_refs = std::move(off_refs); P.rb = rb;
   P.off_small = off_small; P.small = small; P.off_bits = off_bits; P.n_bit_words = n_bit_words;
   return true;
@@ -1155,6 +1190,8 @@ NcchipCoarseJob FindContext::coarse_job(Slot& S) {
   j.n_angles = int(P.A);
   NcchipDenseWeights& dw = j.weights;
   dw.wh = dev + P.off_wh; dw.wl = dev + P.off_wl; dw.wsum_eff = reinterpret_cast<const double*>(dev + P.off_wsum);
+  dw.step0 = reinterpret_cast<const int32_t*>(dev + P.off_step0);
+  dw.segs = reinterpret_cast<const int32_t*>(dev + P.off_segs);
   dw.ymin = dense.ymin; dw.xmin = dense.xmin; dw.yspan = dense.yspan; dw.xchunks = dense.xchunks;
   dw.max_run = dense.max_run;
   dw.common_runs = reinterpret_cast<const int32_t*>(dev + P.off_common);

@@ -6,8 +6,10 @@
 // with T = S - C the centred top-level block sums and W_a the dense weights of
 // coarse patch a over the union bounding box of all coarse patches (zero
 // where the patch has no sample).  v_mfma_f32_32x32x16_bf16 takes 32 anchors
-// (M), 32 angle slots (N) and 16 consecutive template columns of one template
-// row (K).  Precision: T = T_hi + T_lo with T_hi = 4^l * (floor(S / 4^l) - 128)
+// (M), 32 angle slots (N) and 16 taps (K) as two runs of 8 consecutive
+// template columns, one per half-wave, each from any row: the K steps of an
+// angle tile walk 8-column segments covering, row by row, the union of the
+// tile's supports (not the whole bounding box).  Precision: T = T_hi + T_lo with T_hi = 4^l * (floor(S / 4^l) - 128)
 // and T_lo = S mod 4^l, both exact in BF16 for levels <= 4; each weight is
 // split into BF16 w_hi + w_lo (w_lo = bf16(w - w_hi)); the products
 // T_hi*w_hi, T_hi*w_lo and T_lo*w_hi are exact in FP32 and accumulate in FP32
@@ -61,10 +63,12 @@ struct MfmaArgs {
   double C;                  // 128 * 4^level
   Pools pools;
   const int32_t* patch_ids;  // coarse patches as (pool << 24) | index
-  const uint4* wh;           // [tile][Yspan][XC][64 lanes] of 8 bf16 (w_hi)
+  const uint4* wh;           // [K step][64 lanes] of 8 bf16 (w_hi); the steps of tile t from step0[t]
   const uint4* wl;           // same layout (w_lo)
+  const int* step0;          // first K step of each angle tile, n_tiles + 1 entries
+  const int2* segs;          // per K step: (y << 16 | x) of the segment of half-wave 0 and of half-wave 1
   const double* wsum_eff;    // per angle: sum of (w_hi + w_lo) in FP64
-  int ymin, xmin, yspan, xc; // union bounding box (rows, first column) and 16-column chunks
+  int ymin, xmin, yspan, xc; // union bounding box (rows, first column) and 16-column chunks (LDS tile width)
   int pitch;                 // LDS row pitch (elements, even)
   int exact_lattice, absolute;
   const int4* common_runs;   // support shared by every coarse patch
@@ -115,7 +119,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
   if (r0 >= H || c0 >= W) return;  // uniform per block
   const int P = args.pitch;
   const int rows = kMRows + args.yspan - 1;
-  const int cols = kBlockCols + 16 * args.xc + 1;
+  const int cols = kBlockCols + 16 * args.xc + 9;  // a segment may start at the last column of the box
   uint16_t* th0 = lds;                // T_hi bits
   uint16_t* th1 = th0 + rows * P;     // T_hi shifted left by one column
   uint16_t* tl0 = th1 + rows * P;     // T_lo bits
@@ -124,9 +128,11 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
   uint8_t* x8 = reinterpret_cast<uint8_t*>(su + rows * P);  // E4M3 of T * img8_scale (screen only)
   // the angle tile's split weights (MFMA B fragments), read from L2 one
   // iteration ahead of their use
-  const int nb = args.yspan * args.xc * 64;  // B fragments (uint4) per half
-  const uint4* gwh = args.wh + size_t(tile) * nb;
-  const uint4* gwl = args.wl + size_t(tile) * nb;
+  const int step0 = args.step0[tile];
+  const int n_steps = args.step0[tile + 1] - step0;
+  const uint4* gwh = args.wh + size_t(step0) * 64;
+  const uint4* gwl = args.wl + size_t(step0) * 64;
+  const int2* segs = args.segs + step0;
   const int Ci = int(args.C);
   const int shift = lv.shift;
   const int lo_mask = (1 << shift) - 1;
@@ -288,7 +294,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     }
   };
   // ---- MFMA: this wave's 32 anchors x 32 angle slots x kMRows rows
-  const int col_base = wc * kWaveCols + m + 8 * g;  // tile column of this lane's first A element for chunk 0
+  const int col_base = wc * kWaveCols + m;  // tile column of this lane's first A element for segment column 0
   bool run_bf16 = true;  // wave-uniform
   if constexpr (kScreen) {
     // the screen needs the common-support deviations first
@@ -300,16 +306,17 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     for (int r = 0; r < kWaveRows; ++r)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc8[r][v] = 0.f;
-    const int n_it8 = args.yspan * args.xc;
-    for (int it = 0; it < n_it8; ++it) {
-      const int yi = it / args.xc, ci = it - yi * args.xc;
+    for (int it = 0; it < n_steps; ++it) {
+      const int2 sg = segs[it];
+      const int sv = g ? sg.y : sg.x;
+      const int yi = sv >> 16, xi = sv & 0xffff;
       const uint4 bq = gwh[it * 64 + lane];
       int b0 = fp8_pack2<false>(__uint_as_float(bq.x << 16) * args.w8_scale, __uint_as_float(bq.x & 0xffff0000u) * args.w8_scale, 0);
       b0 = fp8_pack2<true>(__uint_as_float(bq.y << 16) * args.w8_scale, __uint_as_float(bq.y & 0xffff0000u) * args.w8_scale, b0);
       int b1 = fp8_pack2<false>(__uint_as_float(bq.z << 16) * args.w8_scale, __uint_as_float(bq.z & 0xffff0000u) * args.w8_scale, 0);
       b1 = fp8_pack2<true>(__uint_as_float(bq.w << 16) * args.w8_scale, __uint_as_float(bq.w & 0xffff0000u) * args.w8_scale, b1);
       const long B8 = long(uint32_t(b0)) | (long(uint32_t(b1)) << 32);
-      const int cs = col_base + 16 * ci;
+      const int cs = col_base + xi;
       const int sh = cs & 3;
 #pragma unroll
       for (int r = 0; r < kWaveRows; ++r) {
@@ -351,10 +358,16 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
   for (int r = 0; r < kWaveRows; ++r)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[r][v] = 0.f;
-  const int n_iter = args.yspan * args.xc;
-  uint4 bh_next = gwh[lane], bl_next = gwl[lane];
+  const int n_iter = n_steps;
+  uint4 bh_next = make_uint4(0, 0, 0, 0), bl_next = bh_next;
+  if (n_iter > 0) {
+    bh_next = gwh[lane];
+    bl_next = gwl[lane];
+  }
   for (int it = 0; run_bf16 && it < n_iter; ++it) {
-    const int yi = it / args.xc, ci = it - yi * args.xc;
+    const int2 sg = segs[it];
+    const int sv = g ? sg.y : sg.x;
+    const int yi = sv >> 16, xi = sv & 0xffff;
     {
       const uint4 bh4 = bh_next, bl4 = bl_next;
       if (it + 1 < n_iter) {
@@ -362,7 +375,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
         bl_next = gwl[(it + 1) * 64 + lane];
       }
       const bf16x8 bh = __builtin_bit_cast(bf16x8, bh4), bl = __builtin_bit_cast(bf16x8, bl4);
-      const int cs = col_base + 16 * ci;
+      const int cs = col_base + xi;
       const bool odd = cs & 1;
       const int ce = cs - (odd ? 1 : 0);
       const uint16_t* ph = odd ? th1 : th0;
@@ -598,7 +611,7 @@ namespace {
 // level cannot run in the split-BF16 form.
 size_t coarse_mfma_lds(int level, int level_kind, int yspan, int xchunks, bool screened, int* pitch, int* pitch8) {
   if (level_kind == NCC_U32 || level > 4) return 0;  // T_lo exact in BF16 only up to level 4
-  const int cols = kBlockCols + 16 * xchunks + 1;
+  const int cols = kBlockCols + 16 * xchunks + 9;
   const int rows = kMRows + yspan - 1;
   *pitch = (cols + 1) & ~1;
   *pitch8 = (cols + 11) & ~3;  // an 8-byte A fragment read as three dwords from any column
@@ -626,6 +639,7 @@ int make_args(const NccFrame* frames, int n_frames, int level, int level_kind, d
   a.frames = frames; a.level = level; a.n_angles = n_angles; a.n_tiles = (n_angles + 31) / 32; a.C = C;
   a.pools.p0 = pools[0]; a.pools.p1 = pools[1]; a.patch_ids = patch_ids;
   a.wh = reinterpret_cast<const uint4*>(dw->wh); a.wl = reinterpret_cast<const uint4*>(dw->wl);
+  a.step0 = dw->step0; a.segs = reinterpret_cast<const int2*>(dw->segs);
   a.wsum_eff = dw->wsum_eff; a.ymin = dw->ymin; a.xmin = dw->xmin; a.yspan = dw->yspan; a.xc = dw->xchunks;
   a.exact_lattice = exact_lattice; a.absolute = absolute;
   a.common_runs = reinterpret_cast<const int4*>(dw->common_runs); a.n_common_runs = dw->n_common_runs;
