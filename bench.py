@@ -399,6 +399,19 @@ def main():
     log(f"warmup {args.warmup} steps x {F} frames")
     pipelined(opts, args.warmup)
 
+    # the coarse tile shape the autotuner chose on the first batch (the
+    # sample launch's time with it and with the default shape)
+    def tuned_shape(m):
+        out = []
+        for line in m.tuning.splitlines():
+            key, rest = line.split(" : ")
+            f = dict(kv.split("=") for kv in rest.split())
+            out.append({"key": key, "shape": [int(f[k]) for k in ("rows", "col_waves", "row_waves", "tiles")],
+                        "us": round(float(f["us"]), 2), "default_us": round(float(f["default_us"]), 2)})
+        return out
+    coarse_tiles = tuned_shape(model)
+    log(f"autotuned coarse tiles: {coarse_tiles}")
+
     # per-stage device time from a short pass with every stage timed
     # (an event pair around each stage's launches on the library's stream)
     kernel = {k: 0.0 for k in capi.STAGES}
@@ -555,6 +568,7 @@ def main():
                      "traffic": traffic,
                      "us_per_frame": round(per_frame_us, 2), "algorithmic_gflop_per_frame": round(per_frame_flop / 1e9, 4)},
         "kernel_us_per_frame": {k: round(v, 2) for k, v in kernel.items()},
+        "coarse_tile_autotuned": coarse_tiles,
         "dominant_kernel": {"stage": dominant, "share": round(kernel[dominant] / total_kernel, 3) if total_kernel else 0},
         "fp32_form": fp32,
         "speedup_vs_fp32": round(mpx / fp32["value"], 3) if fp32 else None,

@@ -43,8 +43,25 @@ typedef struct MvAmdFindOptions {
   int32_t no_graph;                      /* 0: the device part of a find whose model, options, frame
                                             geometry and workspace repeat is replayed as a captured
                                             HIP graph; 1: always launched kernel by kernel */
-  int32_t reserved;
+  int32_t coarse_tile;                   /* tile shape of the split-BF16 coarse kernel:
+                                            MVAMD_COARSE_TILE_AUTO (0) the model's autotuned shape,
+                                            MVAMD_COARSE_TILE_DEFAULT (-1), or MVAMD_COARSE_TILE(...) */
 } MvAmdFindOptions;
+
+/* Tile shapes of the split-BF16 coarse kernel.  A wave computes `rows`
+ * output rows (4 or 8) x 32 anchors against 32 rotations; a block has
+ * col_waves x row_waves waves (each 1, 2 or 4, product <= 4) and runs `tiles`
+ * angle tiles of 32 rotations (1..8) over one image tile staged in LDS.
+ * Every shape gives bit-identical planes.  With MVAMD_COARSE_TILE_AUTO the
+ * first find of a kind of search (level, pixel type, coarse angles, template
+ * extent, screen, frame count and size class) times the candidate shapes on
+ * its own frames and keeps the fastest in the model for later finds
+ * (mvamd_get_ncc_model_tuning / mvamd_set_ncc_model_tuning read and restore
+ * those choices). */
+#define MVAMD_COARSE_TILE_AUTO 0
+#define MVAMD_COARSE_TILE_DEFAULT (-1)
+#define MVAMD_COARSE_TILE(rows, col_waves, row_waves, tiles) \
+  ((rows) | ((col_waves) << 4) | ((row_waves) << 8) | ((tiles) << 12))
 
 /* Pixel types of device-resident search images ("byte" and "uint2" in the
  * reference's image objects, c_api/ncc_matching_api.cpp:167-168). */
@@ -72,7 +89,8 @@ typedef struct MvAmdFindStats {
   double track_flops;                    /* algorithmic FLOPs of tracking (when its stage is profiled) */
   double track_mfma_flops;               /* the part of track_flops tracked on matrix cores */
   int32_t coarse_form;                   /* MVAMD_FORM_* the coarse search ran (-1: none ran) */
-  int32_t reserved;
+  int32_t coarse_tile;                   /* MVAMD_COARSE_TILE(...) shape of the split-BF16 coarse
+                                            kernel that ran (0: it did not run) */
 } MvAmdFindStats;
 
 /* Find on a u8 (or, with pixel_type MVAMD_PIXEL_UINT2, u16) image given as
@@ -213,6 +231,21 @@ MV_API MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* image
 #define MVAMD_CREATE_DEVICE 2
 MV_API MvStatus mvamd_create_ncc_model(MvObject Template, MvTuple NumLevels, MvTuple AngleStart, MvTuple AngleExtent,
                                        MvTuple AngleStep, MvTuple Metric, int32_t builder, MvTuple* ModelID);
+
+/* The model's autotuned coarse tile shapes as text, one line per kind of
+ * search: "<key> : rows=R col_waves=C row_waves=W tiles=T us=U default_us=D"
+ * (key: level, pixel kind, coarse angles, union box rows and 16-column
+ * chunks, screen, frame count and top-level size classes; us and default_us
+ * the sample launch's time with the chosen and the default shape).  text
+ * receives at most cap bytes including the terminating zero; *length the
+ * full length without it (call with cap 0 to size the buffer).  No
+ * counterpart in the reference (its CPU kernels have no tile shapes). */
+MV_API MvStatus mvamd_get_ncc_model_tuning(MvTuple ModelID, char* text, size_t cap, size_t* length);
+/* Replaces the model's tuned shapes with the lines of `text` (as
+ * mvamd_get_ncc_model_tuning writes them; empty text clears them, so the next
+ * find of each kind tunes again).  Lines whose shape is invalid are rejected
+ * (INVALID_ARGUMENT) and nothing is changed. */
+MV_API MvStatus mvamd_set_ncc_model_tuning(MvTuple ModelID, const char* text);
 
 /* Model introspection.  info: levels, angles per level, angle step,
  * origin row, origin column, sample count. */

@@ -126,16 +126,29 @@ typedef struct NcchipDenseWeights {
  * windows reaches (|dot8| + C |wsum|) / sd_common >= screen skips its
  * split-BF16 products (needs weights->w8_scale; the surviving waves' scores
  * are the split-BF16 form's). */
+/* Tile shape of the split-BF16 coarse kernel: each wave computes `rows`
+ * output rows (4 or 8) x 32 anchors; a block has col_waves waves across and
+ * row_waves waves down (col_waves * row_waves <= 4; each 1, 2 or 4) and runs
+ * `tiles` angle tiles of 32 rotations (1..8) over one staged image tile.  The
+ * planes are bit-identical for every shape; only the time differs.  The
+ * default shape is {4, 4, 1, 1}. */
+typedef struct NcchipCoarseTile {
+  int32_t rows, col_waves, row_waves, tiles;
+} NcchipCoarseTile;
+
 /* 1 when ncchip_coarse_mfma can run the top level `level` of pixel kind
  * level_kind over a union bounding box of yspan rows and xchunks 16-column
- * chunks (its image tile fits in LDS; levels up to 4 of u8/u16 block sums),
- * else 0.  The host driver then runs the FP32 form for that search. */
-int ncchip_coarse_mfma_fits(int level, int level_kind, int yspan, int xchunks, int screened);
+ * chunks with tile shape `tile` (null: the default shape): the shape is
+ * valid and its image tile fits in LDS (levels up to 4 of u8/u16 block
+ * sums), else 0.  The host driver then runs the FP32 form for that search
+ * (default shape) or another shape. */
+int ncchip_coarse_mfma_fits(int level, int level_kind, int yspan, int xchunks, int screened,
+                            const NcchipCoarseTile* tile);
 
 int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w, int max_h,
                        double C, const NccPatchPool* pools, const int32_t* patch_ids, int n_angles,
                        const NcchipDenseWeights* weights, int exact_lattice, int absolute, double screen,
-                       void* stream);
+                       const NcchipCoarseTile* tile, void* stream);
 
 /* The split-BF16 coarse searches of several queries (models of uneven
  * template size and angle count, over frames of uneven size) in one
@@ -161,6 +174,7 @@ typedef struct NcchipCoarseJob {
   NcchipDenseWeights weights;
   int32_t exact_lattice, absolute;
   double screen;
+  NcchipCoarseTile tile;      /* tile shape; rows == 0: the default shape */
 } NcchipCoarseJob;
 size_t ncchip_coarse_mfma_queue_bytes(int n_jobs, int n_frames_total);
 int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs, void* d_work, size_t work_bytes, void* h_stage,

@@ -37,6 +37,7 @@ PUBLIC_SYMBOLS = [
     "mvamd_find_ncc_models_device", "mvamd_find_ncc_models_batch", "mvamd_find_ncc_model_batch",
     "mvamd_server_create", "mvamd_server_destroy", "mvamd_server_submit", "mvamd_server_wait", "mvamd_find_ncc_model_batch_submit",
                   "mvamd_find_ncc_model_batch_wait", "mvamd_find_ncc_model_trace", "mvamd_model_info", "mvamd_model_raster",
+    "mvamd_get_ncc_model_tuning", "mvamd_set_ncc_model_tuning",
 ]
 # Symbols declared by include/ncchip.h.
 NCCHIP_SYMBOLS = ["ncchip_pyramid_level", "ncchip_row_prefix", "ncchip_pyramid_batch", "ncchip_row_prefix_batch",
@@ -65,7 +66,7 @@ class FindOptions(C.Structure):
                 ("highest_level", C.c_int32), ("lowest_level", C.c_int32), ("form", C.c_int32),
                 ("screen", C.c_double), ("profile", C.c_int32), ("vector_tracking", C.c_int32),
                 ("pixel_type", C.c_int32), ("max_candidates", C.c_int64), ("no_graph", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("coarse_tile", C.c_int32)]
 
 
 class FindStats(C.Structure):
@@ -74,7 +75,7 @@ class FindStats(C.Structure):
                 ("t_track_us", C.c_double), ("t_refine_us", C.c_double), ("t_total_us", C.c_double),
                 ("kernel_us", C.c_double * 8), ("coarse_flops", C.c_double),
                 ("track_flops", C.c_double), ("track_mfma_flops", C.c_double), ("coarse_form", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("coarse_tile", C.c_int32)]
 
 class ServerConfig(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_batch", C.c_int32), ("hbm_budget", C.c_int64),
@@ -93,6 +94,18 @@ class BatchStats(C.Structure):
 
 
 PROFILE_ALL = 0xff
+COARSE_TILE_AUTO = 0
+COARSE_TILE_DEFAULT = -1
+
+
+def coarse_tile(rows: int, col_waves: int, row_waves: int, tiles: int) -> int:
+    """MVAMD_COARSE_TILE: a split-BF16 coarse tile shape packed for the options."""
+    return rows | (col_waves << 4) | (row_waves << 8) | (tiles << 12)
+
+
+def tile_fields(v: int) -> tuple:
+    """(rows, col_waves, row_waves, tiles) of a packed shape (0: none)."""
+    return (v & 15, (v >> 4) & 15, (v >> 8) & 15, (v >> 12) & 15)
 STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine", "track_mfma"]
 
 
@@ -178,6 +191,8 @@ def lib() -> C.CDLL:
         L.mvamd_find_ncc_model_trace.argtypes = [P, P, I64, I64, I64, C.POINTER(FindOptions), P, S, C.POINTER(S), P, S,
                                                  C.POINTER(S), P, P, S, C.POINTER(S), C.POINTER(FindStats)]
         L.mvamd_model_info.argtypes = [P, P]
+        L.mvamd_get_ncc_model_tuning.argtypes = [P, C.c_char_p, S, C.POINTER(S)]
+        L.mvamd_set_ncc_model_tuning.argtypes = [P, C.c_char_p]
         L.mvamd_model_raster.argtypes = [P, I32, I32, P, P, P, P, S]
         _lib = L
     return _lib
@@ -426,6 +441,19 @@ class NccModel:
     def clear(self):
         _check(lib().mv_clear_handle(self.tuple.h))
 
+    @property
+    def tuning(self) -> str:
+        """The autotuned coarse tile shapes (mvamd_get_ncc_model_tuning)."""
+        n = C.c_size_t()
+        _check(lib().mvamd_get_ncc_model_tuning(self.tuple.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(lib().mvamd_get_ncc_model_tuning(self.tuple.h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    @tuning.setter
+    def tuning(self, text: str) -> None:
+        _check(lib().mvamd_set_ncc_model_tuning(self.tuple.h, text.encode()))
+
     def find(self, image, angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5,
              subpixel="true", num_levels=0):
         """mv_find_ncc_model; returns (n, 4) array of Row, Column, Angle, Score
@@ -441,16 +469,19 @@ class NccModel:
     @staticmethod
     def options(angle_start=0.0, angle_extent=0.0, min_score=0.8, num_matches=1, max_overlap=0.5, subpixel=True,
                 highest=0, lowest=1, form=FORM_FP32, screen=-math.inf, profile=0,
-                vector_tracking=False, pixel_type=PIXEL_BYTE, max_candidates=0, graph=True) -> FindOptions:
+                vector_tracking=False, pixel_type=PIXEL_BYTE, max_candidates=0, graph=True,
+                tile=COARSE_TILE_AUTO) -> FindOptions:
         """profile: True times every stage, an int is a mask of stage bits
         (bit i = STAGES[i]).  vector_tracking: track every level with the FP64
         vector kernel instead of matrix cores (same results).  pixel_type:
         PIXEL_BYTE or PIXEL_UINT2 for the device images (pitches in bytes).
-        graph: replay the device part as a captured HIP graph when it repeats."""
+        graph: replay the device part as a captured HIP graph when it repeats.
+        tile: the split-BF16 coarse tile shape (COARSE_TILE_AUTO: the model's
+        autotuned shape, COARSE_TILE_DEFAULT, or coarse_tile(...))."""
         mask = PROFILE_ALL if profile is True else int(profile)
         return FindOptions(angle_start, angle_extent, min_score, max_overlap, num_matches, int(subpixel), highest,
                            lowest, form, screen, mask, int(vector_tracking), int(pixel_type), int(max_candidates),
-                           int(not graph), 0)
+                           int(not graph), int(tile))
 
     def find_image(self, image, opts: FindOptions, cap: int = 4096):
         """mvamd_find_ncc_model_image: find on an Image (or array) with the

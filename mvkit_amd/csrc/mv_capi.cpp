@@ -419,6 +419,9 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
          "candidate budget must be 0 (the default) or 1..2^24");
   if (o->max_candidates > 0) q.max_candidates = size_t(o->max_candidates);
   q.graph = o->no_graph == 0;
+  if (o->coarse_tile < MVAMD_COARSE_TILE_DEFAULT || o->coarse_tile >= (1 << 16))
+    fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, "find_ncc_model", "coarse_tile", "invalid coarse tile shape");
+  q.coarse_tile = o->coarse_tile;
   if (q.form < MVAMD_FORM_AUTO || q.form > MVAMD_FORM_FP8_SCREEN)
     fail(mvamd::kInvalidArgument, mvamd::kErrUnsupportedValue, "find_ncc_model", "form", "unknown correlation form");
   if (o->pixel_type != MVAMD_PIXEL_BYTE && o->pixel_type != MVAMD_PIXEL_UINT2)
@@ -1191,6 +1194,7 @@ void fill_stats(const mvamd::FindTrace& t, MvAmdFindStats* s) {
   s->track_flops = t.track_flops;
   s->track_mfma_flops = t.track_mfma_flops;
   s->coarse_form = t.coarse_form;
+  s->coarse_tile = t.coarse_tile;
 }
 void put_poses(const std::vector<mvamd::Pose>& v, double* dst, size_t cap) {
   for (size_t i = 0; i < v.size() && i < cap; ++i) {
@@ -1564,6 +1568,64 @@ extern "C" MvStatus mvamd_find_ncc_model_trace(MvTuple ModelID, const uint8_t* i
       }
     if (n_prepared) *n_prepared = trace.prepared.size();
     fill_stats(trace, stats);
+  });
+}
+
+extern "C" MvStatus mvamd_get_ncc_model_tuning(MvTuple ModelID, char* text, size_t cap, size_t* length) {
+  return translate([&] {
+    constexpr auto op = "get_ncc_model_tuning";
+    required(length, op, "length");
+    if (cap) required(text, op, "text");
+    const auto m = model_of(ModelID, op);
+    std::string out;
+    {
+      std::lock_guard<std::mutex> lock(m->tune_mutex);
+      char line[320];
+      for (const auto& [key, t] : m->coarse_tuning) {
+        std::snprintf(line, sizeof line, "%s : rows=%d col_waves=%d row_waves=%d tiles=%d us=%.17g default_us=%.17g\n",
+                      key.c_str(), t.rows, t.col_waves, t.row_waves, t.tiles, t.us, t.default_us);
+        out += line;
+      }
+    }
+    *length = out.size();
+    if (cap) {
+      const size_t n = std::min(cap - 1, out.size());
+      std::memcpy(text, out.data(), n);
+      text[n] = 0;
+    }
+  });
+}
+
+extern "C" MvStatus mvamd_set_ncc_model_tuning(MvTuple ModelID, const char* text) {
+  return translate([&] {
+    constexpr auto op = "set_ncc_model_tuning";
+    required(text, op, "text");
+    const auto m = model_of(ModelID, op);
+    std::map<std::string, mvamd::CoarseTune> table;
+    const std::string all(text);
+    size_t pos = 0;
+    int64_t index = 0;
+    while (pos < all.size()) {
+      size_t end = all.find('\n', pos);
+      if (end == std::string::npos) end = all.size();
+      const std::string line = all.substr(pos, end - pos);
+      pos = end + 1;
+      if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+      const size_t sep = line.find(" : ");
+      mvamd::CoarseTune t;
+      const bool ok = sep != std::string::npos && sep > 0 &&
+                      std::sscanf(line.c_str() + sep + 3, "rows=%d col_waves=%d row_waves=%d tiles=%d us=%lf default_us=%lf",
+                                  &t.rows, &t.col_waves, &t.row_waves, &t.tiles, &t.us, &t.default_us) == 6;
+      const NcchipCoarseTile shape{t.rows, t.col_waves, t.row_waves, t.tiles};
+      // the shape alone (no level or box): valid when it fits a one-row,
+      // one-chunk box at level 1
+      if (!ok || !ncchip_coarse_mfma_fits(1, NCC_U16, 1, 1, 0, &shape))
+        fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, op, "text", "invalid tuning line", index);
+      table[line.substr(0, sep)] = t;
+      ++index;
+    }
+    std::lock_guard<std::mutex> lock(m->tune_mutex);
+    m->coarse_tuning = std::move(table);
   });
 }
 

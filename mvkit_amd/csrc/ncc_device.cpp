@@ -9,6 +9,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -441,6 +442,9 @@ struct FindContext::Plan {
   std::vector<int32_t> frame_dims;  // top-level (w, h) per frame, for the coarse queue
   size_t off_small = 0, small = 0, off_bits = 0, n_bit_words = 0;
   std::vector<int> track_rc;        // per tracking level: ncchip_track_mfma's return (-6: vector kernel)
+  NcchipCoarseTile tile{};          // split-BF16 coarse tile shape (rows == 0: the default shape)
+  std::string tune_key;             // the model's autotuning key of this search
+  bool tune_pending = false;        // the shape is tuned on this batch before its coarse launch
 };
 
 // A captured device part of a batch, replayed while its key repeats.
@@ -486,6 +490,7 @@ struct FindContext::Slot {
   double t0 = 0, t1 = 0, t2 = 0;
   std::unique_ptr<Plan> plan;  // between the two halves of enqueue
   std::vector<int> track_rc;   // the last batch's tracking kernel choices
+  int coarse_tile = 0;         // the split-BF16 coarse tile shape that ran (pack_tile; 0: none)
   std::vector<SlotGraph> graphs;  // captured device parts, most recent first
 };
 
@@ -513,6 +518,10 @@ FindContext::~FindContext() {
     if (sl->done) (void)hipEventDestroy(sl->done);
   }
   if (uploaded_) (void)hipEventDestroy(uploaded_);
+  for (hipEvent_t e : queue_events_)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : tune_events_)
+    if (e) (void)hipEventDestroy(e);
   for (hipStream_t q : slot_streams_)
     if (q) (void)hipStreamDestroy(q);
   if (stream2_) (void)hipStreamDestroy(stream2_);
@@ -653,7 +662,11 @@ void FindContext::enqueue(Slot& S, const NccModel& model, const FrameInput* inpu
                           const FindOptions& req, FindTrace* traces) {
   if (!enqueue_front(S, model, inputs, n_frames, req, traces)) return;
   hipStream_t qs = S.plan->qs;
-  if (graphs_enabled_ && S.plan->o.graph && !S.profiling && !S.plan->reduced) {
+  if (S.plan->tune_pending) {  // the first batch of its kind: tuned, and launched kernel by kernel
+    front_launches(S);
+    tune_coarse(S);
+    enqueue_back(S);
+  } else if (graphs_enabled_ && S.plan->o.graph && !S.profiling && !S.plan->reduced) {
     launch_graph(S);
   } else {
     front_launches(S);
@@ -673,6 +686,7 @@ std::string FindContext::graph_key(Slot& S) {
   const FindOptions& o = P.o;
   put(o.start); put(o.extent); put(o.min_score); put(o.max_candidates); put(o.subpixel); put(o.highest); put(o.lowest);
   put(o.form); put(o.screen); put(o.track_mfma); put(o.query_cache); put(o.whole_level0);
+  put(P.tile);
   put(P.n_frames);
   for (const Job& J : P.jobs) { put(J.in.width); put(J.in.height); put(J.in.pitch); put(J.in.pixel_bytes); }
   return k;
@@ -882,7 +896,7 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
     }
     const int top_kind = wide || top > 4 ? NCC_U32 : (top ? NCC_U16 : NCC_U8);
     if (!ncchip_coarse_mfma_fits(int(top), top_kind, ymax - ymin + 1, (xmax - xmin + 16) / 16,
-                                 o.form == NCCHIP_FORM_FP8_SCREEN)) {
+                                 o.form == NCCHIP_FORM_FP8_SCREEN, nullptr)) {
       o.form = NCCHIP_FORM_FP32;
       use_mfma = false;
     }
@@ -1116,6 +1130,7 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
   P.off_wsum = off_wsum; P.off_common = off_common; P.off_step0 = off_step0; P.off_segs = off_segs; P.rb_full = rb_full; P.rb_counts = rb_counts;
   P.off_refs = std::move(off_refs); P.rb = rb;
   P.off_small = off_small; P.small = small; P.off_bits = off_bits; P.n_bit_words = n_bit_words;
+  if (use_mfma) choose_tile(S);
   return true;
 }
 
@@ -1203,7 +1218,124 @@ NcchipCoarseJob FindContext::coarse_job(Slot& S) {
   j.exact_lattice = !P.wide && P.top <= 2;  // exact byte lattice (matcher.cpp:347, 561)
   j.absolute = model.params.absolute;
   j.screen = o.form != NCCHIP_FORM_FP8_SCREEN ? kMissing : o.screen > kMissing ? o.screen : kDefaultScreen;
+  j.tile = P.tile;
   return j;
+}
+
+namespace {
+constexpr NcchipCoarseTile kDefaultCoarseTile = {4, 4, 1, 1};
+
+// Smallest power of two >= v (v >= 1).
+int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+}  // namespace
+
+// The coarse tile shape of a planned split-BF16 search: forced by the
+// options, the default, or the model's tuned shape for this kind of search
+// (level, pixel kind, coarse angles, union box, screen, frame count and
+// top-level size classes); a search whose kind has no tuned shape yet is
+// tuned on this batch (tune_pending) unless its top level is level 0 (the
+// caller's own image, which a later tuning pass could not rely on).
+void FindContext::choose_tile(Slot& S) {
+  Plan& P = *S.plan;
+  const FindOptions& o = P.o;
+  const bool screened = o.form == NCCHIP_FORM_FP8_SCREEN;
+  const int kind = P.jobs[0].lv[P.top].kind;
+  P.tile = kDefaultCoarseTile;
+  P.tune_pending = false;
+  if (o.coarse_tile > 0) {
+    const NcchipCoarseTile t = unpack_tile(o.coarse_tile);
+    if (!ncchip_coarse_mfma_fits(int(P.top), kind, P.dense.yspan, P.dense.xchunks, screened, &t))
+      raise(kInvalidArgument, "coarse_tile", "tile shape is invalid or does not fit this search");
+    P.tile = t;
+    return;
+  }
+  if (o.coarse_tile < 0) return;
+  char key[160];
+  std::snprintf(key, sizeof key, "L%zu K%d A%zu Y%d X%d S%d F%d W%d H%d", P.top, kind, P.A, P.dense.yspan,
+                P.dense.xchunks, int(screened), pow2_ceil(std::min(P.F, 256)), pow2_ceil(std::max(P.Wt_max, 1)),
+                pow2_ceil(std::max(P.Ht_max, 1)));
+  P.tune_key = key;
+  const NccModel& m = *P.model;
+  std::lock_guard<std::mutex> lock(m.tune_mutex);
+  const auto it = m.coarse_tuning.find(P.tune_key);
+  if (it != m.coarse_tuning.end()) {
+    const NcchipCoarseTile t{it->second.rows, it->second.col_waves, it->second.row_waves, it->second.tiles};
+    if (ncchip_coarse_mfma_fits(int(P.top), kind, P.dense.yspan, P.dense.xchunks, screened, &t)) P.tile = t;
+    return;
+  }
+  P.tune_pending = P.top > 0;
+}
+
+// The autotuner: runs this batch's coarse search with every candidate tile
+// shape (rows per wave 4 or 8; waves across x down 4x1, 2x2, 1x4, 2x1, 1x2;
+// 1, 2 or 4 angle tiles per staged image tile) on the batch's own top level,
+// after the front half's launches, and keeps the fastest.  Each shape runs
+// once to warm up and is then timed over two launches with HIP events on the
+// batch's stream.  Every shape writes the same planes and peak bits (the
+// accumulation order per output does not depend on the shape), so the
+// repeated launches leave the batch's state as one launch would.  A search
+// whose default-shape launch takes over 20 ms is timed on a band of its
+// top-level rows.  Not run inside a graph capture.
+void FindContext::tune_coarse(Slot& S) {
+  Plan& P = *S.plan;
+  hipStream_t qs = P.qs;
+  NcchipCoarseJob j = coarse_job(S);
+  const bool screened = j.screen > kMissing;
+  const int n_tiles = (j.n_angles + 31) / 32;
+  if (!tune_events_[0]) {
+    check(hipEventCreate(&tune_events_[0]), "hipEventCreate");
+    check(hipEventCreate(&tune_events_[1]), "hipEventCreate");
+  }
+  auto run = [&](const NcchipCoarseTile& t, int reps) {
+    check(hipEventRecord(tune_events_[0], qs), "hipEventRecord");
+    for (int r = 0; r < reps; ++r)
+      if (ncchip_coarse_mfma(j.frames, j.n_frames, j.level, j.level_kind, j.max_w, j.max_h, j.C, j.pools, j.patch_ids,
+                             j.n_angles, &j.weights, j.exact_lattice, j.absolute, j.screen, &t, qs))
+        raise(kInternalError, "device", "coarse launch failed");
+    check(hipEventRecord(tune_events_[1], qs), "hipEventRecord");
+    check(hipEventSynchronize(tune_events_[1]), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, tune_events_[0], tune_events_[1]), "hipEventElapsedTime");
+    return double(ms) * 1e3 / reps;
+  };
+  // the default shape first: warm-up, then its time decides the sample band
+  run(kDefaultCoarseTile, 1);
+  double def_us = run(kDefaultCoarseTile, 1);
+  if (def_us > 20000.0) {
+    const int rows = std::max(64, int(double(j.max_h) * 20000.0 / def_us) / 64 * 64);
+    j.max_h = std::min(j.max_h, rows);
+    def_us = run(kDefaultCoarseTile, 1);
+  }
+  def_us = std::min(def_us, run(kDefaultCoarseTile, 2));
+  CoarseTune best;
+  best.us = def_us;
+  best.default_us = def_us;
+  static const int kShapes[][2] = {{4, 1}, {2, 2}, {1, 4}, {2, 1}, {1, 2}};
+  for (const int rows : {4, 8})
+    for (const auto& sh : kShapes)
+      for (const int tiles : {1, 2, 4}) {
+        if (tiles > 1 && tiles / 2 >= n_tiles) continue;  // the same shape as a smaller tile count
+        const NcchipCoarseTile t{rows, sh[0], sh[1], tiles};
+        if (t.rows == kDefaultCoarseTile.rows && t.col_waves == kDefaultCoarseTile.col_waves &&
+            t.row_waves == kDefaultCoarseTile.row_waves && t.tiles == kDefaultCoarseTile.tiles)
+          continue;
+        if (!ncchip_coarse_mfma_fits(j.level, j.level_kind, j.weights.yspan, j.weights.xchunks, screened, &t)) continue;
+        run(t, 1);
+        const double us = run(t, 2);
+        if (us < best.us) {
+          best.rows = t.rows; best.col_waves = t.col_waves; best.row_waves = t.row_waves; best.tiles = t.tiles;
+          best.us = us;
+        }
+      }
+  P.tile = NcchipCoarseTile{best.rows, best.col_waves, best.row_waves, best.tiles};
+  P.tune_pending = false;
+  const NccModel& m = *P.model;
+  std::lock_guard<std::mutex> lock(m.tune_mutex);
+  m.coarse_tuning[P.tune_key] = best;
 }
 
 // The back half: coarse search (unless the caller ran it in a shared
@@ -1250,7 +1382,7 @@ void FindContext::enqueue_back(Slot& S) {
   } else if (use_mfma && !P.coarse_deferred) {
     const NcchipCoarseJob j = coarse_job(S);
     rc = ncchip_coarse_mfma(j.frames, j.n_frames, j.level, j.level_kind, j.max_w, j.max_h, j.C, j.pools, j.patch_ids,
-                            j.n_angles, &j.weights, j.exact_lattice, j.absolute, j.screen, qs);
+                            j.n_angles, &j.weights, j.exact_lattice, j.absolute, j.screen, &j.tile, qs);
   } else if (!use_mfma) {
     rc = ncchip_coarse(dframes, F, int(top), jobs[0].lv[top].kind, Wt_max, Ht_max, C, pools, coarse_refs, int(A),
                        max_bh, max_bw, o.form, exact_top, model.params.absolute, qs);
@@ -1393,6 +1525,7 @@ void FindContext::enqueue_back(Slot& S) {
   S.off_key = off_key; S.dev = dev; S.dframes = dframes; S.lists = lists; S.hfull = hfull; S.hcounts = hcounts;
   S.rb = rb; S.pools[0] = pools[0]; S.pools[1] = pools[1]; S.rp = rp; S.t0 = t0;
   S.track_rc = P.track_rc;
+  S.coarse_tile = use_mfma ? pack_tile(P.tile) : 0;
   S.plan.reset();
 }
 
@@ -1635,6 +1768,7 @@ std::vector<std::vector<Pose>> FindContext::complete_batch(uint64_t ticket) {
       tr.track_flops = J.track_flops;
       tr.track_mfma_flops = J.track_mfma_flops;
       tr.coarse_form = o.form;
+      tr.coarse_tile = S.coarse_tile;
       tr.t_coarse_us = S.t1 - S.t0; tr.t_track_us = S.t2 - S.t1; tr.t_refine_us = t3 - S.t2; tr.t_total_us = now_us() - S.t0;
       if (f > 0 && traces) std::copy(traces[0].kernel_us, traces[0].kernel_us + kStageCount, tr.kernel_us);
     }
@@ -1897,6 +2031,11 @@ std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameReque
       planned.push_back(&S);
     }
   }
+  for (Slot* S : planned)
+    if (S->plan->tune_pending) {
+      cur_ = S;
+      tune_coarse(*S);
+    }
   if (coarse_queue) {
     std::vector<NcchipCoarseJob> jobs;
     std::vector<Slot*> deferred;

@@ -88,7 +88,19 @@ struct FindOptions {
   // replay the device part as a captured HIP graph when the batch's key
   // (model, options, frame geometry, workspace) repeats
   bool graph = true;
+  // tile shape of the split-BF16 coarse kernel: 0 the model's autotuned shape
+  // for this kind of search (tuned on the first batch that needs it), -1 the
+  // default shape, else a shape packed as pack_tile() does
+  int coarse_tile = 0;
 };
+
+// NcchipCoarseTile packed into an int: rows | col_waves << 4 | row_waves << 8 | tiles << 12.
+inline int pack_tile(const NcchipCoarseTile& t) {
+  return t.rows | (t.col_waves << 4) | (t.row_waves << 8) | (t.tiles << 12);
+}
+inline NcchipCoarseTile unpack_tile(int v) {
+  return NcchipCoarseTile{v & 15, (v >> 4) & 15, (v >> 8) & 15, (v >> 12) & 15};
+}
 
 // Quota and overlap shared by all models of find_models (multi.hpp:14);
 // negative fields mean "per model".
@@ -125,6 +137,7 @@ struct FindTrace {
   double track_flops = 0;              // 2 * samples of the tracked 5x5 windows (counted when tracking is profiled)
   double track_mfma_flops = 0;         // the part of track_flops tracked on matrix cores
   int coarse_form = -1;                // correlation form the coarse search ran (NCCHIP_FORM_*)
+  int coarse_tile = 0;                 // tile shape of the split-BF16 coarse kernel (pack_tile; 0: it did not run)
 };
 
 // A u8 or u16 search image in device memory (`pitch` bytes per row), with
@@ -244,6 +257,8 @@ class FindContext {
   void launch_graph(Slot& S);
   void enqueue_back(Slot& S);
   NcchipCoarseJob coarse_job(Slot& S);
+  void choose_tile(Slot& S);
+  void tune_coarse(Slot& S);
   void collect(Slot& S);
   void refine_round(std::vector<RefineRequest>& reqs);
   void upload_image(const uint8_t* image, size_t row, size_t height, size_t pitch);
@@ -267,6 +282,7 @@ class FindContext {
   DeviceBuffer queue_dev_;     // the persistent coarse search's work queue
   PinnedBuffer queue_stage_;   // its host-side staging
   hipEvent_t queue_events_[2] = {nullptr, nullptr};  // around the persistent coarse launches
+  hipEvent_t tune_events_[2] = {nullptr, nullptr};   // around the autotuner's sample launches
 };
 
 // Polygon overlap of two poses for a template geometry (matcher.cpp:252-277).

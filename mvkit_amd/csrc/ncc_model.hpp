@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -85,6 +86,13 @@ struct CreateParams {
 
 class DeviceModel;  // device mirror (ncc_device.cpp)
 
+// A coarse tile shape chosen by the autotuner for one kind of search (the
+// fields of NcchipCoarseTile), with the measured times that chose it.
+struct CoarseTune {
+  int rows = 4, col_waves = 4, row_waves = 1, tiles = 1;
+  double us = 0, default_us = 0;  // time of the sample launch with this shape and with the default shape
+};
+
 struct NccModel {
   CreateParams params;          // step holds the resolved angle step
   size_t sample_count = 0;
@@ -97,6 +105,12 @@ struct NccModel {
   // Lazily created device copies, one per device.
   mutable std::mutex device_mutex;
   mutable std::vector<std::shared_ptr<DeviceModel>> device;
+
+  // Coarse tile shapes chosen by the autotuner, keyed by the search they
+  // were measured on (level, pixel kind, coarse angles, union box, screen,
+  // frame count and size class); shared by every find pipeline.
+  mutable std::mutex tune_mutex;
+  mutable std::map<std::string, CoarseTune> coarse_tuning;
 
   uint32_t raster(size_t level, size_t angle) const { return levels[level].first_raster + uint32_t(angle); }
   ~NccModel();

@@ -227,3 +227,62 @@ def test_extension_pixel_type_checks(capi, scene_plain):
                                                 C.byref(o16), res.ctypes.data_as(C.c_void_p), 4, C.byref(n), None,
                                                 0, None, None, None, 0, None, C.byref(st))
         assert status == -1, (w, pitch, status)
+
+
+def test_coarse_tile_shapes_fit_rule(capi):
+    """ncchip_coarse_mfma_fits: rows per wave 4 or 8, waves across and down
+    1, 2 or 4 with at most four waves, 1-8 angle tiles per image tile, levels
+    up to 4 of u8/u16 block sums, and the staged tile within LDS."""
+    lib = capi.lib()
+
+    class Tile(C.Structure):
+        _fields_ = [("rows", C.c_int32), ("col_waves", C.c_int32), ("row_waves", C.c_int32), ("tiles", C.c_int32)]
+
+    fits = lib.ncchip_coarse_mfma_fits
+    fits.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    u16, u32 = 1, 2
+
+    def ok(t, level=3, kind=u16, yspan=19, xc=2, screened=0):
+        return fits(level, kind, yspan, xc, screened, C.byref(Tile(*t)) if t else None) == 1
+
+    assert ok(None) and ok((4, 4, 1, 1))
+    for t in [(8, 4, 1, 1), (4, 2, 2, 1), (8, 1, 4, 4), (4, 2, 1, 8), (8, 1, 2, 2), (4, 1, 1, 1)]:
+        assert ok(t), t
+    for t in [(2, 4, 1, 1), (16, 4, 1, 1), (4, 3, 1, 1), (4, 4, 2, 1), (4, 2, 4, 1), (4, 4, 1, 0), (4, 4, 1, 9),
+              (4, 0, 1, 1)]:
+        assert not ok(t), t
+    assert not ok((4, 4, 1, 1), level=5) and not ok((4, 4, 1, 1), kind=u32)
+    # the staged tile: 10 bytes per element (11 screened) within LDS
+    assert ok((4, 4, 1, 1), yspan=40, xc=4) and not ok((4, 4, 1, 1), yspan=200, xc=20)
+    assert ok((4, 1, 1, 1), yspan=120, xc=4) and not ok((8, 1, 4, 1), yspan=120, xc=4)
+
+
+def test_model_tuning_text_round_trip(capi, scene_plain):
+    """mvamd_set/get_ncc_model_tuning: the text written is read back line by
+    line (keys sorted), an empty text clears the table, and a line with an
+    invalid shape or no shape is rejected without changing the table."""
+    m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
+    assert m.tuning == ""
+    lines = ["L2 K1 A1 Y17 X2 S0 F1 W64 H64 : rows=8 col_waves=2 row_waves=2 tiles=1 us=12.5 default_us=20.25",
+             "L1 K1 A3 Y33 X3 S1 F16 W128 H128 : rows=4 col_waves=1 row_waves=4 tiles=2 us=100 default_us=100"]
+    m.tuning = "\n".join(lines) + "\n"
+    got = m.tuning.splitlines()
+    assert got == sorted(lines), got
+    for bad in ["L2 K1 : rows=2 col_waves=4 row_waves=1 tiles=1 us=1 default_us=1",
+                "L2 K1 : rows=4 col_waves=4 row_waves=2 tiles=1 us=1 default_us=1",
+                "L2 K1 rows=4 col_waves=4 row_waves=1 tiles=1 us=1 default_us=1",
+                "L2 K1 : rows=4 col_waves=4"]:
+        expect_error(capi, lambda: setattr(m, "tuning", bad), -1, code=4, parameter="text")
+        assert m.tuning.splitlines() == sorted(lines)
+    m.tuning = ""
+    assert m.tuning == ""
+
+
+def test_coarse_tile_option_checks(capi, scene_plain):
+    """coarse_tile in the extension options: -1 (default), 0 (autotuned) or a
+    packed shape; anything else is rejected before any device work."""
+    m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
+    for bad in (-2, 1 << 16):
+        opts = capi.NccModel.options(0.0, 0.0, 0.8, 1, tile=bad)
+        expect_error(capi, lambda: m.find_device(0x1000, 64, 64, 64, opts), -1, parameter="coarse_tile")
+    assert capi.coarse_tile(8, 2, 2, 1) == 0x1228 and capi.tile_fields(0x1228) == (8, 2, 2, 1)
