@@ -417,6 +417,8 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args
   __shared__ double red_s[kTrackThreads / 64];
   __shared__ int red_i[kTrackThreads / 64];
   __shared__ int off[NCC_MAX_BATCH + 1];
+  __shared__ unsigned long long s_work;  // 25 * samples of the block's candidates
+  if (threadIdx.x == 0) s_work = 0;
   list_offsets(args.in_count, args.n_frames, args.cap, off);
   const int total = off[args.n_frames];
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
@@ -444,10 +446,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args
     atomicMin(&box[2], max(lo_c, col - 2 + P.cmin));
     atomicMax(&box[3], min(hi_c, col + 2 + P.cmax));
   }
-  if (a.work) {
-    samples = uint64_t(wave_sum_i64(int64_t(samples)));
-    if ((threadIdx.x & 63) == 0 && samples) atomicAdd(a.work, samples);
-  }
+  if (args.work && samples) atomicAdd(&s_work, samples);
   __syncthreads();
   const int r0 = box[0], c0 = box[2];
   const int th = box[1] - box[0] + 1, tw = box[3] - box[2] + 1;
@@ -504,6 +503,9 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track(const TrackArgs args
   }
   __syncthreads();  // the LDS of this candidate is reused by the next
   }
+  // the work counter: the block's total, added once (one contended global
+  // atomic per candidate would wait at the next barrier)
+  if (args.work && threadIdx.x == 0 && s_work) atomicAdd(args.work, s_work);
 }
 
 // ------------------------------------------------- tracking, window-major
@@ -559,6 +561,8 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs 
   __shared__ double red_s[kTrackThreads / 64];
   __shared__ int red_i[kTrackThreads / 64];
   __shared__ int off[NCC_MAX_BATCH + 1];
+  __shared__ unsigned long long s_work;  // 25 * samples of the block's candidates
+  if (threadIdx.x == 0) s_work = 0;
   __shared__ NccPatch s_pat[kWinPatches];
   __shared__ int32_t s_ref[kWinPatches];
   __shared__ int s_row[kWinPatches], s_col[kWinPatches];
@@ -602,10 +606,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs 
         atomicMax(&box[3], col + dc_hi + P.cmax);
       }
     }
-    if (a.work) {
-      samples = uint64_t(wave_sum_i64(int64_t(samples)));
-      if (lane == 0 && samples) atomicAdd(a.work, samples);
-    }
+    if (a.work && samples) atomicAdd(&s_work, samples);
     __syncthreads();
     if (threadIdx.x == 0) {
       int t = 0;
@@ -692,12 +693,14 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs 
         constexpr int kRing = 4;
         int32_t ring_rc[kRing];
         double ring_w[kRing];
+        // the ring holds the loads as loaded (a select on a loaded value
+        // waits for that load at once); past the slice the weight is zeroed
+        // where the sample is used
         auto fetch = [&](int t, int32_t& r, double& wt) __attribute__((always_inline)) {
           const int j = j0 + lane + 64 * t;
           const bool ok = j < j1;
           r = rc[ok ? j : j0];
-          const double v = w[ok ? j : j0];
-          wt = ok ? v : 0.0;
+          wt = w[ok ? j : j0];
         };
         auto accumulate = [&](const int32_t (&pix)[25], double wt) __attribute__((always_inline)) {
 #pragma unroll
@@ -710,7 +713,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs 
           for (int u = 0; u < kRing; ++u) {
             if (t0 + u >= n_it) break;
             const int32_t r = ring_rc[u];
-            const double wt = ring_w[u];
+            const double wt = j0 + lane + 64 * (t0 + u) < j1 ? ring_w[u] : 0.0;
             fetch(t0 + u + kRing, ring_rc[u], ring_w[u]);
             const int p0 = base0 + sample_row(r) * tw + sample_col(r);
             // the 25 pixels first (all reads in flight), then the products
@@ -859,6 +862,9 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs 
     }
     __syncthreads();  // the LDS of this candidate is reused by the next
   }
+  // the work counter: the block's total, added once (one contended global
+  // atomic per candidate would wait at the next barrier)
+  if (args.work && threadIdx.x == 0 && s_work) atomicAdd(args.work, s_work);
 }
 
 // ----------------------------------------------------------- MFMA tracking
@@ -947,6 +953,8 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
   __shared__ double red_u[kTrackThreads / 64];
   __shared__ int red_i[kTrackThreads / 64];
   __shared__ int off[NCC_MAX_BATCH + 1];
+  __shared__ unsigned long long s_work;  // 25 * samples of the block's candidates
+  if (threadIdx.x == 0) s_work = 0;
   uint16_t* sS = reinterpret_cast<uint16_t*>(dyn + args.mt_s);
   uint32_t* P1 = reinterpret_cast<uint32_t*>(dyn + args.mt_p1);
   P2* Q2 = reinterpret_cast<P2*>(dyn + args.mt_p2);
@@ -996,10 +1004,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       atomicMin(&box[2], col - 2 + P.cmin);
       atomicMax(&box[3], col + 2 + P.cmax);
     }
-    if (a.work) {
-      samples = uint64_t(wave_sum_i64(int64_t(samples)));
-      if (lane == 0 && samples) atomicAdd(a.work, samples);
-    }
+    if (a.work && samples) atomicAdd(&s_work, samples);
     __syncthreads();
     if (threadIdx.x == 0) {
       int t = 0;
@@ -1119,9 +1124,12 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int y_lo = (KH * wave) / 4, y_hi = (KH * (wave + 1)) / 4;
       const int n_my = (y_hi - y_lo) * XC;
       constexpr int kRing = 4;  // weight fragments in flight per lane
-      // Loads are issued unconditionally (clamped addresses, results masked),
-      // so the compiler can count the ones in flight and wait only for the
-      // oldest.  Lanes without a patch read slot 0's raster, unused.
+      // Loads are issued unconditionally and their results used as loaded, so
+      // the compiler can count the ones in flight and wait only for the
+      // oldest (a mask applied to a loaded value waits for that load at
+      // once).  A fragment outside the patch's dense raster, and every
+      // fragment of a lane without a patch (slot 0's raster), is read at
+      // offset 0 of the raster: row 0's eight zero padding columns.
       const uint16_t* bh_safe = has ? bh_base : a.pools.of(s_ref[0]).dense_hi + s_pat[0].dense_off;
       const uint16_t* bl_safe = has ? bl_base : a.pools.of(s_ref[0]).dense_lo + s_pat[0].dense_off;
       auto load_b = [&](bool ok, int Y, int cx, uint4& bh4, uint4& bl4) __attribute__((always_inline)) {
@@ -1129,11 +1137,8 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
         const bool in = ok && has && y >= 0 && y < dh && x0 >= 0 && x0 + 8 <= dw;
         const int e = y * dw + x0;
         const int o = in ? (e & 7) * plane_b + (e & ~7) : 0;
-        const uint4 h = *reinterpret_cast<const uint4*>(bh_safe + o);
-        const uint4 l = *reinterpret_cast<const uint4*>(bl_safe + o);
-        const uint32_t mk = in ? 0xffffffffu : 0u;
-        bh4 = make_uint4(h.x & mk, h.y & mk, h.z & mk, h.w & mk);
-        bl4 = make_uint4(l.x & mk, l.y & mk, l.z & mk, l.w & mk);
+        bh4 = *reinterpret_cast<const uint4*>(bh_safe + o);
+        bl4 = *reinterpret_cast<const uint4*>(bl_safe + o);
       };
       // 8 elements from any column: five aligned dwords, shifted by one
       // element (alignbyte) when the column is odd
@@ -1164,11 +1169,14 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       int Y = y_lo, cx = 0;
       uint4 a_h, a_l;
       load_a(Y, cx, a_h, a_l);
-      for (int t0 = 0; t0 < n_my; t0 += kRing) {
+      // whole groups of kRing steps (no exit inside the unrolled body, so the
+      // loads stay in flight across it); the steps past n_my multiply zero
+      // weight fragments and add nothing
+      const int n_pad = (n_my + kRing - 1) / kRing * kRing;
+      for (int t0 = 0; t0 < n_pad; t0 += kRing) {
 #pragma unroll
         for (int j = 0; j < kRing; ++j) {
           const int t = t0 + j;
-          if (t >= n_my) break;
           int Yn = Y, cxn = cx;
           step(Yn, cxn);
           uint4 n_h, n_l;
@@ -1325,6 +1333,9 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
     }
     __syncthreads();  // LDS is reused by the next candidate
   }
+  // the work counter: the block's total, added once (one contended global
+  // atomic per candidate would wait at the next barrier)
+  if (args.work && threadIdx.x == 0 && s_work) atomicAdd(args.work, s_work);
 }
 
 // ------------------------------------------------------- continuous / refine
