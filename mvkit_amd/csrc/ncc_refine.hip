@@ -34,6 +34,25 @@ __device__ __forceinline__ PatchRef patch_ref(const Pools& pools, int32_t ref) {
 }
 __device__ __forceinline__ double ref_angle(const Pools& pools, int32_t ref) { return pools.patch(ref).angle; }
 
+// A device struct in registers, its 8-byte words loaded in one round (a
+// field-by-field copy whose fields are used as they arrive waits for each
+// load before issuing the next).
+template <class T>
+__device__ __forceinline__ T load_words(const T& src) {
+  static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 8, "a struct of whole 8-byte words");
+  constexpr int kWords = sizeof(T) / 8;
+  const uint2* p = reinterpret_cast<const uint2*>(&src);
+  uint2 w[kWords];
+#pragma unroll
+  for (int k = 0; k < kWords; ++k) w[k] = p[k];
+  T v;
+  __builtin_memcpy(&v, w, sizeof v);
+  return v;
+}
+__device__ __forceinline__ NccPatch load_patch(const Pools& pools, int32_t ref) {
+  return load_words(pools.patch(ref));
+}
+
 // ------------------------------------------------------------------ tracking
 struct TrackArgs {
   const NccFrame* frames;     // per-frame descriptors (device)
@@ -67,17 +86,26 @@ struct TrackArgs {
   const NccFrame* dom;        // the frame when it has a reduced domain, else null
 };
 
-// Loads frame f into the per-block copy of the arguments.
+// Loads frame f into the per-block copy of the arguments: the level's
+// descriptor and the frame's fields in one round of loads, then the
+// reference pixel through its pointer.
 __device__ __forceinline__ void bind_frame(TrackArgs& a, int f) {
   const NccFrame& F = a.frames[f];
-  a.lv = F.levels[a.level];
-  a.dc = double(frame_dc(F));
-  a.D = int64_t(frame_dc(F)) << a.lv.shift;
-  a.crop_r = F.crop_r;
-  a.crop_c = F.crop_c;
-  if (a.level == 0) { a.sr = F.storage0[0]; a.sc = F.storage0[1]; a.sh = F.storage0[2]; a.sw = F.storage0[3]; }
-  else { a.sr = 0; a.sc = 0; a.sh = a.lv.height; a.sw = a.lv.width; }
-  a.dom = F.dom_sat ? &F : nullptr;
+  const NccLevel lv = load_words(F.levels[a.level]);
+  const void* const dc_src = F.dc_src;
+  const int32_t dc_wide = F.dc_wide;
+  const int4 st = make_int4(F.storage0[0], F.storage0[1], F.storage0[2], F.storage0[3]);
+  const double crop_r = F.crop_r, crop_c = F.crop_c;
+  const bool has_dom = F.dom_sat != nullptr;
+  const uint32_t dc = dc_wide ? uint32_t(*static_cast<const uint16_t*>(dc_src)) : uint32_t(*static_cast<const uint8_t*>(dc_src));
+  a.lv = lv;
+  a.dc = double(dc);
+  a.D = int64_t(dc) << lv.shift;
+  a.crop_r = crop_r;
+  a.crop_c = crop_c;
+  if (a.level == 0) { a.sr = st.x; a.sc = st.y; a.sh = st.z; a.sw = st.w; }
+  else { a.sr = 0; a.sc = 0; a.sh = lv.height; a.sw = lv.width; }
+  a.dom = has_dom ? &F : nullptr;
 }
 
 // Pixel source of a tracking window: an LDS tile of the candidate's
@@ -589,7 +617,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_win(const TrackArgs 
     unsigned long long samples = 0;
     for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
       const int32_t ref = a.refs[elig[e]];
-      const NccPatch P = a.pools.patch(ref);
+      const NccPatch P = load_patch(a.pools, ref);
       samples += 25ull * uint32_t(P.n);
       int row, col;
       anchor(a, cd, P, row, col);
@@ -993,7 +1021,7 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
     unsigned long long samples = 0;
     for (int e = threadIdx.x; e < n_e; e += kTrackThreads) {
       const int32_t ref = ang_cached ? s_refs[elig[e]] : a.refs[elig[e]];
-      const NccPatch P = a.pools.patch(ref);
+      const NccPatch P = load_patch(a.pools, ref);
       samples += 25ull * uint32_t(P.n);
       int row, col;
       anchor(a, cd, P, row, col);
@@ -1109,14 +1137,6 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int wsl = min(m, 24);
       const int arow = wsl / 5, acol = 8 * gg + wsl % 5;
       const bool has = m < n_e;
-      const NccPatch* Pm = has ? &s_pat[m] : nullptr;
-      const NccPatchPool& pool_m = a.pools.of(has ? s_ref[m] : 0);
-      const int dh = has ? Pm->dense_h : 0, dw = has ? Pm->dense_w : 0;
-      const int plane_b = dh * dw;
-      const uint16_t* bh_base = has ? pool_m.dense_hi + Pm->dense_off : nullptr;
-      const uint16_t* bl_base = has ? pool_m.dense_lo + Pm->dense_off : nullptr;
-      const int ybase = has ? U0 + 2 - s_row[m] - Pm->rmin : 0;
-      const int xbase = has ? V0 + 2 - s_col[m] - Pm->cmin + 8 + 8 * gg : 0;
       floatx16_t acc;
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[v] = 0.f;
@@ -1124,30 +1144,49 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int y_lo = (KH * wave) / 4, y_hi = (KH * (wave + 1)) / 4;
       const int n_my = (y_hi - y_lo) * XC;
       constexpr int kRing = 4;  // weight fragments in flight per lane
-      // Loads are issued unconditionally and their results used as loaded, so
-      // the compiler can count the ones in flight and wait only for the
-      // oldest (a mask applied to a loaded value waits for that load at
-      // once).  A fragment outside the patch's dense raster, and every
-      // fragment of a lane without a patch (slot 0's raster), is read at
-      // offset 0 of the raster: row 0's eight zero padding columns.
-      const uint16_t* bh_safe = has ? bh_base : a.pools.of(s_ref[0]).dense_hi + s_pat[0].dense_off;
-      const uint16_t* bl_safe = has ? bl_base : a.pools.of(s_ref[0]).dense_lo + s_pat[0].dense_off;
+      // B fragment of step (Y, cx): elements e .. e + 7 of the patch's dense
+      // raster, e = (Y + ybase) dw + 16 cx + xbase.  dw and 16 cx are
+      // multiples of 8, so the copy (e mod 8) is the lane's constant and the
+      // aligned offset moves by dw per row and 16 per step: the 16-byte index
+      // is qrow + 2 cx, qrow advanced by dw / 8 per row.  The fragment is
+      // inside the raster for Y in [ylo, yhi) and cx in [cxl, cxh] (lane
+      // constants; empty for a lane without a patch).  Loads are issued
+      // unconditionally and used as loaded (a mask applied to a loaded value
+      // waits for that load at once): a fragment outside the raster, and every
+      // fragment of a lane without a patch (slot 0's raster), is read at index
+      // 0, row 0's eight zero padding columns.
+      const int ms = has ? m : 0;
+      const NccPatch& Pm = s_pat[ms];
+      const NccPatchPool& pool_m = a.pools.of(s_ref[ms]);
+      const uint4* bh_q = reinterpret_cast<const uint4*>(pool_m.dense_hi + Pm.dense_off);
+      const uint4* bl_q = reinterpret_cast<const uint4*>(pool_m.dense_lo + Pm.dense_off);
+      const int dh = Pm.dense_h, dw = Pm.dense_w;
+      const int ybase = U0 + 2 - s_row[ms] - Pm.rmin;
+      const int xbase = V0 + 2 - s_col[ms] - Pm.cmin + 8 + 8 * gg;
+      const int ylo = has ? -ybase : 1, yhi = has ? dh - ybase : 0;
+      // 16 cx + xbase >= 0 and 16 cx + xbase + 8 <= dw: cx >= ceil(-xbase / 16)
+      // = -floor(xbase / 16) and cx <= floor((dw - 8 - xbase) / 16) (floor
+      // division of possibly negative numerators: an arithmetic shift)
+      const int cxl = -(xbase >> 4), cxh = (dw - 8 - xbase) >> 4;
+      const int qcopy = ((xbase & 7) * dh * dw + (xbase & ~7) + ybase * dw) >> 3;  // qrow at Y = 0
+      const int qdw = dw >> 3;
       auto load_b = [&](bool ok, int Y, int cx, uint4& bh4, uint4& bl4) __attribute__((always_inline)) {
-        const int y = Y + ybase, x0 = 16 * cx + xbase;
-        const bool in = ok && has && y >= 0 && y < dh && x0 >= 0 && x0 + 8 <= dw;
-        const int e = y * dw + x0;
-        const int o = in ? (e & 7) * plane_b + (e & ~7) : 0;
-        bh4 = *reinterpret_cast<const uint4*>(bh_safe + o);
-        bl4 = *reinterpret_cast<const uint4*>(bl_safe + o);
+        const bool in = ok && Y >= ylo && Y < yhi && cx >= cxl && cx <= cxh;
+        const int q = in ? qcopy + Y * qdw + 2 * cx : 0;
+        bh4 = bh_q[q];
+        bl4 = bl_q[q];
       };
-      // 8 elements from any column: five aligned dwords, shifted by one
-      // element (alignbyte) when the column is odd
+      // A fragment of step (Y, cx): 8 elements from tile row min(Y, KH - 1) +
+      // arow, column 16 cx + acol: five aligned dwords, shifted by one element
+      // (alignbyte) for the lanes whose column is odd (acol odd: a lane
+      // constant).  A prefetch past the band reads a valid row, unused.
+      const int sh = (acol & 1) * 2;
+      const uint32_t* ah0 = reinterpret_cast<const uint32_t*>(th0 + arow * PP + (acol & ~1));
+      const uint32_t* al0 = reinterpret_cast<const uint32_t*>(tl0 + arow * PP + (acol & ~1));
       auto load_a = [&](int Y, int cx, uint4& ah4, uint4& al4) __attribute__((always_inline)) {
-        const int col = 16 * cx + acol;
-        const int sh = (col & 1) * 2;
-        const int row = min(Y, KH - 1) + arow;  // a prefetch past the band reads a valid row, unused
-        const uint32_t* ah = reinterpret_cast<const uint32_t*>(th0 + row * PP + (col & ~1));
-        const uint32_t* al = reinterpret_cast<const uint32_t*>(tl0 + row * PP + (col & ~1));
+        const int off = (min(Y, KH - 1) * PP + 16 * cx) >> 1;  // dwords (PP and 16 cx even), wave-uniform
+        const uint32_t* ah = ah0 + off;
+        const uint32_t* al = al0 + off;
         uint32_t h[5], l[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j) { h[j] = ah[j]; l[j] = al[j]; }
