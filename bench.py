@@ -46,7 +46,7 @@ def log(msg):
 HIP_KERNELS = {("track_mfma", f): "ncc::k_track_mfma" for f in range(4)}
 HIP_KERNELS.update({("coarse", 0): "ncc::k_coarse_fp32", ("coarse", 1): "ncc::k_coarse_mfma",
                     ("coarse", 2): "ncc::k_coarse_fp64"})
-HIP_KERNELS.update({("track", f): "ncc::k_track" for f in range(4)})
+HIP_KERNELS.update({("track", f): "ncc::k_track_win" for f in range(4)})
 HIP_KERNELS.update({("refine", f): "ncc::k_refine" for f in range(4)})
 TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                        "ncc_find_rot2048_pm30_b16_hbm_traffic.md")
@@ -160,10 +160,11 @@ def batched_models(capi, torch, args):
         t0 = time.perf_counter()
         res, st = capi.find_models_batch(models, ptrs, ws, hs, ws, opts, 0, 0.5, coarse_queue=queue, stats=True)
         return res, st, time.perf_counter() - t0
-    call(True)
+    _, st0, first_s = call(True)  # the first call tunes each query's coarse tile shape
     runs = [call(True) for _ in range(3)]
     res, st, _ = runs[-1]
     dt = float(np.median([r[2] for r in runs]))
+    call(False)  # tunes the per-query launches' own kinds of search
     per_query = float(np.median([call(False)[2] for _ in range(2)]))
     hit = sum(any(int(x[4]) == k and math.hypot(x[0] - cy, x[1] - cx) < 1.0 for x in r)
               for r, tr in zip(res, w.truth) for k, cy, cx, _ in tr)
@@ -174,6 +175,10 @@ def batched_models(capi, torch, args):
            "per_query_launches_ms": round(per_query * 1e3, 1), "speedup_persistent": round(per_query / dt, 3),
            "instances": sum(len(tr) for tr in w.truth), "instances_found": hit,
            "matches": sum(len(r) for r in res), "candidates": st.candidates, "model_create_s": round(create_s, 2),
+           "first_call_s": round(first_s, 2), "first_call_tune_s": round(st0.tune_us * 1e-6, 2),
+           "first_call_tuned": st0.tuned,
+           "tuned_shapes": sorted({tuple(int(v.split("=")[1]) for v in line.split(" : ")[1].split()[:4])
+                                   for m in models for line in m.tuning.splitlines()}),
            "coarse_queue": {"kernel": "ncc::k_coarse_queue", "us": round(st.coarse_queue_us, 1),
                             "algorithmic_gflop": round(st.coarse_flops / 1e9, 1), "achieved": round(tflops, 2),
                             "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 5)},

@@ -130,8 +130,9 @@ MV_API MvStatus mvamd_find_ncc_models_device(MvTuple ModelIDs, const uint8_t* d_
  * image's result the one mvamd_find_ncc_models_device returns for it.  One
  * query per distinct (model, options) covers all images (one launch per
  * stage); coarse_queue != 0 runs the split-BF16 coarse searches of every
- * query in one persistent launch fed by an (image, angle tile, block)
- * work queue (0: one launch per query).  At most 256 images.  stats
+ * query in one persistent launch per kernel instantiation fed by an (image,
+ * angle tiles, block) work queue, the items of the queries that cost most
+ * first (0: one launch per query).  At most 256 images.  stats
  * (optional) receives the call's totals.
  * Replaces, for many device-resident images, mv_find_ncc_models
  * (mvkit include/mvkit/c/mv_ncc_matching.h:119-123) per image. */
@@ -139,10 +140,11 @@ typedef struct MvAmdBatchStats {
   double coarse_queue_us;   /* device time of the persistent coarse launches (HIP events) */
   double coarse_flops;      /* algorithmic FLOPs of every query's coarse search */
   double total_us;          /* host time of the call */
+  double tune_us;           /* host time of the call spent tuning coarse tile shapes */
   int32_t queries;          /* distinct (model, options) queries */
   int32_t queue_queries;    /* those searched in the persistent launch */
   int32_t candidates;       /* full-scored candidates over all queries and images */
-  int32_t reserved;
+  int32_t tuned;            /* queries whose tile shape this call tuned */
 } MvAmdBatchStats;
 MV_API MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t* const* d_images, const int64_t* widths,
                                             const int64_t* heights, const int64_t* pitches, size_t n_images,
@@ -235,8 +237,11 @@ MV_API MvStatus mvamd_create_ncc_model(MvObject Template, MvTuple NumLevels, MvT
 /* The model's autotuned coarse tile shapes as text, one line per kind of
  * search: "<key> : rows=R col_waves=C row_waves=W tiles=T us=U default_us=D"
  * (key: level, pixel kind, coarse angles, union box rows and 16-column
- * chunks, screen, frame count and top-level size classes; us and default_us
- * the sample launch's time with the chosen and the default shape).  text
+ * chunks, screen, then the frame count and top-level size classes of a
+ * search launched on its own, or "queue" for one run in the shared
+ * persistent launch of mvamd_find_ncc_models_batch and the server; us and
+ * default_us the sample launch's time with the chosen and the default
+ * shape).  text
  * receives at most cap bytes including the terminating zero; *length the
  * full length without it (call with cap 0 to size the buffer).  No
  * counterpart in the reference (its CPU kernels have no tile shapes). */

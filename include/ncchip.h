@@ -177,6 +177,10 @@ typedef struct NcchipCoarseJob {
   NcchipCoarseTile tile;      /* tile shape; rows == 0: the default shape */
 } NcchipCoarseJob;
 size_t ncchip_coarse_mfma_queue_bytes(int n_jobs, int n_frames_total);
+/* The queue of one launch lists the items of the jobs that cost most first
+ * (long items taken last would leave the other CUs idle); jobs share a
+ * launch when they share the kernel instantiation, the block size and the
+ * occupancy their LDS allows. */
 int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs, void* d_work, size_t work_bytes, void* h_stage,
                              size_t stage_bytes, void* stream);
 

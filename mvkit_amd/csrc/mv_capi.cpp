@@ -1330,10 +1330,11 @@ extern "C" MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t*
       stats->coarse_queue_us = bs.coarse_queue_us;
       stats->coarse_flops = bs.coarse_flops;
       stats->total_us = bs.total_us;
+      stats->tune_us = bs.tune_us;
       stats->queries = bs.queries;
       stats->queue_queries = bs.queue_queries;
       stats->candidates = bs.candidates;
-      stats->reserved = 0;
+      stats->tuned = bs.tuned;
     }
     for (size_t k = 0; k < n_images; ++k) {
       for (size_t i = 0; results && i < found[k].size() && i < cap; ++i) {

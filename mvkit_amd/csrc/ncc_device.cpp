@@ -1236,7 +1236,9 @@ int pow2_ceil(int v) {
 // The coarse tile shape of a planned split-BF16 search: forced by the
 // options, the default, or the model's tuned shape for this kind of search
 // (level, pixel kind, coarse angles, union box, screen, frame count and
-// top-level size classes); a search whose kind has no tuned shape yet is
+// top-level size classes; a search run in the shared persistent launch is
+// keyed without the frame count and size, which set only how many items it
+// adds to the queue); a search whose kind has no tuned shape yet is
 // tuned on this batch (tune_pending) unless its top level is level 0 (the
 // caller's own image, which a later tuning pass could not rely on).
 void FindContext::choose_tile(Slot& S) {
@@ -1255,9 +1257,13 @@ void FindContext::choose_tile(Slot& S) {
   }
   if (o.coarse_tile < 0) return;
   char key[160];
-  std::snprintf(key, sizeof key, "L%zu K%d A%zu Y%d X%d S%d F%d W%d H%d", P.top, kind, P.A, P.dense.yspan,
-                P.dense.xchunks, int(screened), pow2_ceil(std::min(P.F, 256)), pow2_ceil(std::max(P.Wt_max, 1)),
-                pow2_ceil(std::max(P.Ht_max, 1)));
+  if (queued_)
+    std::snprintf(key, sizeof key, "L%zu K%d A%zu Y%d X%d S%d queue", P.top, kind, P.A, P.dense.yspan,
+                  P.dense.xchunks, int(screened));
+  else
+    std::snprintf(key, sizeof key, "L%zu K%d A%zu Y%d X%d S%d F%d W%d H%d", P.top, kind, P.A, P.dense.yspan,
+                  P.dense.xchunks, int(screened), pow2_ceil(std::min(P.F, 256)), pow2_ceil(std::max(P.Wt_max, 1)),
+                  pow2_ceil(std::max(P.Ht_max, 1)));
   P.tune_key = key;
   const NccModel& m = *P.model;
   std::lock_guard<std::mutex> lock(m.tune_mutex);
@@ -1273,13 +1279,16 @@ void FindContext::choose_tile(Slot& S) {
 // The autotuner: runs this batch's coarse search with every candidate tile
 // shape (rows per wave 4 or 8; waves across x down 4x1, 2x2, 1x4, 2x1, 1x2;
 // 1, 2 or 4 angle tiles per staged image tile) on the batch's own top level,
-// after the front half's launches, and keeps the fastest.  Each shape runs
-// once to warm up and is then timed over two launches with HIP events on the
-// batch's stream.  Every shape writes the same planes and peak bits (the
+// after the front half's launches, and keeps the fastest.  Each shape is
+// timed over one launch with HIP events on the batch's stream, after one
+// untimed launch for the first shape of each kernel instantiation (its code
+// load).  Every shape writes the same planes and peak bits (the
 // accumulation order per output does not depend on the shape), so the
 // repeated launches leave the batch's state as one launch would.  A search
-// whose default-shape launch takes over 20 ms is timed on a band of its
-// top-level rows.  Not run inside a graph capture.
+// whose default-shape launch takes over 1 ms is timed on its first frames
+// and a band of their top-level rows (at least 2048 default-shape blocks),
+// so tuning costs a few ms per shape.
+// Not run inside a graph capture.
 void FindContext::tune_coarse(Slot& S) {
   Plan& P = *S.plan;
   hipStream_t qs = P.qs;
@@ -1302,18 +1311,48 @@ void FindContext::tune_coarse(Slot& S) {
     check(hipEventElapsedTime(&ms, tune_events_[0], tune_events_[1]), "hipEventElapsedTime");
     return double(ms) * 1e3 / reps;
   };
-  // the default shape first: warm-up, then its time decides the sample band
+  // the default shape first: warm-up, then its time decides the sample: the
+  // first frames of the batch, then a band of their top-level rows, keeping
+  // at least kSampleBlocks default-shape blocks (so that the sample fills
+  // the GPU as the batch does) and about kSampleUs per launch
+  constexpr double kSampleUs = 1000.0;
+  constexpr int64_t kSampleBlocks = 2048;
   run(kDefaultCoarseTile, 1);
   double def_us = run(kDefaultCoarseTile, 1);
-  if (def_us > 20000.0) {
-    const int rows = std::max(64, int(double(j.max_h) * 20000.0 / def_us) / 64 * 64);
-    j.max_h = std::min(j.max_h, rows);
-    def_us = run(kDefaultCoarseTile, 1);
+  if (def_us > kSampleUs) {
+    auto blocks = [&](int f, int rows) {  // default-shape blocks (one per angle tile) of frame f within its first rows
+      const int w = P.jobs[size_t(f)].lv[P.top].w, h = std::min(rows, P.jobs[size_t(f)].lv[P.top].h);
+      return int64_t((w + 127) / 128) * ((h + 3) / 4) * n_tiles;
+    };
+    int64_t all = 0;
+    for (int f = 0; f < j.n_frames; ++f) all += blocks(f, j.max_h);
+    int frames = 0;
+    int64_t kept = 0;
+    while (frames < j.n_frames && (kept < kSampleBlocks || def_us * double(kept) / double(all) < kSampleUs / 2)) {
+      kept += blocks(frames, j.max_h);
+      ++frames;
+    }
+    double est = def_us * double(kept) / double(std::max<int64_t>(all, 1));
+    int rows = j.max_h;
+    while (rows > 16 && est > kSampleUs) {
+      const int r2 = std::max(16, rows / 2 / 16 * 16);
+      int64_t k2 = 0;
+      for (int f = 0; f < frames; ++f) k2 += blocks(f, r2);
+      if (k2 < kSampleBlocks) break;
+      est *= double(k2) / double(std::max<int64_t>(kept, 1));
+      kept = k2;
+      rows = r2;
+    }
+    if (frames < j.n_frames || rows < j.max_h) {
+      j.n_frames = frames;
+      j.max_h = rows;
+      def_us = run(kDefaultCoarseTile, 1);
+    }
   }
-  def_us = std::min(def_us, run(kDefaultCoarseTile, 2));
   CoarseTune best;
   best.us = def_us;
   best.default_us = def_us;
+  bool warm[2][2] = {{true, false}, {false, false}};  // [rows == 8][tiles > 1]: instantiation already run
   static const int kShapes[][2] = {{4, 1}, {2, 2}, {1, 4}, {2, 1}, {1, 2}};
   for (const int rows : {4, 8})
     for (const auto& sh : kShapes)
@@ -1324,8 +1363,12 @@ void FindContext::tune_coarse(Slot& S) {
             t.row_waves == kDefaultCoarseTile.row_waves && t.tiles == kDefaultCoarseTile.tiles)
           continue;
         if (!ncchip_coarse_mfma_fits(j.level, j.level_kind, j.weights.yspan, j.weights.xchunks, screened, &t)) continue;
-        run(t, 1);
-        const double us = run(t, 2);
+        bool& w = warm[rows == 8][std::min(tiles, n_tiles) > 1];
+        if (!w) {
+          run(t, 1);
+          w = true;
+        }
+        const double us = run(t, 1);
         if (us < best.us) {
           best.rows = t.rows; best.col_waves = t.col_waves; best.row_waves = t.row_waves; best.tiles = t.tiles;
           best.us = us;
@@ -1972,6 +2015,7 @@ std::vector<std::vector<ModelMatch>> FindContext::find_models_frames(const NccMo
 std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameRequest* requests, size_t n_requests,
                                                                 bool coarse_queue, BatchStats* stats) {
   const double t_call = now_us();
+  if (stats) *stats = BatchStats{};
   std::vector<std::vector<ModelMatch>> out(n_requests);
   if (!n_requests) return out;
   // every slot of this call is reused: let earlier work on both streams end
@@ -2026,7 +2070,16 @@ std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameReque
     for (size_t r : queries[q].requests) fr.push_back(requests[r].frame);
     Slot& S = *multi_slots_[q];
     cur_ = &S;
-    if (enqueue_front(S, *queries[q].model, fr.data(), fr.size(), o, nullptr)) {
+    queued_ = coarse_queue;
+    bool planned_q = false;
+    try {
+      planned_q = enqueue_front(S, *queries[q].model, fr.data(), fr.size(), o, nullptr);
+    } catch (...) {
+      queued_ = false;
+      throw;
+    }
+    queued_ = false;
+    if (planned_q) {
       front_launches(S);
       planned.push_back(&S);
     }
@@ -2034,7 +2087,12 @@ std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameReque
   for (Slot* S : planned)
     if (S->plan->tune_pending) {
       cur_ = S;
+      const double t_tune = now_us();
       tune_coarse(*S);
+      if (stats) {
+        stats->tune_us += now_us() - t_tune;
+        ++stats->tuned;
+      }
     }
   if (coarse_queue) {
     std::vector<NcchipCoarseJob> jobs;

@@ -113,7 +113,9 @@ struct BatchStats {
   double coarse_queue_us = 0;   // device time of the persistent coarse launches (HIP events on their stream)
   double coarse_flops = 0;      // algorithmic FLOPs of every query's coarse search
   double total_us = 0;          // host time of the call
+  double tune_us = 0;           // host time spent tuning coarse tile shapes
   int queries = 0, queue_queries = 0, candidates = 0;
+  int tuned = 0;                // queries whose tile shape the call tuned
 };
 
 // Kernel stages timed in profiling mode.
@@ -204,9 +206,10 @@ class FindContext {
   // find_models over several frames at once, each frame's result the one
   // find_models returns for it: one query per distinct (model, options) over
   // all frames (one launch per stage for the frames of a query), the
-  // split-BF16 coarse searches of all queries in one persistent launch when
-  // coarse_queue is set, and the frames' greedy selections sharing their
-  // refinement rounds.
+  // split-BF16 coarse searches of all queries in one persistent launch per
+  // kernel instantiation fed by a work queue of (image, angle tiles, block)
+  // items when coarse_queue is set, and the frames' greedy selections sharing
+  // their refinement rounds.
   std::vector<std::vector<ModelMatch>> find_models_frames(const NccModel* const* models, const FindOptions* options,
                                                           size_t n_models, const FrameInput* frames, size_t n_frames,
                                                           const Selection& selection,
@@ -278,6 +281,7 @@ class FindContext {
   PinnedBuffer upload_stage_;  // strided host images gathered for one transfer
   bool dry_ = false;            // the back half runs its host bookkeeping only (graph replay)
   bool graphs_enabled_ = true;
+  bool queued_ = false;         // searches being planned run in the shared persistent coarse launch
   DeviceBuffer create_;  // model construction workspace (grow-only)
   DeviceBuffer queue_dev_;     // the persistent coarse search's work queue
   PinnedBuffer queue_stage_;   // its host-side staging

@@ -95,6 +95,10 @@ int variant(int level_kind, bool narrow, bool screened) {
 }
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// persistent launches of one queue call (kernel instantiation x block size x
+// occupancy class), each with its own item counter
+constexpr int kMaxLaunches = 256;
 }  // namespace
 
 extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int level, int level_kind, int max_w,
@@ -119,7 +123,7 @@ extern "C" int ncchip_coarse_mfma(const NccFrame* frames, int n_frames, int leve
 
 extern "C" size_t ncchip_coarse_mfma_queue_bytes(int n_jobs, int n_units) {
   return align256(size_t(n_jobs) * sizeof(MfmaArgs)) + align256(size_t(n_units) * sizeof(QueueUnit)) +
-         align256((size_t(n_units) + 64) * sizeof(int)) + 256;
+         align256((size_t(n_units) + kMaxLaunches) * sizeof(int)) + kMaxLaunches * sizeof(unsigned);
 }
 
 extern "C" int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs, void* d_work, size_t work_bytes,
@@ -128,8 +132,8 @@ extern "C" int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs,
   if (!jobs || !d_work || !h_stage) return -1;
   std::vector<MfmaArgs> args(static_cast<size_t>(n_jobs));
   std::vector<size_t> lds(static_cast<size_t>(n_jobs));
-  // the launch a job joins: kernel instantiation (variant, rows per wave) and
-  // the block's thread count
+  // the launch a job joins: kernel instantiation (variant, rows per wave),
+  // the block's thread count and the occupancy its LDS allows
   std::vector<int> key(static_cast<size_t>(n_jobs), -1);
   int n_units = 0;
   for (int j = 0; j < n_jobs; ++j) {
@@ -141,9 +145,12 @@ extern "C" int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs,
                              &J.weights, J.exact_lattice, J.absolute, J.screen, t, args[size_t(j)], lds[size_t(j)],
                              narrow);
     if (rc) return rc;
+    // the workgroups per CU its LDS allows: a job sharing a launch with one
+    // of larger LDS would run at that job's lower occupancy
+    const int occ = std::min(8, int((160 * 1024) / (lds[size_t(j)] + static_lds_of(t.rows))));
     key[size_t(j)] = variant(J.level_kind, narrow, J.screen > -__builtin_inf()) |
                      (args[size_t(j)].tiles_per_block > 1 ? 8 : 0) | (t.rows == 8 ? 16 : 0) |
-                     ((t.col_waves * t.row_waves) << 5);
+                     ((t.col_waves * t.row_waves) << 5) | (occ << 8);
     n_units += J.n_frames;
   }
   const size_t need = ncchip_coarse_mfma_queue_bytes(n_jobs, n_units);
@@ -151,22 +158,35 @@ extern "C" int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs,
   auto* base = static_cast<uint8_t*>(h_stage);
   const size_t off_units = align256(size_t(n_jobs) * sizeof(MfmaArgs));
   const size_t off_first = off_units + align256(size_t(n_units) * sizeof(QueueUnit));
-  const size_t off_counters = off_first + align256((size_t(n_units) + 64) * sizeof(int));
+  const size_t off_counters = off_first + align256((size_t(n_units) + kMaxLaunches) * sizeof(int));
   std::memcpy(base, args.data(), size_t(n_jobs) * sizeof(MfmaArgs));
   auto* units = reinterpret_cast<QueueUnit*>(base + off_units);
   auto* first = reinterpret_cast<int*>(base + off_first);
-  std::memset(base + off_counters, 0, 256);
+  std::memset(base + off_counters, 0, kMaxLaunches * sizeof(unsigned));
   // units grouped by launch key; each group's first[] ends with its item count
   struct Group { int key, unit0, n_units, first0; size_t lds; int64_t items; };
   std::vector<Group> groups;
   std::vector<int> keys;
   for (int j = 0; j < n_jobs; ++j)
     if (key[size_t(j)] >= 0 && std::find(keys.begin(), keys.end(), key[size_t(j)]) == keys.end()) keys.push_back(key[size_t(j)]);
-  if (keys.size() > 64) return -103;  // one counter per launch
+  if (keys.size() > size_t(kMaxLaunches)) return -103;  // one counter per launch
+  // within a launch, the jobs whose items cost most come first (items are
+  // taken in order: long items left for the end would idle the other CUs);
+  // an item's cost ~ its angle tiles x block area x the union box
+  std::vector<double> item_cost(static_cast<size_t>(n_jobs), 0.0);
+  for (int j = 0; j < n_jobs; ++j) {
+    if (key[size_t(j)] < 0) continue;
+    const MfmaArgs& a = args[size_t(j)];
+    const NcchipCoarseTile& t = tile_or_default(&jobs[j].tile);
+    item_cost[size_t(j)] = double(a.tiles_per_block) * (t.rows * a.rw) * (kWaveCols * a.cw) * double(a.yspan) * a.xc;
+  }
+  std::vector<int> order(static_cast<size_t>(n_jobs));
+  for (int j = 0; j < n_jobs; ++j) order[size_t(j)] = j;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return item_cost[size_t(x)] > item_cost[size_t(y)]; });
   int u = 0, fpos = 0;
   for (const int k : keys) {
     Group g{k, u, 0, fpos, 0, 0};
-    for (int j = 0; j < n_jobs; ++j) {
+    for (const int j : order) {
       const NcchipCoarseJob& J = jobs[j];
       if (key[size_t(j)] != k) continue;
       const MfmaArgs& a = args[size_t(j)];
@@ -202,7 +222,7 @@ extern "C" int ncchip_coarse_mfma_queue(const NcchipCoarseJob* jobs, int n_jobs,
     q.n_units = g.n_units;
     q.n_items = int(g.items);
     q.counter = reinterpret_cast<unsigned*>(dbase + off_counters) + k;
-    const int v = g.key & 15, threads = 64 * (g.key >> 5);
+    const int v = g.key & 15, threads = 64 * ((g.key >> 5) & 7);
     const int rc = (g.key & 16) ? coarse_queue_r8(v, threads, g.lds, st, q, n_cus)
                                 : coarse_queue_r4(v, threads, g.lds, st, q, n_cus);
     if (rc) return rc;

@@ -611,6 +611,28 @@ struct QueueArgs {
   unsigned* counter;         // next item to take; zero when the launch starts
 };
 
+// Runs one queue item: finds its unit (the last whose first item is <= item)
+// and its (angle-tile group, block row, block column).  The unit and job are
+// wave-uniform: kept in scalar registers, so the job's arguments are read
+// with scalar loads.
+template <class Tpix, class RunAcc, bool kScreen, int kR, bool kMulti>
+__device__ __forceinline__ void queue_item(const QueueArgs& q, const int item) {
+  int lo = 0, hi = q.n_units - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (__builtin_amdgcn_readfirstlane(q.first[mid]) <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  const int job = __builtin_amdgcn_readfirstlane(q.units[lo].job);
+  const int frame = __builtin_amdgcn_readfirstlane(q.units[lo].frame);
+  const int gx = __builtin_amdgcn_readfirstlane(q.units[lo].gx);
+  const int gy = __builtin_amdgcn_readfirstlane(q.units[lo].gy);
+  const int local = item - __builtin_amdgcn_readfirstlane(q.first[lo]);
+  const int per_group = gx * gy;
+  const int group = local / per_group, rem = local - group * per_group;
+  coarse_tile<Tpix, RunAcc, kScreen, kR, kMulti>(q.args[job], frame, group, rem % gx, rem / gx);
+}
+
 // Persistent coarse search: each block takes items from the queue until it
 // is drained (every block exits once the counter passes n_items).  The jobs
 // of one launch share kR and the block's thread count.
@@ -623,17 +645,7 @@ __global__ void __launch_bounds__(kMaxThreads, min_blocks<kR>()) k_coarse_queue(
     __syncthreads();
     const int item = __builtin_amdgcn_readfirstlane(s_item);
     if (item >= q.n_items) return;
-    int lo = 0, hi = q.n_units - 1;  // last unit whose first item is <= item
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (q.first[mid] <= item) lo = mid;
-      else hi = mid - 1;
-    }
-    const QueueUnit u = q.units[lo];
-    const int local = item - q.first[lo];
-    const int per_group = u.gx * u.gy;
-    const int group = local / per_group, rem = local - group * per_group;
-    coarse_tile<Tpix, RunAcc, kScreen, kR, kMulti>(q.args[u.job], u.frame, group, rem % u.gx, rem / u.gx);
+    queue_item<Tpix, RunAcc, kScreen, kR, kMulti>(q, item);
   }
 }
 
