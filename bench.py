@@ -177,7 +177,7 @@ def batched_models(capi, torch, args):
            "matches": sum(len(r) for r in res), "candidates": st.candidates, "model_create_s": round(create_s, 2),
            "first_call_s": round(first_s, 2), "first_call_tune_s": round(st0.tune_us * 1e-6, 2),
            "first_call_tuned": st0.tuned,
-           "tuned_shapes": sorted({tuple(int(v.split("=")[1]) for v in line.split(" : ")[1].split()[:4])
+           "tuned_shapes": sorted({tuple(int(v.split("=")[1]) for v in line.split(" : ")[1].split()[:5])
                                    for m in models for line in m.tuning.splitlines()}),
            "coarse_queue": {"kernel": "ncc::k_coarse_queue", "us": round(st.coarse_queue_us, 1),
                             "algorithmic_gflop": round(st.coarse_flops / 1e9, 1), "achieved": round(tflops, 2),
@@ -411,7 +411,7 @@ def main():
         for line in m.tuning.splitlines():
             key, rest = line.split(" : ")
             f = dict(kv.split("=") for kv in rest.split())
-            out.append({"key": key, "shape": [int(f[k]) for k in ("rows", "col_waves", "row_waves", "tiles")],
+            out.append({"key": key, "shape": [int(f[k]) for k in ("rows", "col_waves", "row_waves", "tiles", "bound")],
                         "us": round(float(f["us"]), 2), "default_us": round(float(f["default_us"]), 2)})
         return out
     coarse_tiles = tuned_shape(model)

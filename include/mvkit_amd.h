@@ -52,6 +52,11 @@ typedef struct MvAmdFindOptions {
  * output rows (4 or 8) x 32 anchors against 32 rotations; a block has
  * col_waves x row_waves waves (each 1, 2 or 4, product <= 4) and runs `tiles`
  * angle tiles of 32 rotations (1..8) over one image tile staged in LDS.
+ * MVAMD_COARSE_TILE_BOUND adds the bound group: 0 bounds each window's score
+ * by the support common to every rotation of the model, 8, 16 or 32 by the
+ * support common to its group of that many consecutive rotations (a tighter
+ * bound computed per group, so fewer windows need exact moments; not with
+ * MVAMD_FORM_FP8_SCREEN).
  * Every shape gives bit-identical planes.  With MVAMD_COARSE_TILE_AUTO the
  * first find of a kind of search (level, pixel type, coarse angles, template
  * extent, screen, frame count and size class) times the candidate shapes on
@@ -62,6 +67,9 @@ typedef struct MvAmdFindOptions {
 #define MVAMD_COARSE_TILE_DEFAULT (-1)
 #define MVAMD_COARSE_TILE(rows, col_waves, row_waves, tiles) \
   ((rows) | ((col_waves) << 4) | ((row_waves) << 8) | ((tiles) << 12))
+#define MVAMD_COARSE_TILE_BOUND(rows, col_waves, row_waves, tiles, bound) \
+  (MVAMD_COARSE_TILE(rows, col_waves, row_waves, tiles) |                  \
+   (((bound) == 8 ? 1 : (bound) == 16 ? 2 : (bound) == 32 ? 3 : 0) << 16))
 
 /* Pixel types of device-resident search images ("byte" and "uint2" in the
  * reference's image objects, c_api/ncc_matching_api.cpp:167-168). */

@@ -107,6 +107,16 @@ typedef struct NcchipDenseWeights {
   const int32_t* common_runs;
   int32_t n_common_runs, n_common;
   double prune_below;
+  /* Bound groups (NcchipCoarseTile.bound): for group sizes 8, 16 and 32
+   * (index i = 0, 1, 2) the angles a / size form groups, each with the runs
+   * of the support common to its own patches: group g of size index i has
+   * runs group_first[first_base[i] + g] .. group_first[first_base[i] + g + 1]
+   * - 1 of group_runs (4 ints each, as common_runs) and group_n[n_base[i] + g]
+   * samples.  Null group_first: no bound groups. */
+  const int32_t* group_first;
+  const int32_t* group_n;
+  const int32_t* group_runs;
+  int32_t first_base[3], n_base[3];
   /* > -inf: sparse planes -- only scores >= peak_thr are written, each with
    * its bit in the frame's peak_bits (zeroed by the caller); the peaks
    * kernel then runs with hwa = 2.  -inf: every plane entry is written. */
@@ -129,11 +139,15 @@ typedef struct NcchipDenseWeights {
 /* Tile shape of the split-BF16 coarse kernel: each wave computes `rows`
  * output rows (4 or 8) x 32 anchors; a block has col_waves waves across and
  * row_waves waves down (col_waves * row_waves <= 4; each 1, 2 or 4) and runs
- * `tiles` angle tiles of 32 rotations (1..8) over one staged image tile.  The
- * planes are bit-identical for every shape; only the time differs.  The
- * default shape is {4, 4, 1, 1}. */
+ * `tiles` angle tiles of 32 rotations (1..8) over one staged image tile.
+ * `bound`: 0 bounds every window's score with the support common to every
+ * coarse patch; 8, 16 or 32 with the support common to the patches of its
+ * group of that many consecutive angles (a tighter bound, computed per group:
+ * fewer windows need exact moments; not with the FP8 screen).  The planes are
+ * bit-identical for every shape; only the time differs.  The default shape is
+ * {4, 4, 1, 1, 0}. */
 typedef struct NcchipCoarseTile {
-  int32_t rows, col_waves, row_waves, tiles;
+  int32_t rows, col_waves, row_waves, tiles, bound;
 } NcchipCoarseTile;
 
 /* 1 when ncchip_coarse_mfma can run the top level `level` of pixel kind

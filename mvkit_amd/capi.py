@@ -98,14 +98,22 @@ COARSE_TILE_AUTO = 0
 COARSE_TILE_DEFAULT = -1
 
 
-def coarse_tile(rows: int, col_waves: int, row_waves: int, tiles: int) -> int:
-    """MVAMD_COARSE_TILE: a split-BF16 coarse tile shape packed for the options."""
-    return rows | (col_waves << 4) | (row_waves << 8) | (tiles << 12)
+def coarse_tile(rows: int, col_waves: int, row_waves: int, tiles: int, bound: int = 0) -> int:
+    """MVAMD_COARSE_TILE_BOUND: a split-BF16 coarse tile shape packed for the
+    options (bound: 0, 8, 16 or 32 angles per bound group)."""
+    b = {0: 0, 8: 1, 16: 2, 32: 3}[bound]
+    return rows | (col_waves << 4) | (row_waves << 8) | (tiles << 12) | (b << 16)
 
 
 def tile_fields(v: int) -> tuple:
     """(rows, col_waves, row_waves, tiles) of a packed shape (0: none)."""
     return (v & 15, (v >> 4) & 15, (v >> 8) & 15, (v >> 12) & 15)
+
+
+def tile_bound(v: int) -> int:
+    """Angles per bound group of a packed shape (0: the common support of every angle)."""
+    b = (v >> 16) & 3
+    return 4 << b if b else 0
 STAGES = ["pyramid", "prefix", "coarse", "peaks", "track", "full_score", "refine", "track_mfma"]
 
 

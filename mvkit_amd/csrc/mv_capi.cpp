@@ -419,7 +419,7 @@ mvamd::FindOptions from_ext(const MvAmdFindOptions* o) {
          "candidate budget must be 0 (the default) or 1..2^24");
   if (o->max_candidates > 0) q.max_candidates = size_t(o->max_candidates);
   q.graph = o->no_graph == 0;
-  if (o->coarse_tile < MVAMD_COARSE_TILE_DEFAULT || o->coarse_tile >= (1 << 16))
+  if (o->coarse_tile < MVAMD_COARSE_TILE_DEFAULT || o->coarse_tile >= (1 << 18))
     fail(mvamd::kInvalidArgument, mvamd::kErrInvalidValue, "find_ncc_model", "coarse_tile", "invalid coarse tile shape");
   q.coarse_tile = o->coarse_tile;
   if (q.form < MVAMD_FORM_AUTO || q.form > MVAMD_FORM_FP8_SCREEN)
@@ -1583,8 +1583,9 @@ extern "C" MvStatus mvamd_get_ncc_model_tuning(MvTuple ModelID, char* text, size
       std::lock_guard<std::mutex> lock(m->tune_mutex);
       char line[320];
       for (const auto& [key, t] : m->coarse_tuning) {
-        std::snprintf(line, sizeof line, "%s : rows=%d col_waves=%d row_waves=%d tiles=%d us=%.17g default_us=%.17g\n",
-                      key.c_str(), t.rows, t.col_waves, t.row_waves, t.tiles, t.us, t.default_us);
+        std::snprintf(line, sizeof line,
+                      "%s : rows=%d col_waves=%d row_waves=%d tiles=%d bound=%d us=%.17g default_us=%.17g\n",
+                      key.c_str(), t.rows, t.col_waves, t.row_waves, t.tiles, t.bound, t.us, t.default_us);
         out += line;
       }
     }
@@ -1614,10 +1615,15 @@ extern "C" MvStatus mvamd_set_ncc_model_tuning(MvTuple ModelID, const char* text
       if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
       const size_t sep = line.find(" : ");
       mvamd::CoarseTune t;
+      // "bound=" may be absent (bound 0)
+      const char* f = sep != std::string::npos ? line.c_str() + sep + 3 : "";
       const bool ok = sep != std::string::npos && sep > 0 &&
-                      std::sscanf(line.c_str() + sep + 3, "rows=%d col_waves=%d row_waves=%d tiles=%d us=%lf default_us=%lf",
-                                  &t.rows, &t.col_waves, &t.row_waves, &t.tiles, &t.us, &t.default_us) == 6;
-      const NcchipCoarseTile shape{t.rows, t.col_waves, t.row_waves, t.tiles};
+                      (std::sscanf(f, "rows=%d col_waves=%d row_waves=%d tiles=%d bound=%d us=%lf default_us=%lf",
+                                   &t.rows, &t.col_waves, &t.row_waves, &t.tiles, &t.bound, &t.us, &t.default_us) == 7 ||
+                       (t.bound = 0, std::sscanf(f, "rows=%d col_waves=%d row_waves=%d tiles=%d us=%lf default_us=%lf",
+                                                 &t.rows, &t.col_waves, &t.row_waves, &t.tiles, &t.us,
+                                                 &t.default_us) == 6));
+      const NcchipCoarseTile shape{t.rows, t.col_waves, t.row_waves, t.tiles, t.bound};
       // the shape alone (no level or box): valid when it fits a one-row,
       // one-chunk box at level 1
       if (!ok || !ncchip_coarse_mfma_fits(1, NCC_U16, 1, 1, 0, &shape))

@@ -184,6 +184,10 @@ struct DenseWeights {
   std::vector<double> wsum_eff;      // per angle
   std::vector<int32_t> common_runs;  // 4 ints per run of the common support
   int n_common = 0;
+  // bound groups (see NcchipDenseWeights): one array of group first runs,
+  // sample counts and runs, sizes 8, 16, 32 in turn
+  std::vector<int32_t> groups;
+  int32_t first_base[3] = {0, 0, 0}, n_base[3] = {0, 0, 0}, n_off = 0, runs_off = 0;
   int ymin = 0, xmin = 0, yspan = 0, xchunks = 0, max_run = 0;
   double w8_scale = 1;  // power of two with max |w_hi| * w8_scale <= 128
 };
@@ -274,6 +278,51 @@ DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std:
       d.common_runs.insert(d.common_runs.end(), {y + d.ymin, b + d.xmin, x - 1 + d.xmin, 0});
       d.n_common += x - b;
     }
+  // supports common to the patches of each bound group of 8, 16 and 32
+  // consecutive angles (groups never straddle an angle tile), within the
+  // rows and columns the group's patches span
+  std::vector<int32_t> first, count, runs;
+  for (int i = 0; i < 3; ++i) {
+    const size_t gs = size_t(8) << i;
+    d.first_base[i] = int32_t(first.size());
+    d.n_base[i] = int32_t(count.size());
+    for (size_t a0 = 0; a0 < patches.size(); a0 += gs) {
+      const size_t a1 = std::min(patches.size(), a0 + gs);
+      int gy0 = d.yspan, gy1 = -1, gx0 = xspan, gx1 = -1;
+      for (size_t a = a0; a < a1; ++a) {
+        gy0 = std::min(gy0, patches[a]->rmin - d.ymin); gy1 = std::max(gy1, patches[a]->rmax - d.ymin);
+        gx0 = std::min(gx0, patches[a]->cmin - d.xmin); gx1 = std::max(gx1, patches[a]->cmax - d.xmin);
+      }
+      for (int y = gy0; y <= gy1; ++y) std::fill(&hits[size_t(y) * xspan + gx0], &hits[size_t(y) * xspan + gx1] + 1, 0);
+      for (size_t a = a0; a < a1; ++a) {
+        const Raster& p = *patches[a];
+        const RasterPool& pool = *pools[a];
+        for (uint32_t k = 0; k < p.count; ++k) {
+          const int32_t rc = pool.rc[p.first + k];
+          ++hits[size_t((rc >> 16) - d.ymin) * xspan + (int(int16_t(rc & 0xffff)) - d.xmin)];
+        }
+      }
+      first.push_back(int32_t(runs.size() / 4));
+      int n = 0;
+      for (int y = gy0; y <= gy1; ++y)
+        for (int x = gx0; x <= gx1;) {
+          if (hits[size_t(y) * xspan + x] != int(a1 - a0)) { ++x; continue; }
+          const int b = x;
+          while (x <= gx1 && hits[size_t(y) * xspan + x] == int(a1 - a0)) ++x;
+          runs.insert(runs.end(), {y + d.ymin, b + d.xmin, x - 1 + d.xmin, 0});
+          n += x - b;
+        }
+      count.push_back(n);
+    }
+    first.push_back(int32_t(runs.size() / 4));
+  }
+  d.n_off = int32_t(first.size());
+  d.runs_off = d.n_off + int32_t(count.size());
+  d.runs_off = (d.runs_off + 3) & ~3;  // 16-byte aligned runs
+  d.groups.assign(size_t(d.runs_off) + runs.size(), 0);
+  std::copy(first.begin(), first.end(), d.groups.begin());
+  std::copy(count.begin(), count.end(), d.groups.begin() + d.n_off);
+  std::copy(runs.begin(), runs.end(), d.groups.begin() + d.runs_off);
   return d;
 }
 
@@ -434,7 +483,7 @@ struct FindContext::Plan {
   unsigned long long* extent = nullptr;
   unsigned long long* work = nullptr;
   NccPatchPool pools[2] = {};
-  size_t off_key = 0, off_raw = 0, off_table = 0, off_wh = 0, off_wl = 0, off_wsum = 0, off_common = 0;
+  size_t off_key = 0, off_raw = 0, off_table = 0, off_wh = 0, off_wl = 0, off_wsum = 0, off_common = 0, off_groups = 0;
   size_t off_step0 = 0, off_segs = 0;
   size_t rb_full = 0, rb_counts = 0;
   std::vector<size_t> off_refs;
@@ -1028,6 +1077,7 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
   const size_t off_segs = off_small + small; small = align_up(small + dense.segs.size() * 4);
   const size_t off_wsum = off_small + small; small = align_up(small + dense.wsum_eff.size() * 8);
   const size_t off_common = off_small + small; small = align_up(small + dense.common_runs.size() * 4);
+  const size_t off_groups = off_small + small; small = align_up(small + dense.groups.size() * 4);
   const size_t off_frames = off_small + small; small = align_up(small + n_frames * sizeof(NccFrame));
   off = off_small + small;
   if (S.work.bytes < off) {
@@ -1068,6 +1118,7 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
       std::memcpy(st + (off_segs - off_small), dense.segs.data(), dense.segs.size() * 4);
       std::memcpy(st + (off_wsum - off_small), dense.wsum_eff.data(), dense.wsum_eff.size() * 8);
       std::memcpy(st + (off_common - off_small), dense.common_runs.data(), dense.common_runs.size() * 4);
+      std::memcpy(st + (off_groups - off_small), dense.groups.data(), dense.groups.size() * 4);
     }
     auto* ext = reinterpret_cast<double*>(st + (off_extent - off_small));
     auto* fr = reinterpret_cast<NccFrame*>(st + (off_frames - off_small));
@@ -1127,7 +1178,7 @@ bool FindContext::enqueue_front(Slot& S, const NccModel& model, const FrameInput
   P.dev = dev; P.dframes = dframes; P.lists = lists; P.counts = counts; P.extent = extent; P.work = work;
   P.pools[0] = pools[0]; P.pools[1] = pools[1];
   P.off_key = off_key; P.off_raw = off_raw; P.off_table = off_table; P.off_wh = off_wh; P.off_wl = off_wl;
-  P.off_wsum = off_wsum; P.off_common = off_common; P.off_step0 = off_step0; P.off_segs = off_segs; P.rb_full = rb_full; P.rb_counts = rb_counts;
+  P.off_wsum = off_wsum; P.off_common = off_common; P.off_groups = off_groups; P.off_step0 = off_step0; P.off_segs = off_segs; P.rb_full = rb_full; P.rb_counts = rb_counts;
   P.off_refs = std::move(off_refs); P.rb = rb;
   P.off_small = off_small; P.small = small; P.off_bits = off_bits; P.n_bit_words = n_bit_words;
   if (use_mfma) choose_tile(S);
@@ -1212,6 +1263,11 @@ NcchipCoarseJob FindContext::coarse_job(Slot& S) {
   dw.common_runs = reinterpret_cast<const int32_t*>(dev + P.off_common);
   dw.n_common_runs = int32_t(dense.common_runs.size() / 4);
   dw.n_common = dense.n_common;
+  const int32_t* groups = reinterpret_cast<const int32_t*>(dev + P.off_groups);
+  dw.group_first = groups;
+  dw.group_n = groups + dense.n_off;
+  dw.group_runs = groups + dense.runs_off;
+  for (int i = 0; i < 3; ++i) { dw.first_base[i] = dense.first_base[i]; dw.n_base[i] = dense.n_base[i]; }
   dw.prune_below = 0.9 * (o.min_score * .4);
   dw.peak_thr = o.min_score * .4;  // sparse planes: the peaks kernel reads the flagged scores only
   dw.w8_scale = dense.w8_scale;
@@ -1223,7 +1279,7 @@ NcchipCoarseJob FindContext::coarse_job(Slot& S) {
 }
 
 namespace {
-constexpr NcchipCoarseTile kDefaultCoarseTile = {4, 4, 1, 1};
+constexpr NcchipCoarseTile kDefaultCoarseTile = {4, 4, 1, 1, 0};
 
 // Smallest power of two >= v (v >= 1).
 int pow2_ceil(int v) {
@@ -1269,22 +1325,26 @@ void FindContext::choose_tile(Slot& S) {
   std::lock_guard<std::mutex> lock(m.tune_mutex);
   const auto it = m.coarse_tuning.find(P.tune_key);
   if (it != m.coarse_tuning.end()) {
-    const NcchipCoarseTile t{it->second.rows, it->second.col_waves, it->second.row_waves, it->second.tiles};
+    const NcchipCoarseTile t{it->second.rows, it->second.col_waves, it->second.row_waves, it->second.tiles,
+                             it->second.bound};
     if (ncchip_coarse_mfma_fits(int(P.top), kind, P.dense.yspan, P.dense.xchunks, screened, &t)) P.tile = t;
     return;
   }
   P.tune_pending = P.top > 0;
 }
 
-// The autotuner: runs this batch's coarse search with every candidate tile
-// shape (rows per wave 4 or 8; waves across x down 4x1, 2x2, 1x4, 2x1, 1x2;
-// 1, 2 or 4 angle tiles per staged image tile) on the batch's own top level,
-// after the front half's launches, and keeps the fastest.  Each shape is
+// The autotuner: runs this batch's coarse search with candidate tile shapes
+// on the batch's own top level, after the front half's launches, and keeps
+// the fastest: the bound groups (32, 16 or 8 angles) on the default shape,
+// then every shape (rows per wave 4 or 8; waves across x down 4x1, 2x2, 1x4,
+// 2x1, 1x2; 1, 2 or 4 angle tiles per staged image tile) with the best
+// bound, then the other bounds on the best shape.  Each shape is
 // timed over one launch with HIP events on the batch's stream, after one
 // untimed launch for the first shape of each kernel instantiation (its code
-// load).  Every shape writes the same planes and peak bits (the
-// accumulation order per output does not depend on the shape), so the
-// repeated launches leave the batch's state as one launch would.  A search
+// load).  Every shape writes the same sparse planes and peak bits (the
+// accumulation order per output does not depend on the shape, and a bound
+// only drops windows below the peak threshold), so the repeated launches
+// leave the batch's state as one launch would.  A search
 // whose default-shape launch takes over 1 ms is timed on its first frames
 // and a band of their top-level rows (at least 2048 default-shape blocks),
 // so tuning costs a few ms per shape.
@@ -1352,29 +1412,45 @@ void FindContext::tune_coarse(Slot& S) {
   CoarseTune best;
   best.us = def_us;
   best.default_us = def_us;
+  NcchipCoarseTile best_t = kDefaultCoarseTile;
+  auto same = [](const NcchipCoarseTile& a, const NcchipCoarseTile& b) {
+    return a.rows == b.rows && a.col_waves == b.col_waves && a.row_waves == b.row_waves && a.tiles == b.tiles &&
+           a.bound == b.bound;
+  };
   bool warm[2][2] = {{true, false}, {false, false}};  // [rows == 8][tiles > 1]: instantiation already run
+  auto consider = [&](const NcchipCoarseTile& t) {
+    if (same(t, best_t) || same(t, kDefaultCoarseTile)) return;
+    if (!ncchip_coarse_mfma_fits(j.level, j.level_kind, j.weights.yspan, j.weights.xchunks, screened, &t)) return;
+    bool& w = warm[t.rows == 8][std::min(t.tiles, n_tiles) > 1];
+    if (!w) {
+      run(t, 1);
+      w = true;
+    }
+    const double us = run(t, 1);
+    if (us < best.us) {
+      best_t = t;
+      best.us = us;
+    }
+  };
+  static const int kBounds[] = {32, 16, 8};
   static const int kShapes[][2] = {{4, 1}, {2, 2}, {1, 4}, {2, 1}, {1, 2}};
+  // the bound group on the default shape, then the shapes with the best
+  // bound, then the other bounds on the best shape
+  if (!screened)
+    for (const int b : kBounds) consider(NcchipCoarseTile{4, 4, 1, 1, b});
+  const int bound = best_t.bound;
   for (const int rows : {4, 8})
     for (const auto& sh : kShapes)
       for (const int tiles : {1, 2, 4}) {
         if (tiles > 1 && tiles / 2 >= n_tiles) continue;  // the same shape as a smaller tile count
-        const NcchipCoarseTile t{rows, sh[0], sh[1], tiles};
-        if (t.rows == kDefaultCoarseTile.rows && t.col_waves == kDefaultCoarseTile.col_waves &&
-            t.row_waves == kDefaultCoarseTile.row_waves && t.tiles == kDefaultCoarseTile.tiles)
-          continue;
-        if (!ncchip_coarse_mfma_fits(j.level, j.level_kind, j.weights.yspan, j.weights.xchunks, screened, &t)) continue;
-        bool& w = warm[rows == 8][std::min(tiles, n_tiles) > 1];
-        if (!w) {
-          run(t, 1);
-          w = true;
-        }
-        const double us = run(t, 1);
-        if (us < best.us) {
-          best.rows = t.rows; best.col_waves = t.col_waves; best.row_waves = t.row_waves; best.tiles = t.tiles;
-          best.us = us;
-        }
+        consider(NcchipCoarseTile{rows, sh[0], sh[1], tiles, bound});
       }
-  P.tile = NcchipCoarseTile{best.rows, best.col_waves, best.row_waves, best.tiles};
+  if (!screened && (best_t.rows != 4 || best_t.col_waves != 4 || best_t.row_waves != 1 || best_t.tiles != 1))
+    for (const int b : {0, 32, 16, 8})
+      if (b != bound) consider(NcchipCoarseTile{best_t.rows, best_t.col_waves, best_t.row_waves, best_t.tiles, b});
+  best.rows = best_t.rows; best.col_waves = best_t.col_waves; best.row_waves = best_t.row_waves;
+  best.tiles = best_t.tiles; best.bound = best_t.bound;
+  P.tile = best_t;
   P.tune_pending = false;
   const NccModel& m = *P.model;
   std::lock_guard<std::mutex> lock(m.tune_mutex);

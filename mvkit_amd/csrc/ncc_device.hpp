@@ -94,12 +94,15 @@ struct FindOptions {
   int coarse_tile = 0;
 };
 
-// NcchipCoarseTile packed into an int: rows | col_waves << 4 | row_waves << 8 | tiles << 12.
+// NcchipCoarseTile packed into an int: rows | col_waves << 4 | row_waves << 8 |
+// tiles << 12 | b << 16 with b = 0, 1, 2, 3 for bound 0, 8, 16, 32.
 inline int pack_tile(const NcchipCoarseTile& t) {
-  return t.rows | (t.col_waves << 4) | (t.row_waves << 8) | (t.tiles << 12);
+  const int b = t.bound == 8 ? 1 : t.bound == 16 ? 2 : t.bound == 32 ? 3 : 0;
+  return t.rows | (t.col_waves << 4) | (t.row_waves << 8) | (t.tiles << 12) | (b << 16);
 }
 inline NcchipCoarseTile unpack_tile(int v) {
-  return NcchipCoarseTile{v & 15, (v >> 4) & 15, (v >> 8) & 15, (v >> 12) & 15};
+  const int b = (v >> 16) & 3;
+  return NcchipCoarseTile{v & 15, (v >> 4) & 15, (v >> 8) & 15, (v >> 12) & 15, b ? 4 << b : 0};
 }
 
 // Quota and overlap shared by all models of find_models (multi.hpp:14);

@@ -14,12 +14,13 @@
 using namespace ncc;
 
 namespace {
-constexpr NcchipCoarseTile kDefaultTile = {4, 4, 1, 1};
+constexpr NcchipCoarseTile kDefaultTile = {4, 4, 1, 1, 0};
 
 bool valid_tile(const NcchipCoarseTile& t) {
   auto pow2_4 = [](int v) { return v == 1 || v == 2 || v == 4; };
   return (t.rows == 4 || t.rows == 8) && pow2_4(t.col_waves) && pow2_4(t.row_waves) &&
-         t.col_waves * t.row_waves <= kMaxWaves && t.tiles >= 1 && t.tiles <= 8;
+         t.col_waves * t.row_waves <= kMaxWaves && t.tiles >= 1 && t.tiles <= 8 &&
+         (t.bound == 0 || t.bound == 8 || t.bound == 16 || t.bound == 32);
 }
 
 size_t static_lds_of(int rows) { return rows == 8 ? static_lds<8>() : static_lds<4>(); }
@@ -30,6 +31,7 @@ size_t coarse_mfma_lds(int level, int level_kind, int yspan, int xchunks, bool s
                        int* pitch, int* pitch8) {
   if (level_kind == NCC_U32 || level > 4) return 0;  // T_lo exact in BF16 only up to level 4
   if (!valid_tile(t)) return 0;
+  if (screened && t.bound) return 0;  // the screen decides on the bound of every patch
   const int cols = kWaveCols * t.col_waves + 16 * xchunks + 9;
   const int rows = t.rows * t.row_waves + yspan - 1;
   // the staging's float-reciprocal index needs rows * cols < 2^16; the run
@@ -39,6 +41,8 @@ size_t coarse_mfma_lds(int level, int level_kind, int yspan, int xchunks, bool s
   *pitch8 = (cols + 11) & ~3;  // an 8-byte A fragment read as three dwords from any column
   size_t lds = size_t(5) * rows * *pitch * sizeof(uint16_t);
   if (screened) lds += size_t(rows) * *pitch8;
+  // bound groups: each group's deviations per wave, row and anchor
+  if (t.bound) lds += size_t(kWaveCols / t.bound) * t.col_waves * t.row_waves * t.rows * kWaveCols * sizeof(float);
   const size_t kLdsMax = 160 * 1024 - static_lds_of(t.rows);
   return lds > kLdsMax ? 0 : lds;
 }
@@ -71,6 +75,14 @@ int make_args(const NccFrame* frames, int n_frames, int level, int level_kind, d
   a.exact_lattice = exact_lattice; a.absolute = absolute;
   a.common_runs = reinterpret_cast<const int4*>(dw->common_runs); a.n_common_runs = dw->n_common_runs;
   a.n_common = dw->n_common; a.prune_below = dw->prune_below; a.peak_thr = dw->peak_thr;
+  if (t.bound) {
+    if (!dw->group_first || !dw->group_n || !dw->group_runs) return -1;
+    const int i = t.bound == 8 ? 0 : t.bound == 16 ? 1 : 2;
+    a.group_slots = t.bound;
+    a.group_first = dw->group_first + dw->first_base[i];
+    a.group_n = dw->group_n + dw->n_base[i];
+    a.group_runs = reinterpret_cast<const int4*>(dw->group_runs);
+  }
   lds = coarse_mfma_lds(level, level_kind, a.yspan, a.xc, screened, t, &a.pitch, &a.pitch8);
   if (!lds) return -6;
   a.cw = t.col_waves; a.rw = t.row_waves;

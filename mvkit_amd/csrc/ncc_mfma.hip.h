@@ -81,6 +81,14 @@ struct MfmaArgs {
   int exact_lattice, absolute;
   const int4* common_runs;   // support shared by every coarse patch
   int n_common_runs, n_common;
+  // bound groups (group_slots > 0): consecutive angles a / group_slots share
+  // the support common to their own patches, runs group_first[g] ..
+  // group_first[g + 1] - 1 of group_runs with group_n[g] samples (g counted
+  // from this size's first group); 0: one group over every angle
+  int group_slots;
+  const int* group_first;
+  const int* group_n;
+  const int4* group_runs;
   double prune_below;        // score bound under which windows are written -inf
   double peak_thr;           // sparse planes: scores below are not written (-inf: dense planes)
   // FP8 screen
@@ -239,17 +247,30 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
   //   |score| <= (|dot| + 128 * 4^l * |wsum|) * inv_sd_common.
   // Lane l: wave row (l >> 4) + 4 j, anchors 2 (l & 15) .. + 1, slid along
   // the shared runs.  Computed once per block (it depends on no angle tile).
+  // With bound groups the support is the one common to the patches of the
+  // group (larger than the support common to every patch, so a tighter
+  // bound), and each angle tile computes every group's deviations into an
+  // area of dynamic LDS after the image planes: [group][wave][row][anchor].
+  // The deviations are private to the wave: written and read by its own
+  // lanes, whose LDS operations complete in order.
   __shared__ float s_inv_sd_f[kMaxWaves][kR][kWaveCols];
-  const bool prune = args.prune_below > 0 && args.n_common > 0;
-  auto common_sd = [&]() __attribute__((always_inline)) {
+  const bool grouped = args.group_slots > 0;
+  const bool prune = args.prune_below > 0 && (grouped || args.n_common > 0);
+  float* const gsd = reinterpret_cast<float*>(su + rows * P);  // no FP8 plane with bound groups
+  const int n_waves = args.cw * args.rw;
+  auto isd_row = [&](const int sub, const int r) __attribute__((always_inline)) -> float* {
+    return grouped ? gsd + ((sub * n_waves + wave) * kR + r) * kWaveCols : &s_inv_sd_f[wave][r][0];
+  };
+  auto common_sd = [&](const int k0, const int k1, const int n_common, const int4* runs, const int sub)
+                       __attribute__((always_inline)) {
     if (prune) {
       const int ca = wc * kWaveCols + 2 * (lane & 15);
 #pragma unroll
       for (int rq = 0; rq < kR / 4; ++rq) {
         const int rr = (lane >> 4) + 4 * rq;
         int64_t m1[2] = {0, 0}, m2[2] = {0, 0};
-        for (int k = 0; k < args.n_common_runs; ++k) {
-          const int4 run = common_cached ? s_common[k] : args.common_runs[k];
+        for (int k = k0; k < k1; ++k) {
+          const int4 run = runs ? runs[k] : s_common[k];
           const uint16_t* src = su + (wr0 + rr + run.x - args.ymin) * P + (ca - args.xmin);
           int32_t s1 = 0;
           RunAcc s2 = 0;
@@ -282,8 +303,8 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           // n * sum(t^2) - sum(t)^2 = n^2 * variance, exact in 128 bits
-          const __int128 vn = (__int128)args.n_common * m2[j] - (__int128)m1[j] * m1[j];
-          s_inv_sd_f[wave][rr][2 * (lane & 15) + j] = vn > 0 ? float(sqrt(double(args.n_common) / double(vn))) : -1.f;
+          const __int128 vn = (__int128)n_common * m2[j] - (__int128)m1[j] * m1[j];
+          isd_row(sub, rr)[2 * (lane & 15) + j] = vn > 0 ? float(sqrt(double(n_common) / double(vn))) : -1.f;
         }
       }
     }
@@ -339,7 +360,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     if constexpr (kScreen) {
       // the screen needs the common-support deviations first
       store_runs();
-      if (tile == tile_lo) common_sd();
+      if (tile == tile_lo) common_sd(0, args.n_common_runs, args.n_common, common_cached ? nullptr : args.common_runs, 0);
       __syncthreads();
       floatx16 acc8[kR];
 #pragma unroll
@@ -437,27 +458,47 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     }
     if constexpr (!kScreen) {
       store_runs();
-      if (tile == tile_lo) common_sd();
+      if (!grouped && tile == tile_lo)
+        common_sd(0, args.n_common_runs, args.n_common, common_cached ? nullptr : args.common_runs, 0);
       __syncthreads();
+    }
+    if (grouped) {  // every bound group's deviations, before the bound pass reads them
+      const int n_sub = kWaveCols / args.group_slots;
+      for (int gi = 0; gi < n_sub && tile * kWaveCols + gi * args.group_slots < args.n_angles; ++gi) {
+        const int grp = (tile * kWaveCols) / args.group_slots + gi;
+        common_sd(args.group_first[grp], args.group_first[grp + 1], args.group_n[grp], args.group_runs, gi);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // part 2.  Lane (slot m = lane & 31, half g = lane >> 5) holds the anchors
     // 8 * (v / 4) + 4 * g + (v % 4) of its wave's 32; one exchange with the
     // partner half gives it the 16 consecutive anchors 16 g .. 16 g + 15.
     // Windows outside the valid range or under the bound are written -inf; the
     // others are queued per wave (at most 64) and scored lane-parallel: exact
-    // moments over the patch's runs and the FP64 normalisation.
+    // moments over the patch's runs and the FP64 normalisation.  A flush of q
+    // windows gives each the largest power of two of lanes with q * lanes <=
+    // 64; a window's lanes take its runs in turn and their integer sums are
+    // added across them, so sparse survivors do not leave one lane walking
+    // every run of its patch while the others wait.
     int queued = 0;  // wave-uniform
     auto flush = [&]() __attribute__((always_inline)) {
       __builtin_amdgcn_wave_barrier();
-      if (lane < queued) {
-        const int code = s_code[wave][lane];
-        const float dv = s_dot[wave][lane];
+      const int lps_log = 31 - __builtin_clz(64 / queued);  // 1 <= queued <= 64
+      const int lps = 1 << lps_log;
+      const int qi = lane >> lps_log, sub = lane & (lps - 1);
+      const bool act = qi < queued;
+      int code = 0;
+      float dv = 0.f;
+      int64_t m1 = 0, m2 = 0;
+      if (act) {
+        code = s_code[wave][qi];
+        dv = s_dot[wave][qi];
         const int r = code >> 16, col = (code >> 8) & 0xff, slot = code & 0xff;
-        const int aa = tile * 32 + slot;
         const int run0 = s_run0[slot], n_runs = s_run0[slot + 1] - run0;
         const int4* rq = reinterpret_cast<const int4*>(args.pools.of(s_slot_ref[slot]).runs) + s_slot_roff[slot];
-        int64_t m1 = 0, m2 = 0;
-        for (int k = 0; k < n_runs; ++k) {
+        for (int k = sub; k < n_runs; k += lps) {
           int4 run;
           if (runs_cached) {
             const int2 c = s_runs[run0 + k];
@@ -487,6 +528,14 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
           m1 += s1;
           m2 += int64_t(s2);
         }
+      }
+      for (int o = 1; o < lps; o <<= 1) {  // wave-uniform: every lane takes part
+        m1 += __shfl_xor(m1, o, 64);
+        m2 += __shfl_xor(m2, o, 64);
+      }
+      if (act && sub == 0) {
+        const int r = code >> 16, col = (code >> 8) & 0xff, slot = code & 0xff;
+        const int aa = tile * 32 + slot;
         const int64_t nq = s_slot_n[slot];
         const int64_t sum_s = m1 + nq * Cl;
         const int64_t sum_s2 = m2 + 2 * Cl * m1 + nq * Cl * Cl;
@@ -506,6 +555,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     const int a = tile * 32 + m;
     const bool active = a < args.n_angles;
     const uint64_t below = (uint64_t(1) << lane) - 1;
+    const int my_sub = grouped ? m / args.group_slots : 0;  // this lane's slot's bound group in the tile
     const int cb = c0 + wc * kWaveCols + cw16;
     const uint32_t in_width = bits_below(W - cb);
     if (!run_bf16) {  // screened out: nothing to score
@@ -528,7 +578,7 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
     // the bound sits a tenth under the peak threshold, far beyond FP32's
     // rounding), one bit per window (wave row r, anchor j: bit 16 r + j); the
     // kept windows are then queued one per lane per pass, so the scoring code
-    // appears once.  Rows are taken four at a time (64 bits of kept windows).
+    // appears once (64 bits of kept windows per four rows).
     const float prune_f = float(args.prune_below), slack_f = float(num_slack);
     // this lane's 16 anchors c_j = cb + j: bits of those inside the frame
     // (in_width) and inside the angle's valid column range (in_range)
@@ -553,11 +603,10 @@ __device__ __forceinline__ void coarse_tile(const MfmaArgs& args, const int f, c
         uint32_t pass = 0xffffu;
         if (prune) {
           pass = 0;
+          const float* isd = isd_row(my_sub, r) + cw16;
 #pragma unroll
-          for (int j = 0; j < 16; ++j) {
-            const float isd = s_inv_sd_f[wave][r][cw16 + j];
-            pass |= uint32_t((isd < 0.f) | ((fabsf(dot[rl][j]) + slack_f) * isd >= prune_f)) << j;
-          }
+          for (int j = 0; j < 16; ++j)
+            pass |= uint32_t((isd[j] < 0.f) | ((fabsf(dot[rl][j]) + slack_f) * isd[j] >= prune_f)) << j;
         }
         const uint32_t kr = (row_in ? in_range : 0u) & pass;
         keep |= uint64_t(kr) << (16 * rl);

@@ -89,7 +89,7 @@ class DeviceModel;  // device mirror (ncc_device.cpp)
 // A coarse tile shape chosen by the autotuner for one kind of search (the
 // fields of NcchipCoarseTile), with the measured times that chose it.
 struct CoarseTune {
-  int rows = 4, col_waves = 4, row_waves = 1, tiles = 1;
+  int rows = 4, col_waves = 4, row_waves = 1, tiles = 1, bound = 0;
   double us = 0, default_us = 0;  // time of the sample launch with this shape and with the default shape
 };
 

@@ -231,12 +231,14 @@ def test_extension_pixel_type_checks(capi, scene_plain):
 
 def test_coarse_tile_shapes_fit_rule(capi):
     """ncchip_coarse_mfma_fits: rows per wave 4 or 8, waves across and down
-    1, 2 or 4 with at most four waves, 1-8 angle tiles per image tile, levels
-    up to 4 of u8/u16 block sums, and the staged tile within LDS."""
+    1, 2 or 4 with at most four waves, 1-8 angle tiles per image tile, bound
+    groups of 0, 8, 16 or 32 angles (0 only with the FP8 screen), levels up
+    to 4 of u8/u16 block sums, and the staged tile within LDS."""
     lib = capi.lib()
 
     class Tile(C.Structure):
-        _fields_ = [("rows", C.c_int32), ("col_waves", C.c_int32), ("row_waves", C.c_int32), ("tiles", C.c_int32)]
+        _fields_ = [("rows", C.c_int32), ("col_waves", C.c_int32), ("row_waves", C.c_int32), ("tiles", C.c_int32),
+                    ("bound", C.c_int32)]
 
     fits = lib.ncchip_coarse_mfma_fits
     fits.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
@@ -255,6 +257,11 @@ def test_coarse_tile_shapes_fit_rule(capi):
     # the staged tile: 10 bytes per element (11 screened) within LDS
     assert ok((4, 4, 1, 1), yspan=40, xc=4) and not ok((4, 4, 1, 1), yspan=200, xc=20)
     assert ok((4, 1, 1, 1), yspan=120, xc=4) and not ok((8, 1, 4, 1), yspan=120, xc=4)
+    for b in (8, 16, 32):
+        assert ok((4, 4, 1, 1, b)) and ok((8, 1, 4, 2, b)) and not ok((4, 4, 1, 1, b), screened=1)
+    for b in (4, 12, 64, -8):
+        assert not ok((4, 4, 1, 1, b)), b
+    assert ok((4, 4, 1, 1, 0), screened=1)
 
 
 def test_model_tuning_text_round_trip(capi, scene_plain):
@@ -263,13 +270,18 @@ def test_model_tuning_text_round_trip(capi, scene_plain):
     invalid shape or no shape is rejected without changing the table."""
     m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
     assert m.tuning == ""
-    lines = ["L2 K1 A1 Y17 X2 S0 F1 W64 H64 : rows=8 col_waves=2 row_waves=2 tiles=1 us=12.5 default_us=20.25",
-             "L1 K1 A3 Y33 X3 S1 F16 W128 H128 : rows=4 col_waves=1 row_waves=4 tiles=2 us=100 default_us=100"]
+    lines = ["L2 K1 A1 Y17 X2 S0 F1 W64 H64 : rows=8 col_waves=2 row_waves=2 tiles=1 bound=16 us=12.5 default_us=20.25",
+             "L1 K1 A3 Y33 X3 S1 F16 W128 H128 : rows=4 col_waves=1 row_waves=4 tiles=2 bound=0 us=100 default_us=100"]
     m.tuning = "\n".join(lines) + "\n"
     got = m.tuning.splitlines()
     assert got == sorted(lines), got
+    # a line without "bound=" reads as bound 0
+    m.tuning = "L3 queue : rows=4 col_waves=2 row_waves=2 tiles=1 us=3 default_us=4\n"
+    assert m.tuning == "L3 queue : rows=4 col_waves=2 row_waves=2 tiles=1 bound=0 us=3 default_us=4\n"
+    m.tuning = "\n".join(lines) + "\n"
     for bad in ["L2 K1 : rows=2 col_waves=4 row_waves=1 tiles=1 us=1 default_us=1",
                 "L2 K1 : rows=4 col_waves=4 row_waves=2 tiles=1 us=1 default_us=1",
+                "L2 K1 : rows=4 col_waves=4 row_waves=1 tiles=1 bound=12 us=1 default_us=1",
                 "L2 K1 rows=4 col_waves=4 row_waves=1 tiles=1 us=1 default_us=1",
                 "L2 K1 : rows=4 col_waves=4"]:
         expect_error(capi, lambda: setattr(m, "tuning", bad), -1, code=4, parameter="text")
@@ -282,7 +294,9 @@ def test_coarse_tile_option_checks(capi, scene_plain):
     """coarse_tile in the extension options: -1 (default), 0 (autotuned) or a
     packed shape; anything else is rejected before any device work."""
     m = capi.NccModel(scene_plain.template, 3, 0.0, 0.0)
-    for bad in (-2, 1 << 16):
+    for bad in (-2, 1 << 18):
         opts = capi.NccModel.options(0.0, 0.0, 0.8, 1, tile=bad)
         expect_error(capi, lambda: m.find_device(0x1000, 64, 64, 64, opts), -1, parameter="coarse_tile")
     assert capi.coarse_tile(8, 2, 2, 1) == 0x1228 and capi.tile_fields(0x1228) == (8, 2, 2, 1)
+    assert capi.coarse_tile(8, 2, 2, 1, 32) == 0x31228 and capi.tile_bound(0x31228) == 32
+    assert [capi.tile_bound(capi.coarse_tile(4, 4, 1, 1, b)) for b in (0, 8, 16, 32)] == [0, 8, 16, 32]

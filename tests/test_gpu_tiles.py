@@ -1,9 +1,10 @@
 """Tile shapes of the split-BF16 coarse kernel and the autotuner that picks
 them.  Every shape (rows per wave, waves across and down a block, angle
-tiles per staged image tile) accumulates each window over the same K steps
-in the same order, so the coarse peaks, the candidates and the matches are
-those of the default shape exactly, for the split-BF16 and the FP8-screened
-forms and for the batched persistent launch.  The autotuner times the shapes
+tiles per staged image tile, bound groups) accumulates each window over the
+same K steps in the same order, and a tighter bound only drops windows that
+can be neither peaks nor suppressors, so the coarse peaks, the candidates
+and the matches are those of the default shape exactly, for the split-BF16
+and the FP8-screened forms and for the batched persistent launch.  The autotuner times the shapes
 on the first find of a kind of search, keeps the fastest in the model, and
 later finds use it (or a shape restored from text)."""
 import math
@@ -16,9 +17,14 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-ONE_TILE = [(4, 4, 1, 1), (8, 4, 1, 1), (4, 2, 2, 1), (8, 2, 2, 1), (4, 1, 4, 1), (8, 1, 4, 1), (4, 2, 1, 1),
-            (8, 2, 1, 1), (4, 1, 2, 1), (8, 1, 2, 1), (4, 1, 1, 1)]
-MULTI_TILE = [(4, 4, 1, 2), (4, 4, 1, 4), (8, 2, 2, 2), (8, 1, 4, 4), (4, 2, 1, 8), (8, 4, 1, 3)]
+ONE_TILE = [(4, 4, 1, 1, 0), (8, 4, 1, 1, 0), (4, 2, 2, 1, 0), (8, 2, 2, 1, 0), (4, 1, 4, 1, 0), (8, 1, 4, 1, 0),
+            (4, 2, 1, 1, 0), (8, 2, 1, 1, 0), (4, 1, 2, 1, 0), (8, 1, 2, 1, 0), (4, 1, 1, 1, 0),
+            (4, 4, 1, 1, 8), (4, 4, 1, 1, 16), (4, 4, 1, 1, 32), (8, 2, 2, 1, 16), (4, 1, 4, 1, 8)]
+MULTI_TILE = [(4, 4, 1, 2, 0), (4, 4, 1, 4, 0), (8, 2, 2, 2, 0), (8, 1, 4, 4, 0), (4, 2, 1, 8, 0), (8, 4, 1, 3, 0)]
+# bound groups on the full turn (179 angles: the last angle tile holds 19,
+# so its last group of 8 is partial and a fourth is empty)
+MULTI_BOUND = [(4, 4, 1, 1, 8), (4, 4, 1, 2, 8), (8, 2, 2, 4, 16), (4, 1, 4, 8, 32), (4, 2, 2, 1, 32),
+               (8, 1, 2, 1, 16)]
 START, EXTENT = -math.pi / 6, math.pi / 3
 
 
@@ -43,6 +49,10 @@ def trace(capi, model, image, start, extent, form, tile, min_score=0.7, matches=
     return t
 
 
+def fields(capi, v):
+    return capi.tile_fields(v) + (capi.tile_bound(v),)
+
+
 def same_trace(a, b):
     assert np.array_equal(a["coarse"], b["coarse"])
     assert np.array_equal(a["prepared"], b["prepared"])
@@ -58,7 +68,7 @@ def test_shape_reproduces_the_default_planes(capi, scenes, shape):
     want = trace(capi, model, rot.image, START, EXTENT, 1, capi.COARSE_TILE_DEFAULT)
     got = trace(capi, model, rot.image, START, EXTENT, 1, capi.coarse_tile(*shape))
     assert len(want["results"]) == 5
-    assert capi.tile_fields(got["stats"].coarse_tile) == shape
+    assert fields(capi, got["stats"].coarse_tile) == shape
     same_trace(got, want)
 
 
@@ -67,15 +77,16 @@ def test_multi_tile_shapes_reproduce_the_default_planes(capi, scenes, form):
     """Full-turn 40 px template, 2 levels: 179 coarse angles in 6 angle
     tiles, run 1..8 at a time over one staged image tile; split-BF16 and
     FP8-screened forms (the screen's skipped waves join every later tile's
-    barriers)."""
+    barriers); bound groups with the split-BF16 form."""
     _, full = scenes
     model = capi.NccModel(full.template, 2, 0.0, 2 * math.pi)
     want = trace(capi, model, full.image, 0.0, 2 * math.pi, form, capi.COARSE_TILE_DEFAULT, matches=2)
     assert len(want["results"]) == 2
-    for shape in MULTI_TILE + [(8, 2, 2, 1), (4, 1, 2, 1)]:
+    shapes = MULTI_TILE + [(8, 2, 2, 1, 0), (4, 1, 2, 1, 0)] + (MULTI_BOUND if form == 1 else [])
+    for shape in shapes:
         say(f"full-turn form {form} shape {shape}")
         got = trace(capi, model, full.image, 0.0, 2 * math.pi, form, capi.coarse_tile(*shape), matches=2)
-        assert capi.tile_fields(got["stats"].coarse_tile) == shape
+        assert fields(capi, got["stats"].coarse_tile) == shape
         same_trace(got, want)
 
 
@@ -94,20 +105,21 @@ def test_autotuner_picks_keeps_and_restores_shapes(capi, scenes):
     lines = model.tuning.splitlines()
     assert len(lines) == 1, lines
     key, rest = lines[0].split(" : ")
-    fields = dict(kv.split("=") for kv in rest.split())
-    shape = tuple(int(fields[k]) for k in ("rows", "col_waves", "row_waves", "tiles"))
-    assert capi.tile_fields(st.coarse_tile) == shape
-    assert float(fields["us"]) <= float(fields["default_us"])
-    say(f"autotune chose {shape} for {key}: {fields['us']} us against {fields['default_us']} us")
+    f = dict(kv.split("=") for kv in rest.split())
+    shape = tuple(int(f[k]) for k in ("rows", "col_waves", "row_waves", "tiles", "bound"))
+    assert fields(capi, st.coarse_tile) == shape
+    assert float(f["us"]) <= float(f["default_us"])
+    say(f"autotune chose {shape} for {key}: {f['us']} us against {f['default_us']} us")
     for _ in range(2):  # later finds (graph replays) keep the shape
         got, st = model.find_device(d.data_ptr(), 2048, 2048, 2048, opt(capi.COARSE_TILE_AUTO))
-        assert np.array_equal(got, want) and capi.tile_fields(st.coarse_tile) == shape
+        assert np.array_equal(got, want) and fields(capi, st.coarse_tile) == shape
     assert model.tuning.splitlines() == lines
     # a restored table decides the shape without tuning
-    other = (8, 1, 4, 1) if shape != (8, 1, 4, 1) else (4, 2, 2, 1)
-    model.tuning = f"{key} : rows={other[0]} col_waves={other[1]} row_waves={other[2]} tiles={other[3]} us=1 default_us=2\n"
+    other = (8, 1, 4, 1, 16) if shape != (8, 1, 4, 1, 16) else (4, 2, 2, 1, 8)
+    model.tuning = (f"{key} : rows={other[0]} col_waves={other[1]} row_waves={other[2]} tiles={other[3]} "
+                    f"bound={other[4]} us=1 default_us=2\n")
     got, st = model.find_device(d.data_ptr(), 2048, 2048, 2048, opt(capi.COARSE_TILE_AUTO))
-    assert np.array_equal(got, want) and capi.tile_fields(st.coarse_tile) == other
+    assert np.array_equal(got, want) and fields(capi, st.coarse_tile) == other
     assert model.tuning.splitlines()[0].startswith(key)
     # another kind of search (one frame of a different size) is tuned apart
     small = np.ascontiguousarray(rot.image[:1024, :1536])
@@ -144,10 +156,12 @@ def test_batched_queue_with_tuned_shapes(capi, scenes):
     for g, w in zip(got, want):
         assert np.array_equal(g, w)
     assert all(m.tuning for m in models)
-    for shapes in ([(8, 2, 2, 1), (4, 1, 2, 4), (4, 4, 1, 2)], [(4, 2, 2, 1), (8, 4, 1, 1), (8, 1, 4, 2)]):
+    for shapes in ([(8, 2, 2, 1, 0), (4, 1, 2, 4, 8), (4, 4, 1, 2, 0)], [(4, 2, 2, 1, 32), (8, 4, 1, 1, 16),
+                                                                        (8, 1, 4, 2, 8)]):
         for m, sh in zip(models, shapes):
             key = m.tuning.splitlines()[0].split(" : ")[0]
-            m.tuning = f"{key} : rows={sh[0]} col_waves={sh[1]} row_waves={sh[2]} tiles={sh[3]} us=1 default_us=1\n"
+            m.tuning = (f"{key} : rows={sh[0]} col_waves={sh[1]} row_waves={sh[2]} tiles={sh[3]} bound={sh[4]} "
+                        "us=1 default_us=1\n")
         say(f"batched: shapes {shapes}")
         got = capi.find_models_batch(models, ptrs, ws, hs, ws, opts(capi.COARSE_TILE_AUTO))
         for g, w in zip(got, want):
