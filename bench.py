@@ -176,7 +176,7 @@ def batched_models(capi, torch, args):
            "instances": sum(len(tr) for tr in w.truth), "instances_found": hit,
            "matches": sum(len(r) for r in res), "candidates": st.candidates, "model_create_s": round(create_s, 2),
            "first_call_s": round(first_s, 2), "first_call_tune_s": round(st0.tune_us * 1e-6, 2),
-           "first_call_tuned": st0.tuned,
+           "first_call_tuned": st0.tuned, "refined": st.refined, "refine_rounds": st.refine_rounds,
            "tuned_shapes": sorted({tuple(int(v.split("=")[1]) for v in line.split(" : ")[1].split()[:5])
                                    for m in models for line in m.tuning.splitlines()}),
            "coarse_queue": {"kernel": "ncc::k_coarse_queue", "us": round(st.coarse_queue_us, 1),

@@ -153,6 +153,8 @@ typedef struct MvAmdBatchStats {
   int32_t queue_queries;    /* those searched in the persistent launch */
   int32_t candidates;       /* full-scored candidates over all queries and images */
   int32_t tuned;            /* queries whose tile shape this call tuned */
+  int32_t refined;          /* candidates refined (the walks' speculative rounds included) */
+  int32_t refine_rounds;    /* refinement rounds (one launch and one synchronisation each) */
 } MvAmdBatchStats;
 MV_API MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t* const* d_images, const int64_t* widths,
                                             const int64_t* heights, const int64_t* pitches, size_t n_images,

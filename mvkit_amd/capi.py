@@ -90,7 +90,8 @@ class RequestTimes(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("coarse_queue_us", C.c_double), ("coarse_flops", C.c_double), ("total_us", C.c_double),
                 ("tune_us", C.c_double), ("queries", C.c_int32), ("queue_queries", C.c_int32),
-                ("candidates", C.c_int32), ("tuned", C.c_int32)]
+                ("candidates", C.c_int32), ("tuned", C.c_int32), ("refined", C.c_int32),
+                ("refine_rounds", C.c_int32)]
 
 
 PROFILE_ALL = 0xff

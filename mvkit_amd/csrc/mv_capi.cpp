@@ -1335,6 +1335,8 @@ extern "C" MvStatus mvamd_find_ncc_models_batch(MvTuple ModelIDs, const uint8_t*
       stats->queue_queries = bs.queue_queries;
       stats->candidates = bs.candidates;
       stats->tuned = bs.tuned;
+      stats->refined = bs.refined;
+      stats->refine_rounds = bs.refine_rounds;
     }
     for (size_t k = 0; k < n_images; ++k) {
       for (size_t i = 0; results && i < found[k].size() && i < cap; ++i) {

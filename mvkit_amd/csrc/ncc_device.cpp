@@ -326,6 +326,20 @@ DenseWeights dense_weights(const std::vector<const Raster*>& patches, const std:
   return d;
 }
 
+// Speculative refinement rounds of walks with no match quota: when the
+// walks of one call hold at most kOneRound candidates together, each refines
+// its whole list in one round; otherwise each starts with
+// kUnboundedFirstRound of its next candidates, each later round twice as
+// many.  Candidates whose unrefined pose an accepted match suppresses are
+// never refined (select_refined checks them first), so bounded rounds let
+// the first accepted matches suppress part of a long list before it is
+// refined, which pays when the refinement is throughput-bound; a round costs
+// at least one candidate's whole Gauss-Newton latency, which dominates
+// smaller calls.  Which candidates the walks accept does not depend on the
+// round sizes.
+constexpr size_t kOneRound = 65536;
+constexpr size_t kUnboundedFirstRound = 1024;
+
 bool pose_less(const Pose& a, const Pose& b) {  // merge_identical order (matcher.cpp:242-244)
   if (a.row != b.row) return a.row < b.row;
   if (a.col != b.col) return a.col < b.col;
@@ -1729,6 +1743,10 @@ void FindContext::collect(Slot& S) {
     J.verdict.assign(J.full.size(), -1);  // -1: not refined
     J.batch_size = o.num_matches ? std::max<size_t>(2 * o.num_matches, 4) : J.full.size();
   }
+  size_t total = 0;
+  for (const Job& J : jobs) total += J.full.size();
+  if (!o.num_matches && total > kOneRound)
+    for (Job& J : jobs) J.batch_size = std::min(J.batch_size, kUnboundedFirstRound);
 }
 
 // One refinement round: every request is refined by its slot's query in one
@@ -2001,6 +2019,7 @@ struct SelectionWalk {
   size_t batch = 0, pos = 0;
   bool stop = false;
 
+  bool unbounded = false;  // no match quota: the round sizes follow the call's candidate count
   void init() {
     counts.assign(n_models, 0);
     batch = s.num_matches > 0 ? std::max<size_t>(2 * size_t(s.num_matches), 4) : 0;
@@ -2008,6 +2027,7 @@ struct SelectionWalk {
       size_t total = 0;
       bool bounded = true;
       for (size_t i = 0; i < n_models; ++i) { if (!opts[i].num_matches) bounded = false; total += opts[i].num_matches; }
+      unbounded = !bounded;
       batch = bounded ? std::max<size_t>(2 * total, 4) : ord.size();
     }
   }
@@ -2272,6 +2292,13 @@ std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameReque
     std::sort(w.ord.begin(), w.ord.end(), [&](size_t a, size_t b) { return multi_less(cand[r][a], cand[r][b]); });
     w.init();
   }
+  {
+    size_t total = 0;
+    for (size_t r = 0; r < n_requests; ++r) total += walks[r].ord.size();
+    if (total > kOneRound)
+      for (size_t r = 0; r < n_requests; ++r)
+        if (walks[r].unbounded) walks[r].batch = std::min(walks[r].batch, kUnboundedFirstRound);
+  }
   // ---- the walks of all requests, with shared speculative refinement rounds
   auto v = [&](const SelectionWalk::Ref& x) { return verdict[x.slot][x.frame][x.index]; };
   auto pose = [&](const SelectionWalk::Ref& x) { return refined[x.slot][x.frame][x.index]; };
@@ -2292,6 +2319,10 @@ std::vector<std::vector<ModelMatch>> FindContext::find_requests(const FrameReque
     }
     if (reqs.empty()) break;
     refine_round(reqs);
+    if (stats) {
+      stats->refined += int(reqs.size());
+      ++stats->refine_rounds;
+    }
     for (size_t k = 0; k < reqs.size(); ++k) {
       const SelectionWalk::Ref& x = targets[k];
       refined[x.slot][x.frame][x.index] = reqs[k].out;

@@ -119,6 +119,7 @@ struct BatchStats {
   double tune_us = 0;           // host time spent tuning coarse tile shapes
   int queries = 0, queue_queries = 0, candidates = 0;
   int tuned = 0;                // queries whose tile shape the call tuned
+  int refined = 0, refine_rounds = 0;  // candidates refined, in this many speculative rounds
 };
 
 // Kernel stages timed in profiling mode.
