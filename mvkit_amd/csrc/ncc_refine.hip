@@ -1410,6 +1410,9 @@ struct RefineArgs {
   // LDS copy of the candidate's neighbourhood (storage-local rectangle)
   const void* tile;
   int tr0, tc0, th, tw;
+  // LDS room after the tile for a sampling pass's rotation tables, in
+  // 16-byte entries (0: none)
+  int rtab_off, rtab_cap;
 };
 
 __device__ __forceinline__ void bind_frame(RefineArgs& a, int f) {
@@ -1465,11 +1468,14 @@ template <class T>
 __device__ __forceinline__ void stage_tile(RefineArgs& a, void* lds, double pr, double pc, double reach) {
   a.tile = lds;
   a.th = 0; a.tw = 0; a.tr0 = 0; a.tc0 = 0;
+  a.rtab_off = 0; a.rtab_cap = kRefineTileBytes / 16;  // no tile: the whole area holds the tables
   if (sizeof(T) > 2 || !isfinite(pr) || !isfinite(pc)) return;
   const int r0 = max(0, int(floor(pr - reach))), r1 = min(a.sh - 1, int(ceil(pr + reach)));
   const int c0 = max(0, int(floor(pc - reach))), c1 = min(a.sw - 1, int(ceil(pc + reach)));
   const int h = r1 - r0 + 1, w = c1 - c0 + 1;
   if (h <= 0 || w <= 0 || h * w * int(sizeof(T)) > kRefineTileBytes) return;
+  a.rtab_off = (h * w * int(sizeof(T)) + 15) & ~15;
+  a.rtab_cap = (kRefineTileBytes - a.rtab_off) / 16;
   const T* src = static_cast<const T*>(a.lv.data);
   constexpr int U = 8;
   for (int base = threadIdx.x; base < h * w; base += blockDim.x * U) {
@@ -1575,6 +1581,96 @@ __device__ __forceinline__ bool bilinear_pass(const RefineArgs& a, const PatchRe
   for (int k = 0; k < 18; ++k) v[k] = 0;
   int bad = 0;
   constexpr int U = 8;  // samples per thread whose template loads are issued together
+  // Fast form of the loop below, for a pass at level 0 whose samples are all
+  // strictly inside: the products ca*dr, sa*dr (per template row) and sa*dc,
+  // ca*dc (per column) come from tables built in LDS for the pass, and the
+  // four bilinear taps are read without bounds checks, from the LDS tile when
+  // the pass's footprint lies in it, else from the level.  Every FP64
+  // operation is the loop's own, in the same order (S - dc is an exact
+  // integer), so the moments are bit-identical to the loop's.
+  const int rows = P.rmax - P.rmin + 1, cols = P.cmax - P.cmin + 1;
+  int mode = 0;  // 0: the loop below; 1: fast, taps from the LDS tile; 2: fast, taps from the level
+  if (interior && a.lv.shift == 0 && rows + cols <= a.rtab_cap) {
+    const double dr0 = P.rmin - a.lv_orow, dr1 = P.rmax - a.lv_orow;
+    const double dc0 = P.cmin - a.lv_ocol, dc1 = P.cmax - a.lv_ocol;
+    double ylo = __builtin_inf(), xlo = ylo, yhi = -ylo, xhi = -ylo;
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        const double dr = i ? dr1 : dr0, dc = j ? dc1 : dc0;
+        const double y = pr + ca * dr - sa * dc, x = pc + sa * dr + ca * dc;
+        ylo = fmin(ylo, y); yhi = fmax(yhi, y); xlo = fmin(xlo, x); xhi = fmax(xhi, x);
+      }
+    // a pixel of margin each side for the rounding of single samples
+    const bool in_tile = a.th > 0 && int(floor(ylo)) - 1 >= a.tr0 && int(floor(yhi)) + 2 <= a.tr0 + a.th - 1 &&
+                         int(floor(xlo)) - 1 >= a.tc0 && int(floor(xhi)) + 2 <= a.tc0 + a.tw - 1;
+    mode = in_tile ? 1 : 2;
+  }
+  if (mode) {
+    extern __shared__ uint8_t lds_tile[];
+    double2* const rt = reinterpret_cast<double2*>(lds_tile + a.rtab_off);  // [row - rmin]: (ca dr, sa dr)
+    double2* const ct = rt + rows;                                          // [col - cmin]: (sa dc, ca dc)
+    for (int i = threadIdx.x; i < rows + cols; i += kRefineThreads) {
+      if (i < rows) {
+        const double dr = (P.rmin + i) - a.lv_orow;
+        rt[i] = make_double2(ca * dr, sa * dr);
+      } else {
+        const double dc = (P.cmin + i - rows) - a.lv_ocol;
+        rt[i] = make_double2(sa * dc, ca * dc);
+      }
+    }
+    __syncthreads();
+    const int idc = int(a.dc);
+    auto fast = [&]<bool kLds>() __attribute__((always_inline)) {
+      const T* lt = reinterpret_cast<const T*>(lds_tile);
+      const T* gl = static_cast<const T*>(a.lv.data) + (size_t(a.sr) * a.lv.pitch + a.sc);
+      const int stride = kLds ? a.tw : int(a.lv.pitch);
+      const int org = kLds ? -a.tr0 * a.tw - a.tc0 : 0;
+      for (int base = threadIdx.x; base < P.n; base += kRefineThreads * U) {
+        int32_t rcv[U];
+        double wv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = base + kRefineThreads * u;
+          rcv[u] = i < P.n ? O.rc[i] : 0;
+          wv[u] = i < P.n ? O.w[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (base + kRefineThreads * u >= P.n) continue;
+          const double2 tr = rt[sample_row(rcv[u]) - P.rmin], tc = ct[sample_col(rcv[u]) - P.cmin];
+          const double y = (pr + tr.x) - tc.x, x = (pc + tr.y) + tc.y;
+          const double yr = floor(y), xc = floor(x), fy = y - yr, fx = x - xc;
+          const int r = int(yr), c = int(xc);
+          double va, vb, vd, ve;
+          if constexpr (kLds) {
+            const int i0 = r * stride + c + org;
+            va = double(int(lt[i0]) - idc); vb = double(int(lt[i0 + 1]) - idc);
+            vd = double(int(lt[i0 + stride]) - idc); ve = double(int(lt[i0 + stride + 1]) - idc);
+          } else {
+            const T* g = gl + (size_t(r) * size_t(stride) + c);
+            va = double(int(g[0]) - idc); vb = double(int(g[1]) - idc);
+            vd = double(int(g[stride]) - idc); ve = double(int(g[stride + 1]) - idc);
+          }
+          const double ux = 1 - fx, uy = 1 - fy;
+          double q = uy * (ux * va + fx * vb) + fy * (ux * vd + fx * ve);
+          const double gy = ux * (vd - va) + fx * (ve - vb);
+          const double gx = uy * (vb - va) + fy * (ve - vd);
+          q = q - off;
+          const double w = wv[u];
+          v[0] += q; v[1] += q * q; v[2] += w * q;
+          if (deriv) {
+            const double g[3] = {gy, gx, gy * ((0 - tr.y) - tc.y) + gx * (tr.x - tc.x)};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) { v[3 + j] += g[j]; v[6 + j] += q * g[j]; v[9 + j] += w * g[j]; }
+            v[12] += g[0] * g[0]; v[13] += g[1] * g[0]; v[14] += g[1] * g[1];
+            v[15] += g[2] * g[0]; v[16] += g[2] * g[1]; v[17] += g[2] * g[2];
+          }
+        }
+      }
+    };
+    if (mode == 1) fast.template operator()<true>();
+    else fast.template operator()<false>();
+  } else
   for (int base = threadIdx.x; base < P.n; base += kRefineThreads * U) {
     int32_t rcv[U];
     double wv[U];
@@ -1857,10 +1953,87 @@ __device__ __forceinline__ double prefix_window_wave(const RefineArgs& a, const 
   return a.absolute ? fabs(sc) : sc;
 }
 
+// The windows (row, col0 + j), j0 <= j <= j1 (within 0..4), of one level-0
+// patch, scored by one wave as direct_window_wave / prefix_window_wave score
+// them, every sample's position and weight loaded once for all of them: per
+// window the same per-lane order of the sums and the same wave sums (the
+// moments are exact integers either way), so the same scores.  The windows
+// must lie in the level's storage rectangle (and, with kLds, their pixels in
+// the LDS tile).
+template <class T, bool kLds>
+__device__ __forceinline__ void window_row_wave(const RefineArgs& a, const PatchRef& p, int row, int col0, int j0, int j1,
+                                                double (&sc)[5]) {
+  extern __shared__ uint8_t lds_tile[];
+  const NccPatch& P = p.P;
+  const T* lt = reinterpret_cast<const T*>(lds_tile);
+  const T* gl = static_cast<const T*>(a.lv.data) + (size_t(a.sr) * a.lv.pitch + a.sc);
+  const int stride = kLds ? a.tw : int(a.lv.pitch);
+  const int org = kLds ? -a.tr0 * a.tw - a.tc0 : 0;
+  const int lr = row - a.sr, lc = col0 - a.sc;
+  auto at = [&](int r, int c) __attribute__((always_inline)) -> int {
+    if constexpr (kLds) return int(lt[r * stride + c + org]);
+    else return int(gl[size_t(r) * size_t(stride) + c]);
+  };
+  // per window the value the moments are taken from: the first sample's
+  // pixel (direct form) or dc (prefix form), S - base an exact integer
+  int base[5];
+  const int32_t rc0 = p.rc[0];
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    base[j] = a.prefix_recovery || j < j0 || j > j1 ? int(a.dc) : at(lr + sample_row(rc0), lc + j + sample_col(rc0));
+  int64_t s[5] = {0, 0, 0, 0, 0}, sq[5] = {0, 0, 0, 0, 0};
+  double d[5] = {0, 0, 0, 0, 0};
+  constexpr int U = 4;
+  for (int b = threadIdx.x & 63; b < P.n; b += 64 * U) {
+    int32_t rcv[U];
+    double wv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = b + 64 * u;
+      rcv[u] = i < P.n ? p.rc[i] : 0;
+      wv[u] = i < P.n ? p.w[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (b + 64 * u >= P.n) continue;
+      const int r = lr + sample_row(rcv[u]), c = lc + sample_col(rcv[u]);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        if (j < j0 || j > j1) continue;  // uniform
+        const int q = at(r, c + j) - base[j];
+        const double qd = double(q);
+        s[j] += q;
+        sq[j] += int64_t(q) * q;
+        d[j] += qd * wv[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    sc[j] = NCC_MISSING;
+    if (j < j0 || j > j1) continue;
+    int64_t ss = s[j], qq = sq[j];
+    for (int o = 32; o > 0; o >>= 1) { ss += __shfl_xor(ss, o, 64); qq += __shfl_xor(qq, o, 64); }
+    const double sum = double(ss), sumsq = double(qq), dot = wave_sum(d[j]);
+    const double n = double(P.n), var = sumsq - sum * sum / n;
+    double v;
+    if (a.prefix_recovery) {
+      if (var <= 128 * 0x1p-52 * sumsq) continue;
+      v = clamp1((dot - sum / n * P.wsum) / sqrt(var));
+    } else {
+      if (!(var > 0)) continue;
+      v = clamp1(dot / sqrt(var));
+    }
+    sc[j] = a.absolute ? fabs(v) : v;
+  }
+}
+
 // Recovery search (matcher.cpp:986-1059): the 5x5 level-0 anchors around the
-// seed for every level-0 patch within one angle step.  Windows are spread over
-// the waves; the first strict maximum above `floor_score` in the reference's
-// (patch, dr, dc) order wins.  Returns false when no window beats it.
+// seed for every level-0 patch within one angle step.  A wave takes one
+// patch's row of five windows at a time (window_row_wave); the first strict
+// maximum above `floor_score` in the reference's (patch, dr, dc) order wins
+// (each wave keeps its best with the lowest index, then the block).  Returns
+// false when no window beats it.
 template <class T>
 __device__ __forceinline__ bool recovery_search(const RefineArgs& a, const NccMatch& seed, double floor_score, NccMatch& best,
                                 int* elig, int* wave_cnt, Shared& sm) {
@@ -1868,13 +2041,26 @@ __device__ __forceinline__ bool recovery_search(const RefineArgs& a, const NccMa
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = kRefineThreads / 64;
   double wb = NCC_MISSING;
   int wi = 0x7fffffff;
-  for (int k = wave; k < n_e * 25; k += n_waves) {
-    const PatchRef p = patch_ref(a.pools, a.refs0[elig[k / 25]]);
-    const int r = int(round(seed.row - a.crop_r - p.P.orow)) + (k % 25) / 5 - 2;
-    const int c = int(round(seed.col - a.crop_c - p.P.ocol)) + k % 5 - 2;
-    if (!domain_contains(a, a.crop_r + r + p.P.orow, a.crop_c + c + p.P.ocol)) continue;
-    const double sc = a.prefix_recovery ? prefix_window_wave<T>(a, p, r, c) : direct_window_wave<T>(a, p, r, c);
-    if (sc > wb) { wb = sc; wi = k; }
+  for (int k = wave; k < n_e * 5; k += n_waves) {
+    const PatchRef p = patch_ref(a.pools, a.refs0[elig[k / 5]]);
+    const NccPatch& P = p.P;
+    const int r = int(round(seed.row - a.crop_r - P.orow)) + k % 5 - 2;
+    const int c0 = int(round(seed.col - a.crop_c - P.ocol)) - 2;
+    // the windows inside level 0 and the storage rectangle: columns c0 + j0 .. c0 + j1
+    if (r + P.rmin < 0 || r + P.rmax >= a.H0 || r + P.rmin < a.sr || r + P.rmax >= a.sr + a.sh) continue;
+    const int j0 = max(0, max(-P.cmin, a.sc - P.cmin) - c0);
+    const int j1 = min(4, min(a.W0 - 1 - P.cmax, a.sc + a.sw - 1 - P.cmax) - c0);
+    if (j0 > j1) continue;
+    const int lr = r - a.sr, lc = c0 - a.sc;
+    const bool in_tile = a.th > 0 && lr + P.rmin >= a.tr0 && lr + P.rmax <= a.tr0 + a.th - 1 &&
+                         lc + j0 + P.cmin >= a.tc0 && lc + j1 + P.cmax <= a.tc0 + a.tw - 1;
+    double sc[5];
+    if (in_tile) window_row_wave<T, true>(a, p, r, c0, j0, j1, sc);
+    else window_row_wave<T, false>(a, p, r, c0, j0, j1, sc);
+    for (int j = j0; j <= j1; ++j) {
+      if (!domain_contains(a, a.crop_r + r + P.orow, a.crop_c + c0 + j + P.ocol)) continue;
+      if (sc[j] > wb) { wb = sc[j]; wi = (k / 5) * 25 + (k % 5) * 5 + j; }
+    }
   }
   if (lane == 0) { sm.win_s[wave] = wb; sm.win_i[wave] = wi; }
   __syncthreads();
@@ -1921,6 +2107,7 @@ __global__ void __launch_bounds__(kRefineThreads) k_full_score(const RefineArgs 
       double pr, pc;
       to_local(a, cd.row, cd.col, pr, pc);
       stage_tile<T>(a, lds_tile, pr, pc, template_reach(a, O.P) + 2);
+      a.rtab_cap = 0;  // one pass per candidate: building the rotation tables does not pay
     }
     if (isfinite(cd.score)) cd.score = continuous<T>(a, O, cd.row, cd.col, cd.angle, sm);
     if (threadIdx.x == 0) a.out_full[slot] = cd;
