@@ -1358,10 +1358,11 @@ void FindContext::choose_tile(Slot& S) {
 // load).  Every shape writes the same sparse planes and peak bits (the
 // accumulation order per output does not depend on the shape, and a bound
 // only drops windows below the peak threshold), so the repeated launches
-// leave the batch's state as one launch would.  A search
-// whose default-shape launch takes over 1 ms is timed on its first frames
-// and a band of their top-level rows (at least 2048 default-shape blocks),
-// so tuning costs a few ms per shape.
+// leave the batch's state as one launch would.  A search whose
+// default-shape launch is estimated (from a probe on its first frame's
+// first rows) to take over 1 ms is timed on its first frames and a band of
+// their top-level rows (at least 2048 default-shape blocks), so tuning costs
+// about a millisecond per shape.
 // Not run inside a graph capture.
 void FindContext::tune_coarse(Slot& S) {
   Plan& P = *S.plan;
@@ -1385,44 +1386,53 @@ void FindContext::tune_coarse(Slot& S) {
     check(hipEventElapsedTime(&ms, tune_events_[0], tune_events_[1]), "hipEventElapsedTime");
     return double(ms) * 1e3 / reps;
   };
-  // the default shape first: warm-up, then its time decides the sample: the
-  // first frames of the batch, then a band of their top-level rows, keeping
-  // at least kSampleBlocks default-shape blocks (so that the sample fills
-  // the GPU as the batch does) and about kSampleUs per launch
+  // The sample: the default shape is timed on a probe (the first frame's
+  // first rows, at least kSampleBlocks default-shape blocks) after one
+  // warm-up launch there, and its time extrapolated to the batch; a batch
+  // estimated over kSampleUs is timed on its first frames, then a band of
+  // their top-level rows, keeping at least kSampleBlocks default blocks (so
+  // that the sample fills the GPU as the batch does) and about kSampleUs per
+  // launch.  No launch of the whole batch is needed to choose it.
   constexpr double kSampleUs = 1000.0;
   constexpr int64_t kSampleBlocks = 2048;
+  auto blocks = [&](int f, int rows) {  // default-shape blocks (one per angle tile) of frame f within its first rows
+    const int w = P.jobs[size_t(f)].lv[P.top].w, h = std::min(rows, P.jobs[size_t(f)].lv[P.top].h);
+    return int64_t((w + 127) / 128) * ((h + 3) / 4) * n_tiles;
+  };
+  int64_t all = 0;
+  for (int f = 0; f < j.n_frames; ++f) all += blocks(f, j.max_h);
+  const NcchipCoarseJob whole = j;
+  int probe_rows = 16;
+  while (probe_rows < j.max_h && blocks(0, probe_rows) < kSampleBlocks) probe_rows *= 2;
+  probe_rows = std::min(probe_rows, j.max_h);
+  j.n_frames = 1;
+  j.max_h = probe_rows;
   run(kDefaultCoarseTile, 1);
-  double def_us = run(kDefaultCoarseTile, 1);
-  if (def_us > kSampleUs) {
-    auto blocks = [&](int f, int rows) {  // default-shape blocks (one per angle tile) of frame f within its first rows
-      const int w = P.jobs[size_t(f)].lv[P.top].w, h = std::min(rows, P.jobs[size_t(f)].lv[P.top].h);
-      return int64_t((w + 127) / 128) * ((h + 3) / 4) * n_tiles;
-    };
-    int64_t all = 0;
-    for (int f = 0; f < j.n_frames; ++f) all += blocks(f, j.max_h);
+  const double probe_us = run(kDefaultCoarseTile, 1);
+  const double est = probe_us * double(all) / double(std::max<int64_t>(blocks(0, probe_rows), 1));
+  j = whole;
+  if (est > kSampleUs) {
     int frames = 0;
     int64_t kept = 0;
-    while (frames < j.n_frames && (kept < kSampleBlocks || def_us * double(kept) / double(all) < kSampleUs / 2)) {
+    while (frames < j.n_frames && (kept < kSampleBlocks || est * double(kept) / double(all) < kSampleUs / 2)) {
       kept += blocks(frames, j.max_h);
       ++frames;
     }
-    double est = def_us * double(kept) / double(std::max<int64_t>(all, 1));
+    double part = est * double(kept) / double(std::max<int64_t>(all, 1));
     int rows = j.max_h;
-    while (rows > 16 && est > kSampleUs) {
+    while (rows > 16 && part > kSampleUs) {
       const int r2 = std::max(16, rows / 2 / 16 * 16);
       int64_t k2 = 0;
       for (int f = 0; f < frames; ++f) k2 += blocks(f, r2);
       if (k2 < kSampleBlocks) break;
-      est *= double(k2) / double(std::max<int64_t>(kept, 1));
+      part *= double(k2) / double(std::max<int64_t>(kept, 1));
       kept = k2;
       rows = r2;
     }
-    if (frames < j.n_frames || rows < j.max_h) {
-      j.n_frames = frames;
-      j.max_h = rows;
-      def_us = run(kDefaultCoarseTile, 1);
-    }
+    j.n_frames = frames;
+    j.max_h = rows;
   }
+  const double def_us = run(kDefaultCoarseTile, 1);
   CoarseTune best;
   best.us = def_us;
   best.default_us = def_us;
