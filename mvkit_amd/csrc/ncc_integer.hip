@@ -148,19 +148,18 @@ __global__ void __launch_bounds__(256) k_support_cols(const uint32_t* __restrict
   }
 }
 
-// Levels 1..n (n <= 3) of a u8 level 0 in one pass: a block stages 8 rows x
-// 1024 columns of level 0 in LDS and derives the 4 x 512, 2 x 256 and
-// 1 x 128 block sums below it with the per-level edge clamp (a clamped source
-// row or column stays inside the block's tile, as the tiles are aligned to
-// the 2x2 lattice of every level), writing each level once.
+// Levels 1..n (n <= 3) of a u8 level 0 in one pass.  The general tile: a
+// block stages 8 rows x 1024 columns of level 0 in LDS and derives the
+// 4 x 512, 2 x 256 and 1 x 128 block sums below it with the per-level edge
+// clamp (a clamped source row or column stays inside the block's tile, as the
+// tiles are aligned to the 2x2 lattice of every level), writing each level
+// once.
 constexpr int kFuseRows = 8, kFuseCols = 1024;
-__global__ void __launch_bounds__(256) k_pyramid3_u8(const NccFrame* __restrict__ frames, int n_levels) {
+__device__ __forceinline__ void pyramid3_tile(const NccFrame& F, int n_levels, const int R0, const int C0) {
   __shared__ uint16_t t0[kFuseRows][kFuseCols];       // level 0 (u8 values)
   __shared__ uint16_t t1[kFuseRows / 2][kFuseCols / 2];
   __shared__ uint16_t t2[kFuseRows / 4][kFuseCols / 4];
-  const NccFrame& F = frames[blockIdx.z];
   const NccLevel& L0 = F.levels[0];
-  const int R0 = blockIdx.y * kFuseRows, C0 = blockIdx.x * kFuseCols;
   if (R0 >= L0.height || C0 >= L0.width) return;  // uniform per block
   const uint8_t* src = static_cast<const uint8_t*>(L0.data);
   // level 0: 4 consecutive bytes per thread and row, one dword when the
@@ -212,6 +211,77 @@ __global__ void __launch_bounds__(256) k_pyramid3_u8(const NccFrame* __restrict_
   __syncthreads();
   step(&t2[0][0], kFuseCols / 4, R0 / 4, C0 / 4, F.levels[2].height, F.levels[2].width, nullptr, 0, F.levels[3],
        kFuseRows / 8, kFuseCols / 8);
+}
+
+// Sums of the byte pairs (0,1) and (2,3) of x, as two 16-bit halves.
+__device__ __forceinline__ uint32_t byte_pairs(uint32_t x) { return (x & 0x00ff00ffu) + ((x >> 8) & 0x00ff00ffu); }
+// Sum of the two 16-bit halves of x.
+__device__ __forceinline__ uint32_t half_sum(uint32_t x) { return (x & 0xffffu) + (x >> 16); }
+
+// A block covers 16 rows x 1024 columns of level 0.  A tile inside the frame
+// whose rows and level rows are 16-byte aligned runs in registers: wave w
+// loads rows 4w .. 4w + 3 and lane l columns 16l .. 16l + 15 (one 16-byte
+// load per row), forms its 2 x 8 level-1 sums as packed byte-pair sums
+// (stored as one 16-byte vector per row) and its 4 level-2 sums from them;
+// level 3 joins the level-2 rows of two waves through LDS.  No clamp applies
+// inside the frame (its last rows and columns lie in edge tiles), and every
+// value is an integer sum of the same four inputs, so the levels equal the
+// general tile's.  Edge and unaligned tiles run the general tile on each
+// half.
+__global__ void __launch_bounds__(256) k_pyramid3_u8(const NccFrame* __restrict__ frames, int n_levels) {
+  __shared__ uint32_t l2p[4][64][2];  // per wave and lane: its level-2 sums added in column pairs
+  const NccFrame& F = frames[blockIdx.z];
+  const NccLevel& L0 = F.levels[0];
+  const int R0 = blockIdx.y * 2 * kFuseRows, C0 = blockIdx.x * kFuseCols;
+  if (R0 >= L0.height || C0 >= L0.width) return;  // uniform per block
+  const uint8_t* src = static_cast<const uint8_t*>(L0.data);
+  uintptr_t align = reinterpret_cast<uintptr_t>(src) | uintptr_t(L0.pitch);
+  for (int l = 1; l <= n_levels; ++l)
+    align |= reinterpret_cast<uintptr_t>(F.levels[l].data) | (uintptr_t(F.levels[l].pitch) * 2);
+  if (R0 + 2 * kFuseRows > L0.height || C0 + kFuseCols > L0.width || (align & 15)) {
+    pyramid3_tile(F, n_levels, R0, C0);
+    __syncthreads();  // the tile's LDS is reused by the second half
+    pyramid3_tile(F, n_levels, R0 + kFuseRows, C0);
+    return;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint4 v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    v[r] = *reinterpret_cast<const uint4*>(src + size_t(R0 + 4 * wave + r) * L0.pitch + C0 + 16 * lane);
+  // level 1: rows R0/2 + 2 wave + {0, 1}, columns C0/2 + 8 lane .. + 7 (two per dword)
+  uint4 q[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    q[h].x = byte_pairs(v[2 * h].x) + byte_pairs(v[2 * h + 1].x);
+    q[h].y = byte_pairs(v[2 * h].y) + byte_pairs(v[2 * h + 1].y);
+    q[h].z = byte_pairs(v[2 * h].z) + byte_pairs(v[2 * h + 1].z);
+    q[h].w = byte_pairs(v[2 * h].w) + byte_pairs(v[2 * h + 1].w);
+    const NccLevel& D = F.levels[1];
+    *reinterpret_cast<uint4*>(static_cast<uint16_t*>(const_cast<void*>(D.data)) +
+                              size_t(R0 / 2 + 2 * wave + h) * D.pitch + C0 / 2 + 8 * lane) = q[h];
+  }
+  if (n_levels < 2) return;
+  // level 2: row R0/4 + wave, columns C0/4 + 4 lane .. + 3
+  const uint32_t s0 = half_sum(q[0].x) + half_sum(q[1].x), s1 = half_sum(q[0].y) + half_sum(q[1].y);
+  const uint32_t s2 = half_sum(q[0].z) + half_sum(q[1].z), s3 = half_sum(q[0].w) + half_sum(q[1].w);
+  {
+    const NccLevel& D = F.levels[2];
+    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(const_cast<void*>(D.data)) + size_t(R0 / 4 + wave) * D.pitch +
+                              C0 / 4 + 4 * lane) = make_uint2(s0 | (s1 << 16), s2 | (s3 << 16));
+  }
+  if (n_levels < 3) return;
+  // level 3: rows R0/8 + {0, 1} from level-2 rows of waves (0, 1) and (2, 3),
+  // columns C0/8 + 2 lane + {0, 1}
+  l2p[wave][lane][0] = s0 + s1;
+  l2p[wave][lane][1] = s2 + s3;
+  __syncthreads();
+  if (wave == 0 || wave == 2) {
+    const uint32_t a = l2p[wave][lane][0] + l2p[wave + 1][lane][0], b = l2p[wave][lane][1] + l2p[wave + 1][lane][1];
+    const NccLevel& D = F.levels[3];
+    *reinterpret_cast<uint32_t*>(static_cast<uint16_t*>(const_cast<void*>(D.data)) +
+                                 size_t(R0 / 8 + wave / 2) * D.pitch + C0 / 8 + 2 * lane) = a | (b << 16);
+  }
 }
 
 // Row pass of a frame's domain summed-area table: entry (r + 1, c + 1) = the
@@ -287,7 +357,8 @@ extern "C" int ncchip_pyramid3_u8(const NccFrame* frames, int n_frames, int n_le
                                   void* stream) {
   if (n_frames <= 0 || n_levels <= 0) return 0;
   if (n_frames > 65535 || n_levels > 3) return -1;
-  const dim3 grid((max_w + kFuseCols - 1) / kFuseCols, (max_h + kFuseRows - 1) / kFuseRows, unsigned(n_frames));
+  const dim3 grid((max_w + kFuseCols - 1) / kFuseCols, (max_h + 2 * kFuseRows - 1) / (2 * kFuseRows),
+                  unsigned(n_frames));
   hipLaunchKernelGGL(k_pyramid3_u8, grid, dim3(256), 0, static_cast<hipStream_t>(stream), frames, n_levels);
   return hipGetLastError() == hipSuccess ? 0 : -1000;
 }
