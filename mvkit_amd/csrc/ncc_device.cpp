@@ -1244,8 +1244,11 @@ void FindContext::front_launches(Slot& S) {
       raise(kInternalError, "device", "pyramid launch failed");
   }
   time_end(qs);
+  // the top level's row prefixes feed the FP32 and FP64 coarse kernels only:
+  // the split-BF16 kernel takes its exact window moments from the image tile
+  // it stages
   time_begin(kStagePrefix, qs);
-  if (ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, qs))
+  if (!P.use_mfma && ncchip_row_prefix_batch(dframes, F, int(top), jobs[0].lv[top].kind, Ht_max, qs))
     raise(kInternalError, "device", "prefix launch failed");
   time_end(qs);
 }
