@@ -313,10 +313,12 @@ __global__ void __launch_bounds__(256) k_peaks(const PeakArgs a) {
 // the threshold are written, each with its bit in the frame's peak_bits
 // (per position, one word per 32 angles).  Windows below the threshold can
 // neither be peaks nor suppress one, so the 3x3 rule only needs the flagged
-// neighbours.  One thread per column, the rows of a block in turn; a lane
-// may emit several angles of its position.  Most positions hold no flag, so
-// a block takes one row: many short blocks finish soonest.
-constexpr int kPeakBitRows = 1;
+// neighbours.  One thread per column over the kPeakBitRows rows of its block;
+// a lane may emit several angles of its positions.  A wave claims its slots
+// in the frame's list with one atomic per word of angles over all its rows,
+// and folds the extent of its candidates into the frame's with one set of
+// atomics at the end (all rows of a frame meet on those few addresses).
+constexpr int kPeakBitRows = 8;
 __global__ void __launch_bounds__(256) k_peaks_bits(const PeakArgs a) {
   const int f = blockIdx.z;
   const NccFrame& F = a.frames[f];
@@ -328,57 +330,61 @@ __global__ void __launch_bounds__(256) k_peaks_bits(const PeakArgs a) {
   const size_t A = size_t(a.n_angles);
   const int r0 = blockIdx.y * kPeakBitRows;
   const int r_end = min(H, r0 + kPeakBitRows);
+  double orow = __builtin_inf(), ocol = __builtin_inf(), xrow = 0, xcol = 0;
   for (int w = 0; w < nw; ++w) {
-    uint32_t words[kPeakBitRows];
+    uint32_t words[kPeakBitRows], peaks[kPeakBitRows];
 #pragma unroll
     for (int i = 0; i < kPeakBitRows; ++i)
       words[i] = (c < W && r0 + i < r_end) ? F.peak_bits[(size_t(r0 + i) * W + c) * nw + w] : 0u;
+    int cnt = 0;
 #pragma unroll
     for (int i = 0; i < kPeakBitRows; ++i) {
       const int r = r0 + i;
-      uint32_t peaks = 0;
-      const size_t pos = size_t(r) * W + c;
-      {
-        const uint32_t bits = words[i];
-        if (bits) {
-          uint32_t nb[9];
+      peaks[i] = 0;
+      const uint32_t bits = words[i];
+      if (bits) {
+        const size_t pos = size_t(r) * W + c;
+        uint32_t nb[9];
 #pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const int rr = r + k / 3 - 1, cc = c + k % 3 - 1;
-            nb[k] = (k != 4 && rr >= 0 && cc >= 0 && rr < H && cc < W) ? F.peak_bits[(size_t(rr) * W + cc) * nw + w] : 0u;
+        for (int k = 0; k < 9; ++k) {
+          const int rr = r + k / 3 - 1, cc = c + k % 3 - 1;
+          nb[k] = (k != 4 && rr >= 0 && cc >= 0 && rr < H && cc < W) ? F.peak_bits[(size_t(rr) * W + cc) * nw + w] : 0u;
+        }
+        uint32_t pend = bits;
+        while (pend) {
+          const int s = __builtin_ctz(pend);
+          pend &= pend - 1;
+          const double v = F.planes[pos * A + size_t(w) * 32 + s];
+          bool peak = true;
+          for (int k = 0; k < 9 && peak; ++k) {
+            if (!((nb[k] >> s) & 1u)) continue;
+            const int dr = k / 3 - 1, dc = k % 3 - 1;
+            const double o = F.planes[(size_t(r + dr) * W + (c + dc)) * A + size_t(w) * 32 + s];
+            if (o > v || (o == v && (dr < 0 || (!dr && dc < 0)))) peak = false;
           }
-          uint32_t pend = bits;
-          while (pend) {
-            const int s = __builtin_ctz(pend);
-            pend &= pend - 1;
-            const double v = F.planes[pos * A + size_t(w) * 32 + s];
-            bool peak = true;
-            for (int k = 0; k < 9 && peak; ++k) {
-              if (!((nb[k] >> s) & 1u)) continue;
-              const int dr = k / 3 - 1, dc = k % 3 - 1;
-              const double o = F.planes[(size_t(r + dr) * W + (c + dc)) * A + size_t(w) * 32 + s];
-              if (o > v || (o == v && (dr < 0 || (!dr && dc < 0)))) peak = false;
-            }
-            if (peak) peaks |= 1u << s;
-          }
+          if (peak) peaks[i] |= 1u << s;
         }
       }
-      // emission: a wave-wide exclusive scan of the peak counts
-      const int cnt = __popc(peaks);
-      int incl = cnt;
+      cnt += __popc(peaks[i]);
+    }
+    // emission: a wave-wide exclusive scan of the lanes' peak counts
+    int incl = cnt;
 #pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-      }
-      const int total = __shfl(incl, 63, 64);
-      if (!total) continue;  // wave-uniform
-      int base = 0;
-      if (lane == 63) base = atomicAdd(a.count + f, total);
-      base = __shfl(base, 63, 64);
-      int k = base + incl - cnt;
-      double orow = __builtin_inf(), ocol = __builtin_inf(), xrow = 0, xcol = 0;
-      uint32_t pm = peaks;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (!total) continue;  // wave-uniform
+    int base = 0;
+    if (lane == 63) base = atomicAdd(a.count + f, total);
+    base = __shfl(base, 63, 64);
+    int k = base + incl - cnt;
+#pragma unroll
+    for (int i = 0; i < kPeakBitRows; ++i) {
+      const int r = r0 + i;
+      const size_t pos = size_t(r) * W + c;
+      uint32_t pm = peaks[i];
       while (pm) {
         const int s = __builtin_ctz(pm);
         pm &= pm - 1;
@@ -398,21 +404,21 @@ __global__ void __launch_bounds__(256) k_peaks_bits(const PeakArgs a) {
         }
         ++k;
       }
-#pragma unroll
-      for (int s = 32; s > 0; s >>= 1) {
-        orow = fmin(orow, __shfl_xor(orow, s, 64));
-        ocol = fmin(ocol, __shfl_xor(ocol, s, 64));
-        xrow = fmax(xrow, __shfl_xor(xrow, s, 64));
-        xcol = fmax(xcol, __shfl_xor(xcol, s, 64));
-      }
-      if (lane == 0 && isfinite(orow)) {
-        unsigned long long* e = a.extent + 4 * size_t(f);
-        atomicMin(e + 0, __double_as_longlong(orow));
-        atomicMax(e + 1, __double_as_longlong(xrow));
-        atomicMin(e + 2, __double_as_longlong(ocol));
-        atomicMax(e + 3, __double_as_longlong(xcol));
-      }
     }
+  }
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    orow = fmin(orow, __shfl_xor(orow, s, 64));
+    ocol = fmin(ocol, __shfl_xor(ocol, s, 64));
+    xrow = fmax(xrow, __shfl_xor(xrow, s, 64));
+    xcol = fmax(xcol, __shfl_xor(xcol, s, 64));
+  }
+  if (lane == 0 && isfinite(orow)) {
+    unsigned long long* e = a.extent + 4 * size_t(f);
+    atomicMin(e + 0, __double_as_longlong(orow));
+    atomicMax(e + 1, __double_as_longlong(xrow));
+    atomicMin(e + 2, __double_as_longlong(ocol));
+    atomicMax(e + 3, __double_as_longlong(xcol));
   }
 }
 
