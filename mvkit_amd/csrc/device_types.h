@@ -36,9 +36,9 @@ typedef struct NccPatch {
   int32_t use_wruns;         // 2*#weight_runs < #samples (kernels_avx2.cpp:150, 175, 211)
   // Split-BF16 dense raster for the MFMA tracking kernel (element offset
   // into dense_hi / dense_lo, -1: none): rows rmin..rmax, columns
-  // cmin-8 .. cmax+8 (zero padded, dense_w a multiple of 8), stored eight
-  // times, copy c shifted left by c elements, so any 8-element run is one
-  // aligned 16-byte load from copy (start mod 8).
+  // cmin-8 .. cmax+8 (zero padded, dense_w a multiple of 8), stored twice,
+  // the second copy shifted left by one element, so any 8-element run is
+  // one 16-byte load at a 4-byte aligned address of copy (start mod 2).
   int32_t dense_off;
   int32_t dense_h, dense_w;
   int32_t pad;

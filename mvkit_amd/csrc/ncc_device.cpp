@@ -44,7 +44,7 @@ DenseGeom dense_geom(const Raster& r) {
   g.w = ((r.cmax - r.cmin + 1) + 16 + 7) & ~7;  // rows of whole 16-byte groups
   const size_t e = size_t(g.h) * size_t(g.w);
   if (r.count == 0 || e > kDenseMaxElems) return DenseGeom{};
-  g.elems = 8 * e;  // eight copies
+  g.elems = 2 * e;  // two copies
   return g;
 }
 
@@ -78,12 +78,13 @@ struct PoolLayout {
 uint16_t bf16_bits(double v);
 double bf16_value(uint16_t h);
 
-// Fills raster r's two dense copies and its split-BF16 error coefficients.
+// Fills raster r's dense copies (hi and lo parts) and its split-BF16 error
+// coefficients.
 void dense_raster(const RasterPool& p, const Raster& r, uint16_t* hi, uint16_t* lo, NccPatch& P) {
   const DenseGeom g = dense_geom(r);
   const size_t plane = size_t(g.h) * size_t(g.w);
-  std::fill(hi, hi + 2 * plane, uint16_t(0));
-  std::fill(lo, lo + 2 * plane, uint16_t(0));
+  std::fill(hi, hi + plane, uint16_t(0));
+  std::fill(lo, lo + plane, uint16_t(0));
   double wh1 = 0, wl1 = 0, res1 = 0;
   for (uint32_t i = 0; i < r.count; ++i) {
     const int32_t rc = p.rc[r.first + i];
@@ -98,11 +99,10 @@ void dense_raster(const RasterPool& p, const Raster& r, uint16_t* hi, uint16_t* 
     wl1 += std::abs(bf16_value(l));
     res1 += std::abs(w - bf16_value(h) - bf16_value(l));
   }
-  for (size_t c = 1; c < 8; ++c)  // copy c = copy 0 shifted left by c elements
-    for (size_t e = 0; e < plane; ++e) {
-      hi[c * plane + e] = e + c < plane ? hi[e + c] : uint16_t(0);
-      lo[c * plane + e] = e + c < plane ? lo[e + c] : uint16_t(0);
-    }
+  for (size_t e = 0; e < plane; ++e) {  // copy 1 = copy 0 shifted left by one element
+    hi[plane + e] = e + 1 < plane ? hi[e + 1] : uint16_t(0);
+    lo[plane + e] = e + 1 < plane ? lo[e + 1] : uint16_t(0);
+  }
   const double k = 3.0 * r.count + 8, u = std::ldexp(1.0, -23);
   const double gam = k * u / (1 - k * u);
   // the sums above round; a relative slack covers them

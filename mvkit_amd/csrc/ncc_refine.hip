@@ -918,6 +918,12 @@ constexpr int kTrkAngCache = 128;
 constexpr int kTrkRunCache = 512;
 
 __device__ __forceinline__ uint16_t bf16_exact_bits(int v) { return uint16_t(__float_as_uint(float(v)) >> 16); }
+// 16 bytes from a 4-byte aligned global address (one dwordx4 load)
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ uint4 load_dw4(const uint32_t* p) {
+  const u32x4_a4 v = *reinterpret_cast<const u32x4_a4*>(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 // Exact FP64 score of one window (one wave, samples lane-strided).
 template <class Reader>
@@ -1145,10 +1151,10 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int n_my = (y_hi - y_lo) * XC;
       constexpr int kRing = 4;  // weight fragments in flight per lane
       // B fragment of step (Y, cx): elements e .. e + 7 of the patch's dense
-      // raster, e = (Y + ybase) dw + 16 cx + xbase.  dw and 16 cx are
-      // multiples of 8, so the copy (e mod 8) is the lane's constant and the
-      // aligned offset moves by dw per row and 16 per step: the 16-byte index
-      // is qrow + 2 cx, qrow advanced by dw / 8 per row.  The fragment is
+      // raster, e = (Y + ybase) dw + 16 cx + xbase.  dw and 16 cx are even,
+      // so the copy (e mod 2) is the lane's constant and the 4-byte aligned
+      // offset moves by dw per row and 16 per step: the dword index is
+      // qrow + 8 cx, qrow advanced by dw / 2 per row.  The fragment is
       // inside the raster for Y in [ylo, yhi) and cx in [cxl, cxh] (lane
       // constants; empty for a lane without a patch).  Loads are issued
       // unconditionally and used as loaded (a mask applied to a loaded value
@@ -1158,8 +1164,8 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       const int ms = has ? m : 0;
       const NccPatch& Pm = s_pat[ms];
       const NccPatchPool& pool_m = a.pools.of(s_ref[ms]);
-      const uint4* bh_q = reinterpret_cast<const uint4*>(pool_m.dense_hi + Pm.dense_off);
-      const uint4* bl_q = reinterpret_cast<const uint4*>(pool_m.dense_lo + Pm.dense_off);
+      const uint32_t* bh_q = reinterpret_cast<const uint32_t*>(pool_m.dense_hi + Pm.dense_off);
+      const uint32_t* bl_q = reinterpret_cast<const uint32_t*>(pool_m.dense_lo + Pm.dense_off);
       const int dh = Pm.dense_h, dw = Pm.dense_w;
       const int ybase = U0 + 2 - s_row[ms] - Pm.rmin;
       const int xbase = V0 + 2 - s_col[ms] - Pm.cmin + 8 + 8 * gg;
@@ -1168,13 +1174,13 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
       // = -floor(xbase / 16) and cx <= floor((dw - 8 - xbase) / 16) (floor
       // division of possibly negative numerators: an arithmetic shift)
       const int cxl = -(xbase >> 4), cxh = (dw - 8 - xbase) >> 4;
-      const int qcopy = ((xbase & 7) * dh * dw + (xbase & ~7) + ybase * dw) >> 3;  // qrow at Y = 0
-      const int qdw = dw >> 3;
+      const int qcopy = ((xbase & 1) * dh * dw + (xbase & ~1) + ybase * dw) >> 1;  // qrow at Y = 0
+      const int qdw = dw >> 1;
       auto load_b = [&](bool ok, int Y, int cx, uint4& bh4, uint4& bl4) __attribute__((always_inline)) {
         const bool in = ok && Y >= ylo && Y < yhi && cx >= cxl && cx <= cxh;
-        const int q = in ? qcopy + Y * qdw + 2 * cx : 0;
-        bh4 = bh_q[q];
-        bl4 = bl_q[q];
+        const int q = in ? qcopy + Y * qdw + 8 * cx : 0;
+        bh4 = load_dw4(bh_q + q);
+        bl4 = load_dw4(bl_q + q);
       };
       // A fragment of step (Y, cx): 8 elements from tile row min(Y, KH - 1) +
       // arow, column 16 cx + acol: five aligned dwords, shifted by one element
@@ -1222,11 +1228,14 @@ __global__ void __launch_bounds__(kTrackThreads, 3) k_track_mfma(const TrackArgs
           load_a(Yn, cxn, n_h, n_l);
           const bf16x8_t A_h = __builtin_bit_cast(bf16x8_t, a_h), A_l = __builtin_bit_cast(bf16x8_t, a_l);
           const bf16x8_t B_h = __builtin_bit_cast(bf16x8_t, rh[j]), B_l = __builtin_bit_cast(bf16x8_t, rl[j]);
-          load_b(t + kRing < n_my, pY, pcx, rh[j], rl[j]);  // the ring slot's next use
-          step(pY, pcx);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_h, acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_l, acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_l, B_h, acc, 0, 0, 0);
+          // the ring slot's next use, loaded once its fragment is consumed, so
+          // that it keeps its registers across the loop (no copies at the
+          // back-edge, which would wait for every load in flight)
+          load_b(t + kRing < n_my, pY, pcx, rh[j], rl[j]);
+          step(pY, pcx);
           a_h = n_h;
           a_l = n_l;
           Y = Yn;
